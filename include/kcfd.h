@@ -1,0 +1,247 @@
+/*
+ * kcfd.h -- C-ABI of libkcfd.so, the MI355X (gfx950) kernels for K-CFD's
+ * per-iteration hot path: FVM coefficient assembly, Green-Gauss gradients,
+ * MUSCL/TVD deferred correction, Jacobi / ILU(0) preconditioners and
+ * BiCGSTAB (FP64, and FP32 inner iterations inside FP64 refinement), for one
+ * system or a batch of uneven systems.
+ *
+ * Conventions
+ *  - Every entry point returns int: KCFD_OK (0) or a negative KCFD_E* code;
+ *    kcfd_last_error() gives a thread-local message for the last failure.
+ *  - Handles are opaque and owned by the library; *_destroy frees them.
+ *  - Arrays marked (dev) are device pointers (from kcfd_buf_alloc or any
+ *    hipMalloc'd memory on the context's device); arrays marked (host) are
+ *    host pointers.  Sizes are element counts.
+ *  - All work is issued on the context's stream; entry points that return
+ *    results to the host synchronise that stream.
+ *  - Calls on one context are serialised by a mutex inside the library.
+ *
+ * Each entry point cites the reference (senily0/K-CFD) interface it replaces.
+ */
+#ifndef KCFD_H
+#define KCFD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KCFD_OK 0
+#define KCFD_EARG -1      /* bad argument (null pointer, size, unknown kind) */
+#define KCFD_EHIP -2      /* HIP runtime error */
+#define KCFD_EOOM -3      /* device allocation failed */
+#define KCFD_ESTATE -4    /* object not in a usable state (e.g. no factors) */
+#define KCFD_EPIVOT -5    /* ILU(0) zero / missing pivot */
+#define KCFD_EBUSY -6     /* admission refused: not enough device memory */
+
+/* preconditioner kinds (core/preconditioner.py:18-76) */
+#define KCFD_PC_NONE 0
+#define KCFD_PC_JACOBI 1
+#define KCFD_PC_ILU0 2
+
+/* precision modes */
+#define KCFD_FP64 0       /* everything in FP64 */
+#define KCFD_MIXED 1      /* FP32 SpMV/preconditioner/vectors inside FP64 refinement */
+
+/* solve status (scipy bicgstab info convention, iterative.py:157-302) */
+#define KCFD_INFO_OK 0
+#define KCFD_INFO_RHO_BREAKDOWN -10
+#define KCFD_INFO_OMEGA_BREAKDOWN -11
+
+/* boundary-condition kinds (core/fvm_operators.py:178-225) */
+#define KCFD_BC_ZERO_GRADIENT 0
+#define KCFD_BC_DIRICHLET 1
+#define KCFD_BC_NEUMANN 2
+
+/* TVD limiters (core/interpolation.py:17-44) */
+#define KCFD_LIM_MINMOD 0
+#define KCFD_LIM_SUPERBEE 1
+#define KCFD_LIM_VAN_ALBADA 2
+#define KCFD_LIM_VAN_LEER 3
+
+typedef struct kcfd_ctx kcfd_ctx;
+typedef struct kcfd_mat kcfd_mat;
+typedef struct kcfd_mesh kcfd_mesh;
+typedef struct kcfd_batch kcfd_batch;
+
+/* ---------------------------------------------------------------- context */
+
+/* Replaces the CuPy device set-up of core/gpu_solver.py:26-52
+ * (detect_gpu_backend).  One context = one device + one stream. */
+int kcfd_init(int device, kcfd_ctx **out);
+int kcfd_destroy(kcfd_ctx *ctx);
+const char *kcfd_last_error(void);
+/* name (host, len bytes), compute units, total / free device bytes */
+int kcfd_device_info(kcfd_ctx *ctx, char *name, int len, int *cus,
+                     long long *total_bytes, long long *free_bytes);
+int kcfd_sync(kcfd_ctx *ctx);
+
+/* ------------------------------------------------------- device buffers */
+
+/* Replaces the per-call host->device uploads of core/gpu_solver.py:113-115
+ * (cp.asarray) and the download at :155 (cp.asnumpy). */
+int kcfd_buf_alloc(kcfd_ctx *ctx, size_t bytes, void **dev);
+int kcfd_buf_free(kcfd_ctx *ctx, void *dev);
+int kcfd_buf_upload(kcfd_ctx *ctx, void *dev, const void *host, size_t bytes);
+int kcfd_buf_download(kcfd_ctx *ctx, void *host, const void *dev, size_t bytes);
+int kcfd_buf_fill(kcfd_ctx *ctx, double *dev, double value, size_t n);
+
+/* ---------------------------------------------------------------- matrix */
+
+/* Device-resident canonical CSR (int32 indices, sorted columns, explicit
+ * zeros kept), with an assembled right-hand side and the diagonal tracker of
+ * FVMSystem.  Replaces FVMSystem.to_sparse (core/fvm_operators.py:43-48)
+ * and cp_sparse.csr_matrix (core/gpu_solver.py:72,113). (host) structure. */
+int kcfd_mat_create(kcfd_ctx *ctx, int n, int nnz, const int *indptr,
+                    const int *indices, kcfd_mat **out);
+int kcfd_mat_destroy(kcfd_mat *m);
+int kcfd_mat_set_values(kcfd_mat *m, const double *vals /*host nnz*/);
+int kcfd_mat_get_values(kcfd_mat *m, double *vals /*host nnz*/);
+/* device pointers of the value array, the rhs and the diag tracker */
+int kcfd_mat_dev_ptrs(kcfd_mat *m, double **vals, double **rhs, double **diag);
+/* y = A x (dev); replaces gpu_spmv, core/gpu_solver.py:55-79 */
+int kcfd_mat_spmv(kcfd_mat *m, const double *x, double *y);
+/* device bytes held by the matrix (structure, values, preconditioner) */
+int kcfd_mat_bytes(kcfd_mat *m, long long *bytes);
+
+/* Preconditioner set-up on the device: Jacobi dinv = where(|a_ii|>1e-30,
+ * 1/a_ii, 0) (core/preconditioner.py:43-47) or structural ILU(0) on A's
+ * pattern, level-scheduled (replaces the spilu of :49-54).  With
+ * precision KCFD_MIXED an FP32 copy of values and factors is kept too. */
+int kcfd_precond_setup(kcfd_mat *m, int kind, int precision);
+/* ILU(0) factors (host nnz) and the number of levels of the L and U solves */
+int kcfd_ilu0_get(kcfd_mat *m, double *lu, int *levels_l, int *levels_u);
+
+/* ---------------------------------------------------------------- solve */
+
+typedef struct kcfd_solve_opts {
+    double rtol;        /* stop when ||r|| < max(atol, rtol*||b||) */
+    double atol;
+    int maxiter;        /* <=0: 10*n, as scipy */
+    int precond;        /* KCFD_PC_* */
+    int precision;      /* KCFD_FP64 / KCFD_MIXED */
+    double inner_rtol;  /* MIXED: inner FP32 solve reduction per refinement */
+    int max_refine;     /* MIXED: refinement steps cap */
+    int check_every;    /* iterations per captured graph between host checks */
+    int rows_per_thread;/* SpMV rows per lane group (0: autotuned default) */
+    int lanes_per_row;  /* 1, 2, 4, 8 (0: autotuned default) */
+} kcfd_solve_opts;
+
+typedef struct kcfd_solve_result {
+    int info;           /* 0 converged, >0 maxiter, -10/-11 breakdown */
+    int iters;          /* completed full BiCGSTAB iterations (scipy callback count) */
+    int half_step;      /* 1 if it stopped at the half-step test on ||s|| */
+    int refinements;    /* MIXED: outer FP64 refinement steps */
+    double rel_resid;   /* FP64 true relative residual ||b-Ax||/||b|| */
+    double bytes;       /* algorithmic HBM bytes of the solve (byte model) */
+    double ms;          /* device time of the solve */
+} kcfd_solve_result;
+
+/* Replaces solve_linear_system(method='bicgstab') (core/linear_solver.py:
+ * 14-62) + scipy bicgstab (iterative.py:157-302) and gpu_bicgstab
+ * (core/gpu_solver.py:82-155).  b, x (dev, n); x holds x0 on entry. */
+int kcfd_solve(kcfd_mat *m, const double *b, double *x,
+               const kcfd_solve_opts *opts, kcfd_solve_result *res);
+
+/* Batched: many uneven systems advanced by the same kernels (one launch per
+ * phase covers every active system; a hipGraph holds the iterations).
+ * No reference counterpart: the reference solves one system per call.
+ * bs[i], xs[i] (dev) belong to system i. */
+int kcfd_batch_create(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats,
+                      const double *const *bs, double *const *xs,
+                      const kcfd_solve_opts *opts, kcfd_batch **out);
+int kcfd_batch_run(kcfd_batch *b, kcfd_solve_result *results /*host nsys*/);
+int kcfd_batch_destroy(kcfd_batch *b);
+/* device bytes of the batch workspace */
+int kcfd_batch_bytes(kcfd_batch *b, long long *bytes);
+/* the single-system workspace kcfd_solve cached on a matrix */
+int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out);
+/* Measurement hooks (no reference counterpart; the reference times whole
+ * solves with time.time, core/gpu_solver.py:202-250).  kcfd_batch_profile
+ * re-launches one phase kernel `reps` times on the context stream between
+ * HIP events and returns the mean duration: phase 1 = v = A M p with the
+ * fused rt.v reduction, 2 = fused x/r update, 3 = ILU(0) lower triangular
+ * solve, 4 = p update.  kcfd_batch_kernel_bytes gives that kernel's
+ * algorithmic HBM bytes per launch (summed over the batch). */
+int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms);
+int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes);
+
+/* ------------------------------------------------------ mesh + assembly */
+
+/* SoA mesh on the device (replaces the AoS FVMesh consumed by the operators,
+ * mesh/mesh_reader.py:13-63).  All arrays (host).  Face-major geometry:
+ * normal/fcenter are nf*3, ccenter n*3.  bpatch[f] = patch index of a
+ * boundary face (-1 internal), bidx[f] = its position in the concatenated
+ * per-patch boundary value arrays.  cf_ptr/cf_face/cf_side: cell->face CSR
+ * in face-id order (side +1 owner, -1 neighbour).  The matrix structure is
+ * the canonical CSR of diag U {(O,N),(N,O)}; slot_on/slot_no give, per
+ * face, the CSR positions of (O,N) and (N,O) (-1 on boundary faces or faces
+ * skipped for d_PN < 1e-30) and diag_slot[c] the position of (c,c). */
+typedef struct kcfd_mesh_desc {
+    int n_cells, n_faces, ndim, n_patches, nnz;
+    const int *owner, *neighbour;
+    const double *area, *normal, *fcenter, *ccenter, *volume;
+    const double *d_pn, *d_pf, *gweight;
+    const int *bpatch, *bidx;
+    const int *cf_ptr, *cf_face;
+    const signed char *cf_side;
+    const int *indptr, *indices, *slot_on, *slot_no, *diag_slot;
+} kcfd_mesh_desc;
+
+int kcfd_mesh_create(kcfd_ctx *ctx, const kcfd_mesh_desc *desc, kcfd_mesh **out);
+int kcfd_mesh_destroy(kcfd_mesh *mesh);
+/* a new matrix with the mesh's CSR structure, values / rhs / tracker zero */
+int kcfd_mesh_new_system(kcfd_mesh *mesh, kcfd_mat **out);
+
+/* FVMSystem.reset (core/fvm_operators.py:50-56) */
+int kcfd_asm_reset(kcfd_mat *m);
+/* diffusion_operator (core/fvm_operators.py:59-100); gamma (dev, n) */
+int kcfd_asm_diffusion(kcfd_mesh *mesh, kcfd_mat *m, const double *gamma);
+/* convection_operator_upwind (core/fvm_operators.py:103-129); F (dev, nf) */
+int kcfd_asm_upwind(kcfd_mesh *mesh, kcfd_mat *m, const double *F);
+/* apply_boundary_conditions (core/fvm_operators.py:178-225).  gamma (dev,n),
+ * F (dev,nf), phib (dev, total boundary faces, concatenated per patch);
+ * kinds/values (host, n_patches): KCFD_BC_* and the Neumann gradient. */
+int kcfd_asm_boundary(kcfd_mesh *mesh, kcfd_mat *m, const double *gamma,
+                      const double *F, const double *phib,
+                      const int *kinds, const double *values);
+/* temporal_operator (core/fvm_operators.py:132-148); phi_old (dev, n) */
+int kcfd_asm_temporal(kcfd_mesh *mesh, kcfd_mat *m, double rho, double dt,
+                      const double *phi_old);
+/* source_term (core/fvm_operators.py:151-161); S (dev, n) */
+int kcfd_asm_source(kcfd_mesh *mesh, kcfd_mat *m, const double *S);
+/* linearized_source (core/fvm_operators.py:164-175); Sp, Su (dev, n) */
+int kcfd_asm_linearized(kcfd_mesh *mesh, kcfd_mat *m, const double *Sp,
+                        const double *Su);
+/* diag += kd*V, rhs += ks*V per cell (the implicit drag of
+ * models/two_fluid.py:592-596: kd = K, ks = K*u_other); kd, ks (dev, n) */
+int kcfd_asm_cell_terms(kcfd_mesh *mesh, kcfd_mat *m, const double *kd,
+                        const double *ks);
+/* under_relax (core/fvm_operators.py:228-240); phi (dev, n) */
+int kcfd_asm_under_relax(kcfd_mat *m, const double *phi, double alpha);
+/* SIMPLE pressure-correction coefficients (models/single_phase.py:288-325):
+ * df = rho*0.5*(dP_O+dP_N)*A/max(|d.n|, 0.1 d) on internal faces, and
+ * rho*dP_O*A/d_Pf on faces of patches with pdirichlet[p] != 0.  dP (dev n),
+ * pdirichlet (host, n_patches).  Writes diag, off-diagonals; rhs untouched. */
+int kcfd_asm_pressure(kcfd_mesh *mesh, kcfd_mat *m, const double *dP,
+                      double rho, const int *pdirichlet);
+/* green_gauss_gradient (core/gradient.py:12-48); phi (dev n), phib (dev,
+ * boundary faces), grad (dev, n*ndim, row-major) */
+int kcfd_grad_gg(kcfd_mesh *mesh, const double *phi, const double *phib,
+                 double *grad);
+/* muscl_deferred_correction (core/interpolation.py:227-303); corr (dev n) */
+int kcfd_muscl(kcfd_mesh *mesh, const double *phi, const double *F,
+               const double *grad, int limiter, double *corr);
+/* compute_mass_flux (core/interpolation.py:163-182); U (dev n*ndim),
+ * Ub (dev, boundary faces*ndim), F (dev nf) */
+int kcfd_mass_flux(kcfd_mesh *mesh, const double *U, const double *Ub,
+                   double rho, double *F);
+/* device-time of the last kernel sequence issued by an asm/grad call (ms) */
+int kcfd_last_kernel_ms(kcfd_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KCFD_H */

@@ -1,0 +1,1205 @@
+// Batched, device-resident BiCGSTAB (Jacobi / ILU(0) / none) for gfx950.
+//
+// One kernel launch per phase covers the row tiles of every active system of
+// a batch; each system carries its own scalar state machine on the device
+// (status, rho, alpha, omega, beta, iteration counts), so scalars never leave
+// the device and converged systems drop out without host involvement.  The
+// host replays a captured hipGraph of `check_every` iterations and reads the
+// status words in between.
+//
+// The recurrences are scipy's right-preconditioned BiCGSTAB
+// (scipy/sparse/linalg/_isolve/iterative.py:157-302, the solver behind
+// core/linear_solver.py:59), with the same stopping tests, breakdown tests
+// (rhotol = omegatol = eps^2, rv == 0), half-step exit on ||s||, iteration
+// count (= callback count) and the same operation order of every vector
+// update (products and sums not fused), so that FP64 iterates differ from
+// scipy's only through the summation order of dot products.
+//
+// Per iteration (Jacobi folded into the gathers; ILU(0) adds two sync-free
+// triangular solves after K1 and after K3):
+//   K1  p = r + beta (p - omega v)          (p = r at iteration 0)
+//   K2  v = A M p,  rv = rt.v -> alpha
+//   K3  s = r - alpha v,  ||s|| -> half-step exit
+//   K4  t = A M s,  t.s, t.t -> omega
+//   K5  x += alpha M p; x += omega M s; r = s - omega t;  ||r||, rt.r -> beta
+// Reductions: wave64 xor-shuffle + LDS per block, block partials in a fixed
+// slot, and the last block of each system (atomic ticket) sums the partials
+// in tile order, so results are deterministic run to run.
+//
+// KCFD_MIXED: the same kernels run on FP32 matrix values, FP32 preconditioner
+// and FP32 vectors (products exact in FP64, sums and every dot product in
+// FP64) to solve A d = r; between inner solves K1/K2 perform the FP64
+// refinement x += d, r = b - A x on the FP64 matrix.
+#include "common.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+
+namespace kcfd {
+
+enum : int {
+    S_RUN = 0,       // inner iteration in progress
+    S_START = 1,     // iteration 0 next (p = r)
+    S_HALF = 2,      // half-step exit: x += alpha M p pending
+    S_REFINE = 3,    // MIXED: x += d, r = b - A x pending
+    S_ZEROX = 4,     // ||b|| == 0: x = 0 pending
+    S_DONE = 16,     // terminal
+};
+
+struct SysDesc {
+    int n, tile0, ntiles, pad0;
+    const int *ip, *ix, *dpos;
+    const double *a64;
+    const float *a32;
+    const double *dinv64;
+    const float *dinv32;
+    const double *lu64;
+    const float *lu32;
+    const double *b;
+    double *x;
+    // inner vectors (double in FP64 mode, float in MIXED mode)
+    void *r, *rt, *p, *v, *s, *t, *ph, *sh, *y, *d;
+};
+
+struct SysState {
+    double rho, rho_prev, alpha, omega, beta;
+    double atol_in;    // inner absolute threshold on ||r||
+    double atol_out;   // outer absolute threshold (MIXED)
+    double bnrm;
+    double rr, rr64;
+    double true_rr, true_bb;
+    int status, iters, info, half;
+    int refinements, total_iters, maxiter, inner_iters;
+    unsigned int ticket;
+    int pad[3];
+};
+
+struct BatchParams {
+    double rtol, atol, inner_rtol, rhotol;
+    int precision, precond, max_refine, nsys;
+    int inner_maxiter, pad[3];
+};
+
+// ILU trisolve work chunk: 64 consecutive rows of one system's level list
+struct Chunk {
+    int sys, start, count, pad;
+};
+
+__device__ __forceinline__ int tile_of_block() { return xcd_tile(blockIdx.x, gridDim.x); }
+
+// ------------------------------------------------------------- reductions
+//
+// Block partial -> ticket -> last block of the system sums partials in
+// tile order.  Returns true in *all threads* of the last block, with the
+// system sums broadcast in `tot`.
+template <int NV>
+__device__ __forceinline__ bool reduce_system(double (&v)[NV], const SysDesc &d, SysState &st,
+                                              double *__restrict__ partials, int tile,
+                                              double (&tot)[NV]) {
+    __shared__ int s_last;
+    __shared__ double s_tot[NV > 0 ? NV : 1];
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) partials[(size_t)tile * kMaxPartials + k] = v[k];
+        __threadfence();
+        unsigned prev = atomicAdd(&st.ticket, 1u);
+        s_last = (prev == (unsigned)(d.ntiles - 1));
+    }
+    __syncthreads();
+    if (!s_last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int q = threadIdx.x; q < d.ntiles; q += kBlock) {
+        const volatile double *pp = partials + (size_t)(d.tile0 + q) * kMaxPartials;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += pp[k];
+    }
+    block_sum<NV>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) s_tot[k] = acc[k];
+        atomicExch(&st.ticket, 0u);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = s_tot[k];
+    return true;
+}
+
+// ticket without values (status transitions of elementwise phases)
+__device__ __forceinline__ bool last_block(const SysDesc &d, SysState &st) {
+    __shared__ int s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        unsigned prev = atomicAdd(&st.ticket, 1u);
+        s_last = (prev == (unsigned)(d.ntiles - 1));
+        if (s_last) atomicExch(&st.ticket, 0u);
+    }
+    __syncthreads();
+    return s_last != 0;
+}
+
+// ----------------------------------------------------- element accessors
+
+template <typename V> __device__ __forceinline__ const V *mat_vals(const SysDesc &d);
+template <> __device__ __forceinline__ const double *mat_vals<double>(const SysDesc &d) { return d.a64; }
+template <> __device__ __forceinline__ const float *mat_vals<float>(const SysDesc &d) { return d.a32; }
+template <typename V> __device__ __forceinline__ const V *jac(const SysDesc &d);
+template <> __device__ __forceinline__ const double *jac<double>(const SysDesc &d) { return d.dinv64; }
+template <> __device__ __forceinline__ const float *jac<float>(const SysDesc &d) { return d.dinv32; }
+template <typename V> __device__ __forceinline__ const V *luv(const SysDesc &d);
+template <> __device__ __forceinline__ const double *luv<double>(const SysDesc &d) { return d.lu64; }
+template <> __device__ __forceinline__ const float *luv<float>(const SysDesc &d) { return d.lu32; }
+
+template <typename V> struct Sentinel;
+template <> struct Sentinel<double> {
+    using U = unsigned long long;
+    static constexpr U bits = 0x7FF4DEADBEEF0B0BULL;  // a signalling-NaN payload
+};
+template <> struct Sentinel<float> {
+    using U = unsigned int;
+    static constexpr U bits = 0x7FA5B0B5u;
+};
+
+template <typename V> __device__ __forceinline__ V sentinel_value() {
+    return __builtin_bit_cast(V, Sentinel<V>::bits);
+}
+
+// preconditioned operand M z at row j: Jacobi dinv_j*z_j, ILU: zh_j, none: z_j
+template <typename V, int PC>
+__device__ __forceinline__ double precond_at(const SysDesc &d, const V *__restrict__ z,
+                                             const V *__restrict__ zh, int j) {
+#pragma clang fp contract(off)
+    if (PC == KCFD_PC_JACOBI) return (double)(jac<V>(d)[j] * z[j]);
+    if (PC == KCFD_PC_ILU0) return (double)zh[j];
+    return (double)z[j];
+}
+
+// (A M z)_i, sequential over the row in CSR order
+template <typename V, int PC>
+__device__ __forceinline__ double row_amz(const SysDesc &d, int i, const V *__restrict__ z,
+                                          const V *__restrict__ zh) {
+#pragma clang fp contract(off)
+    const V *__restrict__ a = mat_vals<V>(d);
+    double acc = 0.0;
+    const int q0 = d.ip[i], q1 = d.ip[i + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int j = d.ix[q];
+        acc += (double)a[q] * precond_at<V, PC>(d, z, zh, j);
+    }
+    return acc;
+}
+
+// ------------------------------------------------------------------ init
+
+// FP64: r = b - A x, rt = r; sums ||b||^2, ||r||^2.  MIXED: ||b||^2; d = 0.
+// ILU: every triangular-solve buffer starts at the sentinel.
+template <typename V, int PC>
+__global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys,
+                                                 SysState *__restrict__ sts,
+                                                 const int2 *__restrict__ tiles, int ntiles,
+                                                 double *__restrict__ partials, BatchParams bp) {
+#pragma clang fp contract(off)
+    const int tile = tile_of_block();
+    if (tile >= ntiles) return;
+    const int2 ti = tiles[tile];
+    const SysDesc &d = sys[ti.x];
+    SysState &st = sts[ti.x];
+    if (st.status != S_START) return;
+    const int i = ti.y + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < d.n) {
+        const double bi = d.b[i];
+        v[0] = bi * bi;
+        if (bp.precision == KCFD_FP64) {
+            const double *__restrict__ a = d.a64;
+            double acc = 0.0;
+            for (int q = d.ip[i]; q < d.ip[i + 1]; ++q) acc += a[q] * d.x[d.ix[q]];
+            const double ri = bi - acc;
+            ((double *)d.r)[i] = ri;
+            ((double *)d.rt)[i] = ri;
+            v[1] = ri * ri;
+        } else {
+            ((V *)d.d)[i] = (V)0;
+        }
+        if (PC == KCFD_PC_ILU0) {
+            ((V *)d.y)[i] = sentinel_value<V>();
+            ((V *)d.ph)[i] = sentinel_value<V>();
+            ((V *)d.sh)[i] = sentinel_value<V>();
+        }
+    }
+    double tot[2];
+    if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
+    if (threadIdx.x != 0) return;
+    const double bnrm = sqrt(tot[0]);
+    st.bnrm = bnrm;
+    const double thr = fmax(bp.atol, bp.rtol * bnrm);
+    st.iters = 0;
+    st.total_iters = 0;
+    st.refinements = 0;
+    st.half = 0;
+    st.info = 0;
+    st.inner_iters = 0;
+    if (bnrm == 0.0) {          // scipy returns x = b (= 0), info 0
+        st.status = S_ZEROX;
+        return;
+    }
+    if (bp.precision == KCFD_FP64) {
+        st.atol_in = thr;
+        st.rr = tot[1];
+        if (st.maxiter <= 0) { st.status = S_DONE; return; }
+        if (sqrt(tot[1]) < thr) { st.status = S_DONE; return; }
+        st.rho = tot[1];           // rt = r, so rt.r = ||r||^2
+        if (fabs(st.rho) < bp.rhotol) { st.info = KCFD_INFO_RHO_BREAKDOWN; st.status = S_DONE; return; }
+        st.status = S_START;
+    } else {
+        st.atol_out = thr;
+        st.status = S_REFINE;
+    }
+}
+
+// -------------------------------------------------------------------- K1
+// p-update / p = r / MIXED x += d, d = 0 / x = 0 for a zero rhs.
+// Block gridDim.x-1 (beyond the tiles) resets the ILU chunk counters.
+template <typename V, int PC>
+__global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
+                                              SysState *__restrict__ sts,
+                                              const int2 *__restrict__ tiles, int ntiles,
+                                              unsigned int *__restrict__ counters) {
+#pragma clang fp contract(off)
+    if (blockIdx.x == gridDim.x - 1) {
+        if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
+        return;
+    }
+    const int tile = xcd_tile(blockIdx.x, gridDim.x - 1);
+    if (tile >= ntiles) return;
+    const int2 ti = tiles[tile];
+    const SysDesc &d = sys[ti.x];
+    SysState &st = sts[ti.x];
+    const int status = st.status;
+    const int i = ti.y + threadIdx.x;
+    if (status == S_RUN || status == S_START) {
+        if (i < d.n) {
+            V *__restrict__ p = (V *)d.p;
+            const V ri = ((const V *)d.r)[i];
+            if (status == S_START) {
+                p[i] = ri;
+            } else {
+                const V om = (V)st.omega, be = (V)st.beta;
+                V pi = p[i] - om * ((const V *)d.v)[i];
+                pi = pi * be;
+                p[i] = pi + ri;
+            }
+        }
+        return;
+    }
+    if (status == S_REFINE) {
+        if (i < d.n) {
+            V *__restrict__ dd = (V *)d.d;
+            d.x[i] = d.x[i] + (double)dd[i];
+            dd[i] = (V)0;
+            if (PC == KCFD_PC_ILU0) {
+                ((V *)d.ph)[i] = sentinel_value<V>();
+                ((V *)d.sh)[i] = sentinel_value<V>();
+                ((V *)d.y)[i] = sentinel_value<V>();
+            }
+        }
+        return;
+    }
+    if (status == S_ZEROX) {
+        if (i < d.n) d.x[i] = 0.0;
+        if (last_block(d, st) && threadIdx.x == 0) st.status = S_DONE;
+    }
+}
+
+// -------------------------------------------------------------------- K2
+// v = A M p; rt.v -> alpha.   MIXED S_REFINE: r64 = b - A64 x; r = rt = (V)r64.
+template <typename V, int PC>
+__global__ __launch_bounds__(kBlock) void k_spmv_p(const SysDesc *__restrict__ sys,
+                                                   SysState *__restrict__ sts,
+                                                   const int2 *__restrict__ tiles, int ntiles,
+                                                   double *__restrict__ partials, BatchParams bp) {
+#pragma clang fp contract(off)
+    const int tile = tile_of_block();
+    if (tile >= ntiles) return;
+    const int2 ti = tiles[tile];
+    const SysDesc &d = sys[ti.x];
+    SysState &st = sts[ti.x];
+    const int status = st.status;
+    const int i = ti.y + threadIdx.x;
+    if (status == S_RUN || status == S_START) {
+        double v[1] = {0.0};
+        if (i < d.n) {
+            const double vi = row_amz<V, PC>(d, i, (const V *)d.p, (const V *)d.ph);
+            const V vv = (V)vi;
+            ((V *)d.v)[i] = vv;
+            v[0] = (double)((const V *)d.rt)[i] * (double)vv;
+        }
+        double tot[1];
+        if (!reduce_system<1>(v, d, st, partials, tile, tot)) return;
+        if (threadIdx.x != 0) return;
+        const double rv = tot[0];
+        if (rv == 0.0) {
+            st.info = KCFD_INFO_OMEGA_BREAKDOWN;
+            st.status = bp.precision == KCFD_MIXED ? S_REFINE : S_DONE;
+            if (bp.precision == KCFD_MIXED && st.iters == 0) st.status = S_DONE;
+            return;
+        }
+        st.alpha = st.rho / rv;
+        st.status = S_RUN;
+        return;
+    }
+    if (status == S_REFINE) {
+        double v[2] = {0.0, 0.0};
+        if (i < d.n) {
+            const double *__restrict__ a = d.a64;
+            double acc = 0.0;
+            for (int q = d.ip[i]; q < d.ip[i + 1]; ++q) acc += a[q] * d.x[d.ix[q]];
+            const double r64 = d.b[i] - acc;
+            const V rv = (V)r64;
+            ((V *)d.r)[i] = rv;
+            ((V *)d.rt)[i] = rv;
+            v[0] = r64 * r64;
+            v[1] = (double)rv * (double)rv;
+        }
+        double tot[2];
+        if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
+        if (threadIdx.x != 0) return;
+        const double prev = st.rr64;
+        st.rr64 = tot[0];
+        const double rn = sqrt(tot[0]);
+        if (rn < st.atol_out) { st.info = 0; st.status = S_DONE; return; }
+        // stop when a refinement step no longer halves the FP64 residual
+        const bool stagnant = st.refinements > 0 && !(tot[0] < 0.25 * prev);
+        if (stagnant || st.refinements >= bp.max_refine || st.total_iters >= st.maxiter) {
+            st.info = st.total_iters > 0 ? st.total_iters : 1;
+            st.status = S_DONE;
+            return;
+        }
+        st.refinements += 1;
+        st.atol_in = bp.inner_rtol * rn;
+        st.rr = tot[1];
+        st.rho = tot[1];
+        st.iters = 0;
+        st.info = 0;
+        if (fabs(st.rho) < bp.rhotol) { st.info = KCFD_INFO_RHO_BREAKDOWN; st.status = S_DONE; return; }
+        st.status = S_START;
+    }
+}
+
+// -------------------------------------------------------------------- K3
+// s = r - alpha v; ||s|| -> half-step exit.  Extra last block resets the
+// counters of the second pair of triangular solves.
+template <typename V, int PC>
+__global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
+                                              SysState *__restrict__ sts,
+                                              const int2 *__restrict__ tiles, int ntiles,
+                                              double *__restrict__ partials,
+                                              unsigned int *__restrict__ counters) {
+#pragma clang fp contract(off)
+    if (blockIdx.x == gridDim.x - 1) {
+        if (threadIdx.x < 2) counters[2 + threadIdx.x] = 0u;
+        return;
+    }
+    const int tile = xcd_tile(blockIdx.x, gridDim.x - 1);
+    if (tile >= ntiles) return;
+    const int2 ti = tiles[tile];
+    const SysDesc &d = sys[ti.x];
+    SysState &st = sts[ti.x];
+    if (st.status != S_RUN) return;
+    const int i = ti.y + threadIdx.x;
+    double v[1] = {0.0};
+    if (i < d.n) {
+        const V al = (V)st.alpha;
+        const V si = ((const V *)d.r)[i] - al * ((const V *)d.v)[i];
+        ((V *)d.s)[i] = si;
+        v[0] = (double)si * (double)si;
+    }
+    double tot[1];
+    if (!reduce_system<1>(v, d, st, partials, tile, tot)) return;
+    if (threadIdx.x != 0) return;
+    if (sqrt(tot[0]) < st.atol_in) st.status = S_HALF;
+}
+
+// -------------------------------------------------------------------- K4
+// t = A M s; t.s, t.t -> omega.   S_HALF: x += alpha M p, then finish.
+template <typename V, int PC>
+__global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ sys,
+                                                   SysState *__restrict__ sts,
+                                                   const int2 *__restrict__ tiles, int ntiles,
+                                                   double *__restrict__ partials, BatchParams bp) {
+#pragma clang fp contract(off)
+    const int tile = tile_of_block();
+    if (tile >= ntiles) return;
+    const int2 ti = tiles[tile];
+    const SysDesc &d = sys[ti.x];
+    SysState &st = sts[ti.x];
+    const int status = st.status;
+    const int i = ti.y + threadIdx.x;
+    if (status == S_RUN) {
+        double v[2] = {0.0, 0.0};
+        if (i < d.n) {
+            const double ti_ = row_amz<V, PC>(d, i, (const V *)d.s, (const V *)d.sh);
+            const V tv = (V)ti_;
+            ((V *)d.t)[i] = tv;
+            const double si = (double)((const V *)d.s)[i];
+            v[0] = (double)tv * si;
+            v[1] = (double)tv * (double)tv;
+        }
+        double tot[2];
+        if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
+        if (threadIdx.x != 0) return;
+        st.omega = tot[0] / tot[1];
+        return;
+    }
+    if (status == S_HALF) {
+        if (i < d.n) {
+            const V al = (V)st.alpha;
+            V *__restrict__ xx = bp.precision == KCFD_FP64 ? (V *)d.x : (V *)d.d;
+            const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
+            xx[i] = xx[i] + al * ph;
+            if (PC == KCFD_PC_ILU0) {
+                ((V *)d.ph)[i] = sentinel_value<V>();
+                ((V *)d.sh)[i] = sentinel_value<V>();
+            }
+        }
+        if (last_block(d, st) && threadIdx.x == 0) {
+            st.half = 1;
+            st.total_iters += 0;
+            st.status = bp.precision == KCFD_MIXED ? S_REFINE : S_DONE;
+            st.info = 0;
+        }
+    }
+}
+
+// -------------------------------------------------------------------- K5
+// x += alpha M p; x += omega M s; r = s - omega t; ||r||^2, rt.r -> checks, beta
+template <typename V, int PC>
+__global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
+                                               SysState *__restrict__ sts,
+                                               const int2 *__restrict__ tiles, int ntiles,
+                                               double *__restrict__ partials, BatchParams bp) {
+#pragma clang fp contract(off)
+    const int tile = tile_of_block();
+    if (tile >= ntiles) return;
+    const int2 ti = tiles[tile];
+    const SysDesc &d = sys[ti.x];
+    SysState &st = sts[ti.x];
+    if (st.status != S_RUN) return;
+    const int i = ti.y + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < d.n) {
+        const V al = (V)st.alpha, om = (V)st.omega;
+        V *__restrict__ xx = bp.precision == KCFD_FP64 ? (V *)d.x : (V *)d.d;
+        const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
+        const V sh = (V)precond_at<V, PC>(d, (const V *)d.s, (const V *)d.sh, i);
+        V xi = xx[i] + al * ph;
+        xi = xi + om * sh;
+        xx[i] = xi;
+        const V ri = ((const V *)d.s)[i] - om * ((const V *)d.t)[i];
+        ((V *)d.r)[i] = ri;
+        v[0] = (double)ri * (double)ri;
+        v[1] = (double)((const V *)d.rt)[i] * (double)ri;
+        if (PC == KCFD_PC_ILU0) {
+            ((V *)d.ph)[i] = sentinel_value<V>();
+            ((V *)d.sh)[i] = sentinel_value<V>();
+        }
+    }
+    double tot[2];
+    if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
+    if (threadIdx.x != 0) return;
+    st.iters += 1;
+    st.total_iters += 1;
+    st.rr = tot[0];
+    const bool mixed = bp.precision == KCFD_MIXED;
+    const int next_exit = mixed ? S_REFINE : S_DONE;
+    if (st.total_iters >= st.maxiter) {          // scipy's for-loop exhausted
+        st.info = st.total_iters;
+        st.status = mixed ? S_REFINE : S_DONE;
+        if (mixed) st.maxiter = st.total_iters;  // REFINE will stop after the update
+        return;
+    }
+    if (sqrt(tot[0]) < st.atol_in) { st.info = 0; st.status = next_exit; return; }
+    if (mixed && st.iters >= bp.inner_maxiter) { st.status = S_REFINE; return; }
+    const double rho = tot[1];
+    if (fabs(rho) < bp.rhotol) {
+        st.info = KCFD_INFO_RHO_BREAKDOWN;
+        st.status = next_exit;
+        return;
+    }
+    if (fabs(st.omega) < bp.rhotol) {
+        st.info = KCFD_INFO_OMEGA_BREAKDOWN;
+        st.status = next_exit;
+        return;
+    }
+    st.beta = (rho / st.rho) * (st.alpha / st.omega);
+    st.rho_prev = st.rho;
+    st.rho = rho;
+}
+
+// ------------------------------------------------------- true residual
+// ||b - A x||^2 and ||b||^2 in FP64 on the FP64 matrix, for every system.
+__global__ __launch_bounds__(kBlock) void k_true_resid(const SysDesc *__restrict__ sys,
+                                                       SysState *__restrict__ sts,
+                                                       const int2 *__restrict__ tiles, int ntiles,
+                                                       double *__restrict__ partials) {
+#pragma clang fp contract(off)
+    const int tile = tile_of_block();
+    if (tile >= ntiles) return;
+    const int2 ti = tiles[tile];
+    const SysDesc &d = sys[ti.x];
+    SysState &st = sts[ti.x];
+    const int i = ti.y + threadIdx.x;
+    double v[2] = {0.0, 0.0};
+    if (i < d.n) {
+        double acc = 0.0;
+        for (int q = d.ip[i]; q < d.ip[i + 1]; ++q) acc += d.a64[q] * d.x[d.ix[q]];
+        const double bi = d.b[i];
+        const double ri = bi - acc;
+        v[0] = ri * ri;
+        v[1] = bi * bi;
+    }
+    double tot[2];
+    if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
+    if (threadIdx.x != 0) return;
+    st.true_rr = tot[0];
+    st.true_bb = tot[1];
+}
+
+// ---------------------------------------------------- ILU(0) trisolves
+//
+// Sync-free level-scheduled triangular solves: one launch per solve covers
+// all systems; 64-row chunks (one per wave, never spanning a level) are
+// taken in level-major order through an atomic ticket, so every row a wave
+// waits on belongs to a chunk already held by a running wave.  The output
+// vector doubles as the ready flag: it holds a NaN sentinel until the row is
+// written (agent-scope atomic store) and waiters poll it with agent-scope
+// atomic loads.  Row arithmetic follows the oracle's order exactly.
+//   LOWER: y_i = z_i - sum_{j<i} l_ij y_j               (unit L)
+//   UPPER: w_i = (y_i - sum_{j>i} u_ij w_j) / u_ii; y_i reset to sentinel
+template <typename V>
+__device__ __forceinline__ V poll_ready(const V *p) {
+    using U = typename Sentinel<V>::U;
+    const U *q = reinterpret_cast<const U *>(p);
+    U b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (b == Sentinel<V>::bits) {
+        __builtin_amdgcn_s_sleep(1);
+        b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return __builtin_bit_cast(V, b);
+}
+
+template <typename V>
+__device__ __forceinline__ void publish(V *p, V v) {
+    using U = typename Sentinel<V>::U;
+    __hip_atomic_store(reinterpret_cast<U *>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename V, bool UPPER, int WHICH>
+__global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys,
+                                                 const SysState *__restrict__ sts,
+                                                 const Chunk *__restrict__ chunks, int nchunks,
+                                                 const int *const *__restrict__ lvl_rows,
+                                                 unsigned int *__restrict__ counter) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    for (;;) {
+        int c = 0;
+        if (lane == 0) c = (int)atomicAdd(counter, 1u);
+        c = __shfl(c, 0, 64);
+        if (c >= nchunks) return;
+        const Chunk ch = chunks[c];
+        const SysDesc &d = sys[ch.sys];
+        const int status = sts[ch.sys].status;
+        if (status != S_RUN && status != S_START) continue;
+        if (lane >= ch.count) continue;
+        const int i = lvl_rows[ch.sys][ch.start + lane];
+        const V *__restrict__ a = luv<V>(d);
+        const int dp = d.dpos[i];
+        if (!UPPER) {
+            const V *z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
+            V *y = (V *)d.y;
+            V acc = z[i];
+            for (int q = d.ip[i]; q < dp; ++q) acc -= a[q] * poll_ready<V>(y + d.ix[q]);
+            publish<V>(y + i, acc);
+        } else {
+            V *y = (V *)d.y;
+            V *w = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
+            V acc = y[i];
+            for (int q = dp + 1; q < d.ip[i + 1]; ++q) acc -= a[q] * poll_ready<V>(w + d.ix[q]);
+            publish<V>(w + i, acc / a[dp]);
+            y[i] = sentinel_value<V>();
+        }
+    }
+}
+
+template <typename V>
+__global__ void k_fill_sentinel(V *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = sentinel_value<V>();
+}
+
+}  // namespace kcfd
+
+using namespace kcfd;
+
+// ======================================================================= host
+
+struct kcfd_batch {
+    kcfd_ctx *ctx = nullptr;
+    int nsys = 0, ntiles = 0, grid = 0;
+    kcfd_solve_opts opts{};
+    BatchParams bp{};
+    std::vector<kcfd_mat *> mats;
+    std::vector<SysDesc> h_desc;
+    std::vector<int> sys_n;
+    SysDesc *d_desc = nullptr;
+    SysState *d_state = nullptr;
+    std::vector<SysState> h_state;
+    int2 *d_tiles = nullptr;
+    double *d_partials = nullptr;
+    unsigned int *d_counters = nullptr;
+    void *slab = nullptr;          // all per-system vectors
+    size_t slab_bytes = 0;
+    // ILU chunk tables (level-major over systems) for the L and U solves
+    Chunk *d_chunks_l = nullptr, *d_chunks_u = nullptr;
+    int nchunks_l = 0, nchunks_u = 0, trsv_grid = 0;
+    const int **d_lvl_l = nullptr, **d_lvl_u = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int graph_iters = 0;
+};
+
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+template <typename V, int PC>
+void launch_iteration(kcfd_batch *b, hipStream_t s) {
+    const int g = b->grid;
+    const dim3 blk(kBlock);
+    hipLaunchKernelGGL((k_p<V, PC>), dim3(g + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
+                       b->ntiles, b->d_counters);
+    if (PC == KCFD_PC_ILU0) {
+        hipLaunchKernelGGL((k_trsv<V, false, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                           b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters + 0);
+        hipLaunchKernelGGL((k_trsv<V, true, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                           b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u, b->d_counters + 1);
+    }
+    hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
+                       b->ntiles, b->d_partials, b->bp);
+    hipLaunchKernelGGL((k_s<V, PC>), dim3(g + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
+                       b->ntiles, b->d_partials, b->d_counters);
+    if (PC == KCFD_PC_ILU0) {
+        hipLaunchKernelGGL((k_trsv<V, false, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                           b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters + 2);
+        hipLaunchKernelGGL((k_trsv<V, true, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                           b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u, b->d_counters + 3);
+    }
+    hipLaunchKernelGGL((k_spmv_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
+                       b->ntiles, b->d_partials, b->bp);
+    hipLaunchKernelGGL((k_xr<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
+                       b->ntiles, b->d_partials, b->bp);
+}
+
+template <typename V, int PC>
+void launch_init(kcfd_batch *b, hipStream_t s) {
+    hipLaunchKernelGGL((k_init<V, PC>), dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                       b->d_tiles, b->ntiles, b->d_partials, b->bp);
+}
+
+using LaunchFn = void (*)(kcfd_batch *, hipStream_t);
+
+LaunchFn pick_iter(int precision, int pc) {
+    if (precision == KCFD_FP64) {
+        if (pc == KCFD_PC_JACOBI) return launch_iteration<double, KCFD_PC_JACOBI>;
+        if (pc == KCFD_PC_ILU0) return launch_iteration<double, KCFD_PC_ILU0>;
+        return launch_iteration<double, KCFD_PC_NONE>;
+    }
+    if (pc == KCFD_PC_JACOBI) return launch_iteration<float, KCFD_PC_JACOBI>;
+    if (pc == KCFD_PC_ILU0) return launch_iteration<float, KCFD_PC_ILU0>;
+    return launch_iteration<float, KCFD_PC_NONE>;
+}
+
+LaunchFn pick_init(int precision, int pc) {
+    if (precision == KCFD_FP64) {
+        if (pc == KCFD_PC_JACOBI) return launch_init<double, KCFD_PC_JACOBI>;
+        if (pc == KCFD_PC_ILU0) return launch_init<double, KCFD_PC_ILU0>;
+        return launch_init<double, KCFD_PC_NONE>;
+    }
+    if (pc == KCFD_PC_JACOBI) return launch_init<float, KCFD_PC_JACOBI>;
+    if (pc == KCFD_PC_ILU0) return launch_init<float, KCFD_PC_ILU0>;
+    return launch_init<float, KCFD_PC_NONE>;
+}
+
+// algorithmic bytes (roofline model): CSR stream, ILU factor stream
+double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st) {
+    const double n = m->n, nnz = m->nnz;
+    auto MA = [&](double sv) { return nnz * (sv + 4) + (n + 1) * 4; };
+    auto MF = [&](double sv) { return nnz * (sv + 4) + 2 * (n + 1) * 4; };
+    auto per_iter = [&](double sv) {
+        if (pc == KCFD_PC_JACOBI) return 2 * MA(sv) + 22 * n * sv;
+        if (pc == KCFD_PC_ILU0) return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv;
+        return 2 * MA(sv) + 19 * n * sv;
+    };
+    if (precision == KCFD_FP64) {
+        return (st.total_iters + 0.5 * st.half) * per_iter(8) + MA(8) + 3 * n * 8;
+    }
+    return st.total_iters * per_iter(4) + (st.refinements + 1) * (MA(8) + 6 * n * 8);
+}
+
+int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *const *bs,
+                double *const *xs, const kcfd_solve_opts *o, kcfd_batch **out) {
+    const int pc = o->precond, prec = o->precision;
+    KCFD_ARG(pc == KCFD_PC_NONE || pc == KCFD_PC_JACOBI || pc == KCFD_PC_ILU0,
+             "unknown preconditioner");
+    KCFD_ARG(prec == KCFD_FP64 || prec == KCFD_MIXED, "unknown precision");
+    kcfd_batch *b = new kcfd_batch();
+    b->ctx = ctx;
+    b->nsys = nsys;
+    b->opts = *o;
+    if (b->opts.check_every <= 0) b->opts.check_every = 8;
+    b->bp.rtol = o->rtol;
+    b->bp.atol = o->atol;
+    b->bp.inner_rtol = o->inner_rtol > 0 ? o->inner_rtol : 1e-4;
+    const double eps = 2.220446049250313e-16;
+    b->bp.rhotol = eps * eps;
+    b->bp.precision = prec;
+    b->bp.precond = pc;
+    b->bp.max_refine = o->max_refine > 0 ? o->max_refine : 30;
+    b->bp.nsys = nsys;
+    b->bp.inner_maxiter = 2000;
+    b->mats.assign(mats, mats + nsys);
+    auto fail = [&](int rc) {
+        kcfd_batch_destroy(b);
+        return rc;
+    };
+    // tiles
+    std::vector<int2> tiles;
+    b->h_desc.resize(nsys);
+    const size_t vs = prec == KCFD_FP64 ? sizeof(double) : sizeof(float);
+    const int nvec = pc == KCFD_PC_ILU0 ? 10 : 7;
+    size_t slab = 0;
+    std::vector<size_t> off(nsys);
+    for (int k = 0; k < nsys; ++k) {
+        kcfd_mat *m = mats[k];
+        if (m->ctx != ctx) { set_error("matrix %d belongs to another context", k); return fail(KCFD_EARG); }
+        if (m->pc_kind != pc) {
+            set_error("matrix %d: preconditioner not set up as requested (have %d, want %d)", k,
+                      m->pc_kind, pc);
+            return fail(KCFD_ESTATE);
+        }
+        if (prec == KCFD_MIXED && !m->have32) {
+            set_error("matrix %d: no FP32 copy (set up the preconditioner with KCFD_MIXED)", k);
+            return fail(KCFD_ESTATE);
+        }
+        SysDesc &d = b->h_desc[k];
+        std::memset(&d, 0, sizeof(d));
+        d.n = m->n;
+        d.tile0 = (int)tiles.size();
+        const int nt = std::max(1, (m->n + kBlock - 1) / kBlock);
+        d.ntiles = nt;
+        for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(k, t * kBlock));
+        d.ip = m->indptr;
+        d.ix = m->indices;
+        d.dpos = m->diagpos;
+        d.a64 = m->val;
+        d.a32 = m->val32;
+        d.dinv64 = m->dinv;
+        d.dinv32 = m->dinv32;
+        d.lu64 = m->lu;
+        d.lu32 = m->lu32;
+        d.b = bs[k];
+        d.x = xs[k];
+        off[k] = slab;
+        slab += align_up((size_t)nvec * align_up((size_t)std::max(m->n, 1) * vs, 256), 256);
+        b->sys_n.push_back(m->n);
+    }
+    b->ntiles = (int)tiles.size();
+    b->grid = xcd_grid(b->ntiles);
+    KCFD_HIP(hipMalloc(&b->slab, slab ? slab : 256));
+    b->slab_bytes = slab;
+    for (int k = 0; k < nsys; ++k) {
+        SysDesc &d = b->h_desc[k];
+        const size_t stride = align_up((size_t)std::max(d.n, 1) * vs, 256);
+        char *base = (char *)b->slab + off[k];
+        void **vecs[10] = {&d.r, &d.rt, &d.p, &d.v, &d.s, &d.t, &d.d, &d.ph, &d.sh, &d.y};
+        for (int q = 0; q < nvec; ++q) *vecs[q] = base + q * stride;
+        if (prec == KCFD_FP64) d.d = d.x;
+    }
+    KCFD_HIP(hipMalloc(&b->d_desc, sizeof(SysDesc) * nsys));
+    KCFD_HIP(hipMemcpy(b->d_desc, b->h_desc.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice));
+    KCFD_HIP(hipMalloc(&b->d_state, sizeof(SysState) * nsys));
+    b->h_state.assign(nsys, SysState{});
+    KCFD_HIP(hipMalloc(&b->d_tiles, sizeof(int2) * b->ntiles));
+    KCFD_HIP(hipMemcpy(b->d_tiles, tiles.data(), sizeof(int2) * b->ntiles, hipMemcpyHostToDevice));
+    KCFD_HIP(hipMalloc(&b->d_partials, sizeof(double) * kMaxPartials * b->ntiles));
+    KCFD_HIP(hipMalloc(&b->d_counters, sizeof(unsigned int) * 8));
+    KCFD_HIP(hipMemset(b->d_counters, 0, sizeof(unsigned int) * 8));
+    if (pc == KCFD_PC_ILU0) {
+        // level-major chunk tables over all systems
+        int maxlev_l = 0, maxlev_u = 0;
+        for (auto *m : b->mats) {
+            maxlev_l = std::max(maxlev_l, m->nlev_l);
+            maxlev_u = std::max(maxlev_u, m->nlev_u);
+        }
+        auto build_chunks = [&](bool upper, std::vector<Chunk> &ch) {
+            const int L = upper ? maxlev_u : maxlev_l;
+            for (int l = 0; l < L; ++l) {
+                for (int k = 0; k < nsys; ++k) {
+                    kcfd_mat *m = b->mats[k];
+                    const std::vector<int> &lp = upper ? m->h_lvl_ptr_u : m->h_lvl_ptr_l;
+                    const int nl = upper ? m->nlev_u : m->nlev_l;
+                    if (l >= nl) continue;
+                    for (int s0 = lp[l]; s0 < lp[l + 1]; s0 += 64) {
+                        Chunk c;
+                        c.sys = k;
+                        c.start = s0;
+                        c.count = std::min(64, lp[l + 1] - s0);
+                        c.pad = 0;
+                        ch.push_back(c);
+                    }
+                }
+            }
+        };
+        std::vector<Chunk> cl, cu;
+        build_chunks(false, cl);
+        build_chunks(true, cu);
+        b->nchunks_l = (int)cl.size();
+        b->nchunks_u = (int)cu.size();
+        KCFD_HIP(hipMalloc(&b->d_chunks_l, sizeof(Chunk) * std::max<size_t>(1, cl.size())));
+        KCFD_HIP(hipMalloc(&b->d_chunks_u, sizeof(Chunk) * std::max<size_t>(1, cu.size())));
+        if (!cl.empty())
+            KCFD_HIP(hipMemcpy(b->d_chunks_l, cl.data(), sizeof(Chunk) * cl.size(), hipMemcpyHostToDevice));
+        if (!cu.empty())
+            KCFD_HIP(hipMemcpy(b->d_chunks_u, cu.data(), sizeof(Chunk) * cu.size(), hipMemcpyHostToDevice));
+        std::vector<const int *> pl(nsys), pu(nsys);
+        for (int k = 0; k < nsys; ++k) {
+            pl[k] = b->mats[k]->lvl_rows_l;
+            pu[k] = b->mats[k]->lvl_rows_u;
+        }
+        KCFD_HIP(hipMalloc((void **)&b->d_lvl_l, sizeof(int *) * nsys));
+        KCFD_HIP(hipMalloc((void **)&b->d_lvl_u, sizeof(int *) * nsys));
+        KCFD_HIP(hipMemcpy((void *)b->d_lvl_l, pl.data(), sizeof(int *) * nsys, hipMemcpyHostToDevice));
+        KCFD_HIP(hipMemcpy((void *)b->d_lvl_u, pu.data(), sizeof(int *) * nsys, hipMemcpyHostToDevice));
+        // persistent-style grid: enough waves to keep every CU busy; chunks
+        // are handed out dynamically, so any grid size is deadlock-free
+        const int want = std::max(b->nchunks_l, b->nchunks_u);
+        int cap = ctx->cus * 8;  // 8 blocks (32 waves) per CU
+        b->trsv_grid = std::max(1, std::min((want + kWaves - 1) / kWaves, cap));
+    }
+    *out = b;
+    return KCFD_OK;
+}
+
+int capture_graph(kcfd_batch *b, int iters) {
+    hipStream_t s = b->ctx->stream;
+    if (b->graph) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    LaunchFn fn = pick_iter(b->bp.precision, b->bp.precond);
+    hipGraph_t g = nullptr;
+    KCFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < iters; ++k) fn(b, s);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (e != hipSuccess) {
+        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+        return KCFD_EHIP;
+    }
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+        return KCFD_EHIP;
+    }
+    b->graph_iters = iters;
+    return KCFD_OK;
+}
+
+int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
+    kcfd_ctx *ctx = b->ctx;
+    hipStream_t s = ctx->stream;
+    // reset states
+    for (int k = 0; k < b->nsys; ++k) {
+        SysState st{};
+        const int n = b->sys_n[k];
+        st.status = S_START;
+        st.maxiter = b->opts.maxiter > 0 ? b->opts.maxiter : 10 * std::max(n, 1);
+        st.ticket = 0;
+        b->h_state[k] = st;
+    }
+    KCFD_HIP(hipMemcpyAsync(b->d_state, b->h_state.data(), sizeof(SysState) * b->nsys,
+                            hipMemcpyHostToDevice, s));
+    if (!b->graph || b->graph_iters != b->opts.check_every)
+        KCFD_CHECK(capture_graph(b, b->opts.check_every));
+    KCFD_HIP(hipEventRecord(ctx->ev0, s));
+    pick_init(b->bp.precision, b->bp.precond)(b, s);
+    KCFD_HIP(hipGetLastError());
+    // host loop: replay the captured iterations until every system is done
+    long long guard = 0;
+    int maxit = 0;
+    for (int k = 0; k < b->nsys; ++k) maxit = std::max(maxit, b->h_state[k].maxiter);
+    const long long limit = (long long)maxit * (b->bp.precision == KCFD_MIXED ? 2 : 1) +
+                            (long long)b->bp.max_refine * 2 + 64;
+    for (;;) {
+        KCFD_HIP(hipGraphLaunch(b->graph, s));
+        guard += b->graph_iters;
+        KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
+                                hipMemcpyDeviceToHost, s));
+        KCFD_HIP(hipStreamSynchronize(s));
+        bool all = true;
+        for (int k = 0; k < b->nsys; ++k)
+            if (b->h_state[k].status != S_DONE) { all = false; break; }
+        if (all) break;
+        if (guard > limit) {
+            set_error("solver did not reach a terminal state after %lld graph iterations", guard);
+            return KCFD_ESTATE;
+        }
+    }
+    KCFD_HIP(hipEventRecord(ctx->ev1, s));
+    hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                       b->d_tiles, b->ntiles, b->d_partials);
+    KCFD_HIP(hipGetLastError());
+    KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
+                            hipMemcpyDeviceToHost, s));
+    KCFD_HIP(hipStreamSynchronize(s));
+    float ms = 0.f;
+    KCFD_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    ctx->last_ms = ms;
+    if (res) {
+        for (int k = 0; k < b->nsys; ++k) {
+            const SysState &st = b->h_state[k];
+            kcfd_solve_result &r = res[k];
+            r.info = st.info;
+            r.iters = b->bp.precision == KCFD_FP64 ? st.iters : st.total_iters;
+            r.half_step = st.half;
+            r.refinements = st.refinements;
+            r.rel_resid = st.true_bb > 0 ? sqrt(st.true_rr / st.true_bb) : sqrt(st.true_rr);
+            r.bytes = model_bytes(b->mats[k], b->bp.precond, b->bp.precision, st);
+            r.ms = ms;
+        }
+    }
+    return KCFD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kcfd_batch_create(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *const *bs,
+                      double *const *xs, const kcfd_solve_opts *opts, kcfd_batch **out) {
+    KCFD_ARG(ctx && mats && bs && xs && opts && out, "null argument");
+    KCFD_ARG(nsys > 0, "empty batch");
+    for (int k = 0; k < nsys; ++k) KCFD_ARG(mats[k] && bs[k] && xs[k], "null system entry");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    KCFD_HIP(hipSetDevice(ctx->device));
+    return build_batch(ctx, nsys, mats, bs, xs, opts, out);
+}
+
+int kcfd_batch_run(kcfd_batch *b, kcfd_solve_result *results) {
+    KCFD_ARG(b != nullptr, "null batch");
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    KCFD_HIP(hipSetDevice(b->ctx->device));
+    return run_batch(b, results);
+}
+
+int kcfd_batch_destroy(kcfd_batch *b) {
+    if (!b) return KCFD_OK;
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    hipSetDevice(b->ctx->device);
+    hipStreamSynchronize(b->ctx->stream);
+    if (b->graph) hipGraphExecDestroy(b->graph);
+    hipFree(b->d_desc);
+    hipFree(b->d_state);
+    hipFree(b->d_tiles);
+    hipFree(b->d_partials);
+    hipFree(b->d_counters);
+    hipFree(b->slab);
+    hipFree(b->d_chunks_l);
+    hipFree(b->d_chunks_u);
+    hipFree((void *)b->d_lvl_l);
+    hipFree((void *)b->d_lvl_u);
+    delete b;
+    return KCFD_OK;
+}
+
+int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out) {
+    KCFD_ARG(m && out, "null argument");
+    *out = m->solver;
+    if (!m->solver) {
+        set_error("no cached solver: call kcfd_solve first");
+        return KCFD_ESTATE;
+    }
+    return KCFD_OK;
+}
+
+int kcfd_batch_bytes(kcfd_batch *b, long long *bytes) {
+    KCFD_ARG(b && bytes, "null argument");
+    *bytes = (long long)b->slab_bytes + (long long)b->ntiles * (8 + 8 * kMaxPartials) +
+             (long long)b->nsys * (long long)(sizeof(SysDesc) + sizeof(SysState)) +
+             (long long)(b->nchunks_l + b->nchunks_u) * (long long)sizeof(Chunk);
+    return KCFD_OK;
+}
+
+// Re-launch one phase kernel `reps` times with every system forced to the
+// running state, bracketed by HIP events on the context stream; restores the
+// states afterwards.  phase: 1 = K2 (v = A M p + rt.v), 2 = K5 (x/r update),
+// 3 = lower triangular solve (ILU only), 4 = K1 (p update).
+int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
+    KCFD_ARG(b && avg_ms && reps > 0, "bad argument");
+    KCFD_ARG(phase >= 1 && phase <= 4, "unknown phase");
+    KCFD_ARG(phase != 3 || b->bp.precond == KCFD_PC_ILU0, "phase 3 needs ILU(0)");
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    KCFD_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    KCFD_HIP(hipStreamSynchronize(s));
+    std::vector<SysState> saved(b->nsys), run(b->nsys);
+    KCFD_HIP(hipMemcpy(saved.data(), b->d_state, sizeof(SysState) * b->nsys, hipMemcpyDeviceToHost));
+    for (int k = 0; k < b->nsys; ++k) {
+        run[k] = saved[k];
+        run[k].status = S_RUN;
+        run[k].ticket = 0;
+        if (run[k].omega == 0.0) run[k].omega = 1.0;
+        if (run[k].rho == 0.0) run[k].rho = 1.0;
+    }
+    KCFD_HIP(hipMemcpy(b->d_state, run.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice));
+    const bool f64 = b->bp.precision == KCFD_FP64;
+    const int pc = b->bp.precond;
+    auto launch = [&]() {
+        const dim3 blk(kBlock);
+        if (phase == 1) {
+            if (f64) {
+                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_spmv_p<double, KCFD_PC_JACOBI>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_spmv_p<double, KCFD_PC_ILU0>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+                else hipLaunchKernelGGL((k_spmv_p<double, KCFD_PC_NONE>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+            } else {
+                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_spmv_p<float, KCFD_PC_JACOBI>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_spmv_p<float, KCFD_PC_ILU0>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+                else hipLaunchKernelGGL((k_spmv_p<float, KCFD_PC_NONE>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+            }
+        } else if (phase == 2) {
+            if (f64) {
+                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_xr<double, KCFD_PC_JACOBI>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_xr<double, KCFD_PC_ILU0>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+                else hipLaunchKernelGGL((k_xr<double, KCFD_PC_NONE>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+            } else {
+                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_xr<float, KCFD_PC_JACOBI>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_xr<float, KCFD_PC_ILU0>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+                else hipLaunchKernelGGL((k_xr<float, KCFD_PC_NONE>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
+            }
+        } else if (phase == 4) {
+            if (f64) {
+                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_p<double, KCFD_PC_JACOBI>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
+                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_p<double, KCFD_PC_ILU0>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
+                else hipLaunchKernelGGL((k_p<double, KCFD_PC_NONE>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
+            } else {
+                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_p<float, KCFD_PC_JACOBI>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
+                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_p<float, KCFD_PC_ILU0>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
+                else hipLaunchKernelGGL((k_p<float, KCFD_PC_NONE>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
+            }
+        }
+    };
+    if (phase == 3) {
+        // the L solve needs a fresh sentinel-filled output and a zero counter
+        // before every launch; time only the solves
+        float tot = 0.f;
+        for (int r = 0; r < reps; ++r) {
+            KCFD_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned) * 8, s));
+            for (int k = 0; k < b->nsys; ++k) {
+                const SysDesc &d = b->h_desc[k];
+                const size_t es = f64 ? 8 : 4;
+                if (f64) hipLaunchKernelGGL(k_fill_sentinel<double>, dim3(std::max(1, (d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, (double *)d.y, d.n);
+                else hipLaunchKernelGGL(k_fill_sentinel<float>, dim3(std::max(1, (d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, (float *)d.y, d.n);
+                (void)es;
+            }
+            KCFD_HIP(hipEventRecord(b->ctx->ev0, s));
+            if (f64) hipLaunchKernelGGL((k_trsv<double, false, 0>), dim3(b->trsv_grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters);
+            else hipLaunchKernelGGL((k_trsv<float, false, 0>), dim3(b->trsv_grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters);
+            KCFD_HIP(hipEventRecord(b->ctx->ev1, s));
+            KCFD_HIP(hipEventSynchronize(b->ctx->ev1));
+            float ms = 0.f;
+            KCFD_HIP(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+            tot += ms;
+        }
+        *avg_ms = tot / reps;
+    } else {
+        launch();  // warm
+        KCFD_HIP(hipEventRecord(b->ctx->ev0, s));
+        for (int r = 0; r < reps; ++r) launch();
+        KCFD_HIP(hipEventRecord(b->ctx->ev1, s));
+        KCFD_HIP(hipEventSynchronize(b->ctx->ev1));
+        float ms = 0.f;
+        KCFD_HIP(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
+        *avg_ms = ms / reps;
+    }
+    KCFD_HIP(hipGetLastError());
+    KCFD_HIP(hipMemcpy(b->d_state, saved.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice));
+    // ILU buffers back to the sentinel state a new solve expects
+    return KCFD_OK;
+}
+
+int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
+    KCFD_ARG(b && bytes, "bad argument");
+    const double sv = b->bp.precision == KCFD_FP64 ? 8.0 : 4.0;
+    double tot = 0.0;
+    for (auto *m : b->mats) {
+        const double n = m->n, nnz = m->nnz;
+        const double MA = nnz * (sv + 4) + (n + 1) * 4;
+        const double MFh = 0.5 * nnz * (sv + 4) + (n + 1) * 4;  // one triangle + its level list
+        if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
+        else if (phase == 2) tot += 8 * n * sv;
+        else if (phase == 3) tot += MFh + 2 * n * sv;
+        else if (phase == 4) tot += 4 * n * sv;
+    }
+    *bytes = tot;
+    return KCFD_OK;
+}
+
+int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts *opts,
+               kcfd_solve_result *res) {
+    KCFD_ARG(m && opts, "null argument");
+    KCFD_ARG(m->n == 0 || (bvec && x), "null vector");
+    std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
+    KCFD_HIP(hipSetDevice(m->ctx->device));
+    if (m->n == 0) {
+        if (res) std::memset(res, 0, sizeof(*res));
+        return KCFD_OK;
+    }
+    // the cached single-system workspace is keyed on (precond, precision, b, x)
+    const int key[4] = {opts->precond, opts->precision, (int)((uintptr_t)bvec & 0x7fffffff),
+                        (int)((uintptr_t)x & 0x7fffffff)};
+    bool reuse = m->solver != nullptr && std::memcmp(key, m->solver_key, sizeof(key)) == 0 &&
+                 m->solver->h_desc[0].b == bvec && m->solver->h_desc[0].x == x;
+    if (!reuse) {
+        if (m->solver) kcfd_batch_destroy(m->solver);
+        m->solver = nullptr;
+        kcfd_mat *mm[1] = {m};
+        const double *bb[1] = {bvec};
+        double *xx[1] = {x};
+        KCFD_CHECK(build_batch(m->ctx, 1, mm, bb, xx, opts, &m->solver));
+        std::memcpy(m->solver_key, key, sizeof(key));
+    } else {
+        // refresh option-dependent parameters and the preconditioner pointers
+        kcfd_batch *b = m->solver;
+        b->opts = *opts;
+        if (b->opts.check_every <= 0) b->opts.check_every = 8;
+        b->bp.rtol = opts->rtol;
+        b->bp.atol = opts->atol;
+        b->bp.inner_rtol = opts->inner_rtol > 0 ? opts->inner_rtol : 1e-4;
+        b->bp.max_refine = opts->max_refine > 0 ? opts->max_refine : 30;
+        if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+        SysDesc &d = b->h_desc[0];
+        d.a32 = m->val32; d.dinv64 = m->dinv; d.dinv32 = m->dinv32; d.lu64 = m->lu; d.lu32 = m->lu32;
+        KCFD_HIP(hipMemcpy(b->d_desc, b->h_desc.data(), sizeof(SysDesc), hipMemcpyHostToDevice));
+    }
+    return run_batch(m->solver, res);
+}
+
+}  // extern "C"

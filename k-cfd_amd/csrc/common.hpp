@@ -1,0 +1,134 @@
+// Internal definitions shared by the libkcfd translation units (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/kcfd.h"
+
+namespace kcfd {
+
+void set_error(const char *fmt, ...);
+
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxPartials = 4;      // doubles per block partial record
+
+// Number of XCDs on MI355X; workgroups are dispatched round-robin over them.
+constexpr int kXcds = 8;
+
+// Remap a dispatch-order block id so that each XCD works on one contiguous
+// range of tiles (neighbouring rows share that XCD's L2).  grid must be a
+// multiple of kXcds; ids >= ntiles are idle.
+__device__ __forceinline__ int xcd_tile(int bid, int grid) {
+    int per = grid / kXcds;
+    return (bid % kXcds) * per + bid / kXcds;
+}
+
+inline int xcd_grid(int ntiles) { return ((ntiles + kXcds - 1) / kXcds) * kXcds; }
+
+// wave64 sum by xor-shuffle (fixed tree: deterministic)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum of NV values; result valid in thread 0.  Fixed order.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double red[kWaves][NV > 0 ? NV : 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) red[w][k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = red[0][k];
+            for (int q = 1; q < kWaves; ++q) s += red[q][k];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+}  // namespace kcfd
+
+#define KCFD_HIP(expr)                                                                 \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            kcfd::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),      \
+                            __FILE__, __LINE__);                                        \
+            return e_ == hipErrorOutOfMemory ? KCFD_EOOM : KCFD_EHIP;                  \
+        }                                                                              \
+    } while (0)
+
+#define KCFD_ARG(cond, msg)                                                            \
+    do {                                                                               \
+        if (!(cond)) {                                                                 \
+            kcfd::set_error("invalid argument: %s", msg);                              \
+            return KCFD_EARG;                                                          \
+        }                                                                              \
+    } while (0)
+
+#define KCFD_CHECK(call)              \
+    do {                              \
+        int rc_ = (call);             \
+        if (rc_ != KCFD_OK) return rc_; \
+    } while (0)
+
+struct kcfd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::recursive_mutex mu;
+    int cus = 0;
+    double last_ms = 0.0;
+    // staging scratch for small host<->device copies
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+};
+
+struct kcfd_batch;
+
+struct kcfd_mat {
+    kcfd_ctx *ctx = nullptr;
+    int n = 0, nnz = 0;
+    int *indptr = nullptr, *indices = nullptr, *diagpos = nullptr;
+    double *val = nullptr, *rhs = nullptr, *diagt = nullptr;
+    std::vector<int> h_indptr, h_indices, h_diagpos;
+    // preconditioner
+    int pc_kind = KCFD_PC_NONE;
+    bool have32 = false;
+    float *val32 = nullptr;
+    double *dinv = nullptr;
+    float *dinv32 = nullptr;
+    // ILU(0): factors on A's pattern, level schedule of the L and U solves
+    double *lu = nullptr;
+    float *lu32 = nullptr;
+    int nlev_l = 0, nlev_u = 0;
+    int *lvl_rows_l = nullptr, *lvl_rows_u = nullptr;   // rows ordered by level
+    std::vector<int> h_lvl_ptr_l, h_lvl_ptr_u;           // level offsets
+    int max_lvl_rows_l = 0, max_lvl_rows_u = 0;
+    // cached single-system solver workspace
+    kcfd_batch *solver = nullptr;
+    int solver_key[4] = {-1, -1, -1, -1};
+};
+
+namespace kcfd {
+int ensure_scratch(kcfd_ctx *ctx, size_t bytes);
+int ilu0_setup(kcfd_mat *m, bool want32);
+int mat_free_precond(kcfd_mat *m);
+}  // namespace kcfd

@@ -1,0 +1,369 @@
+// Context, error reporting, device buffers, CSR matrices and the Jacobi
+// preconditioner set-up.
+#include "common.hpp"
+
+#include <cstdlib>
+#include <cstring>
+
+namespace kcfd {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int ensure_scratch(kcfd_ctx *ctx, size_t bytes) {
+    if (ctx->scratch_bytes >= bytes) return KCFD_OK;
+    if (ctx->scratch) KCFD_HIP(hipFree(ctx->scratch));
+    ctx->scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    KCFD_HIP(hipMalloc(&ctx->scratch, bytes));
+    ctx->scratch_bytes = bytes;
+    return KCFD_OK;
+}
+
+// ---------------------------------------------------------------- kernels
+
+__global__ void k_fill(double *p, double v, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) p[i] = v;
+}
+
+// Row-parallel CSR SpMV in CSR order, products and sums unfused (the
+// sequential row sum of scipy's csr_matvec).
+__global__ void k_spmv(int n, const int *__restrict__ ip, const int *__restrict__ ix,
+                       const double *__restrict__ a, const double *__restrict__ x,
+                       double *__restrict__ y) {
+#pragma clang fp contract(off)
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double s = 0.0;
+    for (int q = ip[i]; q < ip[i + 1]; ++q) s += a[q] * x[ix[q]];
+    y[i] = s;
+}
+
+__global__ void k_diagpos(int n, const int *__restrict__ ip, const int *__restrict__ ix,
+                          int *__restrict__ dp) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int p = -1;
+    for (int q = ip[i]; q < ip[i + 1]; ++q)
+        if (ix[q] == i) { p = q; break; }
+    dp[i] = p;
+}
+
+// Jacobi: dinv = where(|a_ii| > 1e-30, 1/a_ii, 0) (core/preconditioner.py:43-47);
+// a missing diagonal counts as 0.
+__global__ void k_jacobi(int n, const int *__restrict__ dp, const double *__restrict__ a,
+                         double *__restrict__ dinv, float *__restrict__ dinv32) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = dp[i] >= 0 ? a[dp[i]] : 0.0;
+    double r = fabs(d) > 1e-30 ? 1.0 / d : 0.0;
+    dinv[i] = r;
+    if (dinv32) dinv32[i] = (float)r;
+}
+
+__global__ void k_to_f32(size_t n, const double *__restrict__ a, float *__restrict__ b) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t st = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += st) b[i] = (float)a[i];
+}
+
+int mat_free_precond(kcfd_mat *m) {
+    hipFree(m->val32); m->val32 = nullptr;
+    hipFree(m->dinv); m->dinv = nullptr;
+    hipFree(m->dinv32); m->dinv32 = nullptr;
+    hipFree(m->lu); m->lu = nullptr;
+    hipFree(m->lu32); m->lu32 = nullptr;
+    hipFree(m->lvl_rows_l); m->lvl_rows_l = nullptr;
+    hipFree(m->lvl_rows_u); m->lvl_rows_u = nullptr;
+    m->h_lvl_ptr_l.clear();
+    m->h_lvl_ptr_u.clear();
+    m->nlev_l = m->nlev_u = 0;
+    m->have32 = false;
+    m->pc_kind = KCFD_PC_NONE;
+    return KCFD_OK;
+}
+
+}  // namespace kcfd
+
+using namespace kcfd;
+
+extern "C" int kcfd_batch_destroy(kcfd_batch *b);
+
+static inline unsigned nblk(size_t n) {
+    size_t b = (n + kBlock - 1) / kBlock;
+    if (b == 0) b = 1;
+    if (b > 65535u * 8u) b = 65535u * 8u;
+    return (unsigned)b;
+}
+
+extern "C" {
+
+const char *kcfd_last_error(void) { return g_err; }
+
+int kcfd_init(int device, kcfd_ctx **out) {
+    KCFD_ARG(out != nullptr, "out is null");
+    int count = 0;
+    KCFD_HIP(hipGetDeviceCount(&count));
+    KCFD_ARG(device >= 0 && device < count, "device index out of range");
+    KCFD_HIP(hipSetDevice(device));
+    kcfd_ctx *c = new kcfd_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    KCFD_HIP(hipGetDeviceProperties(&prop, device));
+    c->cus = prop.multiProcessorCount;
+    KCFD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    KCFD_HIP(hipEventCreate(&c->ev0));
+    KCFD_HIP(hipEventCreate(&c->ev1));
+    *out = c;
+    return KCFD_OK;
+}
+
+int kcfd_destroy(kcfd_ctx *ctx) {
+    if (!ctx) return KCFD_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    if (ctx->scratch) hipFree(ctx->scratch);
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return KCFD_OK;
+}
+
+int kcfd_device_info(kcfd_ctx *ctx, char *name, int len, int *cus, long long *total_bytes,
+                     long long *free_bytes) {
+    KCFD_ARG(ctx != nullptr, "ctx is null");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    KCFD_HIP(hipSetDevice(ctx->device));
+    hipDeviceProp_t prop;
+    KCFD_HIP(hipGetDeviceProperties(&prop, ctx->device));
+    if (name && len > 0) {
+        snprintf(name, (size_t)len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (cus) *cus = prop.multiProcessorCount;
+    size_t fr = 0, tot = 0;
+    KCFD_HIP(hipMemGetInfo(&fr, &tot));
+    if (total_bytes) *total_bytes = (long long)tot;
+    if (free_bytes) *free_bytes = (long long)fr;
+    return KCFD_OK;
+}
+
+int kcfd_sync(kcfd_ctx *ctx) {
+    KCFD_ARG(ctx != nullptr, "ctx is null");
+    KCFD_HIP(hipStreamSynchronize(ctx->stream));
+    return KCFD_OK;
+}
+
+int kcfd_buf_alloc(kcfd_ctx *ctx, size_t bytes, void **dev) {
+    KCFD_ARG(ctx && dev, "null argument");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    KCFD_HIP(hipSetDevice(ctx->device));
+    *dev = nullptr;
+    KCFD_HIP(hipMalloc(dev, bytes ? bytes : 8));
+    return KCFD_OK;
+}
+
+int kcfd_buf_free(kcfd_ctx *ctx, void *dev) {
+    KCFD_ARG(ctx != nullptr, "ctx is null");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!dev) return KCFD_OK;
+    KCFD_HIP(hipStreamSynchronize(ctx->stream));
+    KCFD_HIP(hipFree(dev));
+    return KCFD_OK;
+}
+
+int kcfd_buf_upload(kcfd_ctx *ctx, void *dev, const void *host, size_t bytes) {
+    KCFD_ARG(ctx && (bytes == 0 || (dev && host)), "null argument");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!bytes) return KCFD_OK;
+    KCFD_HIP(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, ctx->stream));
+    KCFD_HIP(hipStreamSynchronize(ctx->stream));
+    return KCFD_OK;
+}
+
+int kcfd_buf_download(kcfd_ctx *ctx, void *host, const void *dev, size_t bytes) {
+    KCFD_ARG(ctx && (bytes == 0 || (dev && host)), "null argument");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!bytes) return KCFD_OK;
+    KCFD_HIP(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    KCFD_HIP(hipStreamSynchronize(ctx->stream));
+    return KCFD_OK;
+}
+
+int kcfd_buf_fill(kcfd_ctx *ctx, double *dev, double value, size_t n) {
+    KCFD_ARG(ctx && (n == 0 || dev), "null argument");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (!n) return KCFD_OK;
+    hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dev, value, n);
+    KCFD_HIP(hipGetLastError());
+    return KCFD_OK;
+}
+
+int kcfd_mat_create(kcfd_ctx *ctx, int n, int nnz, const int *indptr, const int *indices,
+                    kcfd_mat **out) {
+    KCFD_ARG(ctx && out, "null argument");
+    KCFD_ARG(n >= 0 && nnz >= 0, "negative size");
+    KCFD_ARG(n == 0 || (indptr && (nnz == 0 || indices)), "null structure");
+    if (n > 0) {
+        KCFD_ARG(indptr[0] == 0 && indptr[n] == nnz, "indptr inconsistent with nnz");
+        for (int i = 0; i < n; ++i) {
+            KCFD_ARG(indptr[i + 1] >= indptr[i], "indptr not monotone");
+            for (int q = indptr[i]; q < indptr[i + 1]; ++q) {
+                KCFD_ARG(indices[q] >= 0 && indices[q] < n, "column index out of range");
+                KCFD_ARG(q == indptr[i] || indices[q] > indices[q - 1],
+                         "columns not sorted/unique (canonical CSR required)");
+            }
+        }
+    }
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    KCFD_HIP(hipSetDevice(ctx->device));
+    kcfd_mat *m = new kcfd_mat();
+    m->ctx = ctx;
+    m->n = n;
+    m->nnz = nnz;
+    m->h_indptr.assign(indptr, indptr + n + 1);
+    m->h_indices.assign(indices, indices + nnz);
+    m->h_diagpos.assign(n, -1);
+    for (int i = 0; i < n; ++i)
+        for (int q = indptr[i]; q < indptr[i + 1]; ++q)
+            if (indices[q] == i) m->h_diagpos[i] = q;
+    auto fail = [&](int rc) { kcfd_mat_destroy(m); return rc; };
+    if (hipMalloc(&m->indptr, sizeof(int) * (n + 1)) != hipSuccess ||
+        hipMalloc(&m->indices, sizeof(int) * (nnz ? nnz : 1)) != hipSuccess ||
+        hipMalloc(&m->diagpos, sizeof(int) * (n ? n : 1)) != hipSuccess ||
+        hipMalloc(&m->val, sizeof(double) * (nnz ? nnz : 1)) != hipSuccess ||
+        hipMalloc(&m->rhs, sizeof(double) * (n ? n : 1)) != hipSuccess ||
+        hipMalloc(&m->diagt, sizeof(double) * (n ? n : 1)) != hipSuccess) {
+        set_error("hipMalloc failed for a matrix of n=%d nnz=%d", n, nnz);
+        return fail(KCFD_EOOM);
+    }
+    if (hipMemcpy(m->indptr, indptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
+        (nnz && hipMemcpy(m->indices, indices, sizeof(int) * nnz, hipMemcpyHostToDevice) != hipSuccess) ||
+        (n && hipMemcpy(m->diagpos, m->h_diagpos.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemset(m->val, 0, sizeof(double) * (nnz ? nnz : 1)) != hipSuccess ||
+        hipMemset(m->rhs, 0, sizeof(double) * (n ? n : 1)) != hipSuccess ||
+        hipMemset(m->diagt, 0, sizeof(double) * (n ? n : 1)) != hipSuccess) {
+        set_error("matrix upload failed");
+        return fail(KCFD_EHIP);
+    }
+    *out = m;
+    return KCFD_OK;
+}
+
+int kcfd_mat_destroy(kcfd_mat *m) {
+    if (!m) return KCFD_OK;
+    std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
+    hipSetDevice(m->ctx->device);
+    hipStreamSynchronize(m->ctx->stream);
+    if (m->solver) kcfd_batch_destroy(m->solver);
+    mat_free_precond(m);
+    hipFree(m->indptr);
+    hipFree(m->indices);
+    hipFree(m->diagpos);
+    hipFree(m->val);
+    hipFree(m->rhs);
+    hipFree(m->diagt);
+    delete m;
+    return KCFD_OK;
+}
+
+int kcfd_mat_set_values(kcfd_mat *m, const double *vals) {
+    KCFD_ARG(m && (m->nnz == 0 || vals), "null argument");
+    std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
+    if (!m->nnz) return KCFD_OK;
+    KCFD_HIP(hipMemcpyAsync(m->val, vals, sizeof(double) * m->nnz, hipMemcpyHostToDevice,
+                            m->ctx->stream));
+    KCFD_HIP(hipStreamSynchronize(m->ctx->stream));
+    return KCFD_OK;
+}
+
+int kcfd_mat_get_values(kcfd_mat *m, double *vals) {
+    KCFD_ARG(m && (m->nnz == 0 || vals), "null argument");
+    std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
+    if (!m->nnz) return KCFD_OK;
+    KCFD_HIP(hipMemcpyAsync(vals, m->val, sizeof(double) * m->nnz, hipMemcpyDeviceToHost,
+                            m->ctx->stream));
+    KCFD_HIP(hipStreamSynchronize(m->ctx->stream));
+    return KCFD_OK;
+}
+
+int kcfd_mat_dev_ptrs(kcfd_mat *m, double **vals, double **rhs, double **diag) {
+    KCFD_ARG(m != nullptr, "null matrix");
+    if (vals) *vals = m->val;
+    if (rhs) *rhs = m->rhs;
+    if (diag) *diag = m->diagt;
+    return KCFD_OK;
+}
+
+int kcfd_mat_spmv(kcfd_mat *m, const double *x, double *y) {
+    KCFD_ARG(m && (m->n == 0 || (x && y)), "null argument");
+    std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
+    if (!m->n) return KCFD_OK;
+    hipLaunchKernelGGL(k_spmv, dim3((m->n + kBlock - 1) / kBlock), dim3(kBlock), 0,
+                       m->ctx->stream, m->n, m->indptr, m->indices, m->val, x, y);
+    KCFD_HIP(hipGetLastError());
+    KCFD_HIP(hipStreamSynchronize(m->ctx->stream));
+    return KCFD_OK;
+}
+
+int kcfd_mat_bytes(kcfd_mat *m, long long *bytes) {
+    KCFD_ARG(m && bytes, "null argument");
+    long long b = 4LL * (m->n + 1) + 4LL * m->nnz + 4LL * m->n + 8LL * m->nnz + 16LL * m->n;
+    if (m->val32) b += 4LL * m->nnz;
+    if (m->dinv) b += 8LL * m->n;
+    if (m->dinv32) b += 4LL * m->n;
+    if (m->lu) b += 8LL * m->nnz + 8LL * m->n;
+    if (m->lu32) b += 4LL * m->nnz;
+    *bytes = b;
+    return KCFD_OK;
+}
+
+int kcfd_precond_setup(kcfd_mat *m, int kind, int precision) {
+    KCFD_ARG(m != nullptr, "null matrix");
+    KCFD_ARG(kind == KCFD_PC_NONE || kind == KCFD_PC_JACOBI || kind == KCFD_PC_ILU0,
+             "unknown preconditioner kind");
+    KCFD_ARG(precision == KCFD_FP64 || precision == KCFD_MIXED, "unknown precision");
+    std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
+    KCFD_HIP(hipSetDevice(m->ctx->device));
+    hipStream_t s = m->ctx->stream;
+    KCFD_HIP(hipStreamSynchronize(s));
+    if (m->solver) {
+        kcfd_batch_destroy(m->solver);
+        m->solver = nullptr;
+    }
+    mat_free_precond(m);
+    const int n = m->n;
+    const bool want32 = precision == KCFD_MIXED;
+    if (want32 && m->nnz) {
+        KCFD_HIP(hipMalloc(&m->val32, sizeof(float) * m->nnz));
+        hipLaunchKernelGGL(k_to_f32, dim3(nblk(m->nnz)), dim3(kBlock), 0, s, (size_t)m->nnz,
+                           m->val, m->val32);
+        KCFD_HIP(hipGetLastError());
+        m->have32 = true;
+    }
+    if (kind == KCFD_PC_JACOBI) {
+        KCFD_HIP(hipMalloc(&m->dinv, sizeof(double) * (n ? n : 1)));
+        if (want32) KCFD_HIP(hipMalloc(&m->dinv32, sizeof(float) * (n ? n : 1)));
+        if (n) {
+            hipLaunchKernelGGL(k_jacobi, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
+                               m->diagpos, m->val, m->dinv, m->dinv32);
+            KCFD_HIP(hipGetLastError());
+        }
+    } else if (kind == KCFD_PC_ILU0) {
+        KCFD_CHECK(ilu0_setup(m, want32));
+    }
+    m->pc_kind = kind;
+    m->have32 = want32;
+    KCFD_HIP(hipStreamSynchronize(s));
+    return KCFD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,350 @@
+"""Seeded synthetic workloads (mesh + fields + an operator recipe).
+
+Each generator returns a ``Case``: the mesh and an ordered recipe of the
+reference operator calls with their inputs.  The recipe is played on the
+device by ``assemble_device`` (the product path); tests replay the same
+recipe through the CPU oracle, so both sides assemble the identical system.
+
+Configurations (sizes are parameters; defaults are the full sizes):
+  quad_convdiff  2D unit square quads, lognormal gamma (mu=ln 1e-2,
+                 sigma=0.5), upwind convection by the curl of an 8-mode
+                 stream function (exactly divergence-free face fluxes,
+                 peak |u| at cell centres = 1), Dirichlet 0, b ~ U[-1,1].
+  hex_pressure   3D unit cube hexes, SIMPLE pressure correction
+                 (models/single_phase.py:288-325 form) with a_P ~
+                 lognormal(0,1), dP = V/a_P, Dirichlet p'=0 on "outlet",
+                 b ~ U[-1,1].
+  hex_momentum   3D unit cube hexes, gamma = 1e-2, upwind convection by a
+                 random divergence-free flux (circulation of an 8-mode
+                 vector potential around each face), Dirichlet 0, rhs =
+                 U[-1,1]*V plus the van Leer MUSCL deferred correction from
+                 Green-Gauss gradients of a smooth field.
+  tet_twofluid   Delaunay tetrahedra of jittered lattice points, cells
+                 permuted then RCM-reordered; liquid-phase momentum of
+                 models/two_fluid.py:520-612 (alpha_g ~ U[0.05,0.6],
+                 diffusion alpha_l*mu_l, upwind alpha_l*F, temporal,
+                 Schiller-Naumann implicit drag on the diagonal, Dirichlet
+                 0 walls).
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from .mesh import (build_fvmesh_from_arrays, structured_hex_arrays, structured_quad_arrays,
+                   tet_arrays)
+
+# two-fluid constants (models/two_fluid.py:36-47)
+RHO_L, RHO_G, MU_L, MU_G, D_B = 998.2, 1.225, 1.003e-3, 1.789e-5, 0.005
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    mesh: object
+    recipe: list          # [(op, kwargs)]
+    precond: str
+    rhs: np.ndarray | None = None   # final rhs override (set after assembly)
+    meta: dict = dataclasses.field(default_factory=dict)
+
+
+# --------------------------------------------------------------------- helpers
+
+def _face_sign_2d(mesh):
+    """+1 where the stored normal equals (dy,-dx)/L of the face's node order."""
+    fn = mesh.face_nodes
+    p0 = mesh.nodes[fn[:, 0]]
+    p1 = mesh.nodes[fn[:, 1]]
+    raw = np.stack([p1[:, 1] - p0[:, 1], -(p1[:, 0] - p0[:, 0])], 1)
+    return np.where(np.sum(raw * mesh.face_normal[:, :2], axis=1) >= 0, 1.0, -1.0)
+
+
+def _stream_function(rng, nmodes=8):
+    a = rng.normal(0.0, 1.0, nmodes)
+    m = rng.integers(1, 5, nmodes)
+    k = rng.integers(1, 5, nmodes)
+
+    def psi(x, y):
+        return np.sum(a[None, :] * np.sin(np.pi * m[None, :] * x[:, None]) *
+                      np.sin(np.pi * k[None, :] * y[:, None]), axis=1)
+
+    def vel(x, y):
+        # u = (d psi/dy, -d psi/dx)
+        sx = np.sin(np.pi * m[None, :] * x[:, None])
+        cx = np.cos(np.pi * m[None, :] * x[:, None])
+        sy = np.sin(np.pi * k[None, :] * y[:, None])
+        cy = np.cos(np.pi * k[None, :] * y[:, None])
+        u = np.sum(a * sx * np.pi * k * cy, axis=1)
+        v = -np.sum(a * np.pi * m * cx * sy, axis=1)
+        return u, v
+
+    return psi, vel
+
+
+def _vector_potential(rng, nmodes=8):
+    c = rng.normal(0.0, 1.0, (nmodes, 3))
+    w = rng.integers(1, 4, (nmodes, 3))
+    ph = rng.uniform(0, 2 * np.pi, nmodes)
+
+    def A(p):
+        arg = np.pi * (p @ w.T) + ph[None, :]          # (np, nmodes)
+        return np.sin(arg) @ c                          # (np, 3)
+
+    return A
+
+
+def _circulation_flux(mesh, A):
+    """Face flux = circulation of A around the face (trapezoid per edge), oriented
+    to the stored normal.  Each edge integral is antisymmetric, so the flux
+    out of every cell sums to zero up to rounding."""
+    fn = mesh.face_nodes
+    P = mesh.nodes
+    k = fn.shape[1]
+    Av = A(P)
+    circ = np.zeros(len(fn))
+    for j in range(k):
+        a = fn[:, j]
+        b = fn[:, (j + 1) % k]
+        circ += np.sum(0.5 * (Av[a] + Av[b]) * (P[b] - P[a]), axis=1)
+    # Newell normal of the node order (unnormalised) vs the stored normal
+    pts = P[fn]
+    nxt = np.roll(pts, -1, axis=1)
+    nn = np.stack([np.sum((pts[:, :, 1] - nxt[:, :, 1]) * (pts[:, :, 2] + nxt[:, :, 2]), 1),
+                   np.sum((pts[:, :, 2] - nxt[:, :, 2]) * (pts[:, :, 0] + nxt[:, :, 0]), 1),
+                   np.sum((pts[:, :, 0] - nxt[:, :, 0]) * (pts[:, :, 1] + nxt[:, :, 1]), 1)], 1)
+    sgn = np.where(np.sum(nn * mesh.face_normal, axis=1) >= 0, 1.0, -1.0)
+    return sgn * circ
+
+
+def dirichlet_all(mesh, value=0.0):
+    bc = {nm: {"type": "dirichlet"} for nm in mesh.boundary_patches}
+    phib = {nm: np.full(len(f), value) for nm, f in mesh.boundary_patches.items()}
+    return bc, phib
+
+
+# ------------------------------------------------------------------ configs
+
+def quad_mesh(nx, ny=None):
+    ny = nx if ny is None else ny
+    nodes, cells, bf = structured_quad_arrays(0.0, 0.0, 1.0, 1.0, nx, ny)
+    return build_fvmesh_from_arrays(nodes, cells, bf, ndim=2)
+
+
+def hex_mesh(nx, ny=None, nz=None):
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    nodes, cells, faces, bf = structured_hex_arrays(1.0, 1.0, 1.0, nx, ny, nz)
+    return build_fvmesh_from_arrays(nodes, cells, bf, ndim=3, cell_face_node_list=faces)
+
+
+def quad_convdiff(n=128, seed=1001) -> Case:
+    rng = np.random.default_rng(seed)
+    mesh = quad_mesh(n)
+    nc = mesh.n_cells
+    gamma = np.exp(rng.normal(math.log(1e-2), 0.5, nc))
+    psi, vel = _stream_function(rng)
+    fn = mesh.face_nodes
+    P = mesh.nodes
+    F = _face_sign_2d(mesh) * (psi(P[fn[:, 1], 0], P[fn[:, 1], 1]) - psi(P[fn[:, 0], 0], P[fn[:, 0], 1]))
+    u, v = vel(mesh.cell_center[:, 0], mesh.cell_center[:, 1])
+    F = F / np.max(np.sqrt(u * u + v * v))
+    b = rng.uniform(-1.0, 1.0, nc)
+    bc, phib = dirichlet_all(mesh)
+    recipe = [("diffusion", dict(gamma=gamma)), ("upwind", dict(F=F)),
+              ("boundary", dict(gamma=gamma, F=F, phib=phib, bc=bc))]
+    return Case("quad_convdiff", mesh, recipe, "jacobi", rhs=b,
+                meta=dict(n=n, seed=seed))
+
+
+def hex_pressure(n=128, seed=1002, mesh=None) -> Case:
+    rng = np.random.default_rng(seed)
+    mesh = hex_mesh(n) if mesh is None else mesh
+    nc = mesh.n_cells
+    aP = np.exp(rng.normal(0.0, 1.0, nc))
+    dP = mesh.cell_volume / aP
+    b = rng.uniform(-1.0, 1.0, nc)
+    pdir = [nm for nm in mesh.boundary_patches if nm == "outlet"]
+    recipe = [("pressure", dict(dP=dP, rho=1.0, dirichlet=pdir))]
+    return Case("hex_pressure", mesh, recipe, "ilu0", rhs=b, meta=dict(n=n, seed=seed))
+
+
+def hex_momentum(n=128, seed=1002, mesh=None) -> Case:
+    rng = np.random.default_rng(seed + 7)
+    mesh = hex_mesh(n) if mesh is None else mesh
+    nc = mesh.n_cells
+    gamma = np.full(nc, 1e-2)
+    F = _circulation_flux(mesh, _vector_potential(rng))
+    F = F / np.max(np.abs(F) / mesh.face_area)
+    x = mesh.cell_center
+    phi = np.sin(2 * np.pi * x[:, 0]) * np.cos(np.pi * x[:, 1]) + 0.5 * np.sin(3 * np.pi * x[:, 2])
+    bc, phib = dirichlet_all(mesh)
+    src = rng.uniform(-1.0, 1.0, nc)
+    recipe = [("diffusion", dict(gamma=gamma)), ("upwind", dict(F=F)),
+              ("boundary", dict(gamma=gamma, F=F, phib=phib, bc=bc)),
+              ("source", dict(S=src)),
+              ("muscl", dict(phi=phi, phib=phib, F=F, limiter="van_leer"))]
+    return Case("hex_momentum", mesh, recipe, "ilu0", meta=dict(n=n, seed=seed))
+
+
+def tet_mesh(npts_target=200_000, seed=1003, jitter=0.3):
+    """Delaunay tetrahedra of a jittered lattice; cells randomly permuted,
+    then RCM-reordered."""
+    from scipy.spatial import Delaunay
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import reverse_cuthill_mckee
+
+    rng = np.random.default_rng(seed)
+    m = max(2, int(round(npts_target ** (1.0 / 3.0))))
+    nx, ny, nz = m, m, max(2, int(round(npts_target / (m * m))))
+    h = np.array([1.0 / (nx - 1), 1.0 / (ny - 1), 1.0 / (nz - 1)])
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    idx = np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+    pts = idx * h
+    jit = rng.uniform(-jitter, jitter, pts.shape) * h
+    onb = (idx == 0) | (idx == np.array([nx - 1, ny - 1, nz - 1]))
+    jit[onb] = 0.0            # keep boundary points on the boundary planes
+    pts = pts + jit
+    tri = Delaunay(pts)
+    tets = tri.simplices.astype(np.int64)
+    # drop slivers of (near) zero volume
+    p = pts[tets]
+    vol6 = np.abs(np.einsum("ij,ij->i", np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), p[:, 3] - p[:, 0]))
+    tets = tets[vol6 > 1e-12 * np.prod(h)]
+    perm = rng.permutation(len(tets))
+    tets = tets[perm]
+    mesh0 = build_fvmesh_from_arrays(pts, tets, {}, ndim=3, cell_face_node_list=tet_arrays(tets))
+    o = mesh0.owner.astype(np.int64)
+    nb = mesh0.neighbour.astype(np.int64)
+    it = nb >= 0
+    n = mesh0.n_cells
+    G = csr_matrix((np.ones(2 * it.sum()), (np.r_[o[it], nb[it]], np.r_[nb[it], o[it]])), shape=(n, n))
+    order = reverse_cuthill_mckee(G, symmetric_mode=True)
+    tets = tets[order]
+    return build_fvmesh_from_arrays(pts, tets, {}, ndim=3, cell_face_node_list=tet_arrays(tets))
+
+
+def tet_twofluid(npts=200_000, seed=1003, mesh=None) -> Case:
+    mesh = tet_mesh(npts, seed) if mesh is None else mesh
+    rng = np.random.default_rng(seed + 11)
+    nc = mesh.n_cells
+    nd = 3
+    alpha_g = rng.uniform(0.05, 0.6, nc)
+    alpha_l = 1.0 - alpha_g
+    U_l = rng.normal(0.0, 0.2, (nc, nd))
+    U_g = U_l + rng.normal(0.0, 0.2, (nc, nd))
+    Ub = {nm: np.zeros((len(f), nd)) for nm, f in mesh.boundary_patches.items()}
+    # Schiller-Naumann implicit drag (models/closure.py:72-92)
+    ur = U_g - U_l
+    mag = np.maximum(np.sqrt(np.sum(ur ** 2, axis=1)), 1e-15)
+    Re = np.maximum(RHO_L * mag * D_B / MU_L, 1e-10)
+    cd = np.where(Re < 1000, 24.0 / Re * (1.0 + 0.15 * Re ** 0.687), 0.44)
+    K = 0.75 * cd * alpha_g * RHO_L * mag / D_B
+    dt = 1e-3
+    gamma = alpha_l * MU_L
+    phi_old = U_l[:, 0].copy()
+    bc, phib = dirichlet_all(mesh)
+    recipe = [("mass_flux", dict(U=U_l, Ub=Ub, rho=RHO_L)),
+              ("diffusion", dict(gamma=gamma)),
+              ("upwind_scaled", dict(scale_owner=alpha_l)),
+              ("cell_terms", dict(kd=alpha_l * RHO_L / dt, ks=phi_old)),
+              ("cell_terms", dict(kd=K, ks=U_g[:, 0])),
+              ("boundary_scaled", dict(gamma=gamma, scale_owner=alpha_l, phib=phib, bc=bc))]
+    return Case("tet_twofluid", mesh, recipe, "jacobi",
+                meta=dict(npts=npts, seed=seed, alpha_g=alpha_g))
+
+
+# --------------------------------------------------------------- batch mix
+
+def batch_specs(nsys=512, seed=4000, nmin=2_000, nmax=250_000):
+    """Sizes log-uniform in [nmin, nmax]; types 50% quad, 30% hex, 20% tet."""
+    out = []
+    for i in range(nsys):
+        rng = np.random.default_rng(seed + i)
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        u = rng.uniform()
+        kind = "quad" if u < 0.5 else ("hex" if u < 0.8 else "tet")
+        out.append((kind, n, seed + i))
+    return out
+
+
+def case_of_size(kind, ncells, seed):
+    if kind == "quad":
+        s = max(4, int(round(math.sqrt(ncells))))
+        return quad_convdiff(s, seed)
+    if kind == "hex":
+        s = max(3, int(round(ncells ** (1.0 / 3.0))))
+        return hex_pressure(s, seed)
+    npts = max(64, int(round(ncells / 5.3)))
+    return tet_twofluid(npts, seed)
+
+
+# ---------------------------------------------------------- device assembly
+
+def assemble_device(dmesh, case: Case, ctx=None):
+    """Play the recipe on the device; returns the DeviceMatrix (rhs set)."""
+    from . import _lib as L
+    from ._lib import DevBuf, check, lib
+
+    ctx = dmesh.ctx
+    mesh = case.mesh
+    A = dmesh.new_system()
+    keep = []
+
+    def dev(a):
+        b = DevBuf.from_array(ctx, np.ascontiguousarray(a, np.float64))
+        keep.append(b)
+        return b.p
+
+    F_dev = None
+    F_host = None
+    for op, kw in case.recipe:
+        if op == "mass_flux":
+            F_dev_buf = DevBuf(ctx, mesh.n_faces)
+            keep.append(F_dev_buf)
+            check(lib().kcfd_mass_flux(dmesh.h, dev(kw["U"]), dev(dmesh.boundary_array(kw["Ub"], width=mesh.ndim)),
+                                       float(kw["rho"]), F_dev_buf.p))
+            F_dev = F_dev_buf
+        elif op == "diffusion":
+            check(lib().kcfd_asm_diffusion(dmesh.h, A.h, dev(kw["gamma"])))
+        elif op == "upwind":
+            check(lib().kcfd_asm_upwind(dmesh.h, A.h, dev(kw["F"])))
+        elif op == "upwind_scaled":
+            F_host = F_dev.download() * np.asarray(kw["scale_owner"])[mesh.owner]
+            check(lib().kcfd_asm_upwind(dmesh.h, A.h, dev(F_host)))
+        elif op in ("boundary", "boundary_scaled"):
+            F = kw["F"] if op == "boundary" else F_host
+            kinds = np.array([{"dirichlet": L.BC_DIRICHLET, "neumann": L.BC_NEUMANN,
+                               "zero_gradient": L.BC_ZERO_GRADIENT}[kw["bc"].get(nm, {"type": "zero_gradient"})["type"]]
+                              for nm in dmesh.patch_names], np.int32)
+            vals = np.array([float(kw["bc"].get(nm, {}).get("value", 0.0)) for nm in dmesh.patch_names])
+            phib = dmesh.boundary_array(kw["phib"])
+            check(lib().kcfd_asm_boundary(dmesh.h, A.h, dev(kw["gamma"]), dev(F), dev(phib),
+                                          kinds.ctypes.data, vals.ctypes.data))
+        elif op == "pressure":
+            pd = np.array([1 if nm in kw["dirichlet"] else 0 for nm in dmesh.patch_names], np.int32)
+            check(lib().kcfd_asm_pressure(dmesh.h, A.h, dev(kw["dP"]), float(kw["rho"]), pd.ctypes.data))
+        elif op == "source":
+            check(lib().kcfd_asm_source(dmesh.h, A.h, dev(kw["S"])))
+        elif op == "cell_terms":
+            check(lib().kcfd_asm_cell_terms(dmesh.h, A.h, dev(kw["kd"]), dev(kw["ks"])))
+        elif op == "temporal":
+            check(lib().kcfd_asm_temporal(dmesh.h, A.h, float(kw["rho"]), float(kw["dt"]), dev(kw["phi_old"])))
+        elif op == "muscl":
+            phib = dmesh.boundary_array(kw["phib"])
+            g = DevBuf(ctx, mesh.n_cells * mesh.ndim)
+            c = DevBuf(ctx, mesh.n_cells)
+            keep += [g, c]
+            phi = dev(kw["phi"])
+            check(lib().kcfd_grad_gg(dmesh.h, phi, dev(phib), g.p))
+            check(lib().kcfd_muscl(dmesh.h, phi, dev(kw["F"]), g.p, L.LIMITERS[kw["limiter"]], c.p))
+            rhs = A.get_rhs() + c.download()
+            A.set_rhs(rhs)
+        else:
+            raise ValueError(f"unknown recipe op {op}")
+    ctx.sync()
+    if case.rhs is not None:
+        A.set_rhs(case.rhs)
+    return A

@@ -1,0 +1,319 @@
+"""Vectorised numpy restatement of the reference assembly, gradient, MUSCL and
+mass-flux functions.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+The reference appends COO triplets face by face and converts with
+``coo_matrix(...).tocsr()`` (core/fvm_operators.py:43-48).  ``OSystem`` below
+rebuilds the *same triplet sequence* (same entries, same order, the same
+skipped faces) with array operations and converts with the same scipy call, so
+its CSR equals the reference's.  The ``diag`` tracker and ``rhs`` are updated
+with ``np.add.at`` in the reference's order, which reproduces its sequential
+floating-point sums exactly.
+
+Meshes are duck-typed: any object with the SoA attributes ``owner``,
+``neighbour``, ``face_area``, ``face_normal`` (nf,3), ``face_center`` (nf,3),
+``cell_center`` (n,3), ``cell_volume``, ``boundary_patches``, ``ndim``,
+``n_cells``, ``n_faces`` works (``soa_from_literal`` converts the literal mesh).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy import sparse
+
+
+class SoAMesh:
+    pass
+
+
+def soa_from_literal(lm) -> SoAMesh:
+    """Convert oracle.literal.LMesh to the SoA attribute set."""
+    m = SoAMesh()
+    m.ndim = lm.ndim
+    m.n_cells = lm.n_cells
+    m.n_faces = lm.n_faces
+    nd = lm.ndim
+    m.owner = np.array([f.owner for f in lm.faces], np.int64)
+    m.neighbour = np.array([f.neighbour for f in lm.faces], np.int64)
+    m.face_area = np.array([f.area for f in lm.faces], float)
+    m.face_normal = np.zeros((lm.n_faces, 3))
+    m.face_center = np.zeros((lm.n_faces, 3))
+    for i, f in enumerate(lm.faces):
+        m.face_normal[i, :nd] = f.normal[:nd]
+        m.face_center[i, :nd] = f.center[:nd]
+    m.cell_center = np.zeros((lm.n_cells, 3))
+    for i, c in enumerate(lm.cells):
+        m.cell_center[i, :nd] = c.center[:nd]
+    m.cell_volume = np.array([c.volume for c in lm.cells], float)
+    m.boundary_patches = {k: np.asarray(v, np.int64) for k, v in lm.boundary_patches.items()}
+    return m
+
+
+def _norm(v):
+    return np.sqrt(np.sum(v * v, axis=1))
+
+
+class OSystem:
+    """COO accumulator with the reference's semantics (fvm_operators.py:13-56)."""
+
+    def __init__(self, n):
+        self.n = n
+        self._r, self._c, self._v = [], [], []
+        self.rhs = np.zeros(n)
+        self.diag = np.zeros(n)
+
+    def add_entries(self, rows, cols, vals):
+        rows = np.asarray(rows, np.int64)
+        cols = np.asarray(cols, np.int64)
+        vals = np.asarray(vals, np.float64)
+        self._r.append(rows); self._c.append(cols); self._v.append(vals)
+        dm = rows == cols
+        # add_diagonal updates the tracker; add_off_diagonal does not.  All
+        # diagonal entries appended here come from add_diagonal calls.
+        np.add.at(self.diag, rows[dm], vals[dm])
+
+    def add_source(self, idx, vals):
+        np.add.at(self.rhs, np.asarray(idx, np.int64), np.asarray(vals, np.float64))
+
+    def coo(self):
+        if not self._r:
+            e = np.empty(0, np.int64)
+            return e, e, np.empty(0)
+        return np.concatenate(self._r), np.concatenate(self._c), np.concatenate(self._v)
+
+    def to_sparse(self):
+        r, c, v = self.coo()
+        return sparse.coo_matrix((v, (r, c)), shape=(self.n, self.n)).tocsr()
+
+
+def _interleave(*arrs):
+    return np.stack(arrs, axis=1).ravel()
+
+
+def diffusion_operator(mesh, gamma, system: OSystem):
+    """fvm_operators.py:59-100: per internal face with d_PN >= 1e-30,
+    coeff = gamma_f*A/d_PN, gamma_f harmonic (0 if gamma_O+gamma_N <= 1e-30);
+    entries (O,O,c), (N,N,c), (O,N,-c), (N,O,-c) in that order."""
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    fi = np.nonzero(nb >= 0)[0]
+    o, nb = o[fi], nb[fi]
+    d = _norm(mesh.cell_center[nb] - mesh.cell_center[o])
+    keep = ~(d < 1e-30)
+    fi, o, nb, d = fi[keep], o[keep], nb[keep], d[keep]
+    go, gn = gamma[o], gamma[nb]
+    s = go + gn
+    with np.errstate(invalid="ignore", divide="ignore"):
+        gf = np.where(s > 1e-30, 2.0 * go * gn / np.where(s > 1e-30, s, 1.0), 0.0)
+    c = gf * mesh.face_area[fi] / d
+    system.add_entries(_interleave(o, nb, o, nb), _interleave(o, nb, nb, o),
+                       _interleave(c, c, -c, -c))
+
+
+def convection_operator_upwind(mesh, F, system: OSystem):
+    """fvm_operators.py:103-129."""
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    F = np.asarray(F, np.float64)
+    internal = nb >= 0
+    nbs = np.where(internal, nb, o)
+    rows = np.stack([o, o, nbs, nbs], 1)
+    cols = np.stack([o, nbs, nbs, o], 1)
+    vals = np.stack([np.maximum(F, 0.0), np.minimum(F, 0.0),
+                     np.maximum(-F, 0.0), np.minimum(-F, 0.0)], 1)
+    valid = np.zeros((len(F), 4), bool)
+    valid[internal] = True
+    bpos = (~internal) & (F >= 0)
+    valid[bpos, 0] = True
+    vals[bpos, 0] = F[bpos]
+    system.add_entries(rows[valid], cols[valid], vals[valid])
+
+
+def temporal_operator(mesh, rho, dt, phi_old, system: OSystem):
+    """fvm_operators.py:132-148."""
+    n = mesh.n_cells
+    k = rho * mesh.cell_volume / dt
+    idx = np.arange(n)
+    system.add_entries(idx, idx, k)
+    system.add_source(idx, k * np.asarray(phi_old))
+
+
+def source_term(mesh, sv, system: OSystem):
+    """fvm_operators.py:151-161."""
+    system.add_source(np.arange(mesh.n_cells), np.asarray(sv) * mesh.cell_volume)
+
+
+def linearized_source(mesh, Sp, Su, system: OSystem):
+    """fvm_operators.py:164-175: per cell, diag entry then rhs."""
+    idx = np.arange(mesh.n_cells)
+    system.add_entries(idx, idx, -np.asarray(Sp) * mesh.cell_volume)
+    system.add_source(idx, np.asarray(Su) * mesh.cell_volume)
+
+
+def apply_boundary_conditions(mesh, phi_bnd: dict, gamma, F, system: OSystem, bc_types: dict):
+    """fvm_operators.py:178-225.  Patches in dictionary order, faces in patch
+    order; faces with d_Pf < 1e-30 skipped."""
+    F = np.asarray(F, np.float64)
+    for name, fids in mesh.boundary_patches.items():
+        fids = np.asarray(fids, np.int64)
+        bc = bc_types.get(name, {"type": "zero_gradient"})
+        kind = bc["type"]
+        o = mesh.owner[fids].astype(np.int64)
+        d = _norm(mesh.face_center[fids] - mesh.cell_center[o])
+        keep = ~(d < 1e-30)
+        li = np.arange(len(fids))[keep]
+        fids, o, d = fids[keep], o[keep], d[keep]
+        if kind == "dirichlet":
+            pb = np.asarray(phi_bnd[name], np.float64)[li]
+            c = gamma[o] * mesh.face_area[fids] / d
+            system.add_entries(o, o, c)
+            Ff = F[fids]
+            # rhs: c*phi_b, then -F*phi_b for inflow faces, face by face
+            src_idx = []
+            src_val = []
+            inflow = Ff < 0
+            idx2 = np.stack([o, o], 1)
+            val2 = np.stack([c * pb, -Ff * pb], 1)
+            m2 = np.stack([np.ones_like(inflow), inflow], 1)
+            system.add_source(idx2[m2], val2[m2])
+        elif kind == "neumann":
+            system.add_source(o, bc.get("value", 0.0) * mesh.face_area[fids])
+        elif kind == "zero_gradient":
+            Ff = F[fids]
+            m = Ff < 0
+            system.add_entries(o[m], o[m], -Ff[m])
+
+
+def under_relax(system: OSystem, phi, alpha):
+    """fvm_operators.py:228-240 (reads the diag tracker)."""
+    if alpha >= 1.0:
+        return
+    ap = system.diag.copy()
+    idx = np.arange(system.n)
+    add = ap * (1.0 - alpha) / alpha
+    system.add_entries(idx, idx, add)
+    system.add_source(idx, ap * (1.0 - alpha) / alpha * np.asarray(phi))
+
+
+# ---------------------------------------------------------------------------
+
+def interpolation_weight(mesh):
+    """gradient.py:102-114 for every internal face (1.0 on boundary faces)."""
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    internal = nb >= 0
+    xf = mesh.face_center
+    do = _norm(xf - mesh.cell_center[o])
+    dn = _norm(xf - mesh.cell_center[np.where(internal, nb, o)])
+    t = do + dn
+    with np.errstate(invalid="ignore", divide="ignore"):
+        g = np.where(t < 1e-30, 0.5, dn / np.where(t < 1e-30, 1.0, t))
+    return np.where(internal, g, 1.0)
+
+
+def boundary_face_values(mesh, values, bvals: dict):
+    """gradient.py:117-127: a boundary face takes its patch value; a face in
+    no listed patch takes the owner value.  Returns per-face array (internal
+    faces: nan)."""
+    out = np.full(mesh.n_faces, np.nan) if np.ndim(values) == 1 else \
+        np.full((mesh.n_faces,) + np.shape(values)[1:], np.nan)
+    bmask = mesh.neighbour < 0
+    out[bmask] = np.asarray(values)[mesh.owner[bmask]]
+    # first patch in dict order containing the face wins (linear search)
+    for name, fids in reversed(list(mesh.boundary_patches.items())):
+        if name in bvals:
+            out[np.asarray(fids, np.int64)] = np.asarray(bvals[name])
+    return out
+
+
+def green_gauss_gradient(mesh, values, bvals: dict):
+    """gradient.py:12-48."""
+    nd = mesh.ndim
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    internal = nb >= 0
+    values = np.asarray(values, np.float64)
+    g = interpolation_weight(mesh)
+    pf = boundary_face_values(mesh, values, bvals)
+    nbs = np.where(internal, nb, o)
+    pf = np.where(internal, g * values[o] + (1.0 - g) * values[nbs], pf)
+    flux = pf[:, None] * mesh.face_normal[:, :nd] * mesh.face_area[:, None]
+    grad = np.zeros((mesh.n_cells, nd))
+    # face order: owner += flux, then neighbour -= flux
+    idx = np.stack([o, nbs], 1)
+    val = np.stack([flux, -flux], 1)
+    msk = np.stack([np.ones(len(o), bool), internal], 1)
+    np.add.at(grad, idx[msk], val[msk])
+    vol = mesh.cell_volume
+    big = vol > 1e-30
+    grad[big] /= vol[big, None]
+    return grad
+
+
+LIMITERS = {
+    "minmod": lambda r: np.maximum(0.0, np.minimum(1.0, r)),
+    "superbee": lambda r: np.maximum(0.0, np.maximum(np.minimum(2.0 * r, 1.0), np.minimum(r, 2.0))),
+    "van_albada": lambda r: np.where(r > 0, (r * r + r) / (r * r + 1.0), 0.0),
+    "van_leer": lambda r: np.where(r > 0, 2.0 * r / (1.0 + r), 0.0),
+}  # interpolation.py:17-44
+
+
+def muscl_deferred_correction(mesh, values, F, grad, limiter="van_leer"):
+    """interpolation.py:227-303 (note r uses x_f - x_U, as the reference)."""
+    if limiter not in LIMITERS:
+        raise ValueError(f"Unknown limiter: {limiter}")
+    nd = mesh.ndim
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    fi = np.nonzero(nb >= 0)[0]
+    o, nb = o[fi], nb[fi]
+    q = np.asarray(F, np.float64)[fi]
+    pos = q >= 0
+    up = np.where(pos, o, nb)
+    dn = np.where(pos, nb, o)
+    values = np.asarray(values, np.float64)
+    pu, pd = values[up], values[dn]
+    dx = mesh.face_center[fi, :nd] - mesh.cell_center[up, :nd]
+    lin = pu + np.sum(grad[up] * dx, axis=1)
+    dface = lin - pu
+    dcd = pd - pu
+    act = ~(np.abs(dcd) < 1e-15)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r = 2.0 * dface / np.where(act, dcd, 1.0) - 1.0
+    psi = LIMITERS[limiter](r)
+    fc = np.abs(q) * ((pu + 0.5 * psi * dcd) - pu)
+    corr = np.zeros(mesh.n_cells)
+    idx = np.stack([up, dn], 1)[act]
+    val = np.stack([-fc, fc], 1)[act]
+    np.add.at(corr, idx, val)
+    return corr
+
+
+def compute_mass_flux(mesh, U, Ubnd: dict, rho):
+    """interpolation.py:163-182."""
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    internal = nb >= 0
+    nd = mesh.ndim
+    U = np.asarray(U, np.float64)
+    g = interpolation_weight(mesh)
+    nbs = np.where(internal, nb, o)
+    uf = g[:, None] * U[o] + (1.0 - g[:, None]) * U[nbs]
+    ub = boundary_face_values(mesh, U, Ubnd)
+    uf = np.where(internal[:, None], uf, ub)
+    return rho * np.sum(uf * mesh.face_normal[:, :nd], axis=1) * mesh.face_area
+
+
+def schiller_naumann_drag(Re):
+    """models/closure.py:13-34."""
+    Re = np.maximum(Re, 1e-10)
+    return np.where(Re < 1000, 24.0 / Re * (1.0 + 0.15 * Re ** 0.687), 0.44)
+
+
+def drag_coefficient_implicit(alpha_g, rho_l, u_g, u_l, d_b, mu_l):
+    """models/closure.py:72-92."""
+    ur = u_g - u_l
+    mag = np.maximum(np.sqrt(np.sum(ur ** 2, axis=1)), 1e-15)
+    cd = schiller_naumann_drag(rho_l * mag * d_b / mu_l)
+    return 0.75 * cd * alpha_g * rho_l * mag / d_b

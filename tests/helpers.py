@@ -1,0 +1,90 @@
+"""Test helpers: replay a workload recipe through the CPU oracle."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import fvm as ofvm
+
+
+def oracle_system(case):
+    """Assemble ``case`` with the oracle; returns (A csr, rhs, diag tracker)."""
+    mesh = case.mesh
+    s = ofvm.OSystem(mesh.n_cells)
+    F_cur = None
+    for op, kw in case.recipe:
+        if op == "mass_flux":
+            F_cur = ofvm.compute_mass_flux(mesh, kw["U"], kw["Ub"], kw["rho"])
+        elif op == "diffusion":
+            ofvm.diffusion_operator(mesh, kw["gamma"], s)
+        elif op == "upwind":
+            ofvm.convection_operator_upwind(mesh, kw["F"], s)
+        elif op == "upwind_scaled":
+            F_cur = F_cur * np.asarray(kw["scale_owner"])[mesh.owner]
+            ofvm.convection_operator_upwind(mesh, F_cur, s)
+        elif op == "boundary":
+            ofvm.apply_boundary_conditions(mesh, kw["phib"], kw["gamma"], kw["F"], s, kw["bc"])
+        elif op == "boundary_scaled":
+            ofvm.apply_boundary_conditions(mesh, kw["phib"], kw["gamma"], F_cur, s, kw["bc"])
+        elif op == "pressure":
+            pressure_correction(mesh, kw["dP"], kw["rho"], kw["dirichlet"], s)
+        elif op == "source":
+            ofvm.source_term(mesh, kw["S"], s)
+        elif op == "cell_terms":
+            idx = np.arange(mesh.n_cells)
+            k = np.asarray(kw["kd"]) * mesh.cell_volume
+            s.add_entries(idx, idx, k)
+            s.add_source(idx, k * np.asarray(kw["ks"]))
+        elif op == "temporal":
+            ofvm.temporal_operator(mesh, kw["rho"], kw["dt"], kw["phi_old"], s)
+        elif op == "muscl":
+            g = ofvm.green_gauss_gradient(mesh, kw["phi"], kw["phib"])
+            c = ofvm.muscl_deferred_correction(mesh, kw["phi"], kw["F"], g, kw["limiter"])
+            s.rhs = s.rhs + c
+        else:
+            raise ValueError(op)
+    A = s.to_sparse()
+    rhs = s.rhs.copy() if case.rhs is None else np.asarray(case.rhs, float).copy()
+    return A, rhs, s.diag.copy()
+
+
+def pressure_correction(mesh, dP, rho, dirichlet, s):
+    """models/single_phase.py:288-344 coefficients (COO order of the
+    reference: off-diagonals face by face, then one diagonal entry per cell)."""
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    n = mesh.n_cells
+    aP = np.zeros(n)
+    rows, cols, vals = [], [], []
+    dirset = set(dirichlet)
+    bname = {}
+    for nm, f in mesh.boundary_patches.items():
+        for ff in f:
+            bname[int(ff)] = nm
+    cc = mesh.cell_center
+    for f in range(mesh.n_faces):
+        if nb[f] >= 0:
+            d = np.linalg.norm(cc[nb[f]] - cc[o[f]])
+            if d < 1e-30:
+                continue
+            dv = cc[nb[f]] - cc[o[f]]
+            dorth = abs(np.dot(dv, mesh.face_normal[f]))
+            dorth = max(dorth, 0.1 * d)
+            df = rho * 0.5 * (dP[o[f]] + dP[nb[f]]) * mesh.face_area[f] / dorth
+            aP[o[f]] += df
+            aP[nb[f]] += df
+            rows += [o[f], nb[f]]
+            cols += [nb[f], o[f]]
+            vals += [-df, -df]
+        elif bname.get(f) in dirset:
+            d = np.linalg.norm(mesh.face_center[f] - cc[o[f]])
+            if d > 1e-30:
+                aP[o[f]] += rho * dP[o[f]] * mesh.face_area[f] / d
+    s.add_entries(np.array(rows, np.int64), np.array(cols, np.int64), np.array(vals))
+    idx = np.arange(n)
+    s.add_entries(idx, idx, aP)
+
+
+def rel_l2(a, b):
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / (nb if nb > 0 else 1.0))

@@ -1,0 +1,147 @@
+"""GPU parity of the BiCGSTAB path against the scipy oracle.
+
+Bars (from the task statement of this path): FP64 iterations within +-2 of
+scipy, solution within 1e-10 relative L2 of scipy's, FP64-verified true
+relative residual below the tolerance; ILU(0) factors equal to the C oracle's
+(same operation order) to 1e-14 relative.
+"""
+
+import numpy as np
+import pytest
+
+from helpers import oracle_system, rel_l2
+from oracle import golden, solve as osolve
+
+pytestmark = pytest.mark.gpu
+
+
+def _kc():
+    from kcfd_amd import device, workloads
+    return device, workloads
+
+
+def _gpu_solve(ctx, A, b, **kw):
+    device, _ = _kc()
+    M = device.DeviceMatrix.from_csr(ctx, A)
+    x, res = M.solve(b, **kw)
+    return x, res
+
+
+@pytest.mark.parametrize("pc", ["none", "jacobi"])
+def test_case16_golden_iterations(ctx, pc):
+    """The reference's committed figure: none 105, jacobi 95 (atol 1e-10, rtol 0)."""
+    A, b = golden.case16_system()
+    xo, info_o, it_o = osolve.solve_bicgstab(A, b, rtol=0.0, atol=1e-10, maxiter=2000, precond=pc)
+    x, res = _gpu_solve(ctx, A, b, rtol=0.0, atol=1e-10, maxiter=2000, precond=pc)
+    print(f"case16 {pc}: gpu iters={res.iters} info={res.info} | oracle {it_o} info={info_o} | "
+          f"figure {golden.CASE16_ITERS[pc]}")
+    assert abs(res.iters - it_o) <= 2
+    assert abs(res.iters - golden.CASE16_ITERS[pc]) <= 2
+    assert (res.info == 0) == (info_o == 0)
+
+
+@pytest.mark.parametrize("n", [32, 64, 128])
+def test_quad_convdiff_fp64_jacobi(ctx, n):
+    device, workloads = _kc()
+    case = workloads.quad_convdiff(n, seed=1001)
+    A, b, _ = oracle_system(case)
+    xo, info_o, it_o = osolve.solve_bicgstab(A, b, rtol=1e-8, precond="jacobi")
+    x, res = _gpu_solve(ctx, A, b, rtol=1e-8, precond="jacobi")
+    d = rel_l2(x, xo)
+    print(f"quad{n}: gpu iters={res.iters} info={res.info} resid={res.rel_resid:.2e} | "
+          f"oracle iters={it_o} | relL2={d:.2e}")
+    assert info_o == 0 and res.info == 0
+    assert abs(res.iters - it_o) <= 2
+    assert res.rel_resid < 1e-8
+    assert d < 1e-10
+
+
+@pytest.mark.parametrize("n", [8, 16])
+def test_hex_pressure_ilu0(ctx, n):
+    device, workloads = _kc()
+    case = workloads.hex_pressure(n, seed=1002)
+    A, b, _ = oracle_system(case)
+    M = device.DeviceMatrix.from_csr(ctx, A)
+    M.setup_precond("ilu0", "fp64")
+    lu, nl, nu = M.ilu0_factors()
+    ref = osolve.ILU0(A)
+    err = np.max(np.abs(lu - ref.lu) / np.maximum(np.abs(ref.lu), 1e-300))
+    print(f"hex{n} ilu0 factors: max rel diff {err:.2e}; levels L={nl} U={nu}")
+    assert err <= 1e-14
+    assert nl == 3 * n - 2 and nu == 3 * n - 2
+    xo, info_o, it_o = osolve.solve_bicgstab(A, b, rtol=1e-8, precond="ilu0_struct")
+    x, res = M.solve(b, rtol=1e-8, precond="ilu0")
+    d = rel_l2(x, xo)
+    print(f"hex{n} ilu0: gpu iters={res.iters} info={res.info} resid={res.rel_resid:.2e} | "
+          f"oracle {it_o} | relL2={d:.2e}")
+    assert res.info == 0 and abs(res.iters - it_o) <= 2 and res.rel_resid < 1e-8 and d < 1e-10
+
+
+def test_hex_pressure_jacobi(ctx):
+    device, workloads = _kc()
+    case = workloads.hex_pressure(24, seed=1002)
+    A, b, _ = oracle_system(case)
+    xo, info_o, it_o = osolve.solve_bicgstab(A, b, rtol=1e-8, precond="jacobi")
+    x, res = _gpu_solve(ctx, A, b, rtol=1e-8, precond="jacobi")
+    d = rel_l2(x, xo)
+    print(f"hex24 jacobi: gpu {res.iters} oracle {it_o} relL2={d:.2e}")
+    assert res.info == 0 and abs(res.iters - it_o) <= 2 and d < 1e-10
+
+
+@pytest.mark.parametrize("pc", ["jacobi", "ilu0"])
+def test_mixed_precision_refinement(ctx, pc):
+    """FP32 SpMV/preconditioner/vectors inside FP64 refinement: FP64-verified
+    residual below 1e-8 and the error against the FP64 oracle solution."""
+    device, workloads = _kc()
+    case = workloads.hex_pressure(16, seed=1002) if pc == "ilu0" else workloads.quad_convdiff(64)
+    A, b, _ = oracle_system(case)
+    xo, info_o, it_o = osolve.solve_bicgstab(A, b, rtol=1e-12,
+                                             precond="ilu0_struct" if pc == "ilu0" else "jacobi")
+    x, res = _gpu_solve(ctx, A, b, rtol=1e-8, precond=pc, precision="mixed", inner_rtol=1e-4)
+    err = rel_l2(x, xo)
+    true_res = osolve.true_rel_residual(A, x, b)
+    print(f"mixed {pc}: iters={res.iters} refinements={res.refinements} info={res.info} "
+          f"resid(gpu)={res.rel_resid:.2e} resid(cpu)={true_res:.2e} err vs fp64={err:.2e}")
+    assert res.info == 0
+    assert true_res < 1e-8 and res.rel_resid < 1e-8
+    assert abs(true_res - res.rel_resid) <= 1e-3 * true_res + 1e-15
+
+
+def test_batch_equals_alone(ctx):
+    """Each system of a batch gives bit-identical results alone and in the mix."""
+    device, workloads = _kc()
+    from kcfd_amd._lib import DevBuf
+    cases = [workloads.quad_convdiff(20, seed=1), workloads.hex_pressure(10, seed=2),
+             workloads.quad_convdiff(33, seed=3), workloads.hex_pressure(7, seed=4)]
+    systems = [oracle_system(c)[:2] for c in cases]
+    for prec in ["fp64", "mixed"]:
+        alone = []
+        for A, b in systems:
+            x, res = _gpu_solve(ctx, A, b, rtol=1e-8, precond="jacobi", precision=prec)
+            alone.append((x, res.iters))
+        mats = [device.DeviceMatrix.from_csr(ctx, A) for A, _ in systems]
+        for m in mats:
+            m.setup_precond("jacobi", prec)
+        bs = [DevBuf.from_array(ctx, b) for _, b in systems]
+        xs = [DevBuf.from_array(ctx, np.zeros(len(b))) for _, b in systems]
+        bt = device.Batch(ctx, mats, bs, xs, device.default_opts(rtol=1e-8, precond="jacobi",
+                                                                 precision=prec))
+        res = bt.run()
+        for k, (A, b) in enumerate(systems):
+            xb = xs[k].download()
+            print(f"batch {prec} sys{k}: iters alone={alone[k][1]} batch={res[k].iters}")
+            assert res[k].iters == alone[k][1]
+            assert np.array_equal(xb, alone[k][0])
+        bt.close()
+
+
+def test_zero_rhs_and_maxiter(ctx):
+    device, workloads = _kc()
+    case = workloads.quad_convdiff(16)
+    A, b, _ = oracle_system(case)
+    x, res = _gpu_solve(ctx, A, np.zeros(A.shape[0]), rtol=1e-8, precond="jacobi")
+    assert res.info == 0 and res.iters == 0 and np.all(x == 0)
+    x, res = _gpu_solve(ctx, A, b, rtol=1e-14, precond="jacobi", maxiter=3)
+    xo, info_o, it_o = osolve.solve_bicgstab(A, b, rtol=1e-14, precond="jacobi", maxiter=3)
+    assert res.info == info_o == 3 and res.iters == 3
+    assert rel_l2(x, xo) < 1e-12
