@@ -54,6 +54,7 @@ extern "C" {
 #define KCFD_BC_ZERO_GRADIENT 0
 #define KCFD_BC_DIRICHLET 1
 #define KCFD_BC_NEUMANN 2
+#define KCFD_BC_NONE 3          /* unknown type string: no contribution (as the reference) */
 
 /* TVD limiters (core/interpolation.py:17-44) */
 #define KCFD_LIM_MINMOD 0
@@ -113,6 +114,10 @@ int kcfd_mat_bytes(kcfd_mat *m, long long *bytes);
 int kcfd_precond_setup(kcfd_mat *m, int kind, int precision);
 /* ILU(0) factors (host nnz) and the number of levels of the L and U solves */
 int kcfd_ilu0_get(kcfd_mat *m, double *lu, int *levels_l, int *levels_u);
+/* y = M^{-1} x (dev) with the preconditioner set up on m (identity for
+ * none): the LinearOperator matvec that create_preconditioner returns
+ * (core/preconditioner.py:45-46, :52-53). */
+int kcfd_precond_apply(kcfd_mat *m, const double *x, double *y);
 
 /* ---------------------------------------------------------------- solve */
 

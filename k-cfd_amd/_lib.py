@@ -20,7 +20,7 @@ KCFD_OK = 0
 KCFD_EARG, KCFD_EHIP, KCFD_EOOM, KCFD_ESTATE, KCFD_EPIVOT, KCFD_EBUSY = -1, -2, -3, -4, -5, -6
 PC_NONE, PC_JACOBI, PC_ILU0 = 0, 1, 2
 FP64, MIXED = 0, 1
-BC_ZERO_GRADIENT, BC_DIRICHLET, BC_NEUMANN = 0, 1, 2
+BC_ZERO_GRADIENT, BC_DIRICHLET, BC_NEUMANN, BC_NONE = 0, 1, 2, 3
 LIMITERS = {"minmod": 0, "superbee": 1, "van_albada": 2, "van_leer": 3}
 PRECONDS = {"none": PC_NONE, "jacobi": PC_JACOBI, "ilu0": PC_ILU0}
 
@@ -88,6 +88,7 @@ SIGNATURES = {
     "kcfd_mat_bytes": ([_P, ctypes.POINTER(ctypes.c_longlong)], _I),
     "kcfd_precond_setup": ([_P, _I, _I], _I),
     "kcfd_ilu0_get": ([_P, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
+    "kcfd_precond_apply": ([_P, _P, _P], _I),
     "kcfd_solve": ([_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveResult)], _I),
     "kcfd_batch_create": ([_P, _I, _P, _P, _P, ctypes.POINTER(SolveOpts), _PP], _I),
     "kcfd_batch_run": ([_P, _P], _I),

@@ -605,7 +605,7 @@ int kcfd_asm_boundary(kcfd_mesh *mesh, kcfd_mat *a, const double *gamma, const d
     CHECK_PAIR(mesh, a);
     KCFD_ARG(gamma && F && kinds && (phib || mesh->nbf == 0), "null argument");
     for (int p = 0; p < mesh->npatch; ++p)
-        KCFD_ARG(kinds[p] >= 0 && kinds[p] <= 2, "unknown boundary-condition kind");
+        KCFD_ARG(kinds[p] >= 0 && kinds[p] <= 3, "unknown boundary-condition kind");
     std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
     hipStream_t s = a->ctx->stream;
     if (mesh->npatch) {
