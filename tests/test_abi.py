@@ -80,3 +80,14 @@ def test_argument_validation_without_gpu():
     assert L.kcfd_solve(None, None, None, None, None) == _lib.KCFD_EARG
     assert L.kcfd_batch_run(None, None) == _lib.KCFD_EARG
     assert b"invalid argument" in L.kcfd_last_error()
+
+
+def test_default_context_raises_without_gpu():
+    """default_context() raises (no deadlock, no fallback) when no device is visible."""
+    from kcfd_amd import _lib
+    import ctypes
+    h = ctypes.c_void_p()
+    if _lib.lib().kcfd_init(0, ctypes.byref(h)) == 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(_lib.KcfdError):
+        _lib.default_context()
