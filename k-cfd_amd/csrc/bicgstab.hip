@@ -610,32 +610,37 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
                                                  unsigned int *__restrict__ counter) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
+    // The chunk loop is wave-uniform: the chunk index is read from lane 0 with
+    // all lanes active, the exit test is on that uniform value, and per-lane
+    // work sits inside an if.  (A divergent `continue` here would let lanes
+    // re-enter the loop head without lane 0 and read a stale chunk index.)
     for (;;) {
-        int c = 0;
-        if (lane == 0) c = (int)atomicAdd(counter, 1u);
-        c = __shfl(c, 0, 64);
-        if (c >= nchunks) return;
+        unsigned int got = 0u;
+        if (lane == 0) got = atomicAdd(counter, 1u);
+        const int c = (int)__builtin_amdgcn_readfirstlane(got);
+        if (c >= nchunks) break;
         const Chunk ch = chunks[c];
         const SysDesc &d = sys[ch.sys];
         const int status = sts[ch.sys].status;
-        if (status != S_RUN && status != S_START) continue;
-        if (lane >= ch.count) continue;
-        const int i = lvl_rows[ch.sys][ch.start + lane];
-        const V *__restrict__ a = luv<V>(d);
-        const int dp = d.dpos[i];
-        if (!UPPER) {
-            const V *z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
-            V *y = (V *)d.y;
-            V acc = z[i];
-            for (int q = d.ip[i]; q < dp; ++q) acc -= a[q] * poll_ready<V>(y + d.ix[q]);
-            publish<V>(y + i, acc);
-        } else {
-            V *y = (V *)d.y;
-            V *w = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
-            V acc = y[i];
-            for (int q = dp + 1; q < d.ip[i + 1]; ++q) acc -= a[q] * poll_ready<V>(w + d.ix[q]);
-            publish<V>(w + i, acc / a[dp]);
-            y[i] = sentinel_value<V>();
+        const bool live = (status == S_RUN || status == S_START) && lane < ch.count;
+        if (live) {
+            const int i = lvl_rows[ch.sys][ch.start + lane];
+            const V *__restrict__ a = luv<V>(d);
+            const int dp = d.dpos[i];
+            if (!UPPER) {
+                const V *z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
+                V *y = (V *)d.y;
+                V acc = z[i];
+                for (int q = d.ip[i]; q < dp; ++q) acc -= a[q] * poll_ready<V>(y + d.ix[q]);
+                publish<V>(y + i, acc);
+            } else {
+                V *y = (V *)d.y;
+                V *w = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
+                V acc = y[i];
+                for (int q = dp + 1; q < d.ip[i + 1]; ++q) acc -= a[q] * poll_ready<V>(w + d.ix[q]);
+                publish<V>(w + i, acc / a[dp]);
+                y[i] = sentinel_value<V>();
+            }
         }
     }
 }

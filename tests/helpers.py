@@ -88,3 +88,20 @@ def pressure_correction(mesh, dP, rho, dirichlet, s):
 def rel_l2(a, b):
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / (nb if nb > 0 else 1.0))
+
+
+def oracle_band(A, b, nrep=6, **kw):
+    """scipy BiCGSTAB on b and on nrep-1 copies of b perturbed by one ulp per
+    entry: the reference's own rounding-noise floor.  Returns (iteration
+    counts, infos, x of the unperturbed run, max relative L2 spread of the
+    perturbed solutions from it)."""
+    from oracle import solve as osolve
+    its, infos, xs = [], [], []
+    for k in range(nrep):
+        bb = b if k == 0 else b * (1 + np.random.default_rng(k).choice([-1, 1], len(b)) * 2.0 ** -52)
+        x, info, it = osolve.solve_bicgstab(A, bb, **kw)
+        its.append(it)
+        infos.append(info)
+        xs.append(x)
+    spread = max(rel_l2(x, xs[0]) for x in xs[1:]) if nrep > 1 else 0.0
+    return its, infos, xs[0], spread
