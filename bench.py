@@ -123,8 +123,9 @@ def main():
             xdev.fill(0.0)
             return A.solve_dev(bdev.p, xdev.p, opts)
 
-        for _ in range(args.warmup):
-            step()
+        for w in range(args.warmup):
+            r = step()
+            log(f"{prec} warmup {w}: iters={r.iters} refinements={r.refinements} ms={r.ms:.1f}")
         ctx.sync()
         t0 = time.perf_counter()
         for _ in range(args.steps):
@@ -132,6 +133,8 @@ def main():
             r = step()
             lat.append(time.perf_counter() - ts)
             rs.append(r)
+            log(f"{prec} step: iters={r.iters} info={r.info} resid={r.rel_resid:.2e} "
+                f"{lat[-1] * 1e3:.1f} ms")
         ctx.sync()
         wall = time.perf_counter() - t0
         r = rs[-1]

@@ -589,7 +589,7 @@ __device__ __forceinline__ V poll_ready(const V *p) {
     const U *q = reinterpret_cast<const U *>(p);
     U b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (b == Sentinel<V>::bits) {
-        __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(2);
         b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     return __builtin_bit_cast(V, b);
@@ -622,7 +622,12 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
         const Chunk ch = chunks[c];
         const SysDesc &d = sys[ch.sys];
         const int status = sts[ch.sys].status;
-        const bool live = (status == S_RUN || status == S_START) && lane < ch.count;
+        // the p solves (after K1) serve systems in iteration 0 or later; the s
+        // solves (after K3) only systems still running -- a system that the
+        // FP64 refinement just restarted (S_START set inside K2) must not run
+        // them, or stale values would be left in sh
+        const bool want = WHICH == 0 ? (status == S_RUN || status == S_START) : (status == S_RUN);
+        const bool live = want && lane < ch.count;
         if (live) {
             const int i = lvl_rows[ch.sys][ch.start + lane];
             const V *__restrict__ a = luv<V>(d);
@@ -893,11 +898,22 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
         KCFD_HIP(hipMalloc((void **)&b->d_lvl_u, sizeof(int *) * nsys));
         KCFD_HIP(hipMemcpy((void *)b->d_lvl_l, pl.data(), sizeof(int *) * nsys, hipMemcpyHostToDevice));
         KCFD_HIP(hipMemcpy((void *)b->d_lvl_u, pu.data(), sizeof(int *) * nsys, hipMemcpyHostToDevice));
-        // persistent-style grid: enough waves to keep every CU busy; chunks
-        // are handed out dynamically, so any grid size is deadlock-free
-        const int want = std::max(b->nchunks_l, b->nchunks_u);
-        int cap = ctx->cus * 8;  // 8 blocks (32 waves) per CU
-        b->trsv_grid = std::max(1, std::min((want + kWaves - 1) / kWaves, cap));
+        // Chunks are handed out dynamically in level order, so any grid size is
+        // deadlock-free.  Waves beyond a few levels of look-ahead only poll
+        // (and load the memory system with polls), so the grid covers about
+        // three of the widest levels (summed over the batch), capped at 8
+        // blocks per CU.
+        int widest = 0;
+        for (int l = 0; l < std::max(maxlev_l, maxlev_u); ++l) {
+            int w = 0;
+            for (auto *m : b->mats) {
+                if (l < m->nlev_l) w += (m->h_lvl_ptr_l[l + 1] - m->h_lvl_ptr_l[l] + 63) / 64;
+            }
+            widest = std::max(widest, w);
+        }
+        const int want_waves = std::max(64, 3 * widest);
+        const int cap = ctx->cus * 8;
+        b->trsv_grid = std::max(1, std::min((want_waves + kWaves - 1) / kWaves, cap));
     }
     *out = b;
     return KCFD_OK;
