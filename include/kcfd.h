@@ -127,8 +127,10 @@ typedef struct kcfd_solve_opts {
     int maxiter;        /* <=0: 10*n, as scipy */
     int precond;        /* KCFD_PC_* */
     int precision;      /* KCFD_FP64 / KCFD_MIXED */
-    double inner_rtol;  /* MIXED: inner FP32 solve reduction per refinement */
-    int max_refine;     /* MIXED: refinement steps cap */
+    double inner_rtol;  /* MIXED: reliable-update trigger -- the FP64 residual is
+                           recomputed once the FP32 residual has dropped by this
+                           factor since the last update (default 0.1) */
+    int max_refine;     /* MIXED: cap on reliable updates (default 1000) */
     int check_every;    /* iterations per captured graph between host checks */
     int rows_per_thread;/* SpMV rows per lane group (0: autotuned default) */
     int lanes_per_row;  /* 1, 2, 4, 8 (0: autotuned default) */

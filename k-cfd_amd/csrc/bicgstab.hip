@@ -1,58 +1,66 @@
 // Batched, device-resident BiCGSTAB (Jacobi / ILU(0) / none) for gfx950.
 //
-// One kernel launch per phase covers the row tiles of every active system of
-// a batch; each system carries its own scalar state machine on the device
-// (status, rho, alpha, omega, beta, iteration counts), so scalars never leave
-// the device and converged systems drop out without host involvement.  The
-// host replays a captured hipGraph of `check_every` iterations and reads the
-// status words in between.
+// One launch per phase covers the row tiles of every active system of a
+// batch (persistent, XCD-aware tile walk); each system carries its own scalar
+// state machine on the device (status, rho, alpha, omega, beta, counts), so
+// scalars never leave the device and converged systems drop out without the
+// host.  The host replays a captured hipGraph of `check_every` iterations and
+// reads the status words in between.
 //
-// The recurrences are scipy's right-preconditioned BiCGSTAB
+// Recurrences: scipy's right-preconditioned BiCGSTAB
 // (scipy/sparse/linalg/_isolve/iterative.py:157-302, the solver behind
-// core/linear_solver.py:59), with the same stopping tests, breakdown tests
-// (rhotol = omegatol = eps^2, rv == 0), half-step exit on ||s||, iteration
-// count (= callback count) and the same operation order of every vector
-// update (products and sums not fused), so that FP64 iterates differ from
-// scipy's only through the summation order of dot products.
+// core/linear_solver.py:59): same stopping and breakdown tests (rhotol =
+// omegatol = eps^2, rv == 0), half-step exit on ||s||, iteration count =
+// callback count, and the same operation order in every vector update
+// (products and sums not fused).  The SpMV reads a SELL-64 copy of A whose
+// rows keep CSR order, so A z is bit-identical to scipy's csr_matvec; FP64
+// iterates differ from scipy's only through dot-product summation order.
 //
 // Per iteration (Jacobi folded into the gathers; ILU(0) adds two sync-free
 // triangular solves after K1 and after K3):
-//   K1  p = r + beta (p - omega v)          (p = r at iteration 0)
-//   K2  v = A M p,  rv = rt.v -> alpha
-//   K3  s = r - alpha v,  ||s|| -> half-step exit
-//   K4  t = A M s,  t.s, t.t -> omega
-//   K5  x += alpha M p; x += omega M s; r = s - omega t;  ||r||, rt.r -> beta
-// Reductions: wave64 xor-shuffle + LDS per block, block partials in a fixed
-// slot, and the last block of each system (atomic ticket) sums the partials
-// in tile order, so results are deterministic run to run.
+//   K1 p = r + beta (p - omega v)        (p = r at iteration 0)
+//   K2 v = A M p; rt.v           F2: alpha
+//   K3 s = r - alpha v; ||s||    F3: half-step exit
+//   K4 t = A M s; t.s, t.t       F4: omega
+//   K5 x += alpha M p; x += omega M s; r = s - omega t; ||r||, rt.r
+//                                F5: convergence / breakdown / beta
+// Reductions: wave64 xor-shuffle + LDS per tile, one partial record per tile,
+// and a finalize kernel (one workgroup per system) that sums the partials in
+// tile order -- deterministic, and free of in-kernel fences (on gfx950 an
+// agent-scope fence writes back and invalidates the XCD's L2).
 //
-// KCFD_MIXED: the same kernels run on FP32 matrix values, FP32 preconditioner
-// and FP32 vectors (products exact in FP64, sums and every dot product in
-// FP64) to solve A d = r; between inner solves K1/K2 perform the FP64
-// refinement x += d, r = b - A x on the FP64 matrix.
+// KCFD_MIXED: the iteration runs on FP32 matrix values, FP32 preconditioner
+// and FP32 vectors (products exact in FP64, sums and all dot products in FP64)
+// and solves for the correction d; reliable updates (residual replacement:
+// x += d, r = b - A x in FP64 on the FP64 matrix, r32 = r) are taken whenever
+// the FP32 residual has dropped by `delta` since the last one, when it claims
+// convergence, or at the iteration limit, without restarting the Krylov
+// space (the shadow residual and p are kept; rho and beta are recomputed from
+// the replaced residual).  Convergence is judged on the FP64 true residual.
 #include "common.hpp"
 
 #include <algorithm>
-#include <chrono>
 #include <cmath>
 #include <cstring>
 
 namespace kcfd {
 
 enum : int {
-    S_RUN = 0,       // inner iteration in progress
-    S_START = 1,     // iteration 0 next (p = r)
-    S_HALF = 2,      // half-step exit: x += alpha M p pending
-    S_REFINE = 3,    // MIXED: x += d, r = b - A x pending
-    S_ZEROX = 4,     // ||b|| == 0: x = 0 pending
-    S_DONE = 16,     // terminal
+    S_RUN = 0,      // iteration in progress
+    S_START = 1,    // iteration 0 next: p = r
+    S_HALF = 2,     // half-step exit: x += alpha M p pending
+    S_REFINE = 3,   // MIXED: reliable update (x += d; r = b - A x) pending
+    S_ZEROX = 4,    // ||b|| == 0: x = 0 pending
+    S_RESUME = 5,   // MIXED: continue after a reliable update (p-update with new beta)
+    S_DONE = 16,    // terminal
 };
 
 struct SysDesc {
     int n, tile0, ntiles, pad0;
-    const int *ip, *ix, *dpos;
-    const double *a64;
-    const float *a32;
+    const int *ip, *ix, *dpos;      // CSR (ILU, level lists)
+    const int *sptr, *six;          // SELL-64 structure
+    const double *sa64;
+    const float *sa32;
     const double *dinv64;
     const float *dinv32;
     const double *lu64;
@@ -65,91 +73,39 @@ struct SysDesc {
 
 struct SysState {
     double rho, rho_prev, alpha, omega, beta;
-    double atol_in;    // inner absolute threshold on ||r||
-    double atol_out;   // outer absolute threshold (MIXED)
-    double bnrm;
-    double rr, rr64;
+    double atol_in, atol_out, bnrm;
+    double rr, rr64, rmax, rbest;
     double true_rr, true_bb;
     int status, iters, info, half;
-    int refinements, total_iters, maxiter, inner_iters;
-    unsigned int ticket;
-    int pad[3];
+    int refinements, total_iters, maxiter, restart;
+    int final_, stagnant, pad[2];
 };
 
 struct BatchParams {
-    double rtol, atol, inner_rtol, rhotol;
+    double rtol, atol, delta, rhotol;
     int precision, precond, max_refine, nsys;
     int inner_maxiter, pad[3];
 };
 
-// ILU trisolve work chunk: 64 consecutive rows of one system's level list
+// ILU trisolve work chunk: up to 64 consecutive rows of one system's level list
 struct Chunk {
     int sys, start, count, pad;
 };
 
-__device__ __forceinline__ int tile_of_block() { return xcd_tile(blockIdx.x, gridDim.x); }
-
-// ------------------------------------------------------------- reductions
-//
-// Block partial -> ticket -> last block of the system sums partials in
-// tile order.  Returns true in *all threads* of the last block, with the
-// system sums broadcast in `tot`.
-template <int NV>
-__device__ __forceinline__ bool reduce_system(double (&v)[NV], const SysDesc &d, SysState &st,
-                                              double *__restrict__ partials, int tile,
-                                              double (&tot)[NV]) {
-    __shared__ int s_last;
-    __shared__ double s_tot[NV > 0 ? NV : 1];
-    block_sum<NV>(v);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) partials[(size_t)tile * kMaxPartials + k] = v[k];
-        __threadfence();
-        unsigned prev = atomicAdd(&st.ticket, 1u);
-        s_last = (prev == (unsigned)(d.ntiles - 1));
-    }
-    __syncthreads();
-    if (!s_last) return false;
-    __threadfence();
-    double acc[NV];
-#pragma unroll
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int q = threadIdx.x; q < d.ntiles; q += kBlock) {
-        const volatile double *pp = partials + (size_t)(d.tile0 + q) * kMaxPartials;
-#pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += pp[k];
-    }
-    block_sum<NV>(acc);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < NV; ++k) s_tot[k] = acc[k];
-        atomicExch(&st.ticket, 0u);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NV; ++k) tot[k] = s_tot[k];
-    return true;
-}
-
-// ticket without values (status transitions of elementwise phases)
-__device__ __forceinline__ bool last_block(const SysDesc &d, SysState &st) {
-    __shared__ int s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        unsigned prev = atomicAdd(&st.ticket, 1u);
-        s_last = (prev == (unsigned)(d.ntiles - 1));
-        if (s_last) atomicExch(&st.ticket, 0u);
-    }
-    __syncthreads();
-    return s_last != 0;
-}
+// ----------------------------------------------------------- tile walk
+// Blocks with blockIdx % 8 == x (the XCD they are dispatched to) cover the
+// contiguous tile range [x*per, (x+1)*per).  gridDim.x is a multiple of 8.
+#define KCFD_TILE_LOOP(tile, ntiles)                                                      \
+    for (int per_ = ((ntiles) + kXcds - 1) / kXcds, k_ = blockIdx.x / kXcds,               \
+             tile = (int)(blockIdx.x % kXcds) * per_ + k_;                                 \
+         k_ < per_ && tile < (ntiles);                                                      \
+         k_ += (int)gridDim.x / kXcds, tile = (int)(blockIdx.x % kXcds) * per_ + k_)
 
 // ----------------------------------------------------- element accessors
 
-template <typename V> __device__ __forceinline__ const V *mat_vals(const SysDesc &d);
-template <> __device__ __forceinline__ const double *mat_vals<double>(const SysDesc &d) { return d.a64; }
-template <> __device__ __forceinline__ const float *mat_vals<float>(const SysDesc &d) { return d.a32; }
+template <typename V> __device__ __forceinline__ const V *sell_vals(const SysDesc &d);
+template <> __device__ __forceinline__ const double *sell_vals<double>(const SysDesc &d) { return d.sa64; }
+template <> __device__ __forceinline__ const float *sell_vals<float>(const SysDesc &d) { return d.sa32; }
 template <typename V> __device__ __forceinline__ const V *jac(const SysDesc &d);
 template <> __device__ __forceinline__ const double *jac<double>(const SysDesc &d) { return d.dinv64; }
 template <> __device__ __forceinline__ const float *jac<float>(const SysDesc &d) { return d.dinv32; }
@@ -171,7 +127,7 @@ template <typename V> __device__ __forceinline__ V sentinel_value() {
     return __builtin_bit_cast(V, Sentinel<V>::bits);
 }
 
-// preconditioned operand M z at row j: Jacobi dinv_j*z_j, ILU: zh_j, none: z_j
+// preconditioned operand (M z)_j: Jacobi dinv_j z_j, ILU zh_j, none z_j
 template <typename V, int PC>
 __device__ __forceinline__ double precond_at(const SysDesc &d, const V *__restrict__ z,
                                              const V *__restrict__ zh, int j) {
@@ -181,71 +137,113 @@ __device__ __forceinline__ double precond_at(const SysDesc &d, const V *__restri
     return (double)z[j];
 }
 
-// (A M z)_i, sequential over the row in CSR order
+// (A M z)_i from the SELL-64 copy, entries in CSR order
 template <typename V, int PC>
 __device__ __forceinline__ double row_amz(const SysDesc &d, int i, const V *__restrict__ z,
                                           const V *__restrict__ zh) {
 #pragma clang fp contract(off)
-    const V *__restrict__ a = mat_vals<V>(d);
+    const V *__restrict__ a = sell_vals<V>(d);
+    const int s = i >> 6;
+    const int s0 = d.sptr[s], w = (d.sptr[s + 1] - s0) >> 6;
+    const int base = s0 + (i & 63);
     double acc = 0.0;
-    const int q0 = d.ip[i], q1 = d.ip[i + 1];
-    for (int q = q0; q < q1; ++q) {
-        const int j = d.ix[q];
-        acc += (double)a[q] * precond_at<V, PC>(d, z, zh, j);
+    for (int k = 0; k < w; ++k) {
+        const int q = base + 64 * k;
+        acc += (double)a[q] * precond_at<V, PC>(d, z, zh, d.six[q]);
     }
     return acc;
 }
 
-// ------------------------------------------------------------------ init
+// (A x)_i in FP64 on the FP64 values
+__device__ __forceinline__ double row_ax64(const SysDesc &d, int i, const double *__restrict__ x) {
+#pragma clang fp contract(off)
+    const int s = i >> 6;
+    const int s0 = d.sptr[s], w = (d.sptr[s + 1] - s0) >> 6;
+    const int base = s0 + (i & 63);
+    double acc = 0.0;
+    for (int k = 0; k < w; ++k) {
+        const int q = base + 64 * k;
+        acc += d.sa64[q] * x[d.six[q]];
+    }
+    return acc;
+}
 
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double *__restrict__ partials, int tile) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) partials[(size_t)tile * kMaxPartials + k] = v[k];
+    }
+}
+
+// sum of a system's tile partials in tile order (finalize kernels; one block)
+template <int NV>
+__device__ __forceinline__ void sum_partials(const SysDesc &d, const double *__restrict__ partials,
+                                             double (&tot)[NV]) {
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int q = threadIdx.x; q < d.ntiles; q += kBlock) {
+        const double *pp = partials + (size_t)(d.tile0 + q) * kMaxPartials;
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += pp[k];
+    }
+    block_sum<NV>(acc);
+#pragma unroll
+    for (int k = 0; k < NV; ++k) tot[k] = acc[k];
+}
+
+// ------------------------------------------------------------------ init
 // FP64: r = b - A x, rt = r; sums ||b||^2, ||r||^2.  MIXED: ||b||^2; d = 0.
 // ILU: every triangular-solve buffer starts at the sentinel.
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys,
-                                                 SysState *__restrict__ sts,
+                                                 const SysState *__restrict__ sts,
                                                  const int2 *__restrict__ tiles, int ntiles,
                                                  double *__restrict__ partials, BatchParams bp) {
 #pragma clang fp contract(off)
-    const int tile = tile_of_block();
-    if (tile >= ntiles) return;
-    const int2 ti = tiles[tile];
-    const SysDesc &d = sys[ti.x];
-    SysState &st = sts[ti.x];
-    if (st.status != S_START) return;
-    const int i = ti.y + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (i < d.n) {
-        const double bi = d.b[i];
-        v[0] = bi * bi;
-        if (bp.precision == KCFD_FP64) {
-            const double *__restrict__ a = d.a64;
-            double acc = 0.0;
-            for (int q = d.ip[i]; q < d.ip[i + 1]; ++q) acc += a[q] * d.x[d.ix[q]];
-            const double ri = bi - acc;
-            ((double *)d.r)[i] = ri;
-            ((double *)d.rt)[i] = ri;
-            v[1] = ri * ri;
-        } else {
-            ((V *)d.d)[i] = (V)0;
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        if (sts[ti.x].status != S_START) continue;
+        const int i = ti.y + threadIdx.x;
+        double v[2] = {0.0, 0.0};
+        if (i < d.n) {
+            const double bi = d.b[i];
+            v[0] = bi * bi;
+            if (bp.precision == KCFD_FP64) {
+                const double ri = bi - row_ax64(d, i, d.x);
+                ((double *)d.r)[i] = ri;
+                ((double *)d.rt)[i] = ri;
+                v[1] = ri * ri;
+            } else {
+                ((V *)d.d)[i] = (V)0;
+            }
+            if (PC == KCFD_PC_ILU0) {
+                ((V *)d.y)[i] = sentinel_value<V>();
+                ((V *)d.ph)[i] = sentinel_value<V>();
+                ((V *)d.sh)[i] = sentinel_value<V>();
+            }
         }
-        if (PC == KCFD_PC_ILU0) {
-            ((V *)d.y)[i] = sentinel_value<V>();
-            ((V *)d.ph)[i] = sentinel_value<V>();
-            ((V *)d.sh)[i] = sentinel_value<V>();
-        }
+        write_partials<2>(v, partials, tile);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void f_init(const SysDesc *__restrict__ sys,
+                                                 SysState *__restrict__ sts,
+                                                 const double *__restrict__ partials, BatchParams bp) {
+    const SysDesc &d = sys[blockIdx.x];
+    SysState &st = sts[blockIdx.x];
+    if (st.status != S_START) return;
     double tot[2];
-    if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
+    sum_partials<2>(d, partials, tot);
     if (threadIdx.x != 0) return;
     const double bnrm = sqrt(tot[0]);
     st.bnrm = bnrm;
     const double thr = fmax(bp.atol, bp.rtol * bnrm);
-    st.iters = 0;
-    st.total_iters = 0;
-    st.refinements = 0;
-    st.half = 0;
-    st.info = 0;
-    st.inner_iters = 0;
+    st.iters = st.total_iters = st.refinements = st.half = st.info = 0;
+    st.stagnant = st.final_ = 0;
     if (bnrm == 0.0) {          // scipy returns x = b (= 0), info 0
         st.status = S_ZEROX;
         return;
@@ -253,54 +251,44 @@ __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys
     if (bp.precision == KCFD_FP64) {
         st.atol_in = thr;
         st.rr = tot[1];
-        if (st.maxiter <= 0) { st.status = S_DONE; return; }
-        if (sqrt(tot[1]) < thr) { st.status = S_DONE; return; }
+        if (st.maxiter <= 0 || sqrt(tot[1]) < thr) { st.status = S_DONE; return; }
         st.rho = tot[1];           // rt = r, so rt.r = ||r||^2
         if (fabs(st.rho) < bp.rhotol) { st.info = KCFD_INFO_RHO_BREAKDOWN; st.status = S_DONE; return; }
-        st.status = S_START;
     } else {
         st.atol_out = thr;
+        st.atol_in = thr;
+        st.restart = 1;
+        st.rbest = INFINITY;
         st.status = S_REFINE;
     }
 }
 
 // -------------------------------------------------------------------- K1
 // p-update / p = r / MIXED x += d, d = 0 / x = 0 for a zero rhs.
-// Block gridDim.x-1 (beyond the tiles) resets the ILU chunk counters.
+// Block 0 also clears the chunk counters of the two p triangular solves.
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
-                                              SysState *__restrict__ sts,
+                                              const SysState *__restrict__ sts,
                                               const int2 *__restrict__ tiles, int ntiles,
                                               unsigned int *__restrict__ counters) {
 #pragma clang fp contract(off)
-    if (blockIdx.x == gridDim.x - 1) {
-        if (threadIdx.x < 2) counters[threadIdx.x] = 0u;
-        return;
-    }
-    const int tile = xcd_tile(blockIdx.x, gridDim.x - 1);
-    if (tile >= ntiles) return;
-    const int2 ti = tiles[tile];
-    const SysDesc &d = sys[ti.x];
-    SysState &st = sts[ti.x];
-    const int status = st.status;
-    const int i = ti.y + threadIdx.x;
-    if (status == S_RUN || status == S_START) {
-        if (i < d.n) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) counters[threadIdx.x] = 0u;
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const SysState &st = sts[ti.x];
+        const int status = st.status;
+        const int i = ti.y + threadIdx.x;
+        if (i >= d.n) continue;
+        if (status == S_RUN || status == S_RESUME) {
             V *__restrict__ p = (V *)d.p;
-            const V ri = ((const V *)d.r)[i];
-            if (status == S_START) {
-                p[i] = ri;
-            } else {
-                const V om = (V)st.omega, be = (V)st.beta;
-                V pi = p[i] - om * ((const V *)d.v)[i];
-                pi = pi * be;
-                p[i] = pi + ri;
-            }
-        }
-        return;
-    }
-    if (status == S_REFINE) {
-        if (i < d.n) {
+            const V om = (V)st.omega, be = (V)st.beta;
+            V pi = p[i] - om * ((const V *)d.v)[i];
+            pi = pi * be;
+            p[i] = pi + ((const V *)d.r)[i];
+        } else if (status == S_START) {
+            ((V *)d.p)[i] = ((const V *)d.r)[i];
+        } else if (status == S_REFINE) {
             V *__restrict__ dd = (V *)d.d;
             d.x[i] = d.x[i] + (double)dd[i];
             dd[i] = (V)0;
@@ -309,234 +297,284 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
                 ((V *)d.sh)[i] = sentinel_value<V>();
                 ((V *)d.y)[i] = sentinel_value<V>();
             }
+        } else if (status == S_ZEROX) {
+            d.x[i] = 0.0;
         }
-        return;
-    }
-    if (status == S_ZEROX) {
-        if (i < d.n) d.x[i] = 0.0;
-        if (last_block(d, st) && threadIdx.x == 0) st.status = S_DONE;
     }
 }
 
 // -------------------------------------------------------------------- K2
-// v = A M p; rt.v -> alpha.   MIXED S_REFINE: r64 = b - A64 x; r = rt = (V)r64.
+// v = A M p; rt.v.   MIXED S_REFINE: r64 = b - A64 x; r = (V)r64 (and rt = r
+// on a restart); sums ||r64||^2, rt.r, ||r||^2.
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_spmv_p(const SysDesc *__restrict__ sys,
-                                                   SysState *__restrict__ sts,
+                                                   const SysState *__restrict__ sts,
                                                    const int2 *__restrict__ tiles, int ntiles,
-                                                   double *__restrict__ partials, BatchParams bp) {
+                                                   double *__restrict__ partials) {
 #pragma clang fp contract(off)
-    const int tile = tile_of_block();
-    if (tile >= ntiles) return;
-    const int2 ti = tiles[tile];
-    const SysDesc &d = sys[ti.x];
-    SysState &st = sts[ti.x];
-    const int status = st.status;
-    const int i = ti.y + threadIdx.x;
-    if (status == S_RUN || status == S_START) {
-        double v[1] = {0.0};
-        if (i < d.n) {
-            const double vi = row_amz<V, PC>(d, i, (const V *)d.p, (const V *)d.ph);
-            const V vv = (V)vi;
-            ((V *)d.v)[i] = vv;
-            v[0] = (double)((const V *)d.rt)[i] * (double)vv;
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const SysState &st = sts[ti.x];
+        const int status = st.status;
+        const int i = ti.y + threadIdx.x;
+        if (status == S_RUN || status == S_START || status == S_RESUME) {
+            double v[1] = {0.0};
+            if (i < d.n) {
+                const V vv = (V)row_amz<V, PC>(d, i, (const V *)d.p, (const V *)d.ph);
+                ((V *)d.v)[i] = vv;
+                v[0] = (double)((const V *)d.rt)[i] * (double)vv;
+            }
+            write_partials<1>(v, partials, tile);
+        } else if (status == S_REFINE) {
+            double v[3] = {0.0, 0.0, 0.0};
+            if (i < d.n) {
+                const double r64 = d.b[i] - row_ax64(d, i, d.x);
+                const V rv = (V)r64;
+                ((V *)d.r)[i] = rv;
+                if (st.restart) ((V *)d.rt)[i] = rv;
+                v[0] = r64 * r64;
+                v[1] = (double)((const V *)d.rt)[i] * (double)rv;
+                v[2] = (double)rv * (double)rv;
+            }
+            write_partials<3>(v, partials, tile);
         }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void f_spmv_p(const SysDesc *__restrict__ sys,
+                                                   SysState *__restrict__ sts,
+                                                   const double *__restrict__ partials, BatchParams bp) {
+    const SysDesc &d = sys[blockIdx.x];
+    SysState &st = sts[blockIdx.x];
+    const int status = st.status;
+    const bool mixed = bp.precision == KCFD_MIXED;
+    if (status == S_RUN || status == S_START || status == S_RESUME) {
         double tot[1];
-        if (!reduce_system<1>(v, d, st, partials, tile, tot)) return;
+        sum_partials<1>(d, partials, tot);
         if (threadIdx.x != 0) return;
         const double rv = tot[0];
         if (rv == 0.0) {
             st.info = KCFD_INFO_OMEGA_BREAKDOWN;
-            st.status = bp.precision == KCFD_MIXED ? S_REFINE : S_DONE;
-            if (bp.precision == KCFD_MIXED && st.iters == 0) st.status = S_DONE;
+            if (!mixed || st.iters == 0) { st.status = S_DONE; return; }
+            st.restart = 1;
+            st.status = S_REFINE;
             return;
         }
         st.alpha = st.rho / rv;
         st.status = S_RUN;
-        return;
-    }
-    if (status == S_REFINE) {
-        double v[2] = {0.0, 0.0};
-        if (i < d.n) {
-            const double *__restrict__ a = d.a64;
-            double acc = 0.0;
-            for (int q = d.ip[i]; q < d.ip[i + 1]; ++q) acc += a[q] * d.x[d.ix[q]];
-            const double r64 = d.b[i] - acc;
-            const V rv = (V)r64;
-            ((V *)d.r)[i] = rv;
-            ((V *)d.rt)[i] = rv;
-            v[0] = r64 * r64;
-            v[1] = (double)rv * (double)rv;
-        }
-        double tot[2];
-        if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
+    } else if (status == S_REFINE) {
+        double tot[3];
+        sum_partials<3>(d, partials, tot);
         if (threadIdx.x != 0) return;
-        const double prev = st.rr64;
         st.rr64 = tot[0];
+        st.refinements += 1;
         const double rn = sqrt(tot[0]);
         if (rn < st.atol_out) { st.info = 0; st.status = S_DONE; return; }
-        // stop when a refinement step no longer halves the FP64 residual
-        const bool stagnant = st.refinements > 0 && !(tot[0] < 0.25 * prev);
-        if (stagnant || st.refinements >= bp.max_refine || st.total_iters >= st.maxiter) {
+        if (st.final_) { st.info = st.total_iters > 0 ? st.total_iters : 1; st.status = S_DONE; return; }
+        if (rn < st.rbest) {
+            st.rbest = rn;
+            st.stagnant = 0;
+        } else if (++st.stagnant >= 8 || st.refinements >= bp.max_refine) {
             st.info = st.total_iters > 0 ? st.total_iters : 1;
             st.status = S_DONE;
             return;
         }
-        st.refinements += 1;
-        st.atol_in = bp.inner_rtol * rn;
-        st.rr = tot[1];
-        st.rho = tot[1];
-        st.iters = 0;
-        st.info = 0;
-        if (fabs(st.rho) < bp.rhotol) { st.info = KCFD_INFO_RHO_BREAKDOWN; st.status = S_DONE; return; }
-        st.status = S_START;
+        st.rmax = sqrt(tot[2]);
+        st.rr = tot[2];
+        if (st.restart) {
+            st.rho = tot[1];
+            st.iters = 0;
+            st.restart = 0;
+            if (fabs(st.rho) < bp.rhotol) { st.info = KCFD_INFO_RHO_BREAKDOWN; st.status = S_DONE; return; }
+            st.status = S_START;
+        } else {
+            const double rho = tot[1];
+            if (fabs(rho) < bp.rhotol || fabs(st.omega) < bp.rhotol) {
+                st.restart = 1;        // redo the replacement with a fresh shadow residual
+                st.status = S_REFINE;
+                return;
+            }
+            st.beta = (rho / st.rho) * (st.alpha / st.omega);
+            st.rho_prev = st.rho;
+            st.rho = rho;
+            st.status = S_RESUME;
+        }
+    } else if (status == S_ZEROX) {
+        if (threadIdx.x == 0) { st.info = 0; st.status = S_DONE; }
     }
 }
 
 // -------------------------------------------------------------------- K3
-// s = r - alpha v; ||s|| -> half-step exit.  Extra last block resets the
-// counters of the second pair of triangular solves.
+// s = r - alpha v; ||s||.  Block 0 clears the counters of the s solves.
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
-                                              SysState *__restrict__ sts,
+                                              const SysState *__restrict__ sts,
                                               const int2 *__restrict__ tiles, int ntiles,
                                               double *__restrict__ partials,
                                               unsigned int *__restrict__ counters) {
 #pragma clang fp contract(off)
-    if (blockIdx.x == gridDim.x - 1) {
-        if (threadIdx.x < 2) counters[2 + threadIdx.x] = 0u;
-        return;
+    if (blockIdx.x == 0 && threadIdx.x < 2) counters[2 + threadIdx.x] = 0u;
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const SysState &st = sts[ti.x];
+        if (st.status != S_RUN) continue;
+        const int i = ti.y + threadIdx.x;
+        double v[1] = {0.0};
+        if (i < d.n) {
+            const V al = (V)st.alpha;
+            const V si = ((const V *)d.r)[i] - al * ((const V *)d.v)[i];
+            ((V *)d.s)[i] = si;
+            v[0] = (double)si * (double)si;
+        }
+        write_partials<1>(v, partials, tile);
     }
-    const int tile = xcd_tile(blockIdx.x, gridDim.x - 1);
-    if (tile >= ntiles) return;
-    const int2 ti = tiles[tile];
-    const SysDesc &d = sys[ti.x];
-    SysState &st = sts[ti.x];
+}
+
+__global__ __launch_bounds__(kBlock) void f_s(const SysDesc *__restrict__ sys, SysState *__restrict__ sts,
+                                              const double *__restrict__ partials, BatchParams bp) {
+    const SysDesc &d = sys[blockIdx.x];
+    SysState &st = sts[blockIdx.x];
     if (st.status != S_RUN) return;
-    const int i = ti.y + threadIdx.x;
-    double v[1] = {0.0};
-    if (i < d.n) {
-        const V al = (V)st.alpha;
-        const V si = ((const V *)d.r)[i] - al * ((const V *)d.v)[i];
-        ((V *)d.s)[i] = si;
-        v[0] = (double)si * (double)si;
-    }
     double tot[1];
-    if (!reduce_system<1>(v, d, st, partials, tile, tot)) return;
+    sum_partials<1>(d, partials, tot);
     if (threadIdx.x != 0) return;
     if (sqrt(tot[0]) < st.atol_in) st.status = S_HALF;
 }
 
 // -------------------------------------------------------------------- K4
-// t = A M s; t.s, t.t -> omega.   S_HALF: x += alpha M p, then finish.
+// t = A M s; t.s, t.t.   S_HALF: x += alpha M p.
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ sys,
-                                                   SysState *__restrict__ sts,
+                                                   const SysState *__restrict__ sts,
                                                    const int2 *__restrict__ tiles, int ntiles,
                                                    double *__restrict__ partials, BatchParams bp) {
 #pragma clang fp contract(off)
-    const int tile = tile_of_block();
-    if (tile >= ntiles) return;
-    const int2 ti = tiles[tile];
-    const SysDesc &d = sys[ti.x];
-    SysState &st = sts[ti.x];
-    const int status = st.status;
-    const int i = ti.y + threadIdx.x;
-    if (status == S_RUN) {
-        double v[2] = {0.0, 0.0};
-        if (i < d.n) {
-            const double ti_ = row_amz<V, PC>(d, i, (const V *)d.s, (const V *)d.sh);
-            const V tv = (V)ti_;
-            ((V *)d.t)[i] = tv;
-            const double si = (double)((const V *)d.s)[i];
-            v[0] = (double)tv * si;
-            v[1] = (double)tv * (double)tv;
-        }
-        double tot[2];
-        if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
-        if (threadIdx.x != 0) return;
-        st.omega = tot[0] / tot[1];
-        return;
-    }
-    if (status == S_HALF) {
-        if (i < d.n) {
-            const V al = (V)st.alpha;
-            V *__restrict__ xx = bp.precision == KCFD_FP64 ? (V *)d.x : (V *)d.d;
-            const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
-            xx[i] = xx[i] + al * ph;
-            if (PC == KCFD_PC_ILU0) {
-                ((V *)d.ph)[i] = sentinel_value<V>();
-                ((V *)d.sh)[i] = sentinel_value<V>();
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const SysState &st = sts[ti.x];
+        const int status = st.status;
+        const int i = ti.y + threadIdx.x;
+        if (status == S_RUN) {
+            double v[2] = {0.0, 0.0};
+            if (i < d.n) {
+                const V tv = (V)row_amz<V, PC>(d, i, (const V *)d.s, (const V *)d.sh);
+                ((V *)d.t)[i] = tv;
+                v[0] = (double)tv * (double)((const V *)d.s)[i];
+                v[1] = (double)tv * (double)tv;
+            }
+            write_partials<2>(v, partials, tile);
+        } else if (status == S_HALF) {
+            if (i < d.n) {
+                const V al = (V)st.alpha;
+                V *__restrict__ xx = bp.precision == KCFD_FP64 ? (V *)d.x : (V *)d.d;
+                const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
+                xx[i] = xx[i] + al * ph;
+                if (PC == KCFD_PC_ILU0) {
+                    ((V *)d.ph)[i] = sentinel_value<V>();
+                    ((V *)d.sh)[i] = sentinel_value<V>();
+                }
             }
         }
-        if (last_block(d, st) && threadIdx.x == 0) {
-            st.half = 1;
-            st.total_iters += 0;
-            st.status = bp.precision == KCFD_MIXED ? S_REFINE : S_DONE;
-            st.info = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void f_spmv_s(const SysDesc *__restrict__ sys,
+                                                   SysState *__restrict__ sts,
+                                                   const double *__restrict__ partials, BatchParams bp) {
+    const SysDesc &d = sys[blockIdx.x];
+    SysState &st = sts[blockIdx.x];
+    const int status = st.status;
+    if (status == S_RUN) {
+        double tot[2];
+        sum_partials<2>(d, partials, tot);
+        if (threadIdx.x != 0) return;
+        st.omega = tot[0] / tot[1];
+    } else if (status == S_HALF && threadIdx.x == 0) {
+        st.half = 1;
+        st.info = 0;
+        if (bp.precision == KCFD_MIXED) {
+            st.restart = 1;
+            st.status = S_REFINE;
+        } else {
+            st.status = S_DONE;
         }
     }
 }
 
 // -------------------------------------------------------------------- K5
-// x += alpha M p; x += omega M s; r = s - omega t; ||r||^2, rt.r -> checks, beta
+// x += alpha M p; x += omega M s; r = s - omega t; ||r||^2, rt.r
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
-                                               SysState *__restrict__ sts,
+                                               const SysState *__restrict__ sts,
                                                const int2 *__restrict__ tiles, int ntiles,
                                                double *__restrict__ partials, BatchParams bp) {
 #pragma clang fp contract(off)
-    const int tile = tile_of_block();
-    if (tile >= ntiles) return;
-    const int2 ti = tiles[tile];
-    const SysDesc &d = sys[ti.x];
-    SysState &st = sts[ti.x];
-    if (st.status != S_RUN) return;
-    const int i = ti.y + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (i < d.n) {
-        const V al = (V)st.alpha, om = (V)st.omega;
-        V *__restrict__ xx = bp.precision == KCFD_FP64 ? (V *)d.x : (V *)d.d;
-        const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
-        const V sh = (V)precond_at<V, PC>(d, (const V *)d.s, (const V *)d.sh, i);
-        V xi = xx[i] + al * ph;
-        xi = xi + om * sh;
-        xx[i] = xi;
-        const V ri = ((const V *)d.s)[i] - om * ((const V *)d.t)[i];
-        ((V *)d.r)[i] = ri;
-        v[0] = (double)ri * (double)ri;
-        v[1] = (double)((const V *)d.rt)[i] * (double)ri;
-        if (PC == KCFD_PC_ILU0) {
-            ((V *)d.ph)[i] = sentinel_value<V>();
-            ((V *)d.sh)[i] = sentinel_value<V>();
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const SysState &st = sts[ti.x];
+        if (st.status != S_RUN) continue;
+        const int i = ti.y + threadIdx.x;
+        double v[2] = {0.0, 0.0};
+        if (i < d.n) {
+            const V al = (V)st.alpha, om = (V)st.omega;
+            V *__restrict__ xx = bp.precision == KCFD_FP64 ? (V *)d.x : (V *)d.d;
+            const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
+            const V sh = (V)precond_at<V, PC>(d, (const V *)d.s, (const V *)d.sh, i);
+            V xi = xx[i] + al * ph;
+            xi = xi + om * sh;
+            xx[i] = xi;
+            const V ri = ((const V *)d.s)[i] - om * ((const V *)d.t)[i];
+            ((V *)d.r)[i] = ri;
+            v[0] = (double)ri * (double)ri;
+            v[1] = (double)((const V *)d.rt)[i] * (double)ri;
+            if (PC == KCFD_PC_ILU0) {
+                ((V *)d.ph)[i] = sentinel_value<V>();
+                ((V *)d.sh)[i] = sentinel_value<V>();
+            }
         }
+        write_partials<2>(v, partials, tile);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void f_xr(const SysDesc *__restrict__ sys, SysState *__restrict__ sts,
+                                               const double *__restrict__ partials, BatchParams bp) {
+    const SysDesc &d = sys[blockIdx.x];
+    SysState &st = sts[blockIdx.x];
+    if (st.status != S_RUN) return;
     double tot[2];
-    if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
+    sum_partials<2>(d, partials, tot);
     if (threadIdx.x != 0) return;
     st.iters += 1;
     st.total_iters += 1;
     st.rr = tot[0];
-    const bool mixed = bp.precision == KCFD_MIXED;
-    const int next_exit = mixed ? S_REFINE : S_DONE;
-    if (st.total_iters >= st.maxiter) {          // scipy's for-loop exhausted
-        st.info = st.total_iters;
-        st.status = mixed ? S_REFINE : S_DONE;
-        if (mixed) st.maxiter = st.total_iters;  // REFINE will stop after the update
-        return;
-    }
-    if (sqrt(tot[0]) < st.atol_in) { st.info = 0; st.status = next_exit; return; }
-    if (mixed && st.iters >= bp.inner_maxiter) { st.status = S_REFINE; return; }
+    const double rn = sqrt(tot[0]);
     const double rho = tot[1];
-    if (fabs(rho) < bp.rhotol) {
-        st.info = KCFD_INFO_RHO_BREAKDOWN;
-        st.status = next_exit;
-        return;
-    }
-    if (fabs(st.omega) < bp.rhotol) {
-        st.info = KCFD_INFO_OMEGA_BREAKDOWN;
-        st.status = next_exit;
-        return;
+    if (bp.precision == KCFD_FP64) {
+        if (st.total_iters >= st.maxiter) {          // scipy's for-loop exhausted
+            st.info = st.total_iters;
+            st.status = S_DONE;
+            return;
+        }
+        if (rn < st.atol_in) { st.info = 0; st.status = S_DONE; return; }
+        if (fabs(rho) < bp.rhotol) { st.info = KCFD_INFO_RHO_BREAKDOWN; st.status = S_DONE; return; }
+        if (fabs(st.omega) < bp.rhotol) { st.info = KCFD_INFO_OMEGA_BREAKDOWN; st.status = S_DONE; return; }
+    } else {
+        if (st.total_iters >= st.maxiter) { st.final_ = 1; st.status = S_REFINE; return; }
+        // reliable update when the FP32 residual claims convergence or has
+        // dropped by delta since the last update; restart on divergence,
+        // breakdown or the inner iteration cap
+        if (!(rn == rn) || rn > 1e6 * st.rmax || fabs(rho) < bp.rhotol ||
+            fabs(st.omega) < bp.rhotol || st.iters >= bp.inner_maxiter) {
+            st.restart = 1;
+            st.status = S_REFINE;
+            return;
+        }
+        if (rn < st.atol_out || rn < bp.delta * st.rmax) { st.status = S_REFINE; return; }
+        st.rmax = fmax(st.rmax, rn);
     }
     st.beta = (rho / st.rho) * (st.alpha / st.omega);
     st.rho_prev = st.rho;
@@ -544,29 +582,33 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
 }
 
 // ------------------------------------------------------- true residual
-// ||b - A x||^2 and ||b||^2 in FP64 on the FP64 matrix, for every system.
+// ||b - A x||^2 and ||b||^2 in FP64 on the FP64 matrix, every system.
 __global__ __launch_bounds__(kBlock) void k_true_resid(const SysDesc *__restrict__ sys,
-                                                       SysState *__restrict__ sts,
                                                        const int2 *__restrict__ tiles, int ntiles,
                                                        double *__restrict__ partials) {
 #pragma clang fp contract(off)
-    const int tile = tile_of_block();
-    if (tile >= ntiles) return;
-    const int2 ti = tiles[tile];
-    const SysDesc &d = sys[ti.x];
-    SysState &st = sts[ti.x];
-    const int i = ti.y + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (i < d.n) {
-        double acc = 0.0;
-        for (int q = d.ip[i]; q < d.ip[i + 1]; ++q) acc += d.a64[q] * d.x[d.ix[q]];
-        const double bi = d.b[i];
-        const double ri = bi - acc;
-        v[0] = ri * ri;
-        v[1] = bi * bi;
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const int i = ti.y + threadIdx.x;
+        double v[2] = {0.0, 0.0};
+        if (i < d.n) {
+            const double bi = d.b[i];
+            const double ri = bi - row_ax64(d, i, d.x);
+            v[0] = ri * ri;
+            v[1] = bi * bi;
+        }
+        write_partials<2>(v, partials, tile);
     }
+}
+
+__global__ __launch_bounds__(kBlock) void f_true_resid(const SysDesc *__restrict__ sys,
+                                                       SysState *__restrict__ sts,
+                                                       const double *__restrict__ partials) {
+    const SysDesc &d = sys[blockIdx.x];
+    SysState &st = sts[blockIdx.x];
     double tot[2];
-    if (!reduce_system<2>(v, d, st, partials, tile, tot)) return;
+    sum_partials<2>(d, partials, tot);
     if (threadIdx.x != 0) return;
     st.true_rr = tot[0];
     st.true_bb = tot[1];
@@ -624,9 +666,10 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
         const int status = sts[ch.sys].status;
         // the p solves (after K1) serve systems in iteration 0 or later; the s
         // solves (after K3) only systems still running -- a system that the
-        // FP64 refinement just restarted (S_START set inside K2) must not run
-        // them, or stale values would be left in sh
-        const bool want = WHICH == 0 ? (status == S_RUN || status == S_START) : (status == S_RUN);
+        // refinement just restarted (S_START set inside F2) must not run them,
+        // or stale values would be left in sh
+        const bool want = WHICH == 0 ? (status == S_RUN || status == S_START || status == S_RESUME)
+                                     : (status == S_RUN);
         const bool live = want && lane < ch.count;
         if (live) {
             const int i = lvl_rows[ch.sys][ch.start + lane];
@@ -692,10 +735,10 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 template <typename V, int PC>
 void launch_iteration(kcfd_batch *b, hipStream_t s) {
-    const int g = b->grid;
+    const int g = b->grid, ns = b->nsys;
     const dim3 blk(kBlock);
-    hipLaunchKernelGGL((k_p<V, PC>), dim3(g + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                       b->ntiles, b->d_counters);
+    hipLaunchKernelGGL((k_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles,
+                       b->d_counters);
     if (PC == KCFD_PC_ILU0) {
         hipLaunchKernelGGL((k_trsv<V, false, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
                            b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters + 0);
@@ -703,9 +746,11 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
                            b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u, b->d_counters + 1);
     }
     hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                       b->ntiles, b->d_partials, b->bp);
-    hipLaunchKernelGGL((k_s<V, PC>), dim3(g + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                       b->ntiles, b->d_partials, b->d_counters);
+                       b->ntiles, b->d_partials);
+    hipLaunchKernelGGL(f_spmv_p, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->bp);
+    hipLaunchKernelGGL((k_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles,
+                       b->d_partials, b->d_counters);
+    hipLaunchKernelGGL(f_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->bp);
     if (PC == KCFD_PC_ILU0) {
         hipLaunchKernelGGL((k_trsv<V, false, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
                            b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters + 2);
@@ -714,14 +759,18 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     }
     hipLaunchKernelGGL((k_spmv_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->ntiles, b->d_partials, b->bp);
+    hipLaunchKernelGGL(f_spmv_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->bp);
     hipLaunchKernelGGL((k_xr<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->ntiles, b->d_partials, b->bp);
+    hipLaunchKernelGGL(f_xr, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->bp);
 }
 
 template <typename V, int PC>
 void launch_init(kcfd_batch *b, hipStream_t s) {
     hipLaunchKernelGGL((k_init<V, PC>), dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
                        b->d_tiles, b->ntiles, b->d_partials, b->bp);
+    hipLaunchKernelGGL(f_init, dim3(b->nsys), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                       b->d_partials, b->bp);
 }
 
 using LaunchFn = void (*)(kcfd_batch *, hipStream_t);
@@ -748,7 +797,9 @@ LaunchFn pick_init(int precision, int pc) {
     return launch_init<float, KCFD_PC_NONE>;
 }
 
-// algorithmic bytes (roofline model): CSR stream, ILU factor stream
+// algorithmic bytes (roofline model): CSR stream M_A, ILU factor stream M_F,
+// vector streams of the fused schedule; MIXED adds one FP64 residual
+// replacement (M_A at 8 bytes + 6 n*8) per reliable update
 double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st) {
     const double n = m->n, nnz = m->nnz;
     auto MA = [&](double sv) { return nnz * (sv + 4) + (n + 1) * 4; };
@@ -758,10 +809,22 @@ double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st)
         if (pc == KCFD_PC_ILU0) return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv;
         return 2 * MA(sv) + 19 * n * sv;
     };
-    if (precision == KCFD_FP64) {
-        return (st.total_iters + 0.5 * st.half) * per_iter(8) + MA(8) + 3 * n * 8;
-    }
-    return st.total_iters * per_iter(4) + (st.refinements + 1) * (MA(8) + 6 * n * 8);
+    if (precision == KCFD_FP64) return (st.total_iters + 0.5 * st.half) * per_iter(8) + MA(8) + 3 * n * 8;
+    return st.total_iters * per_iter(4) + st.refinements * (MA(8) + 6 * n * 8);
+}
+
+void fill_desc(SysDesc &d, kcfd_mat *m) {
+    d.ip = m->indptr;
+    d.ix = m->indices;
+    d.dpos = m->diagpos;
+    d.sptr = m->slice_ptr;
+    d.six = m->six;
+    d.sa64 = m->sa64;
+    d.sa32 = m->sa32;
+    d.dinv64 = m->dinv;
+    d.dinv32 = m->dinv32;
+    d.lu64 = m->lu;
+    d.lu32 = m->lu32;
 }
 
 int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *const *bs,
@@ -777,20 +840,19 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
     if (b->opts.check_every <= 0) b->opts.check_every = 8;
     b->bp.rtol = o->rtol;
     b->bp.atol = o->atol;
-    b->bp.inner_rtol = o->inner_rtol > 0 ? o->inner_rtol : 1e-4;
+    b->bp.delta = o->inner_rtol > 0 ? o->inner_rtol : 0.1;
     const double eps = 2.220446049250313e-16;
     b->bp.rhotol = eps * eps;
     b->bp.precision = prec;
     b->bp.precond = pc;
-    b->bp.max_refine = o->max_refine > 0 ? o->max_refine : 30;
+    b->bp.max_refine = o->max_refine > 0 ? o->max_refine : 1000;
     b->bp.nsys = nsys;
-    b->bp.inner_maxiter = 2000;
+    b->bp.inner_maxiter = 5000;
     b->mats.assign(mats, mats + nsys);
     auto fail = [&](int rc) {
         kcfd_batch_destroy(b);
         return rc;
     };
-    // tiles
     std::vector<int2> tiles;
     b->h_desc.resize(nsys);
     const size_t vs = prec == KCFD_FP64 ? sizeof(double) : sizeof(float);
@@ -816,15 +878,7 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
         const int nt = std::max(1, (m->n + kBlock - 1) / kBlock);
         d.ntiles = nt;
         for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(k, t * kBlock));
-        d.ip = m->indptr;
-        d.ix = m->indices;
-        d.dpos = m->diagpos;
-        d.a64 = m->val;
-        d.a32 = m->val32;
-        d.dinv64 = m->dinv;
-        d.dinv32 = m->dinv32;
-        d.lu64 = m->lu;
-        d.lu32 = m->lu32;
+        fill_desc(d, m);
         d.b = bs[k];
         d.x = xs[k];
         off[k] = slab;
@@ -832,7 +886,9 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
         b->sys_n.push_back(m->n);
     }
     b->ntiles = (int)tiles.size();
-    b->grid = xcd_grid(b->ntiles);
+    // persistent grid: at most 4 workgroups per CU, a multiple of the XCD count
+    const int cap = std::max(kXcds, (ctx->cus * 4 / kXcds) * kXcds);
+    b->grid = std::min(xcd_grid(b->ntiles), cap);
     KCFD_HIP(hipMalloc(&b->slab, slab ? slab : 256));
     b->slab_bytes = slab;
     for (int k = 0; k < nsys; ++k) {
@@ -912,8 +968,8 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
             widest = std::max(widest, w);
         }
         const int want_waves = std::max(64, 3 * widest);
-        const int cap = ctx->cus * 8;
-        b->trsv_grid = std::max(1, std::min((want_waves + kWaves - 1) / kWaves, cap));
+        const int tcap = ctx->cus * 8;
+        b->trsv_grid = std::max(1, std::min((want_waves + kWaves - 1) / kWaves, tcap));
     }
     *out = b;
     return KCFD_OK;
@@ -947,13 +1003,11 @@ int capture_graph(kcfd_batch *b, int iters) {
 int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     kcfd_ctx *ctx = b->ctx;
     hipStream_t s = ctx->stream;
-    // reset states
     for (int k = 0; k < b->nsys; ++k) {
         SysState st{};
         const int n = b->sys_n[k];
         st.status = S_START;
         st.maxiter = b->opts.maxiter > 0 ? b->opts.maxiter : 10 * std::max(n, 1);
-        st.ticket = 0;
         b->h_state[k] = st;
     }
     KCFD_HIP(hipMemcpyAsync(b->d_state, b->h_state.data(), sizeof(SysState) * b->nsys,
@@ -963,12 +1017,12 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     KCFD_HIP(hipEventRecord(ctx->ev0, s));
     pick_init(b->bp.precision, b->bp.precond)(b, s);
     KCFD_HIP(hipGetLastError());
-    // host loop: replay the captured iterations until every system is done
     long long guard = 0;
     int maxit = 0;
     for (int k = 0; k < b->nsys; ++k) maxit = std::max(maxit, b->h_state[k].maxiter);
-    const long long limit = (long long)maxit * (b->bp.precision == KCFD_MIXED ? 2 : 1) +
-                            (long long)b->bp.max_refine * 2 + 64;
+    // every graph iteration advances each running system by one iteration or
+    // one reliable update
+    const long long limit = 2LL * maxit + 2LL * b->bp.max_refine + 64;
     for (;;) {
         KCFD_HIP(hipGraphLaunch(b->graph, s));
         guard += b->graph_iters;
@@ -985,8 +1039,10 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
         }
     }
     KCFD_HIP(hipEventRecord(ctx->ev1, s));
-    hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                       b->d_tiles, b->ntiles, b->d_partials);
+    hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_tiles,
+                       b->ntiles, b->d_partials);
+    hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                       b->d_partials);
     KCFD_HIP(hipGetLastError());
     KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
                             hipMemcpyDeviceToHost, s));
@@ -1008,6 +1064,34 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
         }
     }
     return KCFD_OK;
+}
+
+template <typename V, int PC>
+void launch_phase(kcfd_batch *b, int phase, hipStream_t s) {
+    const dim3 blk(kBlock);
+    if (phase == 1)
+        hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state,
+                           b->d_tiles, b->ntiles, b->d_partials);
+    else if (phase == 2)
+        hipLaunchKernelGGL((k_xr<V, PC>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
+                           b->ntiles, b->d_partials, b->bp);
+    else if (phase == 4)
+        hipLaunchKernelGGL((k_p<V, PC>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
+                           b->ntiles, b->d_counters);
+}
+
+void launch_phase_any(kcfd_batch *b, int phase, hipStream_t s) {
+    const bool f64 = b->bp.precision == KCFD_FP64;
+    switch (b->bp.precond) {
+        case KCFD_PC_JACOBI:
+            f64 ? launch_phase<double, KCFD_PC_JACOBI>(b, phase, s) : launch_phase<float, KCFD_PC_JACOBI>(b, phase, s);
+            break;
+        case KCFD_PC_ILU0:
+            f64 ? launch_phase<double, KCFD_PC_ILU0>(b, phase, s) : launch_phase<float, KCFD_PC_ILU0>(b, phase, s);
+            break;
+        default:
+            f64 ? launch_phase<double, KCFD_PC_NONE>(b, phase, s) : launch_phase<float, KCFD_PC_NONE>(b, phase, s);
+    }
 }
 
 }  // namespace
@@ -1071,8 +1155,9 @@ int kcfd_batch_bytes(kcfd_batch *b, long long *bytes) {
 
 // Re-launch one phase kernel `reps` times with every system forced to the
 // running state, bracketed by HIP events on the context stream; restores the
-// states afterwards.  phase: 1 = K2 (v = A M p + rt.v), 2 = K5 (x/r update),
-// 3 = lower triangular solve (ILU only), 4 = K1 (p update).
+// states afterwards.  phase: 1 = K2 (v = A M p + rt.v partials), 2 = K5 (x/r
+// update), 3 = lower triangular solve (ILU only), 4 = K1 (p update).  No
+// finalize kernel runs, so the state stays S_RUN for every repetition.
 int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     KCFD_ARG(b && avg_ms && reps > 0, "bad argument");
     KCFD_ARG(phase >= 1 && phase <= 4, "unknown phase");
@@ -1086,59 +1171,21 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     for (int k = 0; k < b->nsys; ++k) {
         run[k] = saved[k];
         run[k].status = S_RUN;
-        run[k].ticket = 0;
         if (run[k].omega == 0.0) run[k].omega = 1.0;
-        if (run[k].rho == 0.0) run[k].rho = 1.0;
     }
     KCFD_HIP(hipMemcpy(b->d_state, run.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice));
     const bool f64 = b->bp.precision == KCFD_FP64;
-    const int pc = b->bp.precond;
-    auto launch = [&]() {
-        const dim3 blk(kBlock);
-        if (phase == 1) {
-            if (f64) {
-                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_spmv_p<double, KCFD_PC_JACOBI>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_spmv_p<double, KCFD_PC_ILU0>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-                else hipLaunchKernelGGL((k_spmv_p<double, KCFD_PC_NONE>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-            } else {
-                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_spmv_p<float, KCFD_PC_JACOBI>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_spmv_p<float, KCFD_PC_ILU0>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-                else hipLaunchKernelGGL((k_spmv_p<float, KCFD_PC_NONE>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-            }
-        } else if (phase == 2) {
-            if (f64) {
-                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_xr<double, KCFD_PC_JACOBI>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_xr<double, KCFD_PC_ILU0>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-                else hipLaunchKernelGGL((k_xr<double, KCFD_PC_NONE>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-            } else {
-                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_xr<float, KCFD_PC_JACOBI>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_xr<float, KCFD_PC_ILU0>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-                else hipLaunchKernelGGL((k_xr<float, KCFD_PC_NONE>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_partials, b->bp);
-            }
-        } else if (phase == 4) {
-            if (f64) {
-                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_p<double, KCFD_PC_JACOBI>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
-                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_p<double, KCFD_PC_ILU0>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
-                else hipLaunchKernelGGL((k_p<double, KCFD_PC_NONE>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
-            } else {
-                if (pc == KCFD_PC_JACOBI) hipLaunchKernelGGL((k_p<float, KCFD_PC_JACOBI>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
-                else if (pc == KCFD_PC_ILU0) hipLaunchKernelGGL((k_p<float, KCFD_PC_ILU0>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
-                else hipLaunchKernelGGL((k_p<float, KCFD_PC_NONE>), dim3(b->grid + 1), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles, b->d_counters);
-            }
-        }
-    };
     if (phase == 3) {
-        // the L solve needs a fresh sentinel-filled output and a zero counter
-        // before every launch; time only the solves
+        // the L solve needs a sentinel-filled output and a zero counter before
+        // every launch; only the solves are timed
         float tot = 0.f;
         for (int r = 0; r < reps; ++r) {
             KCFD_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned) * 8, s));
             for (int k = 0; k < b->nsys; ++k) {
                 const SysDesc &d = b->h_desc[k];
-                const size_t es = f64 ? 8 : 4;
-                if (f64) hipLaunchKernelGGL(k_fill_sentinel<double>, dim3(std::max(1, (d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, (double *)d.y, d.n);
-                else hipLaunchKernelGGL(k_fill_sentinel<float>, dim3(std::max(1, (d.n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, (float *)d.y, d.n);
-                (void)es;
+                const unsigned nb = (unsigned)std::max(1, (d.n + kBlock - 1) / kBlock);
+                if (f64) hipLaunchKernelGGL(k_fill_sentinel<double>, dim3(nb), dim3(kBlock), 0, s, (double *)d.y, d.n);
+                else hipLaunchKernelGGL(k_fill_sentinel<float>, dim3(nb), dim3(kBlock), 0, s, (float *)d.y, d.n);
             }
             KCFD_HIP(hipEventRecord(b->ctx->ev0, s));
             if (f64) hipLaunchKernelGGL((k_trsv<double, false, 0>), dim3(b->trsv_grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters);
@@ -1151,9 +1198,9 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
         }
         *avg_ms = tot / reps;
     } else {
-        launch();  // warm
+        launch_phase_any(b, phase, s);  // warm
         KCFD_HIP(hipEventRecord(b->ctx->ev0, s));
-        for (int r = 0; r < reps; ++r) launch();
+        for (int r = 0; r < reps; ++r) launch_phase_any(b, phase, s);
         KCFD_HIP(hipEventRecord(b->ctx->ev1, s));
         KCFD_HIP(hipEventSynchronize(b->ctx->ev1));
         float ms = 0.f;
@@ -1162,7 +1209,6 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     }
     KCFD_HIP(hipGetLastError());
     KCFD_HIP(hipMemcpy(b->d_state, saved.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice));
-    // ILU buffers back to the sentinel state a new solve expects
     return KCFD_OK;
 }
 
@@ -1193,11 +1239,11 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
         if (res) std::memset(res, 0, sizeof(*res));
         return KCFD_OK;
     }
-    // the cached single-system workspace is keyed on (precond, precision, b, x)
-    const int key[4] = {opts->precond, opts->precision, (int)((uintptr_t)bvec & 0x7fffffff),
-                        (int)((uintptr_t)x & 0x7fffffff)};
-    bool reuse = m->solver != nullptr && std::memcmp(key, m->solver_key, sizeof(key)) == 0 &&
-                 m->solver->h_desc[0].b == bvec && m->solver->h_desc[0].x == x;
+    // the cached single-system workspace is reused while the preconditioner,
+    // precision and the b / x buffers are unchanged
+    bool reuse = m->solver != nullptr && m->solver->bp.precond == opts->precond &&
+                 m->solver->bp.precision == opts->precision && m->solver->h_desc[0].b == bvec &&
+                 m->solver->h_desc[0].x == x;
     if (!reuse) {
         if (m->solver) kcfd_batch_destroy(m->solver);
         m->solver = nullptr;
@@ -1205,20 +1251,16 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
         const double *bb[1] = {bvec};
         double *xx[1] = {x};
         KCFD_CHECK(build_batch(m->ctx, 1, mm, bb, xx, opts, &m->solver));
-        std::memcpy(m->solver_key, key, sizeof(key));
     } else {
-        // refresh option-dependent parameters and the preconditioner pointers
         kcfd_batch *b = m->solver;
         b->opts = *opts;
         if (b->opts.check_every <= 0) b->opts.check_every = 8;
         b->bp.rtol = opts->rtol;
         b->bp.atol = opts->atol;
-        b->bp.inner_rtol = opts->inner_rtol > 0 ? opts->inner_rtol : 1e-4;
-        b->bp.max_refine = opts->max_refine > 0 ? opts->max_refine : 30;
+        b->bp.delta = opts->inner_rtol > 0 ? opts->inner_rtol : 0.1;
+        b->bp.max_refine = opts->max_refine > 0 ? opts->max_refine : 1000;
+        // kernel parameters are baked into the captured graph: recapture
         if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
-        SysDesc &d = b->h_desc[0];
-        d.a32 = m->val32; d.dinv64 = m->dinv; d.dinv32 = m->dinv32; d.lu64 = m->lu; d.lu32 = m->lu32;
-        KCFD_HIP(hipMemcpy(b->d_desc, b->h_desc.data(), sizeof(SysDesc), hipMemcpyHostToDevice));
     }
     return run_batch(m->solver, res);
 }

@@ -109,10 +109,18 @@ struct kcfd_mat {
     int *indptr = nullptr, *indices = nullptr, *diagpos = nullptr;
     double *val = nullptr, *rhs = nullptr, *diagt = nullptr;
     std::vector<int> h_indptr, h_indices, h_diagpos;
+    // SELL-64 copy of A used by every solver SpMV: slice s holds rows
+    // 64s..64s+63, column-major, width = longest row of the slice; a row's
+    // entries keep their CSR order and padding (column = row, value 0) comes
+    // last, so its sum equals the sequential CSR row sum.
+    int nslices = 0, sell_nnz = 0;
+    std::vector<int> h_slice_ptr;
+    int *slice_ptr = nullptr, *six = nullptr;
+    double *sa64 = nullptr;
+    float *sa32 = nullptr;
     // preconditioner
     int pc_kind = KCFD_PC_NONE;
     bool have32 = false;
-    float *val32 = nullptr;
     double *dinv = nullptr;
     float *dinv32 = nullptr;
     // ILU(0): factors on A's pattern, level schedule of the L and U solves

@@ -47,6 +47,14 @@ __global__ void k_f64_to_f32(size_t n, const double *__restrict__ a, float *__re
     for (; i < n; i += st) b[i] = (float)a[i];
 }
 
+__global__ void k_pivot_check(int n, const int *__restrict__ dpos, const double *__restrict__ lu,
+                              int *__restrict__ bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double u = lu[dpos[i]];
+    if (u == 0.0 || !(u == u)) atomicMin(bad, i);
+}
+
 // level lists: rows sorted by level (stable: increasing row id inside a
 // level for L, decreasing for U), with level offsets
 static void level_schedule(const std::vector<int> &ip, const std::vector<int> &ix,
@@ -92,20 +100,22 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
             return KCFD_EPIVOT;
         }
     }
-    std::vector<int> rl, pl, ru, pu;
-    level_schedule(m->h_indptr, m->h_indices, m->h_diagpos, false, rl, pl);
-    level_schedule(m->h_indptr, m->h_indices, m->h_diagpos, true, ru, pu);
-    m->nlev_l = (int)pl.size() - 1;
-    m->nlev_u = (int)pu.size() - 1;
-    m->h_lvl_ptr_l = pl;
-    m->h_lvl_ptr_u = pu;
-    KCFD_HIP(hipMalloc(&m->lvl_rows_l, sizeof(int) * std::max(n, 1)));
-    KCFD_HIP(hipMalloc(&m->lvl_rows_u, sizeof(int) * std::max(n, 1)));
-    if (n) {
-        KCFD_HIP(hipMemcpy(m->lvl_rows_l, rl.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-        KCFD_HIP(hipMemcpy(m->lvl_rows_u, ru.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    // the level schedule depends on the structure only: built once
+    if (!m->lvl_rows_l) {
+        std::vector<int> rl, ru;
+        level_schedule(m->h_indptr, m->h_indices, m->h_diagpos, false, rl, m->h_lvl_ptr_l);
+        level_schedule(m->h_indptr, m->h_indices, m->h_diagpos, true, ru, m->h_lvl_ptr_u);
+        m->nlev_l = (int)m->h_lvl_ptr_l.size() - 1;
+        m->nlev_u = (int)m->h_lvl_ptr_u.size() - 1;
+        KCFD_HIP(hipMalloc(&m->lvl_rows_l, sizeof(int) * std::max(n, 1)));
+        KCFD_HIP(hipMalloc(&m->lvl_rows_u, sizeof(int) * std::max(n, 1)));
+        if (n) {
+            KCFD_HIP(hipMemcpy(m->lvl_rows_l, rl.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+            KCFD_HIP(hipMemcpy(m->lvl_rows_u, ru.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+        }
     }
-    KCFD_HIP(hipMalloc(&m->lu, sizeof(double) * std::max(m->nnz, 1)));
+    const std::vector<int> &pl = m->h_lvl_ptr_l;
+    if (!m->lu) KCFD_HIP(hipMalloc(&m->lu, sizeof(double) * std::max(m->nnz, 1)));
     if (m->nnz)
         KCFD_HIP(hipMemcpyAsync(m->lu, m->val, sizeof(double) * m->nnz, hipMemcpyDeviceToDevice, s));
     for (int l = 0; l < m->nlev_l; ++l) {
@@ -115,28 +125,27 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
     }
     KCFD_HIP(hipGetLastError());
     if (want32 && m->nnz) {
-        KCFD_HIP(hipMalloc(&m->lu32, sizeof(float) * m->nnz));
+        if (!m->lu32) KCFD_HIP(hipMalloc(&m->lu32, sizeof(float) * m->nnz));
         size_t nb = ((size_t)m->nnz + kBlock - 1) / kBlock;
         if (nb > 65536) nb = 65536;
         hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)nb), dim3(kBlock), 0, s, (size_t)m->nnz,
                            m->lu, m->lu32);
         KCFD_HIP(hipGetLastError());
     }
-    // pivot check on the host copy of the diagonal of U
-    std::vector<double> diag(n);
-    {
-        std::vector<double> luh(m->nnz);
-        if (m->nnz)
-            KCFD_HIP(hipMemcpyAsync(luh.data(), m->lu, sizeof(double) * m->nnz,
-                                    hipMemcpyDeviceToHost, s));
-        KCFD_HIP(hipStreamSynchronize(s));
-        for (int i = 0; i < n; ++i) {
-            const double u = luh[m->h_diagpos[i]];
-            if (u == 0.0 || !(u == u)) {
-                set_error("ILU(0): zero or NaN pivot at row %d", i);
-                return KCFD_EPIVOT;
-            }
-        }
+    // pivot check: first row with a zero or NaN u_ii (or n if none)
+    KCFD_CHECK(ensure_scratch(m->ctx, sizeof(int)));
+    int *bad = (int *)m->ctx->scratch;
+    const int init = n;
+    KCFD_HIP(hipMemcpyAsync(bad, &init, sizeof(int), hipMemcpyHostToDevice, s));
+    if (n)
+        hipLaunchKernelGGL(k_pivot_check, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
+                           m->diagpos, m->lu, bad);
+    int first = n;
+    KCFD_HIP(hipMemcpyAsync(&first, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    KCFD_HIP(hipStreamSynchronize(s));
+    if (first < n) {
+        set_error("ILU(0): zero or NaN pivot at row %d", first);
+        return KCFD_EPIVOT;
     }
     return KCFD_OK;
 }

@@ -2,6 +2,7 @@
 // preconditioner set-up.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -69,14 +70,32 @@ __global__ void k_jacobi(int n, const int *__restrict__ dp, const double *__rest
     if (dinv32) dinv32[i] = (float)r;
 }
 
-__global__ void k_to_f32(size_t n, const double *__restrict__ a, float *__restrict__ b) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    size_t st = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += st) b[i] = (float)a[i];
+// SELL-64 column indices (structure) and values of one row per thread
+__global__ void k_sell_struct(int n, const int *__restrict__ ip, const int *__restrict__ ix,
+                              const int *__restrict__ sptr, int *__restrict__ six) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int s = r >> 6, base = sptr[s] + (r & 63), w = (sptr[s + 1] - sptr[s]) >> 6;
+    const int q0 = ip[r], len = ip[r + 1] - q0;
+    for (int k = 0; k < w; ++k) six[base + 64 * k] = k < len ? ix[q0 + k] : r;
+}
+
+__global__ void k_sell_values(int n, const int *__restrict__ ip, const int *__restrict__ sptr,
+                              const double *__restrict__ a, double *__restrict__ sa64,
+                              float *__restrict__ sa32) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int s = r >> 6, base = sptr[s] + (r & 63), w = (sptr[s + 1] - sptr[s]) >> 6;
+    const int q0 = ip[r], len = ip[r + 1] - q0;
+    for (int k = 0; k < w; ++k) {
+        const double v = k < len ? a[q0 + k] : 0.0;
+        sa64[base + 64 * k] = v;
+        if (sa32) sa32[base + 64 * k] = (float)v;
+    }
 }
 
 int mat_free_precond(kcfd_mat *m) {
-    hipFree(m->val32); m->val32 = nullptr;
+    hipFree(m->sa32); m->sa32 = nullptr;
     hipFree(m->dinv); m->dinv = nullptr;
     hipFree(m->dinv32); m->dinv32 = nullptr;
     hipFree(m->lu); m->lu = nullptr;
@@ -235,6 +254,20 @@ int kcfd_mat_create(kcfd_ctx *ctx, int n, int nnz, const int *indptr, const int 
     for (int i = 0; i < n; ++i)
         for (int q = indptr[i]; q < indptr[i + 1]; ++q)
             if (indices[q] == i) m->h_diagpos[i] = q;
+    // SELL-64 slice offsets (in entries, padded)
+    m->nslices = (n + 63) / 64;
+    m->h_slice_ptr.assign(m->nslices + 1, 0);
+    {
+        long long acc = 0;
+        for (int sl = 0; sl < m->nslices; ++sl) {
+            int w = 0;
+            for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) w = std::max(w, indptr[r + 1] - indptr[r]);
+            acc += 64LL * w;
+            KCFD_ARG(acc < (1LL << 31), "matrix too large for 32-bit SELL offsets");
+            m->h_slice_ptr[sl + 1] = (int)acc;
+        }
+        m->sell_nnz = (int)acc;
+    }
     auto fail = [&](int rc) { kcfd_mat_destroy(m); return rc; };
     if (hipMalloc(&m->indptr, sizeof(int) * (n + 1)) != hipSuccess ||
         hipMalloc(&m->indices, sizeof(int) * (nnz ? nnz : 1)) != hipSuccess ||
@@ -254,6 +287,25 @@ int kcfd_mat_create(kcfd_ctx *ctx, int n, int nnz, const int *indptr, const int 
         set_error("matrix upload failed");
         return fail(KCFD_EHIP);
     }
+    if (hipMalloc(&m->slice_ptr, sizeof(int) * (m->nslices + 1)) != hipSuccess ||
+        hipMalloc(&m->six, sizeof(int) * std::max(m->sell_nnz, 1)) != hipSuccess ||
+        hipMalloc(&m->sa64, sizeof(double) * std::max(m->sell_nnz, 1)) != hipSuccess) {
+        set_error("hipMalloc failed for the SELL copy (n=%d, padded nnz=%d)", n, m->sell_nnz);
+        return fail(KCFD_EOOM);
+    }
+    if (hipMemcpy(m->slice_ptr, m->h_slice_ptr.data(), sizeof(int) * (m->nslices + 1),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        set_error("SELL upload failed");
+        return fail(KCFD_EHIP);
+    }
+    if (n) {
+        hipLaunchKernelGGL(k_sell_struct, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
+                           n, m->indptr, m->indices, m->slice_ptr, m->six);
+        if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+            set_error("SELL structure build failed");
+            return fail(KCFD_EHIP);
+        }
+    }
     *out = m;
     return KCFD_OK;
 }
@@ -271,6 +323,9 @@ int kcfd_mat_destroy(kcfd_mat *m) {
     hipFree(m->val);
     hipFree(m->rhs);
     hipFree(m->diagt);
+    hipFree(m->slice_ptr);
+    hipFree(m->six);
+    hipFree(m->sa64);
     delete m;
     return KCFD_OK;
 }
@@ -317,7 +372,8 @@ int kcfd_mat_spmv(kcfd_mat *m, const double *x, double *y) {
 int kcfd_mat_bytes(kcfd_mat *m, long long *bytes) {
     KCFD_ARG(m && bytes, "null argument");
     long long b = 4LL * (m->n + 1) + 4LL * m->nnz + 4LL * m->n + 8LL * m->nnz + 16LL * m->n;
-    if (m->val32) b += 4LL * m->nnz;
+    b += 4LL * (m->nslices + 1) + 12LL * m->sell_nnz;
+    if (m->sa32) b += 4LL * m->sell_nnz;
     if (m->dinv) b += 8LL * m->n;
     if (m->dinv32) b += 4LL * m->n;
     if (m->lu) b += 8LL * m->nnz + 8LL * m->n;
@@ -334,29 +390,35 @@ int kcfd_precond_setup(kcfd_mat *m, int kind, int precision) {
     std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
     KCFD_HIP(hipSetDevice(m->ctx->device));
     hipStream_t s = m->ctx->stream;
-    KCFD_HIP(hipStreamSynchronize(s));
-    if (m->solver) {
-        kcfd_batch_destroy(m->solver);
-        m->solver = nullptr;
-    }
-    mat_free_precond(m);
     const int n = m->n;
     const bool want32 = precision == KCFD_MIXED;
-    if (want32 && m->nnz) {
-        KCFD_HIP(hipMalloc(&m->val32, sizeof(float) * m->nnz));
-        hipLaunchKernelGGL(k_to_f32, dim3(nblk(m->nnz)), dim3(kBlock), 0, s, (size_t)m->nnz,
-                           m->val, m->val32);
-        KCFD_HIP(hipGetLastError());
-        m->have32 = true;
-    }
-    if (kind == KCFD_PC_JACOBI) {
-        KCFD_HIP(hipMalloc(&m->dinv, sizeof(double) * (n ? n : 1)));
-        if (want32) KCFD_HIP(hipMalloc(&m->dinv32, sizeof(float) * (n ? n : 1)));
-        if (n) {
-            hipLaunchKernelGGL(k_jacobi, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
-                               m->diagpos, m->val, m->dinv, m->dinv32);
-            KCFD_HIP(hipGetLastError());
+    // Same kind and precision as last time: only the values changed, so the
+    // buffers (and any solver workspace that points at them) are reused.
+    const bool same = m->pc_kind == kind && m->have32 == want32 &&
+                      (kind != KCFD_PC_ILU0 || m->lu != nullptr) &&
+                      (kind != KCFD_PC_JACOBI || m->dinv != nullptr);
+    if (!same) {
+        KCFD_HIP(hipStreamSynchronize(s));
+        if (m->solver) {
+            kcfd_batch_destroy(m->solver);
+            m->solver = nullptr;
         }
+        mat_free_precond(m);
+        if (want32) KCFD_HIP(hipMalloc(&m->sa32, sizeof(float) * std::max(m->sell_nnz, 1)));
+        if (kind == KCFD_PC_JACOBI) {
+            KCFD_HIP(hipMalloc(&m->dinv, sizeof(double) * (n ? n : 1)));
+            if (want32) KCFD_HIP(hipMalloc(&m->dinv32, sizeof(float) * (n ? n : 1)));
+        }
+    }
+    if (n) {
+        hipLaunchKernelGGL(k_sell_values, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
+                           m->indptr, m->slice_ptr, m->val, m->sa64, m->sa32);
+        KCFD_HIP(hipGetLastError());
+    }
+    if (kind == KCFD_PC_JACOBI && n) {
+        hipLaunchKernelGGL(k_jacobi, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
+                           m->diagpos, m->val, m->dinv, m->dinv32);
+        KCFD_HIP(hipGetLastError());
     } else if (kind == KCFD_PC_ILU0) {
         KCFD_CHECK(ilu0_setup(m, want32));
     }

@@ -24,7 +24,7 @@ def _f64(a):
 
 
 def default_opts(rtol=1e-8, atol=0.0, maxiter=0, precond="jacobi", precision="fp64",
-                 inner_rtol=1e-4, max_refine=30, check_every=8) -> L.SolveOpts:
+                 inner_rtol=0.1, max_refine=1000, check_every=8) -> L.SolveOpts:
     o = L.SolveOpts()
     o.rtol = float(rtol)
     o.atol = float(atol)
@@ -126,8 +126,12 @@ class DeviceMatrix:
     def spmv(self, x_dev: DevBuf, y_dev: DevBuf):
         check(lib().kcfd_mat_spmv(self.h, x_dev.p, y_dev.p))
 
-    def solve_dev(self, b_ptr, x_ptr, opts: L.SolveOpts) -> L.SolveResult:
-        if self.pc is None or self.pc != (opts.precond, opts.precision):
+    def solve_dev(self, b_ptr, x_ptr, opts: L.SolveOpts, setup: bool = True) -> L.SolveResult:
+        """Solve on device buffers.  The preconditioner is rebuilt from the
+        current values (as create_preconditioner is called on every
+        solve_linear_system, core/linear_solver.py:56) unless ``setup`` is
+        False and it was set up for these options already."""
+        if setup or self.pc != (opts.precond, opts.precision):
             self.setup_precond(opts.precond, opts.precision)
         res = L.SolveResult()
         check(lib().kcfd_solve(self.h, b_ptr, x_ptr, ctypes.byref(opts), ctypes.byref(res)))

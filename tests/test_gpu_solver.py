@@ -106,15 +106,20 @@ def test_hex_pressure_jacobi(ctx):
 
 
 @pytest.mark.parametrize("pc", ["jacobi", "ilu0"])
-def test_mixed_precision_refinement(ctx, pc):
-    """FP32 SpMV/preconditioner/vectors inside FP64 refinement: FP64-verified
-    residual below 1e-8 and the error against the FP64 oracle solution."""
+@pytest.mark.parametrize("size", ["small", "large"])
+def test_mixed_precision_refinement(ctx, pc, size):
+    """FP32 SpMV/preconditioner/vectors with FP64 reliable updates: FP64-verified
+    residual below 1e-8, the GPU's reported residual equal to the host's, and
+    the error against the FP64 oracle solution reported."""
     device, workloads = _kc()
-    case = workloads.hex_pressure(16, seed=1002) if pc == "ilu0" else workloads.quad_convdiff(64)
+    if size == "small":
+        case = workloads.hex_pressure(16, seed=1002) if pc == "ilu0" else workloads.quad_convdiff(64)
+    else:
+        case = workloads.hex_pressure(40, seed=1002)
     A, b, _ = oracle_system(case)
     xo, info_o, it_o = osolve.solve_bicgstab(A, b, rtol=1e-12,
                                              precond="ilu0_struct" if pc == "ilu0" else "jacobi")
-    x, res = _gpu_solve(ctx, A, b, rtol=1e-8, precond=pc, precision="mixed", inner_rtol=1e-4)
+    x, res = _gpu_solve(ctx, A, b, rtol=1e-8, precond=pc, precision="mixed")
     err = rel_l2(x, xo)
     true_res = osolve.true_rel_residual(A, x, b)
     print(f"mixed {pc}: iters={res.iters} refinements={res.refinements} info={res.info} "
