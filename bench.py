@@ -95,6 +95,8 @@ def main():
     ctx = L.default_context()
     info = ctx.info()
     log("device", info)
+    copy_gbs = ctx.stream_copy_gbs(1 << 30, 10)
+    log(f"STREAM copy (1 GiB): {copy_gbs:.1f} GB/s")
     t = time.perf_counter()
     case = workloads.hex_pressure(args.n, seed=1002)
     mesh = case.mesh
@@ -169,7 +171,8 @@ def main():
     achieved = dk["gbs"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "solve_avg_gbs": round(mixed["bytes"] / (mixed["solve_ms"] * 1e-3) / 1e9, 1)}
+                "solve_avg_gbs": round(mixed["bytes"] / (mixed["solve_ms"] * 1e-3) / 1e9, 1),
+                "stream_copy_gbs": round(copy_gbs, 1)}
     cpu = None
     if not args.no_cpu:
         try:

@@ -78,6 +78,10 @@ const char *kcfd_last_error(void);
 int kcfd_device_info(kcfd_ctx *ctx, char *name, int len, int *cus,
                      long long *total_bytes, long long *free_bytes);
 int kcfd_sync(kcfd_ctx *ctx);
+/* STREAM-copy ceiling: mean time of a device-to-device copy of `bytes`
+ * (read + write) over `reps` launches.  Measurement hook with no reference
+ * counterpart; calibrates the roofline and the FETCH/WRITE counters. */
+int kcfd_stream_copy(kcfd_ctx *ctx, size_t bytes, int reps, double *avg_ms);
 
 /* ------------------------------------------------------- device buffers */
 

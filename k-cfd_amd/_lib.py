@@ -74,6 +74,7 @@ SIGNATURES = {
     "kcfd_device_info": ([_P, ctypes.c_char_p, _I, ctypes.POINTER(_I), ctypes.POINTER(ctypes.c_longlong),
                           ctypes.POINTER(ctypes.c_longlong)], _I),
     "kcfd_sync": ([_P], _I),
+    "kcfd_stream_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
     "kcfd_buf_alloc": ([_P, _SZ, _PP], _I),
     "kcfd_buf_free": ([_P, _P], _I),
     "kcfd_buf_upload": ([_P, _P, _P, _SZ], _I),
@@ -168,6 +169,12 @@ class Context:
 
     def sync(self):
         check(lib().kcfd_sync(self.h))
+
+    def stream_copy_gbs(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
+        """Measured device copy bandwidth (read + write bytes / time)."""
+        ms = ctypes.c_double()
+        check(lib().kcfd_stream_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
+        return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
 
     def close(self):
         if getattr(self, "h", None):

@@ -94,6 +94,21 @@ __global__ void k_sell_values(int n, const int *__restrict__ ip, const int *__re
     }
 }
 
+// STREAM copy (16-byte vectors, 4 in flight per thread per step)
+__global__ __launch_bounds__(kBlock) void k_stream_copy(size_t n16, const double2 *__restrict__ src,
+                                                        double2 *__restrict__ dst) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n16; i += 4 * stride) {
+        const double2 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
+        dst[i] = a;
+        dst[i + stride] = b;
+        dst[i + 2 * stride] = c;
+        dst[i + 3 * stride] = d;
+    }
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
 int mat_free_precond(kcfd_mat *m) {
     hipFree(m->sa32); m->sa32 = nullptr;
     hipFree(m->dinv); m->dinv = nullptr;
@@ -223,6 +238,39 @@ int kcfd_buf_fill(kcfd_ctx *ctx, double *dev, double value, size_t n) {
     if (!n) return KCFD_OK;
     hipLaunchKernelGGL(k_fill, dim3(nblk(n)), dim3(kBlock), 0, ctx->stream, dev, value, n);
     KCFD_HIP(hipGetLastError());
+    return KCFD_OK;
+}
+
+int kcfd_stream_copy(kcfd_ctx *ctx, size_t bytes, int reps, double *avg_ms) {
+    KCFD_ARG(ctx && avg_ms && reps > 0 && bytes >= 16, "bad argument");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    KCFD_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    const size_t n16 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    KCFD_HIP(hipMalloc(&a, n16 * 16));
+    if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+        hipFree(a);
+        set_error("stream copy: hipMalloc of %zu bytes failed", n16 * 16);
+        return KCFD_EOOM;
+    }
+    hipMemsetAsync(a, 0, n16 * 16, s);
+    const unsigned grid = (unsigned)(ctx->cus * 8);
+    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, s, n16, (const double2 *)a, (double2 *)b);
+    hipEventRecord(ctx->ev0, s);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, s, n16, (const double2 *)a, (double2 *)b);
+    hipEventRecord(ctx->ev1, s);
+    hipError_t e = hipEventSynchronize(ctx->ev1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    hipFree(a);
+    hipFree(b);
+    if (e != hipSuccess) {
+        set_error("stream copy: %s", hipGetErrorString(e));
+        return KCFD_EHIP;
+    }
+    *avg_ms = ms / reps;
     return KCFD_OK;
 }
 
