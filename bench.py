@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--delta", type=float, default=0.1,
                     help="mixed: reliable-update trigger (FP32 residual reduction)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precisions", default="fp64,mixed",
+                    help="comma list; a profiler run can take one precision per process")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -115,7 +117,8 @@ def main():
         L.check(L.lib().kcfd_asm_pressure(dm.h, A.h, dP.p, float(rec["rho"]), pd.ctypes.data))
 
     results = {}
-    for prec in ["fp64", "mixed"]:
+    precs = [p for p in args.precisions.split(",") if p]
+    for prec in precs:
         opts = device.default_opts(rtol=1e-8, precond=args.precond, precision=prec,
                                    inner_rtol=args.delta, check_every=8)
         lat, rs = [], []
@@ -161,7 +164,8 @@ def main():
         results[prec]["kernels"] = kern
         log(prec, "kernels", kern)
 
-    mixed, fp64 = results["mixed"], results["fp64"]
+    mixed = results.get("mixed") or results[precs[0]]
+    fp64 = results.get("fp64") or results[precs[0]]
     # dominant kernel of the mixed solve: per iteration, 2 SpMV-type launches, 1
     # x/r update and (ILU) 4 triangular solves
     kern = mixed["kernels"]
@@ -203,7 +207,8 @@ def main():
         "cpu_baseline": cpu,
         "mixed": {k: v for k, v in mixed.items() if k != "kernels"},
         "fp64": {k: v for k, v in fp64.items() if k != "kernels"},
-        "mixed_vs_fp64": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3),
+        "mixed_vs_fp64": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3) if len(precs) > 1 else None,
+        "value_precision": "mixed" if "mixed" in results else precs[0],
         "kernels_mixed": kern,
         "kernels_fp64": fp64["kernels"],
         "device": info["name"],
