@@ -201,8 +201,9 @@ template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
                                                  const int2 *__restrict__ tiles, int ntiles,
-                                                 double *__restrict__ partials, BatchParams bp) {
+                                                 double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
 #pragma clang fp contract(off)
+    const BatchParams bp = *bpp;
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
@@ -232,7 +233,8 @@ __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys
 
 __global__ __launch_bounds__(kBlock) void f_init(const SysDesc *__restrict__ sys,
                                                  SysState *__restrict__ sts,
-                                                 const double *__restrict__ partials, BatchParams bp) {
+                                                 const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
+    const BatchParams bp = *bpp;
     const SysDesc &d = sys[blockIdx.x];
     SysState &st = sts[blockIdx.x];
     if (st.status != S_START) return;
@@ -344,7 +346,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_p(const SysDesc *__restrict__ s
 
 __global__ __launch_bounds__(kBlock) void f_spmv_p(const SysDesc *__restrict__ sys,
                                                    SysState *__restrict__ sts,
-                                                   const double *__restrict__ partials, BatchParams bp) {
+                                                   const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
+    const BatchParams bp = *bpp;
     const SysDesc &d = sys[blockIdx.x];
     SysState &st = sts[blockIdx.x];
     const int status = st.status;
@@ -433,7 +436,8 @@ __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
 }
 
 __global__ __launch_bounds__(kBlock) void f_s(const SysDesc *__restrict__ sys, SysState *__restrict__ sts,
-                                              const double *__restrict__ partials, BatchParams bp) {
+                                              const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
+    const BatchParams bp = *bpp;
     const SysDesc &d = sys[blockIdx.x];
     SysState &st = sts[blockIdx.x];
     if (st.status != S_RUN) return;
@@ -449,8 +453,9 @@ template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ sys,
                                                    const SysState *__restrict__ sts,
                                                    const int2 *__restrict__ tiles, int ntiles,
-                                                   double *__restrict__ partials, BatchParams bp) {
+                                                   double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
 #pragma clang fp contract(off)
+    const BatchParams bp = *bpp;
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
@@ -483,7 +488,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ s
 
 __global__ __launch_bounds__(kBlock) void f_spmv_s(const SysDesc *__restrict__ sys,
                                                    SysState *__restrict__ sts,
-                                                   const double *__restrict__ partials, BatchParams bp) {
+                                                   const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
+    const BatchParams bp = *bpp;
     const SysDesc &d = sys[blockIdx.x];
     SysState &st = sts[blockIdx.x];
     const int status = st.status;
@@ -510,8 +516,9 @@ template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
                                                const SysState *__restrict__ sts,
                                                const int2 *__restrict__ tiles, int ntiles,
-                                               double *__restrict__ partials, BatchParams bp) {
+                                               double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
 #pragma clang fp contract(off)
+    const BatchParams bp = *bpp;
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
@@ -541,7 +548,8 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
 }
 
 __global__ __launch_bounds__(kBlock) void f_xr(const SysDesc *__restrict__ sys, SysState *__restrict__ sts,
-                                               const double *__restrict__ partials, BatchParams bp) {
+                                               const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
+    const BatchParams bp = *bpp;
     const SysDesc &d = sys[blockIdx.x];
     SysState &st = sts[blockIdx.x];
     if (st.status != S_RUN) return;
@@ -725,6 +733,7 @@ struct kcfd_batch {
     Chunk *d_chunks_l = nullptr, *d_chunks_u = nullptr;
     int nchunks_l = 0, nchunks_u = 0, trsv_grid = 0;
     const int **d_lvl_l = nullptr, **d_lvl_u = nullptr;
+    BatchParams *d_params = nullptr;   // read by the kernels: the captured graph stays valid
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
 };
@@ -747,10 +756,10 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     }
     hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->ntiles, b->d_partials);
-    hipLaunchKernelGGL(f_spmv_p, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->bp);
+    hipLaunchKernelGGL(f_spmv_p, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     hipLaunchKernelGGL((k_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles,
                        b->d_partials, b->d_counters);
-    hipLaunchKernelGGL(f_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->bp);
+    hipLaunchKernelGGL(f_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     if (PC == KCFD_PC_ILU0) {
         hipLaunchKernelGGL((k_trsv<V, false, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
                            b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters + 2);
@@ -758,19 +767,19 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
                            b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u, b->d_counters + 3);
     }
     hipLaunchKernelGGL((k_spmv_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                       b->ntiles, b->d_partials, b->bp);
-    hipLaunchKernelGGL(f_spmv_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->bp);
+                       b->ntiles, b->d_partials, b->d_params);
+    hipLaunchKernelGGL(f_spmv_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     hipLaunchKernelGGL((k_xr<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                       b->ntiles, b->d_partials, b->bp);
-    hipLaunchKernelGGL(f_xr, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->bp);
+                       b->ntiles, b->d_partials, b->d_params);
+    hipLaunchKernelGGL(f_xr, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
 }
 
 template <typename V, int PC>
 void launch_init(kcfd_batch *b, hipStream_t s) {
     hipLaunchKernelGGL((k_init<V, PC>), dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                       b->d_tiles, b->ntiles, b->d_partials, b->bp);
+                       b->d_tiles, b->ntiles, b->d_partials, b->d_params);
     hipLaunchKernelGGL(f_init, dim3(b->nsys), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                       b->d_partials, b->bp);
+                       b->d_partials, b->d_params);
 }
 
 using LaunchFn = void (*)(kcfd_batch *, hipStream_t);
@@ -908,6 +917,7 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
     KCFD_HIP(hipMalloc(&b->d_partials, sizeof(double) * kMaxPartials * b->ntiles));
     KCFD_HIP(hipMalloc(&b->d_counters, sizeof(unsigned int) * 8));
     KCFD_HIP(hipMemset(b->d_counters, 0, sizeof(unsigned int) * 8));
+    KCFD_HIP(hipMalloc(&b->d_params, sizeof(BatchParams)));
     if (pc == KCFD_PC_ILU0) {
         // level-major chunk tables over all systems
         int maxlev_l = 0, maxlev_u = 0;
@@ -1012,6 +1022,7 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     }
     KCFD_HIP(hipMemcpyAsync(b->d_state, b->h_state.data(), sizeof(SysState) * b->nsys,
                             hipMemcpyHostToDevice, s));
+    KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
     if (!b->graph || b->graph_iters != b->opts.check_every)
         KCFD_CHECK(capture_graph(b, b->opts.check_every));
     KCFD_HIP(hipEventRecord(ctx->ev0, s));
@@ -1074,7 +1085,7 @@ void launch_phase(kcfd_batch *b, int phase, hipStream_t s) {
                            b->d_tiles, b->ntiles, b->d_partials);
     else if (phase == 2)
         hipLaunchKernelGGL((k_xr<V, PC>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                           b->ntiles, b->d_partials, b->bp);
+                           b->ntiles, b->d_partials, b->d_params);
     else if (phase == 4)
         hipLaunchKernelGGL((k_p<V, PC>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                            b->ntiles, b->d_counters);
@@ -1126,6 +1137,7 @@ int kcfd_batch_destroy(kcfd_batch *b) {
     hipFree(b->d_tiles);
     hipFree(b->d_partials);
     hipFree(b->d_counters);
+    hipFree(b->d_params);
     hipFree(b->slab);
     hipFree(b->d_chunks_l);
     hipFree(b->d_chunks_u);
@@ -1259,8 +1271,8 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
         b->bp.atol = opts->atol;
         b->bp.delta = opts->inner_rtol > 0 ? opts->inner_rtol : 0.1;
         b->bp.max_refine = opts->max_refine > 0 ? opts->max_refine : 1000;
-        // kernel parameters are baked into the captured graph: recapture
-        if (b->graph) { hipGraphExecDestroy(b->graph); b->graph = nullptr; }
+        // the parameters live in device memory (uploaded by run_batch), so the
+        // captured graph is reused as is
     }
     return run_batch(m->solver, res);
 }
