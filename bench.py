@@ -80,6 +80,8 @@ def main():
     ap.add_argument("--delta", type=float, default=0.1,
                     help="mixed: reliable-update trigger (FP32 residual reduction)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--check-every", type=int, default=8,
+                    help="iterations per captured graph between host status checks")
     ap.add_argument("--precisions", default="fp64,mixed",
                     help="comma list; a profiler run can take one precision per process")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
@@ -120,7 +122,7 @@ def main():
     precs = [p for p in args.precisions.split(",") if p]
     for prec in precs:
         opts = device.default_opts(rtol=1e-8, precond=args.precond, precision=prec,
-                                   inner_rtol=args.delta, check_every=8)
+                                   inner_rtol=args.delta, check_every=args.check_every)
         lat, rs = [], []
 
         def step():
