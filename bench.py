@@ -79,6 +79,8 @@ def main():
     ap.add_argument("--precond", default="ilu0", choices=["ilu0", "jacobi"])
     ap.add_argument("--delta", type=float, default=0.1,
                     help="mixed: reliable-update trigger (FP32 residual reduction)")
+    ap.add_argument("--order", default="color", choices=["natural", "color"],
+                    help="ILU(0) row ordering (colour: first-fit colouring, 2 levels per solve)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check-every", type=int, default=8,
                     help="iterations per captured graph between host status checks")
@@ -127,7 +129,7 @@ def main():
 
         def step():
             assemble()
-            A.setup_precond(args.precond, prec)
+            A.setup_precond(args.precond, prec, order=args.order)
             xdev.fill(0.0)
             return A.solve_dev(bdev.p, xdev.p, opts, setup=False)
 
@@ -203,7 +205,8 @@ def main():
         "dtype": "f32 inner / f64 refinement",
         "data": "synthetic",
         "config": {"workload": f"hex{args.n}^3 SIMPLE pressure correction, BiCGSTAB+{args.precond}, "
-                               f"rtol 1e-8 (config 2)", "cells": n, "nnz": nnz,
+                               f"rtol 1e-8", "cells": n, "nnz": nnz,
+                   "ilu_order": args.order if args.precond == "ilu0" else None,
                    "step": "device assembly + preconditioner set-up + solve"},
         "roofline": roofline,
         "cpu_baseline": cpu,

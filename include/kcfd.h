@@ -116,8 +116,24 @@ int kcfd_mat_bytes(kcfd_mat *m, long long *bytes);
  * pattern, level-scheduled (replaces the spilu of :49-54).  With
  * precision KCFD_MIXED an FP32 copy of values and factors is kept too. */
 int kcfd_precond_setup(kcfd_mat *m, int kind, int precision);
-/* ILU(0) factors (host nnz) and the number of levels of the L and U solves */
+/* ILU(0) factors (host nnz) and the number of levels of the L and U solves.
+ * The factors are stored on the pattern of P A P^T (see kcfd_ilu0_order):
+ * with the natural ordering that is A's own pattern. */
 int kcfd_ilu0_get(kcfd_mat *m, double *lu, int *levels_l, int *levels_u);
+/* Row ordering of the ILU(0) factorisation (no reference counterpart; the
+ * reference's spilu chooses its own COLAMD ordering, core/preconditioner.py:
+ * 52).  KCFD_ORDER_NATURAL factors A itself.  KCFD_ORDER_COLOR factors
+ * P A P^T, P sorting rows by a first-fit colouring of the graph of A + A^T
+ * (rows of one colour are never coupled), so each triangular solve has as
+ * many dependency levels as colours (2 on the 5- and 7-point grids).  Takes
+ * effect at the next kcfd_precond_setup. */
+#define KCFD_ORDER_NATURAL 0
+#define KCFD_ORDER_COLOR 1
+int kcfd_precond_set_order(kcfd_mat *m, int order);
+/* The ordering of the current ILU(0) factors: perm (host n, may be null)
+ * receives the row of A at each position of P A P^T, ncolors the number of
+ * colours (0 for the natural ordering). */
+int kcfd_ilu0_order(kcfd_mat *m, int *perm, int *ncolors);
 /* y = M^{-1} x (dev) with the preconditioner set up on m (identity for
  * none): the LinearOperator matvec that create_preconditioner returns
  * (core/preconditioner.py:45-46, :52-53). */

@@ -23,6 +23,8 @@ FP64, MIXED = 0, 1
 BC_ZERO_GRADIENT, BC_DIRICHLET, BC_NEUMANN, BC_NONE = 0, 1, 2, 3
 LIMITERS = {"minmod": 0, "superbee": 1, "van_albada": 2, "van_leer": 3}
 PRECONDS = {"none": PC_NONE, "jacobi": PC_JACOBI, "ilu0": PC_ILU0}
+ORDER_NATURAL, ORDER_COLOR = 0, 1
+ORDERS = {"natural": ORDER_NATURAL, "color": ORDER_COLOR}
 
 _ERRNAMES = {KCFD_EARG: "bad argument", KCFD_EHIP: "HIP error", KCFD_EOOM: "out of device memory",
              KCFD_ESTATE: "bad state", KCFD_EPIVOT: "zero pivot", KCFD_EBUSY: "admission refused"}
@@ -89,6 +91,8 @@ SIGNATURES = {
     "kcfd_mat_bytes": ([_P, ctypes.POINTER(ctypes.c_longlong)], _I),
     "kcfd_precond_setup": ([_P, _I, _I], _I),
     "kcfd_ilu0_get": ([_P, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
+    "kcfd_precond_set_order": ([_P, _I], _I),
+    "kcfd_ilu0_order": ([_P, _P, ctypes.POINTER(_I)], _I),
     "kcfd_precond_apply": ([_P, _P, _P], _I),
     "kcfd_solve": ([_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveResult)], _I),
     "kcfd_batch_create": ([_P, _I, _P, _P, _P, ctypes.POINTER(SolveOpts), _PP], _I),

@@ -57,7 +57,10 @@ enum : int {
 
 struct SysDesc {
     int n, tile0, ntiles, pad0;
-    const int *ip, *ix, *dpos;      // CSR (ILU, level lists)
+    // ILU(0) factor structure: CSR of P A P^T (A's own CSR in the natural
+    // ordering); uix = columns as rows of A; perm = row of A at each
+    // position (null: identity)
+    const int *ip, *ix, *dpos, *uix, *perm;
     const int *sptr, *six;          // SELL-64 structure
     const double *sa64;
     const float *sa32;
@@ -626,13 +629,20 @@ __global__ __launch_bounds__(kBlock) void f_true_resid(const SysDesc *__restrict
 //
 // Sync-free level-scheduled triangular solves: one launch per solve covers
 // all systems; 64-row chunks (one per wave, never spanning a level) are
-// taken in level-major order through an atomic ticket, so every row a wave
-// waits on belongs to a chunk already held by a running wave.  The output
+// listed in level-major order and dealt round-robin to the waves (wave w
+// takes chunks w, w + W, w + 2W, ...).  The grid is small enough to be
+// resident at once, so the lowest unfinished chunk always belongs to a
+// running wave whose earlier chunks are done and whose inputs (lower
+// levels, lower chunks) are ready: the solve cannot deadlock.  (A shared
+// atomic ticket would serialise tens of thousands of same-address atomics
+// on the wide levels of a colour ordering.)  The output
 // vector doubles as the ready flag: it holds a NaN sentinel until the row is
 // written (agent-scope atomic store) and waiters poll it with agent-scope
 // atomic loads.  Row arithmetic follows the oracle's order exactly.
-//   LOWER: y_i = z_i - sum_{j<i} l_ij y_j               (unit L)
-//   UPPER: w_i = (y_i - sum_{j>i} u_ij w_j) / u_ii; y_i reset to sentinel
+// Rows are positions i of P A P^T (perm maps them to rows of A; identity in
+// the natural ordering); y is kept in P-order, z and w in A's order.
+//   LOWER: y_i = z_{perm i} - sum_{j<i} l_ij y_j                   (unit L)
+//   UPPER: w_{perm i} = (y_i - sum_{j>i} u_ij w_{perm j}) / u_ii; y_i reset
 template <typename V>
 __device__ __forceinline__ V poll_ready(const V *p) {
     using U = typename Sentinel<V>::U;
@@ -656,19 +666,14 @@ template <typename V, bool UPPER, int WHICH>
 __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
                                                  const Chunk *__restrict__ chunks, int nchunks,
-                                                 const int *const *__restrict__ lvl_rows,
-                                                 unsigned int *__restrict__ counter) {
+                                                 const int *const *__restrict__ lvl_rows) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
-    // The chunk loop is wave-uniform: the chunk index is read from lane 0 with
-    // all lanes active, the exit test is on that uniform value, and per-lane
-    // work sits inside an if.  (A divergent `continue` here would let lanes
-    // re-enter the loop head without lane 0 and read a stale chunk index.)
-    for (;;) {
-        unsigned int got = 0u;
-        if (lane == 0) got = atomicAdd(counter, 1u);
-        const int c = (int)__builtin_amdgcn_readfirstlane(got);
-        if (c >= nchunks) break;
+    // The chunk loop is wave-uniform (the chunk index depends on the wave
+    // only); per-lane work sits inside an if.
+    const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
+    const int wave = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    for (int c = wave; c < nchunks; c += nwaves) {
         const Chunk ch = chunks[c];
         const SysDesc &d = sys[ch.sys];
         const int status = sts[ch.sys].status;
@@ -683,18 +688,19 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
             const int i = lvl_rows[ch.sys][ch.start + lane];
             const V *__restrict__ a = luv<V>(d);
             const int dp = d.dpos[i];
+            const int r = d.perm ? d.perm[i] : i;
             if (!UPPER) {
                 const V *z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
                 V *y = (V *)d.y;
-                V acc = z[i];
+                V acc = z[r];
                 for (int q = d.ip[i]; q < dp; ++q) acc -= a[q] * poll_ready<V>(y + d.ix[q]);
                 publish<V>(y + i, acc);
             } else {
                 V *y = (V *)d.y;
                 V *w = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
                 V acc = y[i];
-                for (int q = dp + 1; q < d.ip[i + 1]; ++q) acc -= a[q] * poll_ready<V>(w + d.ix[q]);
-                publish<V>(w + i, acc / a[dp]);
+                for (int q = dp + 1; q < d.ip[i + 1]; ++q) acc -= a[q] * poll_ready<V>(w + d.uix[q]);
+                publish<V>(w + r, acc / a[dp]);
                 y[i] = sentinel_value<V>();
             }
         }
@@ -750,9 +756,9 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
                        b->d_counters);
     if (PC == KCFD_PC_ILU0) {
         hipLaunchKernelGGL((k_trsv<V, false, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters + 0);
+                           b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l);
         hipLaunchKernelGGL((k_trsv<V, true, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u, b->d_counters + 1);
+                           b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u);
     }
     hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->ntiles, b->d_partials);
@@ -762,9 +768,9 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     hipLaunchKernelGGL(f_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     if (PC == KCFD_PC_ILU0) {
         hipLaunchKernelGGL((k_trsv<V, false, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters + 2);
+                           b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l);
         hipLaunchKernelGGL((k_trsv<V, true, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u, b->d_counters + 3);
+                           b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u);
     }
     hipLaunchKernelGGL((k_spmv_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->ntiles, b->d_partials, b->d_params);
@@ -815,7 +821,8 @@ double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st)
     auto MF = [&](double sv) { return nnz * (sv + 4) + 2 * (n + 1) * 4; };
     auto per_iter = [&](double sv) {
         if (pc == KCFD_PC_JACOBI) return 2 * MA(sv) + 22 * n * sv;
-        if (pc == KCFD_PC_ILU0) return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv;
+        // a permuted ILU(0) reads perm once per triangular solve
+        if (pc == KCFD_PC_ILU0) return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv + (m->perm ? 16 * n : 0);
         return 2 * MA(sv) + 19 * n * sv;
     };
     if (precision == KCFD_FP64) return (st.total_iters + 0.5 * st.half) * per_iter(8) + MA(8) + 3 * n * 8;
@@ -823,9 +830,11 @@ double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st)
 }
 
 void fill_desc(SysDesc &d, kcfd_mat *m) {
-    d.ip = m->indptr;
-    d.ix = m->indices;
-    d.dpos = m->diagpos;
+    d.ip = m->f_indptr;
+    d.ix = m->f_indices;
+    d.dpos = m->f_diagpos;
+    d.uix = m->f_uix;
+    d.perm = m->perm;
     d.sptr = m->slice_ptr;
     d.six = m->six;
     d.sa64 = m->sa64;
@@ -964,11 +973,9 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
         KCFD_HIP(hipMalloc((void **)&b->d_lvl_u, sizeof(int *) * nsys));
         KCFD_HIP(hipMemcpy((void *)b->d_lvl_l, pl.data(), sizeof(int *) * nsys, hipMemcpyHostToDevice));
         KCFD_HIP(hipMemcpy((void *)b->d_lvl_u, pu.data(), sizeof(int *) * nsys, hipMemcpyHostToDevice));
-        // Chunks are handed out dynamically in level order, so any grid size is
-        // deadlock-free.  Waves beyond a few levels of look-ahead only poll
-        // (and load the memory system with polls), so the grid covers about
-        // three of the widest levels (summed over the batch), capped at 8
-        // blocks per CU.
+        // Waves beyond a few levels of look-ahead only poll (and load the
+        // memory system with polls), so the grid covers about three of the
+        // widest levels (summed over the batch).
         int widest = 0;
         for (int l = 0; l < std::max(maxlev_l, maxlev_u); ++l) {
             int w = 0;
@@ -977,8 +984,10 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
             }
             widest = std::max(widest, w);
         }
+        // at most 4 workgroups (16 waves) per CU: the kernel's occupancy limit
+        // is 32 waves per CU, so the whole grid is resident
         const int want_waves = std::max(64, 3 * widest);
-        const int tcap = ctx->cus * 8;
+        const int tcap = ctx->cus * 4;
         b->trsv_grid = std::max(1, std::min((want_waves + kWaves - 1) / kWaves, tcap));
     }
     *out = b;
@@ -1200,8 +1209,8 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
                 else hipLaunchKernelGGL(k_fill_sentinel<float>, dim3(nb), dim3(kBlock), 0, s, (float *)d.y, d.n);
             }
             KCFD_HIP(hipEventRecord(b->ctx->ev0, s));
-            if (f64) hipLaunchKernelGGL((k_trsv<double, false, 0>), dim3(b->trsv_grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters);
-            else hipLaunchKernelGGL((k_trsv<float, false, 0>), dim3(b->trsv_grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l, b->d_counters);
+            if (f64) hipLaunchKernelGGL((k_trsv<double, false, 0>), dim3(b->trsv_grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l);
+            else hipLaunchKernelGGL((k_trsv<float, false, 0>), dim3(b->trsv_grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l);
             KCFD_HIP(hipEventRecord(b->ctx->ev1, s));
             KCFD_HIP(hipEventSynchronize(b->ctx->ev1));
             float ms = 0.f;
@@ -1234,7 +1243,7 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
         const double MFh = 0.5 * nnz * (sv + 4) + (n + 1) * 4;  // one triangle + its level list
         if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
         else if (phase == 2) tot += 8 * n * sv;
-        else if (phase == 3) tot += MFh + 2 * n * sv;
+        else if (phase == 3) tot += MFh + 2 * n * sv + (m->perm ? 4 * n : 0);  // + perm gather
         else if (phase == 4) tot += 4 * n * sv;
     }
     *bytes = tot;

@@ -123,7 +123,16 @@ struct kcfd_mat {
     bool have32 = false;
     double *dinv = nullptr;
     float *dinv32 = nullptr;
-    // ILU(0): factors on A's pattern, level schedule of the L and U solves
+    // ILU(0) ordering.  The factors are those of P A P^T; perm[k] is the row
+    // of A at position k.  The factor structure f_* is P A P^T's canonical
+    // CSR (aliases A's arrays in the natural ordering, where perm is null);
+    // f_vmap[q] is the slot of A feeding factor slot q and f_uix[q] the
+    // column as a row of A (what the U solve, writing in A's numbering, polls).
+    int ilu_order = KCFD_ORDER_NATURAL, ilu_built_order = -1, ncolors = 0;
+    int *perm = nullptr, *f_vmap = nullptr;
+    int *f_indptr = nullptr, *f_indices = nullptr, *f_diagpos = nullptr, *f_uix = nullptr;
+    std::vector<int> h_perm, hf_indptr, hf_indices, hf_diagpos;
+    // ILU(0): factors on the (permuted) pattern, level schedule of the solves
     double *lu = nullptr;
     float *lu32 = nullptr;
     int nlev_l = 0, nlev_u = 0;
@@ -138,5 +147,6 @@ struct kcfd_mat {
 namespace kcfd {
 int ensure_scratch(kcfd_ctx *ctx, size_t bytes);
 int ilu0_setup(kcfd_mat *m, bool want32);
+void ilu_free_order(kcfd_mat *m);
 int mat_free_precond(kcfd_mat *m);
 }  // namespace kcfd

@@ -8,6 +8,13 @@
 // factorisation, j > i for U).  The factorisation runs one launch per level;
 // the triangular solves are sync-free kernels (bicgstab.hip) that walk the
 // same level lists.
+//
+// Ordering.  With KCFD_ORDER_COLOR the factors are those of P A P^T, where P
+// sorts the rows by a first-fit colouring of the graph of A + A^T (a row's
+// colour is the smallest not used by an already coloured neighbour; rows in
+// ascending order; within a colour rows keep their order).  No two rows of a
+// colour are coupled, so every row of colour c depends only on colours < c
+// (L) or > c (U): the solves have as many levels as colours.
 #include "common.hpp"
 
 #include <algorithm>
@@ -91,6 +98,126 @@ static void level_schedule(const std::vector<int> &ip, const std::vector<int> &i
     }
 }
 
+__global__ void k_gather_vals(int nnz, const int *__restrict__ map, const double *__restrict__ a,
+                              double *__restrict__ b) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nnz) b[q] = a[map[q]];
+}
+
+// first-fit colouring of A + A^T, rows visited in ascending order
+static int color_rows(const std::vector<int> &ip, const std::vector<int> &ix, std::vector<int> &col) {
+    const int n = (int)ip.size() - 1;
+    // transpose pattern (rows j listing the i with a_ij != 0)
+    std::vector<int> tp(n + 1, 0), tx(ix.size());
+    for (int c : ix) tp[c + 1]++;
+    for (int i = 0; i < n; ++i) tp[i + 1] += tp[i];
+    {
+        std::vector<int> f(tp.begin(), tp.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int q = ip[i]; q < ip[i + 1]; ++q) tx[f[ix[q]]++] = i;
+    }
+    col.assign(n, -1);
+    std::vector<int> mark;   // mark[c] == i: colour c is taken by a neighbour of i
+    int ncol = 0;
+    for (int i = 0; i < n; ++i) {
+        auto take = [&](int j) {
+            if (j == i) return;
+            const int c = col[j];
+            if (c >= 0) mark[c] = i;
+        };
+        for (int q = ip[i]; q < ip[i + 1]; ++q) take(ix[q]);
+        for (int q = tp[i]; q < tp[i + 1]; ++q) take(tx[q]);
+        int c = 0;
+        while (c < (int)mark.size() && mark[c] == i) ++c;
+        if (c == (int)mark.size()) mark.push_back(-1);
+        col[i] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    return ncol;
+}
+
+void ilu_free_order(kcfd_mat *m) {
+    if (m->perm) {
+        hipFree(m->perm);
+        hipFree(m->f_vmap);
+        hipFree(m->f_indptr);
+        hipFree(m->f_indices);
+        hipFree(m->f_diagpos);
+        hipFree(m->f_uix);
+    }
+    m->perm = m->f_vmap = m->f_indptr = m->f_indices = m->f_diagpos = m->f_uix = nullptr;
+    m->h_perm.clear();
+    m->hf_indptr.clear();
+    m->hf_indices.clear();
+    m->hf_diagpos.clear();
+    m->ncolors = 0;
+    m->ilu_built_order = -1;
+}
+
+// factor structure for m->ilu_order (host copies + device arrays)
+static int build_order(kcfd_mat *m) {
+    ilu_free_order(m);
+    const int n = m->n, nnz = m->nnz;
+    if (m->ilu_order == KCFD_ORDER_NATURAL || n == 0) {
+        m->f_indptr = m->indptr;
+        m->f_indices = m->indices;
+        m->f_diagpos = m->diagpos;
+        m->f_uix = m->indices;
+        m->hf_indptr = m->h_indptr;
+        m->hf_indices = m->h_indices;
+        m->hf_diagpos = m->h_diagpos;
+        m->ilu_built_order = m->ilu_order;
+        return KCFD_OK;
+    }
+    std::vector<int> col;
+    m->ncolors = color_rows(m->h_indptr, m->h_indices, col);
+    // counting sort by colour, stable in the row id
+    std::vector<int> cp(m->ncolors + 1, 0);
+    for (int c : col) cp[c + 1]++;
+    for (int c = 0; c < m->ncolors; ++c) cp[c + 1] += cp[c];
+    std::vector<int> &perm = m->h_perm, iperm(n);
+    perm.assign(n, 0);
+    for (int i = 0; i < n; ++i) perm[cp[col[i]]++] = i;
+    for (int k = 0; k < n; ++k) iperm[perm[k]] = k;
+    // P A P^T: row k = row perm[k] of A, columns renumbered and re-sorted
+    std::vector<int> &fp = m->hf_indptr, &fx = m->hf_indices, &fd = m->hf_diagpos;
+    std::vector<int> vmap(nnz), uix(nnz);
+    fp.assign(n + 1, 0);
+    fx.assign(nnz, 0);
+    fd.assign(n, -1);
+    std::vector<std::pair<int, int>> row;
+    for (int k = 0; k < n; ++k) {
+        const int r = perm[k];
+        row.clear();
+        for (int q = m->h_indptr[r]; q < m->h_indptr[r + 1]; ++q) row.emplace_back(iperm[m->h_indices[q]], q);
+        std::sort(row.begin(), row.end());
+        fp[k + 1] = fp[k] + (int)row.size();
+        for (size_t t = 0; t < row.size(); ++t) {
+            const int q = fp[k] + (int)t;
+            fx[q] = row[t].first;
+            vmap[q] = row[t].second;
+            uix[q] = m->h_indices[row[t].second];
+            if (row[t].first == k) fd[k] = q;
+        }
+    }
+    KCFD_HIP(hipMalloc(&m->perm, sizeof(int) * n));
+    KCFD_HIP(hipMalloc(&m->f_vmap, sizeof(int) * std::max(nnz, 1)));
+    KCFD_HIP(hipMalloc(&m->f_indptr, sizeof(int) * (n + 1)));
+    KCFD_HIP(hipMalloc(&m->f_indices, sizeof(int) * std::max(nnz, 1)));
+    KCFD_HIP(hipMalloc(&m->f_diagpos, sizeof(int) * n));
+    KCFD_HIP(hipMalloc(&m->f_uix, sizeof(int) * std::max(nnz, 1)));
+    KCFD_HIP(hipMemcpy(m->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    KCFD_HIP(hipMemcpy(m->f_indptr, fp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    KCFD_HIP(hipMemcpy(m->f_diagpos, fd.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    if (nnz) {
+        KCFD_HIP(hipMemcpy(m->f_vmap, vmap.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+        KCFD_HIP(hipMemcpy(m->f_indices, fx.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+        KCFD_HIP(hipMemcpy(m->f_uix, uix.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+    }
+    m->ilu_built_order = m->ilu_order;
+    return KCFD_OK;
+}
+
 int ilu0_setup(kcfd_mat *m, bool want32) {
     const int n = m->n;
     hipStream_t s = m->ctx->stream;
@@ -100,11 +227,16 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
             return KCFD_EPIVOT;
         }
     }
-    // the level schedule depends on the structure only: built once
-    if (!m->lvl_rows_l) {
+    // ordering and level schedule depend on the structure only: built once
+    // per ordering
+    if (!m->lvl_rows_l || m->ilu_built_order != m->ilu_order) {
+        hipFree(m->lvl_rows_l);
+        hipFree(m->lvl_rows_u);
+        m->lvl_rows_l = m->lvl_rows_u = nullptr;
+        KCFD_CHECK(build_order(m));
         std::vector<int> rl, ru;
-        level_schedule(m->h_indptr, m->h_indices, m->h_diagpos, false, rl, m->h_lvl_ptr_l);
-        level_schedule(m->h_indptr, m->h_indices, m->h_diagpos, true, ru, m->h_lvl_ptr_u);
+        level_schedule(m->hf_indptr, m->hf_indices, m->hf_diagpos, false, rl, m->h_lvl_ptr_l);
+        level_schedule(m->hf_indptr, m->hf_indices, m->hf_diagpos, true, ru, m->h_lvl_ptr_u);
         m->nlev_l = (int)m->h_lvl_ptr_l.size() - 1;
         m->nlev_u = (int)m->h_lvl_ptr_u.size() - 1;
         KCFD_HIP(hipMalloc(&m->lvl_rows_l, sizeof(int) * std::max(n, 1)));
@@ -116,12 +248,15 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
     }
     const std::vector<int> &pl = m->h_lvl_ptr_l;
     if (!m->lu) KCFD_HIP(hipMalloc(&m->lu, sizeof(double) * std::max(m->nnz, 1)));
-    if (m->nnz)
+    if (m->nnz && !m->perm)
         KCFD_HIP(hipMemcpyAsync(m->lu, m->val, sizeof(double) * m->nnz, hipMemcpyDeviceToDevice, s));
+    else if (m->nnz)
+        hipLaunchKernelGGL(k_gather_vals, dim3((m->nnz + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                           m->nnz, m->f_vmap, m->val, m->lu);
     for (int l = 0; l < m->nlev_l; ++l) {
         const int cnt = pl[l + 1] - pl[l];
         hipLaunchKernelGGL(k_ilu0_level, dim3((cnt + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                           m->lvl_rows_l + pl[l], cnt, m->indptr, m->indices, m->diagpos, m->lu);
+                           m->lvl_rows_l + pl[l], cnt, m->f_indptr, m->f_indices, m->f_diagpos, m->lu);
     }
     KCFD_HIP(hipGetLastError());
     if (want32 && m->nnz) {
@@ -139,37 +274,44 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
     KCFD_HIP(hipMemcpyAsync(bad, &init, sizeof(int), hipMemcpyHostToDevice, s));
     if (n)
         hipLaunchKernelGGL(k_pivot_check, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
-                           m->diagpos, m->lu, bad);
+                           m->f_diagpos, m->lu, bad);
     int first = n;
     KCFD_HIP(hipMemcpyAsync(&first, bad, sizeof(int), hipMemcpyDeviceToHost, s));
     KCFD_HIP(hipStreamSynchronize(s));
     if (first < n) {
-        set_error("ILU(0): zero or NaN pivot at row %d", first);
+        set_error("ILU(0): zero or NaN pivot at row %d", m->perm ? m->h_perm[first] : first);
         return KCFD_EPIVOT;
     }
     return KCFD_OK;
 }
 
-// one level of y = L^{-1} z (unit lower) or w = U^{-1} y, rows of that level
+// One level of the ILU(0) apply, rows i (positions in P A P^T) of that level.
+//   lower: t_i = z_{perm i} - sum_{j<i} l_ij t_j          (t in P-order)
+//   upper: w_{perm i} = (t_i - sum_{j>i} u_ij w_{perm j}) / u_ii  (w in A's order)
+// perm null is the identity; uix[q] = perm[ix[q]].
 __global__ __launch_bounds__(kBlock) void k_trsv_level(const int *__restrict__ rows, int count,
                                                        const int *__restrict__ ip,
                                                        const int *__restrict__ ix,
+                                                       const int *__restrict__ uix,
                                                        const int *__restrict__ dpos,
+                                                       const int *__restrict__ perm,
                                                        const double *__restrict__ lu,
-                                                       const double *z,  // may alias y (U solve in place)
-                                                       double *y, int upper) {
+                                                       const double *z, double *t, double *w,
+                                                       int upper) {
 #pragma clang fp contract(off)
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
-    const int i = rows[t];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int i = rows[k];
+    const int r = perm ? perm[i] : i;
     const int dp = dpos[i];
-    double acc = z[i];
     if (!upper) {
-        for (int q = ip[i]; q < dp; ++q) acc -= lu[q] * y[ix[q]];
-        y[i] = acc;
+        double acc = z[r];
+        for (int q = ip[i]; q < dp; ++q) acc -= lu[q] * t[ix[q]];
+        t[i] = acc;
     } else {
-        for (int q = dp + 1; q < ip[i + 1]; ++q) acc -= lu[q] * y[ix[q]];
-        y[i] = acc / lu[dp];
+        double acc = t[i];
+        for (int q = dp + 1; q < ip[i + 1]; ++q) acc -= lu[q] * w[uix[q]];
+        w[r] = acc / lu[dp];
     }
 }
 
@@ -201,18 +343,20 @@ extern "C" int kcfd_precond_apply(kcfd_mat *m, const double *x, double *y) {
     if (m->pc_kind == KCFD_PC_JACOBI) {
         hipLaunchKernelGGL(k_jacobi_apply, dim3(nb), dim3(kBlock), 0, s, n, m->dinv, x, y);
     } else if (m->pc_kind == KCFD_PC_ILU0) {
-        // level by level: y = L^{-1} x, then y = U^{-1} y in place
+        // level by level: t = L^{-1} P x, then y = P^T U^{-1} t
+        KCFD_CHECK(ensure_scratch(m->ctx, sizeof(double) * n));
+        double *t = (double *)m->ctx->scratch;
         for (int l = 0; l < m->nlev_l; ++l) {
             const int c = m->h_lvl_ptr_l[l + 1] - m->h_lvl_ptr_l[l];
             hipLaunchKernelGGL(k_trsv_level, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                               m->lvl_rows_l + m->h_lvl_ptr_l[l], c, m->indptr, m->indices,
-                               m->diagpos, m->lu, x, y, 0);
+                               m->lvl_rows_l + m->h_lvl_ptr_l[l], c, m->f_indptr, m->f_indices,
+                               m->f_uix, m->f_diagpos, m->perm, m->lu, x, t, y, 0);
         }
         for (int l = 0; l < m->nlev_u; ++l) {
             const int c = m->h_lvl_ptr_u[l + 1] - m->h_lvl_ptr_u[l];
             hipLaunchKernelGGL(k_trsv_level, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                               m->lvl_rows_u + m->h_lvl_ptr_u[l], c, m->indptr, m->indices,
-                               m->diagpos, m->lu, y, y, 1);
+                               m->lvl_rows_u + m->h_lvl_ptr_u[l], c, m->f_indptr, m->f_indices,
+                               m->f_uix, m->f_diagpos, m->perm, m->lu, x, t, y, 1);
         }
     } else {
         hipLaunchKernelGGL(k_copy, dim3(nb), dim3(kBlock), 0, s, n, x, y);
@@ -236,5 +380,30 @@ extern "C" int kcfd_ilu0_get(kcfd_mat *m, double *lu, int *levels_l, int *levels
     }
     if (levels_l) *levels_l = m->nlev_l;
     if (levels_u) *levels_u = m->nlev_u;
+    return KCFD_OK;
+}
+
+extern "C" int kcfd_precond_set_order(kcfd_mat *m, int order) {
+    KCFD_ARG(m != nullptr, "null matrix");
+    KCFD_ARG(order == KCFD_ORDER_NATURAL || order == KCFD_ORDER_COLOR, "unknown ILU(0) ordering");
+    std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
+    m->ilu_order = order;
+    return KCFD_OK;
+}
+
+extern "C" int kcfd_ilu0_order(kcfd_mat *m, int *perm, int *ncolors) {
+    KCFD_ARG(m != nullptr, "null matrix");
+    std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
+    if (!m->lu) {
+        set_error("ILU(0) factors not set up");
+        return KCFD_ESTATE;
+    }
+    if (perm) {
+        if (m->h_perm.empty())
+            for (int i = 0; i < m->n; ++i) perm[i] = i;
+        else
+            std::copy(m->h_perm.begin(), m->h_perm.end(), perm);
+    }
+    if (ncolors) *ncolors = m->ncolors;
     return KCFD_OK;
 }

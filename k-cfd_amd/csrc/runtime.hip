@@ -117,6 +117,7 @@ int mat_free_precond(kcfd_mat *m) {
     hipFree(m->lu32); m->lu32 = nullptr;
     hipFree(m->lvl_rows_l); m->lvl_rows_l = nullptr;
     hipFree(m->lvl_rows_u); m->lvl_rows_u = nullptr;
+    ilu_free_order(m);
     m->h_lvl_ptr_l.clear();
     m->h_lvl_ptr_u.clear();
     m->nlev_l = m->nlev_u = 0;
@@ -425,6 +426,7 @@ int kcfd_mat_bytes(kcfd_mat *m, long long *bytes) {
     if (m->dinv) b += 8LL * m->n;
     if (m->dinv32) b += 4LL * m->n;
     if (m->lu) b += 8LL * m->nnz + 8LL * m->n;
+    if (m->perm) b += 4LL * (m->n + 1 + m->n + m->n) + 12LL * m->nnz;
     if (m->lu32) b += 4LL * m->nnz;
     *bytes = b;
     return KCFD_OK;
@@ -443,7 +445,7 @@ int kcfd_precond_setup(kcfd_mat *m, int kind, int precision) {
     // Same kind and precision as last time: only the values changed, so the
     // buffers (and any solver workspace that points at them) are reused.
     const bool same = m->pc_kind == kind && m->have32 == want32 &&
-                      (kind != KCFD_PC_ILU0 || m->lu != nullptr) &&
+                      (kind != KCFD_PC_ILU0 || (m->lu != nullptr && m->ilu_built_order == m->ilu_order)) &&
                       (kind != KCFD_PC_JACOBI || m->dinv != nullptr);
     if (!same) {
         KCFD_HIP(hipStreamSynchronize(s));

@@ -111,17 +111,32 @@ class DeviceMatrix:
         return b.value
 
     # solver ----------------------------------------------------------------
-    def setup_precond(self, kind="jacobi", precision="fp64"):
+    def setup_precond(self, kind="jacobi", precision="fp64", order=None):
+        """Set up the preconditioner from the current values.  ``order``
+        ('natural' or 'color') selects the ILU(0) row ordering; None keeps
+        the matrix's current one (natural by default)."""
         k = L.PRECONDS[kind] if isinstance(kind, str) else int(kind)
         p = {"fp64": L.FP64, "mixed": L.MIXED}[precision] if isinstance(precision, str) else int(precision)
+        if order is not None:
+            o = L.ORDERS[order] if isinstance(order, str) else int(order)
+            check(lib().kcfd_precond_set_order(self.h, o))
         check(lib().kcfd_precond_setup(self.h, k, p))
         self.pc = (k, p)
 
     def ilu0_factors(self):
+        """(lu on the pattern of P A P^T, levels of L, levels of U)."""
         lu = np.empty(self.nnz)
         a, b = ctypes.c_int(), ctypes.c_int()
         check(lib().kcfd_ilu0_get(self.h, ptr(lu), ctypes.byref(a), ctypes.byref(b)))
         return lu, a.value, b.value
+
+    def ilu0_order(self):
+        """(perm, ncolors) of the current ILU(0) factors: perm[k] is the row of
+        A at position k of P A P^T (identity and 0 colours when natural)."""
+        perm = np.empty(self.n, np.int32)
+        nc = ctypes.c_int()
+        check(lib().kcfd_ilu0_order(self.h, ptr(perm), ctypes.byref(nc)))
+        return perm, nc.value
 
     def spmv(self, x_dev: DevBuf, y_dev: DevBuf):
         check(lib().kcfd_mat_spmv(self.h, x_dev.p, y_dev.p))

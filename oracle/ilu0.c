@@ -69,3 +69,54 @@ void kcfd_oracle_ilu0_solve(int n, const int *indptr, const int *indices,
         x[i] = s / lu[diagpos[i]];
     }
 }
+
+/*
+ * First-fit colouring of the graph of A + A^T (the ILU(0) "color" ordering
+ * of the GPU path).  Rows are visited in ascending order; a row takes the
+ * smallest colour that no already coloured neighbour (row or column
+ * neighbour) has.  perm receives the rows sorted by colour, ascending row id
+ * within a colour; colors (may be NULL) the colour of each row.  Returns the
+ * number of colours, or -1 on allocation failure.
+ */
+int kcfd_oracle_color_order(int n, const int *indptr, const int *indices, int *perm, int *colors)
+{
+    int nnz = n > 0 ? indptr[n] : 0;
+    int *tp = (int *)calloc((size_t)n + 1, sizeof(int));
+    int *tx = (int *)malloc(sizeof(int) * (size_t)(nnz > 0 ? nnz : 1));
+    int *fill = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *col = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *seen = (int *)malloc(sizeof(int) * (size_t)(n + 1));  /* seen[c] == i: c is taken */
+    int ncol = 0;
+    if (!tp || !tx || !fill || !col || !seen) { ncol = -1; goto out; }
+    for (int q = 0; q < nnz; ++q) tp[indices[q] + 1]++;
+    for (int j = 0; j < n; ++j) tp[j + 1] += tp[j];
+    for (int j = 0; j < n; ++j) fill[j] = tp[j];
+    for (int i = 0; i < n; ++i)
+        for (int q = indptr[i]; q < indptr[i + 1]; ++q) tx[fill[indices[q]]++] = i;
+    for (int c = 0; c <= n; ++c) seen[c] = -1;
+    for (int i = 0; i < n; ++i) col[i] = -1;
+    for (int i = 0; i < n; ++i) {
+        for (int q = indptr[i]; q < indptr[i + 1]; ++q) {
+            int j = indices[q];
+            if (j != i && col[j] >= 0) seen[col[j]] = i;
+        }
+        for (int q = tp[i]; q < tp[i + 1]; ++q) {
+            int j = tx[q];
+            if (j != i && col[j] >= 0) seen[col[j]] = i;
+        }
+        int c = 0;
+        while (seen[c] == i) ++c;
+        col[i] = c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    {
+        int k = 0;
+        for (int c = 0; c < ncol; ++c)
+            for (int i = 0; i < n; ++i)
+                if (col[i] == c) perm[k++] = i;
+    }
+    if (colors) for (int i = 0; i < n; ++i) colors[i] = col[i];
+out:
+    free(tp); free(tx); free(fill); free(col); free(seen);
+    return ncol;
+}

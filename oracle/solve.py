@@ -9,7 +9,8 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 * ``make_preconditioner`` restates ``create_preconditioner``
   (core/preconditioner.py:40-54): Jacobi ``where(|diag|>1e-30, 1/diag, 0)``
   and scipy ``spilu(A.tocsc(), fill_factor=1, drop_tol=0)``; ``'ilu0_struct'``
-  is the textbook ILU(0) of ilu0.c that the GPU implements.
+  is the textbook ILU(0) of ilu0.c that the GPU implements, ``'ilu0_color'``
+  the same on P A P^T for the first-fit colour ordering.
 """
 
 from __future__ import annotations
@@ -38,6 +39,8 @@ def _lib():
         lib.kcfd_oracle_ilu0_factor.restype = ctypes.c_int
         lib.kcfd_oracle_ilu0_solve.argtypes = [ctypes.c_int, P, P, P, P, P, P]
         lib.kcfd_oracle_ilu0_solve.restype = None
+        lib.kcfd_oracle_color_order.argtypes = [ctypes.c_int, P, P, P, P]
+        lib.kcfd_oracle_color_order.restype = ctypes.c_int
         _LIB = lib
     return _LIB
 
@@ -107,8 +110,45 @@ def ilu0_python(A):
     return lu, dpos
 
 
+def color_order(A):
+    """First-fit colouring of A + A^T (ilu0.c): (perm sorted by colour,
+    colours per row, number of colours)."""
+    A = sparse.csr_matrix(A)
+    A.sort_indices()
+    n = A.shape[0]
+    ip = np.ascontiguousarray(A.indptr, np.int32)
+    ix = np.ascontiguousarray(A.indices, np.int32)
+    perm = np.empty(n, np.int32)
+    col = np.empty(n, np.int32)
+    nc = _lib().kcfd_oracle_color_order(n, _p(ip), _p(ix), _p(perm), _p(col))
+    if nc < 0:
+        raise MemoryError("colour ordering")
+    return perm, col, nc
+
+
+def permuted(A, perm):
+    """P A P^T as canonical CSR (row k = row perm[k] of A)."""
+    A = sparse.csr_matrix(A)
+    Ap = sparse.csr_matrix(A[perm][:, perm])
+    Ap.sort_indices()
+    return Ap
+
+
+class ILU0Ordered:
+    """M^{-1} z = P^T ILU0(P A P^T)^{-1} P z for a row ordering perm."""
+
+    def __init__(self, A, perm):
+        self.perm = np.asarray(perm, np.int64)
+        self.inv = np.empty_like(self.perm)
+        self.inv[self.perm] = np.arange(len(self.perm))
+        self.f = ILU0(permuted(A, self.perm))
+
+    def solve(self, z):
+        return self.f.solve(np.asarray(z, np.float64)[self.perm])[self.inv]
+
+
 def make_preconditioner(A, method: str):
-    """core/preconditioner.py:40-54 (+ 'ilu0_struct')."""
+    """core/preconditioner.py:40-54 (+ 'ilu0_struct', 'ilu0_color')."""
     n = A.shape[0]
     if method == "none":
         return None
@@ -122,6 +162,9 @@ def make_preconditioner(A, method: str):
         return LinearOperator((n, n), matvec=ilu.solve)
     if method == "ilu0_struct":
         f = ILU0(A)
+        return LinearOperator((n, n), matvec=f.solve)
+    if method == "ilu0_color":
+        f = ILU0Ordered(A, color_order(A)[0])
         return LinearOperator((n, n), matvec=f.solve)
     raise ValueError(f"Unknown preconditioner: {method}")
 
