@@ -81,6 +81,8 @@ def main():
                     help="mixed: reliable-update trigger (FP32 residual reduction)")
     ap.add_argument("--order", default="color", choices=["natural", "color"],
                     help="ILU(0) row ordering (colour: first-fit colouring, 2 levels per solve)")
+    ap.add_argument("--grid-per-cu", type=int, default=0, help="launch shape (0: library default)")
+    ap.add_argument("--trsv-per-cu", type=int, default=0, help="launch shape (0: library default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check-every", type=int, default=8,
                     help="iterations per captured graph between host status checks")
@@ -124,7 +126,8 @@ def main():
     precs = [p for p in args.precisions.split(",") if p]
     for prec in precs:
         opts = device.default_opts(rtol=1e-8, precond=args.precond, precision=prec,
-                                   inner_rtol=args.delta, check_every=args.check_every)
+                                   inner_rtol=args.delta, check_every=args.check_every,
+                                   grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu)
         lat, rs = [], []
 
         def step():

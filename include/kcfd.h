@@ -152,8 +152,10 @@ typedef struct kcfd_solve_opts {
                            factor since the last update (default 0.1) */
     int max_refine;     /* MIXED: cap on reliable updates (default 1000) */
     int check_every;    /* iterations per captured graph between host checks */
-    int rows_per_thread;/* SpMV rows per lane group (0: autotuned default) */
-    int lanes_per_row;  /* 1, 2, 4, 8 (0: autotuned default) */
+    int grid_per_cu;    /* launch shape: persistent workgroups (256 lanes) per CU of
+                           the SpMV / vector kernels, 1..8 (0: 8) */
+    int trsv_per_cu;    /* launch shape: workgroups per CU of the ILU(0) solves,
+                           1..4 (0: 4; the solve needs its grid resident) */
 } kcfd_solve_opts;
 
 typedef struct kcfd_solve_result {
@@ -187,10 +189,12 @@ int kcfd_batch_bytes(kcfd_batch *b, long long *bytes);
 int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out);
 /* Measurement hooks (no reference counterpart; the reference times whole
  * solves with time.time, core/gpu_solver.py:202-250).  kcfd_batch_profile
- * re-launches one phase kernel `reps` times on the context stream between
- * HIP events and returns the mean duration: phase 1 = v = A M p with the
- * fused rt.v reduction, 2 = fused x/r update, 3 = ILU(0) lower triangular
- * solve, 4 = p update.  kcfd_batch_kernel_bytes gives that kernel's
+ * runs `reps` solver iterations (after one warm-up) launched directly on the
+ * context stream, with HIP events around one phase kernel in each, and
+ * returns that kernel's mean duration: phase 1 = v = A M p with the fused
+ * rt.v reduction, 2 = fused x/r update, 3 = ILU(0) lower triangular solve,
+ * 4 = p update.  The solver state is restored afterwards; the vectors (x
+ * included) are overwritten.  kcfd_batch_kernel_bytes gives that kernel's
  * algorithmic HBM bytes per launch (summed over the batch). */
 int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms);
 int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes);

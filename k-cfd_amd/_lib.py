@@ -40,8 +40,8 @@ class SolveOpts(ctypes.Structure):
     _fields_ = [("rtol", ctypes.c_double), ("atol", ctypes.c_double), ("maxiter", ctypes.c_int),
                 ("precond", ctypes.c_int), ("precision", ctypes.c_int),
                 ("inner_rtol", ctypes.c_double), ("max_refine", ctypes.c_int),
-                ("check_every", ctypes.c_int), ("rows_per_thread", ctypes.c_int),
-                ("lanes_per_row", ctypes.c_int)]
+                ("check_every", ctypes.c_int), ("grid_per_cu", ctypes.c_int),
+                ("trsv_per_cu", ctypes.c_int)]
 
 
 class SolveResult(ctypes.Structure):

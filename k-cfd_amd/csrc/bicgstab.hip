@@ -140,7 +140,12 @@ __device__ __forceinline__ double precond_at(const SysDesc &d, const V *__restri
     return (double)z[j];
 }
 
-// (A M z)_i from the SELL-64 copy, entries in CSR order
+// (A M z)_i from the SELL-64 copy, entries in CSR order.  A wave's 64 rows
+// are one slice, so the width w is wave-uniform; entries are taken in groups
+// of 8 whose index and value loads, then gathers, are all issued before the
+// (in-order) accumulation, keeping 8 loads per lane in flight.
+constexpr int kRowUnroll = 8;
+
 template <typename V, int PC>
 __device__ __forceinline__ double row_amz(const SysDesc &d, int i, const V *__restrict__ z,
                                           const V *__restrict__ zh) {
@@ -150,9 +155,22 @@ __device__ __forceinline__ double row_amz(const SysDesc &d, int i, const V *__re
     const int s0 = d.sptr[s], w = (d.sptr[s + 1] - s0) >> 6;
     const int base = s0 + (i & 63);
     double acc = 0.0;
-    for (int k = 0; k < w; ++k) {
-        const int q = base + 64 * k;
-        acc += (double)a[q] * precond_at<V, PC>(d, z, zh, d.six[q]);
+    for (int k0 = 0; k0 < w; k0 += kRowUnroll) {
+        int col[kRowUnroll];
+        V av[kRowUnroll];
+        double zv[kRowUnroll];
+#pragma unroll
+        for (int k = 0; k < kRowUnroll; ++k)
+            if (k0 + k < w) {
+                col[k] = d.six[base + 64 * (k0 + k)];
+                av[k] = a[base + 64 * (k0 + k)];
+            }
+#pragma unroll
+        for (int k = 0; k < kRowUnroll; ++k)
+            if (k0 + k < w) zv[k] = precond_at<V, PC>(d, z, zh, col[k]);
+#pragma unroll
+        for (int k = 0; k < kRowUnroll; ++k)
+            if (k0 + k < w) acc += (double)av[k] * zv[k];
     }
     return acc;
 }
@@ -164,9 +182,21 @@ __device__ __forceinline__ double row_ax64(const SysDesc &d, int i, const double
     const int s0 = d.sptr[s], w = (d.sptr[s + 1] - s0) >> 6;
     const int base = s0 + (i & 63);
     double acc = 0.0;
-    for (int k = 0; k < w; ++k) {
-        const int q = base + 64 * k;
-        acc += d.sa64[q] * x[d.six[q]];
+    for (int k0 = 0; k0 < w; k0 += kRowUnroll) {
+        int col[kRowUnroll];
+        double av[kRowUnroll], xv[kRowUnroll];
+#pragma unroll
+        for (int k = 0; k < kRowUnroll; ++k)
+            if (k0 + k < w) {
+                col[k] = d.six[base + 64 * (k0 + k)];
+                av[k] = d.sa64[base + 64 * (k0 + k)];
+            }
+#pragma unroll
+        for (int k = 0; k < kRowUnroll; ++k)
+            if (k0 + k < w) xv[k] = x[col[k]];
+#pragma unroll
+        for (int k = 0; k < kRowUnroll; ++k)
+            if (k0 + k < w) acc += av[k] * xv[k];
     }
     return acc;
 }
@@ -644,22 +674,49 @@ __global__ __launch_bounds__(kBlock) void f_true_resid(const SysDesc *__restrict
 //   LOWER: y_i = z_{perm i} - sum_{j<i} l_ij y_j                   (unit L)
 //   UPPER: w_{perm i} = (y_i - sum_{j>i} u_ij w_{perm j}) / u_ii; y_i reset
 template <typename V>
-__device__ __forceinline__ V poll_ready(const V *p) {
-    using U = typename Sentinel<V>::U;
-    const U *q = reinterpret_cast<const U *>(p);
-    U b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    while (b == Sentinel<V>::bits) {
-        __builtin_amdgcn_s_sleep(2);
-        b = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    return __builtin_bit_cast(V, b);
-}
-
-template <typename V>
 __device__ __forceinline__ void publish(V *p, V v) {
     using U = typename Sentinel<V>::U;
     __hip_atomic_store(reinterpret_cast<U *>(p), __builtin_bit_cast(U, v), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// acc - sum_{q0<=q<q1} a_q vec[cols_q], subtracting in q order.  Groups of 8
+// entries: their indices and values, then their (polled) operands are all
+// requested before the first subtraction, so a row costs about one memory
+// round trip per group instead of one per entry.
+template <typename V>
+__device__ __forceinline__ V sub_ready(V acc, const V *__restrict__ a, const int *__restrict__ cols,
+                                       int q0, int q1, const V *vec) {
+#pragma clang fp contract(off)
+    using U = typename Sentinel<V>::U;
+    constexpr int G = 8;
+    for (int qb = q0; qb < q1; qb += G) {
+        int c[G];
+        V av[G];
+        U bits[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (qb + k < q1) {
+                c[k] = cols[qb + k];
+                av[k] = a[qb + k];
+            }
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (qb + k < q1)
+                bits[k] = __hip_atomic_load(reinterpret_cast<const U *>(vec + c[k]), __ATOMIC_RELAXED,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (qb + k < q1) {
+                while (bits[k] == Sentinel<V>::bits) {
+                    __builtin_amdgcn_s_sleep(2);
+                    bits[k] = __hip_atomic_load(reinterpret_cast<const U *>(vec + c[k]), __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+                }
+                acc -= av[k] * __builtin_bit_cast(V, bits[k]);
+            }
+    }
+    return acc;
 }
 
 template <typename V, bool UPPER, int WHICH>
@@ -692,14 +749,12 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
             if (!UPPER) {
                 const V *z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
                 V *y = (V *)d.y;
-                V acc = z[r];
-                for (int q = d.ip[i]; q < dp; ++q) acc -= a[q] * poll_ready<V>(y + d.ix[q]);
+                const V acc = sub_ready<V>(z[r], a, d.ix, d.ip[i], dp, y);
                 publish<V>(y + i, acc);
             } else {
                 V *y = (V *)d.y;
                 V *w = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
-                V acc = y[i];
-                for (int q = dp + 1; q < d.ip[i + 1]; ++q) acc -= a[q] * poll_ready<V>(w + d.uix[q]);
+                const V acc = sub_ready<V>(y[i], a, d.uix, dp + 1, d.ip[i + 1], w);
                 publish<V>(w + r, acc / a[dp]);
                 y[i] = sentinel_value<V>();
             }
@@ -742,6 +797,9 @@ struct kcfd_batch {
     BatchParams *d_params = nullptr;   // read by the kernels: the captured graph stays valid
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
+    // kcfd_batch_profile: launch_iteration records ctx->ev0 / ev1 around the
+    // kernel of this phase (0: none, as during graph capture)
+    int timed_phase = 0;
 };
 
 namespace {
@@ -752,16 +810,25 @@ template <typename V, int PC>
 void launch_iteration(kcfd_batch *b, hipStream_t s) {
     const int g = b->grid, ns = b->nsys;
     const dim3 blk(kBlock);
+    auto mark = [&](int phase, hipEvent_t ev) {
+        if (b->timed_phase == phase) (void)hipEventRecord(ev, s);
+    };
+    mark(4, b->ctx->ev0);
     hipLaunchKernelGGL((k_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles,
                        b->d_counters);
+    mark(4, b->ctx->ev1);
     if (PC == KCFD_PC_ILU0) {
+        mark(3, b->ctx->ev0);
         hipLaunchKernelGGL((k_trsv<V, false, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
                            b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l);
+        mark(3, b->ctx->ev1);
         hipLaunchKernelGGL((k_trsv<V, true, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
                            b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u);
     }
+    mark(1, b->ctx->ev0);
     hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->ntiles, b->d_partials);
+    mark(1, b->ctx->ev1);
     hipLaunchKernelGGL(f_spmv_p, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     hipLaunchKernelGGL((k_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles,
                        b->d_partials, b->d_counters);
@@ -775,8 +842,10 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     hipLaunchKernelGGL((k_spmv_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->ntiles, b->d_partials, b->d_params);
     hipLaunchKernelGGL(f_spmv_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
+    mark(2, b->ctx->ev0);
     hipLaunchKernelGGL((k_xr<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->ntiles, b->d_partials, b->d_params);
+    mark(2, b->ctx->ev1);
     hipLaunchKernelGGL(f_xr, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
 }
 
@@ -851,6 +920,8 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
     KCFD_ARG(pc == KCFD_PC_NONE || pc == KCFD_PC_JACOBI || pc == KCFD_PC_ILU0,
              "unknown preconditioner");
     KCFD_ARG(prec == KCFD_FP64 || prec == KCFD_MIXED, "unknown precision");
+    KCFD_ARG(o->grid_per_cu >= 0 && o->grid_per_cu <= 8, "grid_per_cu must be 0..8");
+    KCFD_ARG(o->trsv_per_cu >= 0 && o->trsv_per_cu <= 4, "trsv_per_cu must be 0..4");
     kcfd_batch *b = new kcfd_batch();
     b->ctx = ctx;
     b->nsys = nsys;
@@ -905,7 +976,8 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
     }
     b->ntiles = (int)tiles.size();
     // persistent grid: at most 4 workgroups per CU, a multiple of the XCD count
-    const int cap = std::max(kXcds, (ctx->cus * 4 / kXcds) * kXcds);
+    const int gpc = o->grid_per_cu > 0 ? o->grid_per_cu : 8;
+    const int cap = std::max(kXcds, (ctx->cus * gpc / kXcds) * kXcds);
     b->grid = std::min(xcd_grid(b->ntiles), cap);
     KCFD_HIP(hipMalloc(&b->slab, slab ? slab : 256));
     b->slab_bytes = slab;
@@ -987,7 +1059,7 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
         // at most 4 workgroups (16 waves) per CU: the kernel's occupancy limit
         // is 32 waves per CU, so the whole grid is resident
         const int want_waves = std::max(64, 3 * widest);
-        const int tcap = ctx->cus * 4;
+        const int tcap = ctx->cus * (o->trsv_per_cu > 0 ? o->trsv_per_cu : 4);
         b->trsv_grid = std::max(1, std::min((want_waves + kWaves - 1) / kWaves, tcap));
     }
     *out = b;
@@ -1086,34 +1158,6 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     return KCFD_OK;
 }
 
-template <typename V, int PC>
-void launch_phase(kcfd_batch *b, int phase, hipStream_t s) {
-    const dim3 blk(kBlock);
-    if (phase == 1)
-        hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state,
-                           b->d_tiles, b->ntiles, b->d_partials);
-    else if (phase == 2)
-        hipLaunchKernelGGL((k_xr<V, PC>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                           b->ntiles, b->d_partials, b->d_params);
-    else if (phase == 4)
-        hipLaunchKernelGGL((k_p<V, PC>), dim3(b->grid), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                           b->ntiles, b->d_counters);
-}
-
-void launch_phase_any(kcfd_batch *b, int phase, hipStream_t s) {
-    const bool f64 = b->bp.precision == KCFD_FP64;
-    switch (b->bp.precond) {
-        case KCFD_PC_JACOBI:
-            f64 ? launch_phase<double, KCFD_PC_JACOBI>(b, phase, s) : launch_phase<float, KCFD_PC_JACOBI>(b, phase, s);
-            break;
-        case KCFD_PC_ILU0:
-            f64 ? launch_phase<double, KCFD_PC_ILU0>(b, phase, s) : launch_phase<float, KCFD_PC_ILU0>(b, phase, s);
-            break;
-        default:
-            f64 ? launch_phase<double, KCFD_PC_NONE>(b, phase, s) : launch_phase<float, KCFD_PC_NONE>(b, phase, s);
-    }
-}
-
 }  // namespace
 
 extern "C" {
@@ -1187,48 +1231,44 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     KCFD_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
     KCFD_HIP(hipStreamSynchronize(s));
+    // Full iterations, launched directly (not from the graph) so that events
+    // can bracket the one kernel: it runs between the same kernels as in a
+    // solve, with the caches in the state they leave.  Every system is put
+    // back into S_RUN with its saved scalars before each iteration (a
+    // converged system would otherwise skip the work); the vectors are
+    // scratch afterwards and the states are restored at the end.
     std::vector<SysState> saved(b->nsys), run(b->nsys);
     KCFD_HIP(hipMemcpy(saved.data(), b->d_state, sizeof(SysState) * b->nsys, hipMemcpyDeviceToHost));
     for (int k = 0; k < b->nsys; ++k) {
         run[k] = saved[k];
         run[k].status = S_RUN;
         if (run[k].omega == 0.0) run[k].omega = 1.0;
+        if (run[k].rho == 0.0) run[k].rho = 1.0;
     }
-    KCFD_HIP(hipMemcpy(b->d_state, run.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice));
-    const bool f64 = b->bp.precision == KCFD_FP64;
-    if (phase == 3) {
-        // the L solve needs a sentinel-filled output and a zero counter before
-        // every launch; only the solves are timed
-        float tot = 0.f;
-        for (int r = 0; r < reps; ++r) {
-            KCFD_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned) * 8, s));
-            for (int k = 0; k < b->nsys; ++k) {
-                const SysDesc &d = b->h_desc[k];
-                const unsigned nb = (unsigned)std::max(1, (d.n + kBlock - 1) / kBlock);
-                if (f64) hipLaunchKernelGGL(k_fill_sentinel<double>, dim3(nb), dim3(kBlock), 0, s, (double *)d.y, d.n);
-                else hipLaunchKernelGGL(k_fill_sentinel<float>, dim3(nb), dim3(kBlock), 0, s, (float *)d.y, d.n);
-            }
-            KCFD_HIP(hipEventRecord(b->ctx->ev0, s));
-            if (f64) hipLaunchKernelGGL((k_trsv<double, false, 0>), dim3(b->trsv_grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l);
-            else hipLaunchKernelGGL((k_trsv<float, false, 0>), dim3(b->trsv_grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l);
-            KCFD_HIP(hipEventRecord(b->ctx->ev1, s));
-            KCFD_HIP(hipEventSynchronize(b->ctx->ev1));
-            float ms = 0.f;
-            KCFD_HIP(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
-            tot += ms;
+    const LaunchFn it = pick_iter(b->bp.precision, b->bp.precond);
+    KCFD_HIP(hipMemcpy(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice));
+    double tot = 0.0;
+    int rc = KCFD_OK;
+    b->timed_phase = phase;
+    for (int r = 0; r <= reps && rc == KCFD_OK; ++r) {   // r == 0 warms up
+        hipError_t e = hipMemcpy(b->d_state, run.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            it(b, s);
+            e = hipEventSynchronize(b->ctx->ev1);
         }
-        *avg_ms = tot / reps;
-    } else {
-        launch_phase_any(b, phase, s);  // warm
-        KCFD_HIP(hipEventRecord(b->ctx->ev0, s));
-        for (int r = 0; r < reps; ++r) launch_phase_any(b, phase, s);
-        KCFD_HIP(hipEventRecord(b->ctx->ev1, s));
-        KCFD_HIP(hipEventSynchronize(b->ctx->ev1));
         float ms = 0.f;
-        KCFD_HIP(hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1));
-        *avg_ms = ms / reps;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1);
+        if (e == hipSuccess) e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error("kcfd_batch_profile: %s", hipGetErrorString(e));
+            rc = KCFD_EHIP;
+        }
+        if (r > 0) tot += ms;
     }
-    KCFD_HIP(hipGetLastError());
+    b->timed_phase = 0;
+    KCFD_CHECK(rc);
+    *avg_ms = tot / reps;
+    KCFD_HIP(hipStreamSynchronize(s));
     KCFD_HIP(hipMemcpy(b->d_state, saved.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice));
     return KCFD_OK;
 }
@@ -1261,10 +1301,11 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
         return KCFD_OK;
     }
     // the cached single-system workspace is reused while the preconditioner,
-    // precision and the b / x buffers are unchanged
+    // precision, launch shape and the b / x buffers are unchanged
     bool reuse = m->solver != nullptr && m->solver->bp.precond == opts->precond &&
                  m->solver->bp.precision == opts->precision && m->solver->h_desc[0].b == bvec &&
-                 m->solver->h_desc[0].x == x;
+                 m->solver->h_desc[0].x == x && m->solver->opts.grid_per_cu == opts->grid_per_cu &&
+                 m->solver->opts.trsv_per_cu == opts->trsv_per_cu;
     if (!reuse) {
         if (m->solver) kcfd_batch_destroy(m->solver);
         m->solver = nullptr;

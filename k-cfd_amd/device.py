@@ -24,7 +24,8 @@ def _f64(a):
 
 
 def default_opts(rtol=1e-8, atol=0.0, maxiter=0, precond="jacobi", precision="fp64",
-                 inner_rtol=0.1, max_refine=1000, check_every=8) -> L.SolveOpts:
+                 inner_rtol=0.1, max_refine=1000, check_every=8, grid_per_cu=0,
+                 trsv_per_cu=0) -> L.SolveOpts:
     o = L.SolveOpts()
     o.rtol = float(rtol)
     o.atol = float(atol)
@@ -35,6 +36,8 @@ def default_opts(rtol=1e-8, atol=0.0, maxiter=0, precond="jacobi", precision="fp
     o.inner_rtol = float(inner_rtol)
     o.max_refine = int(max_refine)
     o.check_every = int(check_every)
+    o.grid_per_cu = int(grid_per_cu)
+    o.trsv_per_cu = int(trsv_per_cu)
     return o
 
 
