@@ -66,17 +66,22 @@ def cpu_baseline(A, b, precond, iters_needed, budget_s=15.0):
     return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": (f"oracle {precond} set-up ({t_setup:.2f} s) + {k} scipy bicgstab iterations "
                        f"({t_iter * 1e3:.1f} ms/iter) on the same {n}-row system, scaled to the "
-                       f"{iters_needed} iterations of the FP64 GPU solve; 1 thread"),
+                       f"{iters_needed} iterations an FP64 solve with this preconditioner takes; 1 thread"),
             "s_per_solve": t_solve}
 
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="pressure", choices=["pressure", "batched"],
+                    help="pressure: one hex n^3 pressure-correction system (default); "
+                         "batched: --nsys uneven quad/hex/tet systems in one batch")
+    ap.add_argument("--nsys", type=int, default=512, help="batched: number of systems")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--precond", default="ilu0", choices=["ilu0", "jacobi"])
+    ap.add_argument("--no-tune", action="store_true", help="batched: library default launch shape")
     ap.add_argument("--delta", type=float, default=0.1,
                     help="mixed: reliable-update trigger (FP32 residual reduction)")
     ap.add_argument("--order", default="color", choices=["natural", "color"],
@@ -94,7 +99,12 @@ def main():
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
                                    "(the multi-GPU sharding design is described in DESIGN.md)"}))
         sys.exit(2)
+    if args.workload == "batched":
+        return bench_batched(args)
+    return bench_pressure(args)
 
+
+def bench_pressure(args):
     import __graft_entry__
     __graft_entry__.load_package()
     from kcfd_amd import _lib as L, device, workloads
@@ -189,7 +199,16 @@ def main():
         try:
             Ah = A.to_scipy()
             bh_ = case.rhs
-            cpu = cpu_baseline(Ah, bh_, args.precond, fp64["iters"], args.cpu_budget)
+            its = fp64["iters"]
+            if args.precond == "ilu0" and args.order != "natural":
+                # the CPU path factors in the natural order, which needs fewer
+                # iterations than the colour ordering: count them on the GPU
+                A.setup_precond("ilu0", "fp64", order="natural")
+                xdev.fill(0.0)
+                its = A.solve_dev(bdev.p, xdev.p, device.default_opts(rtol=1e-8, precond="ilu0"),
+                                  setup=False).iters
+                log(f"natural-order ILU(0) FP64 iterations (CPU baseline scaling): {its}")
+            cpu = cpu_baseline(Ah, bh_, args.precond, its, args.cpu_budget)
             cpu["cores"] = 1
             cpu.pop("s_per_solve", None)
         except Exception as e:  # the baseline is informative; never fail the bench on it
@@ -222,6 +241,138 @@ def main():
         "device": info["name"],
     }
     print(json.dumps(line))
+
+
+
+def bench_batched(args):
+    """Batched workload: --nsys systems, cells log-uniform in [2e3, 2.5e5],
+    50% quad convection-diffusion / 30% hex pressure correction / 20% tet
+    two-fluid momentum (seeds 4000+i), each assembled on the device, solved
+    together to rtol 1e-8 in FP64 and in mixed precision with the autotuned
+    launch shape.  One step = one batched solve of every system from x0 = 0."""
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from kcfd_amd import _lib as L, batch as B, device, tune, workloads
+
+    specs = workloads.batch_specs(args.nsys)
+    t = time.perf_counter()
+    cases = B.build_cases(specs)          # forks builders: before the GPU is touched
+    log(f"built {len(cases)} host cases in {time.perf_counter() - t:.1f}s: {B.specs_summary(specs)}")
+    ctx = L.default_context()
+    info = ctx.info()
+    log("device", info)
+    copy_gbs = ctx.stream_copy_gbs(1 << 30, 10)
+    t = time.perf_counter()
+    sb = B.SystemBatch.assemble(ctx, cases)
+    del cases
+    log(f"assembled {len(sb.mats)} systems ({sb.total_rows} rows) on the device in "
+        f"{time.perf_counter() - t:.1f}s")
+    order = args.order if args.precond == "ilu0" else None
+    results, sols = {}, {}
+    for prec in [p for p in args.precisions.split(",") if p]:
+        t = time.perf_counter()
+        sb.setup(args.precond, prec, order or "natural")
+        log(f"{prec}: preconditioners set up in {time.perf_counter() - t:.1f}s")
+        if args.no_tune:
+            choice = dict(grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu,
+                          check_every=args.check_every)
+        else:
+            choice = tune.autotune(sb, prec, args.precond, order or "natural", delta=args.delta, log=log)
+        log(f"{prec}: launch shape {tune.describe(choice)}")
+        opts = dict(precision=prec, precond=args.precond, order=order or "natural", delta=args.delta,
+                    **choice)
+        bobj = sb.make_batch(device.default_opts(
+            rtol=1e-8, precond=args.precond, precision=prec, inner_rtol=args.delta,
+            check_every=choice["check_every"], grid_per_cu=choice["grid_per_cu"],
+            trsv_per_cu=choice["trsv_per_cu"]))
+        for w in range(args.warmup):
+            r = sb.solve(batch=bobj, **opts)
+            log(f"{prec} warmup {w}: {r.summary()}")
+        runs = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            runs.append(sb.solve(batch=bobj, **opts))
+            log(f"{prec} step: wall {runs[-1].wall_s * 1e3:.1f} ms")
+        wall = time.perf_counter() - t0
+        r = runs[-1]
+        summ = r.summary()
+        summ["solves_per_s"] = round(args.steps * len(sb.mats) / wall, 2)
+        summ["ms_per_step"] = round(1e3 * wall / args.steps, 3)
+        lat = [q.wall_s for q in runs]
+        summ["p50_ms"] = round(1e3 * float(np.percentile(lat, 50)), 3)
+        summ["p99_ms"] = round(1e3 * float(np.percentile(lat, 99)), 3)
+        summ["achieved_gbs"] = round(float(np.mean([q.bytes / (q.device_ms * 1e-3) / 1e9 for q in runs])), 1)
+        # the dominant kernel (SpMV) timed inside real iterations of this batch
+        log(f"{prec}: timing the SpMV inside batch iterations")
+        ms = L.ctypes.c_double()
+        by = L.ctypes.c_double()
+        L.check(L.lib().kcfd_batch_profile(bobj.h, 1, 10, L.ctypes.byref(ms)))
+        L.check(L.lib().kcfd_batch_kernel_bytes(bobj.h, 1, L.ctypes.byref(by)))
+        summ["spmv"] = dict(avg_ms=ms.value, bytes=by.value, gbs=by.value / (ms.value * 1e-3) / 1e9)
+        if args.precond == "ilu0":
+            L.check(L.lib().kcfd_batch_profile(bobj.h, 3, 10, L.ctypes.byref(ms)))
+            L.check(L.lib().kcfd_batch_kernel_bytes(bobj.h, 3, L.ctypes.byref(by)))
+            summ["ilu_lower_solve"] = dict(avg_ms=ms.value, bytes=by.value,
+                                           gbs=by.value / (ms.value * 1e-3) / 1e9)
+        # the profile overwrote x: solve once more for the solutions
+        log(f"{prec}: kernel timing done; solving again for the solutions")
+        sb.solve(batch=bobj, **opts)
+        sols[prec] = sb.solutions()
+        bobj.close()
+        results[prec] = summ
+        log(prec, summ)
+    mixed = results.get("mixed") or next(iter(results.values()))
+    fp64 = results.get("fp64") or next(iter(results.values()))
+    err = None
+    if "mixed" in sols and "fp64" in sols:
+        e = [float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+             for a, b in zip(sols["mixed"], sols["fp64"])]
+        err = dict(max=max(e), median=float(np.median(e)))
+    roof = {"bound": "hbm", "achieved": round(mixed["spmv"]["gbs"], 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(mixed["spmv"]["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "spmv_p (batch)", "solve_avg_gbs": mixed["achieved_gbs"],
+            "stream_copy_gbs": round(copy_gbs, 1)}
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_batch(sb, specs, args.precond, args.cpu_budget)
+    line = {
+        "metric": "solves/s + %HBM roofline at rel-resid 1e-8; p50/p99 latency, mixed vs FP64",
+        "value": mixed["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": mixed["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 refinement",
+        "data": "synthetic",
+        "config": {"workload": f"{len(sb.mats)} uneven systems in one batch (50% quad conv-diff, "
+                               f"30% hex pressure, 20% tet two-fluid; cells log-uniform 2e3..2.5e5), "
+                               f"BiCGSTAB+{args.precond}, rtol 1e-8", "rows": sb.total_rows,
+                   "ilu_order": order, "step": "one batched solve of every system"},
+        "roofline": roof, "cpu_baseline": cpu, "mixed": mixed, "fp64": fp64,
+        "mixed_vs_fp64": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3) if len(results) > 1 else None,
+        "mixed_error_vs_fp64": err, "device": info["name"],
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline_batch(sb, specs, precond, budget_s):
+    """The reference CPU path (scipy bicgstab + the oracle preconditioner, 1
+    thread) on systems of the batch taken in order until the budget is
+    spent; the batch rate is extrapolated by cell count."""
+    from oracle import solve as osolve
+    t_all, cells, done = 0.0, 0, 0
+    for k, m in enumerate(sb.mats):
+        if t_all > budget_s:
+            break
+        A = m.to_scipy()
+        b = sb.bs[k].download()
+        t0 = time.perf_counter()
+        osolve.solve_bicgstab(A, b, rtol=1e-8, precond="ilu0_struct" if precond == "ilu0" else "jacobi")
+        t_all += time.perf_counter() - t0
+        cells += m.n
+        done += 1
+    rate = cells / t_all                       # cells per second
+    est = sb.total_rows / rate                 # seconds for the whole batch
+    return {"value": len(sb.mats) / est, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": (f"{done} of {len(sb.mats)} systems ({cells} cells) solved by scipy bicgstab + "
+                       f"oracle {precond} in {t_all:.1f} s; batch rate extrapolated by cell count; 1 thread")}
 
 
 if __name__ == "__main__":

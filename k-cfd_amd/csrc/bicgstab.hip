@@ -1238,7 +1238,8 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     // converged system would otherwise skip the work); the vectors are
     // scratch afterwards and the states are restored at the end.
     std::vector<SysState> saved(b->nsys), run(b->nsys);
-    KCFD_HIP(hipMemcpy(saved.data(), b->d_state, sizeof(SysState) * b->nsys, hipMemcpyDeviceToHost));
+    KCFD_HIP(hipMemcpyAsync(saved.data(), b->d_state, sizeof(SysState) * b->nsys, hipMemcpyDeviceToHost, s));
+    KCFD_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < b->nsys; ++k) {
         run[k] = saved[k];
         run[k].status = S_RUN;
@@ -1246,15 +1247,19 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
         if (run[k].rho == 0.0) run[k].rho = 1.0;
     }
     const LaunchFn it = pick_iter(b->bp.precision, b->bp.precond);
-    KCFD_HIP(hipMemcpy(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice));
+    KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
     double tot = 0.0;
     int rc = KCFD_OK;
     b->timed_phase = phase;
     for (int r = 0; r <= reps && rc == KCFD_OK; ++r) {   // r == 0 warms up
-        hipError_t e = hipMemcpy(b->d_state, run.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice);
+        // The state upload is ordered on the solver stream and the whole
+        // iteration is waited for: a status changed under a running kernel
+        // could make a triangular solve skip rows that other rows poll.
+        hipError_t e = hipMemcpyAsync(b->d_state, run.data(), sizeof(SysState) * b->nsys,
+                                      hipMemcpyHostToDevice, s);
         if (e == hipSuccess) {
             it(b, s);
-            e = hipEventSynchronize(b->ctx->ev1);
+            e = hipStreamSynchronize(s);
         }
         float ms = 0.f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->ctx->ev0, b->ctx->ev1);
@@ -1268,8 +1273,8 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     b->timed_phase = 0;
     KCFD_CHECK(rc);
     *avg_ms = tot / reps;
+    KCFD_HIP(hipMemcpyAsync(b->d_state, saved.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice, s));
     KCFD_HIP(hipStreamSynchronize(s));
-    KCFD_HIP(hipMemcpy(b->d_state, saved.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice));
     return KCFD_OK;
 }
 
