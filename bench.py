@@ -72,9 +72,15 @@ def cpu_baseline(A, b, precond, iters_needed, budget_s=15.0):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="pressure", choices=["pressure", "batched"],
+    ap.add_argument("--workload", default="pressure", choices=["pressure", "batched", "serve"],
                     help="pressure: one hex n^3 pressure-correction system (default); "
-                         "batched: --nsys uneven quad/hex/tet systems in one batch")
+                         "batched: --nsys uneven quad/hex/tet systems in one batch; "
+                         "serve: Poisson request stream, continuous batching")
+    ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
+    ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
+    ap.add_argument("--templates", type=int, default=8, help="serve: mesh sizes per kind")
+    ap.add_argument("--baseline-duration", type=float, default=10.0,
+                    help="serve: seconds of the same trace served one at a time in FP64")
     ap.add_argument("--nsys", type=int, default=512, help="batched: number of systems")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -101,6 +107,8 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "serve":
+        return bench_serve(args)
     return bench_pressure(args)
 
 
@@ -373,6 +381,75 @@ def cpu_baseline_batch(sb, specs, precond, budget_s):
     return {"value": len(sb.mats) / est, "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": (f"{done} of {len(sb.mats)} systems ({cells} cells) solved by scipy bicgstab + "
                        f"oracle {precond} in {t_all:.1f} s; batch rate extrapolated by cell count; 1 thread")}
+
+
+
+def bench_serve(args):
+    """Serving workload: Poisson arrivals at --rate per second for --duration
+    seconds; each request = pressure correction + 3 momentum solves on one
+    mesh of 1e3..1e6 cells (log-uniform; quad / hex / tet 50/30/20, snapped
+    to --templates sizes per kind); mixed precision to rtol 1e-8 with
+    continuous batching and HBM admission.  The first --baseline-duration
+    seconds of the same trace are also served one request at a time in FP64."""
+    import __graft_entry__
+    __graft_entry__.load_package()
+    from kcfd_amd import _lib as L, serve as S
+
+    sizes = S.template_sizes(per_kind=args.templates)
+    t = time.perf_counter()
+    built = S.build_templates(sizes=sizes)          # forks builders: before the GPU is touched
+    log(f"built {len(built)} mesh templates in {time.perf_counter() - t:.1f}s (sizes {sizes})")
+    trace = S.make_trace(rate=args.rate, duration=args.duration)
+    kidx = {k: i for i, k in enumerate(S.KINDS)}
+    tr = [(ta, kidx[k] * len(sizes) + S.nearest_template(sizes, n), sd) for ta, k, n, sd in trace]
+    ctx = L.default_context()
+    info = ctx.info()
+    log("device", info)
+    t = time.perf_counter()
+    temps = [S.Template(ctx, k, n, p, m, args.precond, "mixed", args.order) for k, n, (p, m) in built]
+    del built
+    log(f"templates on the device in {time.perf_counter() - t:.1f}s; "
+        f"request footprint {min(x.bytes for x in temps) / 1e6:.1f}..{max(x.bytes for x in temps) / 1e6:.1f} MB")
+    cells = np.array([temps[ti].n for _, ti, _ in tr])
+    log(f"trace: {len(tr)} requests over {args.duration:.0f}s, {cells.sum() / 1e6:.1f}M cells "
+        f"(mean {cells.mean():.0f}), {4 * len(tr)} solves")
+
+    def progress(*a):
+        log(*a)
+
+    # warm-up: a few requests (graph capture, first launches)
+    srv = S.Server(ctx, temps, precision="mixed", precond=args.precond, order=args.order,
+                   batched=True, check_every=args.check_every, log=progress)
+    srv.run([(0.0, ti, sd) for _, ti, sd in tr[:8]])
+    t = time.perf_counter()
+    rep = srv.run(tr)
+    srv.close()
+    mixed = rep.summary()
+    log("batched mixed", mixed, f"(wall {time.perf_counter() - t:.1f}s)")
+    nb = sum(1 for x in tr if x[0] < args.baseline_duration)
+    base = None
+    if nb:
+        b = S.Server(ctx, temps, precision="fp64", precond=args.precond, order=args.order, batched=False)
+        rb = b.run(tr[:nb])
+        base = rb.summary()
+        base["trace_prefix_s"] = args.baseline_duration
+        log("one-at-a-time fp64", base)
+    line = {
+        "metric": "solves/s + %HBM roofline at rel-resid 1e-8; p50/p99 latency, mixed vs FP64",
+        "value": mixed["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(1e3 * rep.elapsed_s, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32 inner / f64 refinement", "data": "synthetic",
+        "config": {"workload": f"serving: Poisson {args.rate:.0f} req/s for {args.duration:.0f} s, "
+                               f"each request pressure correction + 3 momentum solves on a 1e3..1e6-cell "
+                               f"mesh (quad/hex/tet 50/30/20), BiCGSTAB+{args.precond}, rtol 1e-8",
+                   "requests": len(tr), "templates_per_kind": len(sizes),
+                   "step": "the whole trace (continuous batching)"},
+        "latency_ms": {"p50": mixed["p50_ms"], "p99": mixed["p99_ms"]},
+        "serve_mixed_batched": mixed, "baseline_fp64_one_at_a_time": base,
+        "vs_one_at_a_time": round(mixed["solves_per_s"] / base["solves_per_s"], 3) if base else None,
+        "device": info["name"],
+    }
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

@@ -134,6 +134,10 @@ int kcfd_precond_set_order(kcfd_mat *m, int order);
  * receives the row of A at each position of P A P^T, ncolors the number of
  * colours (0 for the natural ordering). */
 int kcfd_ilu0_order(kcfd_mat *m, int *perm, int *ncolors);
+/* Give dst (same structure as src) src's ILU(0) ordering and level schedule,
+ * so that its kcfd_precond_setup skips the structural analysis (no
+ * reference counterpart: a server solving many systems on one mesh). */
+int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src);
 /* y = M^{-1} x (dev) with the preconditioner set up on m (identity for
  * none): the LinearOperator matvec that create_preconditioner returns
  * (core/preconditioner.py:45-46, :52-53). */
@@ -185,6 +189,24 @@ int kcfd_batch_run(kcfd_batch *b, kcfd_solve_result *results /*host nsys*/);
 int kcfd_batch_destroy(kcfd_batch *b);
 /* device bytes of the batch workspace */
 int kcfd_batch_bytes(kcfd_batch *b, long long *bytes);
+
+/* Dynamic batches, for serving (no reference counterpart).  Systems attach
+ * to free slots and detach between rounds; each kcfd_batch_step initialises
+ * the systems attached since the last step, replays one captured graph of
+ * opts->check_every iterations over every member, computes the FP64 true
+ * residual of the systems that finished, and lists their slots (host array
+ * of `capacity` ints).  A system's arithmetic does not depend on the other
+ * members, so its result is the same alone and in any mix.  The matrix of
+ * an attached system (set up for opts' preconditioner and precision) and its
+ * b / x buffers must stay alive until it is detached; one matrix may back
+ * several members (one per right-hand side).  kcfd_batch_attach returns
+ * KCFD_EBUSY when every slot is taken. */
+int kcfd_batch_create_dynamic(kcfd_ctx *ctx, int capacity, const kcfd_solve_opts *opts,
+                              kcfd_batch **out);
+int kcfd_batch_attach(kcfd_batch *b, kcfd_mat *m, const double *bvec, double *x, int *slot);
+int kcfd_batch_step(kcfd_batch *b, int *finished, int *nfinished);
+int kcfd_batch_detach(kcfd_batch *b, int slot, kcfd_solve_result *res);
+int kcfd_batch_members(kcfd_batch *b, int *members);
 /* the single-system workspace kcfd_solve cached on a matrix */
 int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out);
 /* Measurement hooks (no reference counterpart; the reference times whole

@@ -92,6 +92,12 @@ SIGNATURES = {
     "kcfd_precond_setup": ([_P, _I, _I], _I),
     "kcfd_ilu0_get": ([_P, _P, ctypes.POINTER(_I), ctypes.POINTER(_I)], _I),
     "kcfd_precond_set_order": ([_P, _I], _I),
+    "kcfd_mat_copy_order": ([_P, _P], _I),
+    "kcfd_batch_create_dynamic": ([_P, _I, ctypes.POINTER(SolveOpts), _PP], _I),
+    "kcfd_batch_attach": ([_P, _P, _P, _P, ctypes.POINTER(_I)], _I),
+    "kcfd_batch_step": ([_P, _P, ctypes.POINTER(_I)], _I),
+    "kcfd_batch_detach": ([_P, _I, ctypes.POINTER(SolveResult)], _I),
+    "kcfd_batch_members": ([_P, ctypes.POINTER(_I)], _I),
     "kcfd_ilu0_order": ([_P, _P, ctypes.POINTER(_I)], _I),
     "kcfd_precond_apply": ([_P, _P, _P], _I),
     "kcfd_solve": ([_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveResult)], _I),

@@ -85,6 +85,7 @@ class BatchResult:
                     converged=int(sum(r.info == 0 for r in self.results)),
                     iters_mean=round(float(it.mean()), 1), iters_max=int(it.max()),
                     refinements_total=int(sum(r.refinements for r in self.results)),
+                    fp64_fallbacks=int(sum(getattr(r, "fallback", 0) for r in self.results)),
                     max_rel_resid=float(rr.max()), opts=self.opts,
                     not_converged=[dict(index=k, info=r.info, iters=r.iters,
                                         refinements=r.refinements, rel_resid=r.rel_resid)
@@ -134,30 +135,56 @@ class SystemBatch:
         return device.Batch(self.ctx, self.mats, self.bs, self.xs, opts)
 
     def solve(self, precision="mixed", precond="jacobi", order="color", rtol=1e-8,
-              check_every=8, grid_per_cu=0, trsv_per_cu=0, delta=0.1, batch=None) -> BatchResult:
+              check_every=8, grid_per_cu=0, trsv_per_cu=0, delta=0.1, batch=None,
+              fallback=True) -> BatchResult:
         """One batched solve from x0 = 0 (preconditioners set up beforehand,
-        outside the timed region)."""
+        outside the timed region).  In mixed precision, the systems that stop
+        without converging continue in FP64 from their best iterate, as one
+        more batch inside the timed region (``fallback``)."""
         self.setup(precond, precision, order)
         opts = device.default_opts(rtol=rtol, precond=precond, precision=precision, inner_rtol=delta,
                                    check_every=check_every, grid_per_cu=grid_per_cu,
                                    trsv_per_cu=trsv_per_cu)
         own = batch is None
         b = batch if batch is not None else self.make_batch(opts)
+        nfb = 0
         try:
             for x in self.xs:
                 x.fill(0.0)
             self.ctx.sync()
             t0 = time.perf_counter()
             res = b.run()
+            dev_ms = max(r.ms for r in res) if res else 0.0
+            for r in res:
+                r.fallback = 0
+            bad = [k for k, r in enumerate(res) if r.info != 0]
+            if fallback and precision == "mixed" and bad:
+                o64 = device.default_opts(rtol=rtol, precond=precond, precision="fp64",
+                                          check_every=check_every, grid_per_cu=grid_per_cu,
+                                          trsv_per_cu=trsv_per_cu)
+                fb = device.Batch(self.ctx, [self.mats[k] for k in bad], [self.bs[k] for k in bad],
+                                  [self.xs[k] for k in bad], o64)
+                try:
+                    res2 = fb.run()
+                finally:
+                    fb.close()
+                dev_ms += max(r.ms for r in res2)
+                for k, r2 in zip(bad, res2):
+                    r1 = res[k]
+                    r2.iters += r1.iters
+                    r2.refinements = r1.refinements
+                    r2.bytes += r1.bytes
+                    r2.fallback = 1
+                    res[k] = r2
+                nfb = len(bad)
             wall = time.perf_counter() - t0
         finally:
             if own:
                 b.close()
-        dev_ms = max(r.ms for r in res) if res else 0.0
         return BatchResult(precision, precond, wall, dev_ms, res, float(sum(r.bytes for r in res)),
                            dict(check_every=check_every, grid_per_cu=grid_per_cu,
                                 trsv_per_cu=trsv_per_cu, order=order if precond == "ilu0" else None,
-                                delta=delta if precision == "mixed" else None))
+                                delta=delta if precision == "mixed" else None, fp64_fallbacks=nfb))
 
     def solutions(self):
         return [x.download() for x in self.xs]

@@ -52,7 +52,9 @@ enum : int {
     S_REFINE = 3,   // MIXED: reliable update (x += d; r = b - A x) pending
     S_ZEROX = 4,    // ||b|| == 0: x = 0 pending
     S_RESUME = 5,   // MIXED: continue after a reliable update (p-update with new beta)
+    S_NEW = 6,      // attached, initialisation (k_init / f_init) pending
     S_DONE = 16,    // terminal
+    S_FREE = 32,    // empty slot of a dynamic batch
 };
 
 struct SysDesc {
@@ -69,9 +71,10 @@ struct SysDesc {
     const double *lu64;
     const float *lu32;
     const double *b;
-    double *x;
+    double *x;        // the iterate, FP64 in both modes
+    double *xb;       // MIXED: the iterate with the lowest true residual so far
     // inner vectors (double in FP64 mode, float in MIXED mode)
-    void *r, *rt, *p, *v, *s, *t, *ph, *sh, *y, *d;
+    void *r, *rt, *p, *v, *s, *t, *ph, *sh, *y;
 };
 
 struct SysState {
@@ -81,7 +84,9 @@ struct SysState {
     double true_rr, true_bb;
     int status, iters, info, half;
     int refinements, total_iters, maxiter, restart;
-    int final_, stagnant, pad[2];
+    int final_, stagnant, true_done, save;   // save: copy x to xb in the next K1
+    int restore, best_iter, rollback, pad;   // restore: x = xb before the true residual;
+                                             // rollback: x = xb before a residual replacement
 };
 
 struct BatchParams {
@@ -146,14 +151,20 @@ __device__ __forceinline__ double precond_at(const SysDesc &d, const V *__restri
 // (in-order) accumulation, keeping 8 loads per lane in flight.
 constexpr int kRowUnroll = 8;
 
+//
+// MIXED (V = float): the FP32 values are in difference form (row sum on the
+// diagonal), so the row is sum_{j!=i} a_ij (z_j - z_i) + rowsum_i z_i with
+// the differences taken in FP64.
 template <typename V, int PC>
 __device__ __forceinline__ double row_amz(const SysDesc &d, int i, const V *__restrict__ z,
                                           const V *__restrict__ zh) {
 #pragma clang fp contract(off)
+    constexpr bool kDiff = sizeof(V) == sizeof(float);
     const V *__restrict__ a = sell_vals<V>(d);
     const int s = i >> 6;
     const int s0 = d.sptr[s], w = (d.sptr[s + 1] - s0) >> 6;
     const int base = s0 + (i & 63);
+    const double zi = kDiff ? precond_at<V, PC>(d, z, zh, i) : 0.0;
     double acc = 0.0;
     for (int k0 = 0; k0 < w; k0 += kRowUnroll) {
         int col[kRowUnroll];
@@ -170,7 +181,10 @@ __device__ __forceinline__ double row_amz(const SysDesc &d, int i, const V *__re
             if (k0 + k < w) zv[k] = precond_at<V, PC>(d, z, zh, col[k]);
 #pragma unroll
         for (int k = 0; k < kRowUnroll; ++k)
-            if (k0 + k < w) acc += (double)av[k] * zv[k];
+            if (k0 + k < w) {
+                if (kDiff) acc += (double)av[k] * (col[k] == i ? zi : zv[k] - zi);
+                else acc += (double)av[k] * zv[k];
+            }
     }
     return acc;
 }
@@ -228,19 +242,20 @@ __device__ __forceinline__ void sum_partials(const SysDesc &d, const double *__r
 }
 
 // ------------------------------------------------------------------ init
-// FP64: r = b - A x, rt = r; sums ||b||^2, ||r||^2.  MIXED: ||b||^2; d = 0.
+// FP64: r = b - A x, rt = r; sums ||b||^2, ||r||^2.  MIXED: ||b||^2.
 // ILU: every triangular-solve buffer starts at the sentinel.
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
-                                                 const int2 *__restrict__ tiles, int ntiles,
+                                                 const int2 *__restrict__ tiles, const int *__restrict__ counts,
                                                  double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
 #pragma clang fp contract(off)
     const BatchParams bp = *bpp;
+    const int ntiles = counts[0];   // tiles of the current membership
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
-        if (sts[ti.x].status != S_START) continue;
+        if (sts[ti.x].status != S_NEW) continue;
         const int i = ti.y + threadIdx.x;
         double v[2] = {0.0, 0.0};
         if (i < d.n) {
@@ -251,8 +266,6 @@ __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys
                 ((double *)d.r)[i] = ri;
                 ((double *)d.rt)[i] = ri;
                 v[1] = ri * ri;
-            } else {
-                ((V *)d.d)[i] = (V)0;
             }
             if (PC == KCFD_PC_ILU0) {
                 ((V *)d.y)[i] = sentinel_value<V>();
@@ -270,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void f_init(const SysDesc *__restrict__ sys
     const BatchParams bp = *bpp;
     const SysDesc &d = sys[blockIdx.x];
     SysState &st = sts[blockIdx.x];
-    if (st.status != S_START) return;
+    if (st.status != S_NEW) return;
     double tot[2];
     sum_partials<2>(d, partials, tot);
     if (threadIdx.x != 0) return;
@@ -289,6 +302,7 @@ __global__ __launch_bounds__(kBlock) void f_init(const SysDesc *__restrict__ sys
         if (st.maxiter <= 0 || sqrt(tot[1]) < thr) { st.status = S_DONE; return; }
         st.rho = tot[1];           // rt = r, so rt.r = ||r||^2
         if (fabs(st.rho) < bp.rhotol) { st.info = KCFD_INFO_RHO_BREAKDOWN; st.status = S_DONE; return; }
+        st.status = S_START;       // iteration 0 next: p = r
     } else {
         st.atol_out = thr;
         st.atol_in = thr;
@@ -299,15 +313,17 @@ __global__ __launch_bounds__(kBlock) void f_init(const SysDesc *__restrict__ sys
 }
 
 // -------------------------------------------------------------------- K1
-// p-update / p = r / MIXED x += d, d = 0 / x = 0 for a zero rhs.
+// p-update / p = r / x = 0 for a zero rhs; MIXED: xb = x after an improving
+// reliable update, solve buffers reset before one.
 // Block 0 also clears the chunk counters of the two p triangular solves.
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
                                               const SysState *__restrict__ sts,
-                                              const int2 *__restrict__ tiles, int ntiles,
+                                              const int2 *__restrict__ tiles, const int *__restrict__ counts,
                                               unsigned int *__restrict__ counters) {
 #pragma clang fp contract(off)
     if (blockIdx.x == 0 && threadIdx.x < 2) counters[threadIdx.x] = 0u;
+    const int ntiles = counts[0];   // tiles of the current membership
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
@@ -315,6 +331,7 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
         const int status = st.status;
         const int i = ti.y + threadIdx.x;
         if (i >= d.n) continue;
+        if (st.save) d.xb[i] = d.x[i];     // MIXED: the last update improved on the best
         if (status == S_RUN || status == S_RESUME) {
             V *__restrict__ p = (V *)d.p;
             const V om = (V)st.omega, be = (V)st.beta;
@@ -324,9 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
         } else if (status == S_START) {
             ((V *)d.p)[i] = ((const V *)d.r)[i];
         } else if (status == S_REFINE) {
-            V *__restrict__ dd = (V *)d.d;
-            d.x[i] = d.x[i] + (double)dd[i];
-            dd[i] = (V)0;
+            if (st.rollback) d.x[i] = d.xb[i];   // the inner iteration diverged
             if (PC == KCFD_PC_ILU0) {
                 ((V *)d.ph)[i] = sentinel_value<V>();
                 ((V *)d.sh)[i] = sentinel_value<V>();
@@ -344,9 +359,10 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_spmv_p(const SysDesc *__restrict__ sys,
                                                    const SysState *__restrict__ sts,
-                                                   const int2 *__restrict__ tiles, int ntiles,
+                                                   const int2 *__restrict__ tiles, const int *__restrict__ counts,
                                                    double *__restrict__ partials) {
 #pragma clang fp contract(off)
+    const int ntiles = counts[0];   // tiles of the current membership
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
@@ -389,6 +405,7 @@ __global__ __launch_bounds__(kBlock) void f_spmv_p(const SysDesc *__restrict__ s
         double tot[1];
         sum_partials<1>(d, partials, tot);
         if (threadIdx.x != 0) return;
+        st.save = 0;
         const double rv = tot[0];
         if (rv == 0.0) {
             st.info = KCFD_INFO_OMEGA_BREAKDOWN;
@@ -403,18 +420,33 @@ __global__ __launch_bounds__(kBlock) void f_spmv_p(const SysDesc *__restrict__ s
         double tot[3];
         sum_partials<3>(d, partials, tot);
         if (threadIdx.x != 0) return;
+        st.save = 0;
+        st.rollback = 0;
         st.rr64 = tot[0];
         st.refinements += 1;
         const double rn = sqrt(tot[0]);
         if (rn < st.atol_out) { st.info = 0; st.status = S_DONE; return; }
-        if (st.final_) { st.info = st.total_iters > 0 ? st.total_iters : 1; st.status = S_DONE; return; }
-        if (rn < st.rbest) {
-            st.rbest = rn;
-            st.stagnant = 0;
-        } else if (++st.stagnant >= 8 || st.refinements >= bp.max_refine) {
+        // A stop without convergence returns the best iterate seen (xb).
+        if (st.final_) {
             st.info = st.total_iters > 0 ? st.total_iters : 1;
+            st.restore = rn > st.rbest;
             st.status = S_DONE;
             return;
+        }
+        if (rn < st.rbest) {
+            st.rbest = rn;
+            st.best_iter = st.total_iters;
+            st.save = 1;
+        } else if (st.total_iters - st.best_iter > max(200, st.best_iter) ||
+                   st.refinements >= bp.max_refine) {
+            // stagnation: more iterations without a better true residual than
+            // it took to reach the best one (at least 200)
+            st.info = st.total_iters > 0 ? st.total_iters : 1;
+            st.restore = 1;
+            st.status = S_DONE;
+            return;
+        } else {
+            st.restart = 1;   // a replacement that did not improve: fresh shadow residual
         }
         st.rmax = sqrt(tot[2]);
         st.rr = tot[2];
@@ -446,11 +478,12 @@ __global__ __launch_bounds__(kBlock) void f_spmv_p(const SysDesc *__restrict__ s
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
                                               const SysState *__restrict__ sts,
-                                              const int2 *__restrict__ tiles, int ntiles,
+                                              const int2 *__restrict__ tiles, const int *__restrict__ counts,
                                               double *__restrict__ partials,
                                               unsigned int *__restrict__ counters) {
 #pragma clang fp contract(off)
     if (blockIdx.x == 0 && threadIdx.x < 2) counters[2 + threadIdx.x] = 0u;
+    const int ntiles = counts[0];   // tiles of the current membership
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
@@ -485,10 +518,11 @@ __global__ __launch_bounds__(kBlock) void f_s(const SysDesc *__restrict__ sys, S
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ sys,
                                                    const SysState *__restrict__ sts,
-                                                   const int2 *__restrict__ tiles, int ntiles,
+                                                   const int2 *__restrict__ tiles, const int *__restrict__ counts,
                                                    double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
 #pragma clang fp contract(off)
     const BatchParams bp = *bpp;
+    const int ntiles = counts[0];   // tiles of the current membership
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
@@ -507,9 +541,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ s
         } else if (status == S_HALF) {
             if (i < d.n) {
                 const V al = (V)st.alpha;
-                V *__restrict__ xx = bp.precision == KCFD_FP64 ? (V *)d.x : (V *)d.d;
                 const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
-                xx[i] = xx[i] + al * ph;
+                if (bp.precision == KCFD_FP64)       // scipy's operation, in FP64
+                    ((V *)d.x)[i] = ((V *)d.x)[i] + al * ph;
+                else                                 // MIXED: FP32 direction, FP64 iterate
+                    d.x[i] = d.x[i] + st.alpha * (double)ph;
                 if (PC == KCFD_PC_ILU0) {
                     ((V *)d.ph)[i] = sentinel_value<V>();
                     ((V *)d.sh)[i] = sentinel_value<V>();
@@ -548,10 +584,11 @@ __global__ __launch_bounds__(kBlock) void f_spmv_s(const SysDesc *__restrict__ s
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
                                                const SysState *__restrict__ sts,
-                                               const int2 *__restrict__ tiles, int ntiles,
+                                               const int2 *__restrict__ tiles, const int *__restrict__ counts,
                                                double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
 #pragma clang fp contract(off)
     const BatchParams bp = *bpp;
+    const int ntiles = counts[0];   // tiles of the current membership
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
@@ -561,12 +598,19 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
         double v[2] = {0.0, 0.0};
         if (i < d.n) {
             const V al = (V)st.alpha, om = (V)st.omega;
-            V *__restrict__ xx = bp.precision == KCFD_FP64 ? (V *)d.x : (V *)d.d;
             const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
             const V sh = (V)precond_at<V, PC>(d, (const V *)d.s, (const V *)d.sh, i);
-            V xi = xx[i] + al * ph;
-            xi = xi + om * sh;
-            xx[i] = xi;
+            if (bp.precision == KCFD_FP64) {         // scipy's operation order
+                V *__restrict__ xx = (V *)d.x;
+                V xi = xx[i] + al * ph;
+                xi = xi + om * sh;
+                xx[i] = xi;
+            } else {   // MIXED: the iterate accumulates in FP64 (an FP32 correction
+                       // would carry an error of eps_32 |x| into the residual)
+                double xi = d.x[i] + st.alpha * (double)ph;
+                xi = xi + st.omega * (double)sh;
+                d.x[i] = xi;
+            }
             const V ri = ((const V *)d.s)[i] - om * ((const V *)d.t)[i];
             ((V *)d.r)[i] = ri;
             v[0] = (double)ri * (double)ri;
@@ -608,9 +652,11 @@ __global__ __launch_bounds__(kBlock) void f_xr(const SysDesc *__restrict__ sys, 
         // reliable update when the FP32 residual claims convergence or has
         // dropped by delta since the last update; restart on divergence,
         // breakdown or the inner iteration cap
-        if (!(rn == rn) || rn > 1e6 * st.rmax || fabs(rho) < bp.rhotol ||
-            fabs(st.omega) < bp.rhotol || st.iters >= bp.inner_maxiter) {
+        const bool diverged = !(rn == rn) || rn > 1e3 * st.rbest;
+        if (diverged || fabs(rho) < bp.rhotol || fabs(st.omega) < bp.rhotol ||
+            st.iters >= bp.inner_maxiter) {
             st.restart = 1;
+            st.rollback = diverged;   // back to the best iterate
             st.status = S_REFINE;
             return;
         }
@@ -624,13 +670,38 @@ __global__ __launch_bounds__(kBlock) void f_xr(const SysDesc *__restrict__ sys, 
 
 // ------------------------------------------------------- true residual
 // ||b - A x||^2 and ||b||^2 in FP64 on the FP64 matrix, every system.
+// Systems that are done and whose true residual is not yet computed.
+__device__ __forceinline__ bool wants_true(const SysState &st) {
+    return st.status == S_DONE && !st.true_done;
+}
+
+// MIXED stop without convergence: x = xb (the best iterate), in a launch of
+// its own -- the true residual reads the neighbours' x.
+__global__ __launch_bounds__(kBlock) void k_restore(const SysDesc *__restrict__ sys,
+                                                    const SysState *__restrict__ sts,
+                                                    const int2 *__restrict__ tiles,
+                                                    const int *__restrict__ counts) {
+    const int ntiles = counts[0];   // tiles of the current membership
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysState &st = sts[ti.x];
+        if (!wants_true(st) || !st.restore) continue;
+        const SysDesc &d = sys[ti.x];
+        const int i = ti.y + threadIdx.x;
+        if (i < d.n) d.x[i] = d.xb[i];
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_true_resid(const SysDesc *__restrict__ sys,
-                                                       const int2 *__restrict__ tiles, int ntiles,
+                                                       const SysState *__restrict__ sts,
+                                                       const int2 *__restrict__ tiles, const int *__restrict__ counts,
                                                        double *__restrict__ partials) {
 #pragma clang fp contract(off)
+    const int ntiles = counts[0];   // tiles of the current membership
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
+        if (!wants_true(sts[ti.x])) continue;
         const int i = ti.y + threadIdx.x;
         double v[2] = {0.0, 0.0};
         if (i < d.n) {
@@ -648,11 +719,13 @@ __global__ __launch_bounds__(kBlock) void f_true_resid(const SysDesc *__restrict
                                                        const double *__restrict__ partials) {
     const SysDesc &d = sys[blockIdx.x];
     SysState &st = sts[blockIdx.x];
+    if (!wants_true(st)) return;
     double tot[2];
     sum_partials<2>(d, partials, tot);
     if (threadIdx.x != 0) return;
     st.true_rr = tot[0];
     st.true_bb = tot[1];
+    st.true_done = 1;
 }
 
 // ---------------------------------------------------- ILU(0) trisolves
@@ -722,12 +795,14 @@ __device__ __forceinline__ V sub_ready(V acc, const V *__restrict__ a, const int
 template <typename V, bool UPPER, int WHICH>
 __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
-                                                 const Chunk *__restrict__ chunks, int nchunks,
+                                                 const Chunk *__restrict__ chunks,
+                                                 const int *__restrict__ counts,
                                                  const int *const *__restrict__ lvl_rows) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
     // The chunk loop is wave-uniform (the chunk index depends on the wave
     // only); per-lane work sits inside an if.
+    const int nchunks = counts[UPPER ? 2 : 1];
     const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
     const int wave = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     for (int c = wave; c < nchunks; c += nwaves) {
@@ -776,27 +851,34 @@ using namespace kcfd;
 
 struct kcfd_batch {
     kcfd_ctx *ctx = nullptr;
-    int nsys = 0, ntiles = 0, grid = 0;
+    int nsys = 0;                   // slots (the capacity of a dynamic batch)
+    int ntiles = 0, grid = 0;
+    bool dynamic = false;           // systems attach / detach between rounds
     kcfd_solve_opts opts{};
     BatchParams bp{};
-    std::vector<kcfd_mat *> mats;
+    std::vector<kcfd_mat *> mats;   // per slot; null = free slot
     std::vector<SysDesc> h_desc;
     std::vector<int> sys_n;
+    std::vector<void *> slot_mem;   // dynamic: vectors of each slot
+    std::vector<char> fresh, reported;
     SysDesc *d_desc = nullptr;
     SysState *d_state = nullptr;
     std::vector<SysState> h_state;
     int2 *d_tiles = nullptr;
+    int cap_tiles = 0;
     double *d_partials = nullptr;
     unsigned int *d_counters = nullptr;
-    void *slab = nullptr;          // all per-system vectors
+    int *d_counts = nullptr;        // {ntiles, nchunks_l, nchunks_u}, read by the kernels
+    void *slab = nullptr;           // static: all per-system vectors
     size_t slab_bytes = 0;
     // ILU chunk tables (level-major over systems) for the L and U solves
     Chunk *d_chunks_l = nullptr, *d_chunks_u = nullptr;
-    int nchunks_l = 0, nchunks_u = 0, trsv_grid = 0;
+    int nchunks_l = 0, nchunks_u = 0, cap_chunks_l = 0, cap_chunks_u = 0, trsv_grid = 0;
     const int **d_lvl_l = nullptr, **d_lvl_u = nullptr;
     BatchParams *d_params = nullptr;   // read by the kernels: the captured graph stays valid
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
+    bool dirty = false;             // membership changed: tables to rebuild
     // kcfd_batch_profile: launch_iteration records ctx->ev0 / ev1 around the
     // kernel of this phase (0: none, as during graph capture)
     int timed_phase = 0;
@@ -814,37 +896,37 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
         if (b->timed_phase == phase) (void)hipEventRecord(ev, s);
     };
     mark(4, b->ctx->ev0);
-    hipLaunchKernelGGL((k_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles,
+    hipLaunchKernelGGL((k_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
                        b->d_counters);
     mark(4, b->ctx->ev1);
     if (PC == KCFD_PC_ILU0) {
         mark(3, b->ctx->ev0);
         hipLaunchKernelGGL((k_trsv<V, false, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l);
+                           b->d_state, b->d_chunks_l, b->d_counts, b->d_lvl_l);
         mark(3, b->ctx->ev1);
         hipLaunchKernelGGL((k_trsv<V, true, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u);
+                           b->d_state, b->d_chunks_u, b->d_counts, b->d_lvl_u);
     }
     mark(1, b->ctx->ev0);
     hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                       b->ntiles, b->d_partials);
+                       b->d_counts, b->d_partials);
     mark(1, b->ctx->ev1);
     hipLaunchKernelGGL(f_spmv_p, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
-    hipLaunchKernelGGL((k_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->ntiles,
+    hipLaunchKernelGGL((k_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
                        b->d_partials, b->d_counters);
     hipLaunchKernelGGL(f_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     if (PC == KCFD_PC_ILU0) {
         hipLaunchKernelGGL((k_trsv<V, false, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_l, b->nchunks_l, b->d_lvl_l);
+                           b->d_state, b->d_chunks_l, b->d_counts, b->d_lvl_l);
         hipLaunchKernelGGL((k_trsv<V, true, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_u, b->nchunks_u, b->d_lvl_u);
+                           b->d_state, b->d_chunks_u, b->d_counts, b->d_lvl_u);
     }
     hipLaunchKernelGGL((k_spmv_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                       b->ntiles, b->d_partials, b->d_params);
+                       b->d_counts, b->d_partials, b->d_params);
     hipLaunchKernelGGL(f_spmv_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     mark(2, b->ctx->ev0);
     hipLaunchKernelGGL((k_xr<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
-                       b->ntiles, b->d_partials, b->d_params);
+                       b->d_counts, b->d_partials, b->d_params);
     mark(2, b->ctx->ev1);
     hipLaunchKernelGGL(f_xr, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
 }
@@ -852,7 +934,7 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
 template <typename V, int PC>
 void launch_init(kcfd_batch *b, hipStream_t s) {
     hipLaunchKernelGGL((k_init<V, PC>), dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                       b->d_tiles, b->ntiles, b->d_partials, b->d_params);
+                       b->d_tiles, b->d_counts, b->d_partials, b->d_params);
     hipLaunchKernelGGL(f_init, dim3(b->nsys), dim3(kBlock), 0, s, b->d_desc, b->d_state,
                        b->d_partials, b->d_params);
 }
@@ -895,7 +977,9 @@ double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st)
         return 2 * MA(sv) + 19 * n * sv;
     };
     if (precision == KCFD_FP64) return (st.total_iters + 0.5 * st.half) * per_iter(8) + MA(8) + 3 * n * 8;
-    return st.total_iters * per_iter(4) + st.refinements * (MA(8) + 6 * n * 8);
+    // MIXED: the iterate x is FP64 (read + write, +8 n over an FP32 one);
+    // a residual replacement reads A, x, b in FP64 and writes r (+ xb copy)
+    return st.total_iters * (per_iter(4) + 8 * n) + st.refinements * (MA(8) + 2 * n * 8 + n * 4);
 }
 
 void fill_desc(SysDesc &d, kcfd_mat *m) {
@@ -914,17 +998,39 @@ void fill_desc(SysDesc &d, kcfd_mat *m) {
     d.lu32 = m->lu32;
 }
 
-int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *const *bs,
-                double *const *xs, const kcfd_solve_opts *o, kcfd_batch **out) {
+int check_opts(const kcfd_solve_opts *o) {
     const int pc = o->precond, prec = o->precision;
     KCFD_ARG(pc == KCFD_PC_NONE || pc == KCFD_PC_JACOBI || pc == KCFD_PC_ILU0,
              "unknown preconditioner");
     KCFD_ARG(prec == KCFD_FP64 || prec == KCFD_MIXED, "unknown precision");
     KCFD_ARG(o->grid_per_cu >= 0 && o->grid_per_cu <= 8, "grid_per_cu must be 0..8");
     KCFD_ARG(o->trsv_per_cu >= 0 && o->trsv_per_cu <= 4, "trsv_per_cu must be 0..4");
+    return KCFD_OK;
+}
+
+int check_mat(const kcfd_batch *b, const kcfd_mat *m, int k) {
+    if (m->ctx != b->ctx) {
+        set_error("matrix %d belongs to another context", k);
+        return KCFD_EARG;
+    }
+    if (m->pc_kind != b->bp.precond) {
+        set_error("matrix %d: preconditioner not set up as requested (have %d, want %d)", k,
+                  m->pc_kind, b->bp.precond);
+        return KCFD_ESTATE;
+    }
+    if (b->bp.precision == KCFD_MIXED && !m->have32) {
+        set_error("matrix %d: no FP32 copy (set up the preconditioner with KCFD_MIXED)", k);
+        return KCFD_ESTATE;
+    }
+    return KCFD_OK;
+}
+
+// common part of a batch: parameters and the small per-slot device arrays
+int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **out) {
+    KCFD_CHECK(check_opts(o));
     kcfd_batch *b = new kcfd_batch();
     b->ctx = ctx;
-    b->nsys = nsys;
+    b->nsys = slots;
     b->opts = *o;
     if (b->opts.check_every <= 0) b->opts.check_every = 8;
     b->bp.rtol = o->rtol;
@@ -932,85 +1038,122 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
     b->bp.delta = o->inner_rtol > 0 ? o->inner_rtol : 0.1;
     const double eps = 2.220446049250313e-16;
     b->bp.rhotol = eps * eps;
-    b->bp.precision = prec;
-    b->bp.precond = pc;
+    b->bp.precision = o->precision;
+    b->bp.precond = o->precond;
     b->bp.max_refine = o->max_refine > 0 ? o->max_refine : 1000;
-    b->bp.nsys = nsys;
+    b->bp.nsys = slots;
     b->bp.inner_maxiter = 5000;
-    b->mats.assign(mats, mats + nsys);
-    auto fail = [&](int rc) {
-        kcfd_batch_destroy(b);
-        return rc;
-    };
+    b->mats.assign(slots, nullptr);
+    b->h_desc.assign(slots, SysDesc{});
+    b->sys_n.assign(slots, 0);
+    b->slot_mem.assign(slots, nullptr);
+    b->fresh.assign(slots, 0);
+    b->reported.assign(slots, 0);
+    SysState free_st{};
+    free_st.status = S_FREE;
+    b->h_state.assign(slots, free_st);
+    *out = b;
+    // Every host-issued copy and fill goes on the library stream (created
+    // non-blocking, so the null stream is not ordered with it).
+    hipStream_t s = ctx->stream;
+    KCFD_HIP(hipMalloc(&b->d_desc, sizeof(SysDesc) * slots));
+    KCFD_HIP(hipMalloc(&b->d_state, sizeof(SysState) * slots));
+    KCFD_HIP(hipMemcpyAsync(b->d_state, b->h_state.data(), sizeof(SysState) * slots, hipMemcpyHostToDevice, s));
+    KCFD_HIP(hipMalloc(&b->d_counters, sizeof(unsigned int) * 8));
+    KCFD_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * 8, s));
+    KCFD_HIP(hipMalloc(&b->d_counts, sizeof(int) * 4));
+    KCFD_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int) * 4, s));
+    KCFD_HIP(hipMalloc(&b->d_params, sizeof(BatchParams)));
+    KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
+    if (o->precond == KCFD_PC_ILU0) {
+        KCFD_HIP(hipMalloc((void **)&b->d_lvl_l, sizeof(int *) * slots));
+        KCFD_HIP(hipMalloc((void **)&b->d_lvl_u, sizeof(int *) * slots));
+    }
+    KCFD_HIP(hipStreamSynchronize(s));
+    return KCFD_OK;
+}
+
+// FP32 / FP64 vector slots per system; MIXED adds two FP32-sized slots for
+// the FP64 best iterate xb
+int vectors_per_system(const kcfd_batch *b) {
+    return (b->bp.precond == KCFD_PC_ILU0 ? 9 : 6) + (b->bp.precision == KCFD_MIXED ? 2 : 0);
+}
+size_t value_size(const kcfd_batch *b) { return b->bp.precision == KCFD_FP64 ? sizeof(double) : sizeof(float); }
+size_t vec_stride(const kcfd_batch *b, int n) { return align_up((size_t)std::max(n, 1) * value_size(b), 256); }
+
+// slot k's vectors at base (the iterate x is the caller's FP64 buffer)
+void place_vectors(kcfd_batch *b, int k, char *base) {
+    SysDesc &d = b->h_desc[k];
+    const size_t stride = vec_stride(b, d.n);
+    void **vecs[9] = {&d.r, &d.rt, &d.p, &d.v, &d.s, &d.t, &d.ph, &d.sh, &d.y};
+    const int nv = b->bp.precond == KCFD_PC_ILU0 ? 9 : 6;
+    for (int q = 0; q < nv; ++q) *vecs[q] = base + q * stride;
+    d.xb = b->bp.precision == KCFD_MIXED ? (double *)(base + nv * stride) : nullptr;
+}
+
+void describe_slot(kcfd_batch *b, int k, kcfd_mat *m, const double *bvec, double *x) {
+    SysDesc &d = b->h_desc[k];
+    std::memset(&d, 0, sizeof(d));
+    d.n = m->n;
+    fill_desc(d, m);
+    d.b = bvec;
+    d.x = x;
+    b->mats[k] = m;
+    b->sys_n[k] = m->n;
+}
+
+// grow a device table to hold `need` entries (contents are rewritten after)
+template <typename T>
+int ensure_table(T **p, int *cap, int need, bool *grew) {
+    if (need <= *cap && *p) return KCFD_OK;
+    int c = std::max(64, std::max(need, 2 * *cap));
+    hipFree((void *)*p);
+    *p = nullptr;
+    KCFD_HIP(hipMalloc((void **)p, sizeof(T) * (size_t)c));
+    *cap = c;
+    *grew = true;
+    return KCFD_OK;
+}
+
+// Tile list (256 rows, slot order), per-slot tile ranges, and the level-major
+// ILU chunk tables of the current members, uploaded on the solver stream.
+// A grown table invalidates the captured graph (its kernels hold the old
+// pointers).
+int rebuild_tables(kcfd_batch *b) {
+    hipStream_t s = b->ctx->stream;
     std::vector<int2> tiles;
-    b->h_desc.resize(nsys);
-    const size_t vs = prec == KCFD_FP64 ? sizeof(double) : sizeof(float);
-    const int nvec = pc == KCFD_PC_ILU0 ? 10 : 7;
-    size_t slab = 0;
-    std::vector<size_t> off(nsys);
-    for (int k = 0; k < nsys; ++k) {
-        kcfd_mat *m = mats[k];
-        if (m->ctx != ctx) { set_error("matrix %d belongs to another context", k); return fail(KCFD_EARG); }
-        if (m->pc_kind != pc) {
-            set_error("matrix %d: preconditioner not set up as requested (have %d, want %d)", k,
-                      m->pc_kind, pc);
-            return fail(KCFD_ESTATE);
-        }
-        if (prec == KCFD_MIXED && !m->have32) {
-            set_error("matrix %d: no FP32 copy (set up the preconditioner with KCFD_MIXED)", k);
-            return fail(KCFD_ESTATE);
-        }
+    for (int k = 0; k < b->nsys; ++k) {
         SysDesc &d = b->h_desc[k];
-        std::memset(&d, 0, sizeof(d));
-        d.n = m->n;
         d.tile0 = (int)tiles.size();
-        const int nt = std::max(1, (m->n + kBlock - 1) / kBlock);
+        d.ntiles = 0;
+        if (!b->mats[k]) continue;
+        const int nt = std::max(1, (d.n + kBlock - 1) / kBlock);
         d.ntiles = nt;
         for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(k, t * kBlock));
-        fill_desc(d, m);
-        d.b = bs[k];
-        d.x = xs[k];
-        off[k] = slab;
-        slab += align_up((size_t)nvec * align_up((size_t)std::max(m->n, 1) * vs, 256), 256);
-        b->sys_n.push_back(m->n);
     }
     b->ntiles = (int)tiles.size();
-    // persistent grid: at most 4 workgroups per CU, a multiple of the XCD count
-    const int gpc = o->grid_per_cu > 0 ? o->grid_per_cu : 8;
-    const int cap = std::max(kXcds, (ctx->cus * gpc / kXcds) * kXcds);
-    b->grid = std::min(xcd_grid(b->ntiles), cap);
-    KCFD_HIP(hipMalloc(&b->slab, slab ? slab : 256));
-    b->slab_bytes = slab;
-    for (int k = 0; k < nsys; ++k) {
-        SysDesc &d = b->h_desc[k];
-        const size_t stride = align_up((size_t)std::max(d.n, 1) * vs, 256);
-        char *base = (char *)b->slab + off[k];
-        void **vecs[10] = {&d.r, &d.rt, &d.p, &d.v, &d.s, &d.t, &d.d, &d.ph, &d.sh, &d.y};
-        for (int q = 0; q < nvec; ++q) *vecs[q] = base + q * stride;
-        if (prec == KCFD_FP64) d.d = d.x;
+    bool grew = false;
+    const int old_cap = b->cap_tiles;
+    KCFD_CHECK(ensure_table(&b->d_tiles, &b->cap_tiles, b->ntiles, &grew));
+    if (b->cap_tiles != old_cap || !b->d_partials) {
+        hipFree(b->d_partials);
+        b->d_partials = nullptr;
+        KCFD_HIP(hipMalloc(&b->d_partials, sizeof(double) * kMaxPartials * (size_t)b->cap_tiles));
     }
-    KCFD_HIP(hipMalloc(&b->d_desc, sizeof(SysDesc) * nsys));
-    KCFD_HIP(hipMemcpy(b->d_desc, b->h_desc.data(), sizeof(SysDesc) * nsys, hipMemcpyHostToDevice));
-    KCFD_HIP(hipMalloc(&b->d_state, sizeof(SysState) * nsys));
-    b->h_state.assign(nsys, SysState{});
-    KCFD_HIP(hipMalloc(&b->d_tiles, sizeof(int2) * b->ntiles));
-    KCFD_HIP(hipMemcpy(b->d_tiles, tiles.data(), sizeof(int2) * b->ntiles, hipMemcpyHostToDevice));
-    KCFD_HIP(hipMalloc(&b->d_partials, sizeof(double) * kMaxPartials * b->ntiles));
-    KCFD_HIP(hipMalloc(&b->d_counters, sizeof(unsigned int) * 8));
-    KCFD_HIP(hipMemset(b->d_counters, 0, sizeof(unsigned int) * 8));
-    KCFD_HIP(hipMalloc(&b->d_params, sizeof(BatchParams)));
-    if (pc == KCFD_PC_ILU0) {
-        // level-major chunk tables over all systems
+    std::vector<Chunk> cl, cu;
+    if (b->bp.precond == KCFD_PC_ILU0) {
         int maxlev_l = 0, maxlev_u = 0;
-        for (auto *m : b->mats) {
-            maxlev_l = std::max(maxlev_l, m->nlev_l);
-            maxlev_u = std::max(maxlev_u, m->nlev_u);
-        }
+        for (auto *m : b->mats)
+            if (m) {
+                maxlev_l = std::max(maxlev_l, m->nlev_l);
+                maxlev_u = std::max(maxlev_u, m->nlev_u);
+            }
         auto build_chunks = [&](bool upper, std::vector<Chunk> &ch) {
             const int L = upper ? maxlev_u : maxlev_l;
             for (int l = 0; l < L; ++l) {
-                for (int k = 0; k < nsys; ++k) {
+                for (int k = 0; k < b->nsys; ++k) {
                     kcfd_mat *m = b->mats[k];
+                    if (!m) continue;
                     const std::vector<int> &lp = upper ? m->h_lvl_ptr_u : m->h_lvl_ptr_l;
                     const int nl = upper ? m->nlev_u : m->nlev_l;
                     if (l >= nl) continue;
@@ -1025,39 +1168,83 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
                 }
             }
         };
-        std::vector<Chunk> cl, cu;
         build_chunks(false, cl);
         build_chunks(true, cu);
         b->nchunks_l = (int)cl.size();
         b->nchunks_u = (int)cu.size();
-        KCFD_HIP(hipMalloc(&b->d_chunks_l, sizeof(Chunk) * std::max<size_t>(1, cl.size())));
-        KCFD_HIP(hipMalloc(&b->d_chunks_u, sizeof(Chunk) * std::max<size_t>(1, cu.size())));
+        KCFD_CHECK(ensure_table(&b->d_chunks_l, &b->cap_chunks_l, b->nchunks_l, &grew));
+        KCFD_CHECK(ensure_table(&b->d_chunks_u, &b->cap_chunks_u, b->nchunks_u, &grew));
+        std::vector<const int *> pl(b->nsys, nullptr), pu(b->nsys, nullptr);
+        for (int k = 0; k < b->nsys; ++k)
+            if (b->mats[k]) {
+                pl[k] = b->mats[k]->lvl_rows_l;
+                pu[k] = b->mats[k]->lvl_rows_u;
+            }
+        KCFD_HIP(hipMemcpyAsync((void *)b->d_lvl_l, pl.data(), sizeof(int *) * b->nsys, hipMemcpyHostToDevice, s));
+        KCFD_HIP(hipMemcpyAsync((void *)b->d_lvl_u, pu.data(), sizeof(int *) * b->nsys, hipMemcpyHostToDevice, s));
         if (!cl.empty())
-            KCFD_HIP(hipMemcpy(b->d_chunks_l, cl.data(), sizeof(Chunk) * cl.size(), hipMemcpyHostToDevice));
+            KCFD_HIP(hipMemcpyAsync(b->d_chunks_l, cl.data(), sizeof(Chunk) * cl.size(), hipMemcpyHostToDevice, s));
         if (!cu.empty())
-            KCFD_HIP(hipMemcpy(b->d_chunks_u, cu.data(), sizeof(Chunk) * cu.size(), hipMemcpyHostToDevice));
-        std::vector<const int *> pl(nsys), pu(nsys);
-        for (int k = 0; k < nsys; ++k) {
-            pl[k] = b->mats[k]->lvl_rows_l;
-            pu[k] = b->mats[k]->lvl_rows_u;
-        }
-        KCFD_HIP(hipMalloc((void **)&b->d_lvl_l, sizeof(int *) * nsys));
-        KCFD_HIP(hipMalloc((void **)&b->d_lvl_u, sizeof(int *) * nsys));
-        KCFD_HIP(hipMemcpy((void *)b->d_lvl_l, pl.data(), sizeof(int *) * nsys, hipMemcpyHostToDevice));
-        KCFD_HIP(hipMemcpy((void *)b->d_lvl_u, pu.data(), sizeof(int *) * nsys, hipMemcpyHostToDevice));
+            KCFD_HIP(hipMemcpyAsync(b->d_chunks_u, cu.data(), sizeof(Chunk) * cu.size(), hipMemcpyHostToDevice, s));
+    }
+    if (!tiles.empty())
+        KCFD_HIP(hipMemcpyAsync(b->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s));
+    KCFD_HIP(hipMemcpyAsync(b->d_desc, b->h_desc.data(), sizeof(SysDesc) * b->nsys, hipMemcpyHostToDevice, s));
+    const int counts[4] = {b->ntiles, b->nchunks_l, b->nchunks_u, 0};
+    KCFD_HIP(hipMemcpyAsync(b->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice, s));
+    // the host arrays above are locals: finish the copies before they go
+    KCFD_HIP(hipStreamSynchronize(s));
+    if (grew && b->graph) {
+        hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->dirty = false;
+    return KCFD_OK;
+}
+
+int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *const *bs,
+                double *const *xs, const kcfd_solve_opts *o, kcfd_batch **out) {
+    kcfd_batch *b = nullptr;
+    int rc = new_batch(ctx, nsys, o, &b);
+    auto fail = [&](int code) {
+        kcfd_batch_destroy(b);
+        return code;
+    };
+    if (rc != KCFD_OK) return b ? fail(rc) : rc;
+    size_t slab = 0;
+    std::vector<size_t> off(nsys);
+    for (int k = 0; k < nsys; ++k) {
+        if ((rc = check_mat(b, mats[k], k)) != KCFD_OK) return fail(rc);
+        describe_slot(b, k, mats[k], bs[k], xs[k]);
+        off[k] = slab;
+        slab += align_up((size_t)vectors_per_system(b) * vec_stride(b, mats[k]->n), 256);
+    }
+    if (hipMalloc(&b->slab, slab ? slab : 256) != hipSuccess) {
+        set_error("out of device memory for the batch vectors (%zu bytes)", slab);
+        return fail(KCFD_EOOM);
+    }
+    b->slab_bytes = slab;
+    for (int k = 0; k < nsys; ++k) place_vectors(b, k, (char *)b->slab + off[k]);
+    if ((rc = rebuild_tables(b)) != KCFD_OK) return fail(rc);
+    // persistent grid: at most grid_per_cu workgroups per CU, a multiple of
+    // the XCD count
+    const int gpc = o->grid_per_cu > 0 ? o->grid_per_cu : 8;
+    const int cap = std::max(kXcds, (ctx->cus * gpc / kXcds) * kXcds);
+    b->grid = std::min(xcd_grid(b->ntiles), cap);
+    if (o->precond == KCFD_PC_ILU0) {
         // Waves beyond a few levels of look-ahead only poll (and load the
         // memory system with polls), so the grid covers about three of the
-        // widest levels (summed over the batch).
-        int widest = 0;
-        for (int l = 0; l < std::max(maxlev_l, maxlev_u); ++l) {
+        // widest levels (summed over the batch).  At most 4 workgroups (16
+        // waves) per CU: the kernel's occupancy limit is 32 waves per CU, so
+        // the whole grid is resident.
+        int widest = 0, maxlev = 0;
+        for (auto *m : b->mats) maxlev = std::max(maxlev, std::max(m->nlev_l, m->nlev_u));
+        for (int l = 0; l < maxlev; ++l) {
             int w = 0;
-            for (auto *m : b->mats) {
+            for (auto *m : b->mats)
                 if (l < m->nlev_l) w += (m->h_lvl_ptr_l[l + 1] - m->h_lvl_ptr_l[l] + 63) / 64;
-            }
             widest = std::max(widest, w);
         }
-        // at most 4 workgroups (16 waves) per CU: the kernel's occupancy limit
-        // is 32 waves per CU, so the whole grid is resident
         const int want_waves = std::max(64, 3 * widest);
         const int tcap = ctx->cus * (o->trsv_per_cu > 0 ? o->trsv_per_cu : 4);
         b->trsv_grid = std::max(1, std::min((want_waves + kWaves - 1) / kWaves, tcap));
@@ -1097,7 +1284,7 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     for (int k = 0; k < b->nsys; ++k) {
         SysState st{};
         const int n = b->sys_n[k];
-        st.status = S_START;
+        st.status = S_NEW;
         st.maxiter = b->opts.maxiter > 0 ? b->opts.maxiter : 10 * std::max(n, 1);
         b->h_state[k] = st;
     }
@@ -1131,8 +1318,11 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
         }
     }
     KCFD_HIP(hipEventRecord(ctx->ev1, s));
-    hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_tiles,
-                       b->ntiles, b->d_partials);
+    if (b->bp.precision == KCFD_MIXED)
+        hipLaunchKernelGGL(k_restore, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                           b->d_tiles, b->d_counts);
+    hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                       b->d_tiles, b->d_counts, b->d_partials);
     hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kBlock), 0, s, b->d_desc, b->d_state,
                        b->d_partials);
     KCFD_HIP(hipGetLastError());
@@ -1190,13 +1380,165 @@ int kcfd_batch_destroy(kcfd_batch *b) {
     hipFree(b->d_tiles);
     hipFree(b->d_partials);
     hipFree(b->d_counters);
+    hipFree(b->d_counts);
     hipFree(b->d_params);
     hipFree(b->slab);
+    for (void *p : b->slot_mem) hipFree(p);
     hipFree(b->d_chunks_l);
     hipFree(b->d_chunks_u);
     hipFree((void *)b->d_lvl_l);
     hipFree((void *)b->d_lvl_u);
     delete b;
+    return KCFD_OK;
+}
+
+// ------------------------------------------------------- dynamic batches
+
+int kcfd_batch_create_dynamic(kcfd_ctx *ctx, int capacity, const kcfd_solve_opts *opts,
+                              kcfd_batch **out) {
+    KCFD_ARG(ctx && opts && out, "null argument");
+    KCFD_ARG(capacity > 0, "capacity must be positive");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    KCFD_HIP(hipSetDevice(ctx->device));
+    kcfd_batch *b = nullptr;
+    int rc = new_batch(ctx, capacity, opts, &b);
+    if (rc != KCFD_OK) {
+        if (b) kcfd_batch_destroy(b);
+        return rc;
+    }
+    b->dynamic = true;
+    // the graph is captured once for any membership: full-size launches
+    const int gpc = opts->grid_per_cu > 0 ? opts->grid_per_cu : 8;
+    b->grid = std::max(kXcds, (ctx->cus * gpc / kXcds) * kXcds);
+    b->trsv_grid = ctx->cus * (opts->trsv_per_cu > 0 ? opts->trsv_per_cu : 4);
+    if ((rc = rebuild_tables(b)) != KCFD_OK) {
+        kcfd_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return KCFD_OK;
+}
+
+int kcfd_batch_attach(kcfd_batch *b, kcfd_mat *m, const double *bvec, double *x, int *slot) {
+    KCFD_ARG(b && m && slot && (m->n == 0 || (bvec && x)), "null argument");
+    KCFD_ARG(b->dynamic, "not a dynamic batch");
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    KCFD_HIP(hipSetDevice(b->ctx->device));
+    int k = 0;
+    while (k < b->nsys && b->mats[k]) ++k;
+    if (k == b->nsys) {
+        set_error("dynamic batch full (%d slots)", b->nsys);
+        return KCFD_EBUSY;
+    }
+    KCFD_CHECK(check_mat(b, m, k));
+    void *mem = nullptr;
+    const size_t bytes = (size_t)vectors_per_system(b) * vec_stride(b, m->n);
+    if (hipMalloc(&mem, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("out of device memory for a system's vectors (%zu bytes)", bytes);
+        return KCFD_EOOM;
+    }
+    describe_slot(b, k, m, bvec, x);
+    b->slot_mem[k] = mem;
+    place_vectors(b, k, (char *)mem);
+    SysState st{};
+    st.status = S_NEW;
+    st.maxiter = b->opts.maxiter > 0 ? b->opts.maxiter : 10 * std::max(m->n, 1);
+    b->h_state[k] = st;
+    b->fresh[k] = 1;
+    b->reported[k] = 0;
+    b->dirty = true;
+    *slot = k;
+    return KCFD_OK;
+}
+
+int kcfd_batch_step(kcfd_batch *b, int *finished, int *nfinished) {
+    KCFD_ARG(b && nfinished, "null argument");
+    KCFD_ARG(b->dynamic, "not a dynamic batch");
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    KCFD_HIP(hipSetDevice(b->ctx->device));
+    hipStream_t s = b->ctx->stream;
+    *nfinished = 0;
+    if (b->dirty) KCFD_CHECK(rebuild_tables(b));
+    bool any_new = false, any = false;
+    for (int k = 0; k < b->nsys; ++k) {
+        if (!b->mats[k]) continue;
+        any = true;
+        if (b->fresh[k]) {
+            KCFD_HIP(hipMemcpyAsync(b->d_state + k, &b->h_state[k], sizeof(SysState),
+                                    hipMemcpyHostToDevice, s));
+            any_new = true;
+        }
+    }
+    if (!any) return KCFD_OK;
+    KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
+    if (!b->graph || b->graph_iters != b->opts.check_every)
+        KCFD_CHECK(capture_graph(b, b->opts.check_every));
+    if (any_new) pick_init(b->bp.precision, b->bp.precond)(b, s);
+    KCFD_HIP(hipGraphLaunch(b->graph, s));
+    if (b->bp.precision == KCFD_MIXED)
+        hipLaunchKernelGGL(k_restore, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                           b->d_tiles, b->d_counts);
+    hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                       b->d_tiles, b->d_counts, b->d_partials);
+    hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                       b->d_partials);
+    KCFD_HIP(hipGetLastError());
+    KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
+                            hipMemcpyDeviceToHost, s));
+    KCFD_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < b->nsys; ++k) {
+        b->fresh[k] = 0;
+        if (b->mats[k] && !b->reported[k] && b->h_state[k].status == S_DONE && b->h_state[k].true_done) {
+            b->reported[k] = 1;
+            if (finished) finished[*nfinished] = k;
+            ++*nfinished;
+        }
+    }
+    return KCFD_OK;
+}
+
+int kcfd_batch_detach(kcfd_batch *b, int slot, kcfd_solve_result *res) {
+    KCFD_ARG(b != nullptr, "null batch");
+    KCFD_ARG(b->dynamic, "not a dynamic batch");
+    KCFD_ARG(slot >= 0 && slot < b->nsys && b->mats[slot], "no system in this slot");
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    KCFD_HIP(hipSetDevice(b->ctx->device));
+    const SysState &st = b->h_state[slot];
+    if (res) {
+        std::memset(res, 0, sizeof(*res));
+        res->info = st.status == S_DONE ? st.info : KCFD_EBUSY;
+        res->iters = b->bp.precision == KCFD_FP64 ? st.iters : st.total_iters;
+        res->half_step = st.half;
+        res->refinements = st.refinements;
+        res->rel_resid = st.true_done ? (st.true_bb > 0 ? sqrt(st.true_rr / st.true_bb) : sqrt(st.true_rr))
+                                      : NAN;
+        res->bytes = model_bytes(b->mats[slot], b->bp.precond, b->bp.precision, st);
+        res->ms = 0.0;
+    }
+    KCFD_HIP(hipStreamSynchronize(b->ctx->stream));
+    hipFree(b->slot_mem[slot]);
+    b->slot_mem[slot] = nullptr;
+    b->mats[slot] = nullptr;
+    b->sys_n[slot] = 0;
+    std::memset(&b->h_desc[slot], 0, sizeof(SysDesc));
+    SysState fs{};
+    fs.status = S_FREE;
+    b->h_state[slot] = fs;
+    KCFD_HIP(hipMemcpyAsync(b->d_state + slot, &b->h_state[slot], sizeof(SysState),
+                            hipMemcpyHostToDevice, b->ctx->stream));
+    KCFD_HIP(hipStreamSynchronize(b->ctx->stream));
+    b->fresh[slot] = 0;
+    b->reported[slot] = 0;
+    b->dirty = true;
+    return KCFD_OK;
+}
+
+int kcfd_batch_members(kcfd_batch *b, int *members) {
+    KCFD_ARG(b && members, "null argument");
+    int n = 0;
+    for (auto *m : b->mats) n += m != nullptr;
+    *members = n;
     return KCFD_OK;
 }
 
@@ -1212,17 +1554,18 @@ int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out) {
 
 int kcfd_batch_bytes(kcfd_batch *b, long long *bytes) {
     KCFD_ARG(b && bytes, "null argument");
-    *bytes = (long long)b->slab_bytes + (long long)b->ntiles * (8 + 8 * kMaxPartials) +
-             (long long)b->nsys * (long long)(sizeof(SysDesc) + sizeof(SysState)) +
-             (long long)(b->nchunks_l + b->nchunks_u) * (long long)sizeof(Chunk);
+    long long v = (long long)b->slab_bytes;
+    for (int k = 0; k < b->nsys; ++k)
+        if (b->slot_mem[k]) v += (long long)vectors_per_system(b) * (long long)vec_stride(b, b->sys_n[k]);
+    *bytes = v + (long long)b->cap_tiles * (8 + 8 * kMaxPartials) +
+             (long long)b->nsys * (long long)(sizeof(SysDesc) + sizeof(SysState) + 2 * sizeof(int *)) +
+             (long long)(b->cap_chunks_l + b->cap_chunks_u) * (long long)sizeof(Chunk);
     return KCFD_OK;
 }
 
-// Re-launch one phase kernel `reps` times with every system forced to the
-// running state, bracketed by HIP events on the context stream; restores the
-// states afterwards.  phase: 1 = K2 (v = A M p + rt.v partials), 2 = K5 (x/r
-// update), 3 = lower triangular solve (ILU only), 4 = K1 (p update).  No
-// finalize kernel runs, so the state stays S_RUN for every repetition.
+// Time one phase kernel inside real iterations (see the loop below).
+// phase: 1 = K2 (v = A M p + rt.v partials), 2 = K5 (x/r update), 3 = lower
+// triangular solve (ILU only), 4 = K1 (p update).
 int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     KCFD_ARG(b && avg_ms && reps > 0, "bad argument");
     KCFD_ARG(phase >= 1 && phase <= 4, "unknown phase");
@@ -1242,6 +1585,7 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     KCFD_HIP(hipStreamSynchronize(s));
     for (int k = 0; k < b->nsys; ++k) {
         run[k] = saved[k];
+        if (!b->mats[k]) continue;
         run[k].status = S_RUN;
         if (run[k].omega == 0.0) run[k].omega = 1.0;
         if (run[k].rho == 0.0) run[k].rho = 1.0;
@@ -1283,11 +1627,12 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
     const double sv = b->bp.precision == KCFD_FP64 ? 8.0 : 4.0;
     double tot = 0.0;
     for (auto *m : b->mats) {
+        if (!m) continue;
         const double n = m->n, nnz = m->nnz;
         const double MA = nnz * (sv + 4) + (n + 1) * 4;
         const double MFh = 0.5 * nnz * (sv + 4) + (n + 1) * 4;  // one triangle + its level list
         if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
-        else if (phase == 2) tot += 8 * n * sv;
+        else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
         else if (phase == 3) tot += MFh + 2 * n * sv + (m->perm ? 4 * n : 0);  // + perm gather
         else if (phase == 4) tot += 4 * n * sv;
     }

@@ -149,4 +149,5 @@ int ensure_scratch(kcfd_ctx *ctx, size_t bytes);
 int ilu0_setup(kcfd_mat *m, bool want32);
 void ilu_free_order(kcfd_mat *m);
 int mat_free_precond(kcfd_mat *m);
+int mat_free_ilu_structure(kcfd_mat *m);
 }  // namespace kcfd

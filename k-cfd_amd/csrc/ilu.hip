@@ -206,14 +206,19 @@ static int build_order(kcfd_mat *m) {
     KCFD_HIP(hipMalloc(&m->f_indices, sizeof(int) * std::max(nnz, 1)));
     KCFD_HIP(hipMalloc(&m->f_diagpos, sizeof(int) * n));
     KCFD_HIP(hipMalloc(&m->f_uix, sizeof(int) * std::max(nnz, 1)));
-    KCFD_HIP(hipMemcpy(m->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-    KCFD_HIP(hipMemcpy(m->f_indptr, fp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
-    KCFD_HIP(hipMemcpy(m->f_diagpos, fd.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    // on the library stream (non-blocking: the null stream is not ordered
+    // with the kernels that read these); the host vectors are locals, so the
+    // copies finish before return
+    hipStream_t s = m->ctx->stream;
+    KCFD_HIP(hipMemcpyAsync(m->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    KCFD_HIP(hipMemcpyAsync(m->f_indptr, fp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
+    KCFD_HIP(hipMemcpyAsync(m->f_diagpos, fd.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     if (nnz) {
-        KCFD_HIP(hipMemcpy(m->f_vmap, vmap.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-        KCFD_HIP(hipMemcpy(m->f_indices, fx.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
-        KCFD_HIP(hipMemcpy(m->f_uix, uix.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+        KCFD_HIP(hipMemcpyAsync(m->f_vmap, vmap.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+        KCFD_HIP(hipMemcpyAsync(m->f_indices, fx.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, s));
+        KCFD_HIP(hipMemcpyAsync(m->f_uix, uix.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, s));
     }
+    KCFD_HIP(hipStreamSynchronize(s));
     m->ilu_built_order = m->ilu_order;
     return KCFD_OK;
 }
@@ -242,8 +247,9 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
         KCFD_HIP(hipMalloc(&m->lvl_rows_l, sizeof(int) * std::max(n, 1)));
         KCFD_HIP(hipMalloc(&m->lvl_rows_u, sizeof(int) * std::max(n, 1)));
         if (n) {
-            KCFD_HIP(hipMemcpy(m->lvl_rows_l, rl.data(), sizeof(int) * n, hipMemcpyHostToDevice));
-            KCFD_HIP(hipMemcpy(m->lvl_rows_u, ru.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+            KCFD_HIP(hipMemcpyAsync(m->lvl_rows_l, rl.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+            KCFD_HIP(hipMemcpyAsync(m->lvl_rows_u, ru.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+            KCFD_HIP(hipStreamSynchronize(s));   // rl, ru are locals
         }
     }
     const std::vector<int> &pl = m->h_lvl_ptr_l;
@@ -388,6 +394,58 @@ extern "C" int kcfd_precond_set_order(kcfd_mat *m, int order) {
     KCFD_ARG(order == KCFD_ORDER_NATURAL || order == KCFD_ORDER_COLOR, "unknown ILU(0) ordering");
     std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
     m->ilu_order = order;
+    return KCFD_OK;
+}
+
+// device copy of n ints (null stays null)
+static int dup_ints(const int *src, size_t n, int **dst, hipStream_t s) {
+    *dst = nullptr;
+    if (!src) return KCFD_OK;
+    KCFD_HIP(hipMalloc(dst, sizeof(int) * std::max<size_t>(n, 1)));
+    if (n) KCFD_HIP(hipMemcpyAsync(*dst, src, sizeof(int) * n, hipMemcpyDeviceToDevice, s));
+    return KCFD_OK;
+}
+
+extern "C" int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src) {
+    KCFD_ARG(dst && src, "null matrix");
+    KCFD_ARG(dst->ctx == src->ctx, "matrices of different contexts");
+    KCFD_ARG(src->lvl_rows_l != nullptr, "source has no ILU(0) structure (set it up first)");
+    KCFD_ARG(dst->n == src->n && dst->nnz == src->nnz && dst->h_indptr == src->h_indptr &&
+                 dst->h_indices == src->h_indices,
+             "matrices differ in structure");
+    std::lock_guard<std::recursive_mutex> g(dst->ctx->mu);
+    KCFD_HIP(hipSetDevice(dst->ctx->device));
+    KCFD_HIP(hipStreamSynchronize(dst->ctx->stream));
+    if (dst->solver) {
+        kcfd_batch_destroy(dst->solver);
+        dst->solver = nullptr;
+    }
+    mat_free_ilu_structure(dst);
+    const size_t n = (size_t)dst->n, nnz = (size_t)dst->nnz;
+    if (src->perm) {
+        KCFD_CHECK(dup_ints(src->perm, n, &dst->perm, dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->f_vmap, nnz, &dst->f_vmap, dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->f_indptr, n + 1, &dst->f_indptr, dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->f_indices, nnz, &dst->f_indices, dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->f_diagpos, n, &dst->f_diagpos, dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->f_uix, nnz, &dst->f_uix, dst->ctx->stream));
+        dst->h_perm = src->h_perm;
+    } else {
+        dst->f_indptr = dst->indptr;
+        dst->f_indices = dst->indices;
+        dst->f_diagpos = dst->diagpos;
+        dst->f_uix = dst->indices;
+    }
+    KCFD_CHECK(dup_ints(src->lvl_rows_l, n, &dst->lvl_rows_l, dst->ctx->stream));
+    KCFD_CHECK(dup_ints(src->lvl_rows_u, n, &dst->lvl_rows_u, dst->ctx->stream));
+    dst->h_lvl_ptr_l = src->h_lvl_ptr_l;
+    dst->h_lvl_ptr_u = src->h_lvl_ptr_u;
+    dst->nlev_l = src->nlev_l;
+    dst->nlev_u = src->nlev_u;
+    dst->ncolors = src->ncolors;
+    dst->ilu_order = src->ilu_built_order;
+    dst->ilu_built_order = src->ilu_built_order;
+    KCFD_HIP(hipStreamSynchronize(dst->ctx->stream));
     return KCFD_OK;
 }
 

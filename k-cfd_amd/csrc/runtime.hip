@@ -80,17 +80,26 @@ __global__ void k_sell_struct(int n, const int *__restrict__ ip, const int *__re
     for (int k = 0; k < w; ++k) six[base + 64 * k] = k < len ? ix[q0 + k] : r;
 }
 
-__global__ void k_sell_values(int n, const int *__restrict__ ip, const int *__restrict__ sptr,
-                              const double *__restrict__ a, double *__restrict__ sa64,
-                              float *__restrict__ sa32) {
+// SELL values.  The FP32 copy is kept in difference form for the mixed
+// solver: the diagonal position holds the row sum sum_j a_ij (summed in
+// FP64), so that (A z)_i = sum_{j!=i} a_ij (z_j - z_i) + rowsum_i z_i.
+// Rounding the off-diagonals to FP32 then leaves the row sums (and the
+// near-null space of a diffusion operator) intact, where rounding a_ii
+// separately would perturb the smallest eigenvalues by eps_32 |a_ii|.
+__global__ void k_sell_values(int n, const int *__restrict__ ip, const int *__restrict__ ix,
+                              const int *__restrict__ sptr, const double *__restrict__ a,
+                              double *__restrict__ sa64, float *__restrict__ sa32) {
+#pragma clang fp contract(off)
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int s = r >> 6, base = sptr[s] + (r & 63), w = (sptr[s + 1] - sptr[s]) >> 6;
     const int q0 = ip[r], len = ip[r + 1] - q0;
+    double rowsum = 0.0;
+    for (int k = 0; k < len; ++k) rowsum += a[q0 + k];
     for (int k = 0; k < w; ++k) {
         const double v = k < len ? a[q0 + k] : 0.0;
         sa64[base + 64 * k] = v;
-        if (sa32) sa32[base + 64 * k] = (float)v;
+        if (sa32) sa32[base + 64 * k] = (float)(k < len && ix[q0 + k] == r ? rowsum : v);
     }
 }
 
@@ -109,20 +118,26 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(size_t n16, const double
     for (; i < n16; i += stride) dst[i] = src[i];
 }
 
+// Value buffers of the preconditioner.  The ILU(0) ordering and level
+// schedule depend on the structure only and survive (mat_free_ilu_structure).
 int mat_free_precond(kcfd_mat *m) {
     hipFree(m->sa32); m->sa32 = nullptr;
     hipFree(m->dinv); m->dinv = nullptr;
     hipFree(m->dinv32); m->dinv32 = nullptr;
     hipFree(m->lu); m->lu = nullptr;
     hipFree(m->lu32); m->lu32 = nullptr;
+    m->have32 = false;
+    m->pc_kind = KCFD_PC_NONE;
+    return KCFD_OK;
+}
+
+int mat_free_ilu_structure(kcfd_mat *m) {
     hipFree(m->lvl_rows_l); m->lvl_rows_l = nullptr;
     hipFree(m->lvl_rows_u); m->lvl_rows_u = nullptr;
     ilu_free_order(m);
     m->h_lvl_ptr_l.clear();
     m->h_lvl_ptr_u.clear();
     m->nlev_l = m->nlev_u = 0;
-    m->have32 = false;
-    m->pc_kind = KCFD_PC_NONE;
     return KCFD_OK;
 }
 
@@ -327,12 +342,15 @@ int kcfd_mat_create(kcfd_ctx *ctx, int n, int nnz, const int *indptr, const int 
         set_error("hipMalloc failed for a matrix of n=%d nnz=%d", n, nnz);
         return fail(KCFD_EOOM);
     }
-    if (hipMemcpy(m->indptr, indptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice) != hipSuccess ||
-        (nnz && hipMemcpy(m->indices, indices, sizeof(int) * nnz, hipMemcpyHostToDevice) != hipSuccess) ||
-        (n && hipMemcpy(m->diagpos, m->h_diagpos.data(), sizeof(int) * n, hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemset(m->val, 0, sizeof(double) * (nnz ? nnz : 1)) != hipSuccess ||
-        hipMemset(m->rhs, 0, sizeof(double) * (n ? n : 1)) != hipSuccess ||
-        hipMemset(m->diagt, 0, sizeof(double) * (n ? n : 1)) != hipSuccess) {
+    // uploads and fills on the library stream (created non-blocking: the
+    // null stream is not ordered with the kernels that use these arrays)
+    hipStream_t st = ctx->stream;
+    if (hipMemcpyAsync(m->indptr, indptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+        (nnz && hipMemcpyAsync(m->indices, indices, sizeof(int) * nnz, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (n && hipMemcpyAsync(m->diagpos, m->h_diagpos.data(), sizeof(int) * n, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        hipMemsetAsync(m->val, 0, sizeof(double) * (nnz ? nnz : 1), st) != hipSuccess ||
+        hipMemsetAsync(m->rhs, 0, sizeof(double) * (n ? n : 1), st) != hipSuccess ||
+        hipMemsetAsync(m->diagt, 0, sizeof(double) * (n ? n : 1), st) != hipSuccess) {
         set_error("matrix upload failed");
         return fail(KCFD_EHIP);
     }
@@ -342,18 +360,18 @@ int kcfd_mat_create(kcfd_ctx *ctx, int n, int nnz, const int *indptr, const int 
         set_error("hipMalloc failed for the SELL copy (n=%d, padded nnz=%d)", n, m->sell_nnz);
         return fail(KCFD_EOOM);
     }
-    if (hipMemcpy(m->slice_ptr, m->h_slice_ptr.data(), sizeof(int) * (m->nslices + 1),
-                  hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpyAsync(m->slice_ptr, m->h_slice_ptr.data(), sizeof(int) * (m->nslices + 1),
+                       hipMemcpyHostToDevice, st) != hipSuccess) {
         set_error("SELL upload failed");
         return fail(KCFD_EHIP);
     }
-    if (n) {
+    if (n)
         hipLaunchKernelGGL(k_sell_struct, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
                            n, m->indptr, m->indices, m->slice_ptr, m->six);
-        if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
-            set_error("SELL structure build failed");
-            return fail(KCFD_EHIP);
-        }
+    // the caller's index arrays are read by the copies above
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        set_error("SELL structure build failed");
+        return fail(KCFD_EHIP);
     }
     *out = m;
     return KCFD_OK;
@@ -366,6 +384,7 @@ int kcfd_mat_destroy(kcfd_mat *m) {
     hipStreamSynchronize(m->ctx->stream);
     if (m->solver) kcfd_batch_destroy(m->solver);
     mat_free_precond(m);
+    mat_free_ilu_structure(m);
     hipFree(m->indptr);
     hipFree(m->indices);
     hipFree(m->diagpos);
@@ -462,7 +481,7 @@ int kcfd_precond_setup(kcfd_mat *m, int kind, int precision) {
     }
     if (n) {
         hipLaunchKernelGGL(k_sell_values, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
-                           m->indptr, m->slice_ptr, m->val, m->sa64, m->sa32);
+                           m->indptr, m->indices, m->slice_ptr, m->val, m->sa64, m->sa32);
         KCFD_HIP(hipGetLastError());
     }
     if (kind == KCFD_PC_JACOBI && n) {

@@ -23,6 +23,27 @@ def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
 
+def fp64_continuation(mat, b_ptr, x_ptr, opts, mixed_res):
+    """Continue a mixed solve that did not converge in FP64 from x (the
+    best iterate).  The matrix set up for MIXED also holds its FP64 values
+    and preconditioner, so no set-up is repeated."""
+    o = L.SolveOpts()
+    ctypes.pointer(o)[0] = opts
+    o.precision = L.FP64
+    r = L.SolveResult()
+    check(lib().kcfd_solve(mat.h, b_ptr, x_ptr, ctypes.byref(o), ctypes.byref(r)))
+    out = L.SolveResult()
+    out.info = r.info
+    out.iters = mixed_res.iters + r.iters
+    out.half_step = r.half_step
+    out.refinements = mixed_res.refinements
+    out.rel_resid = r.rel_resid
+    out.bytes = mixed_res.bytes + r.bytes
+    out.ms = mixed_res.ms + r.ms
+    out.fallback = 1
+    return out
+
+
 def default_opts(rtol=1e-8, atol=0.0, maxiter=0, precond="jacobi", precision="fp64",
                  inner_rtol=0.1, max_refine=1000, check_every=8, grid_per_cu=0,
                  trsv_per_cu=0) -> L.SolveOpts:
@@ -133,6 +154,10 @@ class DeviceMatrix:
         check(lib().kcfd_ilu0_get(self.h, ptr(lu), ctypes.byref(a), ctypes.byref(b)))
         return lu, a.value, b.value
 
+    def copy_order_from(self, other: "DeviceMatrix"):
+        """Take ``other``'s ILU(0) ordering and level schedule (same structure)."""
+        check(lib().kcfd_mat_copy_order(self.h, other.h))
+
     def ilu0_order(self):
         """(perm, ncolors) of the current ILU(0) factors: perm[k] is the row of
         A at position k of P A P^T (identity and 0 colours when natural)."""
@@ -144,15 +169,24 @@ class DeviceMatrix:
     def spmv(self, x_dev: DevBuf, y_dev: DevBuf):
         check(lib().kcfd_mat_spmv(self.h, x_dev.p, y_dev.p))
 
-    def solve_dev(self, b_ptr, x_ptr, opts: L.SolveOpts, setup: bool = True) -> L.SolveResult:
+    def solve_dev(self, b_ptr, x_ptr, opts: L.SolveOpts, setup: bool = True,
+                  fallback: bool = True) -> L.SolveResult:
         """Solve on device buffers.  The preconditioner is rebuilt from the
         current values (as create_preconditioner is called on every
         solve_linear_system, core/linear_solver.py:56) unless ``setup`` is
-        False and it was set up for these options already."""
+        False and it was set up for these options already.
+
+        A mixed-precision solve that stops without converging (FP32 Krylov
+        recurrences can stall on very ill-conditioned systems) continues in
+        FP64 from the best iterate when ``fallback`` is set; the result then
+        counts both phases and ``res.fallback`` is 1."""
         if setup or self.pc != (opts.precond, opts.precision):
             self.setup_precond(opts.precond, opts.precision)
         res = L.SolveResult()
         check(lib().kcfd_solve(self.h, b_ptr, x_ptr, ctypes.byref(opts), ctypes.byref(res)))
+        res.fallback = 0
+        if fallback and opts.precision == L.MIXED and res.info != 0:
+            res = fp64_continuation(self, b_ptr, x_ptr, opts, res)
         return res
 
     def solve(self, b, x0=None, **kw):

@@ -140,9 +140,9 @@ def hex_mesh(nx, ny=None, nz=None):
     return build_fvmesh_from_arrays(nodes, cells, bf, ndim=3, cell_face_node_list=faces)
 
 
-def quad_convdiff(n=128, seed=1001) -> Case:
+def quad_convdiff(n=128, seed=1001, mesh=None) -> Case:
     rng = np.random.default_rng(seed)
-    mesh = quad_mesh(n)
+    mesh = quad_mesh(n) if mesh is None else mesh
     nc = mesh.n_cells
     gamma = np.exp(rng.normal(math.log(1e-2), 0.5, nc))
     psi, vel = _stream_function(rng)
@@ -169,6 +169,24 @@ def hex_pressure(n=128, seed=1002, mesh=None) -> Case:
     pdir = [nm for nm in mesh.boundary_patches if nm == "outlet"]
     recipe = [("pressure", dict(dP=dP, rho=1.0, dirichlet=pdir))]
     return Case("hex_pressure", mesh, recipe, "ilu0", rhs=b, meta=dict(n=n, seed=seed))
+
+
+def pressure_case(mesh, seed, dirichlet=None) -> Case:
+    """SIMPLE pressure correction on any mesh (the hex_pressure recipe):
+    a_P ~ lognormal(0, 1), d_P = V / a_P, p' = 0 on ``dirichlet`` patches
+    (default: 'outlet' if present, else the right / first patch),
+    b ~ U[-1, 1]."""
+    rng = np.random.default_rng(seed)
+    nc = mesh.n_cells
+    aP = np.exp(rng.normal(0.0, 1.0, nc))
+    dP = mesh.cell_volume / aP
+    b = rng.uniform(-1.0, 1.0, nc)
+    if dirichlet is None:
+        names = list(mesh.boundary_patches)
+        dirichlet = [nm for nm in ("outlet", "right") if nm in names][:1] or names[:1]
+    recipe = [("pressure", dict(dP=dP, rho=1.0, dirichlet=list(dirichlet)))]
+    return Case(f"{'hex' if mesh.ndim == 3 else 'quad'}_pressure", mesh, recipe, "ilu0", rhs=b,
+                meta=dict(seed=seed))
 
 
 def hex_momentum(n=128, seed=1002, mesh=None) -> Case:
@@ -283,68 +301,114 @@ def case_of_size(kind, ncells, seed):
 
 # ---------------------------------------------------------- device assembly
 
+class DeviceRecipe:
+    """A case's assembly recipe with its arrays uploaded once; ``assemble``
+    plays it into a matrix of the mesh's structure (many systems on one
+    mesh share the uploads)."""
+
+    def __init__(self, dmesh, case: Case):
+        from . import _lib as L
+        from ._lib import DevBuf, check, lib
+
+        self.dmesh = dmesh
+        self.case = case
+        ctx = dmesh.ctx
+        mesh = case.mesh
+        self.keep = []
+        self.ops = []
+
+        def dev(a):
+            b = DevBuf.from_array(ctx, np.ascontiguousarray(a, np.float64))
+            self.keep.append(b)
+            return b.p
+
+        F_dev = None
+        F_scaled = None
+        for op, kw in case.recipe:
+            if op == "mass_flux":
+                F_dev = DevBuf(ctx, mesh.n_faces)
+                self.keep.append(F_dev)
+                check(lib().kcfd_mass_flux(dmesh.h, dev(kw["U"]),
+                                           dev(dmesh.boundary_array(kw["Ub"], width=mesh.ndim)),
+                                           float(kw["rho"]), F_dev.p))
+                ctx.sync()
+            elif op == "diffusion":
+                g = dev(kw["gamma"])
+                self.ops.append(lambda A, g=g: check(lib().kcfd_asm_diffusion(dmesh.h, A.h, g)))
+            elif op == "upwind":
+                f = dev(kw["F"])
+                self.ops.append(lambda A, f=f: check(lib().kcfd_asm_upwind(dmesh.h, A.h, f)))
+            elif op == "upwind_scaled":
+                F_scaled = F_dev.download() * np.asarray(kw["scale_owner"])[mesh.owner]
+                f = dev(F_scaled)
+                self.ops.append(lambda A, f=f: check(lib().kcfd_asm_upwind(dmesh.h, A.h, f)))
+            elif op in ("boundary", "boundary_scaled"):
+                F = kw["F"] if op == "boundary" else F_scaled
+                kinds = np.array([{"dirichlet": L.BC_DIRICHLET, "neumann": L.BC_NEUMANN,
+                                   "zero_gradient": L.BC_ZERO_GRADIENT}[
+                                      kw["bc"].get(nm, {"type": "zero_gradient"})["type"]]
+                                  for nm in dmesh.patch_names], np.int32)
+                vals = np.array([float(kw["bc"].get(nm, {}).get("value", 0.0)) for nm in dmesh.patch_names])
+                args = (dev(kw["gamma"]), dev(F), dev(dmesh.boundary_array(kw["phib"])))
+                self.keep += [kinds, vals]
+                self.ops.append(lambda A, a=args, k=kinds, v=vals: check(lib().kcfd_asm_boundary(
+                    dmesh.h, A.h, a[0], a[1], a[2], k.ctypes.data, v.ctypes.data)))
+            elif op == "pressure":
+                pd = np.array([1 if nm in kw["dirichlet"] else 0 for nm in dmesh.patch_names], np.int32)
+                dp = dev(kw["dP"])
+                self.keep.append(pd)
+                self.ops.append(lambda A, dp=dp, r=float(kw["rho"]), pd=pd: check(
+                    lib().kcfd_asm_pressure(dmesh.h, A.h, dp, r, pd.ctypes.data)))
+            elif op == "source":
+                src = dev(kw["S"])
+                self.ops.append(lambda A, src=src: check(lib().kcfd_asm_source(dmesh.h, A.h, src)))
+            elif op == "cell_terms":
+                kd, ks = dev(kw["kd"]), dev(kw["ks"])
+                self.ops.append(lambda A, kd=kd, ks=ks: check(lib().kcfd_asm_cell_terms(dmesh.h, A.h, kd, ks)))
+            elif op == "temporal":
+                po = dev(kw["phi_old"])
+                self.ops.append(lambda A, po=po, r=float(kw["rho"]), dt=float(kw["dt"]): check(
+                    lib().kcfd_asm_temporal(dmesh.h, A.h, r, dt, po)))
+            elif op == "muscl":
+                phib = dev(dmesh.boundary_array(kw["phib"]))
+                g = DevBuf(ctx, mesh.n_cells * mesh.ndim)
+                c = DevBuf(ctx, mesh.n_cells)
+                self.keep += [g, c]
+                phi, F = dev(kw["phi"]), dev(kw["F"])
+                lim = L.LIMITERS[kw["limiter"]]
+
+                def muscl(A, phi=phi, phib=phib, F=F, g=g, c=c, lim=lim):
+                    check(lib().kcfd_grad_gg(dmesh.h, phi, phib, g.p))
+                    check(lib().kcfd_muscl(dmesh.h, phi, F, g.p, lim, c.p))
+                    A.set_rhs(A.get_rhs() + c.download())
+                self.ops.append(muscl)
+            else:
+                raise ValueError(f"unknown recipe op {op}")
+
+    def assemble(self, A, rhs=None):
+        """Play the recipe into A (reset first); rhs overrides the assembled
+        right-hand side (default: the case's own override, if any)."""
+        from ._lib import check, lib
+        check(lib().kcfd_asm_reset(A.h))
+        for f in self.ops:
+            f(A)
+        self.dmesh.ctx.sync()
+        rhs = self.case.rhs if rhs is None else rhs
+        if rhs is not None:
+            A.set_rhs(rhs)
+        return A
+
+    def close(self):
+        for b in self.keep:
+            if hasattr(b, "free"):
+                b.free()
+        self.keep = []
+
+
 def assemble_device(dmesh, case: Case, ctx=None):
     """Play the recipe on the device; returns the DeviceMatrix (rhs set)."""
-    from . import _lib as L
-    from ._lib import DevBuf, check, lib
-
-    ctx = dmesh.ctx
-    mesh = case.mesh
-    A = dmesh.new_system()
-    keep = []
-
-    def dev(a):
-        b = DevBuf.from_array(ctx, np.ascontiguousarray(a, np.float64))
-        keep.append(b)
-        return b.p
-
-    F_dev = None
-    F_host = None
-    for op, kw in case.recipe:
-        if op == "mass_flux":
-            F_dev_buf = DevBuf(ctx, mesh.n_faces)
-            keep.append(F_dev_buf)
-            check(lib().kcfd_mass_flux(dmesh.h, dev(kw["U"]), dev(dmesh.boundary_array(kw["Ub"], width=mesh.ndim)),
-                                       float(kw["rho"]), F_dev_buf.p))
-            F_dev = F_dev_buf
-        elif op == "diffusion":
-            check(lib().kcfd_asm_diffusion(dmesh.h, A.h, dev(kw["gamma"])))
-        elif op == "upwind":
-            check(lib().kcfd_asm_upwind(dmesh.h, A.h, dev(kw["F"])))
-        elif op == "upwind_scaled":
-            F_host = F_dev.download() * np.asarray(kw["scale_owner"])[mesh.owner]
-            check(lib().kcfd_asm_upwind(dmesh.h, A.h, dev(F_host)))
-        elif op in ("boundary", "boundary_scaled"):
-            F = kw["F"] if op == "boundary" else F_host
-            kinds = np.array([{"dirichlet": L.BC_DIRICHLET, "neumann": L.BC_NEUMANN,
-                               "zero_gradient": L.BC_ZERO_GRADIENT}[kw["bc"].get(nm, {"type": "zero_gradient"})["type"]]
-                              for nm in dmesh.patch_names], np.int32)
-            vals = np.array([float(kw["bc"].get(nm, {}).get("value", 0.0)) for nm in dmesh.patch_names])
-            phib = dmesh.boundary_array(kw["phib"])
-            check(lib().kcfd_asm_boundary(dmesh.h, A.h, dev(kw["gamma"]), dev(F), dev(phib),
-                                          kinds.ctypes.data, vals.ctypes.data))
-        elif op == "pressure":
-            pd = np.array([1 if nm in kw["dirichlet"] else 0 for nm in dmesh.patch_names], np.int32)
-            check(lib().kcfd_asm_pressure(dmesh.h, A.h, dev(kw["dP"]), float(kw["rho"]), pd.ctypes.data))
-        elif op == "source":
-            check(lib().kcfd_asm_source(dmesh.h, A.h, dev(kw["S"])))
-        elif op == "cell_terms":
-            check(lib().kcfd_asm_cell_terms(dmesh.h, A.h, dev(kw["kd"]), dev(kw["ks"])))
-        elif op == "temporal":
-            check(lib().kcfd_asm_temporal(dmesh.h, A.h, float(kw["rho"]), float(kw["dt"]), dev(kw["phi_old"])))
-        elif op == "muscl":
-            phib = dmesh.boundary_array(kw["phib"])
-            g = DevBuf(ctx, mesh.n_cells * mesh.ndim)
-            c = DevBuf(ctx, mesh.n_cells)
-            keep += [g, c]
-            phi = dev(kw["phi"])
-            check(lib().kcfd_grad_gg(dmesh.h, phi, dev(phib), g.p))
-            check(lib().kcfd_muscl(dmesh.h, phi, dev(kw["F"]), g.p, L.LIMITERS[kw["limiter"]], c.p))
-            rhs = A.get_rhs() + c.download()
-            A.set_rhs(rhs)
-        else:
-            raise ValueError(f"unknown recipe op {op}")
-    ctx.sync()
-    if case.rhs is not None:
-        A.set_rhs(case.rhs)
-    return A
+    rec = DeviceRecipe(dmesh, case)
+    try:
+        return rec.assemble(dmesh.new_system())
+    finally:
+        rec.close()

@@ -1,0 +1,85 @@
+"""Serving loop on the GPU: continuous batching, HBM admission, and every
+result the same alone and in the mix."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def templates(ctx):
+    from kcfd_amd import serve as S
+    out = []
+    for i, (kind, n) in enumerate([("quad", 1000), ("hex", 3375), ("tet", 4000), ("quad", 9000),
+                                   ("hex", 8000)]):
+        pres, mom = S.build_template_cases(kind, n, 6000 + i)
+        out.append(S.Template(ctx, kind, n, pres, mom, "ilu0", "mixed", "color"))
+    yield out
+    for t in out:
+        t.close()
+
+
+def _trace(n, rate, ntemp, seed=1):
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    return [(float(t[i]), int(rng.integers(ntemp)), 7000 + i) for i in range(n)]
+
+
+def test_serve_mix_equals_alone(ctx, templates):
+    """Requests that overlap on the device give exactly the solutions (and
+    iteration counts) they give when each system is solved alone."""
+    from kcfd_amd import serve as S
+    trace = _trace(24, 2000.0, len(templates))
+    srv = S.Server(ctx, templates, precision="mixed", precond="ilu0", batched=True, check_every=4)
+    srv.keep_solutions = True
+    try:
+        rep = srv.run(trace)
+        print(rep.summary())
+        for rid, (xs, res) in sorted(srv.solutions.items()):
+            for k, q in enumerate(res):
+                if q[0] != 0:
+                    print(f"not converged: request {rid} template {trace[rid][1]} system {k}: "
+                          f"info={q[0]} iters={q[1]} refinements={q[2]} rel_resid={q[3]:.3e}")
+        assert rep.requests == 24 and rep.not_converged == 0 and rep.max_rel_resid < 1e-8
+        assert rep.peak_requests > 1 and rep.mean_members > 4     # work overlapped
+        for rid, (t, ti, seed) in enumerate(trace):
+            req = S.Request(rid, 0.0, templates[ti], seed)
+            srv._admit(req, 0.0)
+            for k, (A, b, x) in enumerate(req.systems()):
+                r = A.solve_dev(b.p, x.p, srv.opts, setup=False)
+                xs_mix, res_mix = srv.solutions[rid]
+                assert r.iters == res_mix[k][1] and r.info == res_mix[k][0] == 0
+                assert np.array_equal(x.download(), xs_mix[k]), (rid, k)
+            req.release()
+    finally:
+        srv.close()
+
+
+def test_serve_admission_by_hbm(ctx, templates):
+    """A burst admitted under an HBM budget: the footprint in use never
+    exceeds it, fewer requests overlap than without it, all complete."""
+    from kcfd_amd import serve as S
+    trace = _trace(10, 5000.0, len(templates), seed=2)
+    reps = {}
+    for budget in (None, int(2.5 * max(t.bytes for t in templates))):
+        srv = S.Server(ctx, templates, precision="mixed", batched=True, hbm_budget=budget)
+        try:
+            reps[budget] = srv.run(trace)
+        finally:
+            srv.close()
+        print(budget, reps[budget].summary())
+        assert reps[budget].requests == 10 and reps[budget].not_converged == 0
+    free, capped = reps[None], reps[budget]
+    assert capped.peak_hbm_bytes <= budget
+    assert capped.peak_requests < free.peak_requests
+    assert capped.queue_ms.max() > free.queue_ms.max()
+
+
+def test_serve_one_at_a_time_fp64(ctx, templates):
+    from kcfd_amd import serve as S
+    srv = S.Server(ctx, templates, precision="fp64", batched=False)
+    # the templates were set up for mixed: the baseline sets up its own
+    rep = srv.run(_trace(6, 1000.0, len(templates), seed=3))
+    print(rep.summary())
+    assert rep.requests == 6 and rep.not_converged == 0 and rep.peak_requests == 1
