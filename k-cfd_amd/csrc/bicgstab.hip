@@ -445,8 +445,6 @@ __global__ __launch_bounds__(kBlock) void f_spmv_p(const SysDesc *__restrict__ s
             st.restore = 1;
             st.status = S_DONE;
             return;
-        } else {
-            st.restart = 1;   // a replacement that did not improve: fresh shadow residual
         }
         st.rmax = sqrt(tot[2]);
         st.rr = tot[2];
@@ -652,7 +650,10 @@ __global__ __launch_bounds__(kBlock) void f_xr(const SysDesc *__restrict__ sys, 
         // reliable update when the FP32 residual claims convergence or has
         // dropped by delta since the last update; restart on divergence,
         // breakdown or the inner iteration cap
-        const bool diverged = !(rn == rn) || rn > 1e3 * st.rbest;
+        // (BiCGSTAB's residual spikes by orders of magnitude on hard systems;
+        // only an explosion counts as divergence -- slow drift is caught by
+        // the stagnation window at the next replacement)
+        const bool diverged = !(rn == rn) || rn > 1e6 * st.rmax;
         if (diverged || fabs(rho) < bp.rhotol || fabs(st.omega) < bp.rhotol ||
             st.iters >= bp.inner_maxiter) {
             st.restart = 1;
