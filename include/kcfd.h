@@ -214,8 +214,9 @@ int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out);
  * runs `reps` solver iterations (after one warm-up) launched directly on the
  * context stream, with HIP events around one phase kernel in each, and
  * returns that kernel's mean duration: phase 1 = v = A M p with the fused
- * rt.v reduction, 2 = fused x/r update, 3 = ILU(0) lower triangular solve,
- * 4 = p update.  The solver state is restored afterwards; the vectors (x
+ * rt.v reduction, 2 = fused x/r update, 3 = ILU(0) apply to p (for a
+ * two-colour ordering both of its kernels, otherwise the sync-free lower
+ * triangular solve), 4 = p update.  The solver state is restored afterwards; the vectors (x
  * included) are overwritten.  kcfd_batch_kernel_bytes gives that kernel's
  * algorithmic HBM bytes per launch (summed over the batch). */
 int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms);

@@ -50,7 +50,9 @@ class SolveResult(ctypes.Structure):
                 ("bytes", ctypes.c_double), ("ms", ctypes.c_double)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_}
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["fallback"] = getattr(self, "fallback", 0)   # mixed solve finished in FP64
+        return d
 
 
 _P = ctypes.c_void_p

@@ -6,7 +6,9 @@ BiCGSTAB (libkcfd) with scipy's semantics: the reference calls
 default ``rtol=1e-5`` (linear_solver.py:59), so the drop-in stops at
 ``||r|| < max(tol, rtol*||b||)`` with ``rtol`` defaulting to 1e-5 unless an
 ``rtol=`` keyword is passed.  Extra keywords: ``rtol``, ``precision``
-('fp64' | 'mixed'), ``inner_rtol``, ``return_info``.  A non-converged solve
+('fp64' | 'mixed'), ``inner_rtol`` (mixed: residual-replacement trigger,
+default 0.1), ``order`` ('natural' | 'color', the ILU(0) row ordering),
+``return_info``.  A non-converged solve
 prints the reference's message and returns x; a breakdown returns x
 silently (linear_solver.py:60-62).
 
@@ -67,8 +69,12 @@ def solve_linear_system(system: FVMSystem, x0: np.ndarray = None, method: str = 
     ctx = A.ctx
     x = L.DevBuf.from_array(ctx, np.zeros(system.n) if x0 is None else np.asarray(x0, dtype=np.float64))
     opts = default_opts(rtol=rtol, atol=tol, maxiter=maxiter, precond=preconditioner, precision=precision,
-                        inner_rtol=float(precond_kwargs.get("inner_rtol", 1e-4)))
-    res = A.solve_dev(b_ptr, x.p, opts)
+                        inner_rtol=float(precond_kwargs.get("inner_rtol", 0.1)))
+    # preconditioner from the current values (as the reference rebuilds it on
+    # every call, core/linear_solver.py:56); order='color' selects the
+    # coloured ILU(0) row ordering
+    A.setup_precond(preconditioner, precision, order=precond_kwargs.get("order"))
+    res = A.solve_dev(b_ptr, x.p, opts, setup=False)
     out = x.download()
     if res.info > 0:
         print(f"[bicgstab] 수렴 실패: {res.info} 반복 후 중단")

@@ -50,6 +50,6 @@ def create_preconditioner(A_csr, method: str = "none", **kwargs):
         raise ValueError(f"Unknown preconditioner: {method}")
     A = A_csr if isinstance(A_csr, DeviceMatrix) else DeviceMatrix.from_csr(L.default_context(),
                                                                             sparse.csr_matrix(A_csr))
-    A.setup_precond(method, "fp64")
+    A.setup_precond(method, "fp64", order=kwargs.get("order"))
     M = DevicePreconditioner(A, method)
     return M, {"method": method, "setup_time": time.time() - t0}

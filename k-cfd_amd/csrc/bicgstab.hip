@@ -20,23 +20,27 @@
 // triangular solves after K1 and after K3):
 //   K1 p = r + beta (p - omega v)        (p = r at iteration 0)
 //   K2 v = A M p; rt.v           F2: alpha
-//   K3 s = r - alpha v; ||s||    F3: half-step exit
-//   K4 t = A M s; t.s, t.t       F4: omega
+//   K3 s = r - alpha v; ||s||
+//   K4 t = A M s; t.s, t.t       F4: half-step exit on ||s|| (scipy's test,
+//                                    made one finalize later), else omega
 //   K5 x += alpha M p; x += omega M s; r = s - omega t; ||r||, rt.r
+//      (half-step exit: x += alpha M p only)
 //                                F5: convergence / breakdown / beta
 // Reductions: wave64 xor-shuffle + LDS per tile, one partial record per tile,
 // and a finalize kernel (one workgroup per system) that sums the partials in
 // tile order -- deterministic, and free of in-kernel fences (on gfx950 an
 // agent-scope fence writes back and invalidates the XCD's L2).
 //
-// KCFD_MIXED: the iteration runs on FP32 matrix values, FP32 preconditioner
-// and FP32 vectors (products exact in FP64, sums and all dot products in FP64)
-// and solves for the correction d; reliable updates (residual replacement:
-// x += d, r = b - A x in FP64 on the FP64 matrix, r32 = r) are taken whenever
-// the FP32 residual has dropped by `delta` since the last one, when it claims
-// convergence, or at the iteration limit, without restarting the Krylov
-// space (the shadow residual and p are kept; rho and beta are recomputed from
-// the replaced residual).  Convergence is judged on the FP64 true residual.
+// KCFD_MIXED: the iteration runs on FP32 matrix values (difference form, see
+// row_amz), FP32 preconditioner and FP32 vectors (products exact in FP64,
+// sums and all dot products in FP64); the iterate x stays FP64.  Reliable
+// updates (residual replacement: r = b - A x in FP64 on the FP64 matrix,
+// r32 = r) are taken whenever the FP32 residual has dropped by `delta` since
+// the last one, when it claims convergence, or at the iteration limit,
+// without restarting the Krylov space (the shadow residual and p are kept;
+// rho and beta are recomputed from the replaced residual).  Convergence is
+// judged on the FP64 true residual; a stop without convergence returns the
+// best iterate seen.
 #include "common.hpp"
 
 #include <algorithm>
@@ -58,11 +62,17 @@ enum : int {
 };
 
 struct SysDesc {
-    int n, tile0, ntiles, pad0;
+    int n, tile0, ntiles;
+    int polled;   // ILU(0) solved by the sync-free kernels (sentinel-flagged buffers)
     // ILU(0) factor structure: CSR of P A P^T (A's own CSR in the natural
     // ordering); uix = columns as rows of A; perm = row of A at each
     // position (null: identity)
     const int *ip, *ix, *dpos, *uix, *perm;
+    // chunked L / U solve layout (kcfd_mat::chl_off): offsets per chunk,
+    // columns, values, and u_ii by row
+    const int *chl_off, *chl_col, *chu_off, *chu_col, *chl_ncol;
+    const double *chl_v64, *chu_v64, *ud64;
+    const float *chl_v32, *chu_v32, *ud32;
     const int *sptr, *six;          // SELL-64 structure
     const double *sa64;
     const float *sa32;
@@ -97,7 +107,7 @@ struct BatchParams {
 
 // ILU trisolve work chunk: up to 64 consecutive rows of one system's level list
 struct Chunk {
-    int sys, start, count, pad;
+    int sys, start, count, local;   // local: the chunk's index in its matrix's layout
 };
 
 // ----------------------------------------------------------- tile walk
@@ -120,6 +130,16 @@ template <> __device__ __forceinline__ const float *jac<float>(const SysDesc &d)
 template <typename V> __device__ __forceinline__ const V *luv(const SysDesc &d);
 template <> __device__ __forceinline__ const double *luv<double>(const SysDesc &d) { return d.lu64; }
 template <> __device__ __forceinline__ const float *luv<float>(const SysDesc &d) { return d.lu32; }
+
+template <typename V> __device__ __forceinline__ const V *chl_vals(const SysDesc &d);
+template <> __device__ __forceinline__ const double *chl_vals<double>(const SysDesc &d) { return d.chl_v64; }
+template <> __device__ __forceinline__ const float *chl_vals<float>(const SysDesc &d) { return d.chl_v32; }
+template <typename V> __device__ __forceinline__ const V *chu_vals(const SysDesc &d);
+template <> __device__ __forceinline__ const double *chu_vals<double>(const SysDesc &d) { return d.chu_v64; }
+template <> __device__ __forceinline__ const float *chu_vals<float>(const SysDesc &d) { return d.chu_v32; }
+template <typename V> __device__ __forceinline__ const V *udiag(const SysDesc &d);
+template <> __device__ __forceinline__ const double *udiag<double>(const SysDesc &d) { return d.ud64; }
+template <> __device__ __forceinline__ const float *udiag<float>(const SysDesc &d) { return d.ud32; }
 
 template <typename V> struct Sentinel;
 template <> struct Sentinel<double> {
@@ -216,11 +236,12 @@ __device__ __forceinline__ double row_ax64(const SysDesc &d, int i, const double
 }
 
 template <int NV>
-__device__ __forceinline__ void write_partials(double (&v)[NV], double *__restrict__ partials, int tile) {
+__device__ __forceinline__ void write_partials(double (&v)[NV], double *__restrict__ partials, int tile,
+                                               int first = 0) {
     block_sum<NV>(v);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < NV; ++k) partials[(size_t)tile * kMaxPartials + k] = v[k];
+        for (int k = 0; k < NV; ++k) partials[(size_t)tile * kMaxPartials + first + k] = v[k];
     }
 }
 
@@ -342,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
             ((V *)d.p)[i] = ((const V *)d.r)[i];
         } else if (status == S_REFINE) {
             if (st.rollback) d.x[i] = d.xb[i];   // the inner iteration diverged
-            if (PC == KCFD_PC_ILU0) {
+            if (PC == KCFD_PC_ILU0 && d.polled) {
                 ((V *)d.ph)[i] = sentinel_value<V>();
                 ((V *)d.sh)[i] = sentinel_value<V>();
                 ((V *)d.y)[i] = sentinel_value<V>();
@@ -472,7 +493,7 @@ __global__ __launch_bounds__(kBlock) void f_spmv_p(const SysDesc *__restrict__ s
 }
 
 // -------------------------------------------------------------------- K3
-// s = r - alpha v; ||s||.  Block 0 clears the counters of the s solves.
+// s = r - alpha v; ||s||^2 (partial slot 2, read by F4).
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
                                               const SysState *__restrict__ sts,
@@ -495,24 +516,12 @@ __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
             ((V *)d.s)[i] = si;
             v[0] = (double)si * (double)si;
         }
-        write_partials<1>(v, partials, tile);
+        write_partials<1>(v, partials, tile, 2);   // ||s||^2 beside K4's t.s, t.t
     }
 }
 
-__global__ __launch_bounds__(kBlock) void f_s(const SysDesc *__restrict__ sys, SysState *__restrict__ sts,
-                                              const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
-    const BatchParams bp = *bpp;
-    const SysDesc &d = sys[blockIdx.x];
-    SysState &st = sts[blockIdx.x];
-    if (st.status != S_RUN) return;
-    double tot[1];
-    sum_partials<1>(d, partials, tot);
-    if (threadIdx.x != 0) return;
-    if (sqrt(tot[0]) < st.atol_in) st.status = S_HALF;
-}
-
 // -------------------------------------------------------------------- K4
-// t = A M s; t.s, t.t.   S_HALF: x += alpha M p.
+// t = A M s; t.s, t.t.
 template <typename V, int PC>
 __global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ sys,
                                                    const SysState *__restrict__ sts,
@@ -536,19 +545,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ s
                 v[1] = (double)tv * (double)tv;
             }
             write_partials<2>(v, partials, tile);
-        } else if (status == S_HALF) {
-            if (i < d.n) {
-                const V al = (V)st.alpha;
-                const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
-                if (bp.precision == KCFD_FP64)       // scipy's operation, in FP64
-                    ((V *)d.x)[i] = ((V *)d.x)[i] + al * ph;
-                else                                 // MIXED: FP32 direction, FP64 iterate
-                    d.x[i] = d.x[i] + st.alpha * (double)ph;
-                if (PC == KCFD_PC_ILU0) {
-                    ((V *)d.ph)[i] = sentinel_value<V>();
-                    ((V *)d.sh)[i] = sentinel_value<V>();
-                }
-            }
         }
     }
 }
@@ -561,19 +557,14 @@ __global__ __launch_bounds__(kBlock) void f_spmv_s(const SysDesc *__restrict__ s
     SysState &st = sts[blockIdx.x];
     const int status = st.status;
     if (status == S_RUN) {
-        double tot[2];
-        sum_partials<2>(d, partials, tot);
+        double tot[3];   // t.s, t.t (K4), ||s||^2 (K3)
+        sum_partials<3>(d, partials, tot);
         if (threadIdx.x != 0) return;
-        st.omega = tot[0] / tot[1];
-    } else if (status == S_HALF && threadIdx.x == 0) {
-        st.half = 1;
-        st.info = 0;
-        if (bp.precision == KCFD_MIXED) {
-            st.restart = 1;
-            st.status = S_REFINE;
-        } else {
-            st.status = S_DONE;
+        if (sqrt(tot[2]) < st.atol_in) {   // scipy's half-step exit: x += alpha M p (K5)
+            st.status = S_HALF;
+            return;
         }
+        st.omega = tot[0] / tot[1];
     }
 }
 
@@ -591,8 +582,23 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
-        if (st.status != S_RUN) continue;
         const int i = ti.y + threadIdx.x;
+        if (st.status == S_HALF) {                   // half-step exit: x += alpha M p
+            if (i < d.n) {
+                const V al = (V)st.alpha;
+                const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
+                if (bp.precision == KCFD_FP64)       // scipy's operation, in FP64
+                    ((V *)d.x)[i] = ((V *)d.x)[i] + al * ph;
+                else                                 // MIXED: FP32 direction, FP64 iterate
+                    d.x[i] = d.x[i] + st.alpha * (double)ph;
+                if (PC == KCFD_PC_ILU0 && d.polled) {
+                    ((V *)d.ph)[i] = sentinel_value<V>();
+                    ((V *)d.sh)[i] = sentinel_value<V>();
+                }
+            }
+            continue;
+        }
+        if (st.status != S_RUN) continue;
         double v[2] = {0.0, 0.0};
         if (i < d.n) {
             const V al = (V)st.alpha, om = (V)st.omega;
@@ -613,7 +619,7 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
             ((V *)d.r)[i] = ri;
             v[0] = (double)ri * (double)ri;
             v[1] = (double)((const V *)d.rt)[i] * (double)ri;
-            if (PC == KCFD_PC_ILU0) {
+            if (PC == KCFD_PC_ILU0 && d.polled) {   // ready flags of the sync-free solves
                 ((V *)d.ph)[i] = sentinel_value<V>();
                 ((V *)d.sh)[i] = sentinel_value<V>();
             }
@@ -627,6 +633,18 @@ __global__ __launch_bounds__(kBlock) void f_xr(const SysDesc *__restrict__ sys, 
     const BatchParams bp = *bpp;
     const SysDesc &d = sys[blockIdx.x];
     SysState &st = sts[blockIdx.x];
+    if (st.status == S_HALF) {    // K5 applied the half step
+        if (threadIdx.x != 0) return;
+        st.half = 1;
+        st.info = 0;
+        if (bp.precision == KCFD_MIXED) {
+            st.restart = 1;
+            st.status = S_REFINE;
+        } else {
+            st.status = S_DONE;
+        }
+        return;
+    }
     if (st.status != S_RUN) return;
     double tot[2];
     sum_partials<2>(d, partials, tot);
@@ -754,34 +772,38 @@ __device__ __forceinline__ void publish(V *p, V v) {
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// acc - sum_{q0<=q<q1} a_q vec[cols_q], subtracting in q order.  Groups of 8
-// entries: their indices and values, then their (polled) operands are all
+// acc - sum_k a_k vec[col_k] over one row of a chunk (entries at base + 64k,
+// k < w, in CSR order; column -1 marks padding), subtracting in k order.
+// Coalesced: the 64 lanes of a wave read 64 consecutive entries.  Groups of
+// 8 entries: their indices and values, then their (polled) operands are all
 // requested before the first subtraction, so a row costs about one memory
 // round trip per group instead of one per entry.
 template <typename V>
 __device__ __forceinline__ V sub_ready(V acc, const V *__restrict__ a, const int *__restrict__ cols,
-                                       int q0, int q1, const V *vec) {
+                                       int base, int w, const V *vec) {
 #pragma clang fp contract(off)
     using U = typename Sentinel<V>::U;
     constexpr int G = 8;
-    for (int qb = q0; qb < q1; qb += G) {
+    for (int kb = 0; kb < w; kb += G) {
         int c[G];
         V av[G];
         U bits[G];
 #pragma unroll
-        for (int k = 0; k < G; ++k)
-            if (qb + k < q1) {
-                c[k] = cols[qb + k];
-                av[k] = a[qb + k];
+        for (int k = 0; k < G; ++k) {
+            c[k] = -1;
+            if (kb + k < w) {
+                c[k] = cols[base + 64 * (kb + k)];
+                av[k] = a[base + 64 * (kb + k)];
             }
+        }
 #pragma unroll
         for (int k = 0; k < G; ++k)
-            if (qb + k < q1)
+            if (c[k] >= 0)
                 bits[k] = __hip_atomic_load(reinterpret_cast<const U *>(vec + c[k]), __ATOMIC_RELAXED,
                                             __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int k = 0; k < G; ++k)
-            if (qb + k < q1) {
+            if (c[k] >= 0) {
                 while (bits[k] == Sentinel<V>::bits) {
                     __builtin_amdgcn_s_sleep(2);
                     bits[k] = __hip_atomic_load(reinterpret_cast<const U *>(vec + c[k]), __ATOMIC_RELAXED,
@@ -819,20 +841,94 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
         const bool live = want && lane < ch.count;
         if (live) {
             const int i = lvl_rows[ch.sys][ch.start + lane];
-            const V *__restrict__ a = luv<V>(d);
-            const int dp = d.dpos[i];
             const int r = d.perm ? d.perm[i] : i;
+            const int *off = UPPER ? d.chu_off : d.chl_off;
+            const int o0 = off[ch.local];
+            const int w = (off[ch.local + 1] - o0) >> 6;
             if (!UPPER) {
                 const V *z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
                 V *y = (V *)d.y;
-                const V acc = sub_ready<V>(z[r], a, d.ix, d.ip[i], dp, y);
+                const V acc = sub_ready<V>(z[r], chl_vals<V>(d), d.chl_col, o0 + lane, w, y);
                 publish<V>(y + i, acc);
             } else {
                 V *y = (V *)d.y;
-                V *w = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
-                const V acc = sub_ready<V>(y[i], a, d.uix, dp + 1, d.ip[i + 1], w);
-                publish<V>(w + r, acc / a[dp]);
+                V *wv = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
+                const V acc = sub_ready<V>(y[i], chu_vals<V>(d), d.chu_col, o0 + lane, w, wv);
+                publish<V>(wv + r, acc / udiag<V>(d)[i]);
                 y[i] = sentinel_value<V>();
+            }
+        }
+    }
+}
+
+// acc - sum_k a_k vec[col_k] over one chunk row (layout as sub_ready), with
+// plain loads: every operand was written by an earlier launch.
+template <typename V>
+__device__ __forceinline__ V sub_plain(V acc, const V *__restrict__ a, const int *__restrict__ cols,
+                                       int base, int w, const V *__restrict__ vec) {
+#pragma clang fp contract(off)
+    constexpr int G = 8;
+    for (int kb = 0; kb < w; kb += G) {
+        int c[G];
+        V av[G], xv[G];
+#pragma unroll
+        for (int k = 0; k < G; ++k) {
+            c[k] = -1;
+            if (kb + k < w) {
+                c[k] = cols[base + 64 * (kb + k)];
+                av[k] = a[base + 64 * (kb + k)];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (c[k] >= 0) xv[k] = vec[c[k]];
+#pragma unroll
+        for (int k = 0; k < G; ++k)
+            if (c[k] >= 0) acc -= av[k] * xv[k];
+    }
+    return acc;
+}
+
+// ILU(0) apply for a two-colour ordering, P A P^T = [[D0, B], [C, D1]] with
+// colour 0 first.  Colour-0 rows have no L entries, so their y is z; colour-1
+// rows have no U entries.  Hence, exactly as the sequential L then U solves
+// compute it (same operations, same order):
+//   FIRST  (colour-1 rows i): y_i = z_{perm i} - sum_j l_ij z_{perm j};
+//                             w_{perm i} = y_i / u_ii
+//   SECOND (colour-0 rows i): w_{perm i} = (z_{perm i} - sum_j u_ij w_{perm j}) / u_ii
+// Two plain launches per apply: no y vector, no ready flags.  Chunks are runs
+// of 64 rows of one colour; one wave per chunk, dealt round-robin.
+template <typename V, bool SECOND, int WHICH>
+__global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys,
+                                                 const SysState *__restrict__ sts,
+                                                 const Chunk *__restrict__ chunks,
+                                                 const int *__restrict__ counts) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int nchunks = counts[SECOND ? 4 : 3];
+    const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
+    const int wave = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
+    for (int c = wave; c < nchunks; c += nwaves) {
+        const Chunk ch = chunks[c];
+        const SysDesc &d = sys[ch.sys];
+        const int status = sts[ch.sys].status;
+        const bool want = WHICH == 0 ? (status == S_RUN || status == S_START || status == S_RESUME)
+                                     : (status == S_RUN);
+        if (want && lane < ch.count) {
+            const int i = ch.start + lane;       // row of P A P^T
+            const int r = d.perm[i];
+            const V *__restrict__ z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
+            V *__restrict__ w = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
+            if (!SECOND) {
+                const int o0 = d.chl_off[ch.local];
+                const V y = sub_plain<V>(z[r], chl_vals<V>(d), d.chl_ncol, o0 + lane,
+                                         (d.chl_off[ch.local + 1] - o0) >> 6, z);
+                w[r] = y / udiag<V>(d)[i];
+            } else {
+                const int o0 = d.chu_off[ch.local];
+                const V v = sub_plain<V>(z[r], chu_vals<V>(d), d.chu_col, o0 + lane,
+                                         (d.chu_off[ch.local + 1] - o0) >> 6, w);
+                w[r] = v / udiag<V>(d)[i];
             }
         }
     }
@@ -875,6 +971,10 @@ struct kcfd_batch {
     // ILU chunk tables (level-major over systems) for the L and U solves
     Chunk *d_chunks_l = nullptr, *d_chunks_u = nullptr;
     int nchunks_l = 0, nchunks_u = 0, cap_chunks_l = 0, cap_chunks_u = 0, trsv_grid = 0;
+    // two-colour members: colour-1 (a) and colour-0 (b) chunk runs
+    Chunk *d_chunks_a = nullptr, *d_chunks_b = nullptr;
+    int nchunks_a = 0, nchunks_b = 0, cap_chunks_a = 0, cap_chunks_b = 0;
+    bool run_two = false, run_polled = false;   // which ILU kernels the graph launches
     const int **d_lvl_l = nullptr, **d_lvl_u = nullptr;
     BatchParams *d_params = nullptr;   // read by the kernels: the captured graph stays valid
     hipGraphExec_t graph = nullptr;
@@ -902,11 +1002,19 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     mark(4, b->ctx->ev1);
     if (PC == KCFD_PC_ILU0) {
         mark(3, b->ctx->ev0);
-        hipLaunchKernelGGL((k_trsv<V, false, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_l, b->d_counts, b->d_lvl_l);
+        if (b->run_two) {
+            hipLaunchKernelGGL((k_ilu2<V, false, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                               b->d_state, b->d_chunks_a, b->d_counts);
+            hipLaunchKernelGGL((k_ilu2<V, true, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                               b->d_state, b->d_chunks_b, b->d_counts);
+        }
+        if (b->run_polled)
+            hipLaunchKernelGGL((k_trsv<V, false, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                               b->d_state, b->d_chunks_l, b->d_counts, b->d_lvl_l);
         mark(3, b->ctx->ev1);
-        hipLaunchKernelGGL((k_trsv<V, true, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_u, b->d_counts, b->d_lvl_u);
+        if (b->run_polled)
+            hipLaunchKernelGGL((k_trsv<V, true, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                               b->d_state, b->d_chunks_u, b->d_counts, b->d_lvl_u);
     }
     mark(1, b->ctx->ev0);
     hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
@@ -915,12 +1023,19 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     hipLaunchKernelGGL(f_spmv_p, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     hipLaunchKernelGGL((k_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
                        b->d_partials, b->d_counters);
-    hipLaunchKernelGGL(f_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     if (PC == KCFD_PC_ILU0) {
-        hipLaunchKernelGGL((k_trsv<V, false, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_l, b->d_counts, b->d_lvl_l);
-        hipLaunchKernelGGL((k_trsv<V, true, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
-                           b->d_state, b->d_chunks_u, b->d_counts, b->d_lvl_u);
+        if (b->run_two) {
+            hipLaunchKernelGGL((k_ilu2<V, false, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                               b->d_state, b->d_chunks_a, b->d_counts);
+            hipLaunchKernelGGL((k_ilu2<V, true, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                               b->d_state, b->d_chunks_b, b->d_counts);
+        }
+        if (b->run_polled) {
+            hipLaunchKernelGGL((k_trsv<V, false, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                               b->d_state, b->d_chunks_l, b->d_counts, b->d_lvl_l);
+            hipLaunchKernelGGL((k_trsv<V, true, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+                               b->d_state, b->d_chunks_u, b->d_counts, b->d_lvl_u);
+        }
     }
     hipLaunchKernelGGL((k_spmv_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->d_counts, b->d_partials, b->d_params);
@@ -997,6 +1112,18 @@ void fill_desc(SysDesc &d, kcfd_mat *m) {
     d.dinv32 = m->dinv32;
     d.lu64 = m->lu;
     d.lu32 = m->lu32;
+    d.chl_ncol = m->chl_ncol;
+    d.polled = m->pc_kind == KCFD_PC_ILU0 && !m->two_colour;
+    d.chl_off = m->chl_off;
+    d.chl_col = m->chl_col;
+    d.chu_off = m->chu_off;
+    d.chu_col = m->chu_col;
+    d.chl_v64 = m->chl_v64;
+    d.chu_v64 = m->chu_v64;
+    d.ud64 = m->ud64;
+    d.chl_v32 = m->chl_v32;
+    d.chu_v32 = m->chu_v32;
+    d.ud32 = m->ud32;
 }
 
 int check_opts(const kcfd_solve_opts *o) {
@@ -1062,8 +1189,8 @@ int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **o
     KCFD_HIP(hipMemcpyAsync(b->d_state, b->h_state.data(), sizeof(SysState) * slots, hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMalloc(&b->d_counters, sizeof(unsigned int) * 8));
     KCFD_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * 8, s));
-    KCFD_HIP(hipMalloc(&b->d_counts, sizeof(int) * 4));
-    KCFD_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int) * 4, s));
+    KCFD_HIP(hipMalloc(&b->d_counts, sizeof(int) * 8));
+    KCFD_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int) * 8, s));
     KCFD_HIP(hipMalloc(&b->d_params, sizeof(BatchParams)));
     KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
     if (o->precond == KCFD_PC_ILU0) {
@@ -1145,7 +1272,7 @@ int rebuild_tables(kcfd_batch *b) {
     if (b->bp.precond == KCFD_PC_ILU0) {
         int maxlev_l = 0, maxlev_u = 0;
         for (auto *m : b->mats)
-            if (m) {
+            if (m && !m->two_colour) {
                 maxlev_l = std::max(maxlev_l, m->nlev_l);
                 maxlev_u = std::max(maxlev_u, m->nlev_u);
             }
@@ -1154,16 +1281,17 @@ int rebuild_tables(kcfd_batch *b) {
             for (int l = 0; l < L; ++l) {
                 for (int k = 0; k < b->nsys; ++k) {
                     kcfd_mat *m = b->mats[k];
-                    if (!m) continue;
+                    if (!m || m->two_colour) continue;
                     const std::vector<int> &lp = upper ? m->h_lvl_ptr_u : m->h_lvl_ptr_l;
                     const int nl = upper ? m->nlev_u : m->nlev_l;
                     if (l >= nl) continue;
+                    const std::vector<int> &cp = upper ? m->h_chu_ptr : m->h_chl_ptr;
                     for (int s0 = lp[l]; s0 < lp[l + 1]; s0 += 64) {
                         Chunk c;
                         c.sys = k;
                         c.start = s0;
                         c.count = std::min(64, lp[l + 1] - s0);
-                        c.pad = 0;
+                        c.local = cp[l] + (s0 - lp[l]) / 64;
                         ch.push_back(c);
                     }
                 }
@@ -1175,6 +1303,23 @@ int rebuild_tables(kcfd_batch *b) {
         b->nchunks_u = (int)cu.size();
         KCFD_CHECK(ensure_table(&b->d_chunks_l, &b->cap_chunks_l, b->nchunks_l, &grew));
         KCFD_CHECK(ensure_table(&b->d_chunks_u, &b->cap_chunks_u, b->nchunks_u, &grew));
+        // two-colour members: runs of 64 rows of colour 1 (a) and colour 0 (b)
+        std::vector<Chunk> ca, cb;
+        for (int k = 0; k < b->nsys; ++k) {
+            kcfd_mat *m = b->mats[k];
+            if (!m || !m->two_colour) continue;
+            const int n0 = m->ncolour0, n1 = m->n - m->ncolour0;
+            for (int s0 = 0; s0 < n1; s0 += 64) ca.push_back(Chunk{k, n0 + s0, std::min(64, n1 - s0), s0 / 64});
+            for (int s0 = 0; s0 < n0; s0 += 64) cb.push_back(Chunk{k, s0, std::min(64, n0 - s0), s0 / 64});
+        }
+        b->nchunks_a = (int)ca.size();
+        b->nchunks_b = (int)cb.size();
+        KCFD_CHECK(ensure_table(&b->d_chunks_a, &b->cap_chunks_a, b->nchunks_a, &grew));
+        KCFD_CHECK(ensure_table(&b->d_chunks_b, &b->cap_chunks_b, b->nchunks_b, &grew));
+        if (!ca.empty())
+            KCFD_HIP(hipMemcpyAsync(b->d_chunks_a, ca.data(), sizeof(Chunk) * ca.size(), hipMemcpyHostToDevice, s));
+        if (!cb.empty())
+            KCFD_HIP(hipMemcpyAsync(b->d_chunks_b, cb.data(), sizeof(Chunk) * cb.size(), hipMemcpyHostToDevice, s));
         std::vector<const int *> pl(b->nsys, nullptr), pu(b->nsys, nullptr);
         for (int k = 0; k < b->nsys; ++k)
             if (b->mats[k]) {
@@ -1191,7 +1336,7 @@ int rebuild_tables(kcfd_batch *b) {
     if (!tiles.empty())
         KCFD_HIP(hipMemcpyAsync(b->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMemcpyAsync(b->d_desc, b->h_desc.data(), sizeof(SysDesc) * b->nsys, hipMemcpyHostToDevice, s));
-    const int counts[4] = {b->ntiles, b->nchunks_l, b->nchunks_u, 0};
+    const int counts[8] = {b->ntiles, b->nchunks_l, b->nchunks_u, b->nchunks_a, b->nchunks_b, 0, 0, 0};
     KCFD_HIP(hipMemcpyAsync(b->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice, s));
     // the host arrays above are locals: finish the copies before they go
     KCFD_HIP(hipStreamSynchronize(s));
@@ -1227,6 +1372,8 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
     b->slab_bytes = slab;
     for (int k = 0; k < nsys; ++k) place_vectors(b, k, (char *)b->slab + off[k]);
     if ((rc = rebuild_tables(b)) != KCFD_OK) return fail(rc);
+    b->run_two = b->nchunks_a + b->nchunks_b > 0;
+    b->run_polled = b->nchunks_l + b->nchunks_u > 0;
     // persistent grid: at most grid_per_cu workgroups per CU, a multiple of
     // the XCD count
     const int gpc = o->grid_per_cu > 0 ? o->grid_per_cu : 8;
@@ -1387,6 +1534,8 @@ int kcfd_batch_destroy(kcfd_batch *b) {
     for (void *p : b->slot_mem) hipFree(p);
     hipFree(b->d_chunks_l);
     hipFree(b->d_chunks_u);
+    hipFree(b->d_chunks_a);
+    hipFree(b->d_chunks_b);
     hipFree((void *)b->d_lvl_l);
     hipFree((void *)b->d_lvl_u);
     delete b;
@@ -1408,6 +1557,7 @@ int kcfd_batch_create_dynamic(kcfd_ctx *ctx, int capacity, const kcfd_solve_opts
         return rc;
     }
     b->dynamic = true;
+    b->run_two = b->run_polled = opts->precond == KCFD_PC_ILU0;   // any membership
     // the graph is captured once for any membership: full-size launches
     const int gpc = opts->grid_per_cu > 0 ? opts->grid_per_cu : 8;
     b->grid = std::max(kXcds, (ctx->cus * gpc / kXcds) * kXcds);
@@ -1584,15 +1734,25 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     std::vector<SysState> saved(b->nsys), run(b->nsys);
     KCFD_HIP(hipMemcpyAsync(saved.data(), b->d_state, sizeof(SysState) * b->nsys, hipMemcpyDeviceToHost, s));
     KCFD_HIP(hipStreamSynchronize(s));
+    // No test may end the iteration early: convergence, half-step and
+    // breakdown thresholds are disabled, so every kernel does its full work.
     for (int k = 0; k < b->nsys; ++k) {
         run[k] = saved[k];
         if (!b->mats[k]) continue;
         run[k].status = S_RUN;
         if (run[k].omega == 0.0) run[k].omega = 1.0;
         if (run[k].rho == 0.0) run[k].rho = 1.0;
+        run[k].atol_in = run[k].atol_out = -1.0;
+        run[k].maxiter = 1 << 30;
+        run[k].iters = run[k].total_iters = 0;
+        run[k].rmax = INFINITY;   // MIXED: no divergence restart
     }
     const LaunchFn it = pick_iter(b->bp.precision, b->bp.precond);
-    KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
+    BatchParams pp = b->bp;
+    pp.rhotol = -1.0;     // no breakdown exit
+    pp.delta = 0.0;       // MIXED: no residual replacement
+    pp.inner_maxiter = 1 << 30;
+    KCFD_HIP(hipMemcpyAsync(b->d_params, &pp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
     double tot = 0.0;
     int rc = KCFD_OK;
     b->timed_phase = phase;
@@ -1618,6 +1778,7 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     b->timed_phase = 0;
     KCFD_CHECK(rc);
     *avg_ms = tot / reps;
+    KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMemcpyAsync(b->d_state, saved.data(), sizeof(SysState) * b->nsys, hipMemcpyHostToDevice, s));
     KCFD_HIP(hipStreamSynchronize(s));
     return KCFD_OK;
@@ -1634,7 +1795,12 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
         const double MFh = 0.5 * nnz * (sv + 4) + (n + 1) * 4;  // one triangle + its level list
         if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
         else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
-        else if (phase == 3) tot += MFh + 2 * n * sv + (m->perm ? 4 * n : 0);  // + perm gather
+        else if (phase == 3) {
+            if (m->two_colour)   // both kernels of the two-colour apply: L and U entries, z, w, perm, u_ii
+                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + n * (4 + sv) + 3 * n * sv;
+            else                 // the sync-free L solve (one triangle + its level list)
+                tot += MFh + 2 * n * sv + (m->perm ? 4 * n : 0);
+        }
         else if (phase == 4) tot += 4 * n * sv;
     }
     *bytes = tot;

@@ -129,6 +129,7 @@ struct kcfd_mat {
     // f_vmap[q] is the slot of A feeding factor slot q and f_uix[q] the
     // column as a row of A (what the U solve, writing in A's numbering, polls).
     int ilu_order = KCFD_ORDER_NATURAL, ilu_built_order = -1, ncolors = 0;
+    int ncolour0 = 0;   // rows of colour 0 (they come first in P A P^T)
     int *perm = nullptr, *f_vmap = nullptr;
     int *f_indptr = nullptr, *f_indices = nullptr, *f_diagpos = nullptr, *f_uix = nullptr;
     std::vector<int> h_perm, hf_indptr, hf_indices, hf_diagpos;
@@ -139,6 +140,25 @@ struct kcfd_mat {
     int *lvl_rows_l = nullptr, *lvl_rows_u = nullptr;   // rows ordered by level
     std::vector<int> h_lvl_ptr_l, h_lvl_ptr_u;           // level offsets
     int max_lvl_rows_l = 0, max_lvl_rows_u = 0;
+    // ILU(0) solve layout.  The level lists are cut into chunks of up to 64
+    // rows (never across a level); chunk c keeps the L (resp. U) entries of
+    // its rows column-major: entry k of the row in lane l at off[c] + 64k + l,
+    // in CSR order, padded with column -1 / value 0.  L columns are rows of
+    // P A P^T, U columns rows of A.  h_ch*_ptr[l]: first chunk of level l;
+    // ch*_map: factor slot of each entry (-1: padding); ud: u_ii by row of
+    // P A P^T.
+    int nch_l = 0, nch_u = 0, nent_l = 0, nent_u = 0;
+    // Two colours: the L layout holds the colour-1 rows (positions
+    // ncolour0..n-1) and the U layout the colour-0 rows, as single runs of
+    // 64-row chunks; a colour-0 row's y is z itself, so the L rows read z
+    // through chl_ncol (columns as rows of A).
+    int *chl_ncol = nullptr;
+    bool two_colour = false;
+    std::vector<int> h_chl_ptr, h_chu_ptr;
+    int *chl_off = nullptr, *chu_off = nullptr, *chl_col = nullptr, *chu_col = nullptr;
+    int *chl_map = nullptr, *chu_map = nullptr;
+    double *chl_v64 = nullptr, *chu_v64 = nullptr, *ud64 = nullptr;
+    float *chl_v32 = nullptr, *chu_v32 = nullptr, *ud32 = nullptr;
     // cached single-system solver workspace
     kcfd_batch *solver = nullptr;
     int solver_key[4] = {-1, -1, -1, -1};

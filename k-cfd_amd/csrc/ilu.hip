@@ -175,6 +175,7 @@ static int build_order(kcfd_mat *m) {
     std::vector<int> cp(m->ncolors + 1, 0);
     for (int c : col) cp[c + 1]++;
     for (int c = 0; c < m->ncolors; ++c) cp[c + 1] += cp[c];
+    m->ncolour0 = cp[1];
     std::vector<int> &perm = m->h_perm, iperm(n);
     perm.assign(n, 0);
     for (int i = 0; i < n; ++i) perm[cp[col[i]]++] = i;
@@ -223,6 +224,66 @@ static int build_order(kcfd_mat *m) {
     return KCFD_OK;
 }
 
+// Chunked column-major layout of the L and U parts of the factor rows (see
+// kcfd_mat::chl_off).  Chunks follow the level lists: per level, runs of up
+// to 64 consecutive list entries.
+static void chunk_layout(const kcfd_mat *m, const std::vector<int> &rows, const std::vector<int> &lvl,
+                         bool upper, std::vector<int> &ch_ptr, std::vector<int> &off,
+                         std::vector<int> &col, std::vector<int> &map, std::vector<int> &ncol) {
+    const std::vector<int> &ip = m->hf_indptr, &ix = m->hf_indices, &dp = m->hf_diagpos;
+    const int L = (int)lvl.size() - 1;
+    ch_ptr.assign(1, 0);
+    off.assign(1, 0);
+    col.clear();
+    map.clear();
+    ncol.clear();
+    for (int l = 0; l < L; ++l) {
+        for (int s0 = lvl[l]; s0 < lvl[l + 1]; s0 += 64) {
+            const int cnt = std::min(64, lvl[l + 1] - s0);
+            int w = 0;
+            for (int t = 0; t < cnt; ++t) {
+                const int i = rows[s0 + t];
+                w = std::max(w, upper ? ip[i + 1] - dp[i] - 1 : dp[i] - ip[i]);
+            }
+            const size_t base = col.size();
+            col.resize(base + (size_t)64 * w, -1);
+            map.resize(base + (size_t)64 * w, -1);
+            ncol.resize(base + (size_t)64 * w, -1);
+            for (int t = 0; t < cnt; ++t) {
+                const int i = rows[s0 + t];
+                const int q0 = upper ? dp[i] + 1 : ip[i], q1 = upper ? ip[i + 1] : dp[i];
+                for (int q = q0; q < q1; ++q) {
+                    const size_t e = base + (size_t)64 * (q - q0) + t;
+                    // U columns as rows of A: what the U solve (writing in A's
+                    // numbering) polls
+                    const int nat = m->h_perm.empty() ? ix[q] : m->h_perm[ix[q]];
+                    col[e] = upper ? nat : ix[q];
+                    ncol[e] = nat;
+                    map[e] = q;
+                }
+            }
+            off.push_back((int)col.size());
+        }
+        ch_ptr.push_back((int)off.size() - 1);
+    }
+}
+
+static int upload_ints(const std::vector<int> &h, int **d, hipStream_t s) {
+    KCFD_HIP(hipMalloc(d, sizeof(int) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) KCFD_HIP(hipMemcpyAsync(*d, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice, s));
+    return KCFD_OK;
+}
+
+__global__ void k_chunk_vals(size_t ne, const int *__restrict__ map, const double *__restrict__ lu,
+                             double *__restrict__ v64, float *__restrict__ v32) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const int q = map[e];
+    const double v = q >= 0 ? lu[q] : 0.0;
+    v64[e] = v;
+    if (v32) v32[e] = (float)v;
+}
+
 int ilu0_setup(kcfd_mat *m, bool want32) {
     const int n = m->n;
     hipStream_t s = m->ctx->stream;
@@ -235,9 +296,7 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
     // ordering and level schedule depend on the structure only: built once
     // per ordering
     if (!m->lvl_rows_l || m->ilu_built_order != m->ilu_order) {
-        hipFree(m->lvl_rows_l);
-        hipFree(m->lvl_rows_u);
-        m->lvl_rows_l = m->lvl_rows_u = nullptr;
+        mat_free_ilu_structure(m);
         KCFD_CHECK(build_order(m));
         std::vector<int> rl, ru;
         level_schedule(m->hf_indptr, m->hf_indices, m->hf_diagpos, false, rl, m->h_lvl_ptr_l);
@@ -249,7 +308,37 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
         if (n) {
             KCFD_HIP(hipMemcpyAsync(m->lvl_rows_l, rl.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
             KCFD_HIP(hipMemcpyAsync(m->lvl_rows_u, ru.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-            KCFD_HIP(hipStreamSynchronize(s));   // rl, ru are locals
+            KCFD_HIP(hipStreamSynchronize(s));   // rl, ru are rewritten below
+        }
+        // chunked L / U layout of the solves: per level of the level lists,
+        // or, with two colours, the colour-1 rows (L) and colour-0 rows (U)
+        m->two_colour = m->perm && m->ncolors == 2;
+        if (m->two_colour) {
+            const int n0 = m->ncolour0;
+            rl.resize(n - n0);
+            for (int k = n0; k < n; ++k) rl[k - n0] = k;
+            ru.resize(n0);
+            for (int k = 0; k < n0; ++k) ru[k] = k;
+        }
+        const std::vector<int> lvl_l = m->two_colour ? std::vector<int>{0, n - m->ncolour0} : m->h_lvl_ptr_l;
+        const std::vector<int> lvl_u = m->two_colour ? std::vector<int>{0, m->ncolour0} : m->h_lvl_ptr_u;
+        {
+            std::vector<int> off, col, map, ncol;
+            chunk_layout(m, rl, lvl_l, false, m->h_chl_ptr, off, col, map, ncol);
+            m->nch_l = (int)off.size() - 1;
+            m->nent_l = (int)col.size();
+            KCFD_CHECK(upload_ints(off, &m->chl_off, s));
+            KCFD_CHECK(upload_ints(col, &m->chl_col, s));
+            KCFD_CHECK(upload_ints(map, &m->chl_map, s));
+            KCFD_CHECK(upload_ints(ncol, &m->chl_ncol, s));
+            KCFD_HIP(hipStreamSynchronize(s));
+            chunk_layout(m, ru, lvl_u, true, m->h_chu_ptr, off, col, map, ncol);
+            m->nch_u = (int)off.size() - 1;
+            m->nent_u = (int)col.size();
+            KCFD_CHECK(upload_ints(off, &m->chu_off, s));
+            KCFD_CHECK(upload_ints(col, &m->chu_col, s));
+            KCFD_CHECK(upload_ints(map, &m->chu_map, s));
+            KCFD_HIP(hipStreamSynchronize(s));   // the host vectors are locals
         }
     }
     const std::vector<int> &pl = m->h_lvl_ptr_l;
@@ -271,6 +360,31 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
         if (nb > 65536) nb = 65536;
         hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)nb), dim3(kBlock), 0, s, (size_t)m->nnz,
                            m->lu, m->lu32);
+        KCFD_HIP(hipGetLastError());
+    }
+    // factor values in the chunked solve layout, u_ii by row
+    {
+        const size_t nl = (size_t)m->nent_l, nu = (size_t)m->nent_u;
+        if (!m->chl_v64) {
+            KCFD_HIP(hipMalloc(&m->chl_v64, sizeof(double) * std::max<size_t>(nl, 1)));
+            KCFD_HIP(hipMalloc(&m->chu_v64, sizeof(double) * std::max<size_t>(nu, 1)));
+            KCFD_HIP(hipMalloc(&m->ud64, sizeof(double) * std::max(n, 1)));
+        }
+        if (want32 && !m->chl_v32) {
+            KCFD_HIP(hipMalloc(&m->chl_v32, sizeof(float) * std::max<size_t>(nl, 1)));
+            KCFD_HIP(hipMalloc(&m->chu_v32, sizeof(float) * std::max<size_t>(nu, 1)));
+            KCFD_HIP(hipMalloc(&m->ud32, sizeof(float) * std::max(n, 1)));
+        }
+        float *l32 = want32 ? m->chl_v32 : nullptr, *u32 = want32 ? m->chu_v32 : nullptr;
+        if (nl)
+            hipLaunchKernelGGL(k_chunk_vals, dim3((unsigned)((nl + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                               nl, m->chl_map, m->lu, m->chl_v64, l32);
+        if (nu)
+            hipLaunchKernelGGL(k_chunk_vals, dim3((unsigned)((nu + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                               nu, m->chu_map, m->lu, m->chu_v64, u32);
+        if (n)
+            hipLaunchKernelGGL(k_chunk_vals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (size_t)n,
+                               m->f_diagpos, m->lu, m->ud64, want32 ? m->ud32 : nullptr);
         KCFD_HIP(hipGetLastError());
     }
     // pivot check: first row with a zero or NaN u_ii (or n if none)
@@ -438,6 +552,21 @@ extern "C" int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src) {
     }
     KCFD_CHECK(dup_ints(src->lvl_rows_l, n, &dst->lvl_rows_l, dst->ctx->stream));
     KCFD_CHECK(dup_ints(src->lvl_rows_u, n, &dst->lvl_rows_u, dst->ctx->stream));
+    KCFD_CHECK(dup_ints(src->chl_off, (size_t)src->nch_l + 1, &dst->chl_off, dst->ctx->stream));
+    KCFD_CHECK(dup_ints(src->chu_off, (size_t)src->nch_u + 1, &dst->chu_off, dst->ctx->stream));
+    KCFD_CHECK(dup_ints(src->chl_col, (size_t)src->nent_l, &dst->chl_col, dst->ctx->stream));
+    KCFD_CHECK(dup_ints(src->chu_col, (size_t)src->nent_u, &dst->chu_col, dst->ctx->stream));
+    KCFD_CHECK(dup_ints(src->chl_map, (size_t)src->nent_l, &dst->chl_map, dst->ctx->stream));
+    KCFD_CHECK(dup_ints(src->chl_ncol, (size_t)src->nent_l, &dst->chl_ncol, dst->ctx->stream));
+    dst->two_colour = src->two_colour;
+    dst->ncolour0 = src->ncolour0;
+    KCFD_CHECK(dup_ints(src->chu_map, (size_t)src->nent_u, &dst->chu_map, dst->ctx->stream));
+    dst->h_chl_ptr = src->h_chl_ptr;
+    dst->h_chu_ptr = src->h_chu_ptr;
+    dst->nch_l = src->nch_l;
+    dst->nch_u = src->nch_u;
+    dst->nent_l = src->nent_l;
+    dst->nent_u = src->nent_u;
     dst->h_lvl_ptr_l = src->h_lvl_ptr_l;
     dst->h_lvl_ptr_u = src->h_lvl_ptr_u;
     dst->nlev_l = src->nlev_l;

@@ -126,6 +126,12 @@ int mat_free_precond(kcfd_mat *m) {
     hipFree(m->dinv32); m->dinv32 = nullptr;
     hipFree(m->lu); m->lu = nullptr;
     hipFree(m->lu32); m->lu32 = nullptr;
+    hipFree(m->chl_v64); m->chl_v64 = nullptr;
+    hipFree(m->chu_v64); m->chu_v64 = nullptr;
+    hipFree(m->ud64); m->ud64 = nullptr;
+    hipFree(m->chl_v32); m->chl_v32 = nullptr;
+    hipFree(m->chu_v32); m->chu_v32 = nullptr;
+    hipFree(m->ud32); m->ud32 = nullptr;
     m->have32 = false;
     m->pc_kind = KCFD_PC_NONE;
     return KCFD_OK;
@@ -134,6 +140,26 @@ int mat_free_precond(kcfd_mat *m) {
 int mat_free_ilu_structure(kcfd_mat *m) {
     hipFree(m->lvl_rows_l); m->lvl_rows_l = nullptr;
     hipFree(m->lvl_rows_u); m->lvl_rows_u = nullptr;
+    int **arrs[] = {&m->chl_off, &m->chu_off, &m->chl_col, &m->chu_col, &m->chl_map, &m->chu_map,
+                    &m->chl_ncol};
+    m->two_colour = false;
+    for (int **a : arrs) {
+        hipFree(*a);
+        *a = nullptr;
+    }
+    m->h_chl_ptr.clear();
+    m->h_chu_ptr.clear();
+    m->nch_l = m->nch_u = m->nent_l = m->nent_u = 0;
+    double **v64[] = {&m->chl_v64, &m->chu_v64, &m->ud64};
+    for (double **a : v64) {
+        hipFree(*a);
+        *a = nullptr;
+    }
+    float **v32[] = {&m->chl_v32, &m->chu_v32, &m->ud32};
+    for (float **a : v32) {
+        hipFree(*a);
+        *a = nullptr;
+    }
     ilu_free_order(m);
     m->h_lvl_ptr_l.clear();
     m->h_lvl_ptr_u.clear();

@@ -308,8 +308,14 @@ class Server:
         nfin = L.ctypes.c_int()
         res = L.SolveResult()
         t0 = time.perf_counter()
+        t_log = 10.0
         while pending or queue or active:
             now = time.perf_counter() - t0
+            if now >= t_log:       # progress, every 10 s of serving
+                self.log(f"serve t={now:.0f}s: {len(done)} done, {active} on the device "
+                         f"({len(owner)} + {len(owner64)} systems), {len(queue)} queued, "
+                         f"{len(pending)} not yet arrived")
+                t_log = now + 10.0
             while pending and pending[0].t_arrival <= now:
                 queue.append(pending.popleft())
             if not self.batched:
