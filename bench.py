@@ -94,6 +94,8 @@ def main():
                     help="ILU(0) row ordering (colour: first-fit colouring, 2 levels per solve)")
     ap.add_argument("--grid-per-cu", type=int, default=0, help="launch shape (0: library default)")
     ap.add_argument("--trsv-per-cu", type=int, default=0, help="launch shape (0: library default)")
+    ap.add_argument("--tile-walk", type=int, default=0,
+                    help="launch shape: 1 interleaved, 2 contiguous tiles per workgroup (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--check-every", type=int, default=8,
                     help="iterations per captured graph between host status checks")
@@ -145,7 +147,8 @@ def bench_pressure(args):
     for prec in precs:
         opts = device.default_opts(rtol=1e-8, precond=args.precond, precision=prec,
                                    inner_rtol=args.delta, check_every=args.check_every,
-                                   grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu)
+                                   grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu,
+                                   tile_walk=args.tile_walk)
         lat, rs = [], []
 
         def step():
@@ -286,7 +289,7 @@ def bench_batched(args):
         log(f"{prec}: preconditioners set up in {time.perf_counter() - t:.1f}s")
         if args.no_tune:
             choice = dict(grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu,
-                          check_every=args.check_every)
+                          tile_walk=args.tile_walk, check_every=args.check_every)
         else:
             choice = tune.autotune(sb, prec, args.precond, order or "natural", delta=args.delta, log=log)
         log(f"{prec}: launch shape {tune.describe(choice)}")
@@ -295,7 +298,7 @@ def bench_batched(args):
         bobj = sb.make_batch(device.default_opts(
             rtol=1e-8, precond=args.precond, precision=prec, inner_rtol=args.delta,
             check_every=choice["check_every"], grid_per_cu=choice["grid_per_cu"],
-            trsv_per_cu=choice["trsv_per_cu"]))
+            trsv_per_cu=choice["trsv_per_cu"], tile_walk=choice.get("tile_walk", 0)))
         for w in range(args.warmup):
             r = sb.solve(batch=bobj, **opts)
             log(f"{prec} warmup {w}: {r.summary()}")

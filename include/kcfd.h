@@ -160,6 +160,10 @@ typedef struct kcfd_solve_opts {
                            the SpMV / vector kernels, 1..8 (0: 8) */
     int trsv_per_cu;    /* launch shape: workgroups per CU of the ILU(0) solves,
                            1..4 (0: 4; the solve needs its grid resident) */
+    int tile_walk;      /* launch shape: how a persistent workgroup walks the row
+                           tiles of its XCD's range -- 1 interleaved (tiles j,
+                           j+J, ...), 2 one contiguous run per workgroup
+                           (0: library default) */
 } kcfd_solve_opts;
 
 typedef struct kcfd_solve_result {

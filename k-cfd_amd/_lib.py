@@ -41,7 +41,7 @@ class SolveOpts(ctypes.Structure):
                 ("precond", ctypes.c_int), ("precision", ctypes.c_int),
                 ("inner_rtol", ctypes.c_double), ("max_refine", ctypes.c_int),
                 ("check_every", ctypes.c_int), ("grid_per_cu", ctypes.c_int),
-                ("trsv_per_cu", ctypes.c_int)]
+                ("trsv_per_cu", ctypes.c_int), ("tile_walk", ctypes.c_int)]
 
 
 class SolveResult(ctypes.Structure):

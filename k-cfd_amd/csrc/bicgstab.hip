@@ -65,23 +65,29 @@ struct SysDesc {
     int n, tile0, ntiles;
     int polled;   // ILU(0) solved by the sync-free kernels (sentinel-flagged buffers)
     // ILU(0) factor structure: CSR of P A P^T (A's own CSR in the natural
-    // ordering); uix = columns as rows of A; perm = row of A at each
-    // position (null: identity)
+    // ordering); uix = columns as rows of A.  perm: the solver's row order,
+    // row of A at each position (null: identity).  With a permuted ILU(0)
+    // every vector and the SELL copy are in P-order; b and x are gathered
+    // from the caller's buffers at the start and x is scattered back at the
+    // end.
     const int *ip, *ix, *dpos, *uix, *perm;
     // chunked L / U solve layout (kcfd_mat::chl_off): offsets per chunk,
-    // columns, values, and u_ii by row
-    const int *chl_off, *chl_col, *chu_off, *chu_col, *chl_ncol;
+    // columns (rows of P A P^T), values, and u_ii by row
+    const int *chl_off, *chl_col, *chu_off, *chu_col;
     const double *chl_v64, *chu_v64, *ud64;
-    const float *chl_v32, *chu_v32, *ud32;
+    const int2 *chl_pk32, *chu_pk32;   // MIXED: packed {column, FP32 value} entries
+    const float *ud32;
     const int *sptr, *six;          // SELL-64 structure
     const double *sa64;
-    const float *sa32;
+    const int2 *spk32;              // MIXED: packed {column, FP32 value} SELL entries
     const double *dinv64;
     const float *dinv32;
     const double *lu64;
     const float *lu32;
-    const double *b;
-    double *x;        // the iterate, FP64 in both modes
+    const double *b;  // in the solver's row order
+    double *x;        // the iterate, FP64 in both modes, in the solver's row order
+    const double *b_user;   // the caller's b and x (natural order)
+    double *x_user;
     double *xb;       // MIXED: the iterate with the lowest true residual so far
     // inner vectors (double in FP64 mode, float in MIXED mode)
     void *r, *rt, *p, *v, *s, *t, *ph, *sh, *y;
@@ -95,8 +101,9 @@ struct SysState {
     int status, iters, info, half;
     int refinements, total_iters, maxiter, restart;
     int final_, stagnant, true_done, save;   // save: copy x to xb in the next K1
-    int restore, best_iter, rollback, pad;   // restore: x = xb before the true residual;
-                                             // rollback: x = xb before a residual replacement
+    int restore, best_iter, rollback, scatter;   // restore: x = xb before the true residual;
+                                                 // rollback: x = xb before a residual replacement;
+                                                 // scatter: x to the caller's order pending
 };
 
 struct BatchParams {
@@ -112,18 +119,21 @@ struct Chunk {
 
 // ----------------------------------------------------------- tile walk
 // Blocks with blockIdx % 8 == x (the XCD they are dispatched to) cover the
-// contiguous tile range [x*per, (x+1)*per).  gridDim.x is a multiple of 8.
-#define KCFD_TILE_LOOP(tile, ntiles)                                                      \
-    for (int per_ = ((ntiles) + kXcds - 1) / kXcds, k_ = blockIdx.x / kXcds,               \
-             tile = (int)(blockIdx.x % kXcds) * per_ + k_;                                 \
-         k_ < per_ && tile < (ntiles);                                                      \
-         k_ += (int)gridDim.x / kXcds, tile = (int)(blockIdx.x % kXcds) * per_ + k_)
+// contiguous tile range [x*per, (x+1)*per) of that XCD, so that neighbouring
+// rows share its L2.  walk 0: block j of the XCD (J blocks per XCD) takes
+// tiles j, j + J, j + 2J, ...; walk 1: block j takes the contiguous run
+// [j*c, (j+1)*c), c = ceil(per / J).  The walk mode is counts[5].
+// gridDim.x is a multiple of 8; the trip count is uniform over a block.
+#define KCFD_TILE_LOOP(tile, ntiles)                                                            \
+    for (int per_ = ((ntiles) + kXcds - 1) / kXcds, J_ = (int)gridDim.x / kXcds,                  \
+             j_ = (int)blockIdx.x / kXcds, x0_ = (int)(blockIdx.x % kXcds) * per_,                \
+             c_ = counts[5] ? (per_ + J_ - 1) / J_ : 1, st_ = counts[5] ? 1 : J_,                 \
+             tile = x0_ + j_ * c_,                                                                 \
+             e_ = min(min(x0_ + (counts[5] ? (j_ + 1) * c_ : per_), x0_ + per_), (ntiles));        \
+         tile < e_; tile += st_)
 
 // ----------------------------------------------------- element accessors
 
-template <typename V> __device__ __forceinline__ const V *sell_vals(const SysDesc &d);
-template <> __device__ __forceinline__ const double *sell_vals<double>(const SysDesc &d) { return d.sa64; }
-template <> __device__ __forceinline__ const float *sell_vals<float>(const SysDesc &d) { return d.sa32; }
 template <typename V> __device__ __forceinline__ const V *jac(const SysDesc &d);
 template <> __device__ __forceinline__ const double *jac<double>(const SysDesc &d) { return d.dinv64; }
 template <> __device__ __forceinline__ const float *jac<float>(const SysDesc &d) { return d.dinv32; }
@@ -131,12 +141,32 @@ template <typename V> __device__ __forceinline__ const V *luv(const SysDesc &d);
 template <> __device__ __forceinline__ const double *luv<double>(const SysDesc &d) { return d.lu64; }
 template <> __device__ __forceinline__ const float *luv<float>(const SysDesc &d) { return d.lu32; }
 
-template <typename V> __device__ __forceinline__ const V *chl_vals(const SysDesc &d);
-template <> __device__ __forceinline__ const double *chl_vals<double>(const SysDesc &d) { return d.chl_v64; }
-template <> __device__ __forceinline__ const float *chl_vals<float>(const SysDesc &d) { return d.chl_v32; }
-template <typename V> __device__ __forceinline__ const V *chu_vals(const SysDesc &d);
-template <> __device__ __forceinline__ const double *chu_vals<double>(const SysDesc &d) { return d.chu_v64; }
-template <> __device__ __forceinline__ const float *chu_vals<float>(const SysDesc &d) { return d.chu_v32; }
+// Factor entries of the chunked L / U layouts: FP64 keeps columns and values
+// in two arrays; MIXED reads one packed {column, FP32 value} pair per entry.
+struct FacRef {
+    const int *col;
+    const double *v64;
+    const int2 *pk32;
+};
+__device__ __forceinline__ FacRef fac_l(const SysDesc &d) { return FacRef{d.chl_col, d.chl_v64, d.chl_pk32}; }
+__device__ __forceinline__ FacRef fac_u(const SysDesc &d) { return FacRef{d.chu_col, d.chu_v64, d.chu_pk32}; }
+template <typename V>
+__device__ __forceinline__ void fac_entry(const FacRef &f, int e, int &c, V &v) {
+    if constexpr (sizeof(V) == sizeof(float)) {
+        const kint2 q = gl(reinterpret_cast<const kint2 *>(f.pk32))[e];
+        c = q.x;
+        v = __int_as_float(q.y);
+    } else {
+        c = gl(f.col)[e];
+        v = gl(f.v64)[e];
+    }
+}
+// typed global view of a workspace vector (stored as void *)
+template <typename V>
+__device__ __forceinline__ KCFD_GLOBAL V *vec(void *p) {
+    return (KCFD_GLOBAL V *)p;
+}
+
 template <typename V> __device__ __forceinline__ const V *udiag(const SysDesc &d);
 template <> __device__ __forceinline__ const double *udiag<double>(const SysDesc &d) { return d.ud64; }
 template <> __device__ __forceinline__ const float *udiag<float>(const SysDesc &d) { return d.ud32; }
@@ -160,77 +190,106 @@ template <typename V, int PC>
 __device__ __forceinline__ double precond_at(const SysDesc &d, const V *__restrict__ z,
                                              const V *__restrict__ zh, int j) {
 #pragma clang fp contract(off)
-    if (PC == KCFD_PC_JACOBI) return (double)(jac<V>(d)[j] * z[j]);
-    if (PC == KCFD_PC_ILU0) return (double)zh[j];
-    return (double)z[j];
+    if (PC == KCFD_PC_JACOBI) return (double)(gl(jac<V>(d))[j] * gl(z)[j]);
+    if (PC == KCFD_PC_ILU0) return (double)gl(zh)[j];
+    return (double)gl(z)[j];
 }
 
 // (A M z)_i from the SELL-64 copy, entries in CSR order.  A wave's 64 rows
-// are one slice, so the width w is wave-uniform; entries are taken in groups
-// of 8 whose index and value loads, then gathers, are all issued before the
-// (in-order) accumulation, keeping 8 loads per lane in flight.
+// are one slice, so the width w is wave-uniform.  The index and value loads
+// of a group of entries, then its gathers, are all issued before the
+// (in-order) accumulation.  The common widths (5- and 7-point stencils, tet
+// rows) run as fixed-size groups without any guard; any other width goes in
+// groups of 8 whose entries past the width re-load the last entry (the same
+// cache line) and are not accumulated, so no load sits behind a branch.
 constexpr int kRowUnroll = 8;
 
-//
-// MIXED (V = float): the FP32 values are in difference form (row sum on the
-// diagonal), so the row is sum_{j!=i} a_ij (z_j - z_i) + rowsum_i z_i with
-// the differences taken in FP64.
+// entry e of the SELL copy: column and value (MIXED: one 8-byte load)
+template <typename V>
+__device__ __forceinline__ void sell_entry(const SysDesc &d, int e, int &c, V &v) {
+    if constexpr (sizeof(V) == sizeof(float)) {
+        const kint2 q = gl(reinterpret_cast<const kint2 *>(d.spk32))[e];
+        c = q.x;
+        v = __int_as_float(q.y);
+    } else {
+        c = gl(d.six)[e];
+        v = gl(d.sa64)[e];
+    }
+}
+
+// G entries starting at entry k0 of the row at `base`, the first `valid`
+// of which are accumulated into acc.  MIXED (V = float): the FP32 values are
+// in difference form (row sum on the diagonal), so the row is
+// sum_{j!=i} a_ij (z_j - z_i) + rowsum_i z_i with the differences in FP64.
+template <typename V, int PC, int G>
+__device__ __forceinline__ void amz_group(const SysDesc &d, int i, const V *__restrict__ z,
+                                          const V *__restrict__ zh, int base, int k0, int last,
+                                          int valid, double zi, double &acc) {
+#pragma clang fp contract(off)
+    constexpr bool kDiff = sizeof(V) == sizeof(float);
+    int col[G];
+    V av[G];
+    double zv[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) sell_entry<V>(d, base + 64 * min(k0 + k, last), col[k], av[k]);
+#pragma unroll
+    for (int k = 0; k < G; ++k) zv[k] = precond_at<V, PC>(d, z, zh, col[k]);
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+        if (k < valid) {
+            if (kDiff) acc += (double)av[k] * (col[k] == i ? zi : zv[k] - zi);
+            else acc += (double)av[k] * zv[k];
+        }
+}
+
 template <typename V, int PC>
 __device__ __forceinline__ double row_amz(const SysDesc &d, int i, const V *__restrict__ z,
                                           const V *__restrict__ zh) {
 #pragma clang fp contract(off)
     constexpr bool kDiff = sizeof(V) == sizeof(float);
-    const V *__restrict__ a = sell_vals<V>(d);
-    const int s = i >> 6;
-    const int s0 = d.sptr[s], w = (d.sptr[s + 1] - s0) >> 6;
+    const int s = uniform(i >> 6);
+    const int s0 = uniform(gl(d.sptr)[s]), w = uniform((gl(d.sptr)[s + 1] - s0) >> 6);
     const int base = s0 + (i & 63);
     const double zi = kDiff ? precond_at<V, PC>(d, z, zh, i) : 0.0;
     double acc = 0.0;
-    for (int k0 = 0; k0 < w; k0 += kRowUnroll) {
-        int col[kRowUnroll];
-        V av[kRowUnroll];
-        double zv[kRowUnroll];
-#pragma unroll
-        for (int k = 0; k < kRowUnroll; ++k)
-            if (k0 + k < w) {
-                col[k] = d.six[base + 64 * (k0 + k)];
-                av[k] = a[base + 64 * (k0 + k)];
-            }
-#pragma unroll
-        for (int k = 0; k < kRowUnroll; ++k)
-            if (k0 + k < w) zv[k] = precond_at<V, PC>(d, z, zh, col[k]);
-#pragma unroll
-        for (int k = 0; k < kRowUnroll; ++k)
-            if (k0 + k < w) {
-                if (kDiff) acc += (double)av[k] * (col[k] == i ? zi : zv[k] - zi);
-                else acc += (double)av[k] * zv[k];
-            }
+    if (w == 7) {
+        amz_group<V, PC, 7>(d, i, z, zh, base, 0, 6, 7, zi, acc);
+    } else if (w == 5) {
+        amz_group<V, PC, 5>(d, i, z, zh, base, 0, 4, 5, zi, acc);
+    } else {
+        for (int k0 = 0; k0 < w; k0 += kRowUnroll)
+            amz_group<V, PC, kRowUnroll>(d, i, z, zh, base, k0, w - 1, w - k0, zi, acc);
     }
     return acc;
 }
 
 // (A x)_i in FP64 on the FP64 values
-__device__ __forceinline__ double row_ax64(const SysDesc &d, int i, const double *__restrict__ x) {
+template <int G>
+__device__ __forceinline__ void ax_group(const SysDesc &d, const double *__restrict__ x, int base, int k0,
+                                         int last, int valid, double &acc) {
 #pragma clang fp contract(off)
-    const int s = i >> 6;
-    const int s0 = d.sptr[s], w = (d.sptr[s + 1] - s0) >> 6;
+    int col[G];
+    double av[G], xv[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) sell_entry<double>(d, base + 64 * min(k0 + k, last), col[k], av[k]);
+#pragma unroll
+    for (int k = 0; k < G; ++k) xv[k] = gl(x)[col[k]];
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+        if (k < valid) acc += av[k] * xv[k];
+}
+
+__device__ __forceinline__ double row_ax64(const SysDesc &d, int i, const double *__restrict__ x) {
+    const int s = uniform(i >> 6);
+    const int s0 = uniform(gl(d.sptr)[s]), w = uniform((gl(d.sptr)[s + 1] - s0) >> 6);
     const int base = s0 + (i & 63);
     double acc = 0.0;
-    for (int k0 = 0; k0 < w; k0 += kRowUnroll) {
-        int col[kRowUnroll];
-        double av[kRowUnroll], xv[kRowUnroll];
-#pragma unroll
-        for (int k = 0; k < kRowUnroll; ++k)
-            if (k0 + k < w) {
-                col[k] = d.six[base + 64 * (k0 + k)];
-                av[k] = d.sa64[base + 64 * (k0 + k)];
-            }
-#pragma unroll
-        for (int k = 0; k < kRowUnroll; ++k)
-            if (k0 + k < w) xv[k] = x[col[k]];
-#pragma unroll
-        for (int k = 0; k < kRowUnroll; ++k)
-            if (k0 + k < w) acc += av[k] * xv[k];
+    if (w == 7) {
+        ax_group<7>(d, x, base, 0, 6, 7, acc);
+    } else if (w == 5) {
+        ax_group<5>(d, x, base, 0, 4, 5, acc);
+    } else {
+        for (int k0 = 0; k0 < w; k0 += kRowUnroll) ax_group<kRowUnroll>(d, x, base, k0, w - 1, w - k0, acc);
     }
     return acc;
 }
@@ -245,19 +304,35 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double *__restri
     }
 }
 
-// sum of a system's tile partials in tile order (finalize kernels; one block)
+// Sum of a system's tile partials (finalize kernels: one kFinBlock-thread
+// workgroup per system).  Thread t adds tiles t, t + kFinBlock, ... in
+// ascending order, then a fixed tree combines the threads: the order depends
+// on the system's tile count only.  The loads of kFinUnroll tiles are all
+// issued before the first add (the partials come from other XCDs' writes, so
+// every load is a long trip).
+constexpr int kFinUnroll = 8;
 template <int NV>
 __device__ __forceinline__ void sum_partials(const SysDesc &d, const double *__restrict__ partials,
                                              double (&tot)[NV]) {
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int q = threadIdx.x; q < d.ntiles; q += kBlock) {
-        const double *pp = partials + (size_t)(d.tile0 + q) * kMaxPartials;
+    const double *base = partials + (size_t)d.tile0 * kMaxPartials;
+    for (int q0 = threadIdx.x; q0 < d.ntiles; q0 += kFinUnroll * kFinBlock) {
+        double v[kFinUnroll][NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += pp[k];
+        for (int u = 0; u < kFinUnroll; ++u) {
+            const int q = q0 + u * kFinBlock;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) v[u][k] = q < d.ntiles ? base[(size_t)q * kMaxPartials + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kFinUnroll; ++u)
+#pragma unroll
+            for (int k = 0; k < NV; ++k)
+                if (q0 + u * kFinBlock < d.ntiles) acc[k] += v[u][k];
     }
-    block_sum<NV>(acc);
+    block_sum<NV, kFinBlock>(acc);
 #pragma unroll
     for (int k = 0; k < NV; ++k) tot[k] = acc[k];
 }
@@ -280,25 +355,25 @@ __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys
         const int i = ti.y + threadIdx.x;
         double v[2] = {0.0, 0.0};
         if (i < d.n) {
-            const double bi = d.b[i];
+            const double bi = gl(d.b)[i];
             v[0] = bi * bi;
             if (bp.precision == KCFD_FP64) {
                 const double ri = bi - row_ax64(d, i, d.x);
-                ((double *)d.r)[i] = ri;
-                ((double *)d.rt)[i] = ri;
+                vec<double>(d.r)[i] = ri;
+                vec<double>(d.rt)[i] = ri;
                 v[1] = ri * ri;
             }
             if (PC == KCFD_PC_ILU0) {
-                ((V *)d.y)[i] = sentinel_value<V>();
-                ((V *)d.ph)[i] = sentinel_value<V>();
-                ((V *)d.sh)[i] = sentinel_value<V>();
+                vec<V>(d.y)[i] = sentinel_value<V>();
+                vec<V>(d.ph)[i] = sentinel_value<V>();
+                vec<V>(d.sh)[i] = sentinel_value<V>();
             }
         }
         write_partials<2>(v, partials, tile);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void f_init(const SysDesc *__restrict__ sys,
+__global__ __launch_bounds__(kFinBlock) void f_init(const SysDesc *__restrict__ sys,
                                                  SysState *__restrict__ sts,
                                                  const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
     const BatchParams bp = *bpp;
@@ -352,24 +427,24 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
         const int status = st.status;
         const int i = ti.y + threadIdx.x;
         if (i >= d.n) continue;
-        if (st.save) d.xb[i] = d.x[i];     // MIXED: the last update improved on the best
+        if (st.save) glw(d.xb)[i] = glw(d.x)[i];     // MIXED: the last update improved on the best
         if (status == S_RUN || status == S_RESUME) {
-            V *__restrict__ p = (V *)d.p;
+            KCFD_GLOBAL V *p = vec<V>(d.p);
             const V om = (V)st.omega, be = (V)st.beta;
-            V pi = p[i] - om * ((const V *)d.v)[i];
+            V pi = p[i] - om * vec<V>(d.v)[i];
             pi = pi * be;
-            p[i] = pi + ((const V *)d.r)[i];
+            p[i] = pi + vec<V>(d.r)[i];
         } else if (status == S_START) {
-            ((V *)d.p)[i] = ((const V *)d.r)[i];
+            vec<V>(d.p)[i] = vec<V>(d.r)[i];
         } else if (status == S_REFINE) {
-            if (st.rollback) d.x[i] = d.xb[i];   // the inner iteration diverged
+            if (st.rollback) glw(d.x)[i] = glw(d.xb)[i];   // the inner iteration diverged
             if (PC == KCFD_PC_ILU0 && d.polled) {
-                ((V *)d.ph)[i] = sentinel_value<V>();
-                ((V *)d.sh)[i] = sentinel_value<V>();
-                ((V *)d.y)[i] = sentinel_value<V>();
+                vec<V>(d.ph)[i] = sentinel_value<V>();
+                vec<V>(d.sh)[i] = sentinel_value<V>();
+                vec<V>(d.y)[i] = sentinel_value<V>();
             }
         } else if (status == S_ZEROX) {
-            d.x[i] = 0.0;
+            glw(d.x)[i] = 0.0;
         }
     }
 }
@@ -394,19 +469,19 @@ __global__ __launch_bounds__(kBlock) void k_spmv_p(const SysDesc *__restrict__ s
             double v[1] = {0.0};
             if (i < d.n) {
                 const V vv = (V)row_amz<V, PC>(d, i, (const V *)d.p, (const V *)d.ph);
-                ((V *)d.v)[i] = vv;
-                v[0] = (double)((const V *)d.rt)[i] * (double)vv;
+                vec<V>(d.v)[i] = vv;
+                v[0] = (double)vec<V>(d.rt)[i] * (double)vv;
             }
             write_partials<1>(v, partials, tile);
         } else if (status == S_REFINE) {
             double v[3] = {0.0, 0.0, 0.0};
             if (i < d.n) {
-                const double r64 = d.b[i] - row_ax64(d, i, d.x);
+                const double r64 = gl(d.b)[i] - row_ax64(d, i, d.x);
                 const V rv = (V)r64;
-                ((V *)d.r)[i] = rv;
-                if (st.restart) ((V *)d.rt)[i] = rv;
+                vec<V>(d.r)[i] = rv;
+                if (st.restart) vec<V>(d.rt)[i] = rv;
                 v[0] = r64 * r64;
-                v[1] = (double)((const V *)d.rt)[i] * (double)rv;
+                v[1] = (double)vec<V>(d.rt)[i] * (double)rv;
                 v[2] = (double)rv * (double)rv;
             }
             write_partials<3>(v, partials, tile);
@@ -414,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_p(const SysDesc *__restrict__ s
     }
 }
 
-__global__ __launch_bounds__(kBlock) void f_spmv_p(const SysDesc *__restrict__ sys,
+__global__ __launch_bounds__(kFinBlock) void f_spmv_p(const SysDesc *__restrict__ sys,
                                                    SysState *__restrict__ sts,
                                                    const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
     const BatchParams bp = *bpp;
@@ -512,8 +587,8 @@ __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
         double v[1] = {0.0};
         if (i < d.n) {
             const V al = (V)st.alpha;
-            const V si = ((const V *)d.r)[i] - al * ((const V *)d.v)[i];
-            ((V *)d.s)[i] = si;
+            const V si = vec<V>(d.r)[i] - al * vec<V>(d.v)[i];
+            vec<V>(d.s)[i] = si;
             v[0] = (double)si * (double)si;
         }
         write_partials<1>(v, partials, tile, 2);   // ||s||^2 beside K4's t.s, t.t
@@ -540,8 +615,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ s
             double v[2] = {0.0, 0.0};
             if (i < d.n) {
                 const V tv = (V)row_amz<V, PC>(d, i, (const V *)d.s, (const V *)d.sh);
-                ((V *)d.t)[i] = tv;
-                v[0] = (double)tv * (double)((const V *)d.s)[i];
+                vec<V>(d.t)[i] = tv;
+                v[0] = (double)tv * (double)vec<V>(d.s)[i];
                 v[1] = (double)tv * (double)tv;
             }
             write_partials<2>(v, partials, tile);
@@ -549,7 +624,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ s
     }
 }
 
-__global__ __launch_bounds__(kBlock) void f_spmv_s(const SysDesc *__restrict__ sys,
+__global__ __launch_bounds__(kFinBlock) void f_spmv_s(const SysDesc *__restrict__ sys,
                                                    SysState *__restrict__ sts,
                                                    const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
     const BatchParams bp = *bpp;
@@ -588,12 +663,12 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
                 const V al = (V)st.alpha;
                 const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
                 if (bp.precision == KCFD_FP64)       // scipy's operation, in FP64
-                    ((V *)d.x)[i] = ((V *)d.x)[i] + al * ph;
+                    vec<V>(d.x)[i] = vec<V>(d.x)[i] + al * ph;
                 else                                 // MIXED: FP32 direction, FP64 iterate
-                    d.x[i] = d.x[i] + st.alpha * (double)ph;
+                    glw(d.x)[i] = glw(d.x)[i] + st.alpha * (double)ph;
                 if (PC == KCFD_PC_ILU0 && d.polled) {
-                    ((V *)d.ph)[i] = sentinel_value<V>();
-                    ((V *)d.sh)[i] = sentinel_value<V>();
+                    vec<V>(d.ph)[i] = sentinel_value<V>();
+                    vec<V>(d.sh)[i] = sentinel_value<V>();
                 }
             }
             continue;
@@ -605,30 +680,30 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
             const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
             const V sh = (V)precond_at<V, PC>(d, (const V *)d.s, (const V *)d.sh, i);
             if (bp.precision == KCFD_FP64) {         // scipy's operation order
-                V *__restrict__ xx = (V *)d.x;
+                KCFD_GLOBAL V *xx = vec<V>(d.x);
                 V xi = xx[i] + al * ph;
                 xi = xi + om * sh;
                 xx[i] = xi;
             } else {   // MIXED: the iterate accumulates in FP64 (an FP32 correction
                        // would carry an error of eps_32 |x| into the residual)
-                double xi = d.x[i] + st.alpha * (double)ph;
+                double xi = glw(d.x)[i] + st.alpha * (double)ph;
                 xi = xi + st.omega * (double)sh;
-                d.x[i] = xi;
+                glw(d.x)[i] = xi;
             }
-            const V ri = ((const V *)d.s)[i] - om * ((const V *)d.t)[i];
-            ((V *)d.r)[i] = ri;
+            const V ri = vec<V>(d.s)[i] - om * vec<V>(d.t)[i];
+            vec<V>(d.r)[i] = ri;
             v[0] = (double)ri * (double)ri;
-            v[1] = (double)((const V *)d.rt)[i] * (double)ri;
+            v[1] = (double)vec<V>(d.rt)[i] * (double)ri;
             if (PC == KCFD_PC_ILU0 && d.polled) {   // ready flags of the sync-free solves
-                ((V *)d.ph)[i] = sentinel_value<V>();
-                ((V *)d.sh)[i] = sentinel_value<V>();
+                vec<V>(d.ph)[i] = sentinel_value<V>();
+                vec<V>(d.sh)[i] = sentinel_value<V>();
             }
         }
         write_partials<2>(v, partials, tile);
     }
 }
 
-__global__ __launch_bounds__(kBlock) void f_xr(const SysDesc *__restrict__ sys, SysState *__restrict__ sts,
+__global__ __launch_bounds__(kFinBlock) void f_xr(const SysDesc *__restrict__ sys, SysState *__restrict__ sts,
                                                const double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
     const BatchParams bp = *bpp;
     const SysDesc &d = sys[blockIdx.x];
@@ -707,7 +782,7 @@ __global__ __launch_bounds__(kBlock) void k_restore(const SysDesc *__restrict__ 
         if (!wants_true(st) || !st.restore) continue;
         const SysDesc &d = sys[ti.x];
         const int i = ti.y + threadIdx.x;
-        if (i < d.n) d.x[i] = d.xb[i];
+        if (i < d.n) glw(d.x)[i] = glw(d.xb)[i];
     }
 }
 
@@ -724,7 +799,7 @@ __global__ __launch_bounds__(kBlock) void k_true_resid(const SysDesc *__restrict
         const int i = ti.y + threadIdx.x;
         double v[2] = {0.0, 0.0};
         if (i < d.n) {
-            const double bi = d.b[i];
+            const double bi = gl(d.b)[i];
             const double ri = bi - row_ax64(d, i, d.x);
             v[0] = ri * ri;
             v[1] = bi * bi;
@@ -733,18 +808,58 @@ __global__ __launch_bounds__(kBlock) void k_true_resid(const SysDesc *__restrict
     }
 }
 
-__global__ __launch_bounds__(kBlock) void f_true_resid(const SysDesc *__restrict__ sys,
+__global__ __launch_bounds__(kFinBlock) void f_true_resid(const SysDesc *__restrict__ sys,
                                                        SysState *__restrict__ sts,
                                                        const double *__restrict__ partials) {
     const SysDesc &d = sys[blockIdx.x];
     SysState &st = sts[blockIdx.x];
-    if (!wants_true(st)) return;
+    if (!wants_true(st)) {
+        if (threadIdx.x == 0) st.scatter = 0;   // scattered by an earlier k_perm_out
+        return;
+    }
     double tot[2];
     sum_partials<2>(d, partials, tot);
     if (threadIdx.x != 0) return;
     st.true_rr = tot[0];
     st.true_bb = tot[1];
     st.true_done = 1;
+    st.scatter = 1;
+}
+
+// Permuted row order: b and the initial x gathered from the caller's
+// buffers for systems about to start (before k_init, which reads x at
+// neighbouring rows) ...
+__global__ __launch_bounds__(kBlock) void k_perm_in(const SysDesc *__restrict__ sys,
+                                                    const SysState *__restrict__ sts,
+                                                    const int2 *__restrict__ tiles,
+                                                    const int *__restrict__ counts) {
+    const int ntiles = counts[0];   // tiles of the current membership
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        if (!d.perm || sts[ti.x].status != S_NEW) continue;
+        const int i = ti.y + threadIdx.x;
+        if (i < d.n) {
+            const int r = gl(d.perm)[i];
+            glw(const_cast<double *>(d.b))[i] = gl(d.b_user)[r];
+            glw(d.x)[i] = glw(d.x_user)[r];
+        }
+    }
+}
+
+// ... and the solution scattered back once its true residual is in.
+__global__ __launch_bounds__(kBlock) void k_perm_out(const SysDesc *__restrict__ sys,
+                                                     const SysState *__restrict__ sts,
+                                                     const int2 *__restrict__ tiles,
+                                                     const int *__restrict__ counts) {
+    const int ntiles = counts[0];   // tiles of the current membership
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        if (!d.perm || !sts[ti.x].scatter) continue;
+        const int i = ti.y + threadIdx.x;
+        if (i < d.n) glw(d.x_user)[gl(d.perm)[i]] = glw(d.x)[i];
+    }
 }
 
 // ---------------------------------------------------- ILU(0) trisolves
@@ -761,10 +876,10 @@ __global__ __launch_bounds__(kBlock) void f_true_resid(const SysDesc *__restrict
 // vector doubles as the ready flag: it holds a NaN sentinel until the row is
 // written (agent-scope atomic store) and waiters poll it with agent-scope
 // atomic loads.  Row arithmetic follows the oracle's order exactly.
-// Rows are positions i of P A P^T (perm maps them to rows of A; identity in
-// the natural ordering); y is kept in P-order, z and w in A's order.
-//   LOWER: y_i = z_{perm i} - sum_{j<i} l_ij y_j                   (unit L)
-//   UPPER: w_{perm i} = (y_i - sum_{j>i} u_ij w_{perm j}) / u_ii; y_i reset
+// Rows are positions i of P A P^T, the solver's row order (every vector is
+// in that order):
+//   LOWER: y_i = z_i - sum_{j<i} l_ij y_j                   (unit L)
+//   UPPER: w_i = (y_i - sum_{j>i} u_ij w_j) / u_ii; y_i reset
 template <typename V>
 __device__ __forceinline__ void publish(V *p, V v) {
     using U = typename Sentinel<V>::U;
@@ -779,8 +894,7 @@ __device__ __forceinline__ void publish(V *p, V v) {
 // requested before the first subtraction, so a row costs about one memory
 // round trip per group instead of one per entry.
 template <typename V>
-__device__ __forceinline__ V sub_ready(V acc, const V *__restrict__ a, const int *__restrict__ cols,
-                                       int base, int w, const V *vec) {
+__device__ __forceinline__ V sub_ready(V acc, const FacRef &f, int base, int w, const V *vec) {
 #pragma clang fp contract(off)
     using U = typename Sentinel<V>::U;
     constexpr int G = 8;
@@ -789,21 +903,14 @@ __device__ __forceinline__ V sub_ready(V acc, const V *__restrict__ a, const int
         V av[G];
         U bits[G];
 #pragma unroll
-        for (int k = 0; k < G; ++k) {
-            c[k] = -1;
-            if (kb + k < w) {
-                c[k] = cols[base + 64 * (kb + k)];
-                av[k] = a[base + 64 * (kb + k)];
-            }
-        }
+        for (int k = 0; k < G; ++k) fac_entry<V>(f, base + 64 * min(kb + k, w - 1), c[k], av[k]);
+#pragma unroll
+        for (int k = 0; k < G; ++k)   // padding (column -1) polls entry 0 and ignores it
+            bits[k] = __hip_atomic_load(reinterpret_cast<const U *>(vec + max(c[k], 0)), __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
 #pragma unroll
         for (int k = 0; k < G; ++k)
-            if (c[k] >= 0)
-                bits[k] = __hip_atomic_load(reinterpret_cast<const U *>(vec + c[k]), __ATOMIC_RELAXED,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-        for (int k = 0; k < G; ++k)
-            if (c[k] >= 0) {
+            if (kb + k < w && c[k] >= 0) {
                 while (bits[k] == Sentinel<V>::bits) {
                     __builtin_amdgcn_s_sleep(2);
                     bits[k] = __hip_atomic_load(reinterpret_cast<const U *>(vec + c[k]), __ATOMIC_RELAXED,
@@ -840,62 +947,66 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
                                      : (status == S_RUN);
         const bool live = want && lane < ch.count;
         if (live) {
-            const int i = lvl_rows[ch.sys][ch.start + lane];
-            const int r = d.perm ? d.perm[i] : i;
+            const int i = gl(lvl_rows[ch.sys])[ch.start + lane];
             const int *off = UPPER ? d.chu_off : d.chl_off;
-            const int o0 = off[ch.local];
-            const int w = (off[ch.local + 1] - o0) >> 6;
+            const int o0 = uniform(gl(off)[ch.local]);
+            const int w = uniform((gl(off)[ch.local + 1] - o0) >> 6);
             if (!UPPER) {
                 const V *z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
                 V *y = (V *)d.y;
-                const V acc = sub_ready<V>(z[r], chl_vals<V>(d), d.chl_col, o0 + lane, w, y);
+                const V acc = sub_ready<V>(gl(z)[i], fac_l(d), o0 + lane, w, y);
                 publish<V>(y + i, acc);
             } else {
                 V *y = (V *)d.y;
                 V *wv = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
-                const V acc = sub_ready<V>(y[i], chu_vals<V>(d), d.chu_col, o0 + lane, w, wv);
-                publish<V>(wv + r, acc / udiag<V>(d)[i]);
-                y[i] = sentinel_value<V>();
+                const V acc = sub_ready<V>(glw(y)[i], fac_u(d), o0 + lane, w, wv);
+                publish<V>(wv + i, acc / gl(udiag<V>(d))[i]);
+                glw(y)[i] = sentinel_value<V>();
             }
         }
     }
 }
 
 // acc - sum_k a_k vec[col_k] over one chunk row (layout as sub_ready), with
-// plain loads: every operand was written by an earlier launch.
-template <typename V>
-__device__ __forceinline__ V sub_plain(V acc, const V *__restrict__ a, const int *__restrict__ cols,
-                                       int base, int w, const V *__restrict__ vec) {
+// plain loads: every operand was written by an earlier launch.  As in
+// row_amz: fixed groups for the common widths, else groups of 8 whose loads
+// past the width re-load the last entry; padding (column -1) gathers entry
+// 0.  Neither is accumulated.
+template <typename V, int G>
+__device__ __forceinline__ void sub_group(V &acc, const FacRef &f, int base, int k0, int last, int valid,
+                                          const KCFD_GLOBAL V *__restrict__ vec) {
 #pragma clang fp contract(off)
-    constexpr int G = 8;
-    for (int kb = 0; kb < w; kb += G) {
-        int c[G];
-        V av[G], xv[G];
+    int c[G];
+    V av[G], xv[G];
 #pragma unroll
-        for (int k = 0; k < G; ++k) {
-            c[k] = -1;
-            if (kb + k < w) {
-                c[k] = cols[base + 64 * (kb + k)];
-                av[k] = a[base + 64 * (kb + k)];
-            }
-        }
+    for (int k = 0; k < G; ++k) fac_entry<V>(f, base + 64 * min(k0 + k, last), c[k], av[k]);
 #pragma unroll
-        for (int k = 0; k < G; ++k)
-            if (c[k] >= 0) xv[k] = vec[c[k]];
+    for (int k = 0; k < G; ++k) xv[k] = vec[max(c[k], 0)];
 #pragma unroll
-        for (int k = 0; k < G; ++k)
-            if (c[k] >= 0) acc -= av[k] * xv[k];
+    for (int k = 0; k < G; ++k)
+        if (k < valid && c[k] >= 0) acc -= av[k] * xv[k];
+}
+
+template <typename V>
+__device__ __forceinline__ V sub_plain(V acc, const FacRef &f, int base, int w,
+                                       const KCFD_GLOBAL V *__restrict__ vec) {
+    if (w == 6) {
+        sub_group<V, 6>(acc, f, base, 0, 5, 6, vec);
+    } else if (w == 4) {
+        sub_group<V, 4>(acc, f, base, 0, 3, 4, vec);
+    } else {
+        constexpr int G = 8;
+        for (int kb = 0; kb < w; kb += G) sub_group<V, G>(acc, f, base, kb, w - 1, w - kb, vec);
     }
     return acc;
 }
 
 // ILU(0) apply for a two-colour ordering, P A P^T = [[D0, B], [C, D1]] with
-// colour 0 first.  Colour-0 rows have no L entries, so their y is z; colour-1
-// rows have no U entries.  Hence, exactly as the sequential L then U solves
-// compute it (same operations, same order):
-//   FIRST  (colour-1 rows i): y_i = z_{perm i} - sum_j l_ij z_{perm j};
-//                             w_{perm i} = y_i / u_ii
-//   SECOND (colour-0 rows i): w_{perm i} = (z_{perm i} - sum_j u_ij w_{perm j}) / u_ii
+// colour 0 first (vectors in P-order).  Colour-0 rows have no L entries, so
+// their y is z; colour-1 rows have no U entries.  Hence, exactly as the
+// sequential L then U solves compute it (same operations, same order):
+//   FIRST  (colour-1 rows i): y_i = z_i - sum_j l_ij z_j;  w_i = y_i / u_ii
+//   SECOND (colour-0 rows i): w_i = (z_i - sum_j u_ij w_j) / u_ii
 // Two plain launches per apply: no y vector, no ready flags.  Chunks are runs
 // of 64 rows of one colour; one wave per chunk, dealt round-robin.
 template <typename V, bool SECOND, int WHICH>
@@ -916,20 +1027,13 @@ __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys
                                      : (status == S_RUN);
         if (want && lane < ch.count) {
             const int i = ch.start + lane;       // row of P A P^T
-            const int r = d.perm[i];
-            const V *__restrict__ z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
-            V *__restrict__ w = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
-            if (!SECOND) {
-                const int o0 = d.chl_off[ch.local];
-                const V y = sub_plain<V>(z[r], chl_vals<V>(d), d.chl_ncol, o0 + lane,
-                                         (d.chl_off[ch.local + 1] - o0) >> 6, z);
-                w[r] = y / udiag<V>(d)[i];
-            } else {
-                const int o0 = d.chu_off[ch.local];
-                const V v = sub_plain<V>(z[r], chu_vals<V>(d), d.chu_col, o0 + lane,
-                                         (d.chu_off[ch.local + 1] - o0) >> 6, w);
-                w[r] = v / udiag<V>(d)[i];
-            }
+            const KCFD_GLOBAL V *__restrict__ z = vec<V>(WHICH == 0 ? d.p : d.s);
+            KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 0 ? d.ph : d.sh);
+            const int *off = SECOND ? d.chu_off : d.chl_off;
+            const int o0 = uniform(gl(off)[ch.local]);
+            const int wd = uniform((gl(off)[ch.local + 1] - o0) >> 6);
+            const V v = sub_plain<V>(z[i], SECOND ? fac_u(d) : fac_l(d), o0 + lane, wd, SECOND ? w : z);
+            w[i] = v / gl(udiag<V>(d))[i];
         }
     }
 }
@@ -965,7 +1069,8 @@ struct kcfd_batch {
     int cap_tiles = 0;
     double *d_partials = nullptr;
     unsigned int *d_counters = nullptr;
-    int *d_counts = nullptr;        // {ntiles, nchunks_l, nchunks_u}, read by the kernels
+    int *d_counts = nullptr;        // {ntiles, nchunks_l, nchunks_u, nchunks_a, nchunks_b, walk},
+                                    // read by the kernels
     void *slab = nullptr;           // static: all per-system vectors
     size_t slab_bytes = 0;
     // ILU chunk tables (level-major over systems) for the L and U solves
@@ -992,7 +1097,7 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 template <typename V, int PC>
 void launch_iteration(kcfd_batch *b, hipStream_t s) {
     const int g = b->grid, ns = b->nsys;
-    const dim3 blk(kBlock);
+    const dim3 blk(kBlock), fin(kFinBlock);
     auto mark = [&](int phase, hipEvent_t ev) {
         if (b->timed_phase == phase) (void)hipEventRecord(ev, s);
     };
@@ -1020,7 +1125,7 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     hipLaunchKernelGGL((k_spmv_p<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->d_counts, b->d_partials);
     mark(1, b->ctx->ev1);
-    hipLaunchKernelGGL(f_spmv_p, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
+    hipLaunchKernelGGL(f_spmv_p, dim3(ns), fin, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     hipLaunchKernelGGL((k_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
                        b->d_partials, b->d_counters);
     if (PC == KCFD_PC_ILU0) {
@@ -1039,19 +1144,22 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     }
     hipLaunchKernelGGL((k_spmv_s<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->d_counts, b->d_partials, b->d_params);
-    hipLaunchKernelGGL(f_spmv_s, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
+    hipLaunchKernelGGL(f_spmv_s, dim3(ns), fin, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
     mark(2, b->ctx->ev0);
     hipLaunchKernelGGL((k_xr<V, PC>), dim3(g), blk, 0, s, b->d_desc, b->d_state, b->d_tiles,
                        b->d_counts, b->d_partials, b->d_params);
     mark(2, b->ctx->ev1);
-    hipLaunchKernelGGL(f_xr, dim3(ns), blk, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
+    hipLaunchKernelGGL(f_xr, dim3(ns), fin, 0, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
 }
 
 template <typename V, int PC>
 void launch_init(kcfd_batch *b, hipStream_t s) {
+    if (PC == KCFD_PC_ILU0)
+        hipLaunchKernelGGL(k_perm_in, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_tiles,
+                           b->d_counts);
     hipLaunchKernelGGL((k_init<V, PC>), dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
                        b->d_tiles, b->d_counts, b->d_partials, b->d_params);
-    hipLaunchKernelGGL(f_init, dim3(b->nsys), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+    hipLaunchKernelGGL(f_init, dim3(b->nsys), dim3(kFinBlock), 0, s, b->d_desc, b->d_state,
                        b->d_partials, b->d_params);
 }
 
@@ -1088,8 +1196,7 @@ double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st)
     auto MF = [&](double sv) { return nnz * (sv + 4) + 2 * (n + 1) * 4; };
     auto per_iter = [&](double sv) {
         if (pc == KCFD_PC_JACOBI) return 2 * MA(sv) + 22 * n * sv;
-        // a permuted ILU(0) reads perm once per triangular solve
-        if (pc == KCFD_PC_ILU0) return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv + (m->perm ? 16 * n : 0);
+        if (pc == KCFD_PC_ILU0) return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv;
         return 2 * MA(sv) + 19 * n * sv;
     };
     if (precision == KCFD_FP64) return (st.total_iters + 0.5 * st.half) * per_iter(8) + MA(8) + 3 * n * 8;
@@ -1103,16 +1210,15 @@ void fill_desc(SysDesc &d, kcfd_mat *m) {
     d.ix = m->f_indices;
     d.dpos = m->f_diagpos;
     d.uix = m->f_uix;
-    d.perm = m->perm;
+    d.perm = m->pc_kind == KCFD_PC_ILU0 && m->sell_permuted ? m->perm : nullptr;
     d.sptr = m->slice_ptr;
     d.six = m->six;
     d.sa64 = m->sa64;
-    d.sa32 = m->sa32;
+    d.spk32 = m->spk32;
     d.dinv64 = m->dinv;
     d.dinv32 = m->dinv32;
     d.lu64 = m->lu;
     d.lu32 = m->lu32;
-    d.chl_ncol = m->chl_ncol;
     d.polled = m->pc_kind == KCFD_PC_ILU0 && !m->two_colour;
     d.chl_off = m->chl_off;
     d.chl_col = m->chl_col;
@@ -1121,8 +1227,8 @@ void fill_desc(SysDesc &d, kcfd_mat *m) {
     d.chl_v64 = m->chl_v64;
     d.chu_v64 = m->chu_v64;
     d.ud64 = m->ud64;
-    d.chl_v32 = m->chl_v32;
-    d.chu_v32 = m->chu_v32;
+    d.chl_pk32 = m->chl_pk32;
+    d.chu_pk32 = m->chu_pk32;
     d.ud32 = m->ud32;
 }
 
@@ -1133,6 +1239,7 @@ int check_opts(const kcfd_solve_opts *o) {
     KCFD_ARG(prec == KCFD_FP64 || prec == KCFD_MIXED, "unknown precision");
     KCFD_ARG(o->grid_per_cu >= 0 && o->grid_per_cu <= 8, "grid_per_cu must be 0..8");
     KCFD_ARG(o->trsv_per_cu >= 0 && o->trsv_per_cu <= 4, "trsv_per_cu must be 0..4");
+    KCFD_ARG(o->tile_walk >= 0 && o->tile_walk <= 2, "tile_walk must be 0..2");
     return KCFD_OK;
 }
 
@@ -1201,22 +1308,38 @@ int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **o
     return KCFD_OK;
 }
 
-// FP32 / FP64 vector slots per system; MIXED adds two FP32-sized slots for
-// the FP64 best iterate xb
-int vectors_per_system(const kcfd_batch *b) {
-    return (b->bp.precond == KCFD_PC_ILU0 ? 9 : 6) + (b->bp.precision == KCFD_MIXED ? 2 : 0);
-}
+// Per-system workspace: 6 Krylov vectors (9 with ILU) of the precision's
+// value type; MIXED adds the FP64 best iterate xb; a permuted row order adds
+// FP64 b and x in that order.
+int vectors_per_system(const kcfd_batch *b) { return b->bp.precond == KCFD_PC_ILU0 ? 9 : 6; }
 size_t value_size(const kcfd_batch *b) { return b->bp.precision == KCFD_FP64 ? sizeof(double) : sizeof(float); }
 size_t vec_stride(const kcfd_batch *b, int n) { return align_up((size_t)std::max(n, 1) * value_size(b), 256); }
+size_t f64_stride(int n) { return align_up((size_t)std::max(n, 1) * sizeof(double), 256); }
+size_t slot_bytes(const kcfd_batch *b, int n, bool permuted) {
+    size_t v = (size_t)vectors_per_system(b) * vec_stride(b, n);
+    if (b->bp.precision == KCFD_MIXED) v += f64_stride(n);
+    if (permuted) v += 2 * f64_stride(n);
+    return v;
+}
 
-// slot k's vectors at base (the iterate x is the caller's FP64 buffer)
+// slot k's vectors at base.  In the natural order the iterate is the
+// caller's FP64 x and b is read in place.
 void place_vectors(kcfd_batch *b, int k, char *base) {
     SysDesc &d = b->h_desc[k];
-    const size_t stride = vec_stride(b, d.n);
+    const size_t stride = vec_stride(b, d.n), s64 = f64_stride(d.n);
     void **vecs[9] = {&d.r, &d.rt, &d.p, &d.v, &d.s, &d.t, &d.ph, &d.sh, &d.y};
-    const int nv = b->bp.precond == KCFD_PC_ILU0 ? 9 : 6;
+    const int nv = vectors_per_system(b);
     for (int q = 0; q < nv; ++q) *vecs[q] = base + q * stride;
-    d.xb = b->bp.precision == KCFD_MIXED ? (double *)(base + nv * stride) : nullptr;
+    char *tail = base + nv * stride;
+    d.xb = nullptr;
+    if (b->bp.precision == KCFD_MIXED) {
+        d.xb = (double *)tail;
+        tail += s64;
+    }
+    if (d.perm) {
+        d.b = (const double *)tail;
+        d.x = (double *)(tail + s64);
+    }
 }
 
 void describe_slot(kcfd_batch *b, int k, kcfd_mat *m, const double *bvec, double *x) {
@@ -1224,8 +1347,8 @@ void describe_slot(kcfd_batch *b, int k, kcfd_mat *m, const double *bvec, double
     std::memset(&d, 0, sizeof(d));
     d.n = m->n;
     fill_desc(d, m);
-    d.b = bvec;
-    d.x = x;
+    d.b = d.b_user = bvec;
+    d.x = d.x_user = x;
     b->mats[k] = m;
     b->sys_n[k] = m->n;
 }
@@ -1336,7 +1459,8 @@ int rebuild_tables(kcfd_batch *b) {
     if (!tiles.empty())
         KCFD_HIP(hipMemcpyAsync(b->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMemcpyAsync(b->d_desc, b->h_desc.data(), sizeof(SysDesc) * b->nsys, hipMemcpyHostToDevice, s));
-    const int counts[8] = {b->ntiles, b->nchunks_l, b->nchunks_u, b->nchunks_a, b->nchunks_b, 0, 0, 0};
+    const int walk = b->opts.tile_walk == 2 ? 1 : 0;
+    const int counts[8] = {b->ntiles, b->nchunks_l, b->nchunks_u, b->nchunks_a, b->nchunks_b, walk, 0, 0};
     KCFD_HIP(hipMemcpyAsync(b->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice, s));
     // the host arrays above are locals: finish the copies before they go
     KCFD_HIP(hipStreamSynchronize(s));
@@ -1363,7 +1487,7 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
         if ((rc = check_mat(b, mats[k], k)) != KCFD_OK) return fail(rc);
         describe_slot(b, k, mats[k], bs[k], xs[k]);
         off[k] = slab;
-        slab += align_up((size_t)vectors_per_system(b) * vec_stride(b, mats[k]->n), 256);
+        slab += align_up(slot_bytes(b, mats[k]->n, b->h_desc[k].perm != nullptr), 256);
     }
     if (hipMalloc(&b->slab, slab ? slab : 256) != hipSuccess) {
         set_error("out of device memory for the batch vectors (%zu bytes)", slab);
@@ -1471,8 +1595,11 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
                            b->d_tiles, b->d_counts);
     hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
                        b->d_tiles, b->d_counts, b->d_partials);
-    hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+    hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kFinBlock), 0, s, b->d_desc, b->d_state,
                        b->d_partials);
+    if (b->bp.precond == KCFD_PC_ILU0)
+        hipLaunchKernelGGL(k_perm_out, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                           b->d_tiles, b->d_counts);
     KCFD_HIP(hipGetLastError());
     KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
                             hipMemcpyDeviceToHost, s));
@@ -1582,14 +1709,17 @@ int kcfd_batch_attach(kcfd_batch *b, kcfd_mat *m, const double *bvec, double *x,
         return KCFD_EBUSY;
     }
     KCFD_CHECK(check_mat(b, m, k));
+    describe_slot(b, k, m, bvec, x);
     void *mem = nullptr;
-    const size_t bytes = (size_t)vectors_per_system(b) * vec_stride(b, m->n);
+    const size_t bytes = slot_bytes(b, m->n, b->h_desc[k].perm != nullptr);
     if (hipMalloc(&mem, bytes) != hipSuccess) {
         (void)hipGetLastError();
+        std::memset(&b->h_desc[k], 0, sizeof(SysDesc));
+        b->mats[k] = nullptr;
+        b->sys_n[k] = 0;
         set_error("out of device memory for a system's vectors (%zu bytes)", bytes);
         return KCFD_EOOM;
     }
-    describe_slot(b, k, m, bvec, x);
     b->slot_mem[k] = mem;
     place_vectors(b, k, (char *)mem);
     SysState st{};
@@ -1632,8 +1762,11 @@ int kcfd_batch_step(kcfd_batch *b, int *finished, int *nfinished) {
                            b->d_tiles, b->d_counts);
     hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
                        b->d_tiles, b->d_counts, b->d_partials);
-    hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+    hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kFinBlock), 0, s, b->d_desc, b->d_state,
                        b->d_partials);
+    if (b->bp.precond == KCFD_PC_ILU0)
+        hipLaunchKernelGGL(k_perm_out, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                           b->d_tiles, b->d_counts);
     KCFD_HIP(hipGetLastError());
     KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
                             hipMemcpyDeviceToHost, s));
@@ -1707,7 +1840,7 @@ int kcfd_batch_bytes(kcfd_batch *b, long long *bytes) {
     KCFD_ARG(b && bytes, "null argument");
     long long v = (long long)b->slab_bytes;
     for (int k = 0; k < b->nsys; ++k)
-        if (b->slot_mem[k]) v += (long long)vectors_per_system(b) * (long long)vec_stride(b, b->sys_n[k]);
+        if (b->slot_mem[k]) v += (long long)slot_bytes(b, b->sys_n[k], b->h_desc[k].perm != nullptr);
     *bytes = v + (long long)b->cap_tiles * (8 + 8 * kMaxPartials) +
              (long long)b->nsys * (long long)(sizeof(SysDesc) + sizeof(SysState) + 2 * sizeof(int *)) +
              (long long)(b->cap_chunks_l + b->cap_chunks_u) * (long long)sizeof(Chunk);
@@ -1796,10 +1929,11 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
         if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
         else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
         else if (phase == 3) {
-            if (m->two_colour)   // both kernels of the two-colour apply: L and U entries, z, w, perm, u_ii
-                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + n * (4 + sv) + 3 * n * sv;
+            if (m->two_colour)   // both kernels of the two-colour apply: L and U entries, z,
+                                 // w written, the other colour's z / w gathered, u_ii
+                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv;
             else                 // the sync-free L solve (one triangle + its level list)
-                tot += MFh + 2 * n * sv + (m->perm ? 4 * n : 0);
+                tot += MFh + 2 * n * sv;
         }
         else if (phase == 4) tot += 4 * n * sv;
     }
@@ -1820,9 +1954,10 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
     // the cached single-system workspace is reused while the preconditioner,
     // precision, launch shape and the b / x buffers are unchanged
     bool reuse = m->solver != nullptr && m->solver->bp.precond == opts->precond &&
-                 m->solver->bp.precision == opts->precision && m->solver->h_desc[0].b == bvec &&
-                 m->solver->h_desc[0].x == x && m->solver->opts.grid_per_cu == opts->grid_per_cu &&
-                 m->solver->opts.trsv_per_cu == opts->trsv_per_cu;
+                 m->solver->bp.precision == opts->precision && m->solver->h_desc[0].b_user == bvec &&
+                 m->solver->h_desc[0].x_user == x && m->solver->opts.grid_per_cu == opts->grid_per_cu &&
+                 m->solver->opts.trsv_per_cu == opts->trsv_per_cu &&
+                 m->solver->opts.tile_walk == opts->tile_walk;
     if (!reuse) {
         if (m->solver) kcfd_batch_destroy(m->solver);
         m->solver = nullptr;

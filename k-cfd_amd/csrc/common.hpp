@@ -19,9 +19,30 @@ void set_error(const char *fmt, ...);
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxPartials = 4;      // doubles per block partial record
+constexpr int kFinBlock = 1024;      // threads of a finalize workgroup (one per system)
 
 // Number of XCDs on MI355X; workgroups are dispatched round-robin over them.
 constexpr int kXcds = 8;
+
+// Global-memory views of pointers.  Pointers read from a descriptor in
+// memory are generic to the compiler, which then emits FLAT loads that wait
+// on both the vector-memory and LDS counters; cast to the global address
+// space, the loads are GLOBAL and can be kept in flight together.
+#define KCFD_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const KCFD_GLOBAL T *gl(const T *p) {
+    return (const KCFD_GLOBAL T *)p;
+}
+template <typename T>
+__device__ __forceinline__ KCFD_GLOBAL T *glw(T *p) {
+    return (KCFD_GLOBAL T *)p;
+}
+// clang's own 2 x int vector: loadable through a global-space pointer (HIP's
+// int2 wrapper type cannot be copied out of one)
+typedef int kint2 __attribute__((ext_vector_type(2)));
+
+// a value the caller knows is the same in every lane of the wave, as such
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // Remap a dispatch-order block id so that each XCD works on one contiguous
 // range of tiles (neighbouring rows share that XCD's L2).  grid must be a
@@ -40,10 +61,12 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block sum of NV values; result valid in thread 0.  Fixed order.
-template <int NV>
+// Block sum of NV values over NT threads; result valid in thread 0.  Fixed
+// order.
+template <int NV, int NT = kBlock>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
-    __shared__ double red[kWaves][NV > 0 ? NV : 1];
+    constexpr int kW = NT / 64;
+    __shared__ double red[kW][NV > 0 ? NV : 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
@@ -56,7 +79,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
 #pragma unroll
         for (int k = 0; k < NV; ++k) {
             double s = red[0][k];
-            for (int q = 1; q < kWaves; ++q) s += red[q][k];
+            for (int q = 1; q < kW; ++q) s += red[q][k];
             v[k] = s;
         }
     }
@@ -114,10 +137,11 @@ struct kcfd_mat {
     // entries keep their CSR order and padding (column = row, value 0) comes
     // last, so its sum equals the sequential CSR row sum.
     int nslices = 0, sell_nnz = 0;
+    bool sell_permuted = false;   // rows and columns in the ILU(0) order (perm)
     std::vector<int> h_slice_ptr;
     int *slice_ptr = nullptr, *six = nullptr;
     double *sa64 = nullptr;
-    float *sa32 = nullptr;
+    int2 *spk32 = nullptr;   // MIXED: {column, FP32 value bits} per SELL entry (one 8-byte load)
     // preconditioner
     int pc_kind = KCFD_PC_NONE;
     bool have32 = false;
@@ -130,7 +154,7 @@ struct kcfd_mat {
     // column as a row of A (what the U solve, writing in A's numbering, polls).
     int ilu_order = KCFD_ORDER_NATURAL, ilu_built_order = -1, ncolors = 0;
     int ncolour0 = 0;   // rows of colour 0 (they come first in P A P^T)
-    int *perm = nullptr, *f_vmap = nullptr;
+    int *perm = nullptr, *iperm = nullptr, *f_vmap = nullptr;
     int *f_indptr = nullptr, *f_indices = nullptr, *f_diagpos = nullptr, *f_uix = nullptr;
     std::vector<int> h_perm, hf_indptr, hf_indices, hf_diagpos;
     // ILU(0): factors on the (permuted) pattern, level schedule of the solves
@@ -143,22 +167,22 @@ struct kcfd_mat {
     // ILU(0) solve layout.  The level lists are cut into chunks of up to 64
     // rows (never across a level); chunk c keeps the L (resp. U) entries of
     // its rows column-major: entry k of the row in lane l at off[c] + 64k + l,
-    // in CSR order, padded with column -1 / value 0.  L columns are rows of
-    // P A P^T, U columns rows of A.  h_ch*_ptr[l]: first chunk of level l;
+    // in CSR order, padded with column -1 / value 0.  Columns are rows of
+    // P A P^T (the solver's row order).  h_ch*_ptr[l]: first chunk of level l;
     // ch*_map: factor slot of each entry (-1: padding); ud: u_ii by row of
     // P A P^T.
     int nch_l = 0, nch_u = 0, nent_l = 0, nent_u = 0;
     // Two colours: the L layout holds the colour-1 rows (positions
     // ncolour0..n-1) and the U layout the colour-0 rows, as single runs of
-    // 64-row chunks; a colour-0 row's y is z itself, so the L rows read z
-    // through chl_ncol (columns as rows of A).
-    int *chl_ncol = nullptr;
+    // 64-row chunks; a colour-0 row's y is z itself.
     bool two_colour = false;
     std::vector<int> h_chl_ptr, h_chu_ptr;
     int *chl_off = nullptr, *chu_off = nullptr, *chl_col = nullptr, *chu_col = nullptr;
     int *chl_map = nullptr, *chu_map = nullptr;
     double *chl_v64 = nullptr, *chu_v64 = nullptr, *ud64 = nullptr;
-    float *chl_v32 = nullptr, *chu_v32 = nullptr, *ud32 = nullptr;
+    // MIXED: {column, FP32 value bits} per chunk entry (one 8-byte load)
+    int2 *chl_pk32 = nullptr, *chu_pk32 = nullptr;
+    float *ud32 = nullptr;
     // cached single-system solver workspace
     kcfd_batch *solver = nullptr;
     int solver_key[4] = {-1, -1, -1, -1};
@@ -166,7 +190,9 @@ struct kcfd_mat {
 
 namespace kcfd {
 int ensure_scratch(kcfd_ctx *ctx, size_t bytes);
+int ilu0_structure(kcfd_mat *m);
 int ilu0_setup(kcfd_mat *m, bool want32);
+int ensure_sell(kcfd_mat *m, bool permuted);
 void ilu_free_order(kcfd_mat *m);
 int mat_free_precond(kcfd_mat *m);
 int mat_free_ilu_structure(kcfd_mat *m);

@@ -139,13 +139,18 @@ static int color_rows(const std::vector<int> &ip, const std::vector<int> &ix, st
 void ilu_free_order(kcfd_mat *m) {
     if (m->perm) {
         hipFree(m->perm);
+        hipFree(m->iperm);
         hipFree(m->f_vmap);
         hipFree(m->f_indptr);
         hipFree(m->f_indices);
         hipFree(m->f_diagpos);
         hipFree(m->f_uix);
     }
-    m->perm = m->f_vmap = m->f_indptr = m->f_indices = m->f_diagpos = m->f_uix = nullptr;
+    m->perm = m->iperm = m->f_vmap = m->f_indptr = m->f_indices = m->f_diagpos = m->f_uix = nullptr;
+    if (m->sell_permuted) {   // the SELL copy was in the dropped order: rebuilt at the next set-up
+        hipFree(m->six);
+        m->six = nullptr;
+    }
     m->h_perm.clear();
     m->hf_indptr.clear();
     m->hf_indices.clear();
@@ -202,6 +207,7 @@ static int build_order(kcfd_mat *m) {
         }
     }
     KCFD_HIP(hipMalloc(&m->perm, sizeof(int) * n));
+    KCFD_HIP(hipMalloc(&m->iperm, sizeof(int) * n));
     KCFD_HIP(hipMalloc(&m->f_vmap, sizeof(int) * std::max(nnz, 1)));
     KCFD_HIP(hipMalloc(&m->f_indptr, sizeof(int) * (n + 1)));
     KCFD_HIP(hipMalloc(&m->f_indices, sizeof(int) * std::max(nnz, 1)));
@@ -212,6 +218,7 @@ static int build_order(kcfd_mat *m) {
     // copies finish before return
     hipStream_t s = m->ctx->stream;
     KCFD_HIP(hipMemcpyAsync(m->perm, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    KCFD_HIP(hipMemcpyAsync(m->iperm, iperm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMemcpyAsync(m->f_indptr, fp.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMemcpyAsync(m->f_diagpos, fd.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     if (nnz) {
@@ -229,14 +236,13 @@ static int build_order(kcfd_mat *m) {
 // to 64 consecutive list entries.
 static void chunk_layout(const kcfd_mat *m, const std::vector<int> &rows, const std::vector<int> &lvl,
                          bool upper, std::vector<int> &ch_ptr, std::vector<int> &off,
-                         std::vector<int> &col, std::vector<int> &map, std::vector<int> &ncol) {
+                         std::vector<int> &col, std::vector<int> &map) {
     const std::vector<int> &ip = m->hf_indptr, &ix = m->hf_indices, &dp = m->hf_diagpos;
     const int L = (int)lvl.size() - 1;
     ch_ptr.assign(1, 0);
     off.assign(1, 0);
     col.clear();
     map.clear();
-    ncol.clear();
     for (int l = 0; l < L; ++l) {
         for (int s0 = lvl[l]; s0 < lvl[l + 1]; s0 += 64) {
             const int cnt = std::min(64, lvl[l + 1] - s0);
@@ -248,17 +254,12 @@ static void chunk_layout(const kcfd_mat *m, const std::vector<int> &rows, const 
             const size_t base = col.size();
             col.resize(base + (size_t)64 * w, -1);
             map.resize(base + (size_t)64 * w, -1);
-            ncol.resize(base + (size_t)64 * w, -1);
             for (int t = 0; t < cnt; ++t) {
                 const int i = rows[s0 + t];
                 const int q0 = upper ? dp[i] + 1 : ip[i], q1 = upper ? ip[i + 1] : dp[i];
                 for (int q = q0; q < q1; ++q) {
                     const size_t e = base + (size_t)64 * (q - q0) + t;
-                    // U columns as rows of A: what the U solve (writing in A's
-                    // numbering) polls
-                    const int nat = m->h_perm.empty() ? ix[q] : m->h_perm[ix[q]];
-                    col[e] = upper ? nat : ix[q];
-                    ncol[e] = nat;
+                    col[e] = ix[q];
                     map[e] = q;
                 }
             }
@@ -274,17 +275,23 @@ static int upload_ints(const std::vector<int> &h, int **d, hipStream_t s) {
     return KCFD_OK;
 }
 
-__global__ void k_chunk_vals(size_t ne, const int *__restrict__ map, const double *__restrict__ lu,
-                             double *__restrict__ v64, float *__restrict__ v32) {
+// chunk-layout values: FP64, and (MIXED) the packed {column, FP32 value}
+// entries; with col null the FP32 copy goes to v32 (u_ii by row)
+__global__ void k_chunk_vals(size_t ne, const int *__restrict__ map, const int *__restrict__ col,
+                             const double *__restrict__ lu, double *__restrict__ v64,
+                             int2 *__restrict__ pk32, float *__restrict__ v32) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= ne) return;
     const int q = map[e];
     const double v = q >= 0 ? lu[q] : 0.0;
     v64[e] = v;
+    if (pk32) pk32[e] = make_int2(col[e], __float_as_int((float)v));
     if (v32) v32[e] = (float)v;
 }
 
-int ilu0_setup(kcfd_mat *m, bool want32) {
+// Ordering, level schedule and chunked solve layout: they depend on the
+// structure only and are built once per ordering.
+int ilu0_structure(kcfd_mat *m) {
     const int n = m->n;
     hipStream_t s = m->ctx->stream;
     for (int i = 0; i < n; ++i) {
@@ -293,8 +300,6 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
             return KCFD_EPIVOT;
         }
     }
-    // ordering and level schedule depend on the structure only: built once
-    // per ordering
     if (!m->lvl_rows_l || m->ilu_built_order != m->ilu_order) {
         mat_free_ilu_structure(m);
         KCFD_CHECK(build_order(m));
@@ -323,16 +328,15 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
         const std::vector<int> lvl_l = m->two_colour ? std::vector<int>{0, n - m->ncolour0} : m->h_lvl_ptr_l;
         const std::vector<int> lvl_u = m->two_colour ? std::vector<int>{0, m->ncolour0} : m->h_lvl_ptr_u;
         {
-            std::vector<int> off, col, map, ncol;
-            chunk_layout(m, rl, lvl_l, false, m->h_chl_ptr, off, col, map, ncol);
+            std::vector<int> off, col, map;
+            chunk_layout(m, rl, lvl_l, false, m->h_chl_ptr, off, col, map);
             m->nch_l = (int)off.size() - 1;
             m->nent_l = (int)col.size();
             KCFD_CHECK(upload_ints(off, &m->chl_off, s));
             KCFD_CHECK(upload_ints(col, &m->chl_col, s));
             KCFD_CHECK(upload_ints(map, &m->chl_map, s));
-            KCFD_CHECK(upload_ints(ncol, &m->chl_ncol, s));
             KCFD_HIP(hipStreamSynchronize(s));
-            chunk_layout(m, ru, lvl_u, true, m->h_chu_ptr, off, col, map, ncol);
+            chunk_layout(m, ru, lvl_u, true, m->h_chu_ptr, off, col, map);
             m->nch_u = (int)off.size() - 1;
             m->nent_u = (int)col.size();
             KCFD_CHECK(upload_ints(off, &m->chu_off, s));
@@ -341,6 +345,15 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
             KCFD_HIP(hipStreamSynchronize(s));   // the host vectors are locals
         }
     }
+    return KCFD_OK;
+}
+
+// Factor values (after ilu0_structure): ILU(0) of P A P^T level by level,
+// FP32 copies, the chunked solve layout's values and a pivot check.
+int ilu0_setup(kcfd_mat *m, bool want32) {
+    const int n = m->n;
+    hipStream_t s = m->ctx->stream;
+    KCFD_CHECK(ilu0_structure(m));
     const std::vector<int> &pl = m->h_lvl_ptr_l;
     if (!m->lu) KCFD_HIP(hipMalloc(&m->lu, sizeof(double) * std::max(m->nnz, 1)));
     if (m->nnz && !m->perm)
@@ -370,21 +383,21 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
             KCFD_HIP(hipMalloc(&m->chu_v64, sizeof(double) * std::max<size_t>(nu, 1)));
             KCFD_HIP(hipMalloc(&m->ud64, sizeof(double) * std::max(n, 1)));
         }
-        if (want32 && !m->chl_v32) {
-            KCFD_HIP(hipMalloc(&m->chl_v32, sizeof(float) * std::max<size_t>(nl, 1)));
-            KCFD_HIP(hipMalloc(&m->chu_v32, sizeof(float) * std::max<size_t>(nu, 1)));
+        if (want32 && !m->chl_pk32) {
+            KCFD_HIP(hipMalloc(&m->chl_pk32, sizeof(int2) * std::max<size_t>(nl, 1)));
+            KCFD_HIP(hipMalloc(&m->chu_pk32, sizeof(int2) * std::max<size_t>(nu, 1)));
             KCFD_HIP(hipMalloc(&m->ud32, sizeof(float) * std::max(n, 1)));
         }
-        float *l32 = want32 ? m->chl_v32 : nullptr, *u32 = want32 ? m->chu_v32 : nullptr;
+        int2 *l32 = want32 ? m->chl_pk32 : nullptr, *u32 = want32 ? m->chu_pk32 : nullptr;
         if (nl)
             hipLaunchKernelGGL(k_chunk_vals, dim3((unsigned)((nl + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                               nl, m->chl_map, m->lu, m->chl_v64, l32);
+                               nl, m->chl_map, m->chl_col, m->lu, m->chl_v64, l32, nullptr);
         if (nu)
             hipLaunchKernelGGL(k_chunk_vals, dim3((unsigned)((nu + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                               nu, m->chu_map, m->lu, m->chu_v64, u32);
+                               nu, m->chu_map, m->chu_col, m->lu, m->chu_v64, u32, nullptr);
         if (n)
             hipLaunchKernelGGL(k_chunk_vals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (size_t)n,
-                               m->f_diagpos, m->lu, m->ud64, want32 ? m->ud32 : nullptr);
+                               m->f_diagpos, nullptr, m->lu, m->ud64, nullptr, want32 ? m->ud32 : nullptr);
         KCFD_HIP(hipGetLastError());
     }
     // pivot check: first row with a zero or NaN u_ii (or n if none)
@@ -538,6 +551,7 @@ extern "C" int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src) {
     const size_t n = (size_t)dst->n, nnz = (size_t)dst->nnz;
     if (src->perm) {
         KCFD_CHECK(dup_ints(src->perm, n, &dst->perm, dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->iperm, n, &dst->iperm, dst->ctx->stream));
         KCFD_CHECK(dup_ints(src->f_vmap, nnz, &dst->f_vmap, dst->ctx->stream));
         KCFD_CHECK(dup_ints(src->f_indptr, n + 1, &dst->f_indptr, dst->ctx->stream));
         KCFD_CHECK(dup_ints(src->f_indices, nnz, &dst->f_indices, dst->ctx->stream));
@@ -557,7 +571,6 @@ extern "C" int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src) {
     KCFD_CHECK(dup_ints(src->chl_col, (size_t)src->nent_l, &dst->chl_col, dst->ctx->stream));
     KCFD_CHECK(dup_ints(src->chu_col, (size_t)src->nent_u, &dst->chu_col, dst->ctx->stream));
     KCFD_CHECK(dup_ints(src->chl_map, (size_t)src->nent_l, &dst->chl_map, dst->ctx->stream));
-    KCFD_CHECK(dup_ints(src->chl_ncol, (size_t)src->nent_l, &dst->chl_ncol, dst->ctx->stream));
     dst->two_colour = src->two_colour;
     dst->ncolour0 = src->ncolour0;
     KCFD_CHECK(dup_ints(src->chu_map, (size_t)src->nent_u, &dst->chu_map, dst->ctx->stream));

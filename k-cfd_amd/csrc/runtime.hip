@@ -70,36 +70,51 @@ __global__ void k_jacobi(int n, const int *__restrict__ dp, const double *__rest
     if (dinv32) dinv32[i] = (float)r;
 }
 
-// SELL-64 column indices (structure) and values of one row per thread
+// SELL-64 column indices (structure) and values of one row per thread.
+// Row k of the SELL is row rperm[k] of A (rperm null: k) with A's entries in
+// A's CSR order and columns renumbered by icol (null: unchanged); the
+// solver runs in that row order (P A P^T for a permuted ILU(0)), and every
+// row sum keeps the sequential CSR order.
 __global__ void k_sell_struct(int n, const int *__restrict__ ip, const int *__restrict__ ix,
-                              const int *__restrict__ sptr, int *__restrict__ six) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int s = r >> 6, base = sptr[s] + (r & 63), w = (sptr[s + 1] - sptr[s]) >> 6;
+                              const int *__restrict__ sptr, int *__restrict__ six,
+                              const int *__restrict__ rperm, const int *__restrict__ icol) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int r = rperm ? rperm[k] : k;
+    const int s = k >> 6, base = sptr[s] + (k & 63), w = (sptr[s + 1] - sptr[s]) >> 6;
     const int q0 = ip[r], len = ip[r + 1] - q0;
-    for (int k = 0; k < w; ++k) six[base + 64 * k] = k < len ? ix[q0 + k] : r;
+    for (int j = 0; j < w; ++j) {
+        const int c = j < len ? ix[q0 + j] : -1;
+        six[base + 64 * j] = c < 0 ? k : (icol ? icol[c] : c);
+    }
 }
 
-// SELL values.  The FP32 copy is kept in difference form for the mixed
-// solver: the diagonal position holds the row sum sum_j a_ij (summed in
-// FP64), so that (A z)_i = sum_{j!=i} a_ij (z_j - z_i) + rowsum_i z_i.
-// Rounding the off-diagonals to FP32 then leaves the row sums (and the
-// near-null space of a diffusion operator) intact, where rounding a_ii
-// separately would perturb the smallest eigenvalues by eps_32 |a_ii|.
+// SELL values (row order as k_sell_struct).  The FP32 copy is kept in
+// difference form for the mixed solver: the diagonal position holds the row
+// sum sum_j a_ij (summed in FP64), so that
+// (A z)_i = sum_{j!=i} a_ij (z_j - z_i) + rowsum_i z_i.  Rounding the
+// off-diagonals to FP32 then leaves the row sums (and the near-null space of
+// a diffusion operator) intact, where rounding a_ii separately would perturb
+// the smallest eigenvalues by eps_32 |a_ii|.
 __global__ void k_sell_values(int n, const int *__restrict__ ip, const int *__restrict__ ix,
-                              const int *__restrict__ sptr, const double *__restrict__ a,
-                              double *__restrict__ sa64, float *__restrict__ sa32) {
+                              const int *__restrict__ sptr, const int *__restrict__ six,
+                              const double *__restrict__ a, double *__restrict__ sa64,
+                              int2 *__restrict__ spk32, const int *__restrict__ rperm) {
 #pragma clang fp contract(off)
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const int s = r >> 6, base = sptr[s] + (r & 63), w = (sptr[s + 1] - sptr[s]) >> 6;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int r = rperm ? rperm[k] : k;
+    const int s = k >> 6, base = sptr[s] + (k & 63), w = (sptr[s + 1] - sptr[s]) >> 6;
     const int q0 = ip[r], len = ip[r + 1] - q0;
     double rowsum = 0.0;
-    for (int k = 0; k < len; ++k) rowsum += a[q0 + k];
-    for (int k = 0; k < w; ++k) {
-        const double v = k < len ? a[q0 + k] : 0.0;
-        sa64[base + 64 * k] = v;
-        if (sa32) sa32[base + 64 * k] = (float)(k < len && ix[q0 + k] == r ? rowsum : v);
+    for (int j = 0; j < len; ++j) rowsum += a[q0 + j];
+    for (int j = 0; j < w; ++j) {
+        const double v = j < len ? a[q0 + j] : 0.0;
+        sa64[base + 64 * j] = v;
+        if (spk32) {
+            const float v32 = (float)(j < len && ix[q0 + j] == r ? rowsum : v);
+            spk32[base + 64 * j] = make_int2(six[base + 64 * j], __float_as_int(v32));
+        }
     }
 }
 
@@ -118,10 +133,64 @@ __global__ __launch_bounds__(kBlock) void k_stream_copy(size_t n16, const double
     for (; i < n16; i += stride) dst[i] = src[i];
 }
 
+// SELL-64 structure of A in the solver's row order: natural, or P A P^T
+// (rows and columns renumbered by m->perm) when `permuted`.  Rebuilt only
+// when the order changes; a rebuild frees the value buffers (and any cached
+// solver, which points at them).
+int ensure_sell(kcfd_mat *m, bool permuted) {
+    if (m->six && m->sell_permuted == permuted) return KCFD_OK;
+    const int n = m->n;
+    hipStream_t st = m->ctx->stream;
+    KCFD_HIP(hipStreamSynchronize(st));
+    if (m->solver) {
+        kcfd_batch_destroy(m->solver);
+        m->solver = nullptr;
+    }
+    hipFree(m->slice_ptr); m->slice_ptr = nullptr;
+    hipFree(m->six); m->six = nullptr;
+    hipFree(m->sa64); m->sa64 = nullptr;
+    hipFree(m->spk32); m->spk32 = nullptr;
+    const std::vector<int> &ip = m->h_indptr;
+    m->nslices = (n + 63) / 64;
+    m->h_slice_ptr.assign(m->nslices + 1, 0);
+    long long acc = 0;
+    for (int sl = 0; sl < m->nslices; ++sl) {
+        int w = 0;
+        for (int k = sl * 64; k < std::min(n, sl * 64 + 64); ++k) {
+            const int r = permuted ? m->h_perm[k] : k;
+            w = std::max(w, ip[r + 1] - ip[r]);
+        }
+        acc += 64LL * w;
+        if (acc >= (1LL << 31)) {
+            set_error("matrix too large for 32-bit SELL offsets");
+            return KCFD_EARG;
+        }
+        m->h_slice_ptr[sl + 1] = (int)acc;
+    }
+    m->sell_nnz = (int)acc;
+    if (hipMalloc(&m->slice_ptr, sizeof(int) * (m->nslices + 1)) != hipSuccess ||
+        hipMalloc(&m->six, sizeof(int) * std::max(m->sell_nnz, 1)) != hipSuccess ||
+        hipMalloc(&m->sa64, sizeof(double) * std::max(m->sell_nnz, 1)) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipMalloc failed for the SELL copy (n=%d, padded nnz=%d)", n, m->sell_nnz);
+        return KCFD_EOOM;
+    }
+    KCFD_HIP(hipMemcpyAsync(m->slice_ptr, m->h_slice_ptr.data(), sizeof(int) * (m->nslices + 1),
+                            hipMemcpyHostToDevice, st));
+    if (n)
+        hipLaunchKernelGGL(k_sell_struct, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, n,
+                           m->indptr, m->indices, m->slice_ptr, m->six, permuted ? m->perm : nullptr,
+                           permuted ? m->iperm : nullptr);
+    KCFD_HIP(hipGetLastError());
+    KCFD_HIP(hipStreamSynchronize(st));   // h_slice_ptr is read by the copy
+    m->sell_permuted = permuted;
+    return KCFD_OK;
+}
+
 // Value buffers of the preconditioner.  The ILU(0) ordering and level
 // schedule depend on the structure only and survive (mat_free_ilu_structure).
 int mat_free_precond(kcfd_mat *m) {
-    hipFree(m->sa32); m->sa32 = nullptr;
+    hipFree(m->spk32); m->spk32 = nullptr;
     hipFree(m->dinv); m->dinv = nullptr;
     hipFree(m->dinv32); m->dinv32 = nullptr;
     hipFree(m->lu); m->lu = nullptr;
@@ -129,8 +198,8 @@ int mat_free_precond(kcfd_mat *m) {
     hipFree(m->chl_v64); m->chl_v64 = nullptr;
     hipFree(m->chu_v64); m->chu_v64 = nullptr;
     hipFree(m->ud64); m->ud64 = nullptr;
-    hipFree(m->chl_v32); m->chl_v32 = nullptr;
-    hipFree(m->chu_v32); m->chu_v32 = nullptr;
+    hipFree(m->chl_pk32); m->chl_pk32 = nullptr;
+    hipFree(m->chu_pk32); m->chu_pk32 = nullptr;
     hipFree(m->ud32); m->ud32 = nullptr;
     m->have32 = false;
     m->pc_kind = KCFD_PC_NONE;
@@ -140,8 +209,7 @@ int mat_free_precond(kcfd_mat *m) {
 int mat_free_ilu_structure(kcfd_mat *m) {
     hipFree(m->lvl_rows_l); m->lvl_rows_l = nullptr;
     hipFree(m->lvl_rows_u); m->lvl_rows_u = nullptr;
-    int **arrs[] = {&m->chl_off, &m->chu_off, &m->chl_col, &m->chu_col, &m->chl_map, &m->chu_map,
-                    &m->chl_ncol};
+    int **arrs[] = {&m->chl_off, &m->chu_off, &m->chl_col, &m->chu_col, &m->chl_map, &m->chu_map};
     m->two_colour = false;
     for (int **a : arrs) {
         hipFree(*a);
@@ -155,11 +223,9 @@ int mat_free_ilu_structure(kcfd_mat *m) {
         hipFree(*a);
         *a = nullptr;
     }
-    float **v32[] = {&m->chl_v32, &m->chu_v32, &m->ud32};
-    for (float **a : v32) {
-        hipFree(*a);
-        *a = nullptr;
-    }
+    hipFree(m->chl_pk32); m->chl_pk32 = nullptr;
+    hipFree(m->chu_pk32); m->chu_pk32 = nullptr;
+    hipFree(m->ud32); m->ud32 = nullptr;
     ilu_free_order(m);
     m->h_lvl_ptr_l.clear();
     m->h_lvl_ptr_u.clear();
@@ -344,20 +410,6 @@ int kcfd_mat_create(kcfd_ctx *ctx, int n, int nnz, const int *indptr, const int 
     for (int i = 0; i < n; ++i)
         for (int q = indptr[i]; q < indptr[i + 1]; ++q)
             if (indices[q] == i) m->h_diagpos[i] = q;
-    // SELL-64 slice offsets (in entries, padded)
-    m->nslices = (n + 63) / 64;
-    m->h_slice_ptr.assign(m->nslices + 1, 0);
-    {
-        long long acc = 0;
-        for (int sl = 0; sl < m->nslices; ++sl) {
-            int w = 0;
-            for (int r = sl * 64; r < std::min(n, sl * 64 + 64); ++r) w = std::max(w, indptr[r + 1] - indptr[r]);
-            acc += 64LL * w;
-            KCFD_ARG(acc < (1LL << 31), "matrix too large for 32-bit SELL offsets");
-            m->h_slice_ptr[sl + 1] = (int)acc;
-        }
-        m->sell_nnz = (int)acc;
-    }
     auto fail = [&](int rc) { kcfd_mat_destroy(m); return rc; };
     if (hipMalloc(&m->indptr, sizeof(int) * (n + 1)) != hipSuccess ||
         hipMalloc(&m->indices, sizeof(int) * (nnz ? nnz : 1)) != hipSuccess ||
@@ -380,24 +432,9 @@ int kcfd_mat_create(kcfd_ctx *ctx, int n, int nnz, const int *indptr, const int 
         set_error("matrix upload failed");
         return fail(KCFD_EHIP);
     }
-    if (hipMalloc(&m->slice_ptr, sizeof(int) * (m->nslices + 1)) != hipSuccess ||
-        hipMalloc(&m->six, sizeof(int) * std::max(m->sell_nnz, 1)) != hipSuccess ||
-        hipMalloc(&m->sa64, sizeof(double) * std::max(m->sell_nnz, 1)) != hipSuccess) {
-        set_error("hipMalloc failed for the SELL copy (n=%d, padded nnz=%d)", n, m->sell_nnz);
-        return fail(KCFD_EOOM);
-    }
-    if (hipMemcpyAsync(m->slice_ptr, m->h_slice_ptr.data(), sizeof(int) * (m->nslices + 1),
-                       hipMemcpyHostToDevice, st) != hipSuccess) {
-        set_error("SELL upload failed");
-        return fail(KCFD_EHIP);
-    }
-    if (n)
-        hipLaunchKernelGGL(k_sell_struct, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, ctx->stream,
-                           n, m->indptr, m->indices, m->slice_ptr, m->six);
-    // the caller's index arrays are read by the copies above
-    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        set_error("SELL structure build failed");
-        return fail(KCFD_EHIP);
+    {
+        const int rc = ensure_sell(m, false);   // waits for the uploads above
+        if (rc != KCFD_OK) return fail(rc);
     }
     *out = m;
     return KCFD_OK;
@@ -467,7 +504,7 @@ int kcfd_mat_bytes(kcfd_mat *m, long long *bytes) {
     KCFD_ARG(m && bytes, "null argument");
     long long b = 4LL * (m->n + 1) + 4LL * m->nnz + 4LL * m->n + 8LL * m->nnz + 16LL * m->n;
     b += 4LL * (m->nslices + 1) + 12LL * m->sell_nnz;
-    if (m->sa32) b += 4LL * m->sell_nnz;
+    if (m->spk32) b += 8LL * m->sell_nnz;
     if (m->dinv) b += 8LL * m->n;
     if (m->dinv32) b += 4LL * m->n;
     if (m->lu) b += 8LL * m->nnz + 8LL * m->n;
@@ -499,15 +536,20 @@ int kcfd_precond_setup(kcfd_mat *m, int kind, int precision) {
             m->solver = nullptr;
         }
         mat_free_precond(m);
-        if (want32) KCFD_HIP(hipMalloc(&m->sa32, sizeof(float) * std::max(m->sell_nnz, 1)));
-        if (kind == KCFD_PC_JACOBI) {
-            KCFD_HIP(hipMalloc(&m->dinv, sizeof(double) * (n ? n : 1)));
-            if (want32) KCFD_HIP(hipMalloc(&m->dinv32, sizeof(float) * (n ? n : 1)));
-        }
+    }
+    // the solver iterates in the ILU(0) ordering (P A P^T) when there is one:
+    // ordering first, then the SELL copy in that row order
+    if (kind == KCFD_PC_ILU0) KCFD_CHECK(ilu0_structure(m));
+    KCFD_CHECK(ensure_sell(m, kind == KCFD_PC_ILU0 && m->perm != nullptr));
+    if (want32 && !m->spk32) KCFD_HIP(hipMalloc(&m->spk32, sizeof(int2) * std::max(m->sell_nnz, 1)));
+    if (kind == KCFD_PC_JACOBI && !m->dinv) {
+        KCFD_HIP(hipMalloc(&m->dinv, sizeof(double) * (n ? n : 1)));
+        if (want32) KCFD_HIP(hipMalloc(&m->dinv32, sizeof(float) * (n ? n : 1)));
     }
     if (n) {
         hipLaunchKernelGGL(k_sell_values, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
-                           m->indptr, m->indices, m->slice_ptr, m->val, m->sa64, m->sa32);
+                           m->indptr, m->indices, m->slice_ptr, m->six, m->val, m->sa64,
+                           want32 ? m->spk32 : nullptr, m->sell_permuted ? m->perm : nullptr);
         KCFD_HIP(hipGetLastError());
     }
     if (kind == KCFD_PC_JACOBI && n) {

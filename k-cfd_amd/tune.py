@@ -5,6 +5,8 @@ The knobs are those of kcfd_solve_opts:
 * ``grid_per_cu``: persistent workgroups (256 lanes, 4 wavefronts) per CU
   of the SpMV and vector kernels -- the block count of the tile walk;
 * ``trsv_per_cu``: workgroups per CU of the ILU(0) triangular solves;
+* ``tile_walk``: 1 interleaved tiles per workgroup, 2 one contiguous run of
+  tiles per workgroup (inside its XCD's range either way);
 * ``check_every``: iterations per captured hipGraph between host checks.
 
 Stage 1 times every launch shape on the batch itself, with the iteration
@@ -28,6 +30,7 @@ from . import device
 
 GRID_PER_CU = (2, 4, 8)
 TRSV_PER_CU = (2, 4)
+TILE_WALK = (1, 2)
 CHECK_EVERY = (8, 16, 32)
 
 
@@ -41,7 +44,7 @@ def signature(sb, precision: str, precond: str, order: str | None, ctx: L.Contex
     info = ctx.info()
     nnz = sum(m.nnz for m in sb.mats)
     return "|".join([
-        precision, precond, str(order if precond == "ilu0" else None),
+        "v2", precision, precond, str(order if precond == "ilu0" else None),
         f"nsys2^{int(round(math.log2(max(1, len(n)))))}",
         f"rows2^{int(round(math.log2(max(1, int(n.sum())))))}",
         f"nnzrow{int(round(nnz / max(1, int(n.sum()))))}",
@@ -87,7 +90,7 @@ def _time_batch(sb, opts, reps=2):
 
 def autotune(sb, precision="mixed", precond="jacobi", order="color", rtol=1e-8, delta=0.1,
              probe_iters=24, use_cache=True, log=None) -> dict:
-    """Best {grid_per_cu, trsv_per_cu, check_every} for this batch."""
+    """Best {grid_per_cu, trsv_per_cu, tile_walk, check_every} for this batch."""
     sb.setup(precond, precision, order)
     key = signature(sb, precision, precond, order, sb.ctx)
     db = _load() if use_cache else {}
@@ -97,26 +100,28 @@ def autotune(sb, precision="mixed", precond="jacobi", order="color", rtol=1e-8, 
         return dict(db[key]["best"])
     trsv = TRSV_PER_CU if precond == "ilu0" else (0,)
     trials = []
-    for g, t in itertools.product(GRID_PER_CU, trsv):
+    for g, t, w in itertools.product(GRID_PER_CU, trsv, TILE_WALK):
         o = device.default_opts(rtol=rtol, precond=precond, precision=precision, inner_rtol=delta,
-                                maxiter=probe_iters, check_every=8, grid_per_cu=g, trsv_per_cu=t)
+                                maxiter=probe_iters, check_every=8, grid_per_cu=g, trsv_per_cu=t,
+                                tile_walk=w)
         ms = _time_batch(sb, o)
-        trials.append(dict(grid_per_cu=g, trsv_per_cu=t, probe_ms=round(ms, 3)))
+        trials.append(dict(grid_per_cu=g, trsv_per_cu=t, tile_walk=w, probe_ms=round(ms, 3)))
         if log:
-            log(f"autotune: grid_per_cu={g} trsv_per_cu={t}: {ms:.2f} ms for <= {probe_iters} iterations")
+            log(f"autotune: grid_per_cu={g} trsv_per_cu={t} tile_walk={w}: {ms:.2f} ms "
+                f"for <= {probe_iters} iterations")
     shape = min(trials, key=lambda r: r["probe_ms"])
     best = None
     for ce in CHECK_EVERY:
         o = device.default_opts(rtol=rtol, precond=precond, precision=precision, inner_rtol=delta,
                                 check_every=ce, grid_per_cu=shape["grid_per_cu"],
-                                trsv_per_cu=shape["trsv_per_cu"])
+                                trsv_per_cu=shape["trsv_per_cu"], tile_walk=shape["tile_walk"])
         ms = _time_batch(sb, o, reps=1)
         if log:
             log(f"autotune: check_every={ce}: {ms:.2f} ms (full solve)")
         if best is None or ms < best[1]:
             best = (ce, ms)
     choice = dict(grid_per_cu=shape["grid_per_cu"], trsv_per_cu=shape["trsv_per_cu"],
-                  check_every=best[0])
+                  tile_walk=shape["tile_walk"], check_every=best[0])
     if use_cache:
         db = _load()
         db[key] = dict(best=choice, trials=trials, full_solve_ms=round(best[1], 3))
@@ -129,4 +134,4 @@ def describe(choice: dict) -> str:
 
 
 __all__ = ["autotune", "signature", "cache_path", "describe", "GRID_PER_CU", "TRSV_PER_CU",
-           "CHECK_EVERY"]
+           "TILE_WALK", "CHECK_EVERY"]
