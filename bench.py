@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--tile-walk", type=int, default=0,
                     help="launch shape: 1 interleaved, 2 contiguous tiles per workgroup (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-single", action="store_true",
+                    help="batched: skip the FP64 one-system-at-a-time GPU baseline")
     ap.add_argument("--check-every", type=int, default=8,
                     help="iterations per captured graph between host status checks")
     ap.add_argument("--precisions", default="fp64,mixed",
@@ -333,6 +335,8 @@ def bench_batched(args):
         sb.solve(batch=bobj, **opts)
         sols[prec] = sb.solutions()
         bobj.close()
+        if prec == "fp64" and not args.no_single:
+            summ["one_at_a_time"] = single_fp64(sb, args, choice, log)
         results[prec] = summ
         log(prec, summ)
     mixed = results.get("mixed") or next(iter(results.values()))
@@ -349,6 +353,7 @@ def bench_batched(args):
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_batch(sb, specs, args.precond, args.cpu_budget)
+    single = fp64.get("one_at_a_time")
     line = {
         "metric": "solves/s + %HBM roofline at rel-resid 1e-8; p50/p99 latency, mixed vs FP64",
         "value": mixed["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
@@ -361,9 +366,40 @@ def bench_batched(args):
                    "ilu_order": order, "step": "one batched solve of every system"},
         "roofline": roof, "cpu_baseline": cpu, "mixed": mixed, "fp64": fp64,
         "mixed_vs_fp64": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3) if len(results) > 1 else None,
+        "mixed_vs_fp64_one_at_a_time": round(mixed["solves_per_s"] / single["solves_per_s"], 3)
+        if single else None,
         "mixed_error_vs_fp64": err, "device": info["name"],
     }
     print(json.dumps(line))
+
+
+def single_fp64(sb, args, choice, log):
+    """The plain FP64 GPU path: every system of the batch solved on its own
+    (kcfd_solve, one system per launch), in order, from x0 = 0.  A first pass
+    builds each matrix's single-system workspace and graph; the second pass
+    is timed."""
+    from kcfd_amd import device
+    opts = device.default_opts(rtol=1e-8, precond=args.precond, precision="fp64",
+                               check_every=choice["check_every"], grid_per_cu=choice["grid_per_cu"],
+                               trsv_per_cu=choice["trsv_per_cu"], tile_walk=choice.get("tile_walk", 0))
+    walls, iters = [], []
+    for rep in range(2):
+        walls.clear()
+        iters.clear()
+        sb.ctx.sync()
+        t0 = time.perf_counter()
+        for m, b, x in zip(sb.mats, sb.bs, sb.xs):
+            ts = time.perf_counter()
+            x.fill(0.0)
+            r = m.solve_dev(b.p, x.p, opts, setup=False)
+            walls.append(time.perf_counter() - ts)
+            iters.append(r.iters)
+        wall = time.perf_counter() - t0
+        log(f"fp64 one at a time, pass {rep}: {wall:.2f} s")
+    return dict(solves_per_s=round(len(sb.mats) / wall, 2), wall_ms=round(1e3 * wall, 1),
+                p50_ms=round(1e3 * float(np.percentile(walls, 50)), 3),
+                p99_ms=round(1e3 * float(np.percentile(walls, 99)), 3),
+                iters_mean=round(float(np.mean(iters)), 1))
 
 
 def cpu_baseline_batch(sb, specs, precond, budget_s):

@@ -187,12 +187,17 @@ template <typename V> __device__ __forceinline__ V sentinel_value() {
 
 // preconditioned operand (M z)_j: Jacobi dinv_j z_j, ILU zh_j, none z_j
 template <typename V, int PC>
+__device__ __forceinline__ V precond_v(const SysDesc &d, const V *__restrict__ z,
+                                       const V *__restrict__ zh, int j) {
+#pragma clang fp contract(off)
+    if (PC == KCFD_PC_JACOBI) return gl(jac<V>(d))[j] * gl(z)[j];
+    if (PC == KCFD_PC_ILU0) return gl(zh)[j];
+    return gl(z)[j];
+}
+template <typename V, int PC>
 __device__ __forceinline__ double precond_at(const SysDesc &d, const V *__restrict__ z,
                                              const V *__restrict__ zh, int j) {
-#pragma clang fp contract(off)
-    if (PC == KCFD_PC_JACOBI) return (double)(gl(jac<V>(d))[j] * gl(z)[j]);
-    if (PC == KCFD_PC_ILU0) return (double)gl(zh)[j];
-    return (double)gl(z)[j];
+    return (double)precond_v<V, PC>(d, z, zh, j);
 }
 
 // (A M z)_i from the SELL-64 copy, entries in CSR order.  A wave's 64 rows
@@ -200,9 +205,9 @@ __device__ __forceinline__ double precond_at(const SysDesc &d, const V *__restri
 // of a group of entries, then its gathers, are all issued before the
 // (in-order) accumulation.  The common widths (5- and 7-point stencils, tet
 // rows) run as fixed-size groups without any guard; any other width goes in
-// groups of 8 whose entries past the width re-load the last entry (the same
+// groups of kRowUnroll whose entries past the width re-load the last entry (the same
 // cache line) and are not accumulated, so no load sits behind a branch.
-constexpr int kRowUnroll = 8;
+constexpr int kRowUnroll = 4;
 
 // entry e of the SELL copy: column and value (MIXED: one 8-byte load)
 template <typename V>
@@ -228,17 +233,16 @@ __device__ __forceinline__ void amz_group(const SysDesc &d, int i, const V *__re
 #pragma clang fp contract(off)
     constexpr bool kDiff = sizeof(V) == sizeof(float);
     int col[G];
-    V av[G];
-    double zv[G];
+    V av[G], zv[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) sell_entry<V>(d, base + 64 * min(k0 + k, last), col[k], av[k]);
 #pragma unroll
-    for (int k = 0; k < G; ++k) zv[k] = precond_at<V, PC>(d, z, zh, col[k]);
+    for (int k = 0; k < G; ++k) zv[k] = precond_v<V, PC>(d, z, zh, col[k]);
 #pragma unroll
     for (int k = 0; k < G; ++k)
         if (k < valid) {
-            if (kDiff) acc += (double)av[k] * (col[k] == i ? zi : zv[k] - zi);
-            else acc += (double)av[k] * zv[k];
+            if (kDiff) acc += (double)av[k] * (col[k] == i ? zi : (double)zv[k] - zi);
+            else acc += (double)av[k] * (double)zv[k];
         }
 }
 
@@ -293,6 +297,27 @@ __device__ __forceinline__ double row_ax64(const SysDesc &d, int i, const double
     }
     return acc;
 }
+
+// Tile kernels: a tile is kTile consecutive rows of one system, and thread t
+// of the workgroup takes rows t, t + kBlock, ... of it (kRows rows, each a
+// coalesced wave access).  Every load of a thread's rows is issued before
+// the first store, and the rows of the last tile past the system's end are
+// loaded at the last row and discarded.
+constexpr int kRows = 2;
+constexpr int kTile = kRows * kBlock;
+
+struct TileRows {
+    int ix[kRows];   // row index, clamped to n - 1
+    bool ok[kRows];  // a row of the system
+    __device__ __forceinline__ TileRows(int t0, int n) {
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            const int i = t0 + r * kBlock + (int)threadIdx.x;
+            ok[r] = i < n;
+            ix[r] = ok[r] ? i : n - 1;
+        }
+    }
+};
 
 template <int NV>
 __device__ __forceinline__ void write_partials(double (&v)[NV], double *__restrict__ partials, int tile,
@@ -351,17 +376,20 @@ __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
-        if (sts[ti.x].status != S_NEW) continue;
-        const int i = ti.y + threadIdx.x;
+        if (sts[ti.x].status != S_NEW || d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
         double v[2] = {0.0, 0.0};
-        if (i < d.n) {
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (!tr.ok[r]) continue;
+            const int i = tr.ix[r];
             const double bi = gl(d.b)[i];
-            v[0] = bi * bi;
+            v[0] += bi * bi;
             if (bp.precision == KCFD_FP64) {
                 const double ri = bi - row_ax64(d, i, d.x);
                 vec<double>(d.r)[i] = ri;
                 vec<double>(d.rt)[i] = ri;
-                v[1] = ri * ri;
+                v[1] += ri * ri;
             }
             if (PC == KCFD_PC_ILU0) {
                 vec<V>(d.y)[i] = sentinel_value<V>();
@@ -425,26 +453,57 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
         const int status = st.status;
-        const int i = ti.y + threadIdx.x;
-        if (i >= d.n) continue;
-        if (st.save) glw(d.xb)[i] = glw(d.x)[i];     // MIXED: the last update improved on the best
+        if (d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+        if (st.save) {                     // MIXED: the last update improved on the best
+            double xv[kRows];
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) xv[r] = glw(d.x)[tr.ix[r]];
+#pragma unroll
+            for (int r = 0; r < kRows; ++r)
+                if (tr.ok[r]) glw(d.xb)[tr.ix[r]] = xv[r];
+        }
         if (status == S_RUN || status == S_RESUME) {
             KCFD_GLOBAL V *p = vec<V>(d.p);
+            const KCFD_GLOBAL V *vv = vec<V>(d.v), *rr = vec<V>(d.r);
             const V om = (V)st.omega, be = (V)st.beta;
-            V pi = p[i] - om * vec<V>(d.v)[i];
-            pi = pi * be;
-            p[i] = pi + vec<V>(d.r)[i];
+            V pv[kRows], vl[kRows], rv[kRows];
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                pv[r] = p[tr.ix[r]];
+                vl[r] = vv[tr.ix[r]];
+                rv[r] = rr[tr.ix[r]];
+            }
+#pragma unroll
+            for (int r = 0; r < kRows; ++r)
+                if (tr.ok[r]) {
+                    V pi = pv[r] - om * vl[r];
+                    pi = pi * be;
+                    p[tr.ix[r]] = pi + rv[r];
+                }
         } else if (status == S_START) {
-            vec<V>(d.p)[i] = vec<V>(d.r)[i];
+            V rv[kRows];
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) rv[r] = vec<V>(d.r)[tr.ix[r]];
+#pragma unroll
+            for (int r = 0; r < kRows; ++r)
+                if (tr.ok[r]) vec<V>(d.p)[tr.ix[r]] = rv[r];
         } else if (status == S_REFINE) {
-            if (st.rollback) glw(d.x)[i] = glw(d.xb)[i];   // the inner iteration diverged
-            if (PC == KCFD_PC_ILU0 && d.polled) {
-                vec<V>(d.ph)[i] = sentinel_value<V>();
-                vec<V>(d.sh)[i] = sentinel_value<V>();
-                vec<V>(d.y)[i] = sentinel_value<V>();
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                if (!tr.ok[r]) continue;
+                const int i = tr.ix[r];
+                if (st.rollback) glw(d.x)[i] = glw(d.xb)[i];   // the inner iteration diverged
+                if (PC == KCFD_PC_ILU0 && d.polled) {
+                    vec<V>(d.ph)[i] = sentinel_value<V>();
+                    vec<V>(d.sh)[i] = sentinel_value<V>();
+                    vec<V>(d.y)[i] = sentinel_value<V>();
+                }
             }
         } else if (status == S_ZEROX) {
-            glw(d.x)[i] = 0.0;
+#pragma unroll
+            for (int r = 0; r < kRows; ++r)
+                if (tr.ok[r]) glw(d.x)[tr.ix[r]] = 0.0;
         }
     }
 }
@@ -453,7 +512,7 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
 // v = A M p; rt.v.   MIXED S_REFINE: r64 = b - A64 x; r = (V)r64 (and rt = r
 // on a restart); sums ||r64||^2, rt.r, ||r||^2.
 template <typename V, int PC>
-__global__ __launch_bounds__(kBlock) void k_spmv_p(const SysDesc *__restrict__ sys,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_spmv_p(const SysDesc *__restrict__ sys,
                                                    const SysState *__restrict__ sts,
                                                    const int2 *__restrict__ tiles, const int *__restrict__ counts,
                                                    double *__restrict__ partials) {
@@ -464,26 +523,44 @@ __global__ __launch_bounds__(kBlock) void k_spmv_p(const SysDesc *__restrict__ s
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
         const int status = st.status;
-        const int i = ti.y + threadIdx.x;
+        if (d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
         if (status == S_RUN || status == S_START || status == S_RESUME) {
+            // one row after the other (the store orders them): a row's loads
+            // already fill the wave's memory pipeline, and two rows in flight
+            // would cost occupancy
             double v[1] = {0.0};
-            if (i < d.n) {
-                const V vv = (V)row_amz<V, PC>(d, i, (const V *)d.p, (const V *)d.ph);
-                vec<V>(d.v)[i] = vv;
-                v[0] = (double)vec<V>(d.rt)[i] * (double)vv;
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                const double a = row_amz<V, PC>(d, tr.ix[r], (const V *)d.p, (const V *)d.ph);
+                const double rtv = (double)vec<V>(d.rt)[tr.ix[r]];
+                if (tr.ok[r]) {
+                    const V vv = (V)a;
+                    vec<V>(d.v)[tr.ix[r]] = vv;
+                    v[0] += rtv * (double)vv;
+                }
             }
             write_partials<1>(v, partials, tile);
         } else if (status == S_REFINE) {
+            double r64[kRows], rto[kRows];
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) r64[r] = gl(d.b)[tr.ix[r]] - row_ax64(d, tr.ix[r], d.x);
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) rto[r] = (double)vec<V>(d.rt)[tr.ix[r]];
             double v[3] = {0.0, 0.0, 0.0};
-            if (i < d.n) {
-                const double r64 = gl(d.b)[i] - row_ax64(d, i, d.x);
-                const V rv = (V)r64;
-                vec<V>(d.r)[i] = rv;
-                if (st.restart) vec<V>(d.rt)[i] = rv;
-                v[0] = r64 * r64;
-                v[1] = (double)vec<V>(d.rt)[i] * (double)rv;
-                v[2] = (double)rv * (double)rv;
-            }
+#pragma unroll
+            for (int r = 0; r < kRows; ++r)
+                if (tr.ok[r]) {
+                    const V rv = (V)r64[r];
+                    vec<V>(d.r)[tr.ix[r]] = rv;
+                    if (st.restart) {
+                        vec<V>(d.rt)[tr.ix[r]] = rv;
+                        rto[r] = (double)rv;
+                    }
+                    v[0] += r64[r] * r64[r];
+                    v[1] += rto[r] * (double)rv;
+                    v[2] += (double)rv * (double)rv;
+                }
             write_partials<3>(v, partials, tile);
         }
     }
@@ -582,15 +659,23 @@ __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
-        if (st.status != S_RUN) continue;
-        const int i = ti.y + threadIdx.x;
-        double v[1] = {0.0};
-        if (i < d.n) {
-            const V al = (V)st.alpha;
-            const V si = vec<V>(d.r)[i] - al * vec<V>(d.v)[i];
-            vec<V>(d.s)[i] = si;
-            v[0] = (double)si * (double)si;
+        if (st.status != S_RUN || d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+        const V al = (V)st.alpha;
+        V rv[kRows], vl[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            rv[r] = vec<V>(d.r)[tr.ix[r]];
+            vl[r] = vec<V>(d.v)[tr.ix[r]];
         }
+        double v[1] = {0.0};
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+            if (tr.ok[r]) {
+                const V si = rv[r] - al * vl[r];
+                vec<V>(d.s)[tr.ix[r]] = si;
+                v[0] += (double)si * (double)si;
+            }
         write_partials<1>(v, partials, tile, 2);   // ||s||^2 beside K4's t.s, t.t
     }
 }
@@ -598,29 +683,31 @@ __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
 // -------------------------------------------------------------------- K4
 // t = A M s; t.s, t.t.
 template <typename V, int PC>
-__global__ __launch_bounds__(kBlock) void k_spmv_s(const SysDesc *__restrict__ sys,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_spmv_s(const SysDesc *__restrict__ sys,
                                                    const SysState *__restrict__ sts,
                                                    const int2 *__restrict__ tiles, const int *__restrict__ counts,
                                                    double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
 #pragma clang fp contract(off)
-    const BatchParams bp = *bpp;
     const int ntiles = counts[0];   // tiles of the current membership
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
-        const int status = st.status;
-        const int i = ti.y + threadIdx.x;
-        if (status == S_RUN) {
-            double v[2] = {0.0, 0.0};
-            if (i < d.n) {
-                const V tv = (V)row_amz<V, PC>(d, i, (const V *)d.s, (const V *)d.sh);
-                vec<V>(d.t)[i] = tv;
-                v[0] = (double)tv * (double)vec<V>(d.s)[i];
-                v[1] = (double)tv * (double)tv;
+        if (st.status != S_RUN || d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+        double v[2] = {0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {   // one row after the other, as in K2
+            const double a = row_amz<V, PC>(d, tr.ix[r], (const V *)d.s, (const V *)d.sh);
+            const double sv = (double)vec<V>(d.s)[tr.ix[r]];
+            if (tr.ok[r]) {
+                const V tv = (V)a;
+                vec<V>(d.t)[tr.ix[r]] = tv;
+                v[0] += (double)tv * sv;
+                v[1] += (double)tv * (double)tv;
             }
-            write_partials<2>(v, partials, tile);
         }
+        write_partials<2>(v, partials, tile);
     }
 }
 
@@ -657,9 +744,13 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
-        const int i = ti.y + threadIdx.x;
+        if (d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
         if (st.status == S_HALF) {                   // half-step exit: x += alpha M p
-            if (i < d.n) {
+#pragma unroll
+            for (int r = 0; r < kRows; ++r) {
+                if (!tr.ok[r]) continue;
+                const int i = tr.ix[r];
                 const V al = (V)st.alpha;
                 const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
                 if (bp.precision == KCFD_FP64)       // scipy's operation, in FP64
@@ -674,26 +765,38 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
             continue;
         }
         if (st.status != S_RUN) continue;
+        const V al = (V)st.alpha, om = (V)st.omega;
+        V ph[kRows], sh[kRows], sv[kRows], tv[kRows], rtv[kRows];
+        double xv[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            const int i = tr.ix[r];
+            ph[r] = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
+            sh[r] = (V)precond_at<V, PC>(d, (const V *)d.s, (const V *)d.sh, i);
+            sv[r] = vec<V>(d.s)[i];
+            tv[r] = vec<V>(d.t)[i];
+            rtv[r] = vec<V>(d.rt)[i];
+            xv[r] = glw(d.x)[i];
+        }
         double v[2] = {0.0, 0.0};
-        if (i < d.n) {
-            const V al = (V)st.alpha, om = (V)st.omega;
-            const V ph = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
-            const V sh = (V)precond_at<V, PC>(d, (const V *)d.s, (const V *)d.sh, i);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (!tr.ok[r]) continue;
+            const int i = tr.ix[r];
             if (bp.precision == KCFD_FP64) {         // scipy's operation order
-                KCFD_GLOBAL V *xx = vec<V>(d.x);
-                V xi = xx[i] + al * ph;
-                xi = xi + om * sh;
-                xx[i] = xi;
+                V xi = (V)xv[r] + al * ph[r];
+                xi = xi + om * sh[r];
+                glw(d.x)[i] = (double)xi;
             } else {   // MIXED: the iterate accumulates in FP64 (an FP32 correction
                        // would carry an error of eps_32 |x| into the residual)
-                double xi = glw(d.x)[i] + st.alpha * (double)ph;
-                xi = xi + st.omega * (double)sh;
+                double xi = xv[r] + st.alpha * (double)ph[r];
+                xi = xi + st.omega * (double)sh[r];
                 glw(d.x)[i] = xi;
             }
-            const V ri = vec<V>(d.s)[i] - om * vec<V>(d.t)[i];
+            const V ri = sv[r] - om * tv[r];
             vec<V>(d.r)[i] = ri;
-            v[0] = (double)ri * (double)ri;
-            v[1] = (double)vec<V>(d.rt)[i] * (double)ri;
+            v[0] += (double)ri * (double)ri;
+            v[1] += (double)rtv[r] * (double)ri;
             if (PC == KCFD_PC_ILU0 && d.polled) {   // ready flags of the sync-free solves
                 vec<V>(d.ph)[i] = sentinel_value<V>();
                 vec<V>(d.sh)[i] = sentinel_value<V>();
@@ -781,8 +884,11 @@ __global__ __launch_bounds__(kBlock) void k_restore(const SysDesc *__restrict__ 
         const SysState &st = sts[ti.x];
         if (!wants_true(st) || !st.restore) continue;
         const SysDesc &d = sys[ti.x];
-        const int i = ti.y + threadIdx.x;
-        if (i < d.n) glw(d.x)[i] = glw(d.xb)[i];
+        if (d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+            if (tr.ok[r]) glw(d.x)[tr.ix[r]] = glw(d.xb)[tr.ix[r]];
     }
 }
 
@@ -795,14 +901,16 @@ __global__ __launch_bounds__(kBlock) void k_true_resid(const SysDesc *__restrict
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
-        if (!wants_true(sts[ti.x])) continue;
-        const int i = ti.y + threadIdx.x;
+        if (!wants_true(sts[ti.x]) || d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
         double v[2] = {0.0, 0.0};
-        if (i < d.n) {
-            const double bi = gl(d.b)[i];
-            const double ri = bi - row_ax64(d, i, d.x);
-            v[0] = ri * ri;
-            v[1] = bi * bi;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (!tr.ok[r]) continue;
+            const double bi = gl(d.b)[tr.ix[r]];
+            const double ri = bi - row_ax64(d, tr.ix[r], d.x);
+            v[0] += ri * ri;
+            v[1] += bi * bi;
         }
         write_partials<2>(v, partials, tile);
     }
@@ -837,12 +945,14 @@ __global__ __launch_bounds__(kBlock) void k_perm_in(const SysDesc *__restrict__ 
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
-        if (!d.perm || sts[ti.x].status != S_NEW) continue;
-        const int i = ti.y + threadIdx.x;
-        if (i < d.n) {
-            const int r = gl(d.perm)[i];
-            glw(const_cast<double *>(d.b))[i] = gl(d.b_user)[r];
-            glw(d.x)[i] = glw(d.x_user)[r];
+        if (!d.perm || sts[ti.x].status != S_NEW || d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (!tr.ok[r]) continue;
+            const int i = tr.ix[r], q = gl(d.perm)[i];
+            glw(const_cast<double *>(d.b))[i] = gl(d.b_user)[q];
+            glw(d.x)[i] = glw(d.x_user)[q];
         }
     }
 }
@@ -856,9 +966,11 @@ __global__ __launch_bounds__(kBlock) void k_perm_out(const SysDesc *__restrict__
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
-        if (!d.perm || !sts[ti.x].scatter) continue;
-        const int i = ti.y + threadIdx.x;
-        if (i < d.n) glw(d.x_user)[gl(d.perm)[i]] = glw(d.x)[i];
+        if (!d.perm || !sts[ti.x].scatter || d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+            if (tr.ok[r]) glw(d.x_user)[gl(d.perm)[tr.ix[r]]] = glw(d.x)[tr.ix[r]];
     }
 }
 
@@ -1008,7 +1120,10 @@ __device__ __forceinline__ V sub_plain(V acc, const FacRef &f, int base, int w,
 //   FIRST  (colour-1 rows i): y_i = z_i - sum_j l_ij z_j;  w_i = y_i / u_ii
 //   SECOND (colour-0 rows i): w_i = (z_i - sum_j u_ij w_j) / u_ii
 // Two plain launches per apply: no y vector, no ready flags.  Chunks are runs
-// of 64 rows of one colour; one wave per chunk, dealt round-robin.
+// of 64 rows of one colour, one wave per chunk.  As in the tile walk, the
+// chunk list is cut into one contiguous range per XCD (blockIdx % 8), whose
+// waves take its chunks round-robin: a range's gathers stay in its XCD's
+// L2.  Launched on the persistent grid (a multiple of 8 blocks).
 template <typename V, bool SECOND, int WHICH>
 __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
@@ -1017,9 +1132,11 @@ __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
     const int nchunks = counts[SECOND ? 4 : 3];
-    const int nwaves = (int)(gridDim.x * (blockDim.x >> 6));
-    const int wave = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
-    for (int c = wave; c < nchunks; c += nwaves) {
+    const int per = (nchunks + kXcds - 1) / kXcds;
+    const int c0 = (int)(blockIdx.x % kXcds) * per, c1 = min(c0 + per, nchunks);
+    const int wx = (int)(gridDim.x / kXcds) * kWaves;   // waves per XCD
+    const int lw = uniform((int)(blockIdx.x / kXcds) * kWaves + (int)(threadIdx.x >> 6));
+    for (int c = c0 + lw; c < c1; c += wx) {
         const Chunk ch = chunks[c];
         const SysDesc &d = sys[ch.sys];
         const int status = sts[ch.sys].status;
@@ -1108,9 +1225,9 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     if (PC == KCFD_PC_ILU0) {
         mark(3, b->ctx->ev0);
         if (b->run_two) {
-            hipLaunchKernelGGL((k_ilu2<V, false, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+            hipLaunchKernelGGL((k_ilu2<V, false, 0>), dim3(b->grid), blk, 0, s, b->d_desc,
                                b->d_state, b->d_chunks_a, b->d_counts);
-            hipLaunchKernelGGL((k_ilu2<V, true, 0>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+            hipLaunchKernelGGL((k_ilu2<V, true, 0>), dim3(b->grid), blk, 0, s, b->d_desc,
                                b->d_state, b->d_chunks_b, b->d_counts);
         }
         if (b->run_polled)
@@ -1130,9 +1247,9 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
                        b->d_partials, b->d_counters);
     if (PC == KCFD_PC_ILU0) {
         if (b->run_two) {
-            hipLaunchKernelGGL((k_ilu2<V, false, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+            hipLaunchKernelGGL((k_ilu2<V, false, 1>), dim3(b->grid), blk, 0, s, b->d_desc,
                                b->d_state, b->d_chunks_a, b->d_counts);
-            hipLaunchKernelGGL((k_ilu2<V, true, 1>), dim3(b->trsv_grid), blk, 0, s, b->d_desc,
+            hipLaunchKernelGGL((k_ilu2<V, true, 1>), dim3(b->grid), blk, 0, s, b->d_desc,
                                b->d_state, b->d_chunks_b, b->d_counts);
         }
         if (b->run_polled) {
@@ -1378,9 +1495,9 @@ int rebuild_tables(kcfd_batch *b) {
         d.tile0 = (int)tiles.size();
         d.ntiles = 0;
         if (!b->mats[k]) continue;
-        const int nt = std::max(1, (d.n + kBlock - 1) / kBlock);
+        const int nt = std::max(1, (d.n + kTile - 1) / kTile);
         d.ntiles = nt;
-        for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(k, t * kBlock));
+        for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(k, t * kTile));
     }
     b->ntiles = (int)tiles.size();
     bool grew = false;
