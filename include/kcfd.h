@@ -216,8 +216,9 @@ int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out);
 /* Measurement hooks (no reference counterpart; the reference times whole
  * solves with time.time, core/gpu_solver.py:202-250).  kcfd_batch_profile
  * runs `reps` solver iterations (after one warm-up) launched directly on the
- * context stream, with HIP events around one phase kernel in each, and
- * returns that kernel's mean duration: phase 1 = v = A M p with the fused
+ * context stream, with HIP events in the dispatch packets of one phase's
+ * kernels in each (hipExtLaunchKernelGGL: the kernels' own start and end),
+ * and returns that phase's mean duration: phase 1 = v = A M p with the fused
  * rt.v reduction, 2 = fused x/r update, 3 = ILU(0) apply to p (for a
  * two-colour ordering both of its kernels, otherwise the sync-free lower
  * triangular solve), 4 = p update.  The solver state is restored afterwards; the vectors (x
