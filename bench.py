@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--tile-walk", type=int, default=0,
                     help="launch shape: 1 interleaved, 2 contiguous tiles per workgroup (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-calibrate", action="store_true",
+                    help="serve: skip the per-template precision calibration")
     ap.add_argument("--no-single", action="store_true",
                     help="batched: skip the FP64 one-system-at-a-time GPU baseline")
     ap.add_argument("--check-every", type=int, default=8,
@@ -451,7 +453,17 @@ def bench_serve(args):
     t = time.perf_counter()
     temps = [S.Template(ctx, k, n, p, m, args.precond, "mixed", args.order) for k, n, (p, m) in built]
     del built
-    log(f"templates on the device in {time.perf_counter() - t:.1f}s; "
+    log(f"templates on the device in {time.perf_counter() - t:.1f}s")
+    if not args.no_calibrate:
+        # per-template precision: one mixed solve of each prototype matrix
+        t = time.perf_counter()
+        for tp in temps:
+            tp.calibrate()
+        log(f"precision calibration in {time.perf_counter() - t:.1f}s: FP64 for "
+            f"{sum(len(tp.fp64_kinds) for tp in temps)} of {2 * len(temps)} (template, kind) pairs: "
+            + ", ".join(f"{tp.kind}{tp.n}:{'p' if True in tp.fp64_kinds else ''}{'m' if False in tp.fp64_kinds else ''}"
+                        for tp in temps if tp.fp64_kinds))
+    log(f"templates ready; "
         f"request footprint {min(x.bytes for x in temps) / 1e6:.1f}..{max(x.bytes for x in temps) / 1e6:.1f} MB")
     cells = np.array([temps[ti].n for _, ti, _ in tr])
     log(f"trace: {len(tr)} requests over {args.duration:.0f}s, {cells.sum() / 1e6:.1f}M cells "

@@ -10,6 +10,14 @@ replays a captured graph of ``check_every`` iterations over all members, so
 the work of requests that arrived at different times overlaps on the device.
 Systems detach as they finish.  A request completes when its four solves do.
 
+Precision is chosen per template and system kind.  A mixed solve that
+stalls (FP32 operator error of about eps_32 * cond(A) per cycle; large
+pressure matrices) continues in FP64 from its best iterate, and from then on
+the same kind of system on the same mesh template goes to the FP64 batch
+directly instead of repeating the mixed attempt.  ``Template.calibrate``
+makes that choice before serving: one mixed solve of each prototype matrix
+(seeded right-hand side) marks the kinds that stall.
+
 Admission is by HBM footprint: a request is admitted only while the bytes
 of the admitted requests (matrices, preconditioners, vectors, right-hand
 sides, solutions) stay within ``hbm_budget`` and four slots are free; the
@@ -134,6 +142,32 @@ class Template:
         # matrices + preconditioners (two), four systems' vectors, b and x
         self.bytes = int(sum(A.bytes() for A in self.proto) + 4 * nvec * (self.n * vs + 256) + 8 * 8 * self.n)
         self.dmesh.mesh = None            # host geometry no longer needed
+        # system kinds (True: pressure, False: momentum) solved in FP64 from
+        # the start by a mixed-precision server (see calibrate)
+        self.fp64_kinds = set()
+        self.precond, self.precision = precond, precision
+
+    def calibrate(self, rtol=1e-8, seed=0, check_every=8):
+        """One mixed solve of each prototype matrix with a seeded right-hand
+        side; a kind whose solve does not converge in mixed precision is
+        marked for FP64.  Returns the solve results."""
+        out = []
+        if self.precision != "mixed":
+            return out
+        rng = np.random.default_rng(seed)
+        for pressure, A in zip((True, False), self.proto):
+            b = DevBuf.from_array(A.ctx, rng.uniform(-1.0, 1.0, self.n))
+            x = DevBuf(A.ctx, self.n)
+            x.fill(0.0)
+            opts = device.default_opts(rtol=rtol, precond=self.precond, precision="mixed",
+                                       check_every=check_every)
+            r = A.solve_dev(b.p, x.p, opts, setup=False, fallback=False)
+            if r.info != 0:
+                self.fp64_kinds.add(pressure)
+            out.append((pressure, r.info, r.iters))
+            b.free()
+            x.free()
+        return out
 
     def new_matrices(self, precond, precision):
         mats = []
@@ -204,6 +238,7 @@ class ServeReport:
     not_converged: int
     max_rel_resid: float
     fp64_fallbacks: int = 0
+    fp64_direct: int = 0
 
     @property
     def solves_per_s(self):
@@ -223,7 +258,7 @@ class ServeReport:
                     rounds=self.rounds, mean_members=round(self.mean_members, 1),
                     peak_hbm_gb=round(self.peak_hbm_bytes / 1e9, 2), peak_requests=self.peak_requests,
                     not_converged=self.not_converged, max_rel_resid=self.max_rel_resid,
-                    fp64_fallbacks=self.fp64_fallbacks)
+                    fp64_fallbacks=self.fp64_fallbacks, fp64_direct=self.fp64_direct)
 
 
 class Server:
@@ -251,6 +286,10 @@ class Server:
             self.batch = h
         self.batch64 = None
         self.fallbacks = 0
+        self.direct64 = 0
+        # (template, system kind) whose mixed solve stalled: later systems of
+        # that kind on that template are solved in FP64 from the start
+        self.force64 = set()
         self.keep_solutions = False
         self.solutions = {}
 
@@ -341,8 +380,15 @@ class Server:
                 self._admit(req, time.perf_counter() - t0)
                 for k, (A, b, x) in enumerate(req.systems()):
                     slot = L.ctypes.c_int()
-                    check(lib().kcfd_batch_attach(self.batch, A.h, b.p, x.p, L.ctypes.byref(slot)))
-                    owner[slot.value] = (req, k)
+                    if self.precision == "mixed" and ((k == 0) in req.template.fp64_kinds or
+                                                      (id(req.template), k == 0) in self.force64):
+                        check(lib().kcfd_batch_attach(self._fp64_batch(), A.h, b.p, x.p,
+                                                      L.ctypes.byref(slot)))
+                        owner64[slot.value] = (req, k, None)
+                        self.direct64 += 1
+                    else:
+                        check(lib().kcfd_batch_attach(self.batch, A.h, b.p, x.p, L.ctypes.byref(slot)))
+                        owner[slot.value] = (req, k)
                     req.slots.append(slot.value)
                 in_use += req.template.bytes
                 active += 1
@@ -374,6 +420,7 @@ class Server:
                                                       L.ctypes.byref(s64)))
                         owner64[s64.value] = (req, k, r)
                         self.fallbacks += 1
+                        self.force64.add((id(req.template), k == 0))
                     else:
                         finished.append((req, k, r))
             if owner64:
@@ -382,7 +429,10 @@ class Server:
                     slot = fin[j]
                     req, k, r1 = owner64.pop(slot)
                     check(lib().kcfd_batch_detach(self.batch64, slot, L.ctypes.byref(res)))
-                    finished.append((req, k, (res.info, r1[1] + res.iters, r1[2], res.rel_resid)))
+                    if r1 is None:        # FP64 from the start
+                        finished.append((req, k, (res.info, res.iters, 0, res.rel_resid)))
+                    else:
+                        finished.append((req, k, (res.info, r1[1] + res.iters, r1[2], res.rel_resid)))
             now = time.perf_counter() - t0
             for req, k, r in finished:
                 req.results[k] = r
@@ -403,4 +453,4 @@ class Server:
                            peak_requests=peak_reqs,
                            not_converged=sum(1 for x in allres if x[0] != 0),
                            max_rel_resid=float(max((x[3] for x in allres), default=0.0)),
-                           fp64_fallbacks=self.fallbacks)
+                           fp64_fallbacks=self.fallbacks, fp64_direct=self.direct64)

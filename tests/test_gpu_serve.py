@@ -83,3 +83,31 @@ def test_serve_one_at_a_time_fp64(ctx, templates):
     rep = srv.run(_trace(6, 1000.0, len(templates), seed=3))
     print(rep.summary())
     assert rep.requests == 6 and rep.not_converged == 0 and rep.peak_requests == 1
+
+
+def test_serve_routes_stalled_kind_to_fp64(ctx, templates):
+    """A (template, system kind) marked as stalling in mixed precision is
+    solved in the FP64 batch from the start; its results equal the FP64
+    solve of the same system alone, and the other systems stay mixed."""
+    from kcfd_amd import device, serve as S
+    trace = [(0.0, 0, 7100 + i) for i in range(3)] + [(0.0, 1, 7200)]
+    srv = S.Server(ctx, templates, precision="mixed", precond="ilu0", batched=True, check_every=4)
+    srv.keep_solutions = True
+    srv.force64.add((id(templates[0]), True))      # template 0's pressure solves
+    try:
+        rep = srv.run(trace)
+        assert rep.requests == 4 and rep.not_converged == 0 and rep.max_rel_resid < 1e-8
+        assert rep.fp64_direct == 3 and rep.fp64_fallbacks == 0
+        o64 = device.default_opts(rtol=1e-8, precond="ilu0", precision="fp64", check_every=4)
+        for rid, (t, ti, seed) in enumerate(trace):
+            req = S.Request(rid, 0.0, templates[ti], seed)
+            srv._admit(req, 0.0)
+            xs_mix, res_mix = srv.solutions[rid]
+            for k, (A, b, x) in enumerate(req.systems()):
+                fp64 = ti == 0 and k == 0
+                r = A.solve_dev(b.p, x.p, o64 if fp64 else srv.opts, setup=False)
+                assert r.iters == res_mix[k][1] and r.info == res_mix[k][0] == 0, (rid, k)
+                assert np.array_equal(x.download(), xs_mix[k]), (rid, k)
+            req.release()
+    finally:
+        srv.close()
