@@ -49,6 +49,12 @@ extern "C" {
 #define KCFD_INFO_OK 0
 #define KCFD_INFO_RHO_BREAKDOWN -10
 #define KCFD_INFO_OMEGA_BREAKDOWN -11
+#define KCFD_INFO_PQ_BREAKDOWN -12    /* CG: p.q == 0 (scipy would divide by zero) */
+
+/* Krylov method (kcfd_solve_opts.method) */
+#define KCFD_METHOD_BICGSTAB 0  /* scipy bicgstab: solve_linear_system (core/linear_solver.py:58-62) */
+#define KCFD_METHOD_CG 1        /* scipy cg: solve_pressure_correction (core/linear_solver.py:88-93);
+                                   FP64 only */
 
 /* boundary-condition kinds (core/fvm_operators.py:178-225) */
 #define KCFD_BC_ZERO_GRADIENT 0
@@ -164,6 +170,7 @@ typedef struct kcfd_solve_opts {
                            tiles of its XCD's range -- 1 interleaved (tiles j,
                            j+J, ...), 2 one contiguous run per workgroup
                            (0: library default) */
+    int method;         /* KCFD_METHOD_* (0: BiCGSTAB) */
 } kcfd_solve_opts;
 
 typedef struct kcfd_solve_result {

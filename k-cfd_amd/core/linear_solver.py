@@ -85,8 +85,35 @@ def solve_linear_system(system: FVMSystem, x0: np.ndarray = None, method: str = 
 
 def solve_pressure_correction(system: FVMSystem, method: str = "direct", tol: float = 1e-6,
                               maxiter: int = 2000, preconditioner: str = "none", **precond_kwargs) -> np.ndarray:
-    """core/linear_solver.py:73-93: 'direct' is scipy spsolve as in the
-    reference; the CG branch has no GPU implementation yet."""
+    """Drop-in ``solve_pressure_correction`` (core/linear_solver.py:73-93).
+
+    'direct' is scipy spsolve as in the reference.  Any other ``method`` is
+    the reference's CG branch, ``cg(A, b, x0=0, atol=tol, maxiter=maxiter,
+    M=M)`` with scipy's default ``rtol=1e-5``: here the device CG in FP64
+    (libkcfd, KCFD_METHOD_CG) with scipy's recurrences, preconditioned by
+    'none', 'jacobi' or 'ilu0' ('amg' falls back to 'ilu0' with the
+    reference's message).  Extra keywords: ``rtol``, ``order`` (ILU(0) row
+    ordering), ``return_info``.  As in the reference, a non-converged solve
+    returns x without a message."""
     if method == "direct":
         return spsolve(system.to_sparse(), system.rhs)
-    raise NotImplementedError("the CG pressure-correction solve has no GPU implementation yet")
+    if preconditioner == "amg":
+        print("  [Preconditioner] pyamg 미설치. ILU(0)으로 폴백합니다.")
+        preconditioner = "ilu0"
+    if preconditioner == "iluk":
+        raise NotImplementedError("iluk has no GPU implementation; use 'ilu0'")
+    if preconditioner not in _PCS:
+        raise ValueError(f"Unknown preconditioner: {preconditioner}")
+    rtol = float(precond_kwargs.get("rtol", 1e-5))
+    return_info = bool(precond_kwargs.get("return_info", False))
+    A, b_ptr, keep = _device_system(system)
+    ctx = A.ctx
+    x = L.DevBuf.from_array(ctx, np.zeros(system.n))
+    opts = default_opts(rtol=rtol, atol=tol, maxiter=maxiter, precond=preconditioner, precision="fp64",
+                        method="cg")
+    A.setup_precond(preconditioner, "fp64", order=precond_kwargs.get("order"))
+    res = A.solve_dev(b_ptr, x.p, opts, setup=False)
+    out = x.download()
+    if return_info:
+        return out, res.as_dict()
+    return out

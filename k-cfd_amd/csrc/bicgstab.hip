@@ -111,7 +111,7 @@ struct SysState {
 struct BatchParams {
     double rtol, atol, delta, rhotol;
     int precision, precond, max_refine, nsys;
-    int inner_maxiter, pad[3];
+    int inner_maxiter, method, pad[2];
 };
 
 // ILU trisolve work chunk: up to 64 consecutive rows of one system's level list
@@ -426,6 +426,7 @@ __global__ __launch_bounds__(kFinBlock) void f_init(const SysDesc *__restrict__ 
         st.atol_in = thr;
         st.rr = tot[1];
         if (st.maxiter <= 0 || sqrt(tot[1]) < thr) { st.status = S_DONE; return; }
+        if (bp.method == KCFD_METHOD_CG) { st.status = S_START; return; }   // iteration 0: p = z
         st.rho = tot[1];           // rt = r, so rt.r = ||r||^2
         if (fabs(st.rho) < bp.rhotol) { st.info = KCFD_INFO_RHO_BREAKDOWN; st.status = S_DONE; return; }
         st.status = S_START;       // iteration 0 next: p = r
@@ -867,6 +868,214 @@ __global__ __launch_bounds__(kFinBlock) void f_xr(const SysDesc *__restrict__ sy
     st.rho = rho;
 }
 
+// ------------------------------------------------ conjugate gradients
+// scipy's cg (scipy/sparse/linalg/_isolve/iterative.py, the solver behind
+// solve_pressure_correction, core/linear_solver.py:88-93), in FP64, with its
+// operation order (p = p beta + z; x += alpha p; r -= alpha q):
+//   ILU(0): z = M r by the triangular-solve kernels (into ph)
+//   C1 z = M r (Jacobi / none inline); r.z        FC1: rho, beta = rho / rho_prev
+//   C2 p = z (iteration 0) | p = p beta + z
+//   C3 q = A p; p.q                               FC3: alpha = rho / p.q
+//   C4 x += alpha p; r -= alpha q; ||r||^2         FC4: iteration count; stop at
+//      maxiter (scipy's loop exhausted) or when ||r|| < atol (its next check)
+// q lives in v.  A zero rhs: C1 sets x = 0, FC1 ends the solve (x = b = 0).
+template <int PC>
+__device__ __forceinline__ double cg_z(const SysDesc &d, int i) {
+#pragma clang fp contract(off)
+    if (PC == KCFD_PC_JACOBI) return gl(d.dinv64)[i] * vec<double>(d.r)[i];
+    if (PC == KCFD_PC_ILU0) return vec<double>(d.ph)[i];
+    return vec<double>(d.r)[i];
+}
+
+template <int PC>
+__global__ __launch_bounds__(kBlock) void k_cg_rz(const SysDesc *__restrict__ sys,
+                                                  const SysState *__restrict__ sts,
+                                                  const int2 *__restrict__ tiles, const int *__restrict__ counts,
+                                                  double *__restrict__ partials) {
+#pragma clang fp contract(off)
+    const int ntiles = counts[0];   // tiles of the current membership
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const int status = sts[ti.x].status;
+        if (d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+        if (status == S_ZEROX) {
+#pragma unroll
+            for (int r = 0; r < kRows; ++r)
+                if (tr.ok[r]) glw(d.x)[tr.ix[r]] = 0.0;
+            continue;
+        }
+        if (status != S_RUN && status != S_START) continue;
+        double rv[kRows], zv[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            rv[r] = vec<double>(d.r)[tr.ix[r]];
+            zv[r] = cg_z<PC>(d, tr.ix[r]);
+        }
+        double v[1] = {0.0};
+#pragma unroll
+        for (int r = 0; r < kRows; ++r)
+            if (tr.ok[r]) v[0] += rv[r] * zv[r];
+        write_partials<1>(v, partials, tile);
+    }
+}
+
+__global__ __launch_bounds__(kFinBlock) void f_cg_rho(const SysDesc *__restrict__ sys,
+                                                    SysState *__restrict__ sts,
+                                                    const double *__restrict__ partials) {
+    const SysDesc &d = sys[blockIdx.x];
+    SysState &st = sts[blockIdx.x];
+    if (st.status == S_ZEROX) {
+        if (threadIdx.x == 0) { st.info = 0; st.status = S_DONE; }
+        return;
+    }
+    if (st.status != S_RUN && st.status != S_START) return;
+    double tot[1];
+    sum_partials<1>(d, partials, tot);
+    if (threadIdx.x != 0) return;
+    st.rho = tot[0];
+    if (st.status == S_RUN) st.beta = st.rho / st.rho_prev;
+}
+
+template <int PC>
+__global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc *__restrict__ sys,
+                                                 const SysState *__restrict__ sts,
+                                                 const int2 *__restrict__ tiles, const int *__restrict__ counts) {
+#pragma clang fp contract(off)
+    const int ntiles = counts[0];   // tiles of the current membership
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const SysState &st = sts[ti.x];
+        const int status = st.status;
+        if ((status != S_RUN && status != S_START) || d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+        double zv[kRows], pv[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            zv[r] = cg_z<PC>(d, tr.ix[r]);
+            pv[r] = vec<double>(d.p)[tr.ix[r]];
+        }
+        const double beta = st.beta;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (!tr.ok[r]) continue;
+            const int i = tr.ix[r];
+            double pi = zv[r];
+            if (status == S_RUN) {
+                pi = pv[r] * beta;
+                pi = pi + zv[r];
+            }
+            vec<double>(d.p)[i] = pi;
+            if (PC == KCFD_PC_ILU0 && d.polled) vec<double>(d.ph)[i] = sentinel_value<double>();
+        }
+    }
+}
+
+template <int PC>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cg_q(
+    const SysDesc *__restrict__ sys, const SysState *__restrict__ sts, const int2 *__restrict__ tiles,
+    const int *__restrict__ counts, double *__restrict__ partials) {
+#pragma clang fp contract(off)
+    const int ntiles = counts[0];   // tiles of the current membership
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const int status = sts[ti.x].status;
+        if ((status != S_RUN && status != S_START) || d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+        double v[1] = {0.0};
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {   // one row after the other, as in K2
+            const double q = row_amz<double, KCFD_PC_NONE>(d, tr.ix[r], (const double *)d.p,
+                                                           (const double *)d.p);
+            const double pv = vec<double>(d.p)[tr.ix[r]];
+            if (tr.ok[r]) {
+                vec<double>(d.v)[tr.ix[r]] = q;
+                v[0] += pv * q;
+            }
+        }
+        write_partials<1>(v, partials, tile);
+    }
+}
+
+__global__ __launch_bounds__(kFinBlock) void f_cg_alpha(const SysDesc *__restrict__ sys,
+                                                      SysState *__restrict__ sts,
+                                                      const double *__restrict__ partials) {
+    const SysDesc &d = sys[blockIdx.x];
+    SysState &st = sts[blockIdx.x];
+    if (st.status != S_RUN && st.status != S_START) return;
+    double tot[1];
+    sum_partials<1>(d, partials, tot);
+    if (threadIdx.x != 0) return;
+    const double pq = tot[0];
+    if (pq == 0.0 || !(pq == pq)) {   // scipy would divide by zero and carry NaN to maxiter
+        st.info = KCFD_INFO_PQ_BREAKDOWN;
+        st.status = S_DONE;
+        return;
+    }
+    st.alpha = st.rho / pq;
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_xr(const SysDesc *__restrict__ sys,
+                                                  const SysState *__restrict__ sts,
+                                                  const int2 *__restrict__ tiles, const int *__restrict__ counts,
+                                                  double *__restrict__ partials) {
+#pragma clang fp contract(off)
+    const int ntiles = counts[0];   // tiles of the current membership
+    KCFD_TILE_LOOP(tile, ntiles) {
+        const int2 ti = tiles[tile];
+        const SysDesc &d = sys[ti.x];
+        const SysState &st = sts[ti.x];
+        if ((st.status != S_RUN && st.status != S_START) || d.n <= 0) continue;
+        const TileRows tr(ti.y, d.n);
+        double xv[kRows], pv[kRows], rv[kRows], qv[kRows];
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            const int i = tr.ix[r];
+            xv[r] = glw(d.x)[i];
+            pv[r] = vec<double>(d.p)[i];
+            rv[r] = vec<double>(d.r)[i];
+            qv[r] = vec<double>(d.v)[i];
+        }
+        const double alpha = st.alpha;
+        double v[1] = {0.0};
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (!tr.ok[r]) continue;
+            const int i = tr.ix[r];
+            glw(d.x)[i] = xv[r] + alpha * pv[r];
+            const double ri = rv[r] - alpha * qv[r];
+            vec<double>(d.r)[i] = ri;
+            v[0] += ri * ri;
+        }
+        write_partials<1>(v, partials, tile);
+    }
+}
+
+__global__ __launch_bounds__(kFinBlock) void f_cg_check(const SysDesc *__restrict__ sys,
+                                                      SysState *__restrict__ sts,
+                                                      const double *__restrict__ partials) {
+    const SysDesc &d = sys[blockIdx.x];
+    SysState &st = sts[blockIdx.x];
+    if (st.status != S_RUN && st.status != S_START) return;
+    double tot[1];
+    sum_partials<1>(d, partials, tot);
+    if (threadIdx.x != 0) return;
+    st.iters += 1;
+    st.total_iters += 1;
+    st.rho_prev = st.rho;
+    st.rr = tot[0];
+    if (st.total_iters >= st.maxiter) {           // scipy's for-loop exhausted
+        st.info = st.total_iters;
+        st.status = S_DONE;
+        return;
+    }
+    if (sqrt(tot[0]) < st.atol_in) { st.info = 0; st.status = S_DONE; return; }
+    st.status = S_RUN;
+}
+
 // ------------------------------------------------------- true residual
 // ||b - A x||^2 and ||b||^2 in FP64 on the FP64 matrix, every system.
 // Systems that are done and whose true residual is not yet computed.
@@ -1058,7 +1267,7 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
         // refinement just restarted (S_START set inside F2) must not run them,
         // or stale values would be left in sh
         const bool want = WHICH == 0 ? (status == S_RUN || status == S_START || status == S_RESUME)
-                                     : (status == S_RUN);
+                                     : (WHICH == 1 ? status == S_RUN : (status == S_RUN || status == S_START));
         const bool live = want && lane < ch.count;
         if (live) {
             const int i = gl(lvl_rows[ch.sys])[ch.start + lane];
@@ -1066,13 +1275,13 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
             const int o0 = uniform(gl(off)[ch.local]);
             const int w = uniform((gl(off)[ch.local + 1] - o0) >> 6);
             if (!UPPER) {
-                const V *z = WHICH == 0 ? (const V *)d.p : (const V *)d.s;
+                const V *z = (const V *)(WHICH == 0 ? d.p : (WHICH == 1 ? d.s : d.r));
                 V *y = (V *)d.y;
                 const V acc = sub_ready<V>(gl(z)[i], fac_l(d), o0 + lane, w, y);
                 publish<V>(y + i, acc);
             } else {
                 V *y = (V *)d.y;
-                V *wv = WHICH == 0 ? (V *)d.ph : (V *)d.sh;
+                V *wv = (V *)(WHICH == 1 ? d.sh : d.ph);
                 const V acc = sub_ready<V>(glw(y)[i], fac_u(d), o0 + lane, w, wv);
                 publish<V>(wv + i, acc / gl(udiag<V>(d))[i]);
                 glw(y)[i] = sentinel_value<V>();
@@ -1143,11 +1352,11 @@ __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys
         const SysDesc &d = sys[ch.sys];
         const int status = sts[ch.sys].status;
         const bool want = WHICH == 0 ? (status == S_RUN || status == S_START || status == S_RESUME)
-                                     : (status == S_RUN);
+                                     : (WHICH == 1 ? status == S_RUN : (status == S_RUN || status == S_START));
         if (want && lane < ch.count) {
             const int i = ch.start + lane;       // row of P A P^T
-            const KCFD_GLOBAL V *__restrict__ z = vec<V>(WHICH == 0 ? d.p : d.s);
-            KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 0 ? d.ph : d.sh);
+            const KCFD_GLOBAL V *__restrict__ z = vec<V>(WHICH == 0 ? d.p : (WHICH == 1 ? d.s : d.r));
+            KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 1 ? d.sh : d.ph);
             const int *off = SECOND ? d.chu_off : d.chl_off;
             const int o0 = uniform(gl(off)[ch.local]);
             const int wd = uniform((gl(off)[ch.local + 1] - o0) >> 6);
@@ -1281,6 +1490,37 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     launch_k(b, 0, false, false, f_xr, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
 }
 
+// One CG iteration (FP64).  Profiled phases: 1 = q = A p (C3), 2 = x/r
+// update (C4), 3 = the ILU(0) apply z = M r, 4 = p update (C2).
+template <typename V, int PC>
+void launch_iteration_cg(kcfd_batch *b, hipStream_t s) {
+    const int g = b->grid, ns = b->nsys, tg = b->trsv_grid;
+    if (PC == KCFD_PC_ILU0) {
+        if (b->run_two) {
+            launch_k(b, 3, true, false, k_ilu2<V, false, 2>, g, kBlock, s, b->d_desc, b->d_state,
+                     b->d_chunks_a, b->d_counts);
+            launch_k(b, 3, false, !b->run_polled, k_ilu2<V, true, 2>, g, kBlock, s, b->d_desc, b->d_state,
+                     b->d_chunks_b, b->d_counts);
+        }
+        if (b->run_polled) {
+            launch_k(b, 3, !b->run_two, false, k_trsv<V, false, 2>, tg, kBlock, s, b->d_desc, b->d_state,
+                     b->d_chunks_l, b->d_counts, b->d_lvl_l);
+            launch_k(b, 3, false, true, k_trsv<V, true, 2>, tg, kBlock, s, b->d_desc, b->d_state,
+                     b->d_chunks_u, b->d_counts, b->d_lvl_u);
+        }
+    }
+    launch_k(b, 0, false, false, k_cg_rz<PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+             b->d_partials);
+    launch_k(b, 0, false, false, f_cg_rho, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials);
+    launch_k(b, 4, true, true, k_cg_p<PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts);
+    launch_k(b, 1, true, true, k_cg_q<PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+             b->d_partials);
+    launch_k(b, 0, false, false, f_cg_alpha, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials);
+    launch_k(b, 2, true, true, k_cg_xr, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+             b->d_partials);
+    launch_k(b, 0, false, false, f_cg_check, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials);
+}
+
 template <typename V, int PC>
 void launch_init(kcfd_batch *b, hipStream_t s) {
     if (PC == KCFD_PC_ILU0)
@@ -1294,7 +1534,12 @@ void launch_init(kcfd_batch *b, hipStream_t s) {
 
 using LaunchFn = void (*)(kcfd_batch *, hipStream_t);
 
-LaunchFn pick_iter(int precision, int pc) {
+LaunchFn pick_iter(int precision, int pc, int method = KCFD_METHOD_BICGSTAB) {
+    if (method == KCFD_METHOD_CG) {   // FP64 only (check_opts)
+        if (pc == KCFD_PC_JACOBI) return launch_iteration_cg<double, KCFD_PC_JACOBI>;
+        if (pc == KCFD_PC_ILU0) return launch_iteration_cg<double, KCFD_PC_ILU0>;
+        return launch_iteration_cg<double, KCFD_PC_NONE>;
+    }
     if (precision == KCFD_FP64) {
         if (pc == KCFD_PC_JACOBI) return launch_iteration<double, KCFD_PC_JACOBI>;
         if (pc == KCFD_PC_ILU0) return launch_iteration<double, KCFD_PC_ILU0>;
@@ -1319,10 +1564,18 @@ LaunchFn pick_init(int precision, int pc) {
 // algorithmic bytes (roofline model): CSR stream M_A, ILU factor stream M_F,
 // vector streams of the fused schedule; MIXED adds one FP64 residual
 // replacement (M_A at 8 bytes + 6 n*8) per reliable update
-double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st) {
+double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st,
+                   int method = KCFD_METHOD_BICGSTAB) {
     const double n = m->n, nnz = m->nnz;
     auto MA = [&](double sv) { return nnz * (sv + 4) + (n + 1) * 4; };
     auto MF = [&](double sv) { return nnz * (sv + 4) + 2 * (n + 1) * 4; };
+    if (method == KCFD_METHOD_CG) {
+        // C1..C4 per iteration: Jacobi M_A + 14 n s_v, ILU(0) M_A + M_F + 17 n s_v
+        // (the apply's 4 streams included), none M_A + 12 n s_v
+        const double per = pc == KCFD_PC_JACOBI ? MA(8) + 14 * n * 8
+                         : pc == KCFD_PC_ILU0 ? MA(8) + MF(8) + 17 * n * 8 : MA(8) + 12 * n * 8;
+        return st.total_iters * per + MA(8) + 3 * n * 8;
+    }
     auto per_iter = [&](double sv) {
         if (pc == KCFD_PC_JACOBI) return 2 * MA(sv) + 22 * n * sv;
         if (pc == KCFD_PC_ILU0) return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv;
@@ -1369,6 +1622,8 @@ int check_opts(const kcfd_solve_opts *o) {
     KCFD_ARG(o->grid_per_cu >= 0 && o->grid_per_cu <= 8, "grid_per_cu must be 0..8");
     KCFD_ARG(o->trsv_per_cu >= 0 && o->trsv_per_cu <= 4, "trsv_per_cu must be 0..4");
     KCFD_ARG(o->tile_walk >= 0 && o->tile_walk <= 2, "tile_walk must be 0..2");
+    KCFD_ARG(o->method == KCFD_METHOD_BICGSTAB || o->method == KCFD_METHOD_CG, "unknown Krylov method");
+    KCFD_ARG(o->method != KCFD_METHOD_CG || prec == KCFD_FP64, "CG runs in FP64 only");
     return KCFD_OK;
 }
 
@@ -1407,6 +1662,7 @@ int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **o
     b->bp.max_refine = o->max_refine > 0 ? o->max_refine : 1000;
     b->bp.nsys = slots;
     b->bp.inner_maxiter = 5000;
+    b->bp.method = o->method;
     b->mats.assign(slots, nullptr);
     b->h_desc.assign(slots, SysDesc{});
     b->sys_n.assign(slots, 0);
@@ -1660,7 +1916,7 @@ int capture_graph(kcfd_batch *b, int iters) {
         hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
-    LaunchFn fn = pick_iter(b->bp.precision, b->bp.precond);
+    LaunchFn fn = pick_iter(b->bp.precision, b->bp.precond, b->bp.method);
     hipGraph_t g = nullptr;
     KCFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < iters; ++k) fn(b, s);
@@ -1745,7 +2001,7 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
             r.half_step = st.half;
             r.refinements = st.refinements;
             r.rel_resid = st.true_bb > 0 ? sqrt(st.true_rr / st.true_bb) : sqrt(st.true_rr);
-            r.bytes = model_bytes(b->mats[k], b->bp.precond, b->bp.precision, st);
+            r.bytes = model_bytes(b->mats[k], b->bp.precond, b->bp.precision, st, b->bp.method);
             r.ms = ms;
         }
     }
@@ -1926,7 +2182,7 @@ int kcfd_batch_detach(kcfd_batch *b, int slot, kcfd_solve_result *res) {
         res->refinements = st.refinements;
         res->rel_resid = st.true_done ? (st.true_bb > 0 ? sqrt(st.true_rr / st.true_bb) : sqrt(st.true_rr))
                                       : NAN;
-        res->bytes = model_bytes(b->mats[slot], b->bp.precond, b->bp.precision, st);
+        res->bytes = model_bytes(b->mats[slot], b->bp.precond, b->bp.precision, st, b->bp.method);
         res->ms = 0.0;
     }
     KCFD_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -2006,12 +2262,13 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
         run[k].status = S_RUN;
         if (run[k].omega == 0.0) run[k].omega = 1.0;
         if (run[k].rho == 0.0) run[k].rho = 1.0;
+        if (run[k].rho_prev == 0.0) run[k].rho_prev = 1.0;   // CG: beta = rho / rho_prev
         run[k].atol_in = run[k].atol_out = -1.0;
         run[k].maxiter = 1 << 30;
         run[k].iters = run[k].total_iters = 0;
         run[k].rmax = INFINITY;   // MIXED: no divergence restart
     }
-    const LaunchFn it = pick_iter(b->bp.precision, b->bp.precond);
+    const LaunchFn it = pick_iter(b->bp.precision, b->bp.precond, b->bp.method);
     BatchParams pp = b->bp;
     pp.rhotol = -1.0;     // no breakdown exit
     pp.delta = 0.0;       // MIXED: no residual replacement
@@ -2057,7 +2314,11 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
         const double n = m->n, nnz = m->nnz;
         const double MA = nnz * (sv + 4) + (n + 1) * 4;
         const double MFh = 0.5 * nnz * (sv + 4) + (n + 1) * 4;  // one triangle + its level list
-        if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
+        const bool cg = b->bp.method == KCFD_METHOD_CG;
+        if (cg && phase == 1) tot += MA + 2 * n * sv;          // q = A p: p gathered, q written
+        else if (cg && phase == 2) tot += 6 * n * sv;          // x, p, r, q read; x, r written
+        else if (cg && phase == 4) tot += (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
+        else if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
         else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
         else if (phase == 3) {
             if (m->two_colour)   // both kernels of the two-colour apply: L and U entries, z,
@@ -2088,7 +2349,7 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
                  m->solver->bp.precision == opts->precision && m->solver->h_desc[0].b_user == bvec &&
                  m->solver->h_desc[0].x_user == x && m->solver->opts.grid_per_cu == opts->grid_per_cu &&
                  m->solver->opts.trsv_per_cu == opts->trsv_per_cu &&
-                 m->solver->opts.tile_walk == opts->tile_walk;
+                 m->solver->opts.tile_walk == opts->tile_walk && m->solver->opts.method == opts->method;
     if (!reuse) {
         if (m->solver) kcfd_batch_destroy(m->solver);
         m->solver = nullptr;

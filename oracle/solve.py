@@ -2,6 +2,9 @@
 
 TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
+* ``solve_cg`` calls ``scipy.sparse.linalg.cg`` as ``solve_pressure_correction``
+  does (core/linear_solver.py:88-93), with explicit ``rtol``/``atol`` and an
+  iteration-counting callback.
 * ``solve_bicgstab`` calls ``scipy.sparse.linalg.bicgstab`` exactly as
   ``solve_linear_system`` does (core/linear_solver.py:56-62) but with explicit
   ``rtol``/``atol`` and an iteration-counting callback (the convention of
@@ -21,7 +24,7 @@ import subprocess
 
 import numpy as np
 from scipy import sparse
-from scipy.sparse.linalg import LinearOperator, bicgstab, spilu
+from scipy.sparse.linalg import LinearOperator, bicgstab, cg, spilu
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -185,6 +188,26 @@ def solve_bicgstab(A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=None, precond="ja
 
     x, info = bicgstab(A, b, x0=np.array(x0, np.float64), rtol=rtol, atol=atol,
                        maxiter=maxiter, M=M, callback=cb)
+    return x, info, count[0]
+
+
+def solve_cg(A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=None, precond="jacobi", M=None):
+    """scipy CG with the call of solve_pressure_correction
+    (core/linear_solver.py:88-93); returns (x, info, iterations), iterations
+    = callback calls = completed iterations."""
+    A = sparse.csr_matrix(A)
+    n = A.shape[0]
+    if x0 is None:
+        x0 = np.zeros(n)
+    if M is None:
+        M = make_preconditioner(A, precond)
+    count = [0]
+
+    def cb(_xk):
+        count[0] += 1
+
+    x, info = cg(A, b, x0=np.array(x0, np.float64), rtol=rtol, atol=atol, maxiter=maxiter, M=M,
+                 callback=cb)
     return x, info, count[0]
 
 

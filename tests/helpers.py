@@ -90,16 +90,17 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / (nb if nb > 0 else 1.0))
 
 
-def oracle_band(A, b, nrep=6, **kw):
-    """scipy BiCGSTAB on b and on nrep-1 copies of b perturbed by one ulp per
-    entry: the reference's own rounding-noise floor.  Returns (iteration
-    counts, infos, x of the unperturbed run, max relative L2 spread of the
-    perturbed solutions from it)."""
+def oracle_band(A, b, nrep=6, solver="bicgstab", **kw):
+    """scipy BiCGSTAB (or CG) on b and on nrep-1 copies of b perturbed by one
+    ulp per entry: the reference's own rounding-noise floor.  Returns
+    (iteration counts, infos, x of the unperturbed run, max relative L2
+    spread of the perturbed solutions from it)."""
     from oracle import solve as osolve
+    run = osolve.solve_cg if solver == "cg" else osolve.solve_bicgstab
     its, infos, xs = [], [], []
     for k in range(nrep):
         bb = b if k == 0 else b * (1 + np.random.default_rng(k).choice([-1, 1], len(b)) * 2.0 ** -52)
-        x, info, it = osolve.solve_bicgstab(A, bb, **kw)
+        x, info, it = run(A, bb, **kw)
         its.append(it)
         infos.append(info)
         xs.append(x)
