@@ -87,6 +87,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--precond", default="ilu0", choices=["ilu0", "jacobi"])
+    ap.add_argument("--method", default="bicgstab", choices=["bicgstab", "cg"],
+                    help="pressure: Krylov method (cg: solve_pressure_correction's solver, FP64 only)")
     ap.add_argument("--no-tune", action="store_true", help="batched: library default launch shape")
     ap.add_argument("--delta", type=float, default=0.1,
                     help="mixed: reliable-update trigger (FP32 residual reduction)")
@@ -148,11 +150,13 @@ def bench_pressure(args):
 
     results = {}
     precs = [p for p in args.precisions.split(",") if p]
+    if args.method == "cg":
+        precs = ["fp64"]                 # the CG path runs in FP64
     for prec in precs:
         opts = device.default_opts(rtol=1e-8, precond=args.precond, precision=prec,
                                    inner_rtol=args.delta, check_every=args.check_every,
                                    grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu,
-                                   tile_walk=args.tile_walk)
+                                   tile_walk=args.tile_walk, method=args.method)
         lat, rs = [], []
 
         def step():
@@ -219,6 +223,8 @@ def bench_pressure(args):
             Ah = A.to_scipy()
             bh_ = case.rhs
             its = fp64["iters"]
+            if args.method == "cg":
+                raise RuntimeError("no CPU baseline for the CG workload")
             if args.precond == "ilu0" and args.order != "natural":
                 # the CPU path factors in the natural order, which needs fewer
                 # iterations than the colour ordering: count them on the GPU
@@ -243,9 +249,10 @@ def bench_pressure(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner / f64 refinement",
+        "dtype": "f32 inner / f64 refinement" if "mixed" in results else "f64",
         "data": "synthetic",
-        "config": {"workload": f"hex{args.n}^3 SIMPLE pressure correction, BiCGSTAB+{args.precond}, "
+        "config": {"workload": f"hex{args.n}^3 SIMPLE pressure correction, "
+                               f"{'BiCGSTAB' if args.method == 'bicgstab' else 'CG'}+{args.precond}, "
                                f"rtol 1e-8", "cells": n, "nnz": nnz,
                    "ilu_order": args.order if args.precond == "ilu0" else None,
                    "step": "device assembly + preconditioner set-up + solve"},

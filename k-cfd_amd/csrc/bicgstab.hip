@@ -1765,7 +1765,19 @@ int rebuild_tables(kcfd_batch *b) {
         if (!b->mats[k]) continue;
         const int nt = std::max(1, (d.n + kTile - 1) / kTile);
         d.ntiles = nt;
-        for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(k, t * kTile));
+        const kcfd_mat *m = b->mats[k];
+        if (d.perm && !m->h_perm.empty()) {
+            // permuted row order: tiles listed by the row of A they start at,
+            // so that the tiles of both colours over one region of A sit
+            // together in the list and fall into the same XCD's range (its
+            // gathers then hit values its own L2 wrote)
+            std::vector<std::pair<int, int>> key(nt);
+            for (int t = 0; t < nt; ++t) key[t] = {m->h_perm[(size_t)t * kTile], t};
+            std::sort(key.begin(), key.end());
+            for (const auto &kt : key) tiles.push_back(make_int2(k, kt.second * kTile));
+        } else {
+            for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(k, t * kTile));
+        }
     }
     b->ntiles = (int)tiles.size();
     bool grew = false;
