@@ -297,6 +297,10 @@ int kcfd_asm_pressure(kcfd_mesh *mesh, kcfd_mat *m, const double *dP,
  * boundary faces), grad (dev, n*ndim, row-major) */
 int kcfd_grad_gg(kcfd_mesh *mesh, const double *phi, const double *phib,
                  double *grad);
+/* least_squares_gradient (core/gradient.py:51-99); arguments as
+ * kcfd_grad_gg; a cell whose normal-equation matrix is singular gets 0 */
+int kcfd_grad_lsq(kcfd_mesh *mesh, const double *phi, const double *phib,
+                  double *grad);
 /* muscl_deferred_correction (core/interpolation.py:227-303); corr (dev n) */
 int kcfd_muscl(kcfd_mesh *mesh, const double *phi, const double *F,
                const double *grad, int limiter, double *corr);

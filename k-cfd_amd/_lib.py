@@ -127,6 +127,7 @@ SIGNATURES = {
     "kcfd_asm_under_relax": ([_P, _P, _D], _I),
     "kcfd_asm_pressure": ([_P, _P, _P, _D, _P], _I),
     "kcfd_grad_gg": ([_P, _P, _P, _P], _I),
+    "kcfd_grad_lsq": ([_P, _P, _P, _P], _I),
     "kcfd_muscl": ([_P, _P, _P, _P, _I, _P], _I),
     "kcfd_mass_flux": ([_P, _P, _P, _D, _P], _I),
     "kcfd_last_kernel_ms": ([_P, ctypes.POINTER(_D)], _I),

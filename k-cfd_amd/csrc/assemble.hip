@@ -335,6 +335,78 @@ __device__ __forceinline__ double limiter(int kind, double r) {
 
 // per internal face: contribution to the owner's correction (the
 // neighbour receives its negation)
+// core/gradient.py:51-99: least-squares gradient.  Per cell, over its faces
+// (face-id order): dx = x_N - x_P, or x_f - x_P on a boundary face; dphi =
+// phi_N - phi_P, or the patch value (owner value if none) - phi_P; then
+// (A^T A) g = A^T dphi solved by LU with partial pivoting (LAPACK gesv's
+// algorithm: first largest |pivot|, row swaps, forward and back
+// substitution).  An exactly zero pivot (LAPACK's singular report) or fewer
+// faces than dimensions leaves g = 0, as the reference does.
+__global__ __launch_bounds__(kBlock) void k_lsq(MeshView m, const double *__restrict__ phi,
+                                                const double *__restrict__ phib,
+                                                double *__restrict__ grad) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    const int nd = m.ndim;
+    double a[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}};
+    double r[3] = {0.0, 0.0, 0.0};
+    const double pc = phi[c];
+    const int q0 = m.cf_ptr[c], q1 = m.cf_ptr[c + 1];
+    for (int q = q0; q < q1; ++q) {
+        const int f = m.cf_face[q];
+        const int o = m.owner[f], nb = m.nb[f];
+        double dx[3], dp;
+        if (nb >= 0) {
+            const int other = m.cf_side[q] > 0 ? nb : o;
+            for (int k = 0; k < 3; ++k) dx[k] = m.cc[3 * other + k] - m.cc[3 * c + k];
+            dp = phi[other] - pc;
+        } else {
+            const int bi = m.bidx[f];
+            for (int k = 0; k < 3; ++k) dx[k] = m.fc[3 * f + k] - m.cc[3 * c + k];
+            dp = (bi >= 0 ? phib[bi] : phi[o]) - pc;
+        }
+        for (int i = 0; i < nd; ++i) {
+            for (int j = 0; j < nd; ++j) a[i][j] += dx[i] * dx[j];
+            r[i] += dx[i] * dp;
+        }
+    }
+    double g[3] = {0.0, 0.0, 0.0};
+    bool ok = q1 - q0 >= nd;
+    // LU with partial pivoting, in place
+    int piv[3] = {0, 1, 2};
+    for (int k = 0; k < nd && ok; ++k) {
+        int p = k;
+        for (int i = k + 1; i < nd; ++i)
+            if (fabs(a[i][k]) > fabs(a[p][k])) p = i;
+        if (a[p][k] == 0.0) { ok = false; break; }
+        if (p != k) {
+            for (int j = 0; j < nd; ++j) {
+                const double t = a[k][j]; a[k][j] = a[p][j]; a[p][j] = t;
+            }
+            const int t = piv[k]; piv[k] = piv[p]; piv[p] = t;
+        }
+        for (int i = k + 1; i < nd; ++i) {
+            a[i][k] = a[i][k] / a[k][k];
+            for (int j = k + 1; j < nd; ++j) a[i][j] -= a[i][k] * a[k][j];
+        }
+    }
+    if (ok) {
+        double y[3];
+        for (int i = 0; i < nd; ++i) {      // L y = P r
+            double s = r[piv[i]];
+            for (int j = 0; j < i; ++j) s -= a[i][j] * y[j];
+            y[i] = s;
+        }
+        for (int i = nd - 1; i >= 0; --i) { // U g = y
+            double s = y[i];
+            for (int j = i + 1; j < nd; ++j) s -= a[i][j] * g[j];
+            g[i] = s / a[i][i];
+        }
+    }
+    for (int k = 0; k < nd; ++k) grad[(size_t)c * nd + k] = g[k];
+}
+
 __global__ __launch_bounds__(kBlock) void k_muscl_face(MeshView m, const double *__restrict__ phi,
                                                        const double *__restrict__ F,
                                                        const double *__restrict__ grad, int lim,
@@ -679,6 +751,15 @@ int kcfd_asm_pressure(kcfd_mesh *mesh, kcfd_mat *a, const double *dP, double rho
     MeshView v = view(mesh);
     hipLaunchKernelGGL(k_press_face, dim3(blocks(mesh->nf)), dim3(kBlock), 0, s, v, dP, rho, mesh->d_kinds, a->val, mesh->fco);
     hipLaunchKernelGGL(k_press_cell, dim3(blocks(mesh->n)), dim3(kBlock), 0, s, v, mesh->fco, a->val, a->diagt);
+    return t.stop();
+}
+
+int kcfd_grad_lsq(kcfd_mesh *mesh, const double *phi, const double *phib, double *grad) {
+    KCFD_ARG(mesh && phi && grad && (phib || mesh->nbf == 0), "null argument");
+    std::lock_guard<std::recursive_mutex> g(mesh->ctx->mu);
+    Timer t(mesh->ctx);
+    hipLaunchKernelGGL(k_lsq, dim3(blocks(mesh->n)), dim3(kBlock), 0, mesh->ctx->stream, view(mesh), phi, phib,
+                       grad);
     return t.stop();
 }
 

@@ -251,6 +251,47 @@ def green_gauss_gradient(mesh, values, bvals: dict):
     return grad
 
 
+def least_squares_gradient(mesh, values, bvals: dict):
+    """gradient.py:51-99: per cell, over its faces, dx = x_N - x_P (x_f - x_P
+    on a boundary face) and dphi = phi_N - phi_P (phi_f - phi_P, the patch
+    value or the owner value); grad = solve(A^T A, A^T dphi); zero where that
+    system is singular or the cell has fewer faces than dimensions.  Faces of
+    a cell are taken in face-id order (the reference's cell face list may
+    order them differently: sums agree to rounding)."""
+    nd = mesh.ndim
+    n = mesh.n_cells
+    values = np.asarray(values, np.float64)
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    internal = nb >= 0
+    xc = mesh.cell_center[:, :nd]
+    pf = boundary_face_values(mesh, values, bvals)
+    # (cell, face) incidences in face order: owner side, then neighbour side
+    cell = np.concatenate([o, nb[internal]])
+    face = np.concatenate([np.arange(mesh.n_faces), np.nonzero(internal)[0]])
+    order = np.lexsort((face, cell))
+    cell, face = cell[order], face[order]
+    other = np.where(internal[face], np.where(o[face] == cell, nb[face], o[face]), -1)
+    dx = np.where(internal[face][:, None], xc[np.maximum(other, 0)], mesh.face_center[face, :nd]) - xc[cell]
+    dphi = np.where(internal[face], values[np.maximum(other, 0)], pf[face]) - values[cell]
+    ata = np.zeros((n, nd, nd))
+    atb = np.zeros((n, nd))
+    np.add.at(ata, cell, dx[:, :, None] * dx[:, None, :])
+    np.add.at(atb, cell, dx * dphi[:, None])
+    cnt = np.bincount(cell, minlength=n)
+    grad = np.zeros((n, nd))
+    ok = cnt >= nd
+    try:
+        grad[ok] = np.linalg.solve(ata[ok], atb[ok][:, :, None])[:, :, 0]
+    except np.linalg.LinAlgError:          # some cell singular: one at a time
+        for c in np.nonzero(ok)[0]:
+            try:
+                grad[c] = np.linalg.solve(ata[c], atb[c])
+            except np.linalg.LinAlgError:
+                pass
+    return grad
+
+
 LIMITERS = {
     "minmod": lambda r: np.maximum(0.0, np.minimum(1.0, r)),
     "superbee": lambda r: np.maximum(0.0, np.maximum(np.minimum(2.0 * r, 1.0), np.minimum(r, 2.0))),

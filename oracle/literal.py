@@ -315,6 +315,33 @@ def green_gauss(m: LMesh, values, bvals):
     return g
 
 
+def least_squares(m: LMesh, values, bvals):
+    """gradient.py:51-99."""
+    nd = m.ndim
+    g = np.zeros((m.n_cells, nd))
+    for ci, c in enumerate(m.cells):
+        xp = c.center[:nd]
+        dxs, dphis = [], []
+        for fid in c.faces:
+            f = m.faces[fid]
+            if f.neighbour >= 0:
+                nbr = f.neighbour if f.owner == ci else f.owner
+                dxs.append(m.cells[nbr].center[:nd] - xp)
+                dphis.append(values[nbr] - values[ci])
+            else:
+                dxs.append(f.center[:nd] - xp)
+                dphis.append(_bvalue(m, values, bvals, fid) - values[ci])
+        if len(dxs) < nd:
+            continue
+        A = np.array(dxs)
+        b = np.array(dphis)
+        try:
+            g[ci] = np.linalg.solve(A.T @ A, A.T @ b)
+        except np.linalg.LinAlgError:
+            pass
+    return g
+
+
 LIMITERS = {
     "minmod": lambda r: np.maximum(0.0, np.minimum(1.0, r)),
     "superbee": lambda r: np.maximum(0.0, np.maximum(np.minimum(2.0 * r, 1.0), np.minimum(r, 2.0))),

@@ -77,6 +77,13 @@ def test_green_gauss_and_muscl(ctx, name):
     eg = _relmax(g, g_o)
     print(f"{name}: GG max rel {eg:.2e}")
     assert eg <= 1e-13 or np.max(np.abs(g - g_o)) <= 1e-13 * np.max(np.abs(g_o))
+    # least squares (core/gradient.py:51-99)
+    l_o = ofvm.least_squares_gradient(mesh, phi, phib)
+    check(lib().kcfd_grad_lsq(dm.h, pd.p, pb.p, gd.p))
+    lg = gd.download().reshape(mesh.n_cells, mesh.ndim)
+    el = np.max(np.abs(lg - l_o)) / np.max(np.abs(l_o))
+    print(f"{name}: least squares max abs/max {el:.2e}")
+    assert el <= 1e-12
     Fd = DevBuf.from_array(ctx, F)
     cd = DevBuf(ctx, mesh.n_cells)
     for lim, code in LIMITERS.items():

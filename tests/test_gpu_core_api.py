@@ -80,6 +80,9 @@ def test_conv_diff_muscl_gradient_and_mixed_solve(ctx):
     grad = core.green_gauss_gradient(phi)
     grad_o = ofvm.green_gauss_gradient(mesh, phi.values, phi.boundary_values)
     assert np.max(np.abs(grad[:, :2] - grad_o[:, :2])) <= 1e-13 * max(1.0, np.max(np.abs(grad_o)))
+    lsq = core.least_squares_gradient(phi)
+    lsq_o = ofvm.least_squares_gradient(mesh, phi.values, phi.boundary_values)
+    assert np.max(np.abs(lsq - lsq_o)) <= 1e-12 * max(1.0, np.max(np.abs(lsq_o)))
     corr = core.muscl_deferred_correction(mesh, phi, F, grad, "van_leer")
     corr_o = ofvm.muscl_deferred_correction(mesh, phi.values, F, grad_o, "van_leer")
     assert np.max(np.abs(corr - corr_o)) <= 1e-13 * max(1.0, np.max(np.abs(corr_o)))

@@ -87,6 +87,32 @@ def test_gradient_muscl_flux_match_literal(name):
                        rtol=1e-14, atol=1e-14)
 
 
+@pytest.mark.parametrize("name", list(MESHES))
+def test_least_squares_matches_literal(name):
+    lm, sm = MESHES[name]()
+    f = _fields(sm, 5)
+    g1 = literal.least_squares(lm, f["phi"], f["phib"])
+    g2 = ofvm.least_squares_gradient(sm, f["phi"], f["phib"])
+    assert np.allclose(g1, g2, rtol=1e-12, atol=1e-12 * np.max(np.abs(g1)))
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_least_squares_linear_exact(name):
+    """Least squares reproduces the gradient of a linear field on any mesh
+    (every cell, boundary cells included, with exact boundary values);
+    grad(const) = 0."""
+    lm, sm = MESHES[name]()
+    nd = sm.ndim
+    a = np.array([0.3, -1.2, 2.5])[:nd]
+    phi = 1.0 + sm.cell_center[:, :nd] @ a
+    phib = {k: 1.0 + sm.face_center[np.asarray(v), :nd] @ a for k, v in sm.boundary_patches.items()}
+    g = ofvm.least_squares_gradient(sm, phi, phib)
+    assert np.allclose(g, a[None, :], atol=1e-10)
+    g0 = ofvm.least_squares_gradient(sm, np.full(sm.n_cells, 3.0),
+                                     {k: np.full(len(v), 3.0) for k, v in sm.boundary_patches.items()})
+    assert np.max(np.abs(g0)) == 0.0
+
+
 def test_uniform_grid_closed_forms_2d():
     """2D uniform h: interior coeff = gamma (A/d = 1), Dirichlet coeff = 2 gamma."""
     lm, sm = _quad(5, 5)
