@@ -293,6 +293,25 @@ int kcfd_asm_under_relax(kcfd_mat *m, const double *phi, double alpha);
  * pdirichlet (host, n_patches).  Writes diag, off-diagonals; rhs untouched. */
 int kcfd_asm_pressure(kcfd_mesh *mesh, kcfd_mat *m, const double *dP,
                       double rho, const int *pdirichlet);
+/* The rest of the SIMPLE pressure-correction step (models/single_phase.py:
+ * 274-365) around the solve, on device arrays:
+ * kcfd_simple_dp: dP = V / max((aP_0 + aP_1 [+ aP_2]) / ndim, 1e-30) from the
+ *   momentum diagonals (:279-283; ap2 ignored in 2D);
+ * kcfd_simple_rhs: rhs += the mass imbalance of the face fluxes mf (dev nf):
+ *   b_O -= F, b_N += F, boundary b_O -= F (:305-311);
+ * kcfd_simple_fix_reference: row 0 := e_0, b_0 = 0, for a system without a
+ *   Dirichlet p' patch (:327-343);
+ * kcfd_simple_correct: grad p' by Green-Gauss with p' = 0 on Dirichlet
+ *   patches (pdirichlet, host, n_patches) and the owner's p' on the other
+ *   boundary faces, then U -= dP grad p' (U dev n*ndim row-major) and
+ *   p += alpha_p p' (:345-363). */
+int kcfd_simple_dp(kcfd_mesh *mesh, const double *ap0, const double *ap1,
+                   const double *ap2, double *dP);
+int kcfd_simple_rhs(kcfd_mesh *mesh, kcfd_mat *m, const double *mf);
+int kcfd_simple_fix_reference(kcfd_mat *m);
+int kcfd_simple_correct(kcfd_mesh *mesh, const double *pp, const double *dP,
+                        const int *pdirichlet, double alpha_p, double *U,
+                        double *p);
 /* green_gauss_gradient (core/gradient.py:12-48); phi (dev n), phib (dev,
  * boundary faces), grad (dev, n*ndim, row-major) */
 int kcfd_grad_gg(kcfd_mesh *mesh, const double *phi, const double *phib,

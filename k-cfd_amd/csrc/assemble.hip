@@ -281,6 +281,87 @@ __global__ __launch_bounds__(kBlock) void k_press_cell(MeshView m, const double 
     diagt[c] += a;
 }
 
+// ------------------------------------------ SIMPLE pressure-correction step
+// models/single_phase.py:274-365 around the solve.
+
+// dP = V / max((aP_0 + aP_1 [+ aP_2]) / ndim, 1e-30) (the momentum diagonals,
+// summed in component order, :279-283)
+__global__ __launch_bounds__(kBlock) void k_simple_dp(int n, int nd, const double *__restrict__ vol,
+                                                      const double *__restrict__ ap0,
+                                                      const double *__restrict__ ap1,
+                                                      const double *__restrict__ ap2,
+                                                      double *__restrict__ dP) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double s = ap0[c] + ap1[c];
+    if (nd == 3) s = s + ap2[c];
+    dP[c] = vol[c] / fmax(s / (double)nd, 1e-30);
+}
+
+// mass imbalance (:305-311): b_O -= F_f, b_N += F_f, boundary b_O -= F_f, in
+// face order per cell
+__global__ __launch_bounds__(kBlock) void k_simple_rhs(MeshView m, const double *__restrict__ mf,
+                                                       double *__restrict__ rhs) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    double b = rhs[c];
+    for (int q = m.cf_ptr[c]; q < m.cf_ptr[c + 1]; ++q) {
+        const double F = mf[m.cf_face[q]];
+        if (m.cf_side[q] > 0) b -= F;
+        else b += F;
+    }
+    rhs[c] = b;
+}
+
+// no Dirichlet p' patch (:327-343): row 0's entries become 0, then the
+// appended (0, 0, 1.0) entry lands on its diagonal; b_0 = 0
+__global__ void k_simple_fix_row0(const int *__restrict__ ip, const int *__restrict__ dpos,
+                                  double *__restrict__ vals, double *__restrict__ rhs) {
+    const int q = ip[0] + (int)threadIdx.x;
+    if (q < ip[1]) vals[q] = q == dpos[0] ? 0.0 + 1.0 : 0.0;
+    if (threadIdx.x == 0) rhs[0] = 0.0;
+}
+
+// velocity and pressure correction (:345-363): grad p' by Green-Gauss with
+// the boundary values 0 on Dirichlet p' patches and the owner's p' elsewhere
+// (faces in no patch take the owner value too), then U -= dP grad p' per
+// component and p += alpha_p p'.  U is (n, ndim) row-major.
+__global__ __launch_bounds__(kBlock) void k_simple_correct(MeshView m, const double *__restrict__ pp,
+                                                           const double *__restrict__ dP,
+                                                           const int *__restrict__ pdir, double alpha_p,
+                                                           double *__restrict__ U, double *__restrict__ p) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    const int nd = m.ndim;
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int q = m.cf_ptr[c]; q < m.cf_ptr[c + 1]; ++q) {
+        const int f = m.cf_face[q];
+        const int o = m.owner[f], nb = m.nb[f];
+        double pf;
+        if (nb >= 0) {
+            const double w = m.gw[f];
+            pf = w * pp[o] + (1.0 - w) * pp[nb];
+        } else {
+            const int pt = m.bpatch[f];
+            pf = (pt >= 0 && pdir[pt]) ? 0.0 : pp[o];
+        }
+        const double A = m.area[f];
+        for (int k = 0; k < 3; ++k) {
+            const double fl = pf * m.normal[3 * f + k] * A;
+            if (m.cf_side[q] > 0) g[k] += fl;
+            else g[k] -= fl;
+        }
+    }
+    const double V = m.vol[c];
+    if (V > 1e-30)
+        for (int k = 0; k < 3; ++k) g[k] /= V;
+    for (int k = 0; k < nd; ++k) U[(size_t)c * nd + k] -= dP[c] * g[k];
+    p[c] += alpha_p * pp[c];
+}
+
 // ------------------------------------------------------- Green-Gauss
 // core/gradient.py:12-48
 __global__ __launch_bounds__(kBlock) void k_gg(MeshView m, const double *__restrict__ phi,
@@ -760,6 +841,54 @@ int kcfd_grad_lsq(kcfd_mesh *mesh, const double *phi, const double *phib, double
     Timer t(mesh->ctx);
     hipLaunchKernelGGL(k_lsq, dim3(blocks(mesh->n)), dim3(kBlock), 0, mesh->ctx->stream, view(mesh), phi, phib,
                        grad);
+    return t.stop();
+}
+
+int kcfd_simple_dp(kcfd_mesh *mesh, const double *ap0, const double *ap1, const double *ap2, double *dP) {
+    KCFD_ARG(mesh && ap0 && ap1 && dP && (mesh->ndim == 2 || ap2), "null argument");
+    std::lock_guard<std::recursive_mutex> g(mesh->ctx->mu);
+    Timer t(mesh->ctx);
+    hipLaunchKernelGGL(k_simple_dp, dim3(blocks(mesh->n)), dim3(kBlock), 0, mesh->ctx->stream, mesh->n,
+                       mesh->ndim, mesh->vol, ap0, ap1, mesh->ndim == 3 ? ap2 : ap0, dP);
+    return t.stop();
+}
+
+int kcfd_simple_rhs(kcfd_mesh *mesh, kcfd_mat *a, const double *mf) {
+    CHECK_PAIR(mesh, a);
+    KCFD_ARG(mf != nullptr, "null mass flux");
+    std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    Timer t(a->ctx);
+    hipLaunchKernelGGL(k_simple_rhs, dim3(blocks(mesh->n)), dim3(kBlock), 0, a->ctx->stream, view(mesh), mf,
+                       a->rhs);
+    return t.stop();
+}
+
+int kcfd_simple_fix_reference(kcfd_mat *a) {
+    KCFD_ARG(a != nullptr, "null matrix");
+    KCFD_ARG(a->n > 0 && !a->h_diagpos.empty() && a->h_diagpos[0] >= 0, "row 0 has no diagonal entry");
+    std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    const int len = a->h_indptr[1] - a->h_indptr[0];
+    KCFD_ARG(len <= 1024, "row 0 longer than 1024 entries");
+    Timer t(a->ctx);
+    hipLaunchKernelGGL(k_simple_fix_row0, dim3(1), dim3(1024), 0, a->ctx->stream, a->indptr, a->diagpos, a->val,
+                       a->rhs);
+    return t.stop();
+}
+
+int kcfd_simple_correct(kcfd_mesh *mesh, const double *pp, const double *dP, const int *pdirichlet,
+                        double alpha_p, double *U, double *p) {
+    KCFD_ARG(mesh && pp && dP && U && p, "null argument");
+    std::lock_guard<std::recursive_mutex> g(mesh->ctx->mu);
+    hipStream_t s = mesh->ctx->stream;
+    if (mesh->npatch) {
+        std::vector<int> pd(mesh->npatch, 0);
+        if (pdirichlet) pd.assign(pdirichlet, pdirichlet + mesh->npatch);
+        KCFD_HIP(hipMemcpyAsync(mesh->d_kinds, pd.data(), sizeof(int) * mesh->npatch, hipMemcpyHostToDevice, s));
+        KCFD_HIP(hipStreamSynchronize(s));
+    }
+    Timer t(mesh->ctx);
+    hipLaunchKernelGGL(k_simple_correct, dim3(blocks(mesh->n)), dim3(kBlock), 0, s, view(mesh), pp, dP,
+                       mesh->d_kinds, alpha_p, U, p);
     return t.stop();
 }
 

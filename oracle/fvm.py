@@ -292,6 +292,93 @@ def least_squares_gradient(mesh, values, bvals: dict):
     return grad
 
 
+def pressure_correction_operator(mesh, dP, rho, dirichlet, s):
+    """models/single_phase.py:288-325 coefficients (COO order of the
+    reference: off-diagonals face by face, then one diagonal entry per cell);
+    no right-hand side (see simple_pressure_correction_step)."""
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    n = mesh.n_cells
+    aP = np.zeros(n)
+    rows, cols, vals = [], [], []
+    dirset = set(dirichlet)
+    bname = {}
+    for nm, f in mesh.boundary_patches.items():
+        for ff in f:
+            bname[int(ff)] = nm
+    cc = mesh.cell_center
+    for f in range(mesh.n_faces):
+        if nb[f] >= 0:
+            d = np.linalg.norm(cc[nb[f]] - cc[o[f]])
+            if d < 1e-30:
+                continue
+            dv = cc[nb[f]] - cc[o[f]]
+            dorth = abs(np.dot(dv, mesh.face_normal[f]))
+            dorth = max(dorth, 0.1 * d)
+            df = rho * 0.5 * (dP[o[f]] + dP[nb[f]]) * mesh.face_area[f] / dorth
+            aP[o[f]] += df
+            aP[nb[f]] += df
+            rows += [o[f], nb[f]]
+            cols += [nb[f], o[f]]
+            vals += [-df, -df]
+        elif bname.get(f) in dirset:
+            d = np.linalg.norm(mesh.face_center[f] - cc[o[f]])
+            if d > 1e-30:
+                aP[o[f]] += rho * dP[o[f]] * mesh.face_area[f] / d
+    s.add_entries(np.array(rows, np.int64), np.array(cols, np.int64), np.array(vals))
+    idx = np.arange(n)
+    s.add_entries(idx, idx, aP)
+
+
+def simple_pressure_correction_step(mesh, aP, U, p, mf, rho, alpha_p, dirichlet):
+    """models/single_phase.py:274-365, the SIMPLE pressure-correction step:
+    dP = V / max(mean momentum a_P, 1e-30) (:279-283); the coefficients of
+    pressure_correction_operator and the mass imbalance b_O -= F, b_N += F
+    (boundary b_O -= F) (:305-311); without a Dirichlet p' patch, row 0 := e_0
+    and b_0 = 0 (:327-343); p' by scipy spsolve (:345-347); grad p' by
+    Green-Gauss with p' = 0 on Dirichlet patches and the owner's p' on the
+    other boundary faces; U -= dP grad p', p += alpha_p p' (:349-363).
+    aP: list of ndim (n,) momentum diagonals; U (n, ndim); mf (n_faces,).
+    Returns (p', U_new, p_new, A, b, dP)."""
+    from scipy.sparse.linalg import spsolve
+    nd = mesh.ndim
+    n = mesh.n_cells
+    s_ap = 0.0
+    for c in range(nd):
+        s_ap = s_ap + np.asarray(aP[c], np.float64)
+    dP = mesh.cell_volume / np.maximum(s_ap / nd, 1e-30)
+    s = OSystem(n)
+    pressure_correction_operator(mesh, dP, rho, dirichlet, s)
+    A = s.to_sparse().tocsr()
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    internal = nb >= 0
+    b = np.zeros(n)
+    # face order: b_O -= F (all faces), b_N += F (internal faces)
+    idx = np.stack([o, np.where(internal, nb, o)], 1)
+    val = np.stack([-mf, mf], 1)
+    msk = np.stack([np.ones(len(o), bool), internal], 1)
+    np.add.at(b, idx[msk], val[msk])
+    if not any(nm in dirichlet for nm in mesh.boundary_patches):
+        # row 0's COO values become 0 (explicit zeros stay in the CSR) and
+        # the appended (0, 0, 1.0) entry sums onto its diagonal
+        q0, q1 = A.indptr[0], A.indptr[1]
+        A.data[q0:q1] = 0.0
+        A.data[q0 + int(np.nonzero(A.indices[q0:q1] == 0)[0][0])] = 0.0 + 1.0
+        b[0] = 0.0
+    pp = spsolve(A.tocsc(), b)
+    if np.any(np.isnan(pp)):
+        pp = np.zeros(n)
+    bv = {nm: (np.zeros(len(f)) if nm in dirichlet else pp[mesh.owner[np.asarray(f)]])
+          for nm, f in mesh.boundary_patches.items()}
+    g = green_gauss_gradient(mesh, pp, bv)
+    U_new = np.array(U, np.float64, copy=True)
+    for comp in range(nd):
+        U_new[:, comp] = U_new[:, comp] - dP * g[:, comp]
+    p_new = np.asarray(p, np.float64) + alpha_p * pp
+    return pp, U_new, p_new, A, b, dP
+
+
 LIMITERS = {
     "minmod": lambda r: np.maximum(0.0, np.minimum(1.0, r)),
     "superbee": lambda r: np.maximum(0.0, np.maximum(np.minimum(2.0 * r, 1.0), np.minimum(r, 2.0))),

@@ -49,40 +49,8 @@ def oracle_system(case):
 
 
 def pressure_correction(mesh, dP, rho, dirichlet, s):
-    """models/single_phase.py:288-344 coefficients (COO order of the
-    reference: off-diagonals face by face, then one diagonal entry per cell)."""
-    o = mesh.owner.astype(np.int64)
-    nb = mesh.neighbour.astype(np.int64)
-    n = mesh.n_cells
-    aP = np.zeros(n)
-    rows, cols, vals = [], [], []
-    dirset = set(dirichlet)
-    bname = {}
-    for nm, f in mesh.boundary_patches.items():
-        for ff in f:
-            bname[int(ff)] = nm
-    cc = mesh.cell_center
-    for f in range(mesh.n_faces):
-        if nb[f] >= 0:
-            d = np.linalg.norm(cc[nb[f]] - cc[o[f]])
-            if d < 1e-30:
-                continue
-            dv = cc[nb[f]] - cc[o[f]]
-            dorth = abs(np.dot(dv, mesh.face_normal[f]))
-            dorth = max(dorth, 0.1 * d)
-            df = rho * 0.5 * (dP[o[f]] + dP[nb[f]]) * mesh.face_area[f] / dorth
-            aP[o[f]] += df
-            aP[nb[f]] += df
-            rows += [o[f], nb[f]]
-            cols += [nb[f], o[f]]
-            vals += [-df, -df]
-        elif bname.get(f) in dirset:
-            d = np.linalg.norm(mesh.face_center[f] - cc[o[f]])
-            if d > 1e-30:
-                aP[o[f]] += rho * dP[o[f]] * mesh.face_area[f] / d
-    s.add_entries(np.array(rows, np.int64), np.array(cols, np.int64), np.array(vals))
-    idx = np.arange(n)
-    s.add_entries(idx, idx, aP)
+    """The oracle's pressure-correction assembly (oracle/fvm.py)."""
+    ofvm.pressure_correction_operator(mesh, dP, rho, dirichlet, s)
 
 
 def rel_l2(a, b):

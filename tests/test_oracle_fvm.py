@@ -209,3 +209,28 @@ def test_drag_closed_form():
     assert np.isclose(cd[0], 24.0 / 1e-10 * (1 + 0.15 * 1e-10 ** 0.687))
     assert np.isclose(cd[1], 24 * 1.15)
     assert np.isclose(cd[4], 0.44) and np.isclose(cd[5], 0.44)
+
+
+def test_simple_correction_closed_form():
+    """The oracle's SIMPLE step: with p' linear (a Dirichlet-free check of the
+    correction alone), interior cells of a uniform grid get U -= dP a and
+    p += alpha p' exactly; the step's p' solves A p' = b."""
+    lm, sm = _hex(5)
+    rng = np.random.default_rng(2)
+    n, nd = sm.n_cells, 3
+    aP = [np.exp(rng.normal(0, 0.3, n)) for _ in range(nd)]
+    U = rng.normal(0, 1, (n, nd))
+    p = rng.normal(0, 1, n)
+    mf = rng.normal(0, 0.1, sm.n_faces)
+    names = list(sm.boundary_patches)
+    pp, U1, p1, A, b, dP = ofvm.simple_pressure_correction_step(sm, aP, U, p, mf, 1.0, 0.4, names[:1])
+    assert np.linalg.norm(A @ pp - b) <= 1e-10 * np.linalg.norm(b)
+    assert np.allclose(p1, p + 0.4 * pp, rtol=0, atol=1e-15)
+    assert np.allclose(dP, sm.cell_volume / np.maximum((aP[0] + aP[1] + aP[2]) / 3, 1e-30), rtol=1e-15)
+    # the correction alone, on a linear p': exact at cells away from the boundary
+    a = np.array([0.7, -0.2, 1.3])
+    ppl = sm.cell_center @ a
+    bv = {nm: ppl[sm.owner[np.asarray(f)]] for nm, f in sm.boundary_patches.items()}
+    g = ofvm.green_gauss_gradient(sm, ppl, bv)
+    inner = np.all((sm.cell_center > 0.2 + 1e-9) & (sm.cell_center < 0.8 - 1e-9), axis=1)
+    assert inner.sum() > 0 and np.allclose(g[inner], a[None, :], atol=1e-12)
