@@ -55,6 +55,9 @@ extern "C" {
 #define KCFD_METHOD_BICGSTAB 0  /* scipy bicgstab: solve_linear_system (core/linear_solver.py:58-62) */
 #define KCFD_METHOD_CG 1        /* scipy cg: solve_pressure_correction (core/linear_solver.py:88-93);
                                    FP64 only */
+#define KCFD_METHOD_GMRES 2     /* scipy gmres, restarted, left-preconditioned:
+                                   solve_linear_system(method='gmres') (core/linear_solver.py:64-68);
+                                   FP64, kcfd_solve only; maxiter counts restart cycles */
 
 /* boundary-condition kinds (core/fvm_operators.py:178-225) */
 #define KCFD_BC_ZERO_GRADIENT 0
@@ -171,13 +174,15 @@ typedef struct kcfd_solve_opts {
                            j+J, ...), 2 one contiguous run per workgroup
                            (0: library default) */
     int method;         /* KCFD_METHOD_* (0: BiCGSTAB) */
+    int restart;        /* GMRES: Krylov vectors per cycle, 1..64 (0: 20, scipy's) */
 } kcfd_solve_opts;
 
 typedef struct kcfd_solve_result {
     int info;           /* 0 converged, >0 maxiter, -10/-11 breakdown */
-    int iters;          /* completed full BiCGSTAB iterations (scipy callback count) */
+    int iters;          /* completed full BiCGSTAB / CG iterations (scipy callback count);
+                           GMRES: inner iterations of all cycles */
     int half_step;      /* 1 if it stopped at the half-step test on ||s|| */
-    int refinements;    /* MIXED: outer FP64 refinement steps */
+    int refinements;    /* MIXED: outer FP64 refinement steps; GMRES: restart cycles */
     double rel_resid;   /* FP64 true relative residual ||b-Ax||/||b|| */
     double bytes;       /* algorithmic HBM bytes of the solve (byte model) */
     double ms;          /* device time of the solve */
@@ -185,7 +190,9 @@ typedef struct kcfd_solve_result {
 
 /* Replaces solve_linear_system(method='bicgstab') (core/linear_solver.py:
  * 14-62) + scipy bicgstab (iterative.py:157-302) and gpu_bicgstab
- * (core/gpu_solver.py:82-155).  b, x (dev, n); x holds x0 on entry. */
+ * (core/gpu_solver.py:82-155); with opts->method, scipy cg
+ * (solve_pressure_correction, core/linear_solver.py:80-93) and scipy gmres
+ * (core/linear_solver.py:64-68).  b, x (dev, n); x holds x0 on entry. */
 int kcfd_solve(kcfd_mat *m, const double *b, double *x,
                const kcfd_solve_opts *opts, kcfd_solve_result *res);
 

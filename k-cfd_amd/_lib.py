@@ -20,8 +20,8 @@ KCFD_OK = 0
 KCFD_EARG, KCFD_EHIP, KCFD_EOOM, KCFD_ESTATE, KCFD_EPIVOT, KCFD_EBUSY = -1, -2, -3, -4, -5, -6
 PC_NONE, PC_JACOBI, PC_ILU0 = 0, 1, 2
 FP64, MIXED = 0, 1
-METHOD_BICGSTAB, METHOD_CG = 0, 1
-METHODS = {"bicgstab": METHOD_BICGSTAB, "cg": METHOD_CG}
+METHOD_BICGSTAB, METHOD_CG, METHOD_GMRES = 0, 1, 2
+METHODS = {"bicgstab": METHOD_BICGSTAB, "cg": METHOD_CG, "gmres": METHOD_GMRES}
 BC_ZERO_GRADIENT, BC_DIRICHLET, BC_NEUMANN, BC_NONE = 0, 1, 2, 3
 LIMITERS = {"minmod": 0, "superbee": 1, "van_albada": 2, "van_leer": 3}
 PRECONDS = {"none": PC_NONE, "jacobi": PC_JACOBI, "ilu0": PC_ILU0}
@@ -44,7 +44,7 @@ class SolveOpts(ctypes.Structure):
                 ("inner_rtol", ctypes.c_double), ("max_refine", ctypes.c_int),
                 ("check_every", ctypes.c_int), ("grid_per_cu", ctypes.c_int),
                 ("trsv_per_cu", ctypes.c_int), ("tile_walk", ctypes.c_int),
-                ("method", ctypes.c_int)]
+                ("method", ctypes.c_int), ("restart", ctypes.c_int)]
 
 
 class SolveResult(ctypes.Structure):

@@ -16,7 +16,13 @@ silently (linear_solver.py:60-62).
 'auto' never tried its GPU branch; there is one implementation here).
 'cpu' is not provided by this package.  ``method='direct'`` is the
 reference's scipy ``spsolve`` call, kept as is: it is not part of the GPU
-path.  'gmres' has no GPU implementation.
+path.
+
+``method='gmres'`` runs the device GMRES (KCFD_METHOD_GMRES, FP64) with the
+reference's call ``gmres(A, b, x0=x0, atol=tol, maxiter=maxiter, M=M)``
+(linear_solver.py:64-68): scipy's restart 20 (``restart=`` keyword to
+change it), ``maxiter`` counting restart cycles, rtol 1e-5 unless given, and
+the reference's message on a non-converged solve.
 """
 
 from __future__ import annotations
@@ -58,18 +64,19 @@ def solve_linear_system(system: FVMSystem, x0: np.ndarray = None, method: str = 
         raise NotImplementedError("iluk has no GPU implementation; use 'ilu0'")
     if preconditioner not in _PCS:
         raise ValueError(f"Unknown preconditioner: {preconditioner}")
-    if method == "gmres":
-        raise NotImplementedError("gmres has no GPU implementation in this package")
-    if method != "bicgstab":
+    if method not in ("bicgstab", "gmres"):
         raise ValueError(f"Unknown solver method: {method}")
     rtol = float(precond_kwargs.get("rtol", 1e-5))
     precision = precond_kwargs.get("precision", "fp64")
+    if method == "gmres" and precision != "fp64":
+        raise ValueError("gmres runs in FP64 only")
     return_info = bool(precond_kwargs.get("return_info", False))
     A, b_ptr, keep = _device_system(system)
     ctx = A.ctx
     x = L.DevBuf.from_array(ctx, np.zeros(system.n) if x0 is None else np.asarray(x0, dtype=np.float64))
     opts = default_opts(rtol=rtol, atol=tol, maxiter=maxiter, precond=preconditioner, precision=precision,
-                        inner_rtol=float(precond_kwargs.get("inner_rtol", 0.1)))
+                        inner_rtol=float(precond_kwargs.get("inner_rtol", 0.1)), method=method,
+                        restart=int(precond_kwargs.get("restart", 0) or 0))
     # preconditioner from the current values (as the reference rebuilds it on
     # every call, core/linear_solver.py:56); order='color' selects the
     # coloured ILU(0) row ordering
@@ -77,7 +84,7 @@ def solve_linear_system(system: FVMSystem, x0: np.ndarray = None, method: str = 
     res = A.solve_dev(b_ptr, x.p, opts, setup=False)
     out = x.download()
     if res.info > 0:
-        print(f"[bicgstab] 수렴 실패: {res.info} 반복 후 중단")
+        print(f"[{method}] 수렴 실패: {res.info} 반복 후 중단")
     if return_info:
         return out, res.as_dict()
     return out

@@ -1622,7 +1622,9 @@ int check_opts(const kcfd_solve_opts *o) {
     KCFD_ARG(o->grid_per_cu >= 0 && o->grid_per_cu <= 8, "grid_per_cu must be 0..8");
     KCFD_ARG(o->trsv_per_cu >= 0 && o->trsv_per_cu <= 4, "trsv_per_cu must be 0..4");
     KCFD_ARG(o->tile_walk >= 0 && o->tile_walk <= 2, "tile_walk must be 0..2");
-    KCFD_ARG(o->method == KCFD_METHOD_BICGSTAB || o->method == KCFD_METHOD_CG, "unknown Krylov method");
+    KCFD_ARG(o->method == KCFD_METHOD_BICGSTAB || o->method == KCFD_METHOD_CG,
+             o->method == KCFD_METHOD_GMRES ? "GMRES runs through kcfd_solve only (no batches)"
+                                            : "unknown Krylov method");
     KCFD_ARG(o->method != KCFD_METHOD_CG || prec == KCFD_FP64, "CG runs in FP64 only");
     return KCFD_OK;
 }
@@ -2355,6 +2357,7 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
         if (res) std::memset(res, 0, sizeof(*res));
         return KCFD_OK;
     }
+    if (opts->method == KCFD_METHOD_GMRES) return gmres_solve(m, bvec, x, opts, res);
     // the cached single-system workspace is reused while the preconditioner,
     // precision, launch shape and the b / x buffers are unchanged
     bool reuse = m->solver != nullptr && m->solver->bp.precond == opts->precond &&

@@ -125,6 +125,9 @@ struct kcfd_ctx {
 };
 
 struct kcfd_batch;
+namespace kcfd {
+struct GmresWs;
+}
 
 struct kcfd_mat {
     kcfd_ctx *ctx = nullptr;
@@ -183,8 +186,9 @@ struct kcfd_mat {
     // MIXED: {column, FP32 value bits} per chunk entry (one 8-byte load)
     int2 *chl_pk32 = nullptr, *chu_pk32 = nullptr;
     float *ud32 = nullptr;
-    // cached single-system solver workspace
+    // cached single-system solver workspaces (BiCGSTAB / CG batch, GMRES)
     kcfd_batch *solver = nullptr;
+    kcfd::GmresWs *gmres = nullptr;
     int solver_key[4] = {-1, -1, -1, -1};
 };
 
@@ -194,6 +198,11 @@ int ilu0_structure(kcfd_mat *m);
 int ilu0_setup(kcfd_mat *m, bool want32);
 int ensure_sell(kcfd_mat *m, bool permuted);
 void ilu_free_order(kcfd_mat *m);
+int precond_apply_async(kcfd_mat *m, const double *x, double *y, double *t, bool porder,
+                        const int *skip, int col);
+int gmres_solve(kcfd_mat *m, const double *b, double *x, const kcfd_solve_opts *o,
+                kcfd_solve_result *res);
+void gmres_free(kcfd_mat *m);
 int mat_free_precond(kcfd_mat *m);
 int mat_free_ilu_structure(kcfd_mat *m);
 }  // namespace kcfd

@@ -421,7 +421,9 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
 // One level of the ILU(0) apply, rows i (positions in P A P^T) of that level.
 //   lower: t_i = z_{perm i} - sum_{j<i} l_ij t_j          (t in P-order)
 //   upper: w_{perm i} = (t_i - sum_{j>i} u_ij w_{perm j}) / u_ii  (w in A's order)
-// perm null is the identity; uix[q] = perm[ix[q]].
+// perm null is the identity; uix[q] = perm[ix[q]] (with perm null and
+// uix = ix, z and w are both in P-order).  The launch is a no-op once
+// *skip < col (a caller's stop flag; skip null: never).
 __global__ __launch_bounds__(kBlock) void k_trsv_level(const int *__restrict__ rows, int count,
                                                        const int *__restrict__ ip,
                                                        const int *__restrict__ ix,
@@ -430,10 +432,10 @@ __global__ __launch_bounds__(kBlock) void k_trsv_level(const int *__restrict__ r
                                                        const int *__restrict__ perm,
                                                        const double *__restrict__ lu,
                                                        const double *z, double *t, double *w,
-                                                       int upper) {
+                                                       int upper, const int *skip, int col) {
 #pragma clang fp contract(off)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
+    if (k >= count || (skip && *skip < col)) return;
     const int i = rows[k];
     const int r = perm ? perm[i] : i;
     const int dp = dpos[i];
@@ -449,15 +451,50 @@ __global__ __launch_bounds__(kBlock) void k_trsv_level(const int *__restrict__ r
 }
 
 __global__ void k_jacobi_apply(int n, const double *__restrict__ dinv, const double *__restrict__ x,
-                               double *__restrict__ y) {
+                               double *__restrict__ y, const int *skip, int col) {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = dinv[i] * x[i];
+    if (i < n && !(skip && *skip < col)) y[i] = dinv[i] * x[i];
 }
 
-__global__ void k_copy(int n, const double *__restrict__ x, double *__restrict__ y) {
+__global__ void k_copy(int n, const double *__restrict__ x, double *__restrict__ y, const int *skip,
+                       int col) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) y[i] = x[i];
+    if (i < n && !(skip && *skip < col)) y[i] = x[i];
+}
+
+// y = M^{-1} x on the library stream, no host synchronisation.  porder: x
+// and y are in the ILU(0) row order (positions of P A P^T) rather than A's;
+// t: n doubles of scratch (ILU(0)).
+int precond_apply_async(kcfd_mat *m, const double *x, double *y, double *t, bool porder,
+                        const int *skip, int col) {
+    hipStream_t s = m->ctx->stream;
+    const int n = m->n;
+    if (!n) return KCFD_OK;
+    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+    if (m->pc_kind == KCFD_PC_JACOBI) {
+        hipLaunchKernelGGL(k_jacobi_apply, dim3(nb), dim3(kBlock), 0, s, n, m->dinv, x, y, skip, col);
+    } else if (m->pc_kind == KCFD_PC_ILU0) {
+        // level by level: t = L^{-1} P x, then y = P^T U^{-1} t
+        const int *perm = porder ? nullptr : m->perm;
+        const int *uix = porder ? m->f_indices : m->f_uix;
+        for (int l = 0; l < m->nlev_l; ++l) {
+            const int c = m->h_lvl_ptr_l[l + 1] - m->h_lvl_ptr_l[l];
+            hipLaunchKernelGGL(k_trsv_level, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                               m->lvl_rows_l + m->h_lvl_ptr_l[l], c, m->f_indptr, m->f_indices, uix,
+                               m->f_diagpos, perm, m->lu, x, t, y, 0, skip, col);
+        }
+        for (int l = 0; l < m->nlev_u; ++l) {
+            const int c = m->h_lvl_ptr_u[l + 1] - m->h_lvl_ptr_u[l];
+            hipLaunchKernelGGL(k_trsv_level, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                               m->lvl_rows_u + m->h_lvl_ptr_u[l], c, m->f_indptr, m->f_indices, uix,
+                               m->f_diagpos, perm, m->lu, x, t, y, 1, skip, col);
+        }
+    } else {
+        hipLaunchKernelGGL(k_copy, dim3(nb), dim3(kBlock), 0, s, n, x, y, skip, col);
+    }
+    KCFD_HIP(hipGetLastError());
+    return KCFD_OK;
 }
 
 }  // namespace kcfd
@@ -469,33 +506,15 @@ extern "C" int kcfd_precond_apply(kcfd_mat *m, const double *x, double *y) {
     KCFD_ARG(x != y, "x and y must not alias");
     std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
     KCFD_HIP(hipSetDevice(m->ctx->device));
-    hipStream_t s = m->ctx->stream;
     const int n = m->n;
     if (!n) return KCFD_OK;
-    const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
-    if (m->pc_kind == KCFD_PC_JACOBI) {
-        hipLaunchKernelGGL(k_jacobi_apply, dim3(nb), dim3(kBlock), 0, s, n, m->dinv, x, y);
-    } else if (m->pc_kind == KCFD_PC_ILU0) {
-        // level by level: t = L^{-1} P x, then y = P^T U^{-1} t
+    double *t = nullptr;
+    if (m->pc_kind == KCFD_PC_ILU0) {
         KCFD_CHECK(ensure_scratch(m->ctx, sizeof(double) * n));
-        double *t = (double *)m->ctx->scratch;
-        for (int l = 0; l < m->nlev_l; ++l) {
-            const int c = m->h_lvl_ptr_l[l + 1] - m->h_lvl_ptr_l[l];
-            hipLaunchKernelGGL(k_trsv_level, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                               m->lvl_rows_l + m->h_lvl_ptr_l[l], c, m->f_indptr, m->f_indices,
-                               m->f_uix, m->f_diagpos, m->perm, m->lu, x, t, y, 0);
-        }
-        for (int l = 0; l < m->nlev_u; ++l) {
-            const int c = m->h_lvl_ptr_u[l + 1] - m->h_lvl_ptr_u[l];
-            hipLaunchKernelGGL(k_trsv_level, dim3((c + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
-                               m->lvl_rows_u + m->h_lvl_ptr_u[l], c, m->f_indptr, m->f_indices,
-                               m->f_uix, m->f_diagpos, m->perm, m->lu, x, t, y, 1);
-        }
-    } else {
-        hipLaunchKernelGGL(k_copy, dim3(nb), dim3(kBlock), 0, s, n, x, y);
+        t = (double *)m->ctx->scratch;
     }
-    KCFD_HIP(hipGetLastError());
-    KCFD_HIP(hipStreamSynchronize(s));
+    KCFD_CHECK(precond_apply_async(m, x, y, t, false, nullptr, 0));
+    KCFD_HIP(hipStreamSynchronize(m->ctx->stream));
     return KCFD_OK;
 }
 

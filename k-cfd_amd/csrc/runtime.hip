@@ -446,6 +446,7 @@ int kcfd_mat_destroy(kcfd_mat *m) {
     hipSetDevice(m->ctx->device);
     hipStreamSynchronize(m->ctx->stream);
     if (m->solver) kcfd_batch_destroy(m->solver);
+    gmres_free(m);
     mat_free_precond(m);
     mat_free_ilu_structure(m);
     hipFree(m->indptr);

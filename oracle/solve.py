@@ -5,6 +5,11 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
 * ``solve_cg`` calls ``scipy.sparse.linalg.cg`` as ``solve_pressure_correction``
   does (core/linear_solver.py:88-93), with explicit ``rtol``/``atol`` and an
   iteration-counting callback.
+* ``solve_gmres`` calls ``scipy.sparse.linalg.gmres`` as ``solve_linear_system``
+  does for method='gmres' (core/linear_solver.py:64-68: restart 20, maxiter
+  counting restart cycles), with explicit ``rtol``/``atol``; it counts inner
+  iterations with a 'pr_norm' callback and restart cycles with an 'x'
+  callback (two runs of the same deterministic solve).
 * ``solve_bicgstab`` calls ``scipy.sparse.linalg.bicgstab`` exactly as
   ``solve_linear_system`` does (core/linear_solver.py:56-62) but with explicit
   ``rtol``/``atol`` and an iteration-counting callback (the convention of
@@ -24,7 +29,7 @@ import subprocess
 
 import numpy as np
 from scipy import sparse
-from scipy.sparse.linalg import LinearOperator, bicgstab, cg, spilu
+from scipy.sparse.linalg import LinearOperator, bicgstab, cg, gmres, spilu
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
@@ -209,6 +214,154 @@ def solve_cg(A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=None, precond="jacobi",
     x, info = cg(A, b, x0=np.array(x0, np.float64), rtol=rtol, atol=atol, maxiter=maxiter, M=M,
                  callback=cb)
     return x, info, count[0]
+
+
+def solve_gmres(A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=None, precond="none", M=None,
+                restart=None, counts=False):
+    """scipy GMRES with the reference's call (core/linear_solver.py:64-68);
+    returns (x, info, inner iterations) and, with counts=True, also the
+    per-iteration preconditioned residual estimates and the restart cycles."""
+    A = sparse.csr_matrix(A)
+    n = A.shape[0]
+    if x0 is None:
+        x0 = np.zeros(n)
+    if M is None:
+        M = make_preconditioner(A, precond)
+    pr = []
+    x, info = gmres(A, b, x0=np.array(x0, np.float64), rtol=rtol, atol=atol, restart=restart,
+                    maxiter=maxiter, M=M, callback=pr.append, callback_type="pr_norm")
+    if not counts:
+        return x, info, len(pr)
+    cyc = [0]
+
+    def cb(_xk):
+        cyc[0] += 1
+
+    x2, _ = gmres(A, b, x0=np.array(x0, np.float64), rtol=rtol, atol=atol, restart=restart,
+                  maxiter=maxiter, M=M, callback=cb, callback_type="x")
+    assert np.array_equal(x, x2)
+    return x, info, len(pr), np.array(pr), cyc[0]
+
+
+def lartg(f, g):
+    """LAPACK xLARTG, the safe-scaling algorithm (Anderson, ACM TOMS 44(1)
+    2017, Algorithm 978) that scipy's gmres calls: (c, s, r) with c >= 0 and
+    r carrying the sign of f.  The device GMRES implements the same steps."""
+    safmin = np.finfo(np.float64).tiny
+    safmax = 1.0 / safmin
+    rtmin, rtmax = np.sqrt(safmin), np.sqrt(safmax / 2)
+    f, g = np.float64(f), np.float64(g)
+    f1, g1 = abs(f), abs(g)
+    if g == 0:
+        return 1.0, 0.0, f
+    if f == 0:
+        return 0.0, np.copysign(1.0, g), g1
+    if rtmin < f1 < rtmax and rtmin < g1 < rtmax:
+        d = np.sqrt(f * f + g * g)
+        r = np.copysign(d, f)
+        return f1 / d, g / r, r
+    u = min(safmax, max(safmin, f1, g1))
+    fs, gs = f / u, g / u
+    d = np.sqrt(fs * fs + gs * gs)
+    r = np.copysign(d, f)
+    return abs(fs) / d, gs / r, r * u
+
+
+def gmres_restated(A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=None, precond="none", restart=20,
+                   dot=None, xupdate="blas", M=None):
+    """GMRES(restart) step by step as the device runs it: left
+    preconditioning, modified Gram-Schmidt, ``lartg`` rotations, inner stop
+    at |S_{col+1}| <= ptol, the ptol adaptation between cycles, and a back
+    substitution that skips zero components.  ``dot(a, b)`` replaces
+    numpy's dot product (every inner product and norm goes through it), and
+    ``xupdate='sequential'`` sums x += sum_k y_k v_k in k order instead of
+    numpy's ``y @ V``, to measure what the summation order alone does.
+    ``M`` (a LinearOperator) replaces ``precond``.  Returns (x, info, inner
+    iterations, restart cycles)."""
+    dot = np.dot if dot is None else dot
+
+    def norm(v):
+        return np.sqrt(dot(v, v))
+
+    A = sparse.csr_matrix(A)
+    n = A.shape[0]
+    Mop = make_preconditioner(A, precond) if M is None else M
+    psolve = (lambda v: v.copy()) if Mop is None else Mop.matvec
+    b = np.asarray(b, np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    bnrm2 = norm(b)
+    atol = max(float(atol), float(rtol) * bnrm2)
+    if bnrm2 == 0:
+        return b.copy(), 0, 0, 0
+    eps = np.finfo(np.float64).eps
+    maxiter = 10 * n if maxiter is None else maxiter
+    m = min(restart, n)
+    ptol_factor = 1.0
+    ptol = norm(psolve(b)) * min(ptol_factor, atol / bnrm2)
+    V = np.zeros((m + 1, n))
+    H = np.zeros((m, m + 1))
+    cs = np.zeros((m, 2))
+    r = b - A @ x
+    rnorm = norm(r)
+    if rnorm < atol:
+        return x, 0, 0, 0
+    inner = cycles = 0
+    for _ in range(maxiter):
+        cycles += 1
+        V[0] = psolve(r)
+        beta = norm(V[0])
+        V[0] *= 1.0 / beta
+        S = np.zeros(m + 1)
+        S[0] = beta
+        broke = False
+        for j in range(m):
+            w = psolve(A @ V[j])
+            h0 = norm(w)
+            for k in range(j + 1):
+                H[j, k] = dot(V[k], w)
+                w -= H[j, k] * V[k]
+            h1 = norm(w)
+            broke = h1 <= eps * h0
+            H[j, j + 1] = 0.0 if broke else h1
+            V[j + 1] = w if broke else w * (1.0 / h1)
+            for k in range(j):
+                c, s = cs[k]
+                a0, a1 = H[j, k], H[j, k + 1]
+                H[j, k], H[j, k + 1] = c * a0 + s * a1, -s * a0 + c * a1
+            c, s, H[j, j] = lartg(H[j, j], H[j, j + 1])
+            H[j, j + 1] = 0.0
+            cs[j] = c, s
+            S[j], S[j + 1] = c * S[j], -s * S[j]
+            presid = abs(S[j + 1])
+            inner += 1
+            if presid <= ptol or broke:
+                break
+        if H[j, j] == 0:
+            S[j] = 0
+        y = S[:j + 1].copy()
+        for k in range(j, 0, -1):
+            if y[k] != 0:
+                y[k] /= H[k, k]
+                y[:k] -= y[k] * H[k, :k]
+        if y[0] != 0:
+            y[0] /= H[0, 0]
+        if xupdate == "sequential":
+            acc = np.zeros(n)
+            for k in range(j + 1):
+                acc = acc + y[k] * V[k]
+            x += acc
+        else:
+            x += y @ V[:j + 1]
+        r = b - A @ x
+        rnorm = norm(r)
+        if rnorm <= atol or broke:
+            break
+        if presid <= ptol:
+            ptol_factor = max(eps, 0.25 * ptol_factor)
+        else:
+            ptol_factor = min(1.0, 1.5 * ptol_factor)
+        ptol = presid * min(ptol_factor, atol / rnorm)
+    return x, (0 if rnorm <= atol else maxiter), inner, cycles
 
 
 def true_rel_residual(A, x, b):

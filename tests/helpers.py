@@ -59,12 +59,12 @@ def rel_l2(a, b):
 
 
 def oracle_band(A, b, nrep=6, solver="bicgstab", **kw):
-    """scipy BiCGSTAB (or CG) on b and on nrep-1 copies of b perturbed by one
-    ulp per entry: the reference's own rounding-noise floor.  Returns
+    """scipy BiCGSTAB (or CG, GMRES) on b and on nrep-1 copies of b perturbed
+    by one ulp per entry: the reference's own rounding-noise floor.  Returns
     (iteration counts, infos, x of the unperturbed run, max relative L2
     spread of the perturbed solutions from it)."""
     from oracle import solve as osolve
-    run = osolve.solve_cg if solver == "cg" else osolve.solve_bicgstab
+    run = {"cg": osolve.solve_cg, "gmres": osolve.solve_gmres}.get(solver, osolve.solve_bicgstab)
     its, infos, xs = [], [], []
     for k in range(nrep):
         bb = b if k == 0 else b * (1 + np.random.default_rng(k).choice([-1, 1], len(b)) * 2.0 ** -52)
@@ -74,3 +74,88 @@ def oracle_band(A, b, nrep=6, solver="bicgstab", **kw):
         xs.append(x)
     spread = max(rel_l2(x, xs[0]) for x in xs[1:]) if nrep > 1 else 0.0
     return its, infos, xs[0], spread
+
+
+def _dot_fsum(a, b):
+    import math
+    return math.fsum((a * b).tolist())
+
+
+def _dot_reversed(a, b):
+    return float(np.dot(a[::-1], b[::-1]))
+
+
+def _dot_chunked(a, b, chunk=256):
+    p = a * b
+    n = len(p)
+    pad = (-n) % chunk
+    if pad:
+        p = np.concatenate([p, np.zeros(pad)])
+    return float(np.sum(np.sum(p.reshape(-1, chunk), axis=1)))
+
+
+DOT_ORDERS = {"fsum": _dot_fsum, "reversed": _dot_reversed, "chunked": _dot_chunked}
+
+
+def _butterfly64(v):
+    """wave64 xor-shuffle sum (v += shfl_xor(v, o), o = 32..1): lane 0's value."""
+    idx = np.arange(64)
+    for o in (32, 16, 8, 4, 2, 1):
+        v = v + v[..., idx ^ o]
+    return v[..., 0]
+
+
+def _block_sum256(vals):
+    """256-lane workgroup sum: each wave's butterfly, then waves 0..3 in order."""
+    ws = _butterfly64(vals.reshape(vals.shape[0], 4, 64))
+    s = ws[:, 0]
+    for q in range(1, 4):
+        s = s + ws[:, q]
+    return s
+
+
+def dot_device_order(cus):
+    """The device GMRES's dot product order: G = min(ceil(n/256), 1024,
+    4 cus) workgroups of 256 lanes, each lane summing its grid-stride
+    products in order, workgroup sums as _block_sum256, then the G partials
+    summed the same way by one workgroup (lane j taking partials j, j+256,
+    ...)."""
+    def dot(a, b):
+        n = len(a)
+        G = max(1, min((n + 255) // 256, 1024, 4 * cus))
+        p = a * b
+        T = G * 256
+        acc = np.zeros(T)
+        for start in range(0, n, T):
+            c = p[start:start + T]
+            acc[:len(c)] = acc[:len(c)] + c
+        part = _block_sum256(acc.reshape(G, 256))
+        t = np.zeros(256)
+        for start in range(0, G, 256):
+            c = part[start:start + 256]
+            t[:len(c)] = t[:len(c)] + c
+        return float(_block_sum256(t.reshape(1, 256))[0])
+    return dot
+
+
+def gmres_band(A, b, **kw):
+    """scipy GMRES, and the restated GMRES (oracle.solve.gmres_restated)
+    with the dot products summed in three other orders (exactly rounded,
+    reversed, in chunks of 256) and the x update summed in k order: GMRES's
+    own summation-order noise, which is the only arithmetic the device's
+    solve does in another order.  Returns (inner iteration counts, cycle
+    counts, infos, scipy's x, max relative L2 spread of the others from
+    it)."""
+    from oracle import solve as osolve
+    x0, info0, it0, _, cyc0 = osolve.solve_gmres(A, b, counts=True, **kw)
+    its, cycs, infos, spread = [it0], [cyc0], [info0], 0.0
+    rk = {k: v for k, v in kw.items() if k in ("x0", "rtol", "atol", "maxiter", "precond", "restart")}
+    if rk.get("restart") is None:
+        rk["restart"] = 20
+    for dot in DOT_ORDERS.values():
+        x, info, it, cyc = osolve.gmres_restated(A, b, dot=dot, xupdate="sequential", **rk)
+        its.append(it)
+        cycs.append(cyc)
+        infos.append(info)
+        spread = max(spread, rel_l2(x, x0))
+    return its, cycs, infos, x0, spread
