@@ -87,8 +87,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--precond", default="ilu0", choices=["ilu0", "jacobi"])
-    ap.add_argument("--method", default="bicgstab", choices=["bicgstab", "cg"],
-                    help="pressure: Krylov method (cg: solve_pressure_correction's solver, FP64 only)")
+    ap.add_argument("--method", default="bicgstab", choices=["bicgstab", "cg", "gmres"],
+                    help="pressure: Krylov method (cg: solve_pressure_correction's solver; gmres: "
+                         "solve_linear_system(method='gmres'), restart 20; both FP64 only)")
     ap.add_argument("--no-tune", action="store_true", help="batched: library default launch shape")
     ap.add_argument("--delta", type=float, default=0.1,
                     help="mixed: reliable-update trigger (FP32 residual reduction)")
@@ -150,8 +151,8 @@ def bench_pressure(args):
 
     results = {}
     precs = [p for p in args.precisions.split(",") if p]
-    if args.method == "cg":
-        precs = ["fp64"]                 # the CG path runs in FP64
+    if args.method in ("cg", "gmres"):
+        precs = ["fp64"]                 # the CG and GMRES paths run in FP64
     for prec in precs:
         opts = device.default_opts(rtol=1e-8, precond=args.precond, precision=prec,
                                    inner_rtol=args.delta, check_every=args.check_every,
@@ -186,6 +187,9 @@ def bench_pressure(args):
                              bytes=r.bytes, p50_ms=1e3 * float(np.percentile(lat, 50)),
                              p99_ms=1e3 * float(np.percentile(lat, 99)))
         log(prec, results[prec])
+        if args.method == "gmres":       # host-driven per cycle: no batch workspace to profile
+            results[prec]["kernels"] = {}
+            continue
         # per-kernel timing of the solve's kernels on the library stream
         bh = L.ctypes.c_void_p()
         L.check(L.lib().kcfd_mat_solver(A.h, L.ctypes.byref(bh)))
@@ -209,22 +213,29 @@ def bench_pressure(args):
     # inside real iterations; its algorithmic bytes are M_A + 3 n s_v (ILU)
     # or M_A + 4 n s_v (Jacobi).
     kern = mixed["kernels"]
-    dom = "spmv_p"
-    dk = kern[dom]
-    achieved = dk["gbs"]
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmv_p",
-                "avg_us": round(1e3 * dk["avg_ms"], 2), "bytes_per_launch": dk["bytes"],
-                "solve_avg_gbs": round(mixed["bytes"] / (mixed["solve_ms"] * 1e-3) / 1e9, 1),
-                "stream_copy_gbs": round(copy_gbs, 1)}
+    solve_gbs = mixed["bytes"] / (mixed["solve_ms"] * 1e-3) / 1e9
+    if args.method == "gmres":
+        # no single dominant kernel (SpMV, level-scheduled ILU(0) and the
+        # Gram-Schmidt sweeps): the whole solve's algorithmic bytes over its
+        # device time
+        roofline = {"bound": "hbm", "achieved": round(solve_gbs, 1), "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": round(solve_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "whole GMRES solve", "stream_copy_gbs": round(copy_gbs, 1)}
+    else:
+        dk = kern["spmv_p"]
+        achieved = dk["gbs"]
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmv_p",
+                    "avg_us": round(1e3 * dk["avg_ms"], 2), "bytes_per_launch": dk["bytes"],
+                    "solve_avg_gbs": round(solve_gbs, 1), "stream_copy_gbs": round(copy_gbs, 1)}
     cpu = None
     if not args.no_cpu:
         try:
             Ah = A.to_scipy()
             bh_ = case.rhs
             its = fp64["iters"]
-            if args.method == "cg":
-                raise RuntimeError("no CPU baseline for the CG workload")
+            if args.method != "bicgstab":
+                raise RuntimeError(f"no CPU baseline for the {args.method} workload")
             if args.precond == "ilu0" and args.order != "natural":
                 # the CPU path factors in the natural order, which needs fewer
                 # iterations than the colour ordering: count them on the GPU
@@ -252,7 +263,8 @@ def bench_pressure(args):
         "dtype": "f32 inner / f64 refinement" if "mixed" in results else "f64",
         "data": "synthetic",
         "config": {"workload": f"hex{args.n}^3 SIMPLE pressure correction, "
-                               f"{'BiCGSTAB' if args.method == 'bicgstab' else 'CG'}+{args.precond}, "
+                               f"{dict(bicgstab='BiCGSTAB', cg='CG', gmres='GMRES(20)')[args.method]}"
+                               f"+{args.precond}, "
                                f"rtol 1e-8", "cells": n, "nnz": nnz,
                    "ilu_order": args.order if args.precond == "ilu0" else None,
                    "step": "device assembly + preconditioner set-up + solve"},

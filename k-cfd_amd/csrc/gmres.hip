@@ -10,10 +10,17 @@
 // true residual after each cycle.  maxiter counts restart cycles.
 //
 // Layout.  One cycle is enqueued without a host synchronisation: the
-// Hessenberg matrix, rotations and S live in a device GmState; every kernel
-// of column col returns at once when the cycle stopped at an earlier column
-// (stop_col < col), so a cycle that converges early costs only launches for
-// its unused columns.  The host reads a few scalars once per cycle.  The
+// Hessenberg matrix, rotations, S and the inner tolerance live in a device
+// GmState; every kernel of column col returns at once when the cycle stopped
+// at an earlier column (stop_col < col), so a cycle that converges early
+// costs only launches for its unused columns.  Every cycle thus enqueues the
+// same launches; from the second cycle on they are replayed from a hipGraph
+// captured once per solve (a cycle is ~15 launches per inner iteration, and
+// replaying them removes the host launch gaps).  b and x0 are staged in the
+// workspace so that the graph holds no caller pointer.  KCFD_GMRES_GRAPH=0
+// in the environment launches every cycle directly instead (for tools that
+// do not follow graph launches; the kernels and results are the same).  The
+// host reads a few scalars once per cycle.  The
 // reductions are two-stage and deterministic: each kernel writes one
 // partial per workgroup, and every workgroup of the next kernel sums all
 // partials in the same fixed order, so each gets the same scalar without a
@@ -27,6 +34,7 @@
 #include <climits>
 #include <cmath>
 #include <cstddef>
+#include <cstdlib>
 #include <cstring>
 
 #include "common.hpp"
@@ -46,6 +54,9 @@ struct GmState {
     int ncol;            // columns built in the current cycle
     int breakdown;       // h_{col+1,col} <= eps h0 at the last column
     int stop_col;        // column at which the cycle stopped (INT_MAX while running)
+    double atol;         // absolute stopping tolerance max(atol, rtol ||b||)
+    double ptol;         // inner stopping tolerance of the current cycle
+    double pmf;          // ptol_max_factor of the gh-8400 adaptation
     double h0;           // ||w|| of the current column before orthogonalisation
     double S[kMaxRestart + 1], y[kMaxRestart], gc[kMaxRestart], gs[kMaxRestart];
     double h[kMaxRestart][kMaxRestart + 1];   // h[col][k] as scipy's h[col, k]
@@ -284,7 +295,7 @@ __device__ void lartg(double f, double g, double &c, double &s, double &r) {
 // and decides whether the cycle stops here.
 __global__ __launch_bounds__(kGmBlock) void g_mgs_last(int n, double *__restrict__ w, int col,
                                                        const double *__restrict__ pin, int G, GmState *st,
-                                                       double ptol, double eps) {
+                                                       double eps) {
 #pragma clang fp contract(off)
     if (skipped(st, col)) return;
     double s1[1];
@@ -317,39 +328,85 @@ __global__ __launch_bounds__(kGmBlock) void g_mgs_last(int n, double *__restrict
     st->inner += 1;
     st->ncol = col + 1;
     st->breakdown = bd ? 1 : 0;
-    if (st->presid <= ptol || bd) st->stop_col = col;
+    if (st->presid <= st->ptol || bd) st->stop_col = col;
 }
 
 // y = H^{-1} S over the ncol built columns (zero components skipped; a zero
-// last diagonal zeroes its S entry), one thread
-__global__ void g_backsub(GmState *st) {
+// last diagonal zeroes its S entry).  The block stages H and S in LDS, then
+// one thread substitutes.
+__global__ __launch_bounds__(kGmBlock) void g_backsub(GmState *st) {
 #pragma clang fp contract(off)
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const int col = st->ncol - 1;
-    if (col < 0) return;
-    if (st->h[col][col] == 0.0) st->S[col] = 0.0;
-    double *y = st->y;
-    for (int k = 0; k <= col; ++k) y[k] = st->S[k];
+    __shared__ double hs[kMaxRestart][kMaxRestart + 1];
+    __shared__ double ss[kMaxRestart + 1];
+    const int nc = uniform(st->ncol);
+    if (nc <= 0) return;
+    for (int e = threadIdx.x; e < nc * (kMaxRestart + 1); e += kGmBlock)
+        hs[e / (kMaxRestart + 1)][e % (kMaxRestart + 1)] = st->h[e / (kMaxRestart + 1)][e % (kMaxRestart + 1)];
+    for (int e = threadIdx.x; e < nc; e += kGmBlock) ss[e] = st->S[e];
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const int col = nc - 1;
+    if (hs[col][col] == 0.0) {
+        ss[col] = 0.0;
+        st->S[col] = 0.0;
+    }
+    double *y = ss;   // solved in place
     for (int k = col; k > 0; --k) {
         if (y[k] != 0.0) {
-            y[k] /= st->h[k][k];
+            y[k] /= hs[k][k];
             const double tmp = y[k];
-            for (int j = 0; j < k; ++j) y[j] -= tmp * st->h[k][j];
+            for (int j = 0; j < k; ++j) y[j] -= tmp * hs[k][j];
         }
     }
-    if (y[0] != 0.0) y[0] /= st->h[0][0];
+    if (y[0] != 0.0) y[0] /= hs[0][0];
+    for (int k = 0; k <= col; ++k) st->y[k] = y[k];
 }
 
 // x += sum_k y_k v_k
 __global__ __launch_bounds__(kGmBlock) void g_xupdate(int n, const double *__restrict__ V, size_t ldv,
                                                       double *__restrict__ x, const GmState *st) {
 #pragma clang fp contract(off)
+    __shared__ double ys[kMaxRestart];
     const int nc = uniform(st->ncol);
+    for (int k = threadIdx.x; k < nc; k += kGmBlock) ys[k] = st->y[k];
+    __syncthreads();
     for (int i = blockIdx.x * kGmBlock + threadIdx.x; i < n; i += gridDim.x * kGmBlock) {
         double acc = 0.0;
-        for (int k = 0; k < nc; ++k) acc += st->y[k] * V[k * ldv + i];
+        for (int k = 0; k < nc; ++k) acc += ys[k] * V[k * ldv + i];
         x[i] += acc;
     }
+}
+
+// End of a cycle, one workgroup: ||r|| from the residual partials, then the
+// gh-8400 adaptation of the inner tolerance for the next cycle when this one
+// neither converged nor broke down.
+__global__ __launch_bounds__(kGmBlock) void g_cycle_end(const double *__restrict__ part, int G, GmState *st,
+                                                        double eps) {
+#pragma clang fp contract(off)
+    double s[1];
+    sum_parts<1>(part, G, s);
+    if (threadIdx.x != 0) return;
+    const double rnorm = sqrt(s[0]);
+    st->rnorm = rnorm;
+    if (rnorm <= st->atol || st->breakdown) return;
+    if (st->presid <= st->ptol)
+        st->pmf = fmax(eps, 0.25 * st->pmf);
+    else
+        st->pmf = fmin(1.0, 1.5 * st->pmf);
+    st->ptol = st->presid * fmin(st->pmf, st->atol / rnorm);
+}
+
+__global__ void g_set_tol(GmState *st, double atol, double ptol) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->atol = atol;
+        st->ptol = ptol;
+        st->pmf = 1.0;
+    }
+}
+
+__global__ void g_copy(int n, const double *__restrict__ src, double *__restrict__ dst) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
 }
 
 __global__ void g_gather(int n, const int *__restrict__ perm, const double *__restrict__ src,
@@ -381,11 +438,11 @@ void gmres_free(kcfd_mat *m) {
     m->gmres = nullptr;
 }
 
-static int ensure_ws(kcfd_mat *m, int cols, bool porder) {
+static int ensure_ws(kcfd_mat *m, int cols) {
     const size_t n = (size_t)m->n;
     const size_t ldv = (n + 63) & ~(size_t)63;
     GmresWs *w = m->gmres;
-    if (w && w->cols >= cols && w->ldv == ldv && (!porder || w->bp)) return KCFD_OK;
+    if (w && w->cols >= cols && w->ldv == ldv) return KCFD_OK;
     gmres_free(m);
     w = new GmresWs();
     m->gmres = w;
@@ -397,8 +454,8 @@ static int ensure_ws(kcfd_mat *m, int cols, bool porder) {
     };
     if (alloc(&w->V, ldv * cols) != hipSuccess || alloc(&w->av, ldv) != hipSuccess ||
         alloc(&w->r, ldv) != hipSuccess || alloc(&w->t, ldv) != hipSuccess ||
-        alloc(&w->part, 2 * 2 * kMaxGmGrid) != hipSuccess ||
-        (porder && (alloc(&w->bp, ldv) != hipSuccess || alloc(&w->xp, ldv) != hipSuccess)) ||
+        alloc(&w->part, 2 * 2 * kMaxGmGrid) != hipSuccess || alloc(&w->bp, ldv) != hipSuccess ||
+        alloc(&w->xp, ldv) != hipSuccess ||
         hipMalloc(&w->st, sizeof(GmState)) != hipSuccess) {
         gmres_free(m);
         set_error("GMRES workspace: hipMalloc failed (n=%zu, %d vectors)", n, cols);
@@ -437,7 +494,7 @@ int gmres_solve(kcfd_mat *m, const double *b, double *x, const kcfd_solve_opts *
         return KCFD_ESTATE;
     }
     const int maxiter = o->maxiter > 0 ? o->maxiter : 10 * n;
-    KCFD_CHECK(ensure_ws(m, restart + 1, porder));
+    KCFD_CHECK(ensure_ws(m, restart + 1));
     GmresWs *w = m->gmres;
     kcfd_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
@@ -448,52 +505,16 @@ int gmres_solve(kcfd_mat *m, const double *b, double *x, const kcfd_solve_opts *
     double *V = w->V, *p0 = w->part, *p1 = w->part + 2 * kMaxGmGrid;
     GmState *st = w->st;
     const int *stop_col = (const int *)((const char *)st + offsetof(GmState, stop_col));
-    const double *B = porder ? w->bp : b;
-    double *X = porder ? w->xp : x;
+    double *B = w->bp, *X = w->xp;   // b and x in the solver's row order
     GmSummary sm{};
     auto summary = [&]() -> int {
         KCFD_HIP(hipMemcpyAsync(&sm, st, sizeof(sm), hipMemcpyDeviceToHost, s));
         KCFD_HIP(hipStreamSynchronize(s));
         return KCFD_OK;
     };
-    double bytes = 0.0;
-    const double vec = 8.0 * n;
-
-    KCFD_HIP(hipEventRecord(ctx->ev0, s));
-    if (porder) {
-        hipLaunchKernelGGL(g_gather, dim3(nb), dim3(kGmBlock), 0, s, n, m->perm, b, w->bp);
-        hipLaunchKernelGGL(g_gather, dim3(nb), dim3(kGmBlock), 0, s, n, m->perm, x, w->xp);
-        bytes += 2 * (12.0 * n + 8.0 * n);
-    }
-    hipLaunchKernelGGL(g_state_init, dim3(1), dim3(64), 0, s, st);
-    // ||b||, ||M^{-1} b|| and the initial residual
-    KCFD_CHECK(precond_apply_async(m, B, V, w->t, porder, nullptr, 0));
-    hipLaunchKernelGGL(g_norm2_part, dim3(G), dim3(kGmBlock), 0, s, n, B, V, p0);
-    hipLaunchKernelGGL(g_finalize, dim3(1), dim3(kGmBlock), 0, s, p0, G, st, 0);
-    hipLaunchKernelGGL(g_resid, dim3(G), dim3(kGmBlock), 0, s, n, m->slice_ptr, m->six, m->sa64, X, B,
-                       w->r, p1);
-    hipLaunchKernelGGL(g_finalize, dim3(1), dim3(kGmBlock), 0, s, p1, G, st, 1);
-    KCFD_HIP(hipGetLastError());
-    bytes += precond_bytes(m) + 2 * vec + spmv_bytes(m) + 2 * vec;
-    KCFD_CHECK(summary());
-
-    const double bnrm2 = sm.fin0;
-    int info = 0, cycles = 0;
-    double rnorm = sm.rnorm;
-    bool done = false, scatter = porder;
-    if (bnrm2 == 0.0) {   // scipy returns b itself (all zeros)
-        KCFD_HIP(hipMemcpyAsync(x, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
-        rnorm = 0.0;
-        done = true;
-        scatter = false;
-    }
-    const double atol = std::max(o->atol, o->rtol * bnrm2);
-    const double Mb_nrm2 = sm.fin1;
-    double ptol_max_factor = 1.0;
-    double ptol = done ? 0.0 : Mb_nrm2 * std::min(ptol_max_factor, atol / bnrm2);
-    if (!done && rnorm < atol) done = true;
-    for (int it = 0; !done && it < maxiter; ++it) {
-        ++cycles;
+    // one restart cycle: the same launches every cycle (all scalars it
+    // depends on are in GmState)
+    auto enqueue_cycle = [&]() -> int {
         KCFD_CHECK(precond_apply_async(m, w->r, V, w->t, porder, nullptr, 0));
         hipLaunchKernelGGL(g_norm2_part, dim3(G), dim3(kGmBlock), 0, s, n, V, (const double *)nullptr, p0);
         hipLaunchKernelGGL(g_cycle_start, dim3(G), dim3(kGmBlock), 0, s, n, V, p0, G, st, restart);
@@ -510,17 +531,92 @@ int gmres_solve(kcfd_mat *m, const double *b, double *x, const kcfd_solve_opts *
                                    V + (size_t)(k + 1) * ldv, wv, k, col, pin, G, pout, st);
             }
             hipLaunchKernelGGL(g_mgs_last, dim3(G), dim3(kGmBlock), 0, s, n, wv, col,
-                               ((col + 1) & 1) ? p1 : p0, G, st, ptol, eps);
+                               ((col + 1) & 1) ? p1 : p0, G, st, eps);
         }
-        hipLaunchKernelGGL(g_backsub, dim3(1), dim3(64), 0, s, st);
+        hipLaunchKernelGGL(g_backsub, dim3(1), dim3(kGmBlock), 0, s, st);
         hipLaunchKernelGGL(g_xupdate, dim3(G), dim3(kGmBlock), 0, s, n, V, ldv, X, st);
         hipLaunchKernelGGL(g_resid, dim3(G), dim3(kGmBlock), 0, s, n, m->slice_ptr, m->six, m->sa64, X, B,
                            w->r, p0);
-        hipLaunchKernelGGL(g_finalize, dim3(1), dim3(kGmBlock), 0, s, p0, G, st, 1);
+        hipLaunchKernelGGL(g_cycle_end, dim3(1), dim3(kGmBlock), 0, s, p0, G, st, eps);
         KCFD_HIP(hipGetLastError());
-        const int inner_before = sm.inner;
-        KCFD_CHECK(summary());
-        const int ncol = sm.inner - inner_before;
+        return KCFD_OK;
+    };
+    double bytes = 0.0;
+    const double vec = 8.0 * n;
+
+    KCFD_HIP(hipEventRecord(ctx->ev0, s));
+    if (porder) {
+        hipLaunchKernelGGL(g_gather, dim3(nb), dim3(kGmBlock), 0, s, n, m->perm, b, B);
+        hipLaunchKernelGGL(g_gather, dim3(nb), dim3(kGmBlock), 0, s, n, m->perm, x, X);
+        bytes += 2 * (12.0 * n + 8.0 * n);
+    } else {
+        hipLaunchKernelGGL(g_copy, dim3(nb), dim3(kGmBlock), 0, s, n, b, B);
+        hipLaunchKernelGGL(g_copy, dim3(nb), dim3(kGmBlock), 0, s, n, x, X);
+        bytes += 2 * (16.0 * n);
+    }
+    hipLaunchKernelGGL(g_state_init, dim3(1), dim3(64), 0, s, st);
+    // ||b||, ||M^{-1} b|| and the initial residual
+    KCFD_CHECK(precond_apply_async(m, B, V, w->t, porder, nullptr, 0));
+    hipLaunchKernelGGL(g_norm2_part, dim3(G), dim3(kGmBlock), 0, s, n, B, V, p0);
+    hipLaunchKernelGGL(g_finalize, dim3(1), dim3(kGmBlock), 0, s, p0, G, st, 0);
+    hipLaunchKernelGGL(g_resid, dim3(G), dim3(kGmBlock), 0, s, n, m->slice_ptr, m->six, m->sa64, X, B,
+                       w->r, p1);
+    hipLaunchKernelGGL(g_finalize, dim3(1), dim3(kGmBlock), 0, s, p1, G, st, 1);
+    KCFD_HIP(hipGetLastError());
+    bytes += precond_bytes(m) + 2 * vec + spmv_bytes(m) + 2 * vec;
+    KCFD_CHECK(summary());
+
+    const double bnrm2 = sm.fin0;
+    int info = 0, cycles = 0;
+    double rnorm = sm.rnorm;
+    bool done = false;
+    if (bnrm2 == 0.0) {   // scipy returns b itself (all zeros)
+        rnorm = 0.0;
+        done = true;
+    }
+    const double atol = std::max(o->atol, o->rtol * bnrm2);
+    if (!done) {
+        const double ptol = sm.fin1 * std::min(1.0, atol / bnrm2);
+        hipLaunchKernelGGL(g_set_tol, dim3(1), dim3(64), 0, s, st, atol, ptol);
+        KCFD_HIP(hipGetLastError());
+        if (rnorm < atol) done = true;
+    }
+    hipGraphExec_t graph = nullptr;
+    int rc = KCFD_OK;
+    const char *genv = std::getenv("KCFD_GMRES_GRAPH");
+    const bool use_graph = !(genv && genv[0] == '0');
+    for (int it = 0; !done && it < maxiter; ++it) {
+        ++cycles;
+        if (it == 0 || !use_graph) {
+            rc = enqueue_cycle();
+        } else {
+            if (!graph) {   // a second cycle: capture one and replay it from now on
+                hipGraph_t g = nullptr;
+                KCFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                rc = enqueue_cycle();
+                hipError_t e = hipStreamEndCapture(s, &g);
+                if (rc == KCFD_OK && e != hipSuccess) {
+                    set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+                    rc = KCFD_EHIP;
+                }
+                if (rc == KCFD_OK) {
+                    e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+                    if (e != hipSuccess) {
+                        set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+                        rc = KCFD_EHIP;
+                        graph = nullptr;
+                    }
+                }
+                if (g) hipGraphDestroy(g);
+            }
+            if (rc == KCFD_OK && hipGraphLaunch(graph, s) != hipSuccess) {
+                set_error("hipGraphLaunch failed");
+                rc = KCFD_EHIP;
+            }
+        }
+        if (rc == KCFD_OK) rc = summary();
+        if (rc != KCFD_OK) break;
+        const int ncol = sm.ncol;
         // bytes: preconditioned residual, v0 norm and scaling, then per column
         // SpMV + apply + Gram-Schmidt (w and v_0..v_col read, w written
         // col + 2 times), then the update and the true residual
@@ -529,18 +625,19 @@ int gmres_solve(kcfd_mat *m, const double *b, double *x, const kcfd_solve_opts *
             bytes += spmv_bytes(m) + precond_bytes(m) + 2 * vec + (3.0 * c + 3) * vec + 2 * vec;
         bytes += (ncol + 2) * vec + spmv_bytes(m) + 2 * vec;
         rnorm = sm.rnorm;
-        if (rnorm <= atol) break;
-        if (sm.breakdown) break;
-        if (sm.presid <= ptol)
-            ptol_max_factor = std::max(eps, 0.25 * ptol_max_factor);
-        else
-            ptol_max_factor = std::min(1.0, 1.5 * ptol_max_factor);
-        ptol = sm.presid * std::min(ptol_max_factor, atol / rnorm);
+        if (rnorm <= atol || sm.breakdown) break;   // otherwise g_cycle_end has set the next ptol
     }
+    if (graph) hipGraphExecDestroy(graph);
+    if (rc != KCFD_OK) return rc;
     if (!(rnorm <= atol)) info = maxiter;
-    if (scatter) {
-        hipLaunchKernelGGL(g_scatter, dim3(nb), dim3(kGmBlock), 0, s, n, m->perm, w->xp, x);
+    if (bnrm2 == 0.0) {
+        hipLaunchKernelGGL(g_copy, dim3(nb), dim3(kGmBlock), 0, s, n, b, x);
+    } else if (porder) {
+        hipLaunchKernelGGL(g_scatter, dim3(nb), dim3(kGmBlock), 0, s, n, m->perm, X, x);
         bytes += 20.0 * n;
+    } else {
+        hipLaunchKernelGGL(g_copy, dim3(nb), dim3(kGmBlock), 0, s, n, X, x);
+        bytes += 16.0 * n;
     }
     KCFD_HIP(hipGetLastError());
     KCFD_HIP(hipEventRecord(ctx->ev1, s));

@@ -177,3 +177,24 @@ def test_gmres_dropin(ctx, capsys):
     x = solve_linear_system(sysm, method="gmres", tol=1e-14, maxiter=1, preconditioner="none",
                             rtol=0.0)
     assert "[gmres]" in capsys.readouterr().out and x.shape == b.shape
+
+
+def test_gmres_graph_replay_equals_direct(ctx, monkeypatch):
+    """From the second cycle on the solve replays a captured hipGraph of one
+    cycle; KCFD_GMRES_GRAPH=0 launches every cycle directly.  Both give the
+    same iterates bit for bit over every replay (and twice in a row with the
+    graph), in the natural and the colour-ordered ILU(0)."""
+    from kcfd_amd import device
+    A, b, _ = oracle_system(_case("quad32"))
+    for pc, order in (("jacobi", None), ("ilu0", "color")):
+        M = device.DeviceMatrix.from_csr(ctx, A)
+        M.setup_precond(pc, "fp64", order=order)
+        runs = []
+        for mode in ("1", "0", "1"):
+            monkeypatch.setenv("KCFD_GMRES_GRAPH", mode)
+            x, res = M.solve(b, rtol=1e-10, precond=pc, method="gmres", restart=8)
+            runs.append((x, res.iters, res.refinements))
+        print(f"graph vs direct {pc}: {runs[0][1]} iterations in {runs[0][2]} cycles")
+        assert runs[0][2] > 3
+        for x, it, cyc in runs[1:]:
+            assert (it, cyc) == runs[0][1:] and np.array_equal(x, runs[0][0])
