@@ -27,9 +27,11 @@ every norm and dot product of the reference unchanged.
 from __future__ import annotations
 
 import numpy as np
+from typing import NamedTuple
 
 __all__ = ["FVMesh", "build_fvmesh_from_arrays", "structured_quad_arrays",
-           "structured_hex_arrays", "tet_arrays"]
+           "structured_hex_arrays", "tet_arrays", "RaggedCells", "hybrid_hex_tet_arrays",
+           "generate_hybrid_hex_tet_mesh"]
 
 
 class FVMesh:
@@ -114,6 +116,46 @@ class FVMesh:
         return s
 
 
+# node id standing for "no node" in sorted face keys: sorts after every real
+# id and keeps the keys non-negative (the stored face arrays pad with -1)
+_NO_NODE = (1 << 31) - 1
+
+
+class RaggedCells(NamedTuple):
+    """3D cells with differing node and face counts (mixed hex/tet meshes).
+
+    cell_nodes: (n, 8) node ids, padded with -1; faces: (E, 4) the faces of
+    every cell in cell order, each padded with -1 (triangles); face_ptr:
+    (n + 1) offsets of each cell's faces in ``faces``."""
+    cell_nodes: np.ndarray
+    faces: np.ndarray
+    face_ptr: np.ndarray
+
+
+def _face_keys(rows: np.ndarray) -> np.ndarray:
+    """Sorted node keys of padded face rows (-1 -> _NO_NODE, so a triangle
+    never equals a quad)."""
+    k = np.where(rows >= 0, rows, _NO_NODE).astype(np.int64)
+    k.sort(axis=1)
+    return k
+
+
+def _face_rows(flist, width: int) -> np.ndarray:
+    """A boundary face list (array, or list of node arrays of any length up
+    to width) as padded (m, width) rows."""
+    if isinstance(flist, np.ndarray) and flist.ndim == 2:
+        if flist.shape[1] == width:
+            return flist.astype(np.int64)
+        out = np.full((len(flist), width), -1, np.int64)
+        out[:, :flist.shape[1]] = flist
+        return out
+    out = np.full((len(flist), width), -1, np.int64)
+    for r, f in enumerate(flist):
+        f = np.asarray(f, dtype=np.int64).ravel()
+        out[r, :len(f)] = f
+    return out
+
+
 def _pad3(a: np.ndarray) -> np.ndarray:
     if a.shape[1] == 3:
         return np.ascontiguousarray(a, dtype=np.float64)
@@ -181,7 +223,7 @@ def _tag_boundaries(mesh: FVMesh, face_keys: np.ndarray, boundary_faces_dict: di
         for ni, (name, flist) in enumerate(boundary_faces_dict.items()):
             if len(flist) == 0:
                 continue
-            arr = np.sort(np.asarray(flist, dtype=np.int64).reshape(len(flist), width), axis=1)
+            arr = _face_keys(_face_rows(flist, width))
             bk.append(arr)
             bname_idx.append(np.full(len(arr), ni, np.int64))
         bk = np.concatenate(bk) if bk else np.empty((0, width), np.int64)
@@ -220,8 +262,14 @@ def build_fvmesh_from_arrays(node_coords, cell_node_list, boundary_faces_dict,
 
     ``cell_node_list`` may be a 2-D int array (uniform cell type) or a list.
     In 3D, ``cell_face_node_list`` may be an int array of shape
-    (n_cells, faces_per_cell, nodes_per_face)."""
+    (n_cells, faces_per_cell, nodes_per_face), a ``RaggedCells`` (mixed cell
+    types; ``cell_node_list`` is then ignored), or the reference's list of
+    per-cell face lists, uniform or not."""
     nodes = np.asarray(node_coords, dtype=np.float64)
+    if ndim == 3 and cell_face_node_list is not None:
+        rag = _as_ragged(cell_node_list, cell_face_node_list)
+        if rag is not None:
+            return _build_3d_ragged(nodes, rag, boundary_faces_dict, cell_zone_map)
     mesh = FVMesh(ndim)
     mesh.nodes = nodes
     cells = np.asarray(cell_node_list, dtype=np.int64)
@@ -316,6 +364,96 @@ def build_fvmesh_from_arrays(node_coords, cell_node_list, boundary_faces_dict,
     return mesh
 
 
+def _as_ragged(cell_node_list, cell_face_node_list):
+    """RaggedCells for mixed 3D cells, or None when the per-cell face lists
+    form one uniform (n, faces, nodes) array."""
+    if isinstance(cell_face_node_list, RaggedCells):
+        return cell_face_node_list
+    if isinstance(cell_face_node_list, np.ndarray):
+        return None
+    try:
+        np.asarray(cell_face_node_list, dtype=np.int64)
+        np.asarray(cell_node_list, dtype=np.int64)
+        return None
+    except ValueError:
+        pass
+    n = len(cell_face_node_list)
+    cn = np.full((n, 8), -1, np.int64)
+    for c, nodes in enumerate(cell_node_list):
+        v = np.asarray(nodes, dtype=np.int64).ravel()
+        cn[c, :len(v)] = v
+    counts = np.array([len(fl) for fl in cell_face_node_list], np.int64)
+    ptr = np.concatenate([[0], np.cumsum(counts)])
+    faces = _face_rows([f for fl in cell_face_node_list for f in fl], 4)
+    return RaggedCells(cn, faces, ptr)
+
+
+def _build_3d_ragged(nodes, rag: RaggedCells, boundary_faces_dict, cell_zone_map) -> FVMesh:
+    """``_build_fvmesh_3d`` (mesh/mesh_reader.py:245-323) for cells of
+    differing node and face counts: the uniform path's arithmetic applied to
+    each group of faces (3 or 4 nodes) and cells (4 or 8 nodes)."""
+    cn = np.asarray(rag.cell_nodes, dtype=np.int64)
+    fe = np.asarray(rag.faces, dtype=np.int64)
+    fptr = np.asarray(rag.face_ptr, dtype=np.int64)
+    n = cn.shape[0]
+    mesh = FVMesh(3)
+    mesh.nodes = nodes
+    mesh.n_cells = n
+    nodes3 = _pad3(nodes)
+    keys = _face_keys(fe)
+    face_of_entry, first, second = _first_encounter_faces(keys)
+    ent_cell = np.repeat(np.arange(n), np.diff(fptr))
+    fnodes = fe[first]
+    owner = ent_cell[first]
+    neighbour = np.where(second >= 0, ent_cell[np.maximum(second, 0)], -1)
+    nf = len(first)
+    fk = (fnodes >= 0).sum(axis=1)
+    center = np.zeros((nf, 3))
+    nrm = np.zeros((nf, 3))
+    for k in np.unique(fk):
+        sel = np.nonzero(fk == k)[0]
+        pts = nodes3[fnodes[sel, :k]]
+        center[sel] = pts.mean(axis=1)
+        nxt = np.roll(pts, -1, axis=1)
+        nrm[sel, 0] = np.sum((pts[:, :, 1] - nxt[:, :, 1]) * (pts[:, :, 2] + nxt[:, :, 2]), axis=1)
+        nrm[sel, 1] = np.sum((pts[:, :, 2] - nxt[:, :, 2]) * (pts[:, :, 0] + nxt[:, :, 0]), axis=1)
+        nrm[sel, 2] = np.sum((pts[:, :, 0] - nxt[:, :, 0]) * (pts[:, :, 1] + nxt[:, :, 1]), axis=1)
+    area = np.sqrt(np.sum(nrm * nrm, axis=1))
+    nz = area > 1e-15
+    nrm[nz] /= area[nz, None]
+    area = area * 0.5
+    ccen = np.zeros((n, 3))
+    cnn = (cn >= 0).sum(axis=1)
+    for k in np.unique(cnn):
+        sel = np.nonzero(cnn == k)[0]
+        ccen[sel] = nodes3[cn[sel, :k]].mean(axis=1)
+    to_face = center - ccen[owner]
+    flip = np.sum(nrm * to_face, axis=1) < 0
+    nrm[flip] = -nrm[flip]
+    ef, ec = face_of_entry, ent_cell
+    tf = center[ef] - ccen[ec]
+    sgn = np.where(np.sum(nrm[ef] * tf, axis=1) < 0, -1.0, 1.0)
+    contrib = sgn * np.sum(center[ef] * nrm[ef], axis=1) * area[ef]
+    vol = np.add.reduceat(contrib, fptr[:-1]) if n else np.zeros(0)
+    vol = np.abs(vol) / 3.0
+    vol = np.where(vol < 1e-30, 1e-30, vol)
+    mesh.owner = owner.astype(np.int32)
+    mesh.neighbour = neighbour.astype(np.int32)
+    mesh.face_area = np.ascontiguousarray(area)
+    mesh.face_normal = np.ascontiguousarray(nrm)
+    mesh.face_center = np.ascontiguousarray(center)
+    mesh.cell_center = np.ascontiguousarray(ccen)
+    mesh.cell_volume = np.ascontiguousarray(vol)
+    mesh.face_nodes = fnodes
+    mesh.n_faces = nf
+    mesh.n_internal_faces = int(np.sum(neighbour >= 0))
+    mesh.n_boundary_faces = nf - mesh.n_internal_faces
+    _tag_boundaries(mesh, _face_keys(fnodes), boundary_faces_dict or {}, 4)
+    if cell_zone_map:
+        mesh.cell_zones = cell_zone_map
+    return mesh
+
+
 # ---------------------------------------------------------------------------
 # generator inputs (node arrays in the reference generators' numbering)
 # ---------------------------------------------------------------------------
@@ -396,3 +534,120 @@ def tet_arrays(tets: np.ndarray):
     faces = np.stack([np.stack([n0, n2, n1], 1), np.stack([n0, n1, n3], 1),
                       np.stack([n1, n2, n3], 1), np.stack([n0, n3, n2], 1)], axis=1)
     return faces
+
+
+# the six faces of a hex (z-, z+, y-, y+, x-, x+) as corner positions
+# n0..n7, each listed counter-clockwise seen from outside
+# (mesh/hybrid_mesh_generator.py:76-86)
+_HEX_QUADS = np.array([[0, 3, 2, 1], [4, 5, 6, 7], [0, 1, 5, 4],
+                       [2, 3, 7, 6], [0, 4, 7, 3], [1, 2, 6, 5]])
+
+
+def _split_quads(q: np.ndarray):
+    """Each quad (…, 4) as two triangles through its smallest node
+    (mesh/hybrid_mesh_generator.py:14-27): (a,b,c),(a,c,d) when the
+    smallest is a or c, else (a,b,d),(b,c,d).  Returns (…, 2, 3)."""
+    a, b, c, d = q[..., 0], q[..., 1], q[..., 2], q[..., 3]
+    m = np.minimum(np.minimum(a, b), np.minimum(c, d))
+    ac = (m == a) | (m == c)
+    t0 = np.stack([a, b, np.where(ac, c, d)], axis=-1)
+    t1 = np.stack([np.where(ac, a, b), c, d], axis=-1)
+    return np.stack([t0, t1], axis=-2)
+
+
+def hybrid_hex_tet_arrays(Lx=2.0, Ly=0.1, Lz=0.1, nx=20, ny=8, nz=8, tet_fraction=0.5,
+                          boundary_names=None):
+    """Arrays of ``generate_hybrid_hex_tet_mesh``
+    (mesh/hybrid_mesh_generator.py:93-270), vectorised: hexes on the inlet
+    side (the last hex layer splits its x+ quad into the two triangles the
+    tets use), and beyond ``nx_hex = max(1, int(nx (1 - tet_fraction)))``
+    each hex cut into 12 tets around an inserted centre node (two per quad
+    face).  Same node numbering, cell order (k, j, i; a hex region cell's 12
+    tets in face and triangle order), face lists and boundary faces (in the
+    order the reference meets them) as the reference.
+
+    Returns (nodes, RaggedCells, boundary_faces_dict, cell_zone_map); the
+    boundary faces of a patch are (m, 4) rows, triangles padded with -1."""
+    if boundary_names is None:
+        boundary_names = {"x_min": "inlet", "x_max": "outlet", "y_min": "wall_bottom",
+                          "y_max": "wall_top", "z_min": "wall_front", "z_max": "wall_back"}
+    dx, dy, dz = Lx / nx, Ly / ny, Lz / nz
+    nxh = max(1, int(nx * (1.0 - tet_fraction)))
+    nxt = nx - nxh
+    nbase = (nx + 1) * (ny + 1) * (nz + 1)
+    kk, jj, ii = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    base = np.stack([ii.ravel() * dx, jj.ravel() * dy, kk.ravel() * dz], axis=1)
+    kc, jc, ic = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nxh, nx), indexing="ij")
+    cen = np.stack([(ic.ravel() + 0.5) * dx, (jc.ravel() + 0.5) * dy, (kc.ravel() + 0.5) * dz], axis=1)
+    nodes = np.concatenate([base, cen.reshape(-1, 3)])
+
+    def nid(i, j, k):
+        return (k * (ny + 1) + j) * (nx + 1) + i
+
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    corners = np.stack([nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                        nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1),
+                        nid(i, j + 1, k + 1)], axis=-1).astype(np.int64)      # (nz, ny, nx, 8)
+    quads = corners[..., _HEX_QUADS]                                           # (nz, ny, nx, 6, 4)
+    per_row = nxh + 12 * nxt
+    # cell nodes, padded to 8
+    cnodes = np.full((nz, ny, per_row, 8), -1, np.int64)
+    cnodes[:, :, :nxh] = corners[:, :, :nxh]
+    # faces, padded to 7 per cell and 4 per face
+    cfaces = np.full((nz, ny, per_row, 7, 4), -1, np.int64)
+    cfaces[:, :, :nxh, :6] = quads[:, :, :nxh]
+    if nxt > 0:
+        cfaces[:, :, nxh - 1, 6] = -1
+        tri_if = _split_quads(quads[:, :, nxh - 1, 5])                        # (nz, ny, 2, 3)
+        cfaces[:, :, nxh - 1, 5, :] = -1
+        cfaces[:, :, nxh - 1, 5, :3] = tri_if[:, :, 0]
+        cfaces[:, :, nxh - 1, 6, :3] = tri_if[:, :, 1]
+        ctr = nbase + np.arange(nz * ny * nxt, dtype=np.int64).reshape(nz, ny, nxt)
+        tris = _split_quads(quads[:, :, nxh:])                                # (nz, ny, nxt, 6, 2, 3)
+        a, b, d = tris[..., 0], tris[..., 1], tris[..., 2]
+        c = np.broadcast_to(ctr[:, :, :, None, None], a.shape)
+        tets = np.stack([a, b, d, c], axis=-1).reshape(nz, ny, 12 * nxt, 4)
+        cnodes[:, :, nxh:, :4] = tets
+        tf = np.stack([np.stack([a, d, b], -1), np.stack([a, b, c], -1),
+                       np.stack([b, d, c], -1), np.stack([a, c, d], -1)], axis=-2)
+        cfaces[:, :, nxh:, :4, :3] = tf.reshape(nz, ny, 12 * nxt, 4, 3)
+    cnodes = cnodes.reshape(-1, 8)
+    cfaces = cfaces.reshape(-1, 7, 4)
+    valid = cfaces[:, :, 0] >= 0
+    faces = cfaces[valid]
+    ptr = np.concatenate([[0], np.cumsum(valid.sum(axis=1))]).astype(np.int64)
+    is_hex = np.zeros((nz, ny, per_row), bool)
+    is_hex[:, :, :nxh] = True
+    is_hex = is_hex.ravel()
+    zones = {"hex": np.nonzero(is_hex)[0], "tet": np.nonzero(~is_hex)[0]}
+    # boundary faces: met once, no centre node, all nodes on one outer plane
+    keys = _face_keys(faces)
+    face_of_entry, _, _ = _first_encounter_faces(keys)
+    once = np.bincount(face_of_entry)[face_of_entry] == 1
+    live = faces >= 0
+    cand = once & ~np.any(live & (faces >= nbase), axis=1)
+    fidx = np.nonzero(cand)[0]
+    fl = faces[fidx]
+    lv = fl >= 0
+    xyz = nodes[np.where(lv, fl, 0)]                                          # (m, 4, 3)
+    plane = np.full(len(fidx), -1, np.int64)
+    for pi, (axis, val) in enumerate([(0, 0.0), (0, Lx), (1, 0.0), (1, Ly), (2, 0.0), (2, Lz)]):
+        on = np.all(~lv | (np.abs(xyz[:, :, axis] - val) < 1e-10), axis=1)
+        plane = np.where((plane < 0) & on, pi, plane)
+    keys6 = ["x_min", "x_max", "y_min", "y_max", "z_min", "z_max"]
+    plane_name = [boundary_names.get(kname, kname) for kname in keys6]
+    bdict = {boundary_names[kname]: np.empty((0, 4), np.int64) for kname in boundary_names}
+    for name in dict.fromkeys(plane_name):
+        mine = [pi for pi in range(6) if plane_name[pi] == name]
+        bdict[name] = fl[np.isin(plane, mine)]        # in the order the faces are met
+    return nodes, RaggedCells(cnodes, faces, ptr), bdict, zones
+
+
+def generate_hybrid_hex_tet_mesh(Lx=2.0, Ly=0.1, Lz=0.1, nx=20, ny=8, nz=8, tet_fraction=0.5,
+                                 boundary_names=None) -> FVMesh:
+    """Drop-in ``generate_hybrid_hex_tet_mesh``
+    (mesh/hybrid_mesh_generator.py:93-270): same arguments, same mesh."""
+    nodes, rag, bdict, zones = hybrid_hex_tet_arrays(Lx, Ly, Lz, nx, ny, nz, tet_fraction,
+                                                     boundary_names)
+    return build_fvmesh_from_arrays(nodes, None, bdict, cell_zone_map=zones, ndim=3,
+                                    cell_face_node_list=rag)

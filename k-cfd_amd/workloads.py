@@ -19,6 +19,9 @@ Configurations (sizes are parameters; defaults are the full sizes):
                  vector potential around each face), Dirichlet 0, rhs =
                  U[-1,1]*V plus the van Leer MUSCL deferred correction from
                  Green-Gauss gradients of a smooth field.
+  hybrid_*       The hex_pressure / hex_momentum recipes on the mixed
+                 hex/tet duct of mesh/hybrid_mesh_generator.py (2 x 0.1 x
+                 0.1, hexes on the inlet half, 12 tets per hex beyond).
   tet_twofluid   Delaunay tetrahedra of jittered lattice points, cells
                  permuted then RCM-reordered; liquid-phase momentum of
                  models/two_fluid.py:520-612 (alpha_g ~ U[0.05,0.6],
@@ -100,23 +103,28 @@ def _circulation_flux(mesh, A):
     """Face flux = circulation of A around the face (trapezoid per edge), oriented
     to the stored normal.  Each edge integral is antisymmetric, so the flux
     out of every cell sums to zero up to rounding."""
-    fn = mesh.face_nodes
+    fn_all = mesh.face_nodes
     P = mesh.nodes
-    k = fn.shape[1]
     Av = A(P)
-    circ = np.zeros(len(fn))
-    for j in range(k):
-        a = fn[:, j]
-        b = fn[:, (j + 1) % k]
-        circ += np.sum(0.5 * (Av[a] + Av[b]) * (P[b] - P[a]), axis=1)
-    # Newell normal of the node order (unnormalised) vs the stored normal
-    pts = P[fn]
-    nxt = np.roll(pts, -1, axis=1)
-    nn = np.stack([np.sum((pts[:, :, 1] - nxt[:, :, 1]) * (pts[:, :, 2] + nxt[:, :, 2]), 1),
-                   np.sum((pts[:, :, 2] - nxt[:, :, 2]) * (pts[:, :, 0] + nxt[:, :, 0]), 1),
-                   np.sum((pts[:, :, 0] - nxt[:, :, 0]) * (pts[:, :, 1] + nxt[:, :, 1]), 1)], 1)
-    sgn = np.where(np.sum(nn * mesh.face_normal, axis=1) >= 0, 1.0, -1.0)
-    return sgn * circ
+    out = np.zeros(len(fn_all))
+    nnodes = (fn_all >= 0).sum(axis=1)      # mixed meshes pad triangles with -1
+    for k in np.unique(nnodes):
+        sel = np.nonzero(nnodes == k)[0]
+        fn = fn_all[sel, :k]
+        circ = np.zeros(len(fn))
+        for j in range(k):
+            a = fn[:, j]
+            b = fn[:, (j + 1) % k]
+            circ += np.sum(0.5 * (Av[a] + Av[b]) * (P[b] - P[a]), axis=1)
+        # Newell normal of the node order (unnormalised) vs the stored normal
+        pts = P[fn]
+        nxt = np.roll(pts, -1, axis=1)
+        nn = np.stack([np.sum((pts[:, :, 1] - nxt[:, :, 1]) * (pts[:, :, 2] + nxt[:, :, 2]), 1),
+                       np.sum((pts[:, :, 2] - nxt[:, :, 2]) * (pts[:, :, 0] + nxt[:, :, 0]), 1),
+                       np.sum((pts[:, :, 0] - nxt[:, :, 0]) * (pts[:, :, 1] + nxt[:, :, 1]), 1)], 1)
+        sgn = np.where(np.sum(nn * mesh.face_normal[sel], axis=1) >= 0, 1.0, -1.0)
+        out[sel] = sgn * circ
+    return out
 
 
 def dirichlet_all(mesh, value=0.0):
@@ -205,6 +213,35 @@ def hex_momentum(n=128, seed=1002, mesh=None) -> Case:
               ("source", dict(S=src)),
               ("muscl", dict(phi=phi, phib=phib, F=F, limiter="van_leer"))]
     return Case("hex_momentum", mesh, recipe, "ilu0", meta=dict(n=n, seed=seed))
+
+
+def hybrid_mesh(nx=64, ny=None, nz=None, tet_fraction=0.5, L=(2.0, 0.1, 0.1)):
+    """Mixed hex/tet duct of mesh/hybrid_mesh_generator.py:93-270 (the
+    geometry of verification/case22_hybrid_mesh.py): hexes on the inlet
+    half, 12 tets per hex on the outlet half."""
+    from .mesh import generate_hybrid_hex_tet_mesh
+    ny = max(1, nx // 2) if ny is None else ny
+    nz = ny if nz is None else nz
+    return generate_hybrid_hex_tet_mesh(L[0], L[1], L[2], nx, ny, nz, tet_fraction)
+
+
+def hybrid_pressure(nx=64, ny=None, nz=None, seed=1007, mesh=None) -> Case:
+    """The hex_pressure recipe on the mixed hex/tet duct (p' = 0 at the
+    outlet)."""
+    mesh = hybrid_mesh(nx, ny, nz) if mesh is None else mesh
+    case = pressure_case(mesh, seed, dirichlet=["outlet"])
+    case.name = "hybrid_pressure"
+    return case
+
+
+def hybrid_momentum(nx=64, ny=None, nz=None, seed=1008, mesh=None) -> Case:
+    """The hex_momentum recipe on the mixed hex/tet duct: diffusion, upwind
+    convection by a divergence-free circulation flux through the hex, tet
+    and interface faces, Dirichlet 0, source, van Leer MUSCL correction."""
+    mesh = hybrid_mesh(nx, ny, nz) if mesh is None else mesh
+    case = hex_momentum(seed=seed, mesh=mesh)
+    case.name = "hybrid_momentum"
+    return case
 
 
 def tet_mesh(npts_target=200_000, seed=1003, jitter=0.3):

@@ -158,6 +158,98 @@ def build_mesh(node_coords, cell_node_list, boundary_faces_dict, ndim=2,
 
 
 # ---------------------------------------------------------------------------
+# hybrid hex/tet generator (mesh/hybrid_mesh_generator.py)
+# ---------------------------------------------------------------------------
+
+def _split_quad(q):
+    """hybrid_mesh_generator.py:14-27: two triangles through the quad's
+    smallest node, so that both cells sharing the quad split it alike."""
+    a, b, c, d = (int(v) for v in q)
+    lo = min(a, b, c, d)
+    return [(a, b, c), (a, c, d)] if lo in (a, c) else [(a, b, d), (b, c, d)]
+
+
+def _hex_quads(h):
+    """hybrid_mesh_generator.py:76-86 (and :50-57): z-, z+, y-, y+, x-, x+."""
+    return [[h[0], h[3], h[2], h[1]], [h[4], h[5], h[6], h[7]], [h[0], h[1], h[5], h[4]],
+            [h[2], h[3], h[7], h[6]], [h[0], h[4], h[7], h[3]], [h[1], h[2], h[6], h[5]]]
+
+
+def hybrid_hex_tet(Lx=2.0, Ly=0.1, Lz=0.1, nx=20, ny=8, nz=8, tet_fraction=0.5,
+                   boundary_names=None):
+    """generate_hybrid_hex_tet_mesh (hybrid_mesh_generator.py:93-270) up to
+    the builder call, one cell at a time: returns (nodes, cell_node_list,
+    cell_face_node_list, boundary_faces_dict, cell_zone_map)."""
+    if boundary_names is None:
+        boundary_names = {"x_min": "inlet", "x_max": "outlet", "y_min": "wall_bottom",
+                          "y_max": "wall_top", "z_min": "wall_front", "z_max": "wall_back"}
+    dx, dy, dz = Lx / nx, Ly / ny, Lz / nz
+    nxh = max(1, int(nx * (1.0 - tet_fraction)))
+    nxt = nx - nxh
+    nbase = (nx + 1) * (ny + 1) * (nz + 1)
+
+    def nid(i, j, k):
+        return (k * (ny + 1) + j) * (nx + 1) + i
+
+    nodes = np.zeros((nbase + nxt * ny * nz, 3))
+    for k in range(nz + 1):
+        for j in range(ny + 1):
+            for i in range(nx + 1):
+                nodes[nid(i, j, k)] = [i * dx, j * dy, k * dz]
+    centre = {}
+    nxt_id = nbase
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nxh, nx):
+                nodes[nxt_id] = [(i + 0.5) * dx, (j + 0.5) * dy, (k + 0.5) * dz]
+                centre[(i, j, k)] = nxt_id
+                nxt_id += 1
+    planes = {}
+    for key, axis, val in [("x_min", 0, 0.0), ("x_max", 0, Lx), ("y_min", 1, 0.0), ("y_max", 1, Ly),
+                           ("z_min", 2, 0.0), ("z_max", 2, Lz)]:
+        planes[key] = {v for v in range(nbase) if abs(nodes[v, axis] - val) < 1e-10}
+    cells, cell_faces, zones = [], [], {"hex": [], "tet": []}
+    count = {}
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                h = [nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j + 1, k), nid(i, j + 1, k),
+                     nid(i, j, k + 1), nid(i + 1, j, k + 1), nid(i + 1, j + 1, k + 1), nid(i, j + 1, k + 1)]
+                if i < nxh:
+                    fl = [np.array(q, dtype=int) for q in _hex_quads(h)]
+                    if i == nxh - 1 and nxt > 0:   # interface: the x+ quad as the tets split it
+                        fl = fl[:5] + [np.array(t, dtype=int) for t in _split_quad(fl[5])]
+                    zones["hex"].append(len(cells))
+                    cells.append(np.array(h, dtype=int))
+                    cell_faces.append(fl)
+                else:
+                    c = centre[(i, j, k)]
+                    for q in _hex_quads(h):
+                        for a, b, d in _split_quad(q):
+                            zones["tet"].append(len(cells))
+                            cells.append(np.array([a, b, d, c], dtype=int))
+                            cell_faces.append([np.array(f, dtype=int) for f in
+                                               ([a, d, b], [a, b, c], [b, d, c], [a, c, d])])
+    for fl in cell_faces:
+        for f in fl:
+            key = tuple(sorted(int(v) for v in f))
+            count[key] = count.get(key, 0) + 1
+    bdict = {boundary_names[k]: [] for k in boundary_names}
+    seen = set()
+    for fl in cell_faces:
+        for f in fl:
+            key = tuple(sorted(int(v) for v in f))
+            if key in seen or count[key] != 1 or any(int(v) >= nbase for v in f):
+                continue
+            seen.add(key)
+            for pk, pset in planes.items():
+                if set(int(v) for v in f) <= pset:
+                    bdict.setdefault(boundary_names.get(pk, pk), []).append(f)
+                    break
+    return nodes, cells, cell_faces, bdict, zones
+
+
+# ---------------------------------------------------------------------------
 # FVMSystem and operators (core/fvm_operators.py)
 # ---------------------------------------------------------------------------
 

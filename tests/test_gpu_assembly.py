@@ -33,10 +33,13 @@ def _cases():
         "hex_pressure": lambda: w.hex_pressure(6, seed=12),
         "hex_momentum": lambda: w.hex_momentum(5, seed=13),
         "tet_twofluid": lambda: w.tet_twofluid(400, seed=14),
+        "hybrid_pressure": lambda: w.hybrid_pressure(8, 3, 3, seed=15),
+        "hybrid_momentum": lambda: w.hybrid_momentum(8, 3, 3, seed=16),
     }
 
 
-@pytest.mark.parametrize("name", ["quad_convdiff", "hex_pressure", "hex_momentum", "tet_twofluid"])
+@pytest.mark.parametrize("name", ["quad_convdiff", "hex_pressure", "hex_momentum", "tet_twofluid",
+                                  "hybrid_pressure", "hybrid_momentum"])
 def test_assembly_parity(ctx, name):
     device, workloads = _kc()
     case = _cases()[name]()
@@ -51,12 +54,12 @@ def test_assembly_parity(ctx, name):
     exact = int(np.sum(Ag.data == Ao.data))
     print(f"{name}: nnz={Ao.nnz} values max rel {ev:.2e} (bit-equal {exact}/{Ao.nnz}); rhs {er:.2e}")
     assert ev <= TOL and er <= TOL
-    if case.rhs is None or name != "hex_pressure":
+    if case.rhs is None or not name.endswith("_pressure"):
         ed = _relmax(A.get_diag_tracker(), diag_o)
         assert ed <= TOL
 
 
-@pytest.mark.parametrize("name", ["quad_convdiff", "hex_momentum", "tet_twofluid"])
+@pytest.mark.parametrize("name", ["quad_convdiff", "hex_momentum", "tet_twofluid", "hybrid_momentum"])
 def test_green_gauss_and_muscl(ctx, name):
     device, workloads = _kc()
     from kcfd_amd._lib import DevBuf, LIMITERS, check, lib

@@ -220,3 +220,60 @@ def test_zero_rhs_and_maxiter(ctx):
     xo, info_o, it_o = osolve.solve_bicgstab(A, b, rtol=1e-14, precond="jacobi", maxiter=3)
     assert res.info == info_o == 3 and res.iters == 3
     assert rel_l2(x, xo) < 1e-12
+
+
+@pytest.mark.parametrize("pc,order", [("jacobi", None), ("ilu0", "natural"), ("ilu0", "color")])
+def test_hybrid_mesh_solve(ctx, pc, order):
+    """Mixed hex/tet duct (mesh/hybrid_mesh_generator.py): the device-assembled
+    pressure system solved in the oracle's band, natural and colour-ordered
+    ILU(0) factors equal to the C oracle's.  The three-colour ILU(0) run is
+    chaotic: over 40 one-ulp perturbations of b scipy itself takes 98 to 154
+    iterations, so its band is sampled 40 times."""
+    device, workloads = _kc()
+    case = workloads.hybrid_pressure(12, 4, 4, seed=1007)
+    A, b, _ = oracle_system(case)
+    dm = device.DeviceMesh(case.mesh, ctx)
+    Ad = workloads.assemble_device(dm, case)
+    assert np.array_equal(Ad.to_scipy().indptr, A.indptr)
+    M = device.DeviceMatrix.from_csr(ctx, A)
+    M.setup_precond(pc, "fp64", order=order)
+    opc = pc
+    if pc == "ilu0":
+        lu, nl, nu = M.ilu0_factors()
+        if order == "color":
+            perm, nc = M.ilu0_order()
+            ref = osolve.ILU0Ordered(A, osolve.color_order(A)[0]).f
+            assert np.array_equal(perm, osolve.color_order(A)[0])
+            opc = "ilu0_color"
+        else:
+            ref = osolve.ILU0(A)
+            opc = "ilu0_struct"
+        assert np.max(np.abs(lu - ref.lu) / np.maximum(np.abs(ref.lu), 1e-300)) <= 1e-14
+    x, res = M.solve(b, rtol=1e-8, precond=pc)
+    assert res.info == 0 and res.rel_resid < 1e-8
+    _check_against_band(x, res, A, b, f"hybrid {pc} {order}", rtol=1e-8, precond=opc,
+                        nrep=40 if order == "color" else 6)
+
+
+def test_hybrid_full_size(ctx):
+    """The mixed hex/tet duct at 64 x 32 x 32 (425,984 cells: 32,768 hexes,
+    393,216 tets): device assembly equal to the oracle's to 1e-14, and the
+    mixed-precision colour-ILU(0) solve FP64-verified below 1e-8."""
+    device, workloads = _kc()
+    case = workloads.hybrid_pressure(64, seed=1007)
+    assert case.mesh.n_cells == 425_984
+    A, b, _ = oracle_system(case)
+    dm = device.DeviceMesh(case.mesh, ctx)
+    Ad = workloads.assemble_device(dm, case)
+    Ag = Ad.to_scipy()
+    assert np.array_equal(Ag.indptr, A.indptr) and np.array_equal(Ag.indices, A.indices)
+    ev = np.max(np.abs(Ag.data - A.data) / np.maximum(np.abs(A.data), 1e-300))
+    M = device.DeviceMatrix.from_csr(ctx, A)
+    M.setup_precond("ilu0", "mixed", order="color")
+    x, res = M.solve(b, rtol=1e-8, precond="ilu0", precision="mixed")
+    tr = osolve.true_rel_residual(A, x, b)
+    print(f"hybrid 64x32x32: nnz={A.nnz} assembly max rel {ev:.2e}; mixed colour ILU(0) "
+          f"{res.iters} it, {res.refinements} replacements, resid gpu {res.rel_resid:.2e} cpu {tr:.2e}, "
+          f"{res.ms:.1f} ms")
+    assert ev <= 1e-14
+    assert res.info == 0 and tr < 1e-8

@@ -97,3 +97,74 @@ def test_structure_counts_config_sizes():
     assert np.array_equal(rows[so[it]], m3.owner[it]) and np.array_equal(ix[so[it]], m3.neighbour[it])
     assert np.array_equal(rows[sn[it]], m3.neighbour[it]) and np.array_equal(ix[sn[it]], m3.owner[it])
     assert np.array_equal(ix[ds], np.arange(m3.n_cells))
+
+
+HYBRID_ARGS = [(2.0, 0.1, 0.1, 4, 2, 2, 0.5), (1.0, 0.5, 0.3, 5, 3, 2, 0.4),
+               (2.0, 0.1, 0.1, 3, 1, 1, 0.0), (1.0, 1.0, 1.0, 2, 2, 2, 1.0)]
+
+
+@pytest.mark.parametrize("args", HYBRID_ARGS)
+def test_hybrid_generator_equals_literal(args):
+    """The vectorised hybrid hex/tet generator against the loop restatement of
+    mesh/hybrid_mesh_generator.py:93-270: identical nodes, cells, per-cell
+    face lists, boundary face lists (in order) and zones; the built meshes
+    equal as for the uniform meshes."""
+    M, _ = _pkg()
+    nodes, rag, bd, zones = M.hybrid_hex_tet_arrays(*args)
+    ln, lc, lf, lbd, lz = literal.hybrid_hex_tet(*args)
+    assert np.array_equal(nodes, ln)
+    assert rag.cell_nodes.shape[0] == len(lc)
+    for c in range(len(lc)):
+        k = len(lc[c])
+        assert np.array_equal(rag.cell_nodes[c, :k], lc[c]) and np.all(rag.cell_nodes[c, k:] == -1)
+        seg = rag.faces[rag.face_ptr[c]:rag.face_ptr[c + 1]]
+        assert len(seg) == len(lf[c])
+        for f, r in zip(lf[c], seg):
+            assert np.array_equal(r[:len(f)], f) and np.all(r[len(f):] == -1)
+    assert list(bd) == list(lbd)
+    for nm in bd:
+        assert len(bd[nm]) == len(lbd[nm])
+        for r, f in zip(bd[nm], lbd[nm]):
+            assert np.array_equal(r[:len(f)], f) and np.all(r[len(f):] == -1)
+    assert list(zones["hex"]) == lz["hex"] and list(zones["tet"]) == lz["tet"]
+    mv = M.build_fvmesh_from_arrays(nodes, None, bd, cell_zone_map=zones, ndim=3,
+                                    cell_face_node_list=rag)
+    _compare(mv, literal.build_mesh(ln, lc, lbd, ndim=3, cell_face_node_list=lf))
+    # the reference's own list-of-lists input takes the same ragged path
+    mv2 = M.build_fvmesh_from_arrays(ln, lc, lbd, ndim=3, cell_face_node_list=lf)
+    assert np.array_equal(mv2.owner, mv.owner) and np.array_equal(mv2.cell_volume, mv.cell_volume)
+
+
+def test_hybrid_closed_forms():
+    """Counts, volumes (hex dx dy dz; each of a hex's 12 centre tets
+    dx dy dz / 12), outer areas per patch, closed cells, and a conforming
+    hex/tet interface (every interface triangle internal, no 'default'
+    patch)."""
+    M, _ = _pkg()
+    Lx, Ly, Lz, nx, ny, nz, tf = 2.0, 0.1, 0.1, 8, 3, 2, 0.5
+    m = M.generate_hybrid_hex_tet_mesh(Lx, Ly, Lz, nx, ny, nz, tf)
+    nxh = max(1, int(nx * (1 - tf)))
+    nxt = nx - nxh
+    nhex, ntet = nxh * ny * nz, 12 * nxt * ny * nz
+    assert m.n_cells == nhex + ntet
+    assert len(m.cell_zones["hex"]) == nhex and len(m.cell_zones["tet"]) == ntet
+    h = (Lx / nx) * (Ly / ny) * (Lz / nz)
+    assert np.allclose(m.cell_volume[m.cell_zones["hex"]], h, rtol=1e-13)
+    assert np.allclose(m.cell_volume[m.cell_zones["tet"]], h / 12, rtol=1e-12)
+    assert abs(m.cell_volume.sum() - Lx * Ly * Lz) <= 1e-13 * Lx * Ly * Lz
+    areas = {nm: m.face_area[f].sum() for nm, f in m.boundary_patches.items()}
+    want = {"inlet": Ly * Lz, "outlet": Ly * Lz, "wall_bottom": Lx * Lz, "wall_top": Lx * Lz,
+            "wall_front": Lx * Ly, "wall_back": Lx * Ly}
+    assert set(areas) == set(want)
+    for nm in want:
+        assert abs(areas[nm] - want[nm]) <= 1e-13 * want[nm]
+    # boundary faces on the outlet are triangles (two per hex face), on the inlet quads
+    assert len(m.boundary_patches["outlet"]) == 2 * ny * nz
+    assert len(m.boundary_patches["inlet"]) == ny * nz
+    S = np.zeros((m.n_cells, 3))
+    Sf = m.face_normal * m.face_area[:, None]
+    np.add.at(S, m.owner, Sf)
+    inner = m.neighbour >= 0
+    np.add.at(S, m.neighbour[inner], -Sf[inner])
+    assert np.max(np.abs(S)) <= 1e-15
+    assert m.n_boundary_faces == sum(len(f) for f in m.boundary_patches.values())
