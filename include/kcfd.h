@@ -319,6 +319,22 @@ int kcfd_simple_fix_reference(kcfd_mat *m);
 int kcfd_simple_correct(kcfd_mesh *mesh, const double *pp, const double *dP,
                         const int *pdirichlet, double alpha_p, double *U,
                         double *p);
+
+/* The two-fluid pressure-correction step, models/two_fluid.py:641-725
+ * (_solve_pressure_correction_two_phase) around its solve.  Adds to a reset
+ * matrix of the mesh: the face coefficients (rho_l al_f V_f / aPl_f +
+ * rho_g ag_f V_f / aPg_f) A / d_PN, the mass imbalance of
+ * mf_l al_O + mf_g ag_O per face, and 1e10 on diagonal 0 (:645-690).
+ * alpha_*, aP_* (dev, n); mf_* (dev, n_faces). */
+int kcfd_twofluid_pressure(kcfd_mesh *mesh, kcfd_mat *m, const double *alpha_l,
+                           const double *alpha_g, const double *aP_l, const double *aP_g,
+                           double rho_l, double rho_g, const double *mf_l, const double *mf_g);
+/* :699-723: g = sum of +-(p'_N - p'_O) n A over internal faces / V;
+ * U_k += alpha_p (-alpha_k V / max(aP_k, 1e-20)) g for both phases;
+ * p += alpha_p p'.  U_l, U_g (dev, n*ndim row-major), p (dev, n), updated. */
+int kcfd_twofluid_correct(kcfd_mesh *mesh, const double *pp, const double *alpha_l,
+                          const double *alpha_g, const double *aP_l, const double *aP_g,
+                          double alpha_p, double *U_l, double *U_g, double *p);
 /* green_gauss_gradient (core/gradient.py:12-48); phi (dev n), phib (dev,
  * boundary faces), grad (dev, n*ndim, row-major) */
 int kcfd_grad_gg(kcfd_mesh *mesh, const double *phi, const double *phib,

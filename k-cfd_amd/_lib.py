@@ -132,6 +132,8 @@ SIGNATURES = {
     "kcfd_simple_rhs": ([_P, _P, _P], _I),
     "kcfd_simple_fix_reference": ([_P], _I),
     "kcfd_simple_correct": ([_P, _P, _P, _P, ctypes.c_double, _P, _P], _I),
+    "kcfd_twofluid_pressure": ([_P, _P, _P, _P, _P, _P, _D, _D, _P, _P], _I),
+    "kcfd_twofluid_correct": ([_P, _P, _P, _P, _P, _P, _D, _P, _P, _P], _I),
     "kcfd_muscl": ([_P, _P, _P, _P, _I, _P], _I),
     "kcfd_mass_flux": ([_P, _P, _P, _D, _P], _I),
     "kcfd_last_kernel_ms": ([_P, ctypes.POINTER(_D)], _I),

@@ -362,6 +362,103 @@ __global__ __launch_bounds__(kBlock) void k_simple_correct(MeshView m, const dou
     p[c] += alpha_p * pp[c];
 }
 
+// ------------------------------ two-fluid pressure-correction step
+// models/two_fluid.py:641-725.
+
+// Per face (:655-688): coefficient (rho_l al_f V_f / max(aPl_f, 1e-30) +
+// rho_g ag_f V_f / max(aPg_f, 1e-30)) A / d_PN with face averages
+// 0.5 (x_O + x_N), written to both off-diagonal slots; the face's total mass
+// flux mf_l al_O + mf_g ag_O for the imbalance.  A face with d_PN < 1e-30
+// contributes nothing (the reference skips it whole); boundary faces add
+// only their mass flux.
+__global__ __launch_bounds__(kBlock) void k_tf_press_face(MeshView m, const double *__restrict__ al,
+                                                          const double *__restrict__ ag,
+                                                          const double *__restrict__ apl,
+                                                          const double *__restrict__ apg, double rho_l,
+                                                          double rho_g, const double *__restrict__ mfl,
+                                                          const double *__restrict__ mfg,
+                                                          double *__restrict__ vals,
+                                                          double *__restrict__ fco,
+                                                          double *__restrict__ tmf) {
+#pragma clang fp contract(off)
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    const int o = m.owner[f], nb = m.nb[f];
+    const double t = mfl[f] * al[o] + mfg[f] * ag[o];
+    if (nb >= 0) {
+        const double d = m.dpn[f];
+        if (d < 1e-30 || m.slot_on[f] < 0) {
+            fco[f] = 0.0;
+            tmf[f] = 0.0;
+            return;
+        }
+        const double alf = 0.5 * (al[o] + al[nb]);
+        const double agf = 0.5 * (ag[o] + ag[nb]);
+        const double aplf = 0.5 * (apl[o] + apl[nb]);
+        const double apgf = 0.5 * (apg[o] + apg[nb]);
+        const double vf = 0.5 * (m.vol[o] + m.vol[nb]);
+        const double cl = rho_l * alf * vf / fmax(aplf, 1e-30);
+        const double cg = rho_g * agf * vf / fmax(apgf, 1e-30);
+        const double c = (cl + cg) * m.area[f] / d;
+        vals[m.slot_on[f]] += -c;
+        vals[m.slot_no[f]] += -c;
+        fco[f] = c;
+    } else {
+        fco[f] = 0.0;
+    }
+    tmf[f] = t;
+}
+
+// the reference cell (:690): diag_0 += 1e10 after every face
+__global__ void k_tf_ref_cell(const int *__restrict__ dpos, double *__restrict__ vals,
+                              double *__restrict__ diagt) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        vals[dpos[0]] += 1e10;
+        diagt[0] += 1e10;
+    }
+}
+
+// Velocity and pressure correction (:699-723): g = sum over the cell's
+// internal faces, in face order, of +-(p'_N - p'_O) n A (owner +, neighbour
+// -), divided by V when V > 1e-30; then for each phase
+// U_k += alpha_p (-alpha_k V / max(aP_k, 1e-20)) g, and p += alpha_p p'.
+// U_l and U_g are (n, ndim) row-major.
+__global__ __launch_bounds__(kBlock) void k_tf_correct(MeshView m, const double *__restrict__ pp,
+                                                       const double *__restrict__ al,
+                                                       const double *__restrict__ ag,
+                                                       const double *__restrict__ apl,
+                                                       const double *__restrict__ apg, double alpha_p,
+                                                       double *__restrict__ Ul, double *__restrict__ Ug,
+                                                       double *__restrict__ p) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    const int nd = m.ndim;
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int q = m.cf_ptr[c]; q < m.cf_ptr[c + 1]; ++q) {
+        const int f = m.cf_face[q];
+        const int nb = m.nb[f];
+        if (nb < 0) continue;
+        const double dp = pp[nb] - pp[m.owner[f]];
+        const double A = m.area[f];
+        for (int k = 0; k < nd; ++k) {
+            const double fl = dp * m.normal[3 * f + k] * A;
+            if (m.cf_side[q] > 0) g[k] += fl;
+            else g[k] -= fl;
+        }
+    }
+    const double V = m.vol[c];
+    if (V > 1e-30)
+        for (int k = 0; k < nd; ++k) g[k] /= V;
+    const double kl = -al[c] * V / fmax(apl[c], 1e-20);
+    const double kg = -ag[c] * V / fmax(apg[c], 1e-20);
+    for (int k = 0; k < nd; ++k) {
+        Ul[(size_t)c * nd + k] += alpha_p * (kl * g[k]);
+        Ug[(size_t)c * nd + k] += alpha_p * (kg * g[k]);
+    }
+    p[c] += alpha_p * pp[c];
+}
+
 // ------------------------------------------------------- Green-Gauss
 // core/gradient.py:12-48
 __global__ __launch_bounds__(kBlock) void k_gg(MeshView m, const double *__restrict__ phi,
@@ -889,6 +986,35 @@ int kcfd_simple_correct(kcfd_mesh *mesh, const double *pp, const double *dP, con
     Timer t(mesh->ctx);
     hipLaunchKernelGGL(k_simple_correct, dim3(blocks(mesh->n)), dim3(kBlock), 0, s, view(mesh), pp, dP,
                        mesh->d_kinds, alpha_p, U, p);
+    return t.stop();
+}
+
+int kcfd_twofluid_pressure(kcfd_mesh *mesh, kcfd_mat *a, const double *alpha_l, const double *alpha_g,
+                           const double *aP_l, const double *aP_g, double rho_l, double rho_g,
+                           const double *mf_l, const double *mf_g) {
+    CHECK_PAIR(mesh, a);
+    KCFD_ARG(alpha_l && alpha_g && aP_l && aP_g && mf_l && mf_g, "null argument");
+    KCFD_ARG(a->n > 0 && !a->h_diagpos.empty() && a->h_diagpos[0] >= 0, "row 0 has no diagonal entry");
+    std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    hipStream_t s = a->ctx->stream;
+    Timer t(a->ctx);
+    MeshView v = view(mesh);
+    hipLaunchKernelGGL(k_tf_press_face, dim3(blocks(mesh->nf)), dim3(kBlock), 0, s, v, alpha_l, alpha_g, aP_l,
+                       aP_g, rho_l, rho_g, mf_l, mf_g, a->val, mesh->fco, mesh->fco2);
+    hipLaunchKernelGGL(k_press_cell, dim3(blocks(mesh->n)), dim3(kBlock), 0, s, v, mesh->fco, a->val, a->diagt);
+    hipLaunchKernelGGL(k_simple_rhs, dim3(blocks(mesh->n)), dim3(kBlock), 0, s, v, mesh->fco2, a->rhs);
+    hipLaunchKernelGGL(k_tf_ref_cell, dim3(1), dim3(64), 0, s, a->diagpos, a->val, a->diagt);
+    return t.stop();
+}
+
+int kcfd_twofluid_correct(kcfd_mesh *mesh, const double *pp, const double *alpha_l, const double *alpha_g,
+                          const double *aP_l, const double *aP_g, double alpha_p, double *U_l, double *U_g,
+                          double *p) {
+    KCFD_ARG(mesh && pp && alpha_l && alpha_g && aP_l && aP_g && U_l && U_g && p, "null argument");
+    std::lock_guard<std::recursive_mutex> g(mesh->ctx->mu);
+    Timer t(mesh->ctx);
+    hipLaunchKernelGGL(k_tf_correct, dim3(blocks(mesh->n)), dim3(kBlock), 0, mesh->ctx->stream, view(mesh), pp,
+                       alpha_l, alpha_g, aP_l, aP_g, alpha_p, U_l, U_g, p);
     return t.stop();
 }
 

@@ -1,5 +1,6 @@
-"""The SIMPLE pressure-correction step on the device
-(models/single_phase.py:274-365), around the hot-path solve.
+"""The SIMPLE pressure-correction steps on the device, around the hot-path
+solve: single-phase (models/single_phase.py:274-365) and two-fluid
+(models/two_fluid.py:641-725).
 
 ``pressure_correction_step`` keeps every array on the device: d_P from the
 momentum diagonals (kcfd_simple_dp), the pressure-correction matrix
@@ -54,4 +55,48 @@ def pressure_correction_step(dm: device.DeviceMesh, aP, U: DevBuf, p: DevBuf, mf
     return StepResult(pp, dP, res.as_dict(), A)
 
 
-__all__ = ["pressure_correction_step", "StepResult"]
+@dataclass
+class TwoFluidStepResult:
+    pp: DevBuf          # p' (device, n)
+    solve: dict         # the solve's result
+    matrix: device.DeviceMatrix
+    ratio: float        # ||p'|| / max(||p||, 1e-15) after the update (the reference's return value)
+
+
+def two_fluid_pressure_correction_step(dm: device.DeviceMesh, alpha_l: DevBuf, alpha_g: DevBuf,
+                                       aP_l: DevBuf, aP_g: DevBuf, rho_l: float, rho_g: float,
+                                       mf_l: DevBuf, mf_g: DevBuf, U_l: DevBuf, U_g: DevBuf, p: DevBuf,
+                                       alpha_p: float, precond="ilu0", order="color", rtol=1e-12,
+                                       method="bicgstab",
+                                       matrix: device.DeviceMatrix | None = None) -> TwoFluidStepResult:
+    """``TwoFluidSolver._solve_pressure_correction_two_phase``
+    (models/two_fluid.py:641-725) on the device: the two-phase
+    pressure-correction system (kcfd_twofluid_pressure: coefficients, mass
+    imbalance, 1e10 on the first diagonal), the solve of A p' = b (the
+    reference calls solve_pressure_correction, i.e. spsolve; here the device
+    Krylov solver to ``rtol``), a p' with NaNs replaced by zeros as in the
+    reference, and the velocity corrections of both phases with
+    p += alpha_p p' (kcfd_twofluid_correct).  U_l, U_g (n*ndim, row-major)
+    and p are updated in place."""
+    n = dm.n
+    A = matrix if matrix is not None else dm.new_system()
+    check(lib().kcfd_asm_reset(A.h))
+    check(lib().kcfd_twofluid_pressure(dm.h, A.h, alpha_l.p, alpha_g.p, aP_l.p, aP_g.p, float(rho_l),
+                                       float(rho_g), mf_l.p, mf_g.p))
+    A.setup_precond(precond, "fp64", order=order if precond == "ilu0" else None)
+    pp = DevBuf(dm.ctx, n)
+    pp.fill(0.0)
+    opts = device.default_opts(rtol=rtol, precond=precond, precision="fp64", method=method)
+    res = A.solve_dev(A.rhs_ptr, pp.p, opts, setup=False)
+    ph = pp.download()
+    if np.any(np.isnan(ph)):           # the reference's NaN guard (:694-695)
+        pp.fill(0.0)
+        ph[:] = 0.0
+    check(lib().kcfd_twofluid_correct(dm.h, pp.p, alpha_l.p, alpha_g.p, aP_l.p, aP_g.p, float(alpha_p),
+                                      U_l.p, U_g.p, p.p))
+    ratio = float(np.linalg.norm(ph) / max(np.linalg.norm(p.download()), 1e-15))
+    return TwoFluidStepResult(pp, res.as_dict(), A, ratio)
+
+
+__all__ = ["pressure_correction_step", "StepResult", "two_fluid_pressure_correction_step",
+           "TwoFluidStepResult"]

@@ -379,6 +379,91 @@ def simple_pressure_correction_step(mesh, aP, U, p, mf, rho, alpha_p, dirichlet)
     return pp, U_new, p_new, A, b, dP
 
 
+def two_fluid_pressure_system(mesh, alpha_l, alpha_g, aP_l, aP_g, rho_l, rho_g, mf_l, mf_g):
+    """models/two_fluid.py:645-690: per internal face with d_PN >= 1e-30,
+    coeff = (rho_l al_f V_f / max(aPl_f, 1e-30) + rho_g ag_f V_f /
+    max(aPg_f, 1e-30)) A / d_PN (face averages 0.5 (x_O + x_N)), entries
+    (O,O,c), (N,N,c), (O,N,-c), (N,O,-c); b_O -= m, b_N += m with
+    m = mf_l al_O + mf_g ag_O (boundary faces: b_O -= m); then (0, 0, 1e10).
+    Returns (A, b)."""
+    n = mesh.n_cells
+    al, ag = np.asarray(alpha_l, np.float64), np.asarray(alpha_g, np.float64)
+    apl, apg = np.asarray(aP_l, np.float64), np.asarray(aP_g, np.float64)
+    mfl, mfg = np.asarray(mf_l, np.float64), np.asarray(mf_g, np.float64)
+    V = mesh.cell_volume
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    internal = nb >= 0
+    nbs = np.where(internal, nb, o)
+    d = _norm(mesh.cell_center[nbs] - mesh.cell_center[o])
+    live = ~internal | ~(d < 1e-30)                    # faces the reference does not skip
+    fi = np.nonzero(internal & live)[0]
+    oi, ni, di = o[fi], nb[fi], d[fi]
+    vf = 0.5 * (V[oi] + V[ni])
+    cl = rho_l * (0.5 * (al[oi] + al[ni])) * vf / np.maximum(0.5 * (apl[oi] + apl[ni]), 1e-30)
+    cg = rho_g * (0.5 * (ag[oi] + ag[ni])) * vf / np.maximum(0.5 * (apg[oi] + apg[ni]), 1e-30)
+    c = (cl + cg) * mesh.face_area[fi] / di
+    s = OSystem(n)
+    s.add_entries(_interleave(oi, ni, oi, ni), _interleave(oi, ni, ni, oi), _interleave(c, c, -c, -c))
+    s.add_entries(np.array([0]), np.array([0]), np.array([1e10]))
+    A = s.to_sparse().tocsr()
+    m = mfl * al[o] + mfg * ag[o]
+    b = np.zeros(n)
+    idx = np.stack([o, nbs], 1)
+    val = np.stack([-m, m], 1)
+    msk = np.stack([live, internal & live], 1)
+    np.add.at(b, idx[msk], val[msk])                   # face order per cell
+    return A, b
+
+
+def two_fluid_correction(mesh, pp, alpha_l, alpha_g, aP_l, aP_g, U_l, U_g, p, alpha_p):
+    """models/two_fluid.py:697-723: p += alpha_p p'; g = sum over internal
+    faces of +-(p'_N - p'_O) n A (owner +, neighbour -), / V if V > 1e-30;
+    U_k += alpha_p (-alpha_k V / max(aP_k, 1e-20)) g.
+    Returns (U_l, U_g, p, g)."""
+    n, nd = mesh.n_cells, mesh.ndim
+    pp = np.asarray(pp, np.float64)
+    al, ag = np.asarray(alpha_l, np.float64), np.asarray(alpha_g, np.float64)
+    V = mesh.cell_volume
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    p_new = np.asarray(p, np.float64) + alpha_p * pp
+    fa = np.nonzero(nb >= 0)[0]
+    dp = pp[nb[fa]] - pp[o[fa]]
+    g = np.zeros((n, nd))
+    cells = np.stack([o[fa], nb[fa]], 1).ravel()
+    for k in range(nd):
+        fl = dp * mesh.face_normal[fa, k] * mesh.face_area[fa]
+        np.add.at(g[:, k], cells, np.stack([fl, -fl], 1).ravel())
+    big = V > 1e-30
+    g[big] = g[big] / V[big, None]
+    kl = -al * V / np.maximum(np.asarray(aP_l, np.float64), 1e-20)
+    kg = -ag * V / np.maximum(np.asarray(aP_g, np.float64), 1e-20)
+    Ul = np.array(U_l, np.float64, copy=True)
+    Ug = np.array(U_g, np.float64, copy=True)
+    for k in range(nd):
+        Ul[:, k] = Ul[:, k] + alpha_p * (kl * g[:, k])
+        Ug[:, k] = Ug[:, k] + alpha_p * (kg * g[:, k])
+    return Ul, Ug, p_new, g
+
+
+def two_fluid_pressure_correction_step(mesh, alpha_l, alpha_g, aP_l, aP_g, rho_l, rho_g, mf_l, mf_g,
+                                       U_l, U_g, p, alpha_p):
+    """models/two_fluid.py:641-725, _solve_pressure_correction_two_phase:
+    the system of two_fluid_pressure_system, p' by spsolve with NaNs zeroed
+    (:692-695), then two_fluid_correction.  Returns (p', U_l, U_g, p, A, b,
+    g, ratio) with ratio = ||p'|| / max(||p||, 1e-15), the reference's return
+    value."""
+    from scipy.sparse.linalg import spsolve
+    A, b = two_fluid_pressure_system(mesh, alpha_l, alpha_g, aP_l, aP_g, rho_l, rho_g, mf_l, mf_g)
+    pp = spsolve(A.tocsc(), b)
+    if np.any(np.isnan(pp)):
+        pp = np.zeros(mesh.n_cells)
+    Ul, Ug, p_new, g = two_fluid_correction(mesh, pp, alpha_l, alpha_g, aP_l, aP_g, U_l, U_g, p, alpha_p)
+    ratio = float(np.linalg.norm(pp) / max(np.linalg.norm(p_new), 1e-15))
+    return pp, Ul, Ug, p_new, A, b, g, ratio
+
+
 LIMITERS = {
     "minmod": lambda r: np.maximum(0.0, np.minimum(1.0, r)),
     "superbee": lambda r: np.maximum(0.0, np.maximum(np.minimum(2.0 * r, 1.0), np.minimum(r, 2.0))),

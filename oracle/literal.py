@@ -482,3 +482,65 @@ def mass_flux(m: LMesh, U, Ubnd, rho):
             uf = _bvalue(m, U, Ubnd, fid)
         F[fid] = rho * np.dot(uf, f.normal) * f.area
     return F
+
+
+# ---------------------------------------------------------------------------
+# two-fluid pressure correction (models/two_fluid.py:641-725)
+# ---------------------------------------------------------------------------
+
+def two_fluid_pressure_step(m: LMesh, alpha_l, alpha_g, aP_l, aP_g, rho_l, rho_g, mf_l, mf_g,
+                            U_l, U_g, p, alpha_p):
+    """models/two_fluid.py:641-725, one face and one cell at a time.
+    Returns (p', U_l, U_g, p, A, b, g)."""
+    from scipy.sparse.linalg import spsolve
+    n = m.n_cells
+    s = LSystem(n)
+    for fid, f in enumerate(m.faces):
+        o = f.owner
+        if f.neighbour >= 0:
+            nb = f.neighbour
+            d = np.linalg.norm(m.cells[nb].center - m.cells[o].center)
+            if d < 1e-30:
+                continue
+            alf = 0.5 * (alpha_l[o] + alpha_l[nb])
+            agf = 0.5 * (alpha_g[o] + alpha_g[nb])
+            aplf = 0.5 * (aP_l[o] + aP_l[nb])
+            apgf = 0.5 * (aP_g[o] + aP_g[nb])
+            vf = 0.5 * (m.cells[o].volume + m.cells[nb].volume)
+            c = (rho_l * alf * vf / max(aplf, 1e-30) + rho_g * agf * vf / max(apgf, 1e-30)) * f.area / d
+            s.add_diagonal(o, c)
+            s.add_diagonal(nb, c)
+            s.add_off_diagonal(o, nb, -c)
+            s.add_off_diagonal(nb, o, -c)
+            tm = mf_l[fid] * alpha_l[o] + mf_g[fid] * alpha_g[o]
+            s.add_source(o, -tm)
+            s.add_source(nb, tm)
+        else:
+            tm = mf_l[fid] * alpha_l[o] + mf_g[fid] * alpha_g[o]
+            s.add_source(o, -tm)
+    s.add_diagonal(0, 1e10)
+    A = s.to_sparse()
+    pp = spsolve(A.tocsc(), s.rhs)
+    if np.any(np.isnan(pp)):
+        pp = np.zeros(n)
+    p = np.array(p, np.float64) + alpha_p * pp
+    nd = m.ndim
+    g = np.zeros((n, nd))
+    for f in m.faces:
+        if f.neighbour >= 0:
+            dp = pp[f.neighbour] - pp[f.owner]
+            for k in range(nd):
+                fl = dp * f.normal[k] * f.area
+                g[f.owner, k] += fl
+                g[f.neighbour, k] -= fl
+    for ci in range(n):
+        if m.cells[ci].volume > 1e-30:
+            g[ci] /= m.cells[ci].volume
+    Ul = np.array(U_l, np.float64, copy=True)
+    Ug = np.array(U_g, np.float64, copy=True)
+    for ci in range(n):
+        v = m.cells[ci].volume
+        for k in range(nd):
+            Ul[ci, k] += alpha_p * (-alpha_l[ci] * v / max(aP_l[ci], 1e-20) * g[ci, k])
+            Ug[ci, k] += alpha_p * (-alpha_g[ci] * v / max(aP_g[ci], 1e-20) * g[ci, k])
+    return pp, Ul, Ug, p, A, s.rhs.copy(), g

@@ -234,3 +234,81 @@ def test_simple_correction_closed_form():
     g = ofvm.green_gauss_gradient(sm, ppl, bv)
     inner = np.all((sm.cell_center > 0.2 + 1e-9) & (sm.cell_center < 0.8 - 1e-9), axis=1)
     assert inner.sum() > 0 and np.allclose(g[inner], a[None, :], atol=1e-12)
+
+
+def _hybrid_lm():
+    ln, lc, lf, lbd, _ = literal.hybrid_hex_tet(1.0, 0.5, 0.5, 4, 2, 2, 0.5)
+    lm = literal.build_mesh(ln, lc, lbd, ndim=3, cell_face_node_list=lf)
+    return lm, ofvm.soa_from_literal(lm)
+
+
+TF_MESHES = dict(MESHES, hybrid=_hybrid_lm)
+
+
+def _tf_fields(sm, seed):
+    rng = np.random.default_rng(seed)
+    n, nd, nf = sm.n_cells, sm.ndim, sm.n_faces
+    al = rng.uniform(0.05, 0.95, n)
+    return dict(alpha_l=al, alpha_g=1.0 - al, aP_l=np.exp(rng.normal(0, 0.5, n)),
+                aP_g=0.1 * np.exp(rng.normal(0, 0.5, n)), rho_l=998.2, rho_g=1.225,
+                mf_l=rng.normal(0, 0.1, nf), mf_g=rng.normal(0, 0.01, nf),
+                U_l=rng.normal(0, 1, (n, nd)), U_g=rng.normal(0, 1, (n, nd)), p=rng.normal(0, 1, n),
+                alpha_p=0.3)
+
+
+@pytest.mark.parametrize("name", list(TF_MESHES))
+def test_two_fluid_step_matches_literal(name):
+    """The vectorised two-fluid pressure-correction step against the loop
+    transcription of models/two_fluid.py:641-725 on quad, hex and mixed
+    hex/tet meshes."""
+    lm, sm = TF_MESHES[name]()
+    f = _tf_fields(sm, 5)
+    pp, Ul, Ug, p1, A, b, g, ratio = ofvm.two_fluid_pressure_correction_step(sm, **f)
+    lpp, lUl, lUg, lp1, lA, lb, lg = literal.two_fluid_pressure_step(lm, **f)
+    assert np.array_equal(A.indptr, lA.indptr) and np.array_equal(A.indices, lA.indices)
+    ea = np.max(np.abs(A.data - lA.data) / np.abs(lA.data))
+    print(f"{name}: A max rel {ea:.1e}, b {np.max(np.abs(b - lb)):.1e}")
+    assert ea <= 1e-15 and np.max(np.abs(b - lb)) <= 1e-15 * np.max(np.abs(lb))
+    for x, y in [(pp, lpp), (Ul, lUl), (Ug, lUg), (p1, lp1), (g, lg)]:
+        assert np.max(np.abs(x - y)) <= 1e-12 * max(1.0, np.max(np.abs(y)))
+    assert abs(ratio - np.linalg.norm(lpp) / max(np.linalg.norm(lp1), 1e-15)) <= 1e-12
+
+
+def test_two_fluid_closed_forms():
+    """Uniform hex grid, constant phase fractions and momentum diagonals:
+    every off-diagonal is -(rho_l al h^3/aPl + rho_g ag h^3/aPg) h, A is
+    symmetric with zero row sums except row 0 (+1e10), sum(b) = -sum of the
+    boundary faces' mass fluxes.  The velocity correction's face-difference
+    gradient of a linear p' = a.x vanishes on interior cells and is
+    (a_x, 0, 0) / (-a_x, 0, 0) on the x_min / x_max layers (the reference
+    sums (p'_N - p'_O) n A over internal faces only)."""
+    lm, sm = _hex(5)
+    n, nf = sm.n_cells, sm.n_faces
+    h = 1.0 / 5
+    al, apl, apg = 0.7, 2.0, 0.3
+    rng = np.random.default_rng(3)
+    mfl, mfg = rng.normal(0, 0.1, nf), rng.normal(0, 0.01, nf)
+    A, b = ofvm.two_fluid_pressure_system(sm, np.full(n, al), np.full(n, 1 - al), np.full(n, apl),
+                                          np.full(n, apg), 998.2, 1.225, mfl, mfg)
+    c = (998.2 * al * h ** 3 / apl + 1.225 * (1 - al) * h ** 3 / apg) * h
+    off = A - __import__("scipy").sparse.diags(A.diagonal())
+    assert np.allclose(off.data[off.data != 0], -c, rtol=1e-13)
+    assert abs(A - A.T).max() == 0.0
+    rs = np.asarray(A.sum(axis=1)).ravel()
+    assert abs(rs[0] - 1e10) <= 1e-3 and np.max(np.abs(rs[1:])) <= 1e-12 * c
+    bnd = sm.neighbour < 0
+    assert abs(b.sum() + np.sum((mfl * al + mfg * (1 - al))[bnd])) <= 1e-13
+    a = np.array([0.7, -0.2, 1.3])
+    ppl = sm.cell_center @ a
+    z = np.zeros((n, 3))
+    _, _, _, g = ofvm.two_fluid_correction(sm, ppl, np.full(n, al), np.full(n, 1 - al), np.full(n, apl),
+                                           np.full(n, apg), z, z, np.zeros(n), 0.5)
+    x = sm.cell_center
+    lo, hi = h / 2 + 1e-9, 1 - h / 2 - 1e-9
+    inner = np.all((x > lo) & (x < hi), axis=1)
+    xmin = (x[:, 0] < lo) & np.all((x[:, 1:] > lo) & (x[:, 1:] < hi), axis=1)
+    xmax = (x[:, 0] > hi) & np.all((x[:, 1:] > lo) & (x[:, 1:] < hi), axis=1)
+    assert inner.sum() == 27 and xmin.sum() == 9 and xmax.sum() == 9
+    assert np.max(np.abs(g[inner])) <= 1e-13
+    assert np.allclose(g[xmin], [a[0], 0, 0], atol=1e-13)
+    assert np.allclose(g[xmax], [-a[0], 0, 0], atol=1e-13)
