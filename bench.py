@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SPMV_P_MIXED_HEX128 = 153.9e6   # bytes per launch (145.4 MB read + 8.5 MB written)
 
 
 def log(*a):
@@ -228,6 +229,14 @@ def bench_pressure(args):
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmv_p",
                     "avg_us": round(1e3 * dk["avg_ms"], 2), "bytes_per_launch": dk["bytes"],
                     "solve_avg_gbs": round(solve_gbs, 1), "stream_copy_gbs": round(copy_gbs, 1)}
+        # HBM bytes per k_spmv_p launch measured by the FETCH_SIZE / WRITE_SIZE
+        # counter passes on this exact workload (default shape, mixed,
+        # two-colour ILU(0), 128^3), scaled by the copy-kernel calibration
+        if (args.method == "bicgstab" and args.n == 128 and args.precond == "ilu0" and args.order == "color"
+                and "mixed" in results and not (args.grid_per_cu or args.trsv_per_cu or args.tile_walk)):
+            roofline["traffic"] = TRAFFIC_SPMV_P_MIXED_HEX128
+            roofline["traffic_source"] = ("profiles/hbm_traffic_bicgstab_ilu0_mixed_hex128.md: FETCH_SIZE x 2.000 "
+                                          "+ WRITE_SIZE x 1.000 KiB per dispatch, median working dispatch")
     cpu = None
     if not args.no_cpu:
         try:
