@@ -64,8 +64,10 @@ def test_case16_golden(ctx, pc, tol):
         if res.info == 0:
             assert np.linalg.norm(b - A @ x) < atol
     else:
+        # chaotic near the end: over 40 one-ulp perturbations of b scipy itself
+        # spans 85..101 iterations without a preconditioner
         _check_against_band(x, res, A, b, f"case16 {pc} rtol {rtol}", rtol=rtol, atol=atol,
-                            maxiter=2000, precond=pc)
+                            maxiter=2000, precond=pc, nrep=40)
 
 
 @pytest.mark.parametrize("n", [32, 64, 128])
