@@ -43,7 +43,8 @@ namespace kcfd {
 
 constexpr int kMaxRestart = 64;
 constexpr int kGmBlock = 256;
-constexpr int kMaxGmGrid = 1024;   // workgroups of the vector kernels (partials per reduction)
+constexpr int kMaxGmGrid = 2048;   // workgroups of the vector kernels (partials per reduction)
+constexpr int kGmPerCu = 8;        // 8 workgroups of 4 wavefronts per CU: 8 wavefronts per SIMD
 
 struct GmState {
     // read back by the host once per cycle
@@ -114,11 +115,34 @@ __device__ __forceinline__ bool skipped(const GmState *st, int col) { return st-
 
 // ------------------------------------------------------------ kernels
 
-// y_i = sum_q a_q x_{col q} over the SELL-64 row (A's CSR order, padding last)
+// y_i = sum_q a_q x_{col q} over the SELL-64 row (A's CSR order, padding
+// last).  A wave's 64 rows are one slice, so the width is uniform: the
+// common widths issue every load of the row before the in-order sum.
+template <int W>
+__device__ __forceinline__ double sell_fixed(int base, const int *__restrict__ six,
+                                             const double *__restrict__ sa, const double *__restrict__ x) {
+#pragma clang fp contract(off)
+    int c[W];
+    double a[W], xv[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        c[k] = gl(six)[base + 64 * k];
+        a[k] = gl(sa)[base + 64 * k];
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) xv[k] = gl(x)[c[k]];
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) acc += a[k] * xv[k];
+    return acc;
+}
+
 __device__ __forceinline__ double sell_row(int i, const int *__restrict__ sptr, const int *__restrict__ six,
                                            const double *__restrict__ sa, const double *__restrict__ x) {
 #pragma clang fp contract(off)
-    const int s = i >> 6, base = sptr[s] + (i & 63), w = (sptr[s + 1] - sptr[s]) >> 6;
+    const int s = i >> 6, base = sptr[s] + (i & 63), w = uniform((sptr[s + 1] - sptr[s]) >> 6);
+    if (w == 7) return sell_fixed<7>(base, six, sa, x);
+    if (w == 5) return sell_fixed<5>(base, six, sa, x);
     double acc = 0.0;
     for (int k = 0; k < w; ++k) acc += sa[base + 64 * k] * x[six[base + 64 * k]];
     return acc;
@@ -130,8 +154,14 @@ __global__ __launch_bounds__(kGmBlock) void g_spmv(int n, const int *__restrict_
                                                    const double *__restrict__ x, double *__restrict__ y,
                                                    const GmState *st, int col) {
     if (skipped(st, col)) return;
-    for (int i = blockIdx.x * kGmBlock + threadIdx.x; i < n; i += gridDim.x * kGmBlock)
-        y[i] = sell_row(i, sptr, six, sa, x);
+    // whole waves (one SELL slice each, so one width); the lanes past n in
+    // the last slice, whose entries are never written, compute row n - 1
+    // (same slice) and store nothing
+    const int nr = (n + 63) & ~63;
+    for (int i = blockIdx.x * kGmBlock + threadIdx.x; i < nr; i += gridDim.x * kGmBlock) {
+        const double v = sell_row(min(i, n - 1), sptr, six, sa, x);
+        if (i < n) y[i] = v;
+    }
 }
 
 // r = b - A x, partial sums of r.r
@@ -143,10 +173,14 @@ __global__ __launch_bounds__(kGmBlock) void g_resid(int n, const int *__restrict
                                                     double *__restrict__ part) {
 #pragma clang fp contract(off)
     double v[1] = {0.0};
-    for (int i = blockIdx.x * kGmBlock + threadIdx.x; i < n; i += gridDim.x * kGmBlock) {
-        const double ri = b[i] - sell_row(i, sptr, six, sa, x);
-        r[i] = ri;
-        v[0] += ri * ri;
+    const int nr = (n + 63) & ~63;   // whole waves; lanes past n compute row n - 1
+    for (int i = blockIdx.x * kGmBlock + threadIdx.x; i < nr; i += gridDim.x * kGmBlock) {
+        const double ax = sell_row(min(i, n - 1), sptr, six, sa, x);
+        if (i < n) {
+            const double ri = b[i] - ax;
+            r[i] = ri;
+            v[0] += ri * ri;
+        }
     }
     put_parts<1>(v, part);
 }
@@ -166,6 +200,67 @@ __global__ __launch_bounds__(kGmBlock) void g_norm2_part(int n, const double *__
         }
     }
     put_parts<2>(v, part);
+}
+
+// Two-colour ILU(0) apply on P-order vectors, P A P^T = [[D0, B], [C, D1]]
+// with the n0 colour-0 rows first.  Colour-0 rows have no L entries and
+// colour-1 rows no U entries, so the sequential L then U solves reduce to
+//   FIRST  (colour-1 rows): w_i = (z_i - sum_j l_ij z_j) / u_ii
+//   SECOND (colour-0 rows): w_i = (z_i - sum_j u_ij w_j) / u_ii
+// with the entries in CSR order: the same operations in the same order.
+// The factors are read in the chunked column-major layout (entry k of the
+// row in lane l of chunk c at off[c] + 64k + l, padded with column -1), one
+// wave per 64-row chunk.  Skipped once *skip < col (skip null: never).
+template <int W, bool SECOND>
+__device__ __forceinline__ double ilu_fixed(double acc, int base, const int *__restrict__ colv,
+                                            const double *__restrict__ val, const double *__restrict__ src) {
+#pragma clang fp contract(off)
+    int c[W];
+    double a[W], xv[W];
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+        c[k] = gl(colv)[base + 64 * k];
+        a[k] = gl(val)[base + 64 * k];
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) xv[k] = c[k] >= 0 ? gl(src)[c[k]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < W; ++k)
+        if (c[k] >= 0) acc -= a[k] * xv[k];
+    return acc;
+}
+
+template <bool SECOND>
+__global__ __launch_bounds__(kGmBlock) void g_ilu2(int n0, int nrows, const int *__restrict__ off,
+                                                   const int *__restrict__ colv,
+                                                   const double *__restrict__ val,
+                                                   const double *__restrict__ ud, const double *__restrict__ z,
+                                                   double *__restrict__ w, const int *skip, int col) {
+#pragma clang fp contract(off)
+    if (skip && *skip < col) return;
+    const int lane = threadIdx.x & 63;
+    const int nch = (nrows + 63) >> 6;
+    const int nw = (int)gridDim.x * (kGmBlock / 64);
+    const double *src = SECOND ? w : z;
+    for (int c = uniform((int)blockIdx.x * (kGmBlock / 64) + (int)(threadIdx.x >> 6)); c < nch; c += nw) {
+        const int i = (SECOND ? 0 : n0) + 64 * c + lane;
+        const int o0 = uniform(off[c]);
+        const int wd = uniform((off[c + 1] - o0) >> 6);
+        const bool live = 64 * c + lane < nrows;
+        const int base = o0 + lane;
+        double acc = live ? z[i] : 0.0;
+        if (wd == 6) {
+            acc = ilu_fixed<6, SECOND>(acc, base, colv, val, src);
+        } else if (wd == 4) {
+            acc = ilu_fixed<4, SECOND>(acc, base, colv, val, src);
+        } else {
+            for (int k = 0; k < wd; ++k) {
+                const int j = colv[base + 64 * k];
+                if (j >= 0) acc -= val[base + 64 * k] * src[j];
+            }
+        }
+        if (live) w[i] = acc / ud[i];
+    }
 }
 
 // one workgroup: mode 0 -> fin0 = sqrt(s0), fin1 = sqrt(s1); mode 1 -> rnorm = sqrt(s0)
@@ -423,6 +518,26 @@ __global__ void g_scatter(int n, const int *__restrict__ perm, const double *__r
 
 // ------------------------------------------------------------ host side
 
+// y = M^{-1} x for the solver: the two-colour ILU(0) through the chunked
+// layout (two launches), every other preconditioner through the library's
+// level-scheduled apply
+static int gm_precond(kcfd_mat *m, const double *x, double *y, double *t, bool porder, const int *skip, int col,
+                      int G) {
+    if (m->pc_kind == KCFD_PC_ILU0 && porder && m->two_colour && m->chl_v64 && m->ud64) {
+        hipStream_t s = m->ctx->stream;
+        const int n0 = m->ncolour0, n1 = m->n - m->ncolour0;
+        if (n1 > 0)
+            hipLaunchKernelGGL(g_ilu2<false>, dim3(G), dim3(kGmBlock), 0, s, n0, n1, m->chl_off, m->chl_col,
+                               m->chl_v64, m->ud64, x, y, skip, col);
+        if (n0 > 0)
+            hipLaunchKernelGGL(g_ilu2<true>, dim3(G), dim3(kGmBlock), 0, s, n0, n0, m->chu_off, m->chu_col,
+                               m->chu_v64, m->ud64, x, y, skip, col);
+        KCFD_HIP(hipGetLastError());
+        return KCFD_OK;
+    }
+    return precond_apply_async(m, x, y, t, porder, skip, col);
+}
+
 void gmres_free(kcfd_mat *m) {
     GmresWs *w = m->gmres;
     if (!w) return;
@@ -498,7 +613,7 @@ int gmres_solve(kcfd_mat *m, const double *b, double *x, const kcfd_solve_opts *
     GmresWs *w = m->gmres;
     kcfd_ctx *ctx = m->ctx;
     hipStream_t s = ctx->stream;
-    const int G = std::max(1, std::min((n + kGmBlock - 1) / kGmBlock, std::min(kMaxGmGrid, 4 * ctx->cus)));
+    const int G = std::max(1, std::min((n + kGmBlock - 1) / kGmBlock, std::min(kMaxGmGrid, kGmPerCu * ctx->cus)));
     const unsigned nb = (unsigned)((n + kGmBlock - 1) / kGmBlock);
     const double eps = 2.220446049250313e-16;
     const size_t ldv = w->ldv;
@@ -515,14 +630,14 @@ int gmres_solve(kcfd_mat *m, const double *b, double *x, const kcfd_solve_opts *
     // one restart cycle: the same launches every cycle (all scalars it
     // depends on are in GmState)
     auto enqueue_cycle = [&]() -> int {
-        KCFD_CHECK(precond_apply_async(m, w->r, V, w->t, porder, nullptr, 0));
+        KCFD_CHECK(gm_precond(m, w->r, V, w->t, porder, nullptr, 0, G));
         hipLaunchKernelGGL(g_norm2_part, dim3(G), dim3(kGmBlock), 0, s, n, V, (const double *)nullptr, p0);
         hipLaunchKernelGGL(g_cycle_start, dim3(G), dim3(kGmBlock), 0, s, n, V, p0, G, st, restart);
         for (int col = 0; col < restart; ++col) {
             double *vc = V + (size_t)col * ldv, *wv = V + (size_t)(col + 1) * ldv;
             hipLaunchKernelGGL(g_spmv, dim3(G), dim3(kGmBlock), 0, s, n, m->slice_ptr, m->six, m->sa64, vc,
                                w->av, st, col);
-            KCFD_CHECK(precond_apply_async(m, w->av, wv, w->t, porder, stop_col, col));
+            KCFD_CHECK(gm_precond(m, w->av, wv, w->t, porder, stop_col, col, G));
             hipLaunchKernelGGL(g_mgs_first, dim3(G), dim3(kGmBlock), 0, s, n, V, wv, p0, st, col);
             for (int k = 0; k <= col; ++k) {
                 const double *pin = (k & 1) ? p1 : p0;
@@ -556,7 +671,7 @@ int gmres_solve(kcfd_mat *m, const double *b, double *x, const kcfd_solve_opts *
     }
     hipLaunchKernelGGL(g_state_init, dim3(1), dim3(64), 0, s, st);
     // ||b||, ||M^{-1} b|| and the initial residual
-    KCFD_CHECK(precond_apply_async(m, B, V, w->t, porder, nullptr, 0));
+    KCFD_CHECK(gm_precond(m, B, V, w->t, porder, nullptr, 0, G));
     hipLaunchKernelGGL(g_norm2_part, dim3(G), dim3(kGmBlock), 0, s, n, B, V, p0);
     hipLaunchKernelGGL(g_finalize, dim3(1), dim3(kGmBlock), 0, s, p0, G, st, 0);
     hipLaunchKernelGGL(g_resid, dim3(G), dim3(kGmBlock), 0, s, n, m->slice_ptr, m->six, m->sa64, X, B,

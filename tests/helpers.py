@@ -115,14 +115,14 @@ def _block_sum256(vals):
 
 
 def dot_device_order(cus):
-    """The device GMRES's dot product order: G = min(ceil(n/256), 1024,
-    4 cus) workgroups of 256 lanes, each lane summing its grid-stride
+    """The device GMRES's dot product order: G = min(ceil(n/256), 2048,
+    8 cus) workgroups of 256 lanes, each lane summing its grid-stride
     products in order, workgroup sums as _block_sum256, then the G partials
     summed the same way by one workgroup (lane j taking partials j, j+256,
     ...)."""
     def dot(a, b):
         n = len(a)
-        G = max(1, min((n + 255) // 256, 1024, 4 * cus))
+        G = max(1, min((n + 255) // 256, 2048, 8 * cus))
         p = a * b
         T = G * 256
         acc = np.zeros(T)
