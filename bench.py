@@ -112,8 +112,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 "
-                                   "(the multi-GPU sharding design is described in DESIGN.md)"}))
+        print(json.dumps({"error": "only one GPU is measured: --gpus must be 1 (independent systems "
+                                   "would shard one batch per GPU; one large system would split into "
+                                   "row slabs with halo exchanges and block-Jacobi ILU(0); neither is built)"}))
         sys.exit(2)
     if args.workload == "batched":
         return bench_batched(args)
