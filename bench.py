@@ -98,6 +98,8 @@ def main():
                     help="ILU(0) row ordering (colour: first-fit colouring, 2 levels per solve)")
     ap.add_argument("--grid-per-cu", type=int, default=0, help="launch shape (0: library default)")
     ap.add_argument("--trsv-per-cu", type=int, default=0, help="launch shape (0: library default)")
+    ap.add_argument("--rows-per-thread", type=int, default=0, choices=[0, 1, 2, 4],
+                    help="launch shape: rows per thread of the tile kernels (0: library default 2)")
     ap.add_argument("--tile-walk", type=int, default=0,
                     help="launch shape: 1 interleaved, 2 contiguous tiles per workgroup (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -159,7 +161,8 @@ def bench_pressure(args):
         opts = device.default_opts(rtol=1e-8, precond=args.precond, precision=prec,
                                    inner_rtol=args.delta, check_every=args.check_every,
                                    grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu,
-                                   tile_walk=args.tile_walk, method=args.method)
+                                   tile_walk=args.tile_walk, method=args.method,
+                                   rows_per_thread=args.rows_per_thread)
         lat, rs = [], []
 
         def step():
@@ -234,7 +237,8 @@ def bench_pressure(args):
         # counter passes on this exact workload (default shape, mixed,
         # two-colour ILU(0), 128^3), scaled by the copy-kernel calibration
         if (args.method == "bicgstab" and args.n == 128 and args.precond == "ilu0" and args.order == "color"
-                and "mixed" in results and not (args.grid_per_cu or args.trsv_per_cu or args.tile_walk)):
+                and "mixed" in results and not (args.grid_per_cu or args.trsv_per_cu or args.tile_walk
+                                                 or args.rows_per_thread not in (0, 2))):
             roofline["traffic"] = TRAFFIC_SPMV_P_MIXED_HEX128
             roofline["traffic_source"] = ("profiles/hbm_traffic_bicgstab_ilu0_mixed_hex128.md: FETCH_SIZE x 2.000 "
                                           "+ WRITE_SIZE x 1.000 KiB per dispatch, median working dispatch")
@@ -323,7 +327,8 @@ def bench_batched(args):
         log(f"{prec}: preconditioners set up in {time.perf_counter() - t:.1f}s")
         if args.no_tune:
             choice = dict(grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu,
-                          tile_walk=args.tile_walk, check_every=args.check_every)
+                          tile_walk=args.tile_walk, rows_per_thread=args.rows_per_thread,
+                          check_every=args.check_every)
         else:
             choice = tune.autotune(sb, prec, args.precond, order or "natural", delta=args.delta, log=log)
         log(f"{prec}: launch shape {tune.describe(choice)}")
@@ -332,7 +337,8 @@ def bench_batched(args):
         bobj = sb.make_batch(device.default_opts(
             rtol=1e-8, precond=args.precond, precision=prec, inner_rtol=args.delta,
             check_every=choice["check_every"], grid_per_cu=choice["grid_per_cu"],
-            trsv_per_cu=choice["trsv_per_cu"], tile_walk=choice.get("tile_walk", 0)))
+            trsv_per_cu=choice["trsv_per_cu"], tile_walk=choice.get("tile_walk", 0),
+            rows_per_thread=choice.get("rows_per_thread", 0)))
         for w in range(args.warmup):
             r = sb.solve(batch=bobj, **opts)
             log(f"{prec} warmup {w}: {r.summary()}")

@@ -175,6 +175,9 @@ typedef struct kcfd_solve_opts {
                            (0: library default) */
     int method;         /* KCFD_METHOD_* (0: BiCGSTAB) */
     int restart;        /* GMRES: Krylov vectors per cycle, 1..64 (0: 20, scipy's) */
+    int rows_per_thread; /* launch shape: rows each thread of the SpMV / vector kernels
+                            takes per tile, 1, 2 or 4 (a wave covers 64x that many rows
+                            of a tile of 256x that many); 0: 2 */
 } kcfd_solve_opts;
 
 typedef struct kcfd_solve_result {

@@ -136,7 +136,7 @@ class SystemBatch:
 
     def solve(self, precision="mixed", precond="jacobi", order="color", rtol=1e-8,
               check_every=8, grid_per_cu=0, trsv_per_cu=0, delta=0.1, batch=None,
-              fallback=True, tile_walk=0) -> BatchResult:
+              fallback=True, tile_walk=0, rows_per_thread=0) -> BatchResult:
         """One batched solve from x0 = 0 (preconditioners set up beforehand,
         outside the timed region).  In mixed precision, the systems that stop
         without converging continue in FP64 from their best iterate, as one
@@ -144,7 +144,8 @@ class SystemBatch:
         self.setup(precond, precision, order)
         opts = device.default_opts(rtol=rtol, precond=precond, precision=precision, inner_rtol=delta,
                                    check_every=check_every, grid_per_cu=grid_per_cu,
-                                   trsv_per_cu=trsv_per_cu, tile_walk=tile_walk)
+                                   trsv_per_cu=trsv_per_cu, tile_walk=tile_walk,
+                                   rows_per_thread=rows_per_thread)
         own = batch is None
         b = batch if batch is not None else self.make_batch(opts)
         nfb = 0
@@ -161,7 +162,8 @@ class SystemBatch:
             if fallback and precision == "mixed" and bad:
                 o64 = device.default_opts(rtol=rtol, precond=precond, precision="fp64",
                                           check_every=check_every, grid_per_cu=grid_per_cu,
-                                          trsv_per_cu=trsv_per_cu, tile_walk=tile_walk)
+                                          trsv_per_cu=trsv_per_cu, tile_walk=tile_walk,
+                                          rows_per_thread=rows_per_thread)
                 fb = device.Batch(self.ctx, [self.mats[k] for k in bad], [self.bs[k] for k in bad],
                                   [self.xs[k] for k in bad], o64)
                 try:
@@ -184,6 +186,7 @@ class SystemBatch:
         return BatchResult(precision, precond, wall, dev_ms, res, float(sum(r.bytes for r in res)),
                            dict(check_every=check_every, grid_per_cu=grid_per_cu,
                                 trsv_per_cu=trsv_per_cu, tile_walk=tile_walk,
+                                rows_per_thread=rows_per_thread,
                                 order=order if precond == "ilu0" else None,
                                 delta=delta if precision == "mixed" else None, fp64_fallbacks=nfb))
 

@@ -300,20 +300,23 @@ __device__ __forceinline__ double row_ax64(const SysDesc &d, int i, const double
     return acc;
 }
 
-// Tile kernels: a tile is kTile consecutive rows of one system, and thread t
-// of the workgroup takes rows t, t + kBlock, ... of it (kRows rows, each a
+// Tile kernels: a tile is R kBlock consecutive rows of one system, and thread t
+// of the workgroup takes rows t, t + kBlock, ... of it (R rows, each a
 // coalesced wave access).  Every load of a thread's rows is issued before
 // the first store, and the rows of the last tile past the system's end are
 // loaded at the last row and discarded.
-constexpr int kRows = 2;
-constexpr int kTile = kRows * kBlock;
+// R (rows per thread: 1, 2 or 4; a wave covers 64 R rows of the tile) is a
+// template parameter of every tile kernel, chosen at run time per batch
+// (kcfd_solve_opts.rows_per_thread, swept by the autotuner).
+constexpr int kDefaultRows = 2;
 
+template <int R>
 struct TileRows {
-    int ix[kRows];   // row index, clamped to n - 1
-    bool ok[kRows];  // a row of the system
+    int ix[R];   // row index, clamped to n - 1
+    bool ok[R];  // a row of the system
     __device__ __forceinline__ TileRows(int t0, int n) {
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             const int i = t0 + r * kBlock + (int)threadIdx.x;
             ok[r] = i < n;
             ix[r] = ok[r] ? i : n - 1;
@@ -367,7 +370,7 @@ __device__ __forceinline__ void sum_partials(const SysDesc &d, const double *__r
 // ------------------------------------------------------------------ init
 // FP64: r = b - A x, rt = r; sums ||b||^2, ||r||^2.  MIXED: ||b||^2.
 // ILU: every triangular-solve buffer starts at the sentinel.
-template <typename V, int PC>
+template <typename V, int PC, int R>
 __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
                                                  const int2 *__restrict__ tiles, const int *__restrict__ counts,
@@ -379,10 +382,10 @@ __global__ __launch_bounds__(kBlock) void k_init(const SysDesc *__restrict__ sys
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         if (sts[ti.x].status != S_NEW || d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
         double v[2] = {0.0, 0.0};
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             if (!tr.ok[r]) continue;
             const int i = tr.ix[r];
             const double bi = gl(d.b)[i];
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(kFinBlock) void f_init(const SysDesc *__restrict__ 
 // p-update / p = r / x = 0 for a zero rhs; MIXED: xb = x after an improving
 // reliable update, solve buffers reset before one.
 // Block 0 also clears the chunk counters of the two p triangular solves.
-template <typename V, int PC>
+template <typename V, int PC, int R>
 __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
                                               const SysState *__restrict__ sts,
                                               const int2 *__restrict__ tiles, const int *__restrict__ counts,
@@ -457,43 +460,43 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
         const SysState &st = sts[ti.x];
         const int status = st.status;
         if (d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
         if (st.save) {                     // MIXED: the last update improved on the best
-            double xv[kRows];
+            double xv[R];
 #pragma unroll
-            for (int r = 0; r < kRows; ++r) xv[r] = glw(d.x)[tr.ix[r]];
+            for (int r = 0; r < R; ++r) xv[r] = glw(d.x)[tr.ix[r]];
 #pragma unroll
-            for (int r = 0; r < kRows; ++r)
+            for (int r = 0; r < R; ++r)
                 if (tr.ok[r]) glw(d.xb)[tr.ix[r]] = xv[r];
         }
         if (status == S_RUN || status == S_RESUME) {
             KCFD_GLOBAL V *p = vec<V>(d.p);
             const KCFD_GLOBAL V *vv = vec<V>(d.v), *rr = vec<V>(d.r);
             const V om = (V)st.omega, be = (V)st.beta;
-            V pv[kRows], vl[kRows], rv[kRows];
+            V pv[R], vl[R], rv[R];
 #pragma unroll
-            for (int r = 0; r < kRows; ++r) {
+            for (int r = 0; r < R; ++r) {
                 pv[r] = p[tr.ix[r]];
                 vl[r] = vv[tr.ix[r]];
                 rv[r] = rr[tr.ix[r]];
             }
 #pragma unroll
-            for (int r = 0; r < kRows; ++r)
+            for (int r = 0; r < R; ++r)
                 if (tr.ok[r]) {
                     V pi = pv[r] - om * vl[r];
                     pi = pi * be;
                     p[tr.ix[r]] = pi + rv[r];
                 }
         } else if (status == S_START) {
-            V rv[kRows];
+            V rv[R];
 #pragma unroll
-            for (int r = 0; r < kRows; ++r) rv[r] = vec<V>(d.r)[tr.ix[r]];
+            for (int r = 0; r < R; ++r) rv[r] = vec<V>(d.r)[tr.ix[r]];
 #pragma unroll
-            for (int r = 0; r < kRows; ++r)
+            for (int r = 0; r < R; ++r)
                 if (tr.ok[r]) vec<V>(d.p)[tr.ix[r]] = rv[r];
         } else if (status == S_REFINE) {
 #pragma unroll
-            for (int r = 0; r < kRows; ++r) {
+            for (int r = 0; r < R; ++r) {
                 if (!tr.ok[r]) continue;
                 const int i = tr.ix[r];
                 if (st.rollback) glw(d.x)[i] = glw(d.xb)[i];   // the inner iteration diverged
@@ -505,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
             }
         } else if (status == S_ZEROX) {
 #pragma unroll
-            for (int r = 0; r < kRows; ++r)
+            for (int r = 0; r < R; ++r)
                 if (tr.ok[r]) glw(d.x)[tr.ix[r]] = 0.0;
         }
     }
@@ -514,7 +517,7 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
 // -------------------------------------------------------------------- K2
 // v = A M p; rt.v.   MIXED S_REFINE: r64 = b - A64 x; r = (V)r64 (and rt = r
 // on a restart); sums ||r64||^2, rt.r, ||r||^2.
-template <typename V, int PC>
+template <typename V, int PC, int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_spmv_p(const SysDesc *__restrict__ sys,
                                                    const SysState *__restrict__ sts,
                                                    const int2 *__restrict__ tiles, const int *__restrict__ counts,
@@ -527,14 +530,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         const SysState &st = sts[ti.x];
         const int status = st.status;
         if (d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
         if (status == S_RUN || status == S_START || status == S_RESUME) {
             // one row after the other (the store orders them): a row's loads
             // already fill the wave's memory pipeline, and two rows in flight
             // would cost occupancy
             double v[1] = {0.0};
 #pragma unroll
-            for (int r = 0; r < kRows; ++r) {
+            for (int r = 0; r < R; ++r) {
                 const double a = row_amz<V, PC>(d, tr.ix[r], (const V *)d.p, (const V *)d.ph);
                 const double rtv = (double)vec<V>(d.rt)[tr.ix[r]];
                 if (tr.ok[r]) {
@@ -545,14 +548,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
             }
             write_partials<1>(v, partials, tile);
         } else if (status == S_REFINE) {
-            double r64[kRows], rto[kRows];
+            double r64[R], rto[R];
 #pragma unroll
-            for (int r = 0; r < kRows; ++r) r64[r] = gl(d.b)[tr.ix[r]] - row_ax64(d, tr.ix[r], d.x);
+            for (int r = 0; r < R; ++r) r64[r] = gl(d.b)[tr.ix[r]] - row_ax64(d, tr.ix[r], d.x);
 #pragma unroll
-            for (int r = 0; r < kRows; ++r) rto[r] = (double)vec<V>(d.rt)[tr.ix[r]];
+            for (int r = 0; r < R; ++r) rto[r] = (double)vec<V>(d.rt)[tr.ix[r]];
             double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-            for (int r = 0; r < kRows; ++r)
+            for (int r = 0; r < R; ++r)
                 if (tr.ok[r]) {
                     const V rv = (V)r64[r];
                     vec<V>(d.r)[tr.ix[r]] = rv;
@@ -649,7 +652,7 @@ __global__ __launch_bounds__(kFinBlock) void f_spmv_p(const SysDesc *__restrict_
 
 // -------------------------------------------------------------------- K3
 // s = r - alpha v; ||s||^2 (partial slot 2, read by F4).
-template <typename V, int PC>
+template <typename V, int PC, int R>
 __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
                                               const SysState *__restrict__ sts,
                                               const int2 *__restrict__ tiles, const int *__restrict__ counts,
@@ -663,17 +666,17 @@ __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
         if (st.status != S_RUN || d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
         const V al = (V)st.alpha;
-        V rv[kRows], vl[kRows];
+        V rv[R], vl[R];
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             rv[r] = vec<V>(d.r)[tr.ix[r]];
             vl[r] = vec<V>(d.v)[tr.ix[r]];
         }
         double v[1] = {0.0};
 #pragma unroll
-        for (int r = 0; r < kRows; ++r)
+        for (int r = 0; r < R; ++r)
             if (tr.ok[r]) {
                 const V si = rv[r] - al * vl[r];
                 vec<V>(d.s)[tr.ix[r]] = si;
@@ -685,7 +688,7 @@ __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
 
 // -------------------------------------------------------------------- K4
 // t = A M s; t.s, t.t.
-template <typename V, int PC>
+template <typename V, int PC, int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_spmv_s(const SysDesc *__restrict__ sys,
                                                    const SysState *__restrict__ sts,
                                                    const int2 *__restrict__ tiles, const int *__restrict__ counts,
@@ -697,10 +700,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
         if (st.status != S_RUN || d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
         double v[2] = {0.0, 0.0};
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {   // one row after the other, as in K2
+        for (int r = 0; r < R; ++r) {   // one row after the other, as in K2
             const double a = row_amz<V, PC>(d, tr.ix[r], (const V *)d.s, (const V *)d.sh);
             const double sv = (double)vec<V>(d.s)[tr.ix[r]];
             if (tr.ok[r]) {
@@ -735,7 +738,7 @@ __global__ __launch_bounds__(kFinBlock) void f_spmv_s(const SysDesc *__restrict_
 
 // -------------------------------------------------------------------- K5
 // x += alpha M p; x += omega M s; r = s - omega t; ||r||^2, rt.r
-template <typename V, int PC>
+template <typename V, int PC, int R>
 __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
                                                const SysState *__restrict__ sts,
                                                const int2 *__restrict__ tiles, const int *__restrict__ counts,
@@ -748,10 +751,10 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
         if (d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
         if (st.status == S_HALF) {                   // half-step exit: x += alpha M p
 #pragma unroll
-            for (int r = 0; r < kRows; ++r) {
+            for (int r = 0; r < R; ++r) {
                 if (!tr.ok[r]) continue;
                 const int i = tr.ix[r];
                 const V al = (V)st.alpha;
@@ -769,10 +772,10 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
         }
         if (st.status != S_RUN) continue;
         const V al = (V)st.alpha, om = (V)st.omega;
-        V ph[kRows], sh[kRows], sv[kRows], tv[kRows], rtv[kRows];
-        double xv[kRows];
+        V ph[R], sh[R], sv[R], tv[R], rtv[R];
+        double xv[R];
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             const int i = tr.ix[r];
             ph[r] = (V)precond_at<V, PC>(d, (const V *)d.p, (const V *)d.ph, i);
             sh[r] = (V)precond_at<V, PC>(d, (const V *)d.s, (const V *)d.sh, i);
@@ -783,7 +786,7 @@ __global__ __launch_bounds__(kBlock) void k_xr(const SysDesc *__restrict__ sys,
         }
         double v[2] = {0.0, 0.0};
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             if (!tr.ok[r]) continue;
             const int i = tr.ix[r];
             if (bp.precision == KCFD_FP64) {         // scipy's operation order
@@ -887,7 +890,7 @@ __device__ __forceinline__ double cg_z(const SysDesc &d, int i) {
     return vec<double>(d.r)[i];
 }
 
-template <int PC>
+template <int PC, int R>
 __global__ __launch_bounds__(kBlock) void k_cg_rz(const SysDesc *__restrict__ sys,
                                                   const SysState *__restrict__ sts,
                                                   const int2 *__restrict__ tiles, const int *__restrict__ counts,
@@ -899,23 +902,23 @@ __global__ __launch_bounds__(kBlock) void k_cg_rz(const SysDesc *__restrict__ sy
         const SysDesc &d = sys[ti.x];
         const int status = sts[ti.x].status;
         if (d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
         if (status == S_ZEROX) {
 #pragma unroll
-            for (int r = 0; r < kRows; ++r)
+            for (int r = 0; r < R; ++r)
                 if (tr.ok[r]) glw(d.x)[tr.ix[r]] = 0.0;
             continue;
         }
         if (status != S_RUN && status != S_START) continue;
-        double rv[kRows], zv[kRows];
+        double rv[R], zv[R];
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             rv[r] = vec<double>(d.r)[tr.ix[r]];
             zv[r] = cg_z<PC>(d, tr.ix[r]);
         }
         double v[1] = {0.0};
 #pragma unroll
-        for (int r = 0; r < kRows; ++r)
+        for (int r = 0; r < R; ++r)
             if (tr.ok[r]) v[0] += rv[r] * zv[r];
         write_partials<1>(v, partials, tile);
     }
@@ -938,7 +941,7 @@ __global__ __launch_bounds__(kFinBlock) void f_cg_rho(const SysDesc *__restrict_
     if (st.status == S_RUN) st.beta = st.rho / st.rho_prev;
 }
 
-template <int PC>
+template <int PC, int R>
 __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
                                                  const int2 *__restrict__ tiles, const int *__restrict__ counts) {
@@ -950,16 +953,16 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc *__restrict__ sys
         const SysState &st = sts[ti.x];
         const int status = st.status;
         if ((status != S_RUN && status != S_START) || d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
-        double zv[kRows], pv[kRows];
+        const TileRows<R> tr(ti.y, d.n);
+        double zv[R], pv[R];
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             zv[r] = cg_z<PC>(d, tr.ix[r]);
             pv[r] = vec<double>(d.p)[tr.ix[r]];
         }
         const double beta = st.beta;
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             if (!tr.ok[r]) continue;
             const int i = tr.ix[r];
             double pi = zv[r];
@@ -973,7 +976,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_p(const SysDesc *__restrict__ sys
     }
 }
 
-template <int PC>
+template <int PC, int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cg_q(
     const SysDesc *__restrict__ sys, const SysState *__restrict__ sts, const int2 *__restrict__ tiles,
     const int *__restrict__ counts, double *__restrict__ partials) {
@@ -984,10 +987,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         const SysDesc &d = sys[ti.x];
         const int status = sts[ti.x].status;
         if ((status != S_RUN && status != S_START) || d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
         double v[1] = {0.0};
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {   // one row after the other, as in K2
+        for (int r = 0; r < R; ++r) {   // one row after the other, as in K2
             const double q = row_amz<double, KCFD_PC_NONE>(d, tr.ix[r], (const double *)d.p,
                                                            (const double *)d.p);
             const double pv = vec<double>(d.p)[tr.ix[r]];
@@ -1018,6 +1021,7 @@ __global__ __launch_bounds__(kFinBlock) void f_cg_alpha(const SysDesc *__restric
     st.alpha = st.rho / pq;
 }
 
+template <int R>
 __global__ __launch_bounds__(kBlock) void k_cg_xr(const SysDesc *__restrict__ sys,
                                                   const SysState *__restrict__ sts,
                                                   const int2 *__restrict__ tiles, const int *__restrict__ counts,
@@ -1029,10 +1033,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(const SysDesc *__restrict__ sy
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
         if ((st.status != S_RUN && st.status != S_START) || d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
-        double xv[kRows], pv[kRows], rv[kRows], qv[kRows];
+        const TileRows<R> tr(ti.y, d.n);
+        double xv[R], pv[R], rv[R], qv[R];
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             const int i = tr.ix[r];
             xv[r] = glw(d.x)[i];
             pv[r] = vec<double>(d.p)[i];
@@ -1042,7 +1046,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_xr(const SysDesc *__restrict__ sy
         const double alpha = st.alpha;
         double v[1] = {0.0};
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             if (!tr.ok[r]) continue;
             const int i = tr.ix[r];
             glw(d.x)[i] = xv[r] + alpha * pv[r];
@@ -1085,6 +1089,7 @@ __device__ __forceinline__ bool wants_true(const SysState &st) {
 
 // MIXED stop without convergence: x = xb (the best iterate), in a launch of
 // its own -- the true residual reads the neighbours' x.
+template <int R>
 __global__ __launch_bounds__(kBlock) void k_restore(const SysDesc *__restrict__ sys,
                                                     const SysState *__restrict__ sts,
                                                     const int2 *__restrict__ tiles,
@@ -1096,13 +1101,14 @@ __global__ __launch_bounds__(kBlock) void k_restore(const SysDesc *__restrict__ 
         if (!wants_true(st) || !st.restore) continue;
         const SysDesc &d = sys[ti.x];
         if (d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
 #pragma unroll
-        for (int r = 0; r < kRows; ++r)
+        for (int r = 0; r < R; ++r)
             if (tr.ok[r]) glw(d.x)[tr.ix[r]] = glw(d.xb)[tr.ix[r]];
     }
 }
 
+template <int R>
 __global__ __launch_bounds__(kBlock) void k_true_resid(const SysDesc *__restrict__ sys,
                                                        const SysState *__restrict__ sts,
                                                        const int2 *__restrict__ tiles, const int *__restrict__ counts,
@@ -1113,10 +1119,10 @@ __global__ __launch_bounds__(kBlock) void k_true_resid(const SysDesc *__restrict
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         if (!wants_true(sts[ti.x]) || d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
         double v[2] = {0.0, 0.0};
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             if (!tr.ok[r]) continue;
             const double bi = gl(d.b)[tr.ix[r]];
             const double ri = bi - row_ax64(d, tr.ix[r], d.x);
@@ -1148,6 +1154,7 @@ __global__ __launch_bounds__(kFinBlock) void f_true_resid(const SysDesc *__restr
 // Permuted row order: b and the initial x gathered from the caller's
 // buffers for systems about to start (before k_init, which reads x at
 // neighbouring rows) ...
+template <int R>
 __global__ __launch_bounds__(kBlock) void k_perm_in(const SysDesc *__restrict__ sys,
                                                     const SysState *__restrict__ sts,
                                                     const int2 *__restrict__ tiles,
@@ -1157,9 +1164,9 @@ __global__ __launch_bounds__(kBlock) void k_perm_in(const SysDesc *__restrict__ 
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         if (!d.perm || sts[ti.x].status != S_NEW || d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
 #pragma unroll
-        for (int r = 0; r < kRows; ++r) {
+        for (int r = 0; r < R; ++r) {
             if (!tr.ok[r]) continue;
             const int i = tr.ix[r], q = gl(d.perm)[i];
             glw(const_cast<double *>(d.b))[i] = gl(d.b_user)[q];
@@ -1169,6 +1176,7 @@ __global__ __launch_bounds__(kBlock) void k_perm_in(const SysDesc *__restrict__ 
 }
 
 // ... and the solution scattered back once its true residual is in.
+template <int R>
 __global__ __launch_bounds__(kBlock) void k_perm_out(const SysDesc *__restrict__ sys,
                                                      const SysState *__restrict__ sts,
                                                      const int2 *__restrict__ tiles,
@@ -1178,9 +1186,9 @@ __global__ __launch_bounds__(kBlock) void k_perm_out(const SysDesc *__restrict__
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         if (!d.perm || !sts[ti.x].scatter || d.n <= 0) continue;
-        const TileRows tr(ti.y, d.n);
+        const TileRows<R> tr(ti.y, d.n);
 #pragma unroll
-        for (int r = 0; r < kRows; ++r)
+        for (int r = 0; r < R; ++r)
             if (tr.ok[r]) glw(d.x_user)[gl(d.perm)[tr.ix[r]]] = glw(d.x)[tr.ix[r]];
     }
 }
@@ -1382,6 +1390,7 @@ struct kcfd_batch {
     kcfd_ctx *ctx = nullptr;
     int nsys = 0;                   // slots (the capacity of a dynamic batch)
     int ntiles = 0, grid = 0;
+    int rows = kDefaultRows;        // rows per thread of the tile kernels (tile = rows * kBlock)
     bool dynamic = false;           // systems attach / detach between rounds
     kcfd_solve_opts opts{};
     BatchParams bp{};
@@ -1441,10 +1450,10 @@ void launch_k(const kcfd_batch *b, int phase, bool first, bool last, F kernel, i
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, s, args...);
 }
 
-template <typename V, int PC>
+template <typename V, int PC, int R>
 void launch_iteration(kcfd_batch *b, hipStream_t s) {
     const int g = b->grid, ns = b->nsys, tg = b->trsv_grid;
-    launch_k(b, 4, true, true, k_p<V, PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+    launch_k(b, 4, true, true, k_p<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
              b->d_counters);
     if (PC == KCFD_PC_ILU0) {
         // phase 3: the p apply (two-colour: both kernels; sync-free: the L solve)
@@ -1461,11 +1470,11 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
                      b->d_chunks_u, b->d_counts, b->d_lvl_u);
         }
     }
-    launch_k(b, 1, true, true, k_spmv_p<V, PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+    launch_k(b, 1, true, true, k_spmv_p<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
              b->d_partials);
     launch_k(b, 0, false, false, f_spmv_p, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials,
              b->d_params);
-    launch_k(b, 0, false, false, k_s<V, PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+    launch_k(b, 0, false, false, k_s<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
              b->d_partials, b->d_counters);
     if (PC == KCFD_PC_ILU0) {
         if (b->run_two) {
@@ -1481,18 +1490,18 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
                      b->d_chunks_u, b->d_counts, b->d_lvl_u);
         }
     }
-    launch_k(b, 0, false, false, k_spmv_s<V, PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles,
+    launch_k(b, 0, false, false, k_spmv_s<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles,
              b->d_counts, b->d_partials, b->d_params);
     launch_k(b, 0, false, false, f_spmv_s, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials,
              b->d_params);
-    launch_k(b, 2, true, true, k_xr<V, PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+    launch_k(b, 2, true, true, k_xr<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
              b->d_partials, b->d_params);
     launch_k(b, 0, false, false, f_xr, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials, b->d_params);
 }
 
 // One CG iteration (FP64).  Profiled phases: 1 = q = A p (C3), 2 = x/r
 // update (C4), 3 = the ILU(0) apply z = M r, 4 = p update (C2).
-template <typename V, int PC>
+template <typename V, int PC, int R>
 void launch_iteration_cg(kcfd_batch *b, hipStream_t s) {
     const int g = b->grid, ns = b->nsys, tg = b->trsv_grid;
     if (PC == KCFD_PC_ILU0) {
@@ -1509,56 +1518,92 @@ void launch_iteration_cg(kcfd_batch *b, hipStream_t s) {
                      b->d_chunks_u, b->d_counts, b->d_lvl_u);
         }
     }
-    launch_k(b, 0, false, false, k_cg_rz<PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+    launch_k(b, 0, false, false, k_cg_rz<PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
              b->d_partials);
     launch_k(b, 0, false, false, f_cg_rho, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials);
-    launch_k(b, 4, true, true, k_cg_p<PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts);
-    launch_k(b, 1, true, true, k_cg_q<PC>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+    launch_k(b, 4, true, true, k_cg_p<PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts);
+    launch_k(b, 1, true, true, k_cg_q<PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
              b->d_partials);
     launch_k(b, 0, false, false, f_cg_alpha, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials);
-    launch_k(b, 2, true, true, k_cg_xr, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
+    launch_k(b, 2, true, true, k_cg_xr<R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
              b->d_partials);
     launch_k(b, 0, false, false, f_cg_check, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials);
 }
 
-template <typename V, int PC>
+template <typename V, int PC, int R>
 void launch_init(kcfd_batch *b, hipStream_t s) {
     if (PC == KCFD_PC_ILU0)
-        hipLaunchKernelGGL(k_perm_in, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_tiles,
+        hipLaunchKernelGGL(k_perm_in<R>, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state, b->d_tiles,
                            b->d_counts);
-    hipLaunchKernelGGL((k_init<V, PC>), dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+    hipLaunchKernelGGL((k_init<V, PC, R>), dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
                        b->d_tiles, b->d_counts, b->d_partials, b->d_params);
     hipLaunchKernelGGL(f_init, dim3(b->nsys), dim3(kFinBlock), 0, s, b->d_desc, b->d_state,
                        b->d_partials, b->d_params);
 }
 
-using LaunchFn = void (*)(kcfd_batch *, hipStream_t);
-
-LaunchFn pick_iter(int precision, int pc, int method = KCFD_METHOD_BICGSTAB) {
-    if (method == KCFD_METHOD_CG) {   // FP64 only (check_opts)
-        if (pc == KCFD_PC_JACOBI) return launch_iteration_cg<double, KCFD_PC_JACOBI>;
-        if (pc == KCFD_PC_ILU0) return launch_iteration_cg<double, KCFD_PC_ILU0>;
-        return launch_iteration_cg<double, KCFD_PC_NONE>;
-    }
-    if (precision == KCFD_FP64) {
-        if (pc == KCFD_PC_JACOBI) return launch_iteration<double, KCFD_PC_JACOBI>;
-        if (pc == KCFD_PC_ILU0) return launch_iteration<double, KCFD_PC_ILU0>;
-        return launch_iteration<double, KCFD_PC_NONE>;
-    }
-    if (pc == KCFD_PC_JACOBI) return launch_iteration<float, KCFD_PC_JACOBI>;
-    if (pc == KCFD_PC_ILU0) return launch_iteration<float, KCFD_PC_ILU0>;
-    return launch_iteration<float, KCFD_PC_NONE>;
+// after the iterations: best-iterate restore (mixed), FP64 true residual,
+// scatter out of the ILU(0) row order
+template <int R>
+void launch_finish(kcfd_batch *b, hipStream_t s) {
+    if (b->bp.precision == KCFD_MIXED)
+        hipLaunchKernelGGL(k_restore<R>, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                           b->d_tiles, b->d_counts);
+    hipLaunchKernelGGL(k_true_resid<R>, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                       b->d_tiles, b->d_counts, b->d_partials);
+    hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kFinBlock), 0, s, b->d_desc, b->d_state,
+                       b->d_partials);
+    if (b->bp.precond == KCFD_PC_ILU0)
+        hipLaunchKernelGGL(k_perm_out<R>, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
+                           b->d_tiles, b->d_counts);
 }
 
-LaunchFn pick_init(int precision, int pc) {
-    if (precision == KCFD_FP64) {
-        if (pc == KCFD_PC_JACOBI) return launch_init<double, KCFD_PC_JACOBI>;
-        if (pc == KCFD_PC_ILU0) return launch_init<double, KCFD_PC_ILU0>;
-        return launch_init<double, KCFD_PC_NONE>;
+using LaunchFn = void (*)(kcfd_batch *, hipStream_t);
+
+template <int R>
+LaunchFn pick_iter_r(int precision, int pc, int method) {
+    if (method == KCFD_METHOD_CG) {   // FP64 only (check_opts)
+        if (pc == KCFD_PC_JACOBI) return launch_iteration_cg<double, KCFD_PC_JACOBI, R>;
+        if (pc == KCFD_PC_ILU0) return launch_iteration_cg<double, KCFD_PC_ILU0, R>;
+        return launch_iteration_cg<double, KCFD_PC_NONE, R>;
     }
-    if (pc == KCFD_PC_JACOBI) return launch_init<float, KCFD_PC_JACOBI>;
-    if (pc == KCFD_PC_ILU0) return launch_init<float, KCFD_PC_ILU0>;
-    return launch_init<float, KCFD_PC_NONE>;
+    if (precision == KCFD_FP64) {
+        if (pc == KCFD_PC_JACOBI) return launch_iteration<double, KCFD_PC_JACOBI, R>;
+        if (pc == KCFD_PC_ILU0) return launch_iteration<double, KCFD_PC_ILU0, R>;
+        return launch_iteration<double, KCFD_PC_NONE, R>;
+    }
+    if (pc == KCFD_PC_JACOBI) return launch_iteration<float, KCFD_PC_JACOBI, R>;
+    if (pc == KCFD_PC_ILU0) return launch_iteration<float, KCFD_PC_ILU0, R>;
+    return launch_iteration<float, KCFD_PC_NONE, R>;
+}
+
+LaunchFn pick_iter(int precision, int pc, int method, int rows) {
+    if (rows == 1) return pick_iter_r<1>(precision, pc, method);
+    if (rows == 4) return pick_iter_r<4>(precision, pc, method);
+    return pick_iter_r<2>(precision, pc, method);
+}
+
+template <int R>
+LaunchFn pick_init_r(int precision, int pc) {
+    if (precision == KCFD_FP64) {
+        if (pc == KCFD_PC_JACOBI) return launch_init<double, KCFD_PC_JACOBI, R>;
+        if (pc == KCFD_PC_ILU0) return launch_init<double, KCFD_PC_ILU0, R>;
+        return launch_init<double, KCFD_PC_NONE, R>;
+    }
+    if (pc == KCFD_PC_JACOBI) return launch_init<float, KCFD_PC_JACOBI, R>;
+    if (pc == KCFD_PC_ILU0) return launch_init<float, KCFD_PC_ILU0, R>;
+    return launch_init<float, KCFD_PC_NONE, R>;
+}
+
+LaunchFn pick_init(int precision, int pc, int rows) {
+    if (rows == 1) return pick_init_r<1>(precision, pc);
+    if (rows == 4) return pick_init_r<4>(precision, pc);
+    return pick_init_r<2>(precision, pc);
+}
+
+LaunchFn pick_finish(int rows) {
+    if (rows == 1) return launch_finish<1>;
+    if (rows == 4) return launch_finish<4>;
+    return launch_finish<2>;
 }
 
 // algorithmic bytes (roofline model): CSR stream M_A, ILU factor stream M_F,
@@ -1622,6 +1667,9 @@ int check_opts(const kcfd_solve_opts *o) {
     KCFD_ARG(o->grid_per_cu >= 0 && o->grid_per_cu <= 8, "grid_per_cu must be 0..8");
     KCFD_ARG(o->trsv_per_cu >= 0 && o->trsv_per_cu <= 4, "trsv_per_cu must be 0..4");
     KCFD_ARG(o->tile_walk >= 0 && o->tile_walk <= 2, "tile_walk must be 0..2");
+    KCFD_ARG(o->rows_per_thread == 0 || o->rows_per_thread == 1 || o->rows_per_thread == 2 ||
+                 o->rows_per_thread == 4,
+             "rows_per_thread must be 0, 1, 2 or 4");
     KCFD_ARG(o->method == KCFD_METHOD_BICGSTAB || o->method == KCFD_METHOD_CG,
              o->method == KCFD_METHOD_GMRES ? "GMRES runs through kcfd_solve only (no batches)"
                                             : "unknown Krylov method");
@@ -1654,6 +1702,7 @@ int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **o
     b->nsys = slots;
     b->opts = *o;
     if (b->opts.check_every <= 0) b->opts.check_every = 8;
+    b->rows = o->rows_per_thread > 0 ? o->rows_per_thread : kDefaultRows;
     b->bp.rtol = o->rtol;
     b->bp.atol = o->atol;
     b->bp.delta = o->inner_rtol > 0 ? o->inner_rtol : 0.1;
@@ -1765,6 +1814,7 @@ int rebuild_tables(kcfd_batch *b) {
         d.tile0 = (int)tiles.size();
         d.ntiles = 0;
         if (!b->mats[k]) continue;
+        const int kTile = b->rows * kBlock;   // rows per tile
         const int nt = std::max(1, (d.n + kTile - 1) / kTile);
         d.ntiles = nt;
         const kcfd_mat *m = b->mats[k];
@@ -1930,7 +1980,7 @@ int capture_graph(kcfd_batch *b, int iters) {
         hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
-    LaunchFn fn = pick_iter(b->bp.precision, b->bp.precond, b->bp.method);
+    LaunchFn fn = pick_iter(b->bp.precision, b->bp.precond, b->bp.method, b->rows);
     hipGraph_t g = nullptr;
     KCFD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < iters; ++k) fn(b, s);
@@ -1965,7 +2015,7 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     if (!b->graph || b->graph_iters != b->opts.check_every)
         KCFD_CHECK(capture_graph(b, b->opts.check_every));
     KCFD_HIP(hipEventRecord(ctx->ev0, s));
-    pick_init(b->bp.precision, b->bp.precond)(b, s);
+    pick_init(b->bp.precision, b->bp.precond, b->rows)(b, s);
     KCFD_HIP(hipGetLastError());
     long long guard = 0;
     int maxit = 0;
@@ -1989,16 +2039,7 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
         }
     }
     KCFD_HIP(hipEventRecord(ctx->ev1, s));
-    if (b->bp.precision == KCFD_MIXED)
-        hipLaunchKernelGGL(k_restore, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                           b->d_tiles, b->d_counts);
-    hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                       b->d_tiles, b->d_counts, b->d_partials);
-    hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kFinBlock), 0, s, b->d_desc, b->d_state,
-                       b->d_partials);
-    if (b->bp.precond == KCFD_PC_ILU0)
-        hipLaunchKernelGGL(k_perm_out, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                           b->d_tiles, b->d_counts);
+    pick_finish(b->rows)(b, s);
     KCFD_HIP(hipGetLastError());
     KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
                             hipMemcpyDeviceToHost, s));
@@ -2154,18 +2195,9 @@ int kcfd_batch_step(kcfd_batch *b, int *finished, int *nfinished) {
     KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
     if (!b->graph || b->graph_iters != b->opts.check_every)
         KCFD_CHECK(capture_graph(b, b->opts.check_every));
-    if (any_new) pick_init(b->bp.precision, b->bp.precond)(b, s);
+    if (any_new) pick_init(b->bp.precision, b->bp.precond, b->rows)(b, s);
     KCFD_HIP(hipGraphLaunch(b->graph, s));
-    if (b->bp.precision == KCFD_MIXED)
-        hipLaunchKernelGGL(k_restore, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                           b->d_tiles, b->d_counts);
-    hipLaunchKernelGGL(k_true_resid, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                       b->d_tiles, b->d_counts, b->d_partials);
-    hipLaunchKernelGGL(f_true_resid, dim3(b->nsys), dim3(kFinBlock), 0, s, b->d_desc, b->d_state,
-                       b->d_partials);
-    if (b->bp.precond == KCFD_PC_ILU0)
-        hipLaunchKernelGGL(k_perm_out, dim3(b->grid), dim3(kBlock), 0, s, b->d_desc, b->d_state,
-                           b->d_tiles, b->d_counts);
+    pick_finish(b->rows)(b, s);
     KCFD_HIP(hipGetLastError());
     KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
                             hipMemcpyDeviceToHost, s));
@@ -2282,7 +2314,7 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
         run[k].iters = run[k].total_iters = 0;
         run[k].rmax = INFINITY;   // MIXED: no divergence restart
     }
-    const LaunchFn it = pick_iter(b->bp.precision, b->bp.precond, b->bp.method);
+    const LaunchFn it = pick_iter(b->bp.precision, b->bp.precond, b->bp.method, b->rows);
     BatchParams pp = b->bp;
     pp.rhotol = -1.0;     // no breakdown exit
     pp.delta = 0.0;       // MIXED: no residual replacement
@@ -2364,7 +2396,8 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
                  m->solver->bp.precision == opts->precision && m->solver->h_desc[0].b_user == bvec &&
                  m->solver->h_desc[0].x_user == x && m->solver->opts.grid_per_cu == opts->grid_per_cu &&
                  m->solver->opts.trsv_per_cu == opts->trsv_per_cu &&
-                 m->solver->opts.tile_walk == opts->tile_walk && m->solver->opts.method == opts->method;
+                 m->solver->opts.tile_walk == opts->tile_walk && m->solver->opts.method == opts->method &&
+                 m->solver->opts.rows_per_thread == opts->rows_per_thread;
     if (!reuse) {
         if (m->solver) kcfd_batch_destroy(m->solver);
         m->solver = nullptr;

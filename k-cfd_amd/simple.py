@@ -28,6 +28,7 @@ class StepResult:
     dP: DevBuf          # d_P (device, n)
     solve: dict         # the solve's result
     matrix: device.DeviceMatrix
+    ratio: float = 0.0  # ||p'|| / max(||p|| + 1e-10, 1e-10) after the update (the reference's return value)
 
 
 def pressure_correction_step(dm: device.DeviceMesh, aP, U: DevBuf, p: DevBuf, mf: DevBuf, rho: float,
@@ -52,7 +53,9 @@ def pressure_correction_step(dm: device.DeviceMesh, aP, U: DevBuf, p: DevBuf, mf
     opts = device.default_opts(rtol=rtol, precond=precond, precision="fp64", method=method)
     res = A.solve_dev(A.rhs_ptr, pp.p, opts, setup=False)
     check(lib().kcfd_simple_correct(dm.h, pp.p, dP.p, pd.ctypes.data, float(alpha_p), U.p, p.p))
-    return StepResult(pp, dP, res.as_dict(), A)
+    # models/single_phase.py:376
+    ratio = float(np.linalg.norm(pp.download()) / max(np.linalg.norm(p.download()) + 1e-10, 1e-10))
+    return StepResult(pp, dP, res.as_dict(), A, ratio)
 
 
 @dataclass

@@ -7,12 +7,17 @@ The knobs are those of kcfd_solve_opts:
 * ``trsv_per_cu``: workgroups per CU of the ILU(0) triangular solves;
 * ``tile_walk``: 1 interleaved tiles per workgroup, 2 one contiguous run of
   tiles per workgroup (inside its XCD's range either way);
+* ``rows_per_thread``: rows each thread takes per tile, 1, 2 or 4 (a wave
+  covers 64, 128 or 256 rows; a tile 256, 512 or 1024);
 * ``check_every``: iterations per captured hipGraph between host checks.
 
-Stage 1 times every launch shape on the batch itself, with the iteration
-count capped (``maxiter``) so that each candidate costs a few milliseconds;
-stage 2 runs full solves for each ``check_every`` candidate at the chosen
-shape.  The winner is cached in a JSON file keyed by a signature of the
+Stage 1 times every launch shape (grid, ILU grid, walk) on the batch itself,
+with the iteration count capped (``maxiter``) so that each candidate costs a
+few milliseconds; stage 2 runs full solves for each (``rows_per_thread``,
+``check_every``) pair at that shape.  Rows per thread is judged on full
+solves because it changes the summation tree of the dot products, and with
+it the iteration count of the hardest systems (a few per cent per iteration
+can be lost many times over in iterations).  The winner is cached in a JSON file keyed by a signature of the
 workload (precision, preconditioner, system count and size class, device),
 at ``$KCFD_TUNE_CACHE`` or ``~/.cache/kcfd_amd/autotune.json``.
 """
@@ -31,6 +36,7 @@ from . import device
 GRID_PER_CU = (2, 4, 8)
 TRSV_PER_CU = (2, 4)
 TILE_WALK = (1, 2)
+ROWS_PER_THREAD = (2, 1, 4)
 CHECK_EVERY = (8, 16, 32)
 
 
@@ -44,7 +50,7 @@ def signature(sb, precision: str, precond: str, order: str | None, ctx: L.Contex
     info = ctx.info()
     nnz = sum(m.nnz for m in sb.mats)
     return "|".join([
-        "v2", precision, precond, str(order if precond == "ilu0" else None),
+        "v3", precision, precond, str(order if precond == "ilu0" else None),
         f"nsys2^{int(round(math.log2(max(1, len(n)))))}",
         f"rows2^{int(round(math.log2(max(1, int(n.sum())))))}",
         f"nnzrow{int(round(nnz / max(1, int(n.sum()))))}",
@@ -90,7 +96,7 @@ def _time_batch(sb, opts, reps=2):
 
 def autotune(sb, precision="mixed", precond="jacobi", order="color", rtol=1e-8, delta=0.1,
              probe_iters=24, use_cache=True, log=None) -> dict:
-    """Best {grid_per_cu, trsv_per_cu, tile_walk, check_every} for this batch."""
+    """Best {grid_per_cu, trsv_per_cu, tile_walk, rows_per_thread, check_every} for this batch."""
     sb.setup(precond, precision, order)
     key = signature(sb, precision, precond, order, sb.ctx)
     db = _load() if use_cache else {}
@@ -110,21 +116,26 @@ def autotune(sb, precision="mixed", precond="jacobi", order="color", rtol=1e-8, 
             log(f"autotune: grid_per_cu={g} trsv_per_cu={t} tile_walk={w}: {ms:.2f} ms "
                 f"for <= {probe_iters} iterations")
     shape = min(trials, key=lambda r: r["probe_ms"])
+    # rows per thread changes the reduction tree, and with it the iteration
+    # counts of the hardest systems: judged on full solves, with check_every
     best = None
-    for ce in CHECK_EVERY:
+    full = []
+    for rpt, ce in itertools.product(ROWS_PER_THREAD, CHECK_EVERY):
         o = device.default_opts(rtol=rtol, precond=precond, precision=precision, inner_rtol=delta,
                                 check_every=ce, grid_per_cu=shape["grid_per_cu"],
-                                trsv_per_cu=shape["trsv_per_cu"], tile_walk=shape["tile_walk"])
+                                trsv_per_cu=shape["trsv_per_cu"], tile_walk=shape["tile_walk"],
+                                rows_per_thread=rpt)
         ms = _time_batch(sb, o, reps=1)
+        full.append(dict(rows_per_thread=rpt, check_every=ce, full_ms=round(ms, 3)))
         if log:
-            log(f"autotune: check_every={ce}: {ms:.2f} ms (full solve)")
-        if best is None or ms < best[1]:
-            best = (ce, ms)
+            log(f"autotune: rows_per_thread={rpt} check_every={ce}: {ms:.2f} ms (full solve)")
+        if best is None or ms < best[2]:
+            best = (rpt, ce, ms)
     choice = dict(grid_per_cu=shape["grid_per_cu"], trsv_per_cu=shape["trsv_per_cu"],
-                  tile_walk=shape["tile_walk"], check_every=best[0])
+                  tile_walk=shape["tile_walk"], rows_per_thread=best[0], check_every=best[1])
     if use_cache:
         db = _load()
-        db[key] = dict(best=choice, trials=trials, full_solve_ms=round(best[1], 3))
+        db[key] = dict(best=choice, trials=trials, full=full, full_solve_ms=round(best[2], 3))
         _store(db)
     return choice
 
@@ -134,4 +145,4 @@ def describe(choice: dict) -> str:
 
 
 __all__ = ["autotune", "signature", "cache_path", "describe", "GRID_PER_CU", "TRSV_PER_CU",
-           "TILE_WALK", "CHECK_EVERY"]
+           "TILE_WALK", "ROWS_PER_THREAD", "CHECK_EVERY"]

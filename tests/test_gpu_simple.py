@@ -49,6 +49,8 @@ def test_pressure_correction_step(ctx, kind, dirichlet):
           f"p' rel {e_pp:.2e}, U {e_U:.2e}, p {e_p:.2e}")
     assert st.solve["info"] == 0
     assert e_pp < 1e-10 and e_U < 1e-10 and e_p < 1e-10
+    ratio_o = np.linalg.norm(pp_o) / max(np.linalg.norm(p_o) + 1e-10, 1e-10)   # single_phase.py:376
+    assert abs(st.ratio - ratio_o) <= 1e-9 * ratio_o
 
 
 def test_correction_matches_oracle(ctx):

@@ -279,3 +279,44 @@ def test_hybrid_full_size(ctx):
           f"{res.ms:.1f} ms")
     assert ev <= 1e-14
     assert res.info == 0 and tr < 1e-8
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4])
+def test_rows_per_thread(ctx, rows):
+    """Each tile-kernel shape (1, 2 or 4 rows per thread; tiles of 256, 512
+    or 1024 rows): FP64 BiCGSTAB and CG inside the oracle's band, the mixed
+    solve FP64-verified, and a batch equal bit for bit to its systems solved
+    alone with the same shape."""
+    device, workloads = _kc()
+    from kcfd_amd._lib import DevBuf
+    A, b, _ = oracle_system(workloads.hex_pressure(14, seed=1002))
+    M = device.DeviceMatrix.from_csr(ctx, A)
+    M.setup_precond("ilu0", "fp64", order="color")
+    x, res = M.solve(b, rtol=1e-8, precond="ilu0", rows_per_thread=rows)
+    assert res.info == 0
+    _check_against_band(x, res, A, b, f"rows {rows} bicgstab", rtol=1e-8, precond="ilu0_color", nrep=16)
+    x, res = M.solve(b, rtol=1e-8, precond="ilu0", method="cg", rows_per_thread=rows)
+    its, infos, xo, spread = oracle_band(A, b, solver="cg", rtol=1e-8, precond="ilu0_color")
+    print(f"rows {rows} cg: gpu {res.iters}, oracle band {min(its)}..{max(its)}")
+    assert res.info == 0 and min(its) - 2 <= res.iters <= max(its) + 2
+    M.setup_precond("ilu0", "mixed", order="color")
+    x, res = M.solve(b, rtol=1e-8, precond="ilu0", precision="mixed", rows_per_thread=rows)
+    assert res.info == 0 and osolve.true_rel_residual(A, x, b) < 1e-8
+    cases = [workloads.quad_convdiff(20, seed=1), workloads.hex_pressure(10, seed=2),
+             workloads.quad_convdiff(33, seed=3)]
+    systems = [oracle_system(c)[:2] for c in cases]
+    alone = []
+    for As, bs in systems:
+        xs, rs = _gpu_solve(ctx, As, bs, rtol=1e-8, precond="jacobi", precision="mixed", rows_per_thread=rows)
+        alone.append((xs, rs.iters))
+    mats = [device.DeviceMatrix.from_csr(ctx, As) for As, _ in systems]
+    for m in mats:
+        m.setup_precond("jacobi", "mixed")
+    bd = [DevBuf.from_array(ctx, bs) for _, bs in systems]
+    xd = [DevBuf.from_array(ctx, np.zeros(len(bs))) for _, bs in systems]
+    bt = device.Batch(ctx, mats, bd, xd, device.default_opts(rtol=1e-8, precond="jacobi", precision="mixed",
+                                                             rows_per_thread=rows))
+    out = bt.run()
+    for k in range(len(systems)):
+        assert out[k].iters == alone[k][1] and np.array_equal(xd[k].download(), alone[k][0])
+    bt.close()
