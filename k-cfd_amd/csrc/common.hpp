@@ -203,6 +203,7 @@ int precond_apply_async(kcfd_mat *m, const double *x, double *y, double *t, bool
 int gmres_solve(kcfd_mat *m, const double *b, double *x, const kcfd_solve_opts *o,
                 kcfd_solve_result *res);
 void gmres_free(kcfd_mat *m);
+size_t gmres_bytes(const kcfd_mat *m);   // device memory of the cached GMRES workspace
 int mat_free_precond(kcfd_mat *m);
 int mat_free_ilu_structure(kcfd_mat *m);
 }  // namespace kcfd

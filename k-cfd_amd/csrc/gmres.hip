@@ -538,6 +538,8 @@ static int gm_precond(kcfd_mat *m, const double *x, double *y, double *t, bool p
     return precond_apply_async(m, x, y, t, porder, skip, col);
 }
 
+size_t gmres_bytes(const kcfd_mat *m) { return m->gmres ? m->gmres->bytes + sizeof(GmState) : 0; }
+
 void gmres_free(kcfd_mat *m) {
     GmresWs *w = m->gmres;
     if (!w) return;

@@ -511,6 +511,7 @@ int kcfd_mat_bytes(kcfd_mat *m, long long *bytes) {
     if (m->lu) b += 8LL * m->nnz + 8LL * m->n;
     if (m->perm) b += 4LL * (m->n + 1 + m->n + m->n) + 12LL * m->nnz;
     if (m->lu32) b += 4LL * m->nnz;
+    b += (long long)gmres_bytes(m);
     *bytes = b;
     return KCFD_OK;
 }
