@@ -198,3 +198,25 @@ def test_gmres_graph_replay_equals_direct(ctx, monkeypatch):
         assert runs[0][2] > 3
         for x, it, cyc in runs[1:]:
             assert (it, cyc) == runs[0][1:] and np.array_equal(x, runs[0][0])
+
+
+def test_invalid_options_raise(ctx):
+    """The C-ABI rejects what it cannot run, with a message: GMRES in mixed
+    precision, a restart above 64, GMRES in a batch, and rows per thread
+    other than 1, 2 or 4; the matrix stays usable afterwards."""
+    from kcfd_amd import device
+    from kcfd_amd._lib import DevBuf, KcfdError
+    A, b, _ = oracle_system(_case("quad32"))
+    M = device.DeviceMatrix.from_csr(ctx, A)
+    M.setup_precond("jacobi", "mixed")
+    with pytest.raises(KcfdError, match="FP64"):
+        M.solve(b, rtol=1e-8, precond="jacobi", precision="mixed", method="gmres")
+    with pytest.raises(KcfdError, match="restart"):
+        M.solve(b, rtol=1e-8, precond="jacobi", method="gmres", restart=65)
+    with pytest.raises(KcfdError, match="rows_per_thread"):
+        M.solve(b, rtol=1e-8, precond="jacobi", rows_per_thread=3)
+    bd, xd = DevBuf.from_array(ctx, b), DevBuf.from_array(ctx, np.zeros(len(b)))
+    with pytest.raises(KcfdError, match="GMRES"):
+        device.Batch(ctx, [M], [bd], [xd], device.default_opts(rtol=1e-8, precond="jacobi", method="gmres"))
+    x, res = M.solve(b, rtol=1e-8, precond="jacobi", method="gmres")
+    assert res.info == 0
