@@ -1678,6 +1678,10 @@ int check_opts(const kcfd_solve_opts *o) {
 }
 
 int check_mat(const kcfd_batch *b, const kcfd_mat *m, int k) {
+    if (m->n <= 0) {   // a tile of no rows would leave its system's partials unwritten
+        set_error("matrix %d is empty (n = 0); kcfd_solve handles it alone", k);
+        return KCFD_EARG;
+    }
     if (m->ctx != b->ctx) {
         set_error("matrix %d belongs to another context", k);
         return KCFD_EARG;

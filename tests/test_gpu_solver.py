@@ -320,3 +320,46 @@ def test_rows_per_thread(ctx, rows):
     for k in range(len(systems)):
         assert out[k].iters == alone[k][1] and np.array_equal(xd[k].download(), alone[k][0])
     bt.close()
+
+
+def test_degenerate_sizes(ctx):
+    """n = 0: every method returns at once with info 0 (a batch rejects the
+    empty system with a message); n = 1: every method solves a x = b with
+    scipy's iteration count, alone and inside a batch next to larger
+    systems."""
+    device, workloads = _kc()
+    from scipy import sparse
+    from kcfd_amd._lib import DevBuf, KcfdError
+    E = sparse.csr_matrix((0, 0))
+    M0 = device.DeviceMatrix.from_csr(ctx, E)
+    for pc in ("jacobi", "ilu0"):
+        for method in ("bicgstab", "cg", "gmres"):
+            x, res = M0.solve(np.zeros(0), rtol=1e-8, precond=pc, method=method)
+            assert x.shape == (0,) and res.info == 0 and res.iters == 0
+    one = sparse.csr_matrix(np.array([[4.0]]))
+    M1 = device.DeviceMatrix.from_csr(ctx, one)
+    for pc in ("none", "jacobi", "ilu0"):
+        for method in ("bicgstab", "cg", "gmres"):
+            x, res = M1.solve(np.array([2.0]), rtol=1e-12, precond=pc, method=method)
+            print(f"n=1 {pc} {method}: x={x[0]!r} iters={res.iters} info={res.info}")
+            assert res.info == 0 and abs(x[0] - 0.5) <= 1e-15
+            # scipy's callback counts on 4 x = 2: bicgstab 0 (half-step exit), cg 1, gmres 1
+            assert res.iters == {"bicgstab": 0, "cg": 1, "gmres": 1}[method]
+        x, res = M1.solve(np.array([2.0]), rtol=1e-12, precond=pc, precision="mixed")
+        assert res.info == 0 and abs(x[0] - 0.5) <= 1e-15
+    A, b, _ = oracle_system(workloads.quad_convdiff(20, seed=1))
+    mats = [device.DeviceMatrix.from_csr(ctx, one), device.DeviceMatrix.from_csr(ctx, A)]
+    for m in mats:
+        m.setup_precond("jacobi", "fp64")
+    bs = [DevBuf.from_array(ctx, np.array([2.0])), DevBuf.from_array(ctx, b)]
+    xs = [DevBuf.from_array(ctx, np.zeros(1)), DevBuf.from_array(ctx, np.zeros(len(b)))]
+    bt = device.Batch(ctx, mats, bs, xs, device.default_opts(rtol=1e-8, precond="jacobi"))
+    out = bt.run()
+    bt.close()
+    xa, ra = _gpu_solve(ctx, A, b, rtol=1e-8, precond="jacobi")
+    assert out[0].info == 0 and abs(xs[0].download()[0] - 0.5) <= 1e-15
+    assert out[1].iters == ra.iters and np.array_equal(xs[1].download(), xa)
+    M0.setup_precond("jacobi", "fp64")
+    with pytest.raises(KcfdError, match="empty"):
+        device.Batch(ctx, [M0, mats[1]], [DevBuf(ctx, 1), bs[1]], [DevBuf(ctx, 1), xs[1]],
+                     device.default_opts(rtol=1e-8, precond="jacobi"))
