@@ -363,3 +363,29 @@ def test_degenerate_sizes(ctx):
     with pytest.raises(KcfdError, match="empty"):
         device.Batch(ctx, [M0, mats[1]], [DevBuf(ctx, 1), bs[1]], [DevBuf(ctx, 1), xs[1]],
                      device.default_opts(rtol=1e-8, precond="jacobi"))
+
+
+def test_ilu0_bad_pivots_raise(ctx):
+    """A zero pivot (row 0 of [[0, 1], [1, 1]]) and a row without a diagonal
+    entry make the device ILU(0) set-up raise a RuntimeError naming the row,
+    as scipy's spilu raises RuntimeError for a singular factor; Jacobi keeps
+    the reference's rule (1/diag where |diag| > 1e-30, else 0) and ends as
+    scipy does (the same breakdown, iteration and x)."""
+    device, _ = _kc()
+    from scipy import sparse
+    Z = sparse.csr_matrix(np.array([[0.0, 1.0], [1.0, 1.0]]))
+    Z.eliminate_zeros()
+    Zd = sparse.csr_matrix((np.array([0.0, 1.0, 1.0, 1.0]), np.array([0, 1, 0, 1]), np.array([0, 2, 4])),
+                           shape=(2, 2))                     # explicit zero on the diagonal
+    for A, msg in ((Zd, "pivot"), (Z, "no diagonal")):
+        M = device.DeviceMatrix.from_csr(ctx, A)
+        with pytest.raises(RuntimeError, match=msg):
+            M.setup_precond("ilu0", "fp64")
+        M.close()
+    M = device.DeviceMatrix.from_csr(ctx, Zd)
+    x, res = M.solve(np.array([1.0, 2.0]), rtol=1e-12, precond="jacobi", maxiter=50)
+    xo, info_o, it_o = osolve.solve_bicgstab(Zd, np.array([1.0, 2.0]), rtol=1e-12, precond="jacobi",
+                                             maxiter=50)
+    print(f"jacobi with a zero diagonal: x={x}, info={res.info}, iters={res.iters}; scipy {xo}, {info_o}, {it_o}")
+    # scipy: omega breakdown (-11) after 2 iterations at x = (0, 1.5)
+    assert (res.info, res.iters) == (info_o, it_o) and np.array_equal(x, xo)
