@@ -153,7 +153,7 @@ def bench_pressure(args):
         L.check(L.lib().kcfd_asm_reset(A.h))
         L.check(L.lib().kcfd_asm_pressure(dm.h, A.h, dP.p, float(rec["rho"]), pd.ctypes.data))
 
-    results = {}
+    results, sols = {}, {}
     precs = [p for p in args.precisions.split(",") if p]
     if args.method in ("cg", "gmres"):
         precs = ["fp64"]                 # the CG and GMRES paths run in FP64
@@ -186,6 +186,7 @@ def bench_pressure(args):
         ctx.sync()
         wall = time.perf_counter() - t0
         r = rs[-1]
+        sols[prec] = xdev.download()
         results[prec] = dict(solves_per_s=args.steps / wall, ms_per_step=1e3 * wall / args.steps,
                              solve_ms=float(np.mean([q.ms for q in rs])), iters=r.iters,
                              refinements=r.refinements, info=r.info, rel_resid=r.rel_resid,
@@ -287,6 +288,10 @@ def bench_pressure(args):
         "mixed": {k: v for k, v in mixed.items() if k != "kernels"},
         "fp64": {k: v for k, v in fp64.items() if k != "kernels"},
         "mixed_vs_fp64": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3) if len(precs) > 1 else None,
+        # relative L2 of the mixed solution against the FP64 solution of the
+        # same system (the FP64 path is the one held to the scipy oracle)
+        "mixed_error_vs_fp64": (float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
+                                if "mixed" in sols and "fp64" in sols else None),
         "value_precision": "mixed" if "mixed" in results else precs[0],
         "kernels_mixed": kern,
         "kernels_fp64": fp64["kernels"],
