@@ -6,7 +6,7 @@ from .fields import ScalarField, VectorField
 from .fvm_operators import (FVMSystem, apply_boundary_conditions, convection_operator_upwind,
                             diffusion_operator, linearized_source, source_term, temporal_operator,
                             under_relax)
-from .gpu_solver import detect_gpu_backend, gpu_bicgstab, gpu_spmv
+from .gpu_solver import benchmark_solvers, detect_gpu_backend, gpu_bicgstab, gpu_spmv
 from .gradient import green_gauss_gradient, least_squares_gradient
 from .interpolation import compute_mass_flux, get_limiter, muscl_deferred_correction
 from .linear_solver import solve_linear_system, solve_pressure_correction
@@ -21,5 +21,5 @@ __all__ = [
     "solve_linear_system", "solve_pressure_correction",
     "compute_mass_flux", "muscl_deferred_correction", "get_limiter",
     "create_preconditioner",
-    "detect_gpu_backend", "gpu_bicgstab", "gpu_spmv",
+    "detect_gpu_backend", "gpu_bicgstab", "gpu_spmv", "benchmark_solvers",
 ]

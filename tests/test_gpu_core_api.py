@@ -118,3 +118,18 @@ def test_create_preconditioner_colour_ilu(ctx):
     assert rel_l2(M.matvec(z), ref.solve(z)) <= 1e-13
     with pytest.raises(ValueError):
         core.create_preconditioner(A, "nonsense")
+
+
+def test_benchmark_solvers(ctx):
+    """core/gpu_solver.py:202-250: the same keys, the HIP backend, finite
+    median times, x_direct = scipy's spsolve."""
+    import math
+    from scipy.sparse.linalg import spsolve
+    from helpers import oracle_system
+    from kcfd_amd import core, workloads
+    A, b, _ = oracle_system(workloads.quad_convdiff(24, seed=1001))
+    r = core.benchmark_solvers(A, b, n_runs=2)
+    assert set(r) == {"backend", "cpu_direct", "x_direct", "cpu_bicgstab", "gpu_bicgstab"}
+    assert r["backend"] == "hip"
+    assert all(math.isfinite(r[k]) and r[k] > 0 for k in ("cpu_direct", "cpu_bicgstab", "gpu_bicgstab"))
+    assert np.array_equal(r["x_direct"], spsolve(A, b))
