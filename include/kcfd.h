@@ -346,6 +346,17 @@ int kcfd_grad_gg(kcfd_mesh *mesh, const double *phi, const double *phib,
  * kcfd_grad_gg; a cell whose normal-equation matrix is singular gets 0 */
 int kcfd_grad_lsq(kcfd_mesh *mesh, const double *phi, const double *phib,
                   double *grad);
+/* Face values, core/interpolation.py:55-161: mode 0 compute_face_values_upwind
+ * (upwind_face_value), 1 compute_face_values_central (central_face_value),
+ * 2 tvd_face_value for every face.  phi (dev, n), phib (dev, boundary faces in
+ * patch order), F (dev, n_faces; modes 0, 2), grad (dev, n*ndim; mode 2),
+ * phif (dev, n_faces). */
+int kcfd_face_values(kcfd_mesh *mesh, int mode, const double *phi,
+                     const double *phib, const double *F, const double *grad,
+                     double *phif);
+/* The TVD limiters of core/interpolation.py:17-44 (limiter_minmod ...
+ * limiter_van_leer, KCFD_LIM_*) on n values: psi = limiter(r), dev arrays. */
+int kcfd_limiter(kcfd_ctx *ctx, int kind, const double *r, double *psi, long long n);
 /* muscl_deferred_correction (core/interpolation.py:227-303); corr (dev n) */
 int kcfd_muscl(kcfd_mesh *mesh, const double *phi, const double *F,
                const double *grad, int limiter, double *corr);

@@ -135,6 +135,8 @@ SIGNATURES = {
     "kcfd_twofluid_pressure": ([_P, _P, _P, _P, _P, _P, _D, _D, _P, _P], _I),
     "kcfd_twofluid_correct": ([_P, _P, _P, _P, _P, _P, _D, _P, _P, _P], _I),
     "kcfd_muscl": ([_P, _P, _P, _P, _I, _P], _I),
+    "kcfd_face_values": ([_P, _I, _P, _P, _P, _P, _P], _I),
+    "kcfd_limiter": ([_P, _I, _P, _P, ctypes.c_longlong], _I),
     "kcfd_mass_flux": ([_P, _P, _P, _D, _P], _I),
     "kcfd_last_kernel_ms": ([_P, ctypes.POINTER(_D)], _I),
 }

@@ -511,6 +511,63 @@ __device__ __forceinline__ double limiter(int kind, double r) {
     }
 }
 
+// Face values of core/interpolation.py:55-161.  mode 0: upwind by the sign
+// of F (F >= 0: the owner's value); 1: central g phi_O + (1 - g) phi_N with
+// the owner weight g = d_N / (d_O + d_N) (0.5 when the sum < 1e-30); 2: the
+// per-face van Leer TVD value of tvd_face_value, phi_U + 0.5 psi (phi_D -
+// phi_U) with psi = (r + |r|) / (1 + |r|), r = 2 (phi_lin - phi_U) /
+// (phi_D - phi_U) - 1, phi_lin = phi_U + grad_U . (x_f - x_U), and phi_U
+// when |phi_D - phi_U| < 1e-15.  Boundary faces take the patch value (the
+// owner's value for a face in no patch).
+__global__ __launch_bounds__(kBlock) void k_face_values(MeshView m, int mode, const double *__restrict__ phi,
+                                                        const double *__restrict__ phib,
+                                                        const double *__restrict__ F,
+                                                        const double *__restrict__ grad,
+                                                        double *__restrict__ phif) {
+#pragma clang fp contract(off)
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.nf) return;
+    const int o = m.owner[f], nb = m.nb[f];
+    if (nb < 0) {
+        const int bi = m.bidx[f];
+        phif[f] = bi >= 0 ? phib[bi] : phi[o];
+        return;
+    }
+    if (mode == 1) {
+        const double w = m.gw[f];
+        phif[f] = w * phi[o] + (1.0 - w) * phi[nb];
+        return;
+    }
+    const bool pos = F[f] >= 0;
+    const int up = pos ? o : nb, dn = pos ? nb : o;
+    const double pu = phi[up];
+    if (mode == 0) {
+        phif[f] = pu;
+        return;
+    }
+    const int nd = m.ndim;
+    double dot = grad[(size_t)up * nd] * (m.fc[3 * f] - m.cc[3 * up]);
+    dot = dot + grad[(size_t)up * nd + 1] * (m.fc[3 * f + 1] - m.cc[3 * up + 1]);
+    if (nd == 3) dot = dot + grad[(size_t)up * nd + 2] * (m.fc[3 * f + 2] - m.cc[3 * up + 2]);
+    const double lin = pu + dot;
+    const double dup = lin - pu;
+    const double dtot = phi[dn] - pu;
+    if (fabs(dtot) < 1e-15) {
+        phif[f] = pu;
+        return;
+    }
+    const double r = 2.0 * dup / dtot - 1.0;
+    const double ar = fabs(r);
+    const double psi = (1.0 + ar) > 1e-15 ? (r + ar) / (1.0 + ar) : 0.0;
+    phif[f] = pu + 0.5 * psi * dtot;
+}
+
+// psi = limiter(r) elementwise (interpolation.py:17-44)
+__global__ void k_limiter(int kind, const double *__restrict__ r, double *__restrict__ psi, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) psi[i] = limiter(kind, r[i]);
+}
+
 // per internal face: contribution to the owner's correction (the
 // neighbour receives its negation)
 // core/gradient.py:51-99: least-squares gradient.  Per cell, over its faces
@@ -1016,6 +1073,31 @@ int kcfd_twofluid_correct(kcfd_mesh *mesh, const double *pp, const double *alpha
     hipLaunchKernelGGL(k_tf_correct, dim3(blocks(mesh->n)), dim3(kBlock), 0, mesh->ctx->stream, view(mesh), pp,
                        alpha_l, alpha_g, aP_l, aP_g, alpha_p, U_l, U_g, p);
     return t.stop();
+}
+
+int kcfd_face_values(kcfd_mesh *mesh, int mode, const double *phi, const double *phib, const double *F,
+                     const double *grad, double *phif) {
+    KCFD_ARG(mesh && phi && phif && (phib || mesh->nbf == 0), "null argument");
+    KCFD_ARG(mode >= 0 && mode <= 2, "mode must be 0 (upwind), 1 (central) or 2 (TVD)");
+    KCFD_ARG(mode == 1 || F, "upwind and TVD values need the face fluxes");
+    KCFD_ARG(mode != 2 || grad, "TVD values need the gradient");
+    std::lock_guard<std::recursive_mutex> g(mesh->ctx->mu);
+    Timer t(mesh->ctx);
+    hipLaunchKernelGGL(k_face_values, dim3(blocks(mesh->nf)), dim3(kBlock), 0, mesh->ctx->stream, view(mesh),
+                       mode, phi, phib, F, grad, phif);
+    return t.stop();
+}
+
+int kcfd_limiter(kcfd_ctx *ctx, int kind, const double *r, double *psi, long long n) {
+    KCFD_ARG(ctx && n >= 0 && (n == 0 || (r && psi)), "null argument");
+    KCFD_ARG(kind >= KCFD_LIM_MINMOD && kind <= KCFD_LIM_VAN_LEER, "unknown limiter");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (n == 0) return KCFD_OK;
+    hipLaunchKernelGGL(k_limiter, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, ctx->stream, kind,
+                       r, psi, (size_t)n);
+    KCFD_HIP(hipGetLastError());
+    KCFD_HIP(hipStreamSynchronize(ctx->stream));
+    return KCFD_OK;
 }
 
 int kcfd_grad_gg(kcfd_mesh *mesh, const double *phi, const double *phib, double *grad) {

@@ -251,6 +251,51 @@ def green_gauss_gradient(mesh, values, bvals: dict):
     return grad
 
 
+def face_values(mesh, values, bvals: dict, mode: str, F=None, grad=None):
+    """interpolation.py:55-161 for every face: 'upwind' (upwind_face_value:
+    the owner's value where F >= 0, else the neighbour's), 'central'
+    (central_face_value: g phi_O + (1 - g) phi_N, _interp_weight :185-196),
+    'tvd' (tvd_face_value :85-134: phi_U + 0.5 psi (phi_D - phi_U), psi =
+    (r + |r|) / (1 + |r|), r = 2 (phi_lin - phi_U) / (phi_D - phi_U) - 1,
+    phi_lin = phi_U + grad_U . (x_f - x_U); phi_U where |phi_D - phi_U| <
+    1e-15).  Boundary faces: the patch value (_get_boundary_value :199-208)."""
+    nd = mesh.ndim
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    internal = nb >= 0
+    values = np.asarray(values, np.float64)
+    nbs = np.where(internal, nb, o)
+    out = boundary_face_values(mesh, values, bvals).copy()
+    if mode == "central":
+        g = interpolation_weight(mesh)
+        cen = g * values[o] + (1.0 - g) * values[nbs]
+        out[internal] = cen[internal]
+        return out
+    F = np.asarray(F, np.float64)
+    pos = F >= 0
+    up = np.where(pos, o, nbs)
+    dn = np.where(pos, nbs, o)
+    pu = values[up]
+    if mode == "upwind":
+        out[internal] = pu[internal]
+        return out
+    grad = np.asarray(grad, np.float64)
+    dx = mesh.face_center[:, :nd] - mesh.cell_center[up, :nd]
+    dot = grad[up, 0] * dx[:, 0]
+    for k in range(1, nd):
+        dot = dot + grad[up, k] * dx[:, k]
+    dup = (pu + dot) - pu
+    dtot = values[dn] - pu
+    flat = np.abs(dtot) < 1e-15
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = 2.0 * dup / np.where(flat, 1.0, dtot) - 1.0
+        ar = np.abs(r)
+        psi = np.where(1.0 + ar > 1e-15, (r + ar) / (1.0 + ar), 0.0)
+    tvd = np.where(flat, pu, pu + 0.5 * psi * dtot)
+    out[internal] = tvd[internal]
+    return out
+
+
 def least_squares_gradient(mesh, values, bvals: dict):
     """gradient.py:51-99: per cell, over its faces, dx = x_N - x_P (x_f - x_P
     on a boundary face) and dphi = phi_N - phi_P (phi_f - phi_P, the patch

@@ -387,6 +387,29 @@ def _bvalue(m: LMesh, values, bvals, fid):
     return values[m.faces[fid].owner]
 
 
+def face_value(m: LMesh, values, bvals, fid, mode, flux_dir=0.0, grad=None):
+    """interpolation.py:55-134 for one face: upwind_face_value,
+    central_face_value, tvd_face_value."""
+    f = m.faces[fid]
+    if f.neighbour < 0:
+        return _bvalue(m, values, bvals, fid)
+    if mode == "central":
+        w = _weight(m, fid)
+        return w * values[f.owner] + (1.0 - w) * values[f.neighbour]
+    up, dn = (f.owner, f.neighbour) if flux_dir >= 0 else (f.neighbour, f.owner)
+    if mode == "upwind":
+        return values[up]
+    pu, pd = values[up], values[dn]
+    dx = f.center[:m.ndim] - m.cells[up].center[:m.ndim]
+    lin = pu + np.dot(grad[up], dx)
+    dup, dtot = lin - pu, pd - pu
+    if abs(dtot) < 1e-15:
+        return pu
+    r = 2.0 * dup / dtot - 1.0
+    psi = (r + abs(r)) / (1.0 + abs(r)) if (1.0 + abs(r)) > 1e-15 else 0.0
+    return pu + 0.5 * psi * dtot
+
+
 def green_gauss(m: LMesh, values, bvals):
     """gradient.py:12-48."""
     nd = m.ndim
