@@ -30,7 +30,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_SPMV_P_MIXED_HEX128 = 153.9e6   # bytes per launch (145.4 MB read + 8.5 MB written)
+# HBM bytes per k_spmv_p launch on the 128^3 mixed default workload, from the
+# FETCH_SIZE / WRITE_SIZE counter passes: the narrow-index FP32 SELL copy
+# (6 bytes an entry, the library's default) and the packed one (8 bytes an
+# entry, KCFD_SELL16=0)
+TRAFFIC_SPMV_P_MIXED_HEX128 = 115.1e6          # 106.6 MB read + 8.5 MB written
+TRAFFIC_SPMV_P_MIXED_HEX128_PACKED = 153.9e6   # 145.4 MB read + 8.5 MB written
 
 
 def log(*a):
@@ -240,9 +245,17 @@ def bench_pressure(args):
         if (args.method == "bicgstab" and args.n == 128 and args.precond == "ilu0" and args.order == "color"
                 and "mixed" in results and not (args.grid_per_cu or args.trsv_per_cu or args.tile_walk
                                                  or args.rows_per_thread not in (0, 2))):
-            roofline["traffic"] = TRAFFIC_SPMV_P_MIXED_HEX128
+            packed = os.environ.get("KCFD_SELL16", "1").startswith("0")
+            traffic = TRAFFIC_SPMV_P_MIXED_HEX128_PACKED if packed else TRAFFIC_SPMV_P_MIXED_HEX128
+            roofline["traffic"] = traffic
+            # the rate of the bytes the kernel actually moves (the algorithmic
+            # model counts 4-byte column indices; the narrow-index copy
+            # streams 2-byte offsets)
+            roofline["traffic_gbs"] = round(traffic / (dk["avg_ms"] * 1e-3) / 1e9, 1)
+            roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
             roofline["traffic_source"] = ("profiles/hbm_traffic_bicgstab_ilu0_mixed_hex128.md: FETCH_SIZE x 2.000 "
-                                          "+ WRITE_SIZE x 1.000 KiB per dispatch, median working dispatch")
+                                          "+ WRITE_SIZE x 1.000 KiB per dispatch, median working dispatch"
+                                          + (" (packed SELL copy, KCFD_SELL16=0)" if packed else ""))
     cpu = None
     if not args.no_cpu:
         try:

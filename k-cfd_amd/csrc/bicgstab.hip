@@ -82,6 +82,9 @@ struct SysDesc {
     const int *sptr, *six;          // SELL-64 structure
     const double *sa64;
     const int2 *spk32;              // MIXED: packed {column, FP32 value} SELL entries
+    const float *sv32;              // MIXED, narrow-index form (kcfd_mat::so16): values,
+    const unsigned short *so16;     //   16-bit column offsets (kSellDiag: the row itself)
+    const int *sbase;               //   and the column base of each slice
     const double *dinv64;
     const float *dinv32;
     const double *lu64;
@@ -211,10 +214,18 @@ __device__ __forceinline__ double precond_at(const SysDesc &d, const V *__restri
 // cache line) and are not accumulated, so no load sits behind a branch.
 constexpr int kRowUnroll = 4;
 
-// entry e of the SELL copy: column and value (MIXED: one 8-byte load)
-template <typename V>
-__device__ __forceinline__ void sell_entry(const SysDesc &d, int e, int &c, V &v) {
-    if constexpr (sizeof(V) == sizeof(float)) {
+// entry e of the SELL copy of row i: column and value.  MIXED: one 8-byte
+// {column, value} load, or (NARROW) a 4-byte value and a 2-byte offset from
+// the slice's column base sb.  The layout is a template parameter so that
+// each form is straight-line code (a per-entry branch would split the group
+// of loads issued ahead of the accumulation).
+template <typename V, bool NARROW>
+__device__ __forceinline__ void sell_entry(const SysDesc &d, int e, int i, int sb, int &c, V &v) {
+    if constexpr (sizeof(V) == sizeof(float) && NARROW) {
+        const unsigned o = gl(d.so16)[e];
+        v = gl(d.sv32)[e];
+        c = o == kSellDiag ? i : sb + (int)o;
+    } else if constexpr (sizeof(V) == sizeof(float)) {
         const kint2 q = gl(reinterpret_cast<const kint2 *>(d.spk32))[e];
         c = q.x;
         v = __int_as_float(q.y);
@@ -228,16 +239,17 @@ __device__ __forceinline__ void sell_entry(const SysDesc &d, int e, int &c, V &v
 // of which are accumulated into acc.  MIXED (V = float): the FP32 values are
 // in difference form (row sum on the diagonal), so the row is
 // sum_{j!=i} a_ij (z_j - z_i) + rowsum_i z_i with the differences in FP64.
-template <typename V, int PC, int G>
+template <typename V, int PC, int G, bool NARROW>
 __device__ __forceinline__ void amz_group(const SysDesc &d, int i, const V *__restrict__ z,
-                                          const V *__restrict__ zh, int base, int k0, int last,
+                                          const V *__restrict__ zh, int base, int sb, int k0, int last,
                                           int valid, double zi, double &acc) {
 #pragma clang fp contract(off)
     constexpr bool kDiff = sizeof(V) == sizeof(float);
     int col[G];
     V av[G], zv[G];
 #pragma unroll
-    for (int k = 0; k < G; ++k) sell_entry<V>(d, base + 64 * min(k0 + k, last), col[k], av[k]);
+    for (int k = 0; k < G; ++k)
+        sell_entry<V, NARROW>(d, base + 64 * min(k0 + k, last), i, sb, col[k], av[k]);
 #pragma unroll
     for (int k = 0; k < G; ++k) zv[k] = precond_v<V, PC>(d, z, zh, col[k]);
 #pragma unroll
@@ -248,25 +260,34 @@ __device__ __forceinline__ void amz_group(const SysDesc &d, int i, const V *__re
         }
 }
 
-template <typename V, int PC>
-__device__ __forceinline__ double row_amz(const SysDesc &d, int i, const V *__restrict__ z,
-                                          const V *__restrict__ zh) {
+template <typename V, int PC, bool NARROW>
+__device__ __forceinline__ double row_amz_layout(const SysDesc &d, int i, const V *__restrict__ z,
+                                                 const V *__restrict__ zh, int s, int s0, int w) {
 #pragma clang fp contract(off)
     constexpr bool kDiff = sizeof(V) == sizeof(float);
-    const int s = uniform(i >> 6);
-    const int s0 = uniform(gl(d.sptr)[s]), w = uniform((gl(d.sptr)[s + 1] - s0) >> 6);
     const int base = s0 + (i & 63);
+    const int sb = NARROW ? uniform(gl(d.sbase)[s]) : 0;
     const double zi = kDiff ? precond_at<V, PC>(d, z, zh, i) : 0.0;
     double acc = 0.0;
     if (w == 7) {
-        amz_group<V, PC, 7>(d, i, z, zh, base, 0, 6, 7, zi, acc);
+        amz_group<V, PC, 7, NARROW>(d, i, z, zh, base, sb, 0, 6, 7, zi, acc);
     } else if (w == 5) {
-        amz_group<V, PC, 5>(d, i, z, zh, base, 0, 4, 5, zi, acc);
+        amz_group<V, PC, 5, NARROW>(d, i, z, zh, base, sb, 0, 4, 5, zi, acc);
     } else {
         for (int k0 = 0; k0 < w; k0 += kRowUnroll)
-            amz_group<V, PC, kRowUnroll>(d, i, z, zh, base, k0, w - 1, w - k0, zi, acc);
+            amz_group<V, PC, kRowUnroll, NARROW>(d, i, z, zh, base, sb, k0, w - 1, w - k0, zi, acc);
     }
     return acc;
+}
+
+template <typename V, int PC>
+__device__ __forceinline__ double row_amz(const SysDesc &d, int i, const V *__restrict__ z,
+                                          const V *__restrict__ zh) {
+    const int s = uniform(i >> 6);
+    const int s0 = uniform(gl(d.sptr)[s]), w = uniform((gl(d.sptr)[s + 1] - s0) >> 6);
+    if constexpr (sizeof(V) == sizeof(float))
+        if (d.so16) return row_amz_layout<V, PC, true>(d, i, z, zh, s, s0, w);
+    return row_amz_layout<V, PC, false>(d, i, z, zh, s, s0, w);
 }
 
 // (A x)_i in FP64 on the FP64 values
@@ -277,7 +298,7 @@ __device__ __forceinline__ void ax_group(const SysDesc &d, const double *__restr
     int col[G];
     double av[G], xv[G];
 #pragma unroll
-    for (int k = 0; k < G; ++k) sell_entry<double>(d, base + 64 * min(k0 + k, last), col[k], av[k]);
+    for (int k = 0; k < G; ++k) sell_entry<double, false>(d, base + 64 * min(k0 + k, last), 0, 0, col[k], av[k]);
 #pragma unroll
     for (int k = 0; k < G; ++k) xv[k] = gl(x)[col[k]];
 #pragma unroll
@@ -1642,6 +1663,9 @@ void fill_desc(SysDesc &d, kcfd_mat *m) {
     d.six = m->six;
     d.sa64 = m->sa64;
     d.spk32 = m->spk32;
+    d.sv32 = m->sv32;
+    d.so16 = m->so16;
+    d.sbase = m->sbase;
     d.dinv64 = m->dinv;
     d.dinv32 = m->dinv32;
     d.lu64 = m->lu;

@@ -24,6 +24,11 @@ constexpr int kFinBlock = 1024;      // threads of a finalize workgroup (one per
 // Number of XCDs on MI355X; workgroups are dispatched round-robin over them.
 constexpr int kXcds = 8;
 
+// narrow-index SELL copy: the offset that marks column = row, and the widest
+// span of off-diagonal columns one 64-row slice may have
+constexpr unsigned kSellDiag = 0xFFFFu;
+constexpr int kSellSpan = 0xFFFE;
+
 // Global-memory views of pointers.  Pointers read from a descriptor in
 // memory are generic to the compiler, which then emits FLAT loads that wait
 // on both the vector-memory and LDS counters; cast to the global address
@@ -145,6 +150,14 @@ struct kcfd_mat {
     int *slice_ptr = nullptr, *six = nullptr;
     double *sa64 = nullptr;
     int2 *spk32 = nullptr;   // MIXED: {column, FP32 value bits} per SELL entry (one 8-byte load)
+    // MIXED, narrow-index form (used instead of spk32 when every slice's
+    // off-diagonal columns span at most kSellSpan): FP32 values, 16-bit
+    // column offsets from the slice's smallest off-diagonal column, and that
+    // base per slice; the diagonal and padding (column = row) as kSellDiag.
+    // 6 bytes per entry instead of 8.
+    float *sv32 = nullptr;
+    unsigned short *so16 = nullptr;
+    int *sbase = nullptr;
     // preconditioner
     int pc_kind = KCFD_PC_NONE;
     bool have32 = false;

@@ -3,6 +3,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 
@@ -96,10 +97,14 @@ __global__ void k_sell_struct(int n, const int *__restrict__ ip, const int *__re
 // off-diagonals to FP32 then leaves the row sums (and the near-null space of
 // a diffusion operator) intact, where rounding a_ii separately would perturb
 // the smallest eigenvalues by eps_32 |a_ii|.
+// With so16 the FP32 copy goes to sv32 / so16 (narrow-index form, the base
+// of each slice from k_sell_base) instead of spk32.
 __global__ void k_sell_values(int n, const int *__restrict__ ip, const int *__restrict__ ix,
                               const int *__restrict__ sptr, const int *__restrict__ six,
                               const double *__restrict__ a, double *__restrict__ sa64,
-                              int2 *__restrict__ spk32, const int *__restrict__ rperm) {
+                              int2 *__restrict__ spk32, float *__restrict__ sv32,
+                              unsigned short *__restrict__ so16, const int *__restrict__ sbase,
+                              const int *__restrict__ rperm) {
 #pragma clang fp contract(off)
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -111,10 +116,44 @@ __global__ void k_sell_values(int n, const int *__restrict__ ip, const int *__re
     for (int j = 0; j < w; ++j) {
         const double v = j < len ? a[q0 + j] : 0.0;
         sa64[base + 64 * j] = v;
-        if (spk32) {
-            const float v32 = (float)(j < len && ix[q0 + j] == r ? rowsum : v);
-            spk32[base + 64 * j] = make_int2(six[base + 64 * j], __float_as_int(v32));
+        const float v32 = (float)(j < len && ix[q0 + j] == r ? rowsum : v);
+        const int c = six[base + 64 * j];
+        if (so16) {
+            sv32[base + 64 * j] = v32;
+            so16[base + 64 * j] = (unsigned short)(c == k ? kSellDiag : (unsigned)(c - sbase[s]));
+        } else if (spk32) {
+            spk32[base + 64 * j] = make_int2(c, __float_as_int(v32));
         }
+    }
+}
+
+// Base of each slice for the narrow-index SELL copy: the smallest column
+// other than the row itself over the slice's entries (0 when there is none),
+// and *wide = 1 when a slice's such columns span more than kSellSpan.  One
+// wave per slice (kBlock is a multiple of 64, so a wave's lanes are the rows
+// of one slice); every lane takes part in the shuffles.
+__global__ __launch_bounds__(kBlock) void k_sell_base(int n, const int *__restrict__ sptr,
+                                                      const int *__restrict__ six, int *__restrict__ sbase,
+                                                      int *__restrict__ wide) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    int lo = INT_MAX, hi = INT_MIN;
+    if (k < n) {
+        const int s = k >> 6, base = sptr[s] + (k & 63), w = (sptr[s + 1] - sptr[s]) >> 6;
+        for (int j = 0; j < w; ++j) {
+            const int c = six[base + 64 * j];
+            if (c != k) {
+                lo = min(lo, c);
+                hi = max(hi, c);
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    if (k < n && (k & 63) == 0) {
+        sbase[k >> 6] = lo == INT_MAX ? 0 : lo;
+        if (lo != INT_MAX && hi - lo > kSellSpan) *wide = 1;
     }
 }
 
@@ -150,6 +189,9 @@ int ensure_sell(kcfd_mat *m, bool permuted) {
     hipFree(m->six); m->six = nullptr;
     hipFree(m->sa64); m->sa64 = nullptr;
     hipFree(m->spk32); m->spk32 = nullptr;
+    hipFree(m->sv32); m->sv32 = nullptr;
+    hipFree(m->so16); m->so16 = nullptr;
+    hipFree(m->sbase); m->sbase = nullptr;
     const std::vector<int> &ip = m->h_indptr;
     m->nslices = (n + 63) / 64;
     m->h_slice_ptr.assign(m->nslices + 1, 0);
@@ -187,10 +229,47 @@ int ensure_sell(kcfd_mat *m, bool permuted) {
     return KCFD_OK;
 }
 
+// Buffers of the FP32 SELL copy: the narrow-index form when every slice's
+// off-diagonal columns fit 16-bit offsets (KCFD_SELL16=0 forces the packed
+// {column, value} form), else spk32.
+int sell_mixed_alloc(kcfd_mat *m) {
+    hipStream_t st = m->ctx->stream;
+    const char *env = std::getenv("KCFD_SELL16");
+    bool narrow = !(env && env[0] == '0') && m->n > 0;
+    if (narrow) {
+        int *wide_d = nullptr;
+        int wide = 0;
+        KCFD_HIP(hipMalloc(&m->sbase, sizeof(int) * m->nslices));
+        KCFD_HIP(hipMalloc(&wide_d, sizeof(int)));
+        KCFD_HIP(hipMemcpyAsync(wide_d, &wide, sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_sell_base, dim3((m->n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, m->n,
+                           m->slice_ptr, m->six, m->sbase, wide_d);
+        KCFD_HIP(hipGetLastError());
+        KCFD_HIP(hipMemcpyAsync(&wide, wide_d, sizeof(int), hipMemcpyDeviceToHost, st));
+        KCFD_HIP(hipStreamSynchronize(st));
+        hipFree(wide_d);
+        narrow = wide == 0;
+        if (!narrow) {
+            hipFree(m->sbase);
+            m->sbase = nullptr;
+        }
+    }
+    if (narrow) {
+        KCFD_HIP(hipMalloc(&m->sv32, sizeof(float) * std::max(m->sell_nnz, 1)));
+        KCFD_HIP(hipMalloc(&m->so16, sizeof(unsigned short) * std::max(m->sell_nnz, 1)));
+    } else {
+        KCFD_HIP(hipMalloc(&m->spk32, sizeof(int2) * std::max(m->sell_nnz, 1)));
+    }
+    return KCFD_OK;
+}
+
 // Value buffers of the preconditioner.  The ILU(0) ordering and level
 // schedule depend on the structure only and survive (mat_free_ilu_structure).
 int mat_free_precond(kcfd_mat *m) {
     hipFree(m->spk32); m->spk32 = nullptr;
+    hipFree(m->sv32); m->sv32 = nullptr;
+    hipFree(m->so16); m->so16 = nullptr;
+    hipFree(m->sbase); m->sbase = nullptr;
     hipFree(m->dinv); m->dinv = nullptr;
     hipFree(m->dinv32); m->dinv32 = nullptr;
     hipFree(m->lu); m->lu = nullptr;
@@ -506,6 +585,7 @@ int kcfd_mat_bytes(kcfd_mat *m, long long *bytes) {
     long long b = 4LL * (m->n + 1) + 4LL * m->nnz + 4LL * m->n + 8LL * m->nnz + 16LL * m->n;
     b += 4LL * (m->nslices + 1) + 12LL * m->sell_nnz;
     if (m->spk32) b += 8LL * m->sell_nnz;
+    if (m->so16) b += 6LL * m->sell_nnz + 4LL * m->nslices;
     if (m->dinv) b += 8LL * m->n;
     if (m->dinv32) b += 4LL * m->n;
     if (m->lu) b += 8LL * m->nnz + 8LL * m->n;
@@ -543,7 +623,7 @@ int kcfd_precond_setup(kcfd_mat *m, int kind, int precision) {
     // ordering first, then the SELL copy in that row order
     if (kind == KCFD_PC_ILU0) KCFD_CHECK(ilu0_structure(m));
     KCFD_CHECK(ensure_sell(m, kind == KCFD_PC_ILU0 && m->perm != nullptr));
-    if (want32 && !m->spk32) KCFD_HIP(hipMalloc(&m->spk32, sizeof(int2) * std::max(m->sell_nnz, 1)));
+    if (want32 && !m->spk32 && !m->so16) KCFD_CHECK(sell_mixed_alloc(m));
     if (kind == KCFD_PC_JACOBI && !m->dinv) {
         KCFD_HIP(hipMalloc(&m->dinv, sizeof(double) * (n ? n : 1)));
         if (want32) KCFD_HIP(hipMalloc(&m->dinv32, sizeof(float) * (n ? n : 1)));
@@ -551,7 +631,8 @@ int kcfd_precond_setup(kcfd_mat *m, int kind, int precision) {
     if (n) {
         hipLaunchKernelGGL(k_sell_values, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, n,
                            m->indptr, m->indices, m->slice_ptr, m->six, m->val, m->sa64,
-                           want32 ? m->spk32 : nullptr, m->sell_permuted ? m->perm : nullptr);
+                           want32 ? m->spk32 : nullptr, want32 ? m->sv32 : nullptr,
+                           want32 ? m->so16 : nullptr, m->sbase, m->sell_permuted ? m->perm : nullptr);
         KCFD_HIP(hipGetLastError());
     }
     if (kind == KCFD_PC_JACOBI && n) {

@@ -389,3 +389,63 @@ def test_ilu0_bad_pivots_raise(ctx):
     print(f"jacobi with a zero diagonal: x={x}, info={res.info}, iters={res.iters}; scipy {xo}, {info_o}, {it_o}")
     # scipy: omega breakdown (-11) after 2 iterations at x = (0, 1.5)
     assert (res.info, res.iters) == (info_o, it_o) and np.array_equal(x, xo)
+
+
+def _wide_span_system(n=70_000):
+    """Tridiagonal diffusion plus a coupling of rows 0 and n-1: slice 0's
+    off-diagonal columns span n - 1 > 65534."""
+    from scipy import sparse
+    main = np.full(n, 2.5)
+    off = np.full(n - 1, -1.0)
+    A = sparse.diags([off, main, off], [-1, 0, 1], format="lil")
+    A[0, n - 1] = -0.25
+    A[n - 1, 0] = -0.25
+    A = A.tocsr()
+    A.sort_indices()
+    return A, np.random.default_rng(77).uniform(-1, 1, n)
+
+
+@pytest.mark.parametrize("which", ["hex32_color", "tet_jacobi", "quad_natural", "hybrid_color", "wide"])
+def test_mixed_narrow_index_sell(ctx, which, monkeypatch):
+    """The mixed SELL copy with 16-bit column offsets from a per-slice base
+    (6 bytes an entry) holds the same columns and values as the packed
+    {column, value} form (KCFD_SELL16=0), so a mixed solve is bit-identical
+    either way.  The narrow form is taken where every 64-row slice's
+    off-diagonal columns span at most 65534, which the device matrix's
+    footprint shows (2 bytes less per entry, 4 more per slice); a matrix with
+    a wider slice keeps the packed form."""
+    device, workloads = _kc()
+    pc, order = "ilu0", "color"
+    if which == "hex32_color":
+        A, b, _ = oracle_system(workloads.hex_pressure(32, seed=1002))
+    elif which == "tet_jacobi":
+        A, b, _ = oracle_system(workloads.tet_twofluid(3000, seed=1003))
+        pc, order = "jacobi", None
+    elif which == "quad_natural":
+        A, b, _ = oracle_system(workloads.quad_convdiff(64, seed=1001))
+        order = "natural"
+    elif which == "hybrid_color":
+        A, b, _ = oracle_system(workloads.hybrid_pressure(12, 4, 4, seed=1007))
+    else:
+        A, b = _wide_span_system()
+        pc, order = "jacobi", None
+    out = {}
+    for mode in ("packed", "narrow"):
+        if mode == "packed":
+            monkeypatch.setenv("KCFD_SELL16", "0")
+        else:
+            monkeypatch.delenv("KCFD_SELL16", raising=False)
+        M = device.DeviceMatrix.from_csr(ctx, A)
+        M.setup_precond(pc, "mixed", **({"order": order} if order else {}))
+        x, res = M.solve(b, rtol=1e-8, precond=pc, precision="mixed")
+        out[mode] = (x, res.iters, res.refinements, res.info, M.bytes())
+        M.close()
+    (xp, ip_, rp, fp, bp), (xn, in_, rn, fn, bn) = out["packed"], out["narrow"]
+    print(f"{which}: n={A.shape[0]} iters {ip_}/{in_} refinements {rp}/{rn} info {fp}/{fn} "
+          f"bytes packed {bp} narrow {bn}")
+    assert np.array_equal(xp, xn) and (ip_, rp, fp) == (in_, rn, fn)
+    assert fp == 0 and osolve.true_rel_residual(A, xn, b) < 1e-8
+    if which == "wide":
+        assert bn == bp
+    else:
+        assert bn < bp
