@@ -78,6 +78,11 @@ struct SysDesc {
     const int *chl_off, *chl_col, *chu_off, *chu_col;
     const double *chl_v64, *chu_v64, *ud64;
     const int2 *chl_pk32, *chu_pk32;   // MIXED: packed {column, FP32 value} entries
+    // MIXED, narrow-index two-colour layouts (kcfd_mat::ch16; null if packed):
+    // FP32 values, 16-bit offsets from the chunk's base column, the bases
+    const float *chl_v32, *chu_v32;
+    const unsigned short *chl_o16, *chu_o16;
+    const int *chl_base, *chu_base;
     const float *ud32;
     const int *sptr, *six;          // SELL-64 structure
     const double *sa64;
@@ -165,6 +170,23 @@ __device__ __forceinline__ void fac_entry(const FacRef &f, int e, int &c, V &v) 
         c = gl(f.col)[e];
         v = gl(f.v64)[e];
     }
+}
+// narrow-index chunk entries: an FP32 value and a 16-bit offset from the
+// chunk's base column cb; padding (kSellDiag) decodes to column -1
+struct FacRef16 {
+    const float *v32;
+    const unsigned short *o16;
+    int cb;
+};
+template <typename V>
+__device__ __forceinline__ void fac_get(const FacRef &f, int e, int &c, V &v) {
+    fac_entry<V>(f, e, c, v);
+}
+template <typename V>
+__device__ __forceinline__ void fac_get(const FacRef16 &f, int e, int &c, V &v) {
+    const unsigned o = gl(f.o16)[e];
+    v = gl(f.v32)[e];
+    c = o == kSellDiag ? -1 : f.cb + (int)o;
 }
 // typed global view of a workspace vector (stored as void *)
 template <typename V>
@@ -1324,14 +1346,14 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
 // row_amz: fixed groups for the common widths, else groups of 8 whose loads
 // past the width re-load the last entry; padding (column -1) gathers entry
 // 0.  Neither is accumulated.
-template <typename V, int G>
-__device__ __forceinline__ void sub_group(V &acc, const FacRef &f, int base, int k0, int last, int valid,
+template <typename V, int G, typename F>
+__device__ __forceinline__ void sub_group(V &acc, const F &f, int base, int k0, int last, int valid,
                                           const KCFD_GLOBAL V *__restrict__ vec) {
 #pragma clang fp contract(off)
     int c[G];
     V av[G], xv[G];
 #pragma unroll
-    for (int k = 0; k < G; ++k) fac_entry<V>(f, base + 64 * min(k0 + k, last), c[k], av[k]);
+    for (int k = 0; k < G; ++k) fac_get<V>(f, base + 64 * min(k0 + k, last), c[k], av[k]);
 #pragma unroll
     for (int k = 0; k < G; ++k) xv[k] = vec[max(c[k], 0)];
 #pragma unroll
@@ -1339,8 +1361,8 @@ __device__ __forceinline__ void sub_group(V &acc, const FacRef &f, int base, int
         if (k < valid && c[k] >= 0) acc -= av[k] * xv[k];
 }
 
-template <typename V>
-__device__ __forceinline__ V sub_plain(V acc, const FacRef &f, int base, int w,
+template <typename V, typename F>
+__device__ __forceinline__ V sub_plain(V acc, const F &f, int base, int w,
                                        const KCFD_GLOBAL V *__restrict__ vec) {
     if (w == 6) {
         sub_group<V, 6>(acc, f, base, 0, 5, 6, vec);
@@ -1389,7 +1411,18 @@ __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys
             const int *off = SECOND ? d.chu_off : d.chl_off;
             const int o0 = uniform(gl(off)[ch.local]);
             const int wd = uniform((gl(off)[ch.local + 1] - o0) >> 6);
-            const V v = sub_plain<V>(z[i], SECOND ? fac_u(d) : fac_l(d), o0 + lane, wd, SECOND ? w : z);
+            // the factor layout is chosen once per chunk (wave-uniform), so
+            // each form is straight-line code
+            V v;
+            const unsigned short *o16 = nullptr;
+            if constexpr (sizeof(V) == sizeof(float)) o16 = SECOND ? d.chu_o16 : d.chl_o16;
+            if (o16) {
+                const FacRef16 f{SECOND ? d.chu_v32 : d.chl_v32, o16,
+                                 uniform(gl(SECOND ? d.chu_base : d.chl_base)[ch.local])};
+                v = sub_plain<V>(z[i], f, o0 + lane, wd, SECOND ? w : z);
+            } else {
+                v = sub_plain<V>(z[i], SECOND ? fac_u(d) : fac_l(d), o0 + lane, wd, SECOND ? w : z);
+            }
             w[i] = v / gl(udiag<V>(d))[i];
         }
     }
@@ -1680,6 +1713,13 @@ void fill_desc(SysDesc &d, kcfd_mat *m) {
     d.ud64 = m->ud64;
     d.chl_pk32 = m->chl_pk32;
     d.chu_pk32 = m->chu_pk32;
+    const bool ch16 = m->ch16 && m->chl_v32 != nullptr;
+    d.chl_v32 = ch16 ? m->chl_v32 : nullptr;
+    d.chu_v32 = ch16 ? m->chu_v32 : nullptr;
+    d.chl_o16 = ch16 ? m->chl_o16 : nullptr;
+    d.chu_o16 = ch16 ? m->chu_o16 : nullptr;
+    d.chl_base = ch16 ? m->chl_base : nullptr;
+    d.chu_base = ch16 ? m->chu_base : nullptr;
     d.ud32 = m->ud32;
 }
 

@@ -199,6 +199,14 @@ struct kcfd_mat {
     // MIXED: {column, FP32 value bits} per chunk entry (one 8-byte load)
     int2 *chl_pk32 = nullptr, *chu_pk32 = nullptr;
     float *ud32 = nullptr;
+    // MIXED, narrow-index form of the two-colour layouts (instead of the
+    // packed pairs when every chunk's columns span at most kSellSpan):
+    // 16-bit column offsets from each chunk's smallest column (structure,
+    // padding as kSellDiag), the bases, and FP32 values
+    bool ch16 = false;
+    unsigned short *chl_o16 = nullptr, *chu_o16 = nullptr;
+    int *chl_base = nullptr, *chu_base = nullptr;
+    float *chl_v32 = nullptr, *chu_v32 = nullptr;
     // cached single-system solver workspaces (BiCGSTAB / CG batch, GMRES)
     kcfd_batch *solver = nullptr;
     kcfd::GmresWs *gmres = nullptr;

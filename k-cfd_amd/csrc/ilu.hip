@@ -18,6 +18,8 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <climits>
+#include <cstdlib>
 
 namespace kcfd {
 
@@ -275,6 +277,37 @@ static int upload_ints(const std::vector<int> &h, int **d, hipStream_t s) {
     return KCFD_OK;
 }
 
+template <typename T>
+static int upload_vec(const std::vector<T> &h, T **d, hipStream_t s) {
+    KCFD_HIP(hipMalloc(d, sizeof(T) * std::max<size_t>(h.size(), 1)));
+    if (!h.empty()) KCFD_HIP(hipMemcpyAsync(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    return KCFD_OK;
+}
+
+// Narrow-index form of a chunk layout: per chunk its smallest column as the
+// base and each entry's column minus it, padding (column -1) as kSellDiag.
+// False when a chunk's columns span more than kSellSpan.
+static bool chunk_offsets16(const std::vector<int> &off, const std::vector<int> &col,
+                            std::vector<unsigned short> &o16, std::vector<int> &base) {
+    const size_t nch = off.size() - 1;
+    o16.assign(col.size(), (unsigned short)kSellDiag);
+    base.assign(nch, 0);
+    for (size_t c = 0; c < nch; ++c) {
+        int lo = INT_MAX, hi = INT_MIN;
+        for (int e = off[c]; e < off[c + 1]; ++e)
+            if (col[e] >= 0) {
+                lo = std::min(lo, col[e]);
+                hi = std::max(hi, col[e]);
+            }
+        if (lo == INT_MAX) continue;
+        if (hi - lo > kSellSpan) return false;
+        base[c] = lo;
+        for (int e = off[c]; e < off[c + 1]; ++e)
+            if (col[e] >= 0) o16[e] = (unsigned short)(col[e] - lo);
+    }
+    return true;
+}
+
 // chunk-layout values: FP64, and (MIXED) the packed {column, FP32 value}
 // entries; with col null the FP32 copy goes to v32 (u_ii by row)
 __global__ void k_chunk_vals(size_t ne, const int *__restrict__ map, const int *__restrict__ col,
@@ -328,10 +361,16 @@ int ilu0_structure(kcfd_mat *m) {
         const std::vector<int> lvl_l = m->two_colour ? std::vector<int>{0, n - m->ncolour0} : m->h_lvl_ptr_l;
         const std::vector<int> lvl_u = m->two_colour ? std::vector<int>{0, m->ncolour0} : m->h_lvl_ptr_u;
         {
-            std::vector<int> off, col, map;
+            // the two-colour layouts also get their narrow-index form (for
+            // the mixed solver) when both fit (KCFD_SELL16=0: never)
+            const char *env = std::getenv("KCFD_SELL16");
+            bool narrow = m->two_colour && !(env && env[0] == '0');
+            std::vector<int> off, col, map, lbase, ubase;
+            std::vector<unsigned short> lo16, uo16;
             chunk_layout(m, rl, lvl_l, false, m->h_chl_ptr, off, col, map);
             m->nch_l = (int)off.size() - 1;
             m->nent_l = (int)col.size();
+            if (narrow) narrow = chunk_offsets16(off, col, lo16, lbase);
             KCFD_CHECK(upload_ints(off, &m->chl_off, s));
             KCFD_CHECK(upload_ints(col, &m->chl_col, s));
             KCFD_CHECK(upload_ints(map, &m->chl_map, s));
@@ -339,9 +378,17 @@ int ilu0_structure(kcfd_mat *m) {
             chunk_layout(m, ru, lvl_u, true, m->h_chu_ptr, off, col, map);
             m->nch_u = (int)off.size() - 1;
             m->nent_u = (int)col.size();
+            if (narrow) narrow = chunk_offsets16(off, col, uo16, ubase);
             KCFD_CHECK(upload_ints(off, &m->chu_off, s));
             KCFD_CHECK(upload_ints(col, &m->chu_col, s));
             KCFD_CHECK(upload_ints(map, &m->chu_map, s));
+            if (narrow) {
+                KCFD_CHECK(upload_vec(lo16, &m->chl_o16, s));
+                KCFD_CHECK(upload_vec(uo16, &m->chu_o16, s));
+                KCFD_CHECK(upload_vec(lbase, &m->chl_base, s));
+                KCFD_CHECK(upload_vec(ubase, &m->chu_base, s));
+            }
+            m->ch16 = narrow;
             KCFD_HIP(hipStreamSynchronize(s));   // the host vectors are locals
         }
     }
@@ -383,18 +430,25 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
             KCFD_HIP(hipMalloc(&m->chu_v64, sizeof(double) * std::max<size_t>(nu, 1)));
             KCFD_HIP(hipMalloc(&m->ud64, sizeof(double) * std::max(n, 1)));
         }
-        if (want32 && !m->chl_pk32) {
-            KCFD_HIP(hipMalloc(&m->chl_pk32, sizeof(int2) * std::max<size_t>(nl, 1)));
-            KCFD_HIP(hipMalloc(&m->chu_pk32, sizeof(int2) * std::max<size_t>(nu, 1)));
+        const bool narrow = want32 && m->ch16;   // FP32 values beside the 16-bit offsets
+        if (want32 && !m->ud32) {
+            if (narrow) {
+                KCFD_HIP(hipMalloc(&m->chl_v32, sizeof(float) * std::max<size_t>(nl, 1)));
+                KCFD_HIP(hipMalloc(&m->chu_v32, sizeof(float) * std::max<size_t>(nu, 1)));
+            } else {
+                KCFD_HIP(hipMalloc(&m->chl_pk32, sizeof(int2) * std::max<size_t>(nl, 1)));
+                KCFD_HIP(hipMalloc(&m->chu_pk32, sizeof(int2) * std::max<size_t>(nu, 1)));
+            }
             KCFD_HIP(hipMalloc(&m->ud32, sizeof(float) * std::max(n, 1)));
         }
-        int2 *l32 = want32 ? m->chl_pk32 : nullptr, *u32 = want32 ? m->chu_pk32 : nullptr;
+        int2 *l32 = want32 && !narrow ? m->chl_pk32 : nullptr, *u32 = want32 && !narrow ? m->chu_pk32 : nullptr;
+        float *lv32 = narrow ? m->chl_v32 : nullptr, *uv32 = narrow ? m->chu_v32 : nullptr;
         if (nl)
             hipLaunchKernelGGL(k_chunk_vals, dim3((unsigned)((nl + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                               nl, m->chl_map, m->chl_col, m->lu, m->chl_v64, l32, nullptr);
+                               nl, m->chl_map, m->chl_col, m->lu, m->chl_v64, l32, lv32);
         if (nu)
             hipLaunchKernelGGL(k_chunk_vals, dim3((unsigned)((nu + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                               nu, m->chu_map, m->chu_col, m->lu, m->chu_v64, u32, nullptr);
+                               nu, m->chu_map, m->chu_col, m->lu, m->chu_v64, u32, uv32);
         if (n)
             hipLaunchKernelGGL(k_chunk_vals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (size_t)n,
                                m->f_diagpos, nullptr, m->lu, m->ud64, nullptr, want32 ? m->ud32 : nullptr);
@@ -592,6 +646,19 @@ extern "C" int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src) {
     KCFD_CHECK(dup_ints(src->chl_map, (size_t)src->nent_l, &dst->chl_map, dst->ctx->stream));
     dst->two_colour = src->two_colour;
     dst->ncolour0 = src->ncolour0;
+    if (src->ch16) {
+        KCFD_HIP(hipMalloc(&dst->chl_o16, sizeof(unsigned short) * std::max(src->nent_l, 1)));
+        KCFD_HIP(hipMalloc(&dst->chu_o16, sizeof(unsigned short) * std::max(src->nent_u, 1)));
+        if (src->nent_l)
+            KCFD_HIP(hipMemcpyAsync(dst->chl_o16, src->chl_o16, sizeof(unsigned short) * src->nent_l,
+                                    hipMemcpyDeviceToDevice, dst->ctx->stream));
+        if (src->nent_u)
+            KCFD_HIP(hipMemcpyAsync(dst->chu_o16, src->chu_o16, sizeof(unsigned short) * src->nent_u,
+                                    hipMemcpyDeviceToDevice, dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->chl_base, (size_t)src->nch_l, &dst->chl_base, dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->chu_base, (size_t)src->nch_u, &dst->chu_base, dst->ctx->stream));
+    }
+    dst->ch16 = src->ch16;
     KCFD_CHECK(dup_ints(src->chu_map, (size_t)src->nent_u, &dst->chu_map, dst->ctx->stream));
     dst->h_chl_ptr = src->h_chl_ptr;
     dst->h_chu_ptr = src->h_chu_ptr;

@@ -279,6 +279,8 @@ int mat_free_precond(kcfd_mat *m) {
     hipFree(m->ud64); m->ud64 = nullptr;
     hipFree(m->chl_pk32); m->chl_pk32 = nullptr;
     hipFree(m->chu_pk32); m->chu_pk32 = nullptr;
+    hipFree(m->chl_v32); m->chl_v32 = nullptr;
+    hipFree(m->chu_v32); m->chu_v32 = nullptr;
     hipFree(m->ud32); m->ud32 = nullptr;
     m->have32 = false;
     m->pc_kind = KCFD_PC_NONE;
@@ -304,7 +306,14 @@ int mat_free_ilu_structure(kcfd_mat *m) {
     }
     hipFree(m->chl_pk32); m->chl_pk32 = nullptr;
     hipFree(m->chu_pk32); m->chu_pk32 = nullptr;
+    hipFree(m->chl_v32); m->chl_v32 = nullptr;
+    hipFree(m->chu_v32); m->chu_v32 = nullptr;
     hipFree(m->ud32); m->ud32 = nullptr;
+    hipFree(m->chl_o16); m->chl_o16 = nullptr;
+    hipFree(m->chu_o16); m->chu_o16 = nullptr;
+    hipFree(m->chl_base); m->chl_base = nullptr;
+    hipFree(m->chu_base); m->chu_base = nullptr;
+    m->ch16 = false;
     ilu_free_order(m);
     m->h_lvl_ptr_l.clear();
     m->h_lvl_ptr_u.clear();
