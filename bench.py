@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE counter passes: the narrow-index FP32 SELL copy
 # (6 bytes an entry, the library's default) and the packed one (8 bytes an
 # entry, KCFD_SELL16=0)
-TRAFFIC_SPMV_P_MIXED_HEX128 = 115.1e6          # 106.6 MB read + 8.5 MB written
+TRAFFIC_SPMV_P_MIXED_HEX128 = 115.2e6          # 106.7 MB read + 8.5 MB written
 TRAFFIC_SPMV_P_MIXED_HEX128_PACKED = 153.9e6   # 145.4 MB read + 8.5 MB written
 
 
