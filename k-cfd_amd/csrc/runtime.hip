@@ -241,13 +241,17 @@ int sell_mixed_alloc(kcfd_mat *m) {
         int wide = 0;
         KCFD_HIP(hipMalloc(&m->sbase, sizeof(int) * m->nslices));
         KCFD_HIP(hipMalloc(&wide_d, sizeof(int)));
-        KCFD_HIP(hipMemcpyAsync(wide_d, &wide, sizeof(int), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_sell_base, dim3((m->n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, m->n,
-                           m->slice_ptr, m->six, m->sbase, wide_d);
-        KCFD_HIP(hipGetLastError());
-        KCFD_HIP(hipMemcpyAsync(&wide, wide_d, sizeof(int), hipMemcpyDeviceToHost, st));
-        KCFD_HIP(hipStreamSynchronize(st));
+        // every step runs and the flag buffer is freed before an error returns
+        hipError_t e = hipMemcpyAsync(wide_d, &wide, sizeof(int), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(k_sell_base, dim3((m->n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, m->n,
+                               m->slice_ptr, m->six, m->sbase, wide_d);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(&wide, wide_d, sizeof(int), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         hipFree(wide_d);
+        KCFD_HIP(e);
         narrow = wide == 0;
         if (!narrow) {
             hipFree(m->sbase);
