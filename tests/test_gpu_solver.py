@@ -185,12 +185,15 @@ def test_mixed_precision_refinement(ctx, pc, size):
 
 
 def test_batch_equals_alone(ctx):
-    """Each system of a batch gives bit-identical results alone and in the mix."""
+    """Each system of a batch gives bit-identical results alone and in the mix.
+    The last system's SELL slices span more than 16-bit column offsets reach,
+    so in mixed precision the batch holds both FP32 layouts (narrow-index and
+    packed) at once."""
     device, workloads = _kc()
     from kcfd_amd._lib import DevBuf
     cases = [workloads.quad_convdiff(20, seed=1), workloads.hex_pressure(10, seed=2),
              workloads.quad_convdiff(33, seed=3), workloads.hex_pressure(7, seed=4)]
-    systems = [oracle_system(c)[:2] for c in cases]
+    systems = [oracle_system(c)[:2] for c in cases] + [_wide_span_system()]
     for prec in ["fp64", "mixed"]:
         alone = []
         for A, b in systems:
