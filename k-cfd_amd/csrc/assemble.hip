@@ -867,6 +867,7 @@ int kcfd_mesh_new_system(kcfd_mesh *m, kcfd_mat **out) {
 int kcfd_asm_reset(kcfd_mat *a) {
     KCFD_ARG(a != nullptr, "null matrix");
     std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    ++a->val_gen;
     hipStream_t s = a->ctx->stream;
     if (a->nnz) hipLaunchKernelGGL(k_zero, dim3(blocks(std::min(a->nnz, 1 << 22))), dim3(kBlock), 0, s, a->val, (size_t)a->nnz);
     if (a->n) {
@@ -887,6 +888,7 @@ int kcfd_asm_diffusion(kcfd_mesh *mesh, kcfd_mat *a, const double *gamma) {
     CHECK_PAIR(mesh, a);
     KCFD_ARG(gamma != nullptr, "null gamma");
     std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    ++a->val_gen;
     hipStream_t s = a->ctx->stream;
     Timer t(a->ctx);
     MeshView v = view(mesh);
@@ -899,6 +901,7 @@ int kcfd_asm_upwind(kcfd_mesh *mesh, kcfd_mat *a, const double *F) {
     CHECK_PAIR(mesh, a);
     KCFD_ARG(F != nullptr, "null flux");
     std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    ++a->val_gen;
     hipStream_t s = a->ctx->stream;
     Timer t(a->ctx);
     MeshView v = view(mesh);
@@ -914,6 +917,7 @@ int kcfd_asm_boundary(kcfd_mesh *mesh, kcfd_mat *a, const double *gamma, const d
     for (int p = 0; p < mesh->npatch; ++p)
         KCFD_ARG(kinds[p] >= 0 && kinds[p] <= 3, "unknown boundary-condition kind");
     std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    ++a->val_gen;
     hipStream_t s = a->ctx->stream;
     if (mesh->npatch) {
         std::vector<double> vv(mesh->npatch, 0.0);
@@ -931,6 +935,7 @@ int kcfd_asm_boundary(kcfd_mesh *mesh, kcfd_mat *a, const double *gamma, const d
 static int cell_terms(kcfd_mesh *mesh, kcfd_mat *a, int mode, const double *a1, const double *a2,
                       double s1, double s2) {
     std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    if (mode != 1) ++a->val_gen;   // every term but the source writes the diagonal
     hipStream_t s = a->ctx->stream;
     Timer t(a->ctx);
     hipLaunchKernelGGL(k_cell_terms, dim3(blocks(a->n)), dim3(kBlock), 0, s, a->n, mode,
@@ -975,6 +980,7 @@ int kcfd_asm_pressure(kcfd_mesh *mesh, kcfd_mat *a, const double *dP, double rho
     CHECK_PAIR(mesh, a);
     KCFD_ARG(dP != nullptr, "null dP");
     std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    ++a->val_gen;
     hipStream_t s = a->ctx->stream;
     if (mesh->npatch) {
         std::vector<int> pd(mesh->npatch, 0);
@@ -1023,6 +1029,7 @@ int kcfd_simple_fix_reference(kcfd_mat *a) {
     std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
     const int len = a->h_indptr[1] - a->h_indptr[0];
     KCFD_ARG(len <= 1024, "row 0 longer than 1024 entries");
+    ++a->val_gen;
     Timer t(a->ctx);
     hipLaunchKernelGGL(k_simple_fix_row0, dim3(1), dim3(1024), 0, a->ctx->stream, a->indptr, a->diagpos, a->val,
                        a->rhs);
@@ -1053,6 +1060,7 @@ int kcfd_twofluid_pressure(kcfd_mesh *mesh, kcfd_mat *a, const double *alpha_l, 
     KCFD_ARG(alpha_l && alpha_g && aP_l && aP_g && mf_l && mf_g, "null argument");
     KCFD_ARG(a->n > 0 && !a->h_diagpos.empty() && a->h_diagpos[0] >= 0, "row 0 has no diagonal entry");
     std::lock_guard<std::recursive_mutex> g(a->ctx->mu);
+    ++a->val_gen;
     hipStream_t s = a->ctx->stream;
     Timer t(a->ctx);
     MeshView v = view(mesh);

@@ -47,6 +47,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace kcfd {
@@ -119,8 +120,15 @@ struct SysState {
 struct BatchParams {
     double rtol, atol, delta, rhotol;
     int precision, precond, max_refine, nsys;
-    int inner_maxiter, method, pad[2];
+    int inner_maxiter, method;
+    int poll_limit;   // sync-free triangular solves: polls of one operand before giving up
+    int pad;
 };
+
+// counters[kTrsvFail] != 0: a sync-free triangular solve gave up waiting for
+// an operand (the solve's result is invalid; the host reports KCFD_ESTATE)
+constexpr int kTrsvFail = 6;
+constexpr int kDefaultPollLimit = 1 << 20;
 
 // ILU trisolve work chunk: up to 64 consecutive rows of one system's level list
 struct Chunk {
@@ -1267,8 +1275,13 @@ __device__ __forceinline__ void publish(V *p, V v) {
 // 8 entries: their indices and values, then their (polled) operands are all
 // requested before the first subtraction, so a row costs about one memory
 // round trip per group instead of one per entry.
+// The wait for an operand is bounded: after `limit` polls that still find
+// the sentinel the lane stops waiting, raises the fail word (the host turns
+// it into KCFD_ESTATE) and uses a quiet NaN, so that its own row is still
+// published and every wave of the launch runs to its end.
 template <typename V>
-__device__ __forceinline__ V sub_ready(V acc, const FacRef &f, int base, int w, const V *vec) {
+__device__ __forceinline__ V sub_ready(V acc, const FacRef &f, int base, int w, const V *vec, int limit,
+                                       unsigned int *fail) {
 #pragma clang fp contract(off)
     using U = typename Sentinel<V>::U;
     constexpr int G = 8;
@@ -1285,7 +1298,13 @@ __device__ __forceinline__ V sub_ready(V acc, const FacRef &f, int base, int w, 
 #pragma unroll
         for (int k = 0; k < G; ++k)
             if (kb + k < w && c[k] >= 0) {
+                int polls = 0;
                 while (bits[k] == Sentinel<V>::bits) {
+                    if (polls++ >= limit) {
+                        __hip_atomic_store(fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        bits[k] = __builtin_bit_cast(U, (V)NAN);
+                        break;
+                    }
                     __builtin_amdgcn_s_sleep(2);
                     bits[k] = __hip_atomic_load(reinterpret_cast<const U *>(vec + c[k]), __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
@@ -1301,9 +1320,13 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
                                                  const SysState *__restrict__ sts,
                                                  const Chunk *__restrict__ chunks,
                                                  const int *__restrict__ counts,
-                                                 const int *const *__restrict__ lvl_rows) {
+                                                 const int *const *__restrict__ lvl_rows,
+                                                 const BatchParams *__restrict__ bpp,
+                                                 unsigned int *__restrict__ counters) {
 #pragma clang fp contract(off)
     const int lane = threadIdx.x & 63;
+    const int limit = bpp->poll_limit;
+    unsigned int *fail = counters + kTrsvFail;
     // The chunk loop is wave-uniform (the chunk index depends on the wave
     // only); per-lane work sits inside an if.
     const int nchunks = counts[UPPER ? 2 : 1];
@@ -1328,12 +1351,12 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
             if (!UPPER) {
                 const V *z = (const V *)(WHICH == 0 ? d.p : (WHICH == 1 ? d.s : d.r));
                 V *y = (V *)d.y;
-                const V acc = sub_ready<V>(gl(z)[i], fac_l(d), o0 + lane, w, y);
+                const V acc = sub_ready<V>(gl(z)[i], fac_l(d), o0 + lane, w, y, limit, fail);
                 publish<V>(y + i, acc);
             } else {
                 V *y = (V *)d.y;
                 V *wv = (V *)(WHICH == 1 ? d.sh : d.ph);
-                const V acc = sub_ready<V>(glw(y)[i], fac_u(d), o0 + lane, w, wv);
+                const V acc = sub_ready<V>(glw(y)[i], fac_u(d), o0 + lane, w, wv, limit, fail);
                 publish<V>(wv + i, acc / gl(udiag<V>(d))[i]);
                 glw(y)[i] = sentinel_value<V>();
             }
@@ -1519,9 +1542,9 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
         }
         if (b->run_polled) {
             launch_k(b, 3, !b->run_two, true, k_trsv<V, false, 0>, tg, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_l, b->d_counts, b->d_lvl_l);
+                     b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
             launch_k(b, 0, false, false, k_trsv<V, true, 0>, tg, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_u, b->d_counts, b->d_lvl_u);
+                     b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
         }
     }
     launch_k(b, 1, true, true, k_spmv_p<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
@@ -1539,9 +1562,9 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
         }
         if (b->run_polled) {
             launch_k(b, 0, false, false, k_trsv<V, false, 1>, tg, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_l, b->d_counts, b->d_lvl_l);
+                     b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
             launch_k(b, 0, false, false, k_trsv<V, true, 1>, tg, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_u, b->d_counts, b->d_lvl_u);
+                     b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
         }
     }
     launch_k(b, 0, false, false, k_spmv_s<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles,
@@ -1567,9 +1590,9 @@ void launch_iteration_cg(kcfd_batch *b, hipStream_t s) {
         }
         if (b->run_polled) {
             launch_k(b, 3, !b->run_two, false, k_trsv<V, false, 2>, tg, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_l, b->d_counts, b->d_lvl_l);
+                     b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
             launch_k(b, 3, false, true, k_trsv<V, true, 2>, tg, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_u, b->d_counts, b->d_lvl_u);
+                     b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
         }
     }
     launch_k(b, 0, false, false, k_cg_rz<PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
@@ -1750,6 +1773,11 @@ int check_mat(const kcfd_batch *b, const kcfd_mat *m, int k) {
         set_error("matrix %d belongs to another context", k);
         return KCFD_EARG;
     }
+    {
+        char what[48];
+        snprintf(what, sizeof(what), "matrix %d", k);
+        KCFD_CHECK(check_current(m, what));
+    }
     if (m->pc_kind != b->bp.precond) {
         set_error("matrix %d: preconditioner not set up as requested (have %d, want %d)", k,
                   m->pc_kind, b->bp.precond);
@@ -1782,6 +1810,11 @@ int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **o
     b->bp.nsys = slots;
     b->bp.inner_maxiter = 5000;
     b->bp.method = o->method;
+    // polls of one operand in the sync-free triangular solves before they
+    // give up (KCFD_TRSV_POLL_LIMIT overrides; at ~1 us a poll the default is
+    // about a second, against ~1 ms for a whole 382-level solve)
+    b->bp.poll_limit = kDefaultPollLimit;
+    if (const char *e = std::getenv("KCFD_TRSV_POLL_LIMIT")) b->bp.poll_limit = std::max(0, std::atoi(e));
     b->mats.assign(slots, nullptr);
     b->h_desc.assign(slots, SysDesc{});
     b->sys_n.assign(slots, 0);
@@ -2067,6 +2100,41 @@ int capture_graph(kcfd_batch *b, int iters) {
     return KCFD_OK;
 }
 
+// The iterations between host checks run as one captured graph, or, with
+// KCFD_BATCH_GRAPH=0 in the environment, as direct launches of the same
+// kernels (same arguments: everything they read lives in device memory).
+// The direct mode is for tools that trace kernels: rocprofv3's kernel trace
+// has aborted the process inside hipGraphLaunch of the batch's graph.
+bool batch_graph_enabled() {
+    const char *e = std::getenv("KCFD_BATCH_GRAPH");
+    return !(e && e[0] == '0');
+}
+
+int replay_iterations(kcfd_batch *b, hipStream_t s) {
+    if (!batch_graph_enabled()) {
+        LaunchFn fn = pick_iter(b->bp.precision, b->bp.precond, b->bp.method, b->rows);
+        for (int k = 0; k < b->opts.check_every; ++k) fn(b, s);
+        KCFD_HIP(hipGetLastError());
+        b->graph_iters = b->opts.check_every;
+        return KCFD_OK;
+    }
+    if (!b->graph || b->graph_iters != b->opts.check_every)
+        KCFD_CHECK(capture_graph(b, b->opts.check_every));
+    KCFD_HIP(hipGraphLaunch(b->graph, s));
+    return KCFD_OK;
+}
+
+// the fail word of the sync-free triangular solves (read after the state
+// copy that the caller has synchronised)
+int check_trsv(const kcfd_batch *b, unsigned int fail) {
+    if (fail) {
+        set_error("ILU(0) sync-free triangular solve: a wait for an operand exceeded %d polls; "
+                  "the solve's result is invalid", b->bp.poll_limit);
+        return KCFD_ESTATE;
+    }
+    return KCFD_OK;
+}
+
 int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     kcfd_ctx *ctx = b->ctx;
     hipStream_t s = ctx->stream;
@@ -2080,11 +2148,13 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     KCFD_HIP(hipMemcpyAsync(b->d_state, b->h_state.data(), sizeof(SysState) * b->nsys,
                             hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
-    if (!b->graph || b->graph_iters != b->opts.check_every)
+    if (batch_graph_enabled() && (!b->graph || b->graph_iters != b->opts.check_every))
         KCFD_CHECK(capture_graph(b, b->opts.check_every));
+    KCFD_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * 8, s));
     KCFD_HIP(hipEventRecord(ctx->ev0, s));
     pick_init(b->bp.precision, b->bp.precond, b->rows)(b, s);
     KCFD_HIP(hipGetLastError());
+    unsigned int fail = 0;
     long long guard = 0;
     int maxit = 0;
     for (int k = 0; k < b->nsys; ++k) maxit = std::max(maxit, b->h_state[k].maxiter);
@@ -2092,11 +2162,15 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     // one reliable update
     const long long limit = 2LL * maxit + 2LL * b->bp.max_refine + 64;
     for (;;) {
-        KCFD_HIP(hipGraphLaunch(b->graph, s));
+        KCFD_CHECK(replay_iterations(b, s));
         guard += b->graph_iters;
         KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
                                 hipMemcpyDeviceToHost, s));
+        if (b->run_polled)
+            KCFD_HIP(hipMemcpyAsync(&fail, b->d_counters + kTrsvFail, sizeof(unsigned int),
+                                    hipMemcpyDeviceToHost, s));
         KCFD_HIP(hipStreamSynchronize(s));
+        KCFD_CHECK(check_trsv(b, fail));
         bool all = true;
         for (int k = 0; k < b->nsys; ++k)
             if (b->h_state[k].status != S_DONE) { all = false; break; }
@@ -2261,15 +2335,19 @@ int kcfd_batch_step(kcfd_batch *b, int *finished, int *nfinished) {
     }
     if (!any) return KCFD_OK;
     KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
-    if (!b->graph || b->graph_iters != b->opts.check_every)
+    if (batch_graph_enabled() && (!b->graph || b->graph_iters != b->opts.check_every))
         KCFD_CHECK(capture_graph(b, b->opts.check_every));
     if (any_new) pick_init(b->bp.precision, b->bp.precond, b->rows)(b, s);
-    KCFD_HIP(hipGraphLaunch(b->graph, s));
+    KCFD_CHECK(replay_iterations(b, s));
     pick_finish(b->rows)(b, s);
     KCFD_HIP(hipGetLastError());
     KCFD_HIP(hipMemcpyAsync(b->h_state.data(), b->d_state, sizeof(SysState) * b->nsys,
                             hipMemcpyDeviceToHost, s));
+    unsigned int fail = 0;
+    if (b->run_polled)
+        KCFD_HIP(hipMemcpyAsync(&fail, b->d_counters + kTrsvFail, sizeof(unsigned int), hipMemcpyDeviceToHost, s));
     KCFD_HIP(hipStreamSynchronize(s));
+    KCFD_CHECK(check_trsv(b, fail));
     for (int k = 0; k < b->nsys; ++k) {
         b->fresh[k] = 0;
         if (b->mats[k] && !b->reported[k] && b->h_state[k].status == S_DONE && b->h_state[k].true_done) {
@@ -2457,6 +2535,7 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
         if (res) std::memset(res, 0, sizeof(*res));
         return KCFD_OK;
     }
+    KCFD_CHECK(check_current(m, "kcfd_solve"));
     if (opts->method == KCFD_METHOD_GMRES) return gmres_solve(m, bvec, x, opts, res);
     // the cached single-system workspace is reused while the preconditioner,
     // precision, launch shape and the b / x buffers are unchanged

@@ -207,6 +207,11 @@ struct kcfd_mat {
     unsigned short *chl_o16 = nullptr, *chu_o16 = nullptr;
     int *chl_base = nullptr, *chu_base = nullptr;
     float *chl_v32 = nullptr, *chu_v32 = nullptr;
+    // Values generation: bumped by every entry point that writes the values
+    // (kcfd_mat_set_values, the assembly calls); kcfd_precond_setup records
+    // it.  The solvers read the SELL copy and the preconditioner that the
+    // set-up built, so they refuse a matrix whose values changed since.
+    unsigned long long val_gen = 1, setup_gen = 0;
     // cached single-system solver workspaces (BiCGSTAB / CG batch, GMRES)
     kcfd_batch *solver = nullptr;
     kcfd::GmresWs *gmres = nullptr;
@@ -214,6 +219,9 @@ struct kcfd_mat {
 };
 
 namespace kcfd {
+// KCFD_ESTATE unless the preconditioner (and the SELL copy) was set up from
+// the current values; `what` names the matrix in the message
+int check_current(const kcfd_mat *m, const char *what);
 int ensure_scratch(kcfd_ctx *ctx, size_t bytes);
 int ilu0_structure(kcfd_mat *m);
 int ilu0_setup(kcfd_mat *m, bool want32);

@@ -18,6 +18,15 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
 }
 
+int check_current(const kcfd_mat *m, const char *what) {
+    if (m->setup_gen != m->val_gen) {
+        set_error("%s: the values changed after the last kcfd_precond_setup (or it was never called); "
+                  "set the preconditioner up again before solving", what);
+        return KCFD_ESTATE;
+    }
+    return KCFD_OK;
+}
+
 int ensure_scratch(kcfd_ctx *ctx, size_t bytes) {
     if (ctx->scratch_bytes >= bytes) return KCFD_OK;
     if (ctx->scratch) KCFD_HIP(hipFree(ctx->scratch));
@@ -558,6 +567,7 @@ int kcfd_mat_set_values(kcfd_mat *m, const double *vals) {
     KCFD_ARG(m && (m->nnz == 0 || vals), "null argument");
     std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
     if (!m->nnz) return KCFD_OK;
+    ++m->val_gen;
     KCFD_HIP(hipMemcpyAsync(m->val, vals, sizeof(double) * m->nnz, hipMemcpyHostToDevice,
                             m->ctx->stream));
     KCFD_HIP(hipStreamSynchronize(m->ctx->stream));
@@ -619,6 +629,17 @@ int kcfd_precond_setup(kcfd_mat *m, int kind, int precision) {
     hipStream_t s = m->ctx->stream;
     const int n = m->n;
     const bool want32 = precision == KCFD_MIXED;
+    if (want32) {
+        // the FP32 copy holds each row's sum at its diagonal entry
+        // (difference form): a row without a structural diagonal has no
+        // place for it
+        for (int i = 0; i < n; ++i)
+            if (m->h_diagpos[i] < 0) {
+                set_error("mixed precision needs a diagonal entry in every row (row %d has none in the "
+                          "sparsity pattern); use KCFD_FP64", i);
+                return KCFD_EARG;
+            }
+    }
     // Same kind and precision as last time: only the values changed, so the
     // buffers (and any solver workspace that points at them) are reused.
     const bool same = m->pc_kind == kind && m->have32 == want32 &&
@@ -658,6 +679,7 @@ int kcfd_precond_setup(kcfd_mat *m, int kind, int precision) {
     m->pc_kind = kind;
     m->have32 = want32;
     KCFD_HIP(hipStreamSynchronize(s));
+    m->setup_gen = m->val_gen;
     return KCFD_OK;
 }
 

@@ -340,7 +340,61 @@ def least_squares_gradient(mesh, values, bvals: dict):
 def pressure_correction_operator(mesh, dP, rho, dirichlet, s):
     """models/single_phase.py:288-325 coefficients (COO order of the
     reference: off-diagonals face by face, then one diagonal entry per cell);
-    no right-hand side (see simple_pressure_correction_step)."""
+    no right-hand side (see simple_pressure_correction_step).  Internal faces
+    with d_PN >= 1e-30: df = rho 0.5 (dP_O + dP_N) A / max(|d.n|, 0.1 d),
+    a_P(O) += df, a_P(N) += df, entries (O,N,-df), (N,O,-df); faces of a
+    Dirichlet patch with d_Pf > 1e-30: a_P(O) += rho dP_O A / d_Pf.  The a_P
+    sums are accumulated in face order, as the reference's loop does."""
+    o = mesh.owner.astype(np.int64)
+    nb = mesh.neighbour.astype(np.int64)
+    n = mesh.n_cells
+    nf = mesh.n_faces
+    dP = np.asarray(dP, np.float64)
+    cc = mesh.cell_center
+    internal = nb >= 0
+    fi = np.nonzero(internal)[0]
+    oi, ni = o[fi], nb[fi]
+    dv = cc[ni] - cc[oi]
+    d = _norm(dv)
+    keep = ~(d < 1e-30)
+    fi, oi, ni, dv, d = fi[keep], oi[keep], ni[keep], dv[keep], d[keep]
+    nrm = mesh.face_normal[fi]
+    dorth = np.abs(dv[:, 0] * nrm[:, 0] + dv[:, 1] * nrm[:, 1] + dv[:, 2] * nrm[:, 2])
+    dorth = np.maximum(dorth, 0.1 * d)
+    df = rho * 0.5 * (dP[oi] + dP[ni]) * mesh.face_area[fi] / dorth
+    # Dirichlet boundary faces
+    isdir = np.zeros(nf, bool)
+    for nm, f in mesh.boundary_patches.items():
+        if nm in set(dirichlet):
+            isdir[np.asarray(f, np.int64)] = True
+    isdir &= ~internal
+    fb = np.nonzero(isdir)[0]
+    ob = o[fb]
+    db = _norm(mesh.face_center[fb] - cc[ob])
+    okb = db > 1e-30
+    fb, ob, db = fb[okb], ob[okb], db[okb]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ab = rho * dP[ob] * mesh.face_area[fb] / db
+    # a_P contributions in face order: (O, df), (N, df) per internal face,
+    # (O, ab) per Dirichlet face
+    key = np.concatenate([2 * fi, 2 * fi + 1, 2 * fb])
+    cell = np.concatenate([oi, ni, ob])
+    val = np.concatenate([df, df, ab])
+    order = np.argsort(key, kind="stable")
+    aP = np.zeros(n)
+    np.add.at(aP, cell[order], val[order])
+    s.add_entries(_interleave(oi, ni), _interleave(ni, oi), _interleave(-df, -df))
+    idx = np.arange(n)
+    s.add_entries(idx, idx, aP)
+
+
+def pressure_correction_operator_loop(mesh, dP, rho, dirichlet, s):
+    """The face loop of models/single_phase.py:288-325 as written (one face
+    at a time); pins the vectorised pressure_correction_operator.  The
+    reference's np.dot(d, n) of two 3-vectors is a BLAS call whose rounding
+    depends on the machine (fused multiply-adds on some); it is written here
+    as the unfused sum d_x n_x + d_y n_y + d_z n_z, the same on every
+    machine (and what the device computes)."""
     o = mesh.owner.astype(np.int64)
     nb = mesh.neighbour.astype(np.int64)
     n = mesh.n_cells
@@ -358,7 +412,8 @@ def pressure_correction_operator(mesh, dP, rho, dirichlet, s):
             if d < 1e-30:
                 continue
             dv = cc[nb[f]] - cc[o[f]]
-            dorth = abs(np.dot(dv, mesh.face_normal[f]))
+            nf_ = mesh.face_normal[f]
+            dorth = abs(dv[0] * nf_[0] + dv[1] * nf_[1] + dv[2] * nf_[2])
             dorth = max(dorth, 0.1 * d)
             df = rho * 0.5 * (dP[o[f]] + dP[nb[f]]) * mesh.face_area[f] / dorth
             aP[o[f]] += df

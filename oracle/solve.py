@@ -364,6 +364,124 @@ def gmres_restated(A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=None, precond="no
     return x, (0 if rnorm <= atol else maxiter), inner, cycles
 
 
+def _psolve_of(A, precond, M):
+    Mop = make_preconditioner(A, precond) if M is None else M
+    return (lambda v: v.copy()) if Mop is None else Mop.matvec
+
+
+def bicgstab_restated(A, b, x0=None, rtol=1e-5, atol=0.0, maxiter=None, precond="none", M=None,
+                      dot=None):
+    """scipy 1.15.3 ``bicgstab`` (scipy/sparse/linalg/_isolve/iterative.py:
+    157-302, the solver ``solve_linear_system`` calls at
+    core/linear_solver.py:59) written out step by step, with ``dot(a, b)``
+    in place of numpy's dot product.  Every inner product and every norm
+    (``np.linalg.norm(v)`` is ``sqrt(v.dot(v))`` for a real vector) goes
+    through ``dot``; everything else is scipy's arithmetic in scipy's
+    order: r = b - A x (b itself when x0 = 0), the stop test on ||r|| at the
+    top of each iteration, the rho and omega breakdown tests against eps^2,
+    beta = (rho / rho_prev) (alpha / omega), p -= omega v; p *= beta;
+    p += r, the rv == 0 test, the half-step exit on ||s||, omega = t.s / t.t,
+    x += alpha M p; x += omega M s; r -= omega t.  With ``dot=np.dot`` it is
+    scipy bit for bit.  Returns (x, info, iterations, half_step):
+    iterations = completed full iterations (scipy's callback count),
+    half_step = 1 when it stopped at the test on ||s||."""
+    dot = np.dot if dot is None else dot
+    A = sparse.csr_matrix(A)
+    n = A.shape[0]
+    psolve = _psolve_of(A, precond, M)
+    b = np.asarray(b, np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    bnrm2 = np.sqrt(dot(b, b))
+    atol = max(float(atol), float(rtol) * float(bnrm2))
+    if bnrm2 == 0:
+        return b.copy(), 0, 0, 0
+    maxiter = 10 * n if maxiter is None else maxiter
+    rhotol = np.finfo(np.float64).eps ** 2
+    r = b - A @ x if x.any() else b.copy()
+    rt = r.copy()
+    rho_prev = omega = alpha = p = v = None
+    done = 0
+    for iteration in range(maxiter):
+        if np.sqrt(dot(r, r)) < atol:
+            return x, 0, done, 0
+        rho = dot(rt, r)
+        if np.abs(rho) < rhotol:
+            return x, -10, done, 0
+        if iteration > 0:
+            if np.abs(omega) < rhotol:
+                return x, -11, done, 0
+            beta = (rho / rho_prev) * (alpha / omega)
+            p -= omega * v
+            p *= beta
+            p += r
+        else:
+            p = r.copy()
+        phat = psolve(p)
+        v = A @ phat
+        rv = dot(rt, v)
+        if rv == 0:
+            return x, -11, done, 0
+        alpha = rho / rv
+        r -= alpha * v
+        s = r.copy()
+        if np.sqrt(dot(s, s)) < atol:
+            x += alpha * phat
+            return x, 0, done, 1
+        shat = psolve(s)
+        t = A @ shat
+        omega = dot(t, s) / dot(t, t)
+        x += alpha * phat
+        x += omega * shat
+        r -= omega * t
+        rho_prev = rho
+        done += 1
+    return x, maxiter, done, 0
+
+
+def cg_restated(A, b, x0=None, rtol=1e-5, atol=0.0, maxiter=None, precond="none", M=None, dot=None):
+    """scipy 1.15.3 ``cg`` (iterative.py, the solver of
+    ``solve_pressure_correction``, core/linear_solver.py:88-93) step by step
+    with a pluggable ``dot``, as ``bicgstab_restated``: the stop test on
+    ||r|| at the top of each iteration, z = M r, rho = r.z, p = z
+    (iteration 0) or p *= beta; p += z with beta = rho / rho_prev, q = A p,
+    alpha = rho / p.q, x += alpha p, r -= alpha q.  A zero p.q (where scipy
+    divides by zero) returns info -12.  Returns (x, info, iterations)."""
+    dot = np.dot if dot is None else dot
+    A = sparse.csr_matrix(A)
+    n = A.shape[0]
+    psolve = _psolve_of(A, precond, M)
+    b = np.asarray(b, np.float64)
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    bnrm2 = np.sqrt(dot(b, b))
+    atol = max(float(atol), float(rtol) * float(bnrm2))
+    if bnrm2 == 0:
+        return b.copy(), 0, 0
+    maxiter = 10 * n if maxiter is None else maxiter
+    r = b - A @ x if x.any() else b.copy()
+    rho_prev = p = None
+    done = 0
+    for iteration in range(maxiter):
+        if np.sqrt(dot(r, r)) < atol:
+            return x, 0, done
+        z = psolve(r)
+        rho = dot(r, z)
+        if iteration > 0:
+            p *= rho / rho_prev
+            p += z
+        else:
+            p = z.copy()
+        q = A @ p
+        pq = dot(p, q)
+        if pq == 0 or pq != pq:
+            return x, -12, done
+        alpha = rho / pq
+        x += alpha * p
+        r -= alpha * q
+        rho_prev = rho
+        done += 1
+    return x, maxiter, done
+
+
 def true_rel_residual(A, x, b):
     nb = np.linalg.norm(b)
     return float(np.linalg.norm(b - A @ x) / (nb if nb > 0 else 1.0))

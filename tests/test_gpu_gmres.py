@@ -2,12 +2,10 @@
 64-68) against the GMRES oracle.  The device's solve equals the restated
 GMRES (oracle.solve.gmres_restated, itself equal to scipy's gmres in
 scipy's order) evaluated in the device's summation order: same inner
-iterations, cycles and info, x to 1e-12.  Against scipy: inner iterations
-within 2, and restart cycles within 1, of the band that scipy and the
-restatement span when only the summation order of the dot products and of
-the x update changes (the one arithmetic the device does in another
-order), info among theirs, solution within max(1e-10, 10x the band's
-spread), FP64-verified residual below the tolerance.  Known answers
+iterations, cycles and info, and x bit for bit.  scipy's own result and the
+band the restatement spans under other summation orders of the dot
+products and of the x update (the one arithmetic the device does in another
+order) are printed beside it; FP64-verified residual below the tolerance.  Known answers
 (distinct eigenvalues, exact preconditioner, maxiter = restart cycles, zero
 right-hand side, x0 already converged); the drop-in keeps the reference's
 call semantics (atol = tol, rtol = 1e-5)."""
@@ -60,12 +58,12 @@ def _device_order_restated(A, b, restart=None, perm=None, **kw):
 
 
 def _check(x, res, A, b, label, restart=None, perm=None, **kw):
-    """The restated GMRES summed in the device's own order: the same
-    iterations, cycles and info, x to 1e-12 (in practice bit for bit).
-    Then the bar against scipy: the band spans scipy and the restatement
-    under four summation orders, the device's among them (long restarted
-    runs move by a few iterations between orders: hex12 unpreconditioned
-    takes 1283 in scipy's order and 1280 in the device's)."""
+    """The bar: the restated GMRES summed in the device's own order gives
+    the same iterations, cycles and info and the same x, bit for bit.  The
+    band of scipy and the restatement under four other summation orders is
+    printed beside it (long restarted runs move by a few iterations between
+    orders: hex12 unpreconditioned takes 1283 in scipy's order and 1280 in
+    the device's)."""
     its, cycs, infos, xo, spread = gmres_band(A, b, restart=restart, **kw)
     xd, info_d, it_d, cyc_d = _device_order_restated(A, b, restart=restart, perm=perm, **kw)
     its, cycs, infos = its + [it_d], cycs + [cyc_d], infos + [info_d]
@@ -77,11 +75,7 @@ def _check(x, res, A, b, label, restart=None, perm=None, **kw):
           f"{min(its)}..{max(its)} it, {min(cycs)}..{max(cycs)} cycles) | relL2={d:.2e} "
           f"(spread {spread:.2e}) | device-order restatement {it_d} it / {cyc_d} cycles, relL2={dd:.2e}")
     assert (res.iters, res.refinements, res.info) == (it_d, cyc_d, info_d)
-    assert dd <= 1e-12
-    assert res.info in infos
-    assert min(its) - 2 <= res.iters <= max(its) + 2
-    assert min(cycs) - 1 <= res.refinements <= max(cycs) + 1
-    assert d <= max(1e-10, 10 * spread)
+    assert np.array_equal(x, xd), dd
     return d
 
 

@@ -5,13 +5,21 @@
   Green-Gauss gradients): device assembly equal to the oracle's (structure
   exact, values and right-hand side within 1e-14), then BiCGSTAB + ILU(0)
   with the FP64 true residual checked on the host.
-* The SIMPLE pressure-correction system: FP64 and mixed precision (FP32
-  SpMV / preconditioner / Krylov vectors inside FP64 reliable updates) both
-  to relative residual 1e-10, each verified in FP64 on the host with the
-  device-assembled matrix, and the mixed solution's error against the FP64
-  one reported.  (The oracle's per-face pressure assembly is a Python loop,
-  too slow at 6.3M faces; the pressure coefficients are checked against it
-  at smaller sizes.)"""
+* The SIMPLE pressure-correction system of the benchmark (config 2): the
+  device assembly equal to the oracle's (structure exact, values within
+  1e-14); the FP64 BiCGSTAB + colour-ordered ILU(0) solve to 1e-8 equal bit
+  for bit to the restated scipy solver in the device's dot order (its
+  iterations, info and the SHA-256 of x are a fixture,
+  tests/golden/restated_hex128.json, written by
+  tests/golden/make_restated_fixtures.py: the restatement takes two minutes
+  on a CPU); FP64 and mixed precision (FP32 SpMV / preconditioner / Krylov
+  vectors inside FP64 reliable updates) both to relative residual 1e-10,
+  each verified in FP64 on the host, and the mixed solution's error against
+  the FP64 one reported."""
+
+import hashlib
+import json
+import os
 
 import numpy as np
 import pytest
@@ -47,6 +55,37 @@ def test_hex128_momentum(ctx, cases):
           f"{res.iters} it, resid gpu {res.rel_resid:.2e} cpu {tr:.2e}")
     assert ev <= 1e-14 and eb <= 1e-14
     assert res.info == 0 and tr < 1e-8
+
+
+def _sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, "<f8").tobytes()).hexdigest()
+
+
+def test_hex128_pressure_assembly_and_fp64_solve(ctx, cases):
+    from kcfd_amd import device, workloads
+    case = cases[1]
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "restated_hex128.json")))
+    fx = fx["hex128_pressure_fp64_ilu0_color_rtol1e-8_rows2"]
+    print("hex 128^3 pressure: device assembly vs the oracle", flush=True)
+    dm = device.DeviceMesh(case.mesh, ctx)
+    Ad = workloads.assemble_device(dm, case)
+    Ao, bo, diag_o = oracle_system(case)
+    assert _sha(Ao.data) == fx["A_sha256"] and _sha(bo) == fx["b_sha256"]
+    Ag = Ad.to_scipy()
+    assert np.array_equal(Ag.indptr, Ao.indptr) and np.array_equal(Ag.indices, Ao.indices)
+    ev = np.max(np.abs(Ag.data - Ao.data) / np.maximum(np.abs(Ao.data), 1e-300))
+    et = np.max(np.abs(Ad.get_diag_tracker() - diag_o) / np.maximum(np.abs(diag_o), 1e-300))
+    print(f"  nnz {Ao.nnz}: values max rel {ev:.2e} (bit-equal {np.array_equal(Ag.data, Ao.data)}), "
+          f"tracker {et:.2e}", flush=True)
+    assert ev <= 1e-14 and et <= 1e-14
+    M = device.DeviceMatrix.from_csr(ctx, Ao)
+    M.setup_precond("ilu0", "fp64", order="color")
+    x, res = M.solve(bo, rtol=1e-8, precond="ilu0", rows_per_thread=2)
+    print(f"  FP64 BiCGSTAB + colour ILU(0): gpu {res.iters} it info {res.info} half {res.half_step}, "
+          f"resid {res.rel_resid:.2e}, {res.ms:.1f} ms | restated (device order) {fx['iters']} it "
+          f"info {fx['info']}", flush=True)
+    assert (res.info, res.iters, res.half_step) == (fx["info"], fx["iters"], fx["half_step"])
+    assert _sha(x) == fx["x_sha256"]
 
 
 def test_hex128_pressure_mixed_vs_fp64_to_1e10(ctx, cases):
