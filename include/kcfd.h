@@ -199,6 +199,21 @@ typedef struct kcfd_solve_result {
 int kcfd_solve(kcfd_mat *m, const double *b, double *x,
                const kcfd_solve_opts *opts, kcfd_solve_result *res);
 
+/* kcfd_solve without any device allocation: no reference counterpart.  For
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * BiCGSTAB / CG kernels and arguments as kcfd_solve (GMRES is not offered),
+ * iterations launched directly, results equal to kcfd_solve's bit for bit;
+ * the matrix must be set up (kcfd_precond_setup) beforehand.  The whole
+ * workspace lives in one static 64 MiB device array per GPU, held until the
+ * process ends: a system whose workspace does not fit (about 1.2M rows in
+ * FP64 with Jacobi) is refused with KCFD_EARG.  Calls on one GPU share that
+ * array and are serialised by a per-device lock (overlapping calls would
+ * race on it); each call rewrites what it uses, and the array keeps the
+ * last call's contents. */
+int kcfd_solve_noalloc(kcfd_mat *m, const double *b, double *x,
+                       const kcfd_solve_opts *opts, kcfd_solve_result *res);
+
 /* Batched: many uneven systems advanced by the same kernels (one launch per
  * phase covers every active system; a hipGraph holds the iterations).
  * No reference counterpart: the reference solves one system per call.

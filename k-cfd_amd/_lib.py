@@ -106,6 +106,7 @@ SIGNATURES = {
     "kcfd_ilu0_order": ([_P, _P, ctypes.POINTER(_I)], _I),
     "kcfd_precond_apply": ([_P, _P, _P], _I),
     "kcfd_solve": ([_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveResult)], _I),
+    "kcfd_solve_noalloc": ([_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveResult)], _I),
     "kcfd_batch_create": ([_P, _I, _P, _P, _P, ctypes.POINTER(SolveOpts), _PP], _I),
     "kcfd_batch_run": ([_P, _P], _I),
     "kcfd_batch_destroy": ([_P], _I),

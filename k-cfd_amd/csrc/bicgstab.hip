@@ -1461,6 +1461,27 @@ __global__ void k_fill_sentinel(V *p, int n) {
 
 using namespace kcfd;
 
+// The workspace of kcfd_solve_noalloc, one copy per device (see
+// solve_noalloc below).  64 MiB: a FP64 Jacobi system of about 1.2M rows,
+// a mixed colour-ILU(0) one of about 1.1M.
+static constexpr size_t kNoAllocBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace kcfd {
+// Device address of `scratch` on the current device; kcfd_init calls it once
+// per context, with the context's device current.
+int noalloc_resolve(kcfd_ctx *ctx) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error("hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        return KCFD_EHIP;
+    }
+    ctx->noalloc = p;
+    return KCFD_OK;
+}
+}  // namespace kcfd
+
 // ======================================================================= host
 
 struct kcfd_batch {
@@ -1499,6 +1520,7 @@ struct kcfd_batch {
     hipGraphExec_t graph = nullptr;
     int graph_iters = 0;
     bool dirty = false;             // membership changed: tables to rebuild
+    bool no_graph = false;          // direct launches only (kcfd_solve_noalloc)
     // kcfd_batch_profile: launch_iteration records ctx->ev0 / ev1 around the
     // kernel of this phase (0: none, as during graph capture)
     int timed_phase = 0;
@@ -1791,9 +1813,8 @@ int check_mat(const kcfd_batch *b, const kcfd_mat *m, int k) {
 }
 
 // common part of a batch: parameters and the small per-slot device arrays
-int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **out) {
-    KCFD_CHECK(check_opts(o));
-    kcfd_batch *b = new kcfd_batch();
+// the host side of a batch: options, parameters and per-slot host arrays
+void init_batch_host(kcfd_batch *b, kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o) {
     b->ctx = ctx;
     b->nsys = slots;
     b->opts = *o;
@@ -1824,6 +1845,12 @@ int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **o
     SysState free_st{};
     free_st.status = S_FREE;
     b->h_state.assign(slots, free_st);
+}
+
+int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **out) {
+    KCFD_CHECK(check_opts(o));
+    kcfd_batch *b = new kcfd_batch();
+    init_batch_host(b, ctx, slots, o);
     *out = b;
     // Every host-issued copy and fill goes on the library stream (created
     // non-blocking, so the null stream is not ordered with it).
@@ -2022,6 +2049,8 @@ int rebuild_tables(kcfd_batch *b) {
     return KCFD_OK;
 }
 
+void set_launch_shape(kcfd_batch *b, const kcfd_solve_opts *o);
+
 int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *const *bs,
                 double *const *xs, const kcfd_solve_opts *o, kcfd_batch **out) {
     kcfd_batch *b = nullptr;
@@ -2046,6 +2075,14 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
     b->slab_bytes = slab;
     for (int k = 0; k < nsys; ++k) place_vectors(b, k, (char *)b->slab + off[k]);
     if ((rc = rebuild_tables(b)) != KCFD_OK) return fail(rc);
+    set_launch_shape(b, o);
+    *out = b;
+    return KCFD_OK;
+}
+
+// which ILU(0) kernels run, the persistent grid and the triangular-solve grid
+void set_launch_shape(kcfd_batch *b, const kcfd_solve_opts *o) {
+    kcfd_ctx *ctx = b->ctx;
     b->run_two = b->nchunks_a + b->nchunks_b > 0;
     b->run_polled = b->nchunks_l + b->nchunks_u > 0;
     // persistent grid: at most grid_per_cu workgroups per CU, a multiple of
@@ -2071,8 +2108,6 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
         const int tcap = ctx->cus * (o->trsv_per_cu > 0 ? o->trsv_per_cu : 4);
         b->trsv_grid = std::max(1, std::min((want_waves + kWaves - 1) / kWaves, tcap));
     }
-    *out = b;
-    return KCFD_OK;
 }
 
 int capture_graph(kcfd_batch *b, int iters) {
@@ -2105,13 +2140,14 @@ int capture_graph(kcfd_batch *b, int iters) {
 // kernels (same arguments: everything they read lives in device memory).
 // The direct mode is for tools that trace kernels: rocprofv3's kernel trace
 // has aborted the process inside hipGraphLaunch of the batch's graph.
-bool batch_graph_enabled() {
+bool batch_graph_enabled(const kcfd_batch *b) {
+    if (b->no_graph) return false;
     const char *e = std::getenv("KCFD_BATCH_GRAPH");
     return !(e && e[0] == '0');
 }
 
 int replay_iterations(kcfd_batch *b, hipStream_t s) {
-    if (!batch_graph_enabled()) {
+    if (!batch_graph_enabled(b)) {
         LaunchFn fn = pick_iter(b->bp.precision, b->bp.precond, b->bp.method, b->rows);
         for (int k = 0; k < b->opts.check_every; ++k) fn(b, s);
         KCFD_HIP(hipGetLastError());
@@ -2148,7 +2184,7 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     KCFD_HIP(hipMemcpyAsync(b->d_state, b->h_state.data(), sizeof(SysState) * b->nsys,
                             hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
-    if (batch_graph_enabled() && (!b->graph || b->graph_iters != b->opts.check_every))
+    if (batch_graph_enabled(b) && (!b->graph || b->graph_iters != b->opts.check_every))
         KCFD_CHECK(capture_graph(b, b->opts.check_every));
     KCFD_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * 8, s));
     KCFD_HIP(hipEventRecord(ctx->ev0, s));
@@ -2203,6 +2239,85 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
         }
     }
     return KCFD_OK;
+}
+
+// ------------------------------------------------- allocation-free solve
+// kcfd_solve_noalloc keeps its whole workspace -- descriptor, state,
+// parameters, counts, tile list, partials, ILU(0) chunk tables and the
+// Krylov vectors -- in this array, one per device, held until the process
+// ends.  Host code never uses the array's name as an address: kcfd_init
+// resolves the device address with hipGetSymbolAddress (kcfd_ctx::noalloc)
+// and everything below works from that pointer.
+// one lock per device: calls on one GPU share its array
+std::mutex g_noalloc_mu[64];
+
+struct Carve {
+    char *base;
+    size_t used, cap;
+    template <typename T>
+    T *take(size_t count) {
+        const size_t off = align_up(used, 256);
+        used = off + align_up(std::max<size_t>(count, 1) * sizeof(T), 256);
+        return used <= cap ? reinterpret_cast<T *>(base + off) : nullptr;
+    }
+};
+
+int solve_noalloc(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts *o,
+                  kcfd_solve_result *res) {
+    kcfd_ctx *ctx = m->ctx;
+    if (!ctx->noalloc) {
+        set_error("kcfd_solve_noalloc: the context has no scratch address (kcfd_init failed to resolve it)");
+        return KCFD_ESTATE;
+    }
+    kcfd_batch bb;
+    kcfd_batch *b = &bb;
+    init_batch_host(b, ctx, 1, o);
+    b->no_graph = true;
+    KCFD_CHECK(check_mat(b, m, 0));
+    describe_slot(b, 0, m, bvec, x);
+    // table sizes of this system
+    const int ntiles = std::max(1, (m->n + b->rows * kBlock - 1) / (b->rows * kBlock));
+    int nl = 0, nu = 0, na = 0, nb = 0;
+    if (b->bp.precond == KCFD_PC_ILU0) {
+        if (m->two_colour) {
+            na = (m->n - m->ncolour0 + 63) / 64;
+            nb = (m->ncolour0 + 63) / 64;
+        } else {
+            for (int l = 0; l < m->nlev_l; ++l) nl += (m->h_lvl_ptr_l[l + 1] - m->h_lvl_ptr_l[l] + 63) / 64;
+            for (int l = 0; l < m->nlev_u; ++l) nu += (m->h_lvl_ptr_u[l + 1] - m->h_lvl_ptr_u[l] + 63) / 64;
+        }
+    }
+    Carve c{reinterpret_cast<char *>(ctx->noalloc), 0, kNoAllocBytes};
+    b->d_desc = c.take<SysDesc>(1);
+    b->d_state = c.take<SysState>(1);
+    b->d_params = c.take<BatchParams>(1);
+    b->d_counts = c.take<int>(8);
+    b->d_counters = c.take<unsigned int>(8);
+    b->d_tiles = c.take<int2>(ntiles);
+    b->d_partials = c.take<double>((size_t)kMaxPartials * ntiles);
+    b->d_chunks_l = c.take<Chunk>(nl);
+    b->d_chunks_u = c.take<Chunk>(nu);
+    b->d_chunks_a = c.take<Chunk>(na);
+    b->d_chunks_b = c.take<Chunk>(nb);
+    b->d_lvl_l = c.take<const int *>(1);
+    b->d_lvl_u = c.take<const int *>(1);
+    char *vecs = c.take<char>(slot_bytes(b, m->n, b->h_desc[0].perm != nullptr));
+    if (c.used > c.cap) {
+        set_error("kcfd_solve_noalloc: a system of %d rows needs %zu bytes of workspace, more than the "
+                  "%zu-byte scratch array holds", m->n, c.used, c.cap);
+        return KCFD_EARG;
+    }
+    b->cap_tiles = ntiles;
+    b->cap_chunks_l = nl;
+    b->cap_chunks_u = nu;
+    b->cap_chunks_a = na;
+    b->cap_chunks_b = nb;
+    place_vectors(b, 0, vecs);
+    // every table fits its slice, so rebuild_tables allocates nothing
+    KCFD_CHECK(rebuild_tables(b));
+    set_launch_shape(b, o);
+    // the device arrays are slices of the scratch array: nothing to free
+    return run_batch(b, res);
 }
 
 }  // namespace
@@ -2335,7 +2450,7 @@ int kcfd_batch_step(kcfd_batch *b, int *finished, int *nfinished) {
     }
     if (!any) return KCFD_OK;
     KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
-    if (batch_graph_enabled() && (!b->graph || b->graph_iters != b->opts.check_every))
+    if (batch_graph_enabled(b) && (!b->graph || b->graph_iters != b->opts.check_every))
         KCFD_CHECK(capture_graph(b, b->opts.check_every));
     if (any_new) pick_init(b->bp.precision, b->bp.precond, b->rows)(b, s);
     KCFD_CHECK(replay_iterations(b, s));
@@ -2564,6 +2679,25 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
         // captured graph is reused as is
     }
     return run_batch(m->solver, res);
+}
+
+int kcfd_solve_noalloc(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts *opts,
+                       kcfd_solve_result *res) {
+    KCFD_ARG(m && opts, "null argument");
+    KCFD_ARG(m->n == 0 || (bvec && x), "null vector");
+    KCFD_CHECK(check_opts(opts));
+    KCFD_ARG(opts->method == KCFD_METHOD_BICGSTAB || opts->method == KCFD_METHOD_CG,
+             "kcfd_solve_noalloc runs BiCGSTAB and CG");
+    KCFD_ARG(m->ctx->device >= 0 && m->ctx->device < 64, "device index beyond the scratch table");
+    std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
+    std::lock_guard<std::mutex> gd(g_noalloc_mu[m->ctx->device]);
+    KCFD_HIP(hipSetDevice(m->ctx->device));
+    if (m->n == 0) {
+        if (res) std::memset(res, 0, sizeof(*res));
+        return KCFD_OK;
+    }
+    KCFD_CHECK(check_current(m, "kcfd_solve_noalloc"));
+    return solve_noalloc(m, bvec, x, opts, res);
 }
 
 }  // extern "C"

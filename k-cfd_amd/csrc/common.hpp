@@ -127,6 +127,9 @@ struct kcfd_ctx {
     // staging scratch for small host<->device copies
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    // device address of kcfd_solve_noalloc's static workspace array on this
+    // context's device (hipGetSymbolAddress at kcfd_init)
+    void *noalloc = nullptr;
 };
 
 struct kcfd_batch;
@@ -222,6 +225,7 @@ namespace kcfd {
 // KCFD_ESTATE unless the preconditioner (and the SELL copy) was set up from
 // the current values; `what` names the matrix in the message
 int check_current(const kcfd_mat *m, const char *what);
+int noalloc_resolve(kcfd_ctx *ctx);
 int ensure_scratch(kcfd_ctx *ctx, size_t bytes);
 int ilu0_structure(kcfd_mat *m);
 int ilu0_setup(kcfd_mat *m, bool want32);

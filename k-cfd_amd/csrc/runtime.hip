@@ -365,6 +365,7 @@ int kcfd_init(int device, kcfd_ctx **out) {
     KCFD_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     KCFD_HIP(hipEventCreate(&c->ev0));
     KCFD_HIP(hipEventCreate(&c->ev1));
+    KCFD_CHECK(noalloc_resolve(c));
     *out = c;
     return KCFD_OK;
 }
