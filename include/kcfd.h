@@ -372,6 +372,17 @@ int kcfd_face_values(kcfd_mesh *mesh, int mode, const double *phi,
 /* The TVD limiters of core/interpolation.py:17-44 (limiter_minmod ...
  * limiter_van_leer, KCFD_LIM_*) on n values: psi = limiter(r), dev arrays. */
 int kcfd_limiter(kcfd_ctx *ctx, int kind, const double *r, double *psi, long long n);
+/* schiller_naumann_drag (models/closure.py:13-34): cd = C_D(max(Re, 1e-10)),
+ * 24/Re (1 + 0.15 Re^0.687) below Re 1000, else 0.44; re, cd (dev, n). */
+int kcfd_schiller_naumann(kcfd_ctx *ctx, const double *re, double *cd, long long n);
+/* drag_coefficient_implicit (models/closure.py:72-92), the interphase drag
+ * coefficient the two-fluid momentum assembly puts on the diagonal
+ * (models/two_fluid.py:592-596, then kcfd_asm_cell_terms with kd = K):
+ * K = 0.75 C_D(Re) alpha_g rho_l |u_g - u_l| / d_b, Re = rho_l |u_g - u_l|
+ * d_b / mu_l, |u_g - u_l| floored at 1e-15.  alpha_g, K (dev, n); u_g, u_l
+ * (dev, n*ndim row-major). */
+int kcfd_drag_implicit(kcfd_ctx *ctx, int n, int ndim, const double *alpha_g, double rho_l,
+                       const double *u_g, const double *u_l, double d_b, double mu_l, double *K);
 /* muscl_deferred_correction (core/interpolation.py:227-303); corr (dev n) */
 int kcfd_muscl(kcfd_mesh *mesh, const double *phi, const double *F,
                const double *grad, int limiter, double *corr);

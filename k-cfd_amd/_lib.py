@@ -106,6 +106,8 @@ SIGNATURES = {
     "kcfd_ilu0_order": ([_P, _P, ctypes.POINTER(_I)], _I),
     "kcfd_precond_apply": ([_P, _P, _P], _I),
     "kcfd_solve": ([_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveResult)], _I),
+    "kcfd_schiller_naumann": ([_P, _P, _P, ctypes.c_longlong], _I),
+    "kcfd_drag_implicit": ([_P, _I, _I, _P, _D, _P, _P, _D, _D, _P], _I),
     "kcfd_solve_noalloc": ([_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveResult)], _I),
     "kcfd_batch_create": ([_P, _I, _P, _P, _P, ctypes.POINTER(SolveOpts), _PP], _I),
     "kcfd_batch_run": ([_P, _P], _I),

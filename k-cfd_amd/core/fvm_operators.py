@@ -215,10 +215,14 @@ def under_relax(system: FVMSystem, phi, alpha: float):
         return
     if system._A is None:
         # nothing assembled on the device yet: the tracker lives on the host
+        # the reference's per-cell add_diagonal / add_source, as arrays
         ap = system.diag.copy()
         k = ap * (1.0 - alpha) / alpha
-        for ci in range(system.n):
-            system.add_diagonal(ci, k[ci])
+        idx = range(system.n)
+        system.rows.extend(idx)
+        system.cols.extend(idx)
+        system.vals.extend(k.tolist())
+        system.diag[:] = ap + k
         system.rhs[:] += ap * (1.0 - alpha) / alpha * values_of(phi)
         return
     system._sync_to_device()

@@ -233,6 +233,45 @@ __global__ __launch_bounds__(kBlock) void k_cell_terms(int n, int mode, const in
     }
 }
 
+// ---------------------------------------------- interphase drag closure
+// schiller_naumann_drag (models/closure.py:13-34): Re floored at 1e-10 (NaN
+// kept, as np.maximum does), C_D = 24/Re (1 + 0.15 Re^0.687) below 1000,
+// else 0.44; operation order as the numpy expression.
+__device__ __forceinline__ double schiller_naumann(double re) {
+#pragma clang fp contract(off)
+    re = (re >= 1e-10 || re != re) ? re : 1e-10;
+    return re < 1000.0 ? 24.0 / re * (1.0 + 0.15 * pow(re, 0.687)) : 0.44;
+}
+
+__global__ __launch_bounds__(kBlock) void k_schiller_naumann(const double *__restrict__ re,
+                                                             double *__restrict__ cd, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cd[i] = schiller_naumann(re[i]);
+}
+
+// drag_coefficient_implicit (models/closure.py:72-92): |u_rel| (squares
+// summed over the ndim components in order, floored at 1e-15),
+// Re = rho_l |u_rel| d_b / mu_l, K = 0.75 C_D alpha_g rho_l |u_rel| / d_b.
+// u_g, u_l: n x ndim row-major.
+__global__ __launch_bounds__(kBlock) void k_drag(int n, int ndim, const double *__restrict__ alpha_g, double rho_l,
+                                                 const double *__restrict__ ug, const double *__restrict__ ul,
+                                                 double d_b, double mu_l, double *__restrict__ K) {
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double s = 0.0;
+    for (int k = 0; k < ndim; ++k) {
+        const double u = ug[(size_t)c * ndim + k] - ul[(size_t)c * ndim + k];
+        const double u2 = u * u;
+        s = k == 0 ? u2 : s + u2;
+    }
+    double mag = sqrt(s);
+    mag = (mag >= 1e-15 || mag != mag) ? mag : 1e-15;
+    const double re = rho_l * mag * d_b / mu_l;
+    const double cd = schiller_naumann(re);
+    K[c] = 0.75 * cd * alpha_g[c] * rho_l * mag / d_b;
+}
+
 // ---------------------------------------------- pressure correction
 // models/single_phase.py:288-325 coefficients
 __global__ __launch_bounds__(kBlock) void k_press_face(MeshView m, const double *__restrict__ dP,
@@ -1093,6 +1132,29 @@ int kcfd_face_values(kcfd_mesh *mesh, int mode, const double *phi, const double 
     Timer t(mesh->ctx);
     hipLaunchKernelGGL(k_face_values, dim3(blocks(mesh->nf)), dim3(kBlock), 0, mesh->ctx->stream, view(mesh),
                        mode, phi, phib, F, grad, phif);
+    return t.stop();
+}
+
+int kcfd_schiller_naumann(kcfd_ctx *ctx, const double *re, double *cd, long long n) {
+    KCFD_ARG(ctx && n >= 0 && (n == 0 || (re && cd)), "null argument");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (n == 0) return KCFD_OK;
+    hipLaunchKernelGGL(k_schiller_naumann, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       ctx->stream, re, cd, (size_t)n);
+    KCFD_HIP(hipGetLastError());
+    KCFD_HIP(hipStreamSynchronize(ctx->stream));
+    return KCFD_OK;
+}
+
+int kcfd_drag_implicit(kcfd_ctx *ctx, int n, int ndim, const double *alpha_g, double rho_l,
+                       const double *u_g, const double *u_l, double d_b, double mu_l, double *K) {
+    KCFD_ARG(ctx && n >= 0 && (n == 0 || (alpha_g && u_g && u_l && K)), "null argument");
+    KCFD_ARG(ndim >= 1 && ndim <= 3, "ndim must be 1..3");
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    if (n == 0) return KCFD_OK;
+    Timer t(ctx);
+    hipLaunchKernelGGL(k_drag, dim3(blocks(n)), dim3(kBlock), 0, ctx->stream, n, ndim, alpha_g, rho_l, u_g, u_l,
+                       d_b, mu_l, K);
     return t.stop();
 }
 

@@ -27,7 +27,8 @@ Configurations (sizes are parameters; defaults are the full sizes):
                  models/two_fluid.py:520-612 (alpha_g ~ U[0.05,0.6],
                  diffusion alpha_l*mu_l, upwind alpha_l*F, temporal,
                  Schiller-Naumann implicit drag on the diagonal, Dirichlet
-                 0 walls).
+                 0 walls); the drag coefficient is computed on the
+                 device (kcfd_drag_implicit).
 """
 
 from __future__ import annotations
@@ -291,12 +292,6 @@ def tet_twofluid(npts=200_000, seed=1003, mesh=None) -> Case:
     U_l = rng.normal(0.0, 0.2, (nc, nd))
     U_g = U_l + rng.normal(0.0, 0.2, (nc, nd))
     Ub = {nm: np.zeros((len(f), nd)) for nm, f in mesh.boundary_patches.items()}
-    # Schiller-Naumann implicit drag (models/closure.py:72-92)
-    ur = U_g - U_l
-    mag = np.maximum(np.sqrt(np.sum(ur ** 2, axis=1)), 1e-15)
-    Re = np.maximum(RHO_L * mag * D_B / MU_L, 1e-10)
-    cd = np.where(Re < 1000, 24.0 / Re * (1.0 + 0.15 * Re ** 0.687), 0.44)
-    K = 0.75 * cd * alpha_g * RHO_L * mag / D_B
     dt = 1e-3
     gamma = alpha_l * MU_L
     phi_old = U_l[:, 0].copy()
@@ -305,7 +300,9 @@ def tet_twofluid(npts=200_000, seed=1003, mesh=None) -> Case:
               ("diffusion", dict(gamma=gamma)),
               ("upwind_scaled", dict(scale_owner=alpha_l)),
               ("cell_terms", dict(kd=alpha_l * RHO_L / dt, ks=phi_old)),
-              ("cell_terms", dict(kd=K, ks=U_g[:, 0])),
+              # Schiller-Naumann implicit drag (models/closure.py:72-92, applied
+              # at models/two_fluid.py:592-596): K computed on the device
+              ("drag", dict(alpha_g=alpha_g, U_g=U_g, U_l=U_l, comp=0, rho_l=RHO_L, d_b=D_B, mu_l=MU_L)),
               ("boundary_scaled", dict(gamma=gamma, scale_owner=alpha_l, phib=phib, bc=bc))]
     return Case("tet_twofluid", mesh, recipe, "jacobi",
                 meta=dict(npts=npts, seed=seed, alpha_g=alpha_g))
@@ -399,6 +396,15 @@ class DeviceRecipe:
             elif op == "source":
                 src = dev(kw["S"])
                 self.ops.append(lambda A, src=src: check(lib().kcfd_asm_source(dmesh.h, A.h, src)))
+            elif op == "drag":
+                K = DevBuf(ctx, mesh.n_cells)
+                self.keep.append(K)
+                U_g = np.ascontiguousarray(kw["U_g"], np.float64)
+                check(lib().kcfd_drag_implicit(ctx.h, mesh.n_cells, U_g.shape[1], dev(kw["alpha_g"]),
+                                               float(kw["rho_l"]), dev(U_g), dev(kw["U_l"]), float(kw["d_b"]),
+                                               float(kw["mu_l"]), K.p))
+                ks = dev(U_g[:, kw["comp"]])
+                self.ops.append(lambda A, kd=K.p, ks=ks: check(lib().kcfd_asm_cell_terms(dmesh.h, A.h, kd, ks)))
             elif op == "cell_terms":
                 kd, ks = dev(kw["kd"]), dev(kw["ks"])
                 self.ops.append(lambda A, kd=kd, ks=ks: check(lib().kcfd_asm_cell_terms(dmesh.h, A.h, kd, ks)))

@@ -35,6 +35,13 @@ def oracle_system(case):
             k = np.asarray(kw["kd"]) * mesh.cell_volume
             s.add_entries(idx, idx, k)
             s.add_source(idx, k * np.asarray(kw["ks"]))
+        elif op == "drag":
+            K = ofvm.drag_coefficient_implicit(kw["alpha_g"], kw["rho_l"], kw["U_g"], kw["U_l"], kw["d_b"],
+                                               kw["mu_l"])
+            idx = np.arange(mesh.n_cells)
+            k = K * mesh.cell_volume
+            s.add_entries(idx, idx, k)
+            s.add_source(idx, k * np.asarray(kw["U_g"])[:, kw["comp"]])
         elif op == "temporal":
             ofvm.temporal_operator(mesh, kw["rho"], kw["dt"], kw["phi_old"], s)
         elif op == "muscl":
