@@ -91,6 +91,10 @@ int kcfd_sync(kcfd_ctx *ctx);
  * (read + write) over `reps` launches.  Measurement hook with no reference
  * counterpart; calibrates the roofline and the FETCH/WRITE counters. */
 int kcfd_stream_copy(kcfd_ctx *ctx, size_t bytes, int reps, double *avg_ms);
+/* Read ceiling: mean time to stream `bytes` from HBM (loads only, one small
+ * result per thread).  Measurement hook with no reference counterpart: the
+ * solver kernels are read-mostly, so this is their practical ceiling. */
+int kcfd_stream_read(kcfd_ctx *ctx, size_t bytes, int reps, double *avg_ms);
 
 /* ------------------------------------------------------- device buffers */
 

@@ -82,6 +82,7 @@ SIGNATURES = {
                           ctypes.POINTER(ctypes.c_longlong)], _I),
     "kcfd_sync": ([_P], _I),
     "kcfd_stream_copy": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
+    "kcfd_stream_read": ([_P, _SZ, _I, ctypes.POINTER(_D)], _I),
     "kcfd_buf_alloc": ([_P, _SZ, _PP], _I),
     "kcfd_buf_free": ([_P, _P], _I),
     "kcfd_buf_upload": ([_P, _P, _P, _SZ], _I),
@@ -202,6 +203,12 @@ class Context:
         ms = ctypes.c_double()
         check(lib().kcfd_stream_copy(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
         return 2.0 * nbytes / (ms.value * 1e-3) / 1e9
+
+    def stream_read_gbs(self, nbytes: int = 1 << 30, reps: int = 10) -> float:
+        """Measured device read bandwidth (bytes read / time)."""
+        ms = ctypes.c_double()
+        check(lib().kcfd_stream_read(self.h, int(nbytes), int(reps), ctypes.byref(ms)))
+        return nbytes / (ms.value * 1e-3) / 1e9
 
     def close(self):
         if getattr(self, "h", None):

@@ -1484,6 +1484,14 @@ int noalloc_resolve(kcfd_ctx *ctx) {
 
 // ======================================================================= host
 
+namespace kcfd {
+struct SysTables {
+    std::vector<int2> tiles;
+    std::vector<std::vector<Chunk>> lvl_l, lvl_u;
+    std::vector<Chunk> ca, cb;
+};
+}  // namespace kcfd
+
 struct kcfd_batch {
     kcfd_ctx *ctx = nullptr;
     int nsys = 0;                   // slots (the capacity of a dynamic batch)
@@ -1521,6 +1529,11 @@ struct kcfd_batch {
     int graph_iters = 0;
     bool dirty = false;             // membership changed: tables to rebuild
     bool no_graph = false;          // direct launches only (kcfd_solve_noalloc)
+    // per-system table pieces (rebuild_tables); tables_dirty: the membership
+    // or a member's structure changed since they were built; compacted: the
+    // device tables hold only the systems that were still running
+    std::vector<SysTables> sys_tables;
+    bool tables_dirty = true, compacted = false;
     // kcfd_batch_profile: launch_iteration records ctx->ev0 / ev1 around the
     // kernel of this phase (0: none, as during graph capture)
     int timed_phase = 0;
@@ -1930,34 +1943,73 @@ int ensure_table(T **p, int *cap, int need, bool *grew) {
     return KCFD_OK;
 }
 
-// Tile list (256 rows, slot order), per-slot tile ranges, and the level-major
-// ILU chunk tables of the current members, uploaded on the solver stream.
-// A grown table invalidates the captured graph (its kernels hold the old
-// pointers).
-int rebuild_tables(kcfd_batch *b) {
+// Per-system pieces of the tile and chunk tables: its tiles in the order
+// they are listed, its sync-free ILU(0) chunks per level, its two-colour
+// chunk runs.  Built when the membership changes; the device tables are
+// concatenations of these, so dropping finished systems from the tables is
+// a cheap copy.
+void build_sys_tables(kcfd_batch *b, int k) {
+    SysTables &t = b->sys_tables[k];
+    t = SysTables{};
+    const kcfd_mat *m = b->mats[k];
+    if (!m) return;
+    const SysDesc &d = b->h_desc[k];
+    const int kTile = b->rows * kBlock;   // rows per tile
+    const int nt = std::max(1, (d.n + kTile - 1) / kTile);
+    if (d.perm && !m->h_perm.empty()) {
+        // permuted row order: tiles listed by the row of A they start at, so
+        // that the tiles of both colours over one region of A sit together in
+        // the list and fall into the same XCD's range (its gathers then hit
+        // values its own L2 wrote)
+        std::vector<std::pair<int, int>> key(nt);
+        for (int q = 0; q < nt; ++q) key[q] = {m->h_perm[(size_t)q * kTile], q};
+        std::sort(key.begin(), key.end());
+        for (const auto &kt : key) t.tiles.push_back(make_int2(k, kt.second * kTile));
+    } else {
+        for (int q = 0; q < nt; ++q) t.tiles.push_back(make_int2(k, q * kTile));
+    }
+    if (b->bp.precond != KCFD_PC_ILU0) return;
+    if (m->two_colour) {
+        const int n0 = m->ncolour0, n1 = m->n - m->ncolour0;
+        for (int s0 = 0; s0 < n1; s0 += 64) t.ca.push_back(Chunk{k, n0 + s0, std::min(64, n1 - s0), s0 / 64});
+        for (int s0 = 0; s0 < n0; s0 += 64) t.cb.push_back(Chunk{k, s0, std::min(64, n0 - s0), s0 / 64});
+        return;
+    }
+    for (int upper = 0; upper < 2; ++upper) {
+        const std::vector<int> &lp = upper ? m->h_lvl_ptr_u : m->h_lvl_ptr_l;
+        const std::vector<int> &cp = upper ? m->h_chu_ptr : m->h_chl_ptr;
+        const int nl = upper ? m->nlev_u : m->nlev_l;
+        std::vector<std::vector<Chunk>> &lv = upper ? t.lvl_u : t.lvl_l;
+        lv.assign(nl, {});
+        for (int l = 0; l < nl; ++l)
+            for (int s0 = lp[l]; s0 < lp[l + 1]; s0 += 64)
+                lv[l].push_back(Chunk{k, s0, std::min(64, lp[l + 1] - s0), cp[l] + (s0 - lp[l]) / 64});
+    }
+}
+
+// Tile list (slot order), per-slot tile ranges, and the level-major ILU chunk
+// tables of the systems in `only` (null: every member), uploaded on the
+// solver stream.  A system's tiles stay contiguous and in their order, so
+// its partial sums -- and its arithmetic -- do not depend on which other
+// systems are in the tables.  A grown table invalidates the captured graph
+// (its kernels hold the old pointers).
+int rebuild_tables(kcfd_batch *b, const std::vector<char> *only = nullptr) {
     hipStream_t s = b->ctx->stream;
+    if (b->tables_dirty || (int)b->sys_tables.size() != b->nsys) {
+        b->sys_tables.assign(b->nsys, SysTables{});
+        for (int k = 0; k < b->nsys; ++k) build_sys_tables(b, k);
+        b->tables_dirty = false;
+    }
+    auto in = [&](int k) { return b->mats[k] && (!only || (*only)[k]); };
     std::vector<int2> tiles;
     for (int k = 0; k < b->nsys; ++k) {
         SysDesc &d = b->h_desc[k];
         d.tile0 = (int)tiles.size();
         d.ntiles = 0;
-        if (!b->mats[k]) continue;
-        const int kTile = b->rows * kBlock;   // rows per tile
-        const int nt = std::max(1, (d.n + kTile - 1) / kTile);
-        d.ntiles = nt;
-        const kcfd_mat *m = b->mats[k];
-        if (d.perm && !m->h_perm.empty()) {
-            // permuted row order: tiles listed by the row of A they start at,
-            // so that the tiles of both colours over one region of A sit
-            // together in the list and fall into the same XCD's range (its
-            // gathers then hit values its own L2 wrote)
-            std::vector<std::pair<int, int>> key(nt);
-            for (int t = 0; t < nt; ++t) key[t] = {m->h_perm[(size_t)t * kTile], t};
-            std::sort(key.begin(), key.end());
-            for (const auto &kt : key) tiles.push_back(make_int2(k, kt.second * kTile));
-        } else {
-            for (int t = 0; t < nt; ++t) tiles.push_back(make_int2(k, t * kTile));
-        }
+        if (!in(k)) continue;
+        const SysTables &t = b->sys_tables[k];
+        d.ntiles = (int)t.tiles.size();
+        tiles.insert(tiles.end(), t.tiles.begin(), t.tiles.end());
     }
     b->ntiles = (int)tiles.size();
     bool grew = false;
@@ -1968,52 +2020,37 @@ int rebuild_tables(kcfd_batch *b) {
         b->d_partials = nullptr;
         KCFD_HIP(hipMalloc(&b->d_partials, sizeof(double) * kMaxPartials * (size_t)b->cap_tiles));
     }
-    std::vector<Chunk> cl, cu;
+    std::vector<Chunk> cl, cu, ca, cb;
     if (b->bp.precond == KCFD_PC_ILU0) {
-        int maxlev_l = 0, maxlev_u = 0;
-        for (auto *m : b->mats)
-            if (m && !m->two_colour) {
-                maxlev_l = std::max(maxlev_l, m->nlev_l);
-                maxlev_u = std::max(maxlev_u, m->nlev_u);
+        // sync-free chunks in level-major order over the systems (the solve's
+        // progress argument needs every chunk of level l listed before any of
+        // level l + 1)
+        size_t maxlev_l = 0, maxlev_u = 0;
+        for (int k = 0; k < b->nsys; ++k)
+            if (in(k)) {
+                maxlev_l = std::max(maxlev_l, b->sys_tables[k].lvl_l.size());
+                maxlev_u = std::max(maxlev_u, b->sys_tables[k].lvl_u.size());
             }
-        auto build_chunks = [&](bool upper, std::vector<Chunk> &ch) {
-            const int L = upper ? maxlev_u : maxlev_l;
-            for (int l = 0; l < L; ++l) {
-                for (int k = 0; k < b->nsys; ++k) {
-                    kcfd_mat *m = b->mats[k];
-                    if (!m || m->two_colour) continue;
-                    const std::vector<int> &lp = upper ? m->h_lvl_ptr_u : m->h_lvl_ptr_l;
-                    const int nl = upper ? m->nlev_u : m->nlev_l;
-                    if (l >= nl) continue;
-                    const std::vector<int> &cp = upper ? m->h_chu_ptr : m->h_chl_ptr;
-                    for (int s0 = lp[l]; s0 < lp[l + 1]; s0 += 64) {
-                        Chunk c;
-                        c.sys = k;
-                        c.start = s0;
-                        c.count = std::min(64, lp[l + 1] - s0);
-                        c.local = cp[l] + (s0 - lp[l]) / 64;
-                        ch.push_back(c);
-                    }
-                }
+        for (size_t l = 0; l < maxlev_l; ++l)
+            for (int k = 0; k < b->nsys; ++k)
+                if (in(k) && l < b->sys_tables[k].lvl_l.size())
+                    cl.insert(cl.end(), b->sys_tables[k].lvl_l[l].begin(), b->sys_tables[k].lvl_l[l].end());
+        for (size_t l = 0; l < maxlev_u; ++l)
+            for (int k = 0; k < b->nsys; ++k)
+                if (in(k) && l < b->sys_tables[k].lvl_u.size())
+                    cu.insert(cu.end(), b->sys_tables[k].lvl_u[l].begin(), b->sys_tables[k].lvl_u[l].end());
+        // two-colour members: runs of 64 rows of colour 1 (a) and colour 0 (b)
+        for (int k = 0; k < b->nsys; ++k)
+            if (in(k)) {
+                ca.insert(ca.end(), b->sys_tables[k].ca.begin(), b->sys_tables[k].ca.end());
+                cb.insert(cb.end(), b->sys_tables[k].cb.begin(), b->sys_tables[k].cb.end());
             }
-        };
-        build_chunks(false, cl);
-        build_chunks(true, cu);
         b->nchunks_l = (int)cl.size();
         b->nchunks_u = (int)cu.size();
-        KCFD_CHECK(ensure_table(&b->d_chunks_l, &b->cap_chunks_l, b->nchunks_l, &grew));
-        KCFD_CHECK(ensure_table(&b->d_chunks_u, &b->cap_chunks_u, b->nchunks_u, &grew));
-        // two-colour members: runs of 64 rows of colour 1 (a) and colour 0 (b)
-        std::vector<Chunk> ca, cb;
-        for (int k = 0; k < b->nsys; ++k) {
-            kcfd_mat *m = b->mats[k];
-            if (!m || !m->two_colour) continue;
-            const int n0 = m->ncolour0, n1 = m->n - m->ncolour0;
-            for (int s0 = 0; s0 < n1; s0 += 64) ca.push_back(Chunk{k, n0 + s0, std::min(64, n1 - s0), s0 / 64});
-            for (int s0 = 0; s0 < n0; s0 += 64) cb.push_back(Chunk{k, s0, std::min(64, n0 - s0), s0 / 64});
-        }
         b->nchunks_a = (int)ca.size();
         b->nchunks_b = (int)cb.size();
+        KCFD_CHECK(ensure_table(&b->d_chunks_l, &b->cap_chunks_l, b->nchunks_l, &grew));
+        KCFD_CHECK(ensure_table(&b->d_chunks_u, &b->cap_chunks_u, b->nchunks_u, &grew));
         KCFD_CHECK(ensure_table(&b->d_chunks_a, &b->cap_chunks_a, b->nchunks_a, &grew));
         KCFD_CHECK(ensure_table(&b->d_chunks_b, &b->cap_chunks_b, b->nchunks_b, &grew));
         if (!ca.empty())
@@ -2046,6 +2083,7 @@ int rebuild_tables(kcfd_batch *b) {
         b->graph = nullptr;
     }
     b->dirty = false;
+    b->compacted = only != nullptr;
     return KCFD_OK;
 }
 
@@ -2215,7 +2253,21 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
             set_error("solver did not reach a terminal state after %lld graph iterations", guard);
             return KCFD_ESTATE;
         }
+        // Systems that finished leave the tables once they hold a fifth of
+        // the tiles: every kernel walks the tile and chunk lists, and a
+        // finished system's tiles would still cost a load chain each.
+        std::vector<char> act(b->nsys, 0);
+        int act_tiles = 0;
+        for (int k = 0; k < b->nsys; ++k)
+            if (b->mats[k] && b->h_state[k].status != S_DONE) {
+                act[k] = 1;
+                act_tiles += (int)b->sys_tables[k].tiles.size();
+            }
+        if (act_tiles < b->ntiles - b->ntiles / 5) KCFD_CHECK(rebuild_tables(b, &act));
     }
+    // the finishing kernels (true residual, restore, scatter) serve every
+    // system: the full tables again
+    if (b->compacted) KCFD_CHECK(rebuild_tables(b));
     KCFD_HIP(hipEventRecord(ctx->ev1, s));
     pick_finish(b->rows)(b, s);
     KCFD_HIP(hipGetLastError());
@@ -2426,6 +2478,8 @@ int kcfd_batch_attach(kcfd_batch *b, kcfd_mat *m, const double *bvec, double *x,
     b->fresh[k] = 1;
     b->reported[k] = 0;
     b->dirty = true;
+    if ((int)b->sys_tables.size() == b->nsys) build_sys_tables(b, k);
+    else b->tables_dirty = true;
     *slot = k;
     return KCFD_OK;
 }
@@ -2507,6 +2561,8 @@ int kcfd_batch_detach(kcfd_batch *b, int slot, kcfd_solve_result *res) {
     b->fresh[slot] = 0;
     b->reported[slot] = 0;
     b->dirty = true;
+    if ((int)b->sys_tables.size() == b->nsys) build_sys_tables(b, slot);
+    else b->tables_dirty = true;
     return KCFD_OK;
 }
 

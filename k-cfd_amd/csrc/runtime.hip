@@ -166,19 +166,62 @@ __global__ __launch_bounds__(kBlock) void k_sell_base(int n, const int *__restri
     }
 }
 
-// STREAM copy (16-byte vectors, 4 in flight per thread per step)
+// Bandwidth ceilings.  Each workgroup streams one contiguous chunk of the
+// array (whole 4 KB rows of 256 x 16-byte vectors), four rows in flight per
+// thread, with non-temporal loads and stores (the data is touched once, so
+// it should not displace anything in the caches).  copy: dst = src; read:
+// every vector is loaded and folded into one value per thread.
+constexpr int kCeilUnroll = 4;
+
 __global__ __launch_bounds__(kBlock) void k_stream_copy(size_t n16, const double2 *__restrict__ src,
                                                         double2 *__restrict__ dst) {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n16; i += 4 * stride) {
-        const double2 a = src[i], b = src[i + stride], c = src[i + 2 * stride], d = src[i + 3 * stride];
-        dst[i] = a;
-        dst[i + stride] = b;
-        dst[i + 2 * stride] = c;
-        dst[i + 3 * stride] = d;
+    const size_t row = (size_t)kBlock * kCeilUnroll;
+    const size_t per = (n16 + gridDim.x - 1) / gridDim.x;
+    const size_t chunk = (per + row - 1) / row * row;
+    const size_t b0 = (size_t)blockIdx.x * chunk, b1 = min(b0 + chunk, n16);
+    for (size_t base = b0; base < b1; base += row) {
+        double2 v[kCeilUnroll];
+#pragma unroll
+        for (int u = 0; u < kCeilUnroll; ++u) {
+            const size_t i = base + (size_t)u * kBlock + threadIdx.x;
+            if (i < b1) {
+                v[u].x = __builtin_nontemporal_load(&src[i].x);
+                v[u].y = __builtin_nontemporal_load(&src[i].y);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCeilUnroll; ++u) {
+            const size_t i = base + (size_t)u * kBlock + threadIdx.x;
+            if (i < b1) {
+                __builtin_nontemporal_store(v[u].x, &dst[i].x);
+                __builtin_nontemporal_store(v[u].y, &dst[i].y);
+            }
+        }
     }
-    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_stream_read(size_t n16, const double2 *__restrict__ src,
+                                                        double *__restrict__ out) {
+    const size_t row = (size_t)kBlock * kCeilUnroll;
+    const size_t per = (n16 + gridDim.x - 1) / gridDim.x;
+    const size_t chunk = (per + row - 1) / row * row;
+    const size_t b0 = (size_t)blockIdx.x * chunk, b1 = min(b0 + chunk, n16);
+    double acc = 0.0;
+    for (size_t base = b0; base < b1; base += row) {
+        double2 v[kCeilUnroll];
+#pragma unroll
+        for (int u = 0; u < kCeilUnroll; ++u) {
+            const size_t i = base + (size_t)u * kBlock + threadIdx.x;
+            v[u] = make_double2(0.0, 0.0);
+            if (i < b1) {
+                v[u].x = __builtin_nontemporal_load(&src[i].x);
+                v[u].y = __builtin_nontemporal_load(&src[i].y);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kCeilUnroll; ++u) acc += v[u].x + v[u].y;
+    }
+    out[(size_t)blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
 // SELL-64 structure of A in the solver's row order: natural, or P A P^T
@@ -451,25 +494,33 @@ int kcfd_buf_fill(kcfd_ctx *ctx, double *dev, double value, size_t n) {
     return KCFD_OK;
 }
 
-int kcfd_stream_copy(kcfd_ctx *ctx, size_t bytes, int reps, double *avg_ms) {
-    KCFD_ARG(ctx && avg_ms && reps > 0 && bytes >= 16, "bad argument");
+// mean time of `reps` launches of a ceiling kernel over a `bytes` array (one
+// untimed launch first); read_only: k_stream_read, else k_stream_copy
+static int stream_ceiling(kcfd_ctx *ctx, size_t bytes, int reps, bool read_only, double *avg_ms) {
     std::lock_guard<std::recursive_mutex> g(ctx->mu);
     KCFD_HIP(hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
     const size_t n16 = bytes / 16;
+    const unsigned grid = (unsigned)(ctx->cus * 8);
     void *a = nullptr, *b = nullptr;
+    const size_t bbytes = read_only ? (size_t)grid * kBlock * sizeof(double) : n16 * 16;
     KCFD_HIP(hipMalloc(&a, n16 * 16));
-    if (hipMalloc(&b, n16 * 16) != hipSuccess) {
+    if (hipMalloc(&b, bbytes) != hipSuccess) {
         hipFree(a);
-        set_error("stream copy: hipMalloc of %zu bytes failed", n16 * 16);
+        set_error("stream ceiling: hipMalloc of %zu bytes failed", bbytes);
         return KCFD_EOOM;
     }
     hipMemsetAsync(a, 0, n16 * 16, s);
-    const unsigned grid = (unsigned)(ctx->cus * 8);
-    hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, s, n16, (const double2 *)a, (double2 *)b);
+    auto launch = [&]() {
+        if (read_only)
+            hipLaunchKernelGGL(k_stream_read, dim3(grid), dim3(kBlock), 0, s, n16, (const double2 *)a, (double *)b);
+        else
+            hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, s, n16, (const double2 *)a,
+                               (double2 *)b);
+    };
+    launch();
     hipEventRecord(ctx->ev0, s);
-    for (int r = 0; r < reps; ++r)
-        hipLaunchKernelGGL(k_stream_copy, dim3(grid), dim3(kBlock), 0, s, n16, (const double2 *)a, (double2 *)b);
+    for (int r = 0; r < reps; ++r) launch();
     hipEventRecord(ctx->ev1, s);
     hipError_t e = hipEventSynchronize(ctx->ev1);
     float ms = 0.f;
@@ -477,11 +528,21 @@ int kcfd_stream_copy(kcfd_ctx *ctx, size_t bytes, int reps, double *avg_ms) {
     hipFree(a);
     hipFree(b);
     if (e != hipSuccess) {
-        set_error("stream copy: %s", hipGetErrorString(e));
+        set_error("stream ceiling: %s", hipGetErrorString(e));
         return KCFD_EHIP;
     }
     *avg_ms = ms / reps;
     return KCFD_OK;
+}
+
+int kcfd_stream_copy(kcfd_ctx *ctx, size_t bytes, int reps, double *avg_ms) {
+    KCFD_ARG(ctx && avg_ms && reps > 0 && bytes >= 16, "bad argument");
+    return stream_ceiling(ctx, bytes, reps, false, avg_ms);
+}
+
+int kcfd_stream_read(kcfd_ctx *ctx, size_t bytes, int reps, double *avg_ms) {
+    KCFD_ARG(ctx && avg_ms && reps > 0 && bytes >= 16, "bad argument");
+    return stream_ceiling(ctx, bytes, reps, true, avg_ms);
 }
 
 int kcfd_mat_create(kcfd_ctx *ctx, int n, int nnz, const int *indptr, const int *indices,
