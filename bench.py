@@ -10,10 +10,15 @@ matrix + preconditioner set-up + solve, with mesh, fields, b and x resident
 in HBM.  `value` is the mixed-precision (FP32 SpMV/preconditioner inside FP64
 refinement) rate; the plain FP64 rate, per-solve p50/p99 latency and the
 dominant kernel's roofline are reported beside it, and the scipy CPU
-baseline (oracle ILU(0) + scipy bicgstab) is timed on a bounded sample.
+baseline (oracle ILU(0) + scipy bicgstab) is timed to completion.
+
+Beside it the same line carries the 512-system batch (config 4), the quad
+128^2 Jacobi system (config 1) and the tet two-fluid Jacobi system
+(config 3), each FP64 and mixed, with CPU baselines timed to completion on
+the host before the GPU is touched.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 128]
-                       [--precond ilu0|jacobi] [--no-cpu]
+                       [--precond ilu0|jacobi] [--no-cpu] [--no-extra]
 """
 
 from __future__ import annotations
@@ -42,46 +47,97 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(A, b, precond, iters_needed, budget_s=15.0):
-    """scipy BiCGSTAB (reference call) + the oracle preconditioner on the host,
-    timed on a bounded number of iterations and scaled to a full solve."""
-    from oracle import solve as osolve
-    from scipy.sparse.linalg import LinearOperator, bicgstab
+# The reference's own CPU path for the 128^3 pressure system -- scipy
+# bicgstab with spilu(A.tocsc(), fill_factor=1, drop_tol=0) as
+# core/preconditioner.py:49-54 and core/linear_solver.py:59 call them -- run
+# once to completion on the GPU box's host (1 thread): about seven minutes,
+# too long for every bench run.  Written by `bench.py --workload
+# cpu-reference`; profiles/cpu_reference_hex128.md has the run.
+CPU_REFERENCE_HEX128 = None
 
-    os.environ.setdefault("OMP_NUM_THREADS", "1")
-    n = A.shape[0]
+
+def cpu_info():
+    """(threads this process may use, CPU model)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return len(os.sched_getaffinity(0)), model
+
+
+def _one_thread():
+    """BLAS pinned to one thread for the timed CPU solves (threadpoolctl
+    limits the libraries numpy and scipy already loaded)."""
+    from threadpoolctl import threadpool_limits
+    return threadpool_limits(limits=1)
+
+
+def cpu_solve(A, b, precond, reps=1):
+    """scipy bicgstab with the reference's call, rtol 1e-8, x0 = 0, timed to
+    completion: set-up + solve, one warm-up when reps > 1, the median of
+    reps runs.  precond: 'jacobi' (the reference's LinearOperator), 'spilu'
+    (the reference's spilu call) or 'ilu0_struct' (the oracle's structural
+    ILU(0), the preconditioner the device runs)."""
+    from oracle import solve as osolve
+    times, its = [], []
+    with _one_thread():
+        for r in range(reps + (1 if reps > 1 else 0)):
+            t0 = time.perf_counter()
+            M = osolve.make_preconditioner(A, "ilu0" if precond == "spilu" else precond)
+            x, info, it = osolve.solve_bicgstab(A, b, rtol=1e-8, M=M)
+            dt = time.perf_counter() - t0
+            if reps == 1 or r > 0:
+                times.append(dt)
+                its.append(it)
+    return dict(s_per_solve=float(np.median(times)), iters=its[-1], info=int(info),
+                rel_resid=osolve.true_rel_residual(A, x, b), runs=len(times))
+
+
+_POOL_CASES = None
+
+
+def _pool_solve(k):
+    from oracle import fvm as ofvm
+    case = _POOL_CASES[k]
+    A, b, _ = ofvm.system_of_case(case)
+    r = cpu_solve(A, b, "spilu")
+    return case.mesh.n_cells, r["s_per_solve"], r["iters"], r["info"]
+
+
+def cpu_batch_pool(cases, every, workers):
+    """The reference CPU path (spilu + bicgstab, one thread per process) on a
+    fixed sample of the batch -- every `every`-th system -- solved to
+    completion by `workers` processes (forked: before the GPU is touched).
+    Returns the N-core throughput and the 1-core rate of the same sample."""
+    import multiprocessing as mp
+    global _POOL_CASES
+    _POOL_CASES = cases
+    idx = list(range(0, len(cases), every))
     t0 = time.perf_counter()
-    if precond == "ilu0":
-        f = osolve.ILU0(A)
-        M = LinearOperator((n, n), matvec=f.solve)
-    else:
-        M = osolve.make_preconditioner(A, "jacobi")
-    t_setup = time.perf_counter() - t0
-    # time k iterations (maxiter=k) within the budget
-    k = 2
-    t_iter = None
-    while True:
-        t1 = time.perf_counter()
-        bicgstab(A, b, x0=np.zeros(n), rtol=1e-8, atol=0.0, maxiter=k, M=M)
-        dt = time.perf_counter() - t1
-        t_iter = dt / k
-        if dt * 3 > budget_s or k >= 64:
-            break
-        k *= 2
-    t_solve = t_setup + t_iter * iters_needed
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle {precond} set-up ({t_setup:.2f} s) + {k} scipy bicgstab iterations "
-                       f"({t_iter * 1e3:.1f} ms/iter) on the same {n}-row system, scaled to the "
-                       f"{iters_needed} iterations an FP64 solve with this preconditioner takes; 1 thread"),
-            "s_per_solve": t_solve}
+    with mp.get_context("fork").Pool(workers) as pool:
+        out = pool.map(_pool_solve, idx, chunksize=1)
+    wall = time.perf_counter() - t0
+    _POOL_CASES = None
+    cpu_s = sum(o[1] for o in out)
+    return dict(systems=len(idx), cells=int(sum(o[0] for o in out)), wall_s=round(wall, 2),
+                cpu_s=round(cpu_s, 2), workers=workers, solves_per_s_ncore=round(len(idx) / wall, 3),
+                solves_per_s_1core=round(len(idx) / cpu_s, 4),
+                not_converged=sum(1 for o in out if o[3] != 0))
 
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="pressure", choices=["pressure", "batched", "serve"],
-                    help="pressure: one hex n^3 pressure-correction system (default); "
-                         "batched: --nsys uneven quad/hex/tet systems in one batch; "
-                         "serve: Poisson request stream, continuous batching")
+    ap.add_argument("--workload", default="pressure", choices=["pressure", "batched", "serve", "cpu-reference"],
+                    help="pressure: the default line -- the hex n^3 pressure-correction system (value), "
+                         "with the 512-system batch, the quad 128^2 and the tet two-fluid configurations "
+                         "beside it; batched: the batch alone; serve: Poisson request stream, continuous "
+                         "batching; cpu-reference: the reference's own CPU path on the n^3 system, to "
+                         "completion (minutes)")
     ap.add_argument("--rate", type=float, default=200.0, help="serve: arrivals per second")
     ap.add_argument("--duration", type=float, default=60.0, help="serve: seconds of arrivals")
     ap.add_argument("--templates", type=int, default=8, help="serve: mesh sizes per kind")
@@ -91,6 +147,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch-steps", type=int, default=2, help="default line: timed steps of the batch")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--precond", default="ilu0", choices=["ilu0", "jacobi"])
     ap.add_argument("--method", default="bicgstab", choices=["bicgstab", "cg", "gmres"],
@@ -108,6 +165,8 @@ def main():
     ap.add_argument("--tile-walk", type=int, default=0,
                     help="launch shape: 1 interleaved, 2 contiguous tiles per workgroup (0: default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="default line: only the pressure system (no batch, no configs 1 and 3)")
     ap.add_argument("--no-calibrate", action="store_true",
                     help="serve: skip the per-template precision calibration")
     ap.add_argument("--no-single", action="store_true",
@@ -116,7 +175,7 @@ def main():
                     help="iterations per captured graph between host status checks")
     ap.add_argument("--precisions", default="fp64,mixed",
                     help="comma list; a profiler run can take one precision per process")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-workers", type=int, default=16, help="processes of the batch's CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "only one GPU is measured: --gpus must be 1 (independent systems "
@@ -124,27 +183,162 @@ def main():
                                    "row slabs with halo exchanges and block-Jacobi ILU(0); neither is built)"}))
         sys.exit(2)
     if args.workload == "batched":
-        return bench_batched(args)
+        return bench_batched_only(args)
     if args.workload == "serve":
         return bench_serve(args)
-    return bench_pressure(args)
+    if args.workload == "cpu-reference":
+        return bench_cpu_reference(args)
+    return bench_default(args)
 
 
-def bench_pressure(args):
+def _kc():
     import __graft_entry__
     __graft_entry__.load_package()
-    from kcfd_amd import _lib as L, device, workloads
-    from kcfd_amd._lib import DevBuf
+    from kcfd_amd import _lib as L, batch as B, device, tune, workloads
+    return L, B, device, tune, workloads
 
+
+def bench_cpu_reference(args):
+    """The reference's CPU path on the n^3 pressure system, to completion."""
+    L, B, device, tune, workloads = _kc()
+    from oracle import fvm as ofvm
+    cores, model = cpu_info()
+    case = workloads.hex_pressure(args.n, seed=1002)
+    A, b, _ = ofvm.system_of_case(case)
+    log(f"reference CPU path on {A.shape[0]} rows: spilu + bicgstab, 1 thread ({model})")
+    r = cpu_solve(A, b, "spilu")
+    r.update(cells=A.shape[0], nnz=int(A.nnz), cores_available=cores, cpu_model=model,
+             solves_per_s=1.0 / r["s_per_solve"])
+    print(json.dumps(r))
+
+
+def bench_default(args):
+    """The line the driver records: config 2 (hex n^3 pressure correction,
+    BiCGSTAB + ILU(0)) as `value`, with the 512-system batch (config 4), the
+    quad 128^2 Jacobi system (config 1) and the tet two-fluid Jacobi system
+    (config 3) measured beside it.  Host meshes and every CPU baseline come
+    first, before the GPU is touched (the batch's CPU pool forks)."""
+    L, B, device, tune, workloads = _kc()
+    from oracle import fvm as ofvm
+    extra = not args.no_extra and args.method == "bicgstab" and args.n == 128
+    t = time.perf_counter()
+    case = workloads.hex_pressure(args.n, seed=1002)
+    log(f"mesh {case.mesh.n_cells} cells built in {time.perf_counter() - t:.1f}s")
+    cases4 = c1 = c3 = None
+    if extra:
+        t = time.perf_counter()
+        specs = workloads.batch_specs(args.nsys)
+        cases4 = B.build_cases(specs)
+        c1 = workloads.quad_convdiff(128, seed=1001)
+        c3 = workloads.tet_twofluid(200_000, seed=1003)
+        log(f"batch, quad 128^2 and tet meshes built in {time.perf_counter() - t:.1f}s: "
+            f"{B.specs_summary(specs)}; tets {c3.mesh.n_cells}")
+    cores, model = cpu_info()
+    cpu = {}
+    if not args.no_cpu:
+        t = time.perf_counter()
+        A2, b2, _ = ofvm.system_of_case(case)
+        if args.method == "bicgstab" and args.precond == "ilu0":
+            # the oracle's structural ILU(0) in the natural order (the device's
+            # preconditioner; the colour ordering is a GPU choice) with scipy
+            # bicgstab, to completion; one run (about 20 s at 128^3)
+            r = cpu_solve(A2, b2, "ilu0_struct")
+            cpu["config2"] = dict(value=round(1.0 / r["s_per_solve"], 5), unit="solves/s", cores=1, kind="port",
+                                  sample=(f"the whole solve, once: oracle structural ILU(0), natural order, "
+                                          f"set-up + scipy bicgstab (the reference's call, rtol 1e-8) on the "
+                                          f"same {A2.shape[0]}-row system, {r['iters']} iterations, "
+                                          f"{r['s_per_solve']:.1f} s; BLAS pinned to 1 thread"),
+                                  iters=r["iters"], s_per_solve=round(r["s_per_solve"], 3))
+            if CPU_REFERENCE_HEX128 and args.n == 128:
+                cpu["config2_reference_path"] = CPU_REFERENCE_HEX128
+        if extra:
+            A1, b1, _ = ofvm.system_of_case(c1)
+            r = cpu_solve(A1, b1, "jacobi", reps=5)
+            cpu["config1"] = dict(value=round(1.0 / r["s_per_solve"], 3), unit="solves/s", cores=1,
+                                  kind="reference path", iters=r["iters"],
+                                  sample=(f"the reference's call exactly (Jacobi LinearOperator + scipy "
+                                          f"bicgstab, rtol 1e-8) on the quad 128^2 system, median of 5 after "
+                                          f"a warm-up, {r['iters']} iterations; 1 thread"))
+            A3, b3, _ = ofvm.system_of_case(c3)
+            r = cpu_solve(A3, b3, "jacobi")
+            cpu["config3"] = dict(value=round(1.0 / r["s_per_solve"], 4), unit="solves/s", cores=1,
+                                  kind="reference path", iters=r["iters"],
+                                  sample=(f"the reference's call exactly (Jacobi + scipy bicgstab, rtol 1e-8) "
+                                          f"on the {A3.shape[0]}-tet two-fluid momentum system, once, "
+                                          f"{r['s_per_solve']:.1f} s; 1 thread"))
+            workers = max(1, min(args.cpu_workers, cores))
+            pool = cpu_batch_pool(cases4, 16, workers)
+            cpu["config4"] = dict(value=pool["solves_per_s_ncore"], unit="solves/s", cores=workers,
+                                  kind="reference path", value_1core=pool["solves_per_s_1core"],
+                                  sample=(f"every 16th system of the batch ({pool['systems']} systems, "
+                                          f"{pool['cells']} cells) solved to completion by the reference's "
+                                          f"call (spilu(fill_factor=1, drop_tol=0) + scipy bicgstab, rtol "
+                                          f"1e-8) in {workers} processes of 1 thread: {pool['wall_s']} s wall, "
+                                          f"{pool['cpu_s']} s of solver time"), pool=pool)
+        log(f"CPU baselines in {time.perf_counter() - t:.1f}s: {cpu}")
     ctx = L.default_context()
     info = ctx.info()
     log("device", info)
     copy_gbs = ctx.stream_copy_gbs(1 << 30, 10)
-    log(f"STREAM copy (1 GiB): {copy_gbs:.1f} GB/s")
-    t = time.perf_counter()
-    case = workloads.hex_pressure(args.n, seed=1002)
+    read_gbs = ctx.stream_read_gbs(1 << 30, 10)
+    log(f"copy ceiling {copy_gbs:.1f} GB/s, read ceiling {read_gbs:.1f} GB/s (1 GiB)")
+    p = run_pressure(args, ctx, case)
+    mixed, fp64 = p["mixed"], p["fp64"]
+    roof = p["roofline"]
+    roof["stream_copy_gbs"] = round(copy_gbs, 1)
+    roof["stream_read_gbs"] = round(read_gbs, 1)
+    cfg = {}
+    if extra:
+        cfg["config1_quad128_jacobi"] = run_single(args, ctx, c1, "jacobi", steps=20)
+        cfg["config3_tet_twofluid_jacobi"] = run_single(args, ctx, c3, "jacobi", steps=5)
+        cfg["config4_batch512"] = run_batched(args, ctx, cases4, steps=args.batch_steps)
+        del cases4
+        for k, key in (("config1", "config1_quad128_jacobi"), ("config3", "config3_tet_twofluid_jacobi"),
+                       ("config4", "config4_batch512")):
+            if k in cpu:
+                cfg[key]["cpu_baseline"] = cpu[k]
+    line = {
+        "metric": "solves/s + %HBM roofline at rel-resid 1e-8; p50/p99 latency, mixed vs FP64",
+        "value": round(mixed["solves_per_s"], 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 refinement" if p["precs"] != ["fp64"] else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"hex{args.n}^3 SIMPLE pressure correction, "
+                               f"{dict(bicgstab='BiCGSTAB', cg='CG', gmres='GMRES(20)')[args.method]}"
+                               f"+{args.precond}, rtol 1e-8", "cells": p["n"], "nnz": p["nnz"],
+                   "ilu_order": args.order if args.precond == "ilu0" else None,
+                   "step": "device assembly + preconditioner set-up + solve"},
+        "roofline": roof,
+        "cpu_baseline": cpu.get("config2"),
+        "cpu_reference_path": cpu.get("config2_reference_path"),
+        "host": {"cores": cores, "cpu_model": model},
+        "mixed": {k: v for k, v in mixed.items() if k != "kernels"},
+        "fp64": {k: v for k, v in fp64.items() if k != "kernels"},
+        "mixed_vs_fp64": p["mixed_vs_fp64"],
+        "mixed_error_vs_fp64": p["mixed_error_vs_fp64"],
+        "value_precision": "mixed" if "mixed" in p["precs"] else p["precs"][0],
+        "kernels_mixed": mixed.get("kernels"),
+        "kernels_fp64": fp64.get("kernels"),
+        "configs": cfg,
+        "device": info["name"],
+    }
+    print(json.dumps(line))
+
+
+def run_pressure(args, ctx, case):
+    """Config 2: one step = device assembly of the pressure matrix +
+    preconditioner set-up + solve, FP64 and mixed; per-phase kernel timing
+    and the roofline of the SpMV k_spmv_p."""
+    L, B, device, tune, workloads = _kc()
+    from kcfd_amd._lib import DevBuf
     mesh = case.mesh
-    log(f"mesh {mesh.n_cells} cells built in {time.perf_counter() - t:.1f}s")
     dm = device.DeviceMesh(mesh, ctx)
     rec = dict(case.recipe)["pressure"]
     pd = np.array([1 if nm in rec["dirichlet"] else 0 for nm in dm.patch_names], np.int32)
@@ -231,14 +425,14 @@ def bench_pressure(args):
         # device time
         roofline = {"bound": "hbm", "achieved": round(solve_gbs, 1), "peak": HBM_PEAK_GBS,
                     "unit": "GB/s", "frac": round(solve_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "whole GMRES solve", "stream_copy_gbs": round(copy_gbs, 1)}
+                    "kernel": "whole GMRES solve"}
     else:
         dk = kern["spmv_p"]
         achieved = dk["gbs"]
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmv_p",
                     "avg_us": round(1e3 * dk["avg_ms"], 2), "bytes_per_launch": dk["bytes"],
-                    "solve_avg_gbs": round(solve_gbs, 1), "stream_copy_gbs": round(copy_gbs, 1)}
+                    "solve_avg_gbs": round(solve_gbs, 1)}
         # HBM bytes per k_spmv_p launch measured by the FETCH_SIZE / WRITE_SIZE
         # counter passes on this exact workload (default shape, mixed,
         # two-colour ILU(0), 128^3), scaled by the copy-kernel calibration
@@ -256,85 +450,71 @@ def bench_pressure(args):
             roofline["traffic_source"] = ("profiles/hbm_traffic_bicgstab_ilu0_mixed_hex128.md: FETCH_SIZE x 2.000 "
                                           "+ WRITE_SIZE x 1.000 KiB per dispatch, median working dispatch"
                                           + (" (packed SELL copy, KCFD_SELL16=0)" if packed else ""))
-    cpu = None
-    if not args.no_cpu:
-        try:
-            Ah = A.to_scipy()
-            bh_ = case.rhs
-            its = fp64["iters"]
-            if args.method != "bicgstab":
-                raise RuntimeError(f"no CPU baseline for the {args.method} workload")
-            if args.precond == "ilu0" and args.order != "natural":
-                # the CPU path factors in the natural order, which needs fewer
-                # iterations than the colour ordering: count them on the GPU
-                A.setup_precond("ilu0", "fp64", order="natural")
-                xdev.fill(0.0)
-                its = A.solve_dev(bdev.p, xdev.p, device.default_opts(rtol=1e-8, precond="ilu0"),
-                                  setup=False).iters
-                log(f"natural-order ILU(0) FP64 iterations (CPU baseline scaling): {its}")
-            cpu = cpu_baseline(Ah, bh_, args.precond, its, args.cpu_budget)
-            cpu["cores"] = 1
-            cpu.pop("s_per_solve", None)
-        except Exception as e:  # the baseline is informative; never fail the bench on it
-            log("cpu baseline failed:", repr(e))
-    line = {
-        "metric": "solves/s + %HBM roofline at rel-resid 1e-8; p50/p99 latency, mixed vs FP64",
-        "value": round(mixed["solves_per_s"], 4),
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(mixed["ms_per_step"], 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 inner / f64 refinement" if "mixed" in results else "f64",
-        "data": "synthetic",
-        "config": {"workload": f"hex{args.n}^3 SIMPLE pressure correction, "
-                               f"{dict(bicgstab='BiCGSTAB', cg='CG', gmres='GMRES(20)')[args.method]}"
-                               f"+{args.precond}, "
-                               f"rtol 1e-8", "cells": n, "nnz": nnz,
-                   "ilu_order": args.order if args.precond == "ilu0" else None,
-                   "step": "device assembly + preconditioner set-up + solve"},
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "mixed": {k: v for k, v in mixed.items() if k != "kernels"},
-        "fp64": {k: v for k, v in fp64.items() if k != "kernels"},
-        "mixed_vs_fp64": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3) if len(precs) > 1 else None,
-        # relative L2 of the mixed solution against the FP64 solution of the
-        # same system (the FP64 path is the one held to the scipy oracle)
-        "mixed_error_vs_fp64": (float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
-                                if "mixed" in sols and "fp64" in sols else None),
-        "value_precision": "mixed" if "mixed" in results else precs[0],
-        "kernels_mixed": kern,
-        "kernels_fp64": fp64["kernels"],
-        "device": info["name"],
-    }
-    print(json.dumps(line))
+    err = (float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
+           if "mixed" in sols and "fp64" in sols else None)
+    return dict(mixed=mixed, fp64=fp64, precs=precs, n=n, nnz=nnz, roofline=roofline,
+                mixed_vs_fp64=round(mixed["solves_per_s"] / fp64["solves_per_s"], 3) if len(precs) > 1 else None,
+                mixed_error_vs_fp64=err)
 
 
+def run_single(args, ctx, case, precond, steps):
+    """One system solved on its own (device-assembled matrix, x0 = 0, rtol
+    1e-8), FP64 and mixed: solves/s, per-solve latency, iterations and the
+    SpMV's rate inside real iterations."""
+    L, B, device, tune, workloads = _kc()
+    from kcfd_amd._lib import DevBuf
+    dm = device.DeviceMesh(case.mesh, ctx)
+    A = workloads.assemble_device(dm, case)
+    b = DevBuf.from_array(ctx, A.get_rhs())
+    x = DevBuf(ctx, A.n)
+    out = {"workload": case.name, "cells": A.n, "nnz": A.nnz, "precond": precond}
+    sols = {}
+    for prec in ("fp64", "mixed"):
+        A.setup_precond(precond, prec)
+        opts = device.default_opts(rtol=1e-8, precond=precond, precision=prec, check_every=args.check_every)
+        for _ in range(2):
+            x.fill(0.0)
+            A.solve_dev(b.p, x.p, opts, setup=False)
+        lat, rs = [], []
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts = time.perf_counter()
+            x.fill(0.0)
+            r = A.solve_dev(b.p, x.p, opts, setup=False)
+            lat.append(time.perf_counter() - ts)
+            rs.append(r)
+        wall = time.perf_counter() - t0
+        sols[prec] = x.download()
+        bh = L.ctypes.c_void_p()
+        L.check(L.lib().kcfd_mat_solver(A.h, L.ctypes.byref(bh)))
+        ms, by = L.ctypes.c_double(), L.ctypes.c_double()
+        L.check(L.lib().kcfd_batch_profile(bh, 1, 10, L.ctypes.byref(ms)))
+        L.check(L.lib().kcfd_batch_kernel_bytes(bh, 1, L.ctypes.byref(by)))
+        r = rs[-1]
+        out[prec] = dict(solves_per_s=round(steps / wall, 3), p50_ms=round(1e3 * float(np.percentile(lat, 50)), 3),
+                         p99_ms=round(1e3 * float(np.percentile(lat, 99)), 3), iters=r.iters,
+                         refinements=r.refinements, info=r.info, rel_resid=r.rel_resid, fallback=getattr(r, "fallback", 0),
+                         spmv=dict(avg_us=round(1e3 * ms.value, 2), bytes=by.value,
+                                   gbs=round(by.value / (ms.value * 1e-3) / 1e9, 1),
+                                   frac=round(by.value / (ms.value * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)))
+        log(case.name, prec, out[prec])
+    out["mixed_vs_fp64"] = round(out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"], 3)
+    out["mixed_error_vs_fp64"] = float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
+    A.close()
+    dm.close()
+    return out
 
-def bench_batched(args):
-    """Batched workload: --nsys systems, cells log-uniform in [2e3, 2.5e5],
-    50% quad convection-diffusion / 30% hex pressure correction / 20% tet
-    two-fluid momentum (seeds 4000+i), each assembled on the device, solved
-    together to rtol 1e-8 in FP64 and in mixed precision with the autotuned
-    launch shape.  One step = one batched solve of every system from x0 = 0."""
-    import __graft_entry__
-    __graft_entry__.load_package()
-    from kcfd_amd import _lib as L, batch as B, device, tune, workloads
 
-    specs = workloads.batch_specs(args.nsys)
-    t = time.perf_counter()
-    cases = B.build_cases(specs)          # forks builders: before the GPU is touched
-    log(f"built {len(cases)} host cases in {time.perf_counter() - t:.1f}s: {B.specs_summary(specs)}")
-    ctx = L.default_context()
-    info = ctx.info()
-    log("device", info)
-    copy_gbs = ctx.stream_copy_gbs(1 << 30, 10)
+def run_batched(args, ctx, cases, steps):
+    """Config 4: the systems of `cases` assembled on the device and solved as
+    one batch to rtol 1e-8, FP64 and mixed, each with its autotuned launch
+    shape; one step = one batched solve of every system from x0 = 0.  The
+    SpMV and the ILU(0) apply are timed inside batch iterations; the plain
+    FP64 path solving the systems one at a time is timed too."""
+    L, B, device, tune, workloads = _kc()
     t = time.perf_counter()
     sb = B.SystemBatch.assemble(ctx, cases)
-    del cases
     log(f"assembled {len(sb.mats)} systems ({sb.total_rows} rows) on the device in "
         f"{time.perf_counter() - t:.1f}s")
     order = args.order if args.precond == "ilu0" else None
@@ -362,20 +542,19 @@ def bench_batched(args):
             log(f"{prec} warmup {w}: {r.summary()}")
         runs = []
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             runs.append(sb.solve(batch=bobj, **opts))
             log(f"{prec} step: wall {runs[-1].wall_s * 1e3:.1f} ms")
         wall = time.perf_counter() - t0
         r = runs[-1]
         summ = r.summary()
-        summ["solves_per_s"] = round(args.steps * len(sb.mats) / wall, 2)
-        summ["ms_per_step"] = round(1e3 * wall / args.steps, 3)
+        summ["solves_per_s"] = round(steps * len(sb.mats) / wall, 2)
+        summ["ms_per_step"] = round(1e3 * wall / steps, 3)
         lat = [q.wall_s for q in runs]
         summ["p50_ms"] = round(1e3 * float(np.percentile(lat, 50)), 3)
         summ["p99_ms"] = round(1e3 * float(np.percentile(lat, 99)), 3)
         summ["achieved_gbs"] = round(float(np.mean([q.bytes / (q.device_ms * 1e-3) / 1e9 for q in runs])), 1)
-        # the dominant kernel (SpMV) timed inside real iterations of this batch
-        log(f"{prec}: timing the SpMV inside batch iterations")
+        # the dominant kernels timed inside real iterations of this batch
         ms = L.ctypes.c_double()
         by = L.ctypes.c_double()
         L.check(L.lib().kcfd_batch_profile(bobj.h, 1, 10, L.ctypes.byref(ms)))
@@ -387,7 +566,6 @@ def bench_batched(args):
             summ["ilu_apply" if order == "color" else "ilu_lower_solve"] = dict(
                 avg_ms=ms.value, bytes=by.value, gbs=by.value / (ms.value * 1e-3) / 1e9)
         # the profile overwrote x: solve once more for the solutions
-        log(f"{prec}: kernel timing done; solving again for the solutions")
         sb.solve(batch=bobj, **opts)
         sols[prec] = sb.solutions()
         bobj.close()
@@ -402,29 +580,50 @@ def bench_batched(args):
         e = [float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
              for a, b in zip(sols["mixed"], sols["fp64"])]
         err = dict(max=max(e), median=float(np.median(e)))
-    roof = {"bound": "hbm", "achieved": round(mixed["spmv"]["gbs"], 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(mixed["spmv"]["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "spmv_p (batch)", "solve_avg_gbs": mixed["achieved_gbs"],
-            "stream_copy_gbs": round(copy_gbs, 1)}
+    single = fp64.get("one_at_a_time")
+    out = {"systems": len(sb.mats), "rows": sb.total_rows, "ilu_order": order, "mixed": mixed, "fp64": fp64,
+           "roofline": {"bound": "hbm", "achieved": round(mixed["spmv"]["gbs"], 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(mixed["spmv"]["gbs"] / HBM_PEAK_GBS, 4),
+                        "kernel": "k_spmv_p (batch)", "solve_avg_gbs": mixed["achieved_gbs"]},
+           "mixed_vs_fp64": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3) if len(results) > 1 else None,
+           "mixed_vs_fp64_one_at_a_time": round(mixed["solves_per_s"] / single["solves_per_s"], 3)
+           if single else None,
+           "mixed_error_vs_fp64": err}
+    sb.close()
+    return out
+
+
+def bench_batched_only(args):
+    """The batch alone (config 4), as its own JSON line."""
+    L, B, device, tune, workloads = _kc()
+    specs = workloads.batch_specs(args.nsys)
+    t = time.perf_counter()
+    cases = B.build_cases(specs)          # forks builders: before the GPU is touched
+    log(f"built {len(cases)} host cases in {time.perf_counter() - t:.1f}s: {B.specs_summary(specs)}")
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline_batch(sb, specs, args.precond, args.cpu_budget)
-    single = fp64.get("one_at_a_time")
+        cores, model = cpu_info()
+        workers = max(1, min(args.cpu_workers, cores))
+        pool = cpu_batch_pool(cases, 16, workers)
+        cpu = dict(value=pool["solves_per_s_ncore"], unit="solves/s", cores=workers, kind="reference path",
+                   value_1core=pool["solves_per_s_1core"], cpu_model=model, pool=pool,
+                   sample=f"every 16th system of the batch, spilu + scipy bicgstab, {workers} processes")
+    ctx = L.default_context()
+    info = ctx.info()
+    copy_gbs = ctx.stream_copy_gbs(1 << 30, 10)
+    b = run_batched(args, ctx, cases, steps=args.steps)
+    roof = dict(b["roofline"], stream_copy_gbs=round(copy_gbs, 1), traffic=None)
     line = {
         "metric": "solves/s + %HBM roofline at rel-resid 1e-8; p50/p99 latency, mixed vs FP64",
-        "value": mixed["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": mixed["ms_per_step"], "higher_is_better": True,
+        "value": b["mixed"]["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": b["mixed"]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32 inner / f64 refinement",
         "data": "synthetic",
-        "config": {"workload": f"{len(sb.mats)} uneven systems in one batch (50% quad conv-diff, "
+        "config": {"workload": f"{b['systems']} uneven systems in one batch (50% quad conv-diff, "
                                f"30% hex pressure, 20% tet two-fluid; cells log-uniform 2e3..2.5e5), "
-                               f"BiCGSTAB+{args.precond}, rtol 1e-8", "rows": sb.total_rows,
-                   "ilu_order": order, "step": "one batched solve of every system"},
-        "roofline": roof, "cpu_baseline": cpu, "mixed": mixed, "fp64": fp64,
-        "mixed_vs_fp64": round(mixed["solves_per_s"] / fp64["solves_per_s"], 3) if len(results) > 1 else None,
-        "mixed_vs_fp64_one_at_a_time": round(mixed["solves_per_s"] / single["solves_per_s"], 3)
-        if single else None,
-        "mixed_error_vs_fp64": err, "device": info["name"],
+                               f"BiCGSTAB+{args.precond}, rtol 1e-8", "rows": b["rows"],
+                   "ilu_order": b["ilu_order"], "step": "one batched solve of every system"},
+        "roofline": roof, "cpu_baseline": cpu, "batch": b, "device": info["name"],
     }
     print(json.dumps(line))
 
@@ -456,30 +655,6 @@ def single_fp64(sb, args, choice, log):
                 p50_ms=round(1e3 * float(np.percentile(walls, 50)), 3),
                 p99_ms=round(1e3 * float(np.percentile(walls, 99)), 3),
                 iters_mean=round(float(np.mean(iters)), 1))
-
-
-def cpu_baseline_batch(sb, specs, precond, budget_s):
-    """The reference CPU path (scipy bicgstab + the oracle preconditioner, 1
-    thread) on systems of the batch taken in order until the budget is
-    spent; the batch rate is extrapolated by cell count."""
-    from oracle import solve as osolve
-    t_all, cells, done = 0.0, 0, 0
-    for k, m in enumerate(sb.mats):
-        if t_all > budget_s:
-            break
-        A = m.to_scipy()
-        b = sb.bs[k].download()
-        t0 = time.perf_counter()
-        osolve.solve_bicgstab(A, b, rtol=1e-8, precond="ilu0_struct" if precond == "ilu0" else "jacobi")
-        t_all += time.perf_counter() - t0
-        cells += m.n
-        done += 1
-    rate = cells / t_all                       # cells per second
-    est = sb.total_rows / rate                 # seconds for the whole batch
-    return {"value": len(sb.mats) / est, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": (f"{done} of {len(sb.mats)} systems ({cells} cells) solved by scipy bicgstab + "
-                       f"oracle {precond} in {t_all:.1f} s; batch rate extrapolated by cell count; 1 thread")}
-
 
 
 def bench_serve(args):

@@ -630,3 +630,53 @@ def drag_coefficient_implicit(alpha_g, rho_l, u_g, u_l, d_b, mu_l):
     mag = np.maximum(np.sqrt(np.sum(ur ** 2, axis=1)), 1e-15)
     cd = schiller_naumann_drag(rho_l * mag * d_b / mu_l)
     return 0.75 * cd * alpha_g * rho_l * mag / d_b
+
+
+def system_of_case(case):
+    """Replay a workload recipe (kcfd_amd.workloads.Case: the ordered list of
+    reference operator calls with their inputs) through this oracle; returns
+    (A csr, rhs, diag tracker).  The device plays the same recipe."""
+    mesh = case.mesh
+    s = OSystem(mesh.n_cells)
+    F_cur = None
+    for op, kw in case.recipe:
+        if op == "mass_flux":
+            F_cur = compute_mass_flux(mesh, kw["U"], kw["Ub"], kw["rho"])
+        elif op == "diffusion":
+            diffusion_operator(mesh, kw["gamma"], s)
+        elif op == "upwind":
+            convection_operator_upwind(mesh, kw["F"], s)
+        elif op == "upwind_scaled":
+            F_cur = F_cur * np.asarray(kw["scale_owner"])[mesh.owner]
+            convection_operator_upwind(mesh, F_cur, s)
+        elif op == "boundary":
+            apply_boundary_conditions(mesh, kw["phib"], kw["gamma"], kw["F"], s, kw["bc"])
+        elif op == "boundary_scaled":
+            apply_boundary_conditions(mesh, kw["phib"], kw["gamma"], F_cur, s, kw["bc"])
+        elif op == "pressure":
+            pressure_correction_operator(mesh, kw["dP"], kw["rho"], kw["dirichlet"], s)
+        elif op == "source":
+            source_term(mesh, kw["S"], s)
+        elif op == "cell_terms":
+            idx = np.arange(mesh.n_cells)
+            k = np.asarray(kw["kd"]) * mesh.cell_volume
+            s.add_entries(idx, idx, k)
+            s.add_source(idx, k * np.asarray(kw["ks"]))
+        elif op == "drag":
+            K = drag_coefficient_implicit(kw["alpha_g"], kw["rho_l"], kw["U_g"], kw["U_l"], kw["d_b"],
+                                          kw["mu_l"])
+            idx = np.arange(mesh.n_cells)
+            k = K * mesh.cell_volume
+            s.add_entries(idx, idx, k)
+            s.add_source(idx, k * np.asarray(kw["U_g"])[:, kw["comp"]])
+        elif op == "temporal":
+            temporal_operator(mesh, kw["rho"], kw["dt"], kw["phi_old"], s)
+        elif op == "muscl":
+            g = green_gauss_gradient(mesh, kw["phi"], kw["phib"])
+            c = muscl_deferred_correction(mesh, kw["phi"], kw["F"], g, kw["limiter"])
+            s.rhs = s.rhs + c
+        else:
+            raise ValueError(op)
+    A = s.to_sparse()
+    rhs = s.rhs.copy() if case.rhs is None else np.asarray(case.rhs, float).copy()
+    return A, rhs, s.diag.copy()

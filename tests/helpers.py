@@ -9,50 +9,7 @@ from oracle import fvm as ofvm
 
 def oracle_system(case):
     """Assemble ``case`` with the oracle; returns (A csr, rhs, diag tracker)."""
-    mesh = case.mesh
-    s = ofvm.OSystem(mesh.n_cells)
-    F_cur = None
-    for op, kw in case.recipe:
-        if op == "mass_flux":
-            F_cur = ofvm.compute_mass_flux(mesh, kw["U"], kw["Ub"], kw["rho"])
-        elif op == "diffusion":
-            ofvm.diffusion_operator(mesh, kw["gamma"], s)
-        elif op == "upwind":
-            ofvm.convection_operator_upwind(mesh, kw["F"], s)
-        elif op == "upwind_scaled":
-            F_cur = F_cur * np.asarray(kw["scale_owner"])[mesh.owner]
-            ofvm.convection_operator_upwind(mesh, F_cur, s)
-        elif op == "boundary":
-            ofvm.apply_boundary_conditions(mesh, kw["phib"], kw["gamma"], kw["F"], s, kw["bc"])
-        elif op == "boundary_scaled":
-            ofvm.apply_boundary_conditions(mesh, kw["phib"], kw["gamma"], F_cur, s, kw["bc"])
-        elif op == "pressure":
-            pressure_correction(mesh, kw["dP"], kw["rho"], kw["dirichlet"], s)
-        elif op == "source":
-            ofvm.source_term(mesh, kw["S"], s)
-        elif op == "cell_terms":
-            idx = np.arange(mesh.n_cells)
-            k = np.asarray(kw["kd"]) * mesh.cell_volume
-            s.add_entries(idx, idx, k)
-            s.add_source(idx, k * np.asarray(kw["ks"]))
-        elif op == "drag":
-            K = ofvm.drag_coefficient_implicit(kw["alpha_g"], kw["rho_l"], kw["U_g"], kw["U_l"], kw["d_b"],
-                                               kw["mu_l"])
-            idx = np.arange(mesh.n_cells)
-            k = K * mesh.cell_volume
-            s.add_entries(idx, idx, k)
-            s.add_source(idx, k * np.asarray(kw["U_g"])[:, kw["comp"]])
-        elif op == "temporal":
-            ofvm.temporal_operator(mesh, kw["rho"], kw["dt"], kw["phi_old"], s)
-        elif op == "muscl":
-            g = ofvm.green_gauss_gradient(mesh, kw["phi"], kw["phib"])
-            c = ofvm.muscl_deferred_correction(mesh, kw["phi"], kw["F"], g, kw["limiter"])
-            s.rhs = s.rhs + c
-        else:
-            raise ValueError(op)
-    A = s.to_sparse()
-    rhs = s.rhs.copy() if case.rhs is None else np.asarray(case.rhs, float).copy()
-    return A, rhs, s.diag.copy()
+    return ofvm.system_of_case(case)
 
 
 def pressure_correction(mesh, dP, rho, dirichlet, s):
