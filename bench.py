@@ -686,7 +686,7 @@ def bench_serve(args):
         # per-template precision: one mixed solve of each prototype matrix
         t = time.perf_counter()
         for tp in temps:
-            tp.calibrate()
+            tp.calibrate(pressure=False)
         log(f"precision calibration in {time.perf_counter() - t:.1f}s: FP64 for "
             f"{sum(len(tp.fp64_kinds) for tp in temps)} of {2 * len(temps)} (template, kind) pairs: "
             + ", ".join(f"{tp.kind}{tp.n}:{'p' if True in tp.fp64_kinds else ''}{'m' if False in tp.fp64_kinds else ''}"

@@ -10,13 +10,20 @@ replays a captured graph of ``check_every`` iterations over all members, so
 the work of requests that arrived at different times overlaps on the device.
 Systems detach as they finish.  A request completes when its four solves do.
 
-Precision is chosen per template and system kind.  A mixed solve that
-stalls (FP32 operator error of about eps_32 * cond(A) per cycle; large
-pressure matrices) continues in FP64 from its best iterate, and from then on
-the same kind of system on the same mesh template goes to the FP64 batch
-directly instead of repeating the mixed attempt.  ``Template.calibrate``
-makes that choice before serving: one mixed solve of each prototype matrix
-(seeded right-hand side) marks the kinds that stall.
+The pressure-correction system is symmetric positive definite (a
+symmetric M-matrix, whose ILU(0) is a symmetric positive-definite
+preconditioner), and the reference solves it with CG
+(solve_pressure_correction, core/linear_solver.py:73-93); by default
+(``pressure_method='cg'``) the server does the same, in FP64, in a CG batch
+of its own.  BiCGSTAB with the two-colour ILU(0) stagnates on the largest
+one-sided-Dirichlet pressure systems, in FP32 and even in FP64, where CG
+converges.  The momentum solves run in mixed precision.  A mixed solve that
+stalls (FP32 operator error of about eps_32 * cond(A) per cycle) continues
+in FP64 from its best iterate, and from then on the same kind of system on
+the same mesh template goes to the FP64 batch directly instead of repeating
+the mixed attempt.  ``Template.calibrate`` makes that choice before serving:
+one mixed solve of each prototype matrix (seeded right-hand side) marks the
+kinds that stall.
 
 Admission is by HBM footprint: a request is admitted only while the bytes
 of the admitted requests (matrices, preconditioners, vectors, right-hand
@@ -26,7 +33,8 @@ from admission to completion.
 
 The reference solves one system per call (core/linear_solver.py:14-62);
 ``Server(batched=False, precision="fp64")`` is that one-at-a-time mode on
-the GPU, the baseline the batched server is compared with.
+the GPU (pressure by CG as well), the baseline the batched server is
+compared with.
 """
 
 from __future__ import annotations
@@ -147,15 +155,20 @@ class Template:
         self.fp64_kinds = set()
         self.precond, self.precision = precond, precision
 
-    def calibrate(self, rtol=1e-8, seed=0, check_every=8):
+    def calibrate(self, rtol=1e-8, seed=0, check_every=8, pressure=True):
         """One mixed solve of each prototype matrix with a seeded right-hand
-        side; a kind whose solve does not converge in mixed precision is
-        marked for FP64.  Returns the solve results."""
+        side (of the momentum matrix only when ``pressure`` is False: the
+        pressure system then goes to CG); a kind whose solve does not
+        converge in mixed precision is marked for FP64.  Returns the solve
+        results."""
         out = []
         if self.precision != "mixed":
             return out
         rng = np.random.default_rng(seed)
-        for pressure, A in zip((True, False), self.proto):
+        for kind_p, A in zip((True, False), self.proto):
+            if kind_p and not pressure:
+                continue
+            pressure_ = kind_p
             b = DevBuf.from_array(A.ctx, rng.uniform(-1.0, 1.0, self.n))
             x = DevBuf(A.ctx, self.n)
             x.fill(0.0)
@@ -163,8 +176,8 @@ class Template:
                                        check_every=check_every)
             r = A.solve_dev(b.p, x.p, opts, setup=False, fallback=False)
             if r.info != 0:
-                self.fp64_kinds.add(pressure)
-            out.append((pressure, r.info, r.iters))
+                self.fp64_kinds.add(pressure_)
+            out.append((pressure_, r.info, r.iters))
             b.free()
             x.free()
         return out
@@ -266,7 +279,7 @@ class Server:
 
     def __init__(self, ctx, templates, precision="mixed", precond="ilu0", order="color",
                  batched=True, capacity=2048, hbm_budget=None, rtol=1e-8, check_every=8,
-                 grid_per_cu=0, trsv_per_cu=0, delta=0.1, log=None):
+                 grid_per_cu=0, trsv_per_cu=0, delta=0.1, log=None, pressure_method="cg"):
         self.ctx = ctx
         self.templates = templates
         self.precision, self.precond, self.order = precision, precond, order
@@ -285,6 +298,11 @@ class Server:
                                                   L.ctypes.byref(h)))
             self.batch = h
         self.batch64 = None
+        self.batch_cg = None
+        self.pressure_cg = pressure_method == "cg"
+        self.opts_cg = device.default_opts(rtol=rtol, precond=precond, precision="fp64", method="cg",
+                                           check_every=check_every, grid_per_cu=grid_per_cu,
+                                           trsv_per_cu=trsv_per_cu)
         self.fallbacks = 0
         self.direct64 = 0
         # (template, system kind) whose mixed solve stalled: later systems of
@@ -321,8 +339,17 @@ class Server:
             self.batch64 = h
         return self.batch64
 
+    def _cg_batch(self):
+        """The FP64 CG batch of the pressure-correction solves."""
+        if self.batch_cg is None:
+            h = L.ctypes.c_void_p()
+            check(lib().kcfd_batch_create_dynamic(self.ctx.h, self.capacity, L.ctypes.byref(self.opts_cg),
+                                                  L.ctypes.byref(h)))
+            self.batch_cg = h
+        return self.batch_cg
+
     def close(self):
-        for name in ("batch", "batch64"):
+        for name in ("batch", "batch64", "batch_cg"):
             h = getattr(self, name, None)
             if h is not None:
                 lib().kcfd_batch_destroy(h)
@@ -366,7 +393,8 @@ class Server:
                 peak_reqs = max(peak_reqs, 1)
                 peak_bytes = max(peak_bytes, req.template.bytes)
                 for k, (A, b, x) in enumerate(req.systems()):
-                    r = A.solve_dev(b.p, x.p, self.opts, setup=False)
+                    o = self.opts_cg if (k == 0 and self.pressure_cg) else self.opts
+                    r = A.solve_dev(b.p, x.p, o, setup=False)
                     req.results[k] = (r.info, r.iters, r.refinements, r.rel_resid)
                 self._finish(req, time.perf_counter() - t0)
                 done.append(req)
@@ -380,7 +408,10 @@ class Server:
                 self._admit(req, time.perf_counter() - t0)
                 for k, (A, b, x) in enumerate(req.systems()):
                     slot = L.ctypes.c_int()
-                    if self.precision == "mixed" and ((k == 0) in req.template.fp64_kinds or
+                    if k == 0 and self.pressure_cg:
+                        check(lib().kcfd_batch_attach(self._cg_batch(), A.h, b.p, x.p, L.ctypes.byref(slot)))
+                        owner64[("cg", slot.value)] = (req, k, None)
+                    elif self.precision == "mixed" and ((k == 0) in req.template.fp64_kinds or
                                                       (id(req.template), k == 0) in self.force64):
                         check(lib().kcfd_batch_attach(self._fp64_batch(), A.h, b.p, x.p,
                                                       L.ctypes.byref(slot)))
@@ -423,12 +454,16 @@ class Server:
                         self.force64.add((id(req.template), k == 0))
                     else:
                         finished.append((req, k, r))
-            if owner64:
-                check(lib().kcfd_batch_step(self.batch64, fin, L.ctypes.byref(nfin)))
+            # the FP64 batches: pressure by CG, and FP64 BiCGSTAB (stalled or
+            # FP64-routed mixed solves, or every solve of an FP64 server)
+            for tag, bh in (("cg", self.batch_cg), (None, self.batch64)):
+                if bh is None or not any(isinstance(key, tuple) == (tag == "cg") for key in owner64):
+                    continue
+                check(lib().kcfd_batch_step(bh, fin, L.ctypes.byref(nfin)))
                 for j in range(nfin.value):
-                    slot = fin[j]
-                    req, k, r1 = owner64.pop(slot)
-                    check(lib().kcfd_batch_detach(self.batch64, slot, L.ctypes.byref(res)))
+                    key = (tag, fin[j]) if tag else fin[j]
+                    req, k, r1 = owner64.pop(key)
+                    check(lib().kcfd_batch_detach(bh, fin[j], L.ctypes.byref(res)))
                     if r1 is None:        # FP64 from the start
                         finished.append((req, k, (res.info, res.iters, 0, res.rel_resid)))
                     else:

@@ -28,7 +28,9 @@ def _trace(n, rate, ntemp, seed=1):
 
 def test_serve_mix_equals_alone(ctx, templates):
     """Requests that overlap on the device give exactly the solutions (and
-    iteration counts) they give when each system is solved alone."""
+    iteration counts) they give when each system is solved alone: the
+    pressure corrections by CG in FP64, the momentum solves by
+    mixed-precision BiCGSTAB."""
     from kcfd_amd import serve as S
     trace = _trace(24, 2000.0, len(templates))
     srv = S.Server(ctx, templates, precision="mixed", precond="ilu0", batched=True, check_every=4)
@@ -47,7 +49,9 @@ def test_serve_mix_equals_alone(ctx, templates):
             req = S.Request(rid, 0.0, templates[ti], seed)
             srv._admit(req, 0.0)
             for k, (A, b, x) in enumerate(req.systems()):
-                r = A.solve_dev(b.p, x.p, srv.opts, setup=False)
+                # the pressure correction is solved by CG (FP64), the momentum
+                # components by mixed-precision BiCGSTAB
+                r = A.solve_dev(b.p, x.p, srv.opts_cg if k == 0 else srv.opts, setup=False)
                 xs_mix, res_mix = srv.solutions[rid]
                 assert r.iters == res_mix[k][1] and r.info == res_mix[k][0] == 0
                 assert np.array_equal(x.download(), xs_mix[k]), (rid, k)
@@ -86,12 +90,14 @@ def test_serve_one_at_a_time_fp64(ctx, templates):
 
 
 def test_serve_routes_stalled_kind_to_fp64(ctx, templates):
-    """A (template, system kind) marked as stalling in mixed precision is
-    solved in the FP64 batch from the start; its results equal the FP64
-    solve of the same system alone, and the other systems stay mixed."""
+    """With every system solved by BiCGSTAB (pressure_method='bicgstab'), a
+    (template, system kind) marked as stalling in mixed precision is solved
+    in the FP64 batch from the start; its results equal the FP64 solve of
+    the same system alone, and the other systems stay mixed."""
     from kcfd_amd import device, serve as S
     trace = [(0.0, 0, 7100 + i) for i in range(3)] + [(0.0, 1, 7200)]
-    srv = S.Server(ctx, templates, precision="mixed", precond="ilu0", batched=True, check_every=4)
+    srv = S.Server(ctx, templates, precision="mixed", precond="ilu0", batched=True, check_every=4,
+                   pressure_method="bicgstab")
     srv.keep_solutions = True
     srv.force64.add((id(templates[0]), True))      # template 0's pressure solves
     try:
