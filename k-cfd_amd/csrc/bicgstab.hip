@@ -18,10 +18,11 @@
 //
 // Per iteration (Jacobi folded into the gathers; ILU(0) adds two sync-free
 // triangular solves after K1 and after K3):
-//   K1 p = r + beta (p - omega v)        (p = r at iteration 0)
+//   K1 p = r + beta (p - omega v)        (p = r at iteration 0; with ILU(0)
+//                                         the p apply forms p as it reads it)
 //   K2 v = A M p; rt.v           F2: alpha
-//   K3 s = r - alpha v; ||s||
-//   K4 t = A M s; t.s, t.t       F4: half-step exit on ||s|| (scipy's test,
+//   K3 s = r - alpha v                   (with ILU(0): formed by the s apply)
+//   K4 t = A M s; t.s, t.t, ||s||^2  F4: half-step exit on ||s|| (scipy's test,
 //                                    made one finalize later), else omega
 //   K5 x += alpha M p; x += omega M s; r = s - omega t; ||r||, rt.r
 //      (half-step exit: x += alpha M p only)
@@ -520,7 +521,8 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
             for (int r = 0; r < R; ++r)
                 if (tr.ok[r]) glw(d.xb)[tr.ix[r]] = xv[r];
         }
-        if (status == S_RUN || status == S_RESUME) {
+        // ILU(0): the p apply forms p itself (PSrc)
+        if (PC != KCFD_PC_ILU0 && (status == S_RUN || status == S_RESUME)) {
             KCFD_GLOBAL V *p = vec<V>(d.p);
             const KCFD_GLOBAL V *vv = vec<V>(d.v), *rr = vec<V>(d.r);
             const V om = (V)st.omega, be = (V)st.beta;
@@ -538,7 +540,7 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
                     pi = pi * be;
                     p[tr.ix[r]] = pi + rv[r];
                 }
-        } else if (status == S_START) {
+        } else if (PC != KCFD_PC_ILU0 && status == S_START) {
             V rv[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) rv[r] = vec<V>(d.r)[tr.ix[r]];
@@ -702,15 +704,13 @@ __global__ __launch_bounds__(kFinBlock) void f_spmv_p(const SysDesc *__restrict_
 }
 
 // -------------------------------------------------------------------- K3
-// s = r - alpha v; ||s||^2 (partial slot 2, read by F4).
+// s = r - alpha v (Jacobi / none; with ILU(0) the s apply forms s itself).
+// ||s||^2 is summed by K4 over the same rows in the same order.
 template <typename V, int PC, int R>
 __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
                                               const SysState *__restrict__ sts,
-                                              const int2 *__restrict__ tiles, const int *__restrict__ counts,
-                                              double *__restrict__ partials,
-                                              unsigned int *__restrict__ counters) {
+                                              const int2 *__restrict__ tiles, const int *__restrict__ counts) {
 #pragma clang fp contract(off)
-    if (blockIdx.x == 0 && threadIdx.x < 2) counters[2 + threadIdx.x] = 0u;
     const int ntiles = counts[0];   // tiles of the current membership
     KCFD_TILE_LOOP(tile, ntiles) {
         const int2 ti = tiles[tile];
@@ -725,20 +725,14 @@ __global__ __launch_bounds__(kBlock) void k_s(const SysDesc *__restrict__ sys,
             rv[r] = vec<V>(d.r)[tr.ix[r]];
             vl[r] = vec<V>(d.v)[tr.ix[r]];
         }
-        double v[1] = {0.0};
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            if (tr.ok[r]) {
-                const V si = rv[r] - al * vl[r];
-                vec<V>(d.s)[tr.ix[r]] = si;
-                v[0] += (double)si * (double)si;
-            }
-        write_partials<1>(v, partials, tile, 2);   // ||s||^2 beside K4's t.s, t.t
+            if (tr.ok[r]) vec<V>(d.s)[tr.ix[r]] = rv[r] - al * vl[r];
     }
 }
 
 // -------------------------------------------------------------------- K4
-// t = A M s; t.s, t.t.
+// t = A M s; t.s, t.t, ||s||^2.
 template <typename V, int PC, int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_spmv_s(const SysDesc *__restrict__ sys,
                                                    const SysState *__restrict__ sts,
@@ -752,7 +746,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         const SysState &st = sts[ti.x];
         if (st.status != S_RUN || d.n <= 0) continue;
         const TileRows<R> tr(ti.y, d.n);
-        double v[2] = {0.0, 0.0};
+        double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < R; ++r) {   // one row after the other, as in K2
             const double a = row_amz<V, PC>(d, tr.ix[r], (const V *)d.s, (const V *)d.sh);
@@ -762,9 +756,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                 vec<V>(d.t)[tr.ix[r]] = tv;
                 v[0] += (double)tv * sv;
                 v[1] += (double)tv * (double)tv;
+                v[2] += sv * sv;
             }
         }
-        write_partials<2>(v, partials, tile);
+        write_partials<3>(v, partials, tile);
     }
 }
 
@@ -1244,6 +1239,40 @@ __global__ __launch_bounds__(kBlock) void k_perm_out(const SysDesc *__restrict__
     }
 }
 
+// Operands of the ILU(0) applies.  PlainSrc reads a vector.  The p and s
+// applies form their operand where they read it, with scipy's operations
+// (the p-update and the s-update of the BiCGSTAB loop), so p and s are not
+// a pass of their own; the own row's value is stored by the apply.
+template <typename V>
+struct PlainSrc {
+    const KCFD_GLOBAL V *__restrict__ a;
+    __device__ __forceinline__ V operator()(int c) const { return a[c]; }
+};
+// p = r (iteration 0) or p = (p - omega v) beta + r, from the previous p
+template <typename V>
+struct PSrc {
+    const KCFD_GLOBAL V *__restrict__ p, *__restrict__ v, *__restrict__ r;
+    V om, be;
+    bool start;
+    __device__ __forceinline__ V operator()(int c) const {
+#pragma clang fp contract(off)
+        if (start) return r[c];
+        V pi = p[c] - om * v[c];
+        pi = pi * be;
+        return pi + r[c];
+    }
+};
+// s = r - alpha v
+template <typename V>
+struct SSrc {
+    const KCFD_GLOBAL V *__restrict__ r, *__restrict__ v;
+    V al;
+    __device__ __forceinline__ V operator()(int c) const {
+#pragma clang fp contract(off)
+        return r[c] - al * v[c];
+    }
+};
+
 // ---------------------------------------------------- ILU(0) trisolves
 //
 // Sync-free level-scheduled triangular solves: one launch per solve covers
@@ -1349,9 +1378,22 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
             const int o0 = uniform(gl(off)[ch.local]);
             const int w = uniform((gl(off)[ch.local + 1] - o0) >> 6);
             if (!UPPER) {
-                const V *z = (const V *)(WHICH == 0 ? d.p : (WHICH == 1 ? d.s : d.r));
+                // the own operand: p and s are formed here (and stored), as in
+                // the two-colour apply
+                V zi;
+                const SysState &st = sts[ch.sys];
+                if constexpr (WHICH == 0) {
+                    zi = PSrc<V>{vec<V>(d.p), vec<V>(d.v), vec<V>(d.r), (V)st.omega, (V)st.beta,
+                                 status == S_START}(i);
+                    vec<V>(d.p)[i] = zi;
+                } else if constexpr (WHICH == 1) {
+                    zi = SSrc<V>{vec<V>(d.r), vec<V>(d.v), (V)st.alpha}(i);
+                    vec<V>(d.s)[i] = zi;
+                } else {
+                    zi = vec<V>(d.r)[i];
+                }
                 V *y = (V *)d.y;
-                const V acc = sub_ready<V>(gl(z)[i], fac_l(d), o0 + lane, w, y, limit, fail);
+                const V acc = sub_ready<V>(zi, fac_l(d), o0 + lane, w, y, limit, fail);
                 publish<V>(y + i, acc);
             } else {
                 V *y = (V *)d.y;
@@ -1369,31 +1411,30 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
 // row_amz: fixed groups for the common widths, else groups of 8 whose loads
 // past the width re-load the last entry; padding (column -1) gathers entry
 // 0.  Neither is accumulated.
-template <typename V, int G, typename F>
+template <typename V, int G, typename F, typename S>
 __device__ __forceinline__ void sub_group(V &acc, const F &f, int base, int k0, int last, int valid,
-                                          const KCFD_GLOBAL V *__restrict__ vec) {
+                                          const S &src) {
 #pragma clang fp contract(off)
     int c[G];
     V av[G], xv[G];
 #pragma unroll
     for (int k = 0; k < G; ++k) fac_get<V>(f, base + 64 * min(k0 + k, last), c[k], av[k]);
 #pragma unroll
-    for (int k = 0; k < G; ++k) xv[k] = vec[max(c[k], 0)];
+    for (int k = 0; k < G; ++k) xv[k] = src(max(c[k], 0));
 #pragma unroll
     for (int k = 0; k < G; ++k)
         if (k < valid && c[k] >= 0) acc -= av[k] * xv[k];
 }
 
-template <typename V, typename F>
-__device__ __forceinline__ V sub_plain(V acc, const F &f, int base, int w,
-                                       const KCFD_GLOBAL V *__restrict__ vec) {
+template <typename V, typename F, typename S>
+__device__ __forceinline__ V sub_plain(V acc, const F &f, int base, int w, const S &src) {
     if (w == 6) {
-        sub_group<V, 6>(acc, f, base, 0, 5, 6, vec);
+        sub_group<V, 6>(acc, f, base, 0, 5, 6, src);
     } else if (w == 4) {
-        sub_group<V, 4>(acc, f, base, 0, 3, 4, vec);
+        sub_group<V, 4>(acc, f, base, 0, 3, 4, src);
     } else {
         constexpr int G = 8;
-        for (int kb = 0; kb < w; kb += G) sub_group<V, G>(acc, f, base, kb, w - 1, w - kb, vec);
+        for (int kb = 0; kb < w; kb += G) sub_group<V, G>(acc, f, base, kb, w - 1, w - kb, src);
     }
     return acc;
 }
@@ -1409,6 +1450,33 @@ __device__ __forceinline__ V sub_plain(V acc, const F &f, int base, int w,
 // chunk list is cut into one contiguous range per XCD (blockIdx % 8), whose
 // waves take its chunks round-robin: a range's gathers stay in its XCD's
 // L2.  Launched on the persistent grid (a multiple of 8 blocks).
+// One row i of a two-colour apply.  The own operand z_i comes from src and
+// is stored to zout (the p or s vector) when given; FIRST gathers the
+// colour-0 operands from src, SECOND the colour-1 results from w.  The
+// factor layout is chosen once per chunk (wave-uniform), so each form is
+// straight-line code.
+template <typename V, bool SECOND, typename S>
+__device__ __forceinline__ void ilu2_row(const SysDesc &d, const Chunk &ch, int i, int o0, int wd, const S &src,
+                                         KCFD_GLOBAL V *__restrict__ zout, KCFD_GLOBAL V *__restrict__ w) {
+#pragma clang fp contract(off)
+    const V zi = src(i);
+    if (zout) zout[i] = zi;
+    const PlainSrc<V> wsrc{w};
+    V v;
+    const unsigned short *o16 = nullptr;
+    if constexpr (sizeof(V) == sizeof(float)) o16 = SECOND ? d.chu_o16 : d.chl_o16;
+    if (o16) {
+        const FacRef16 f{SECOND ? d.chu_v32 : d.chl_v32, o16,
+                         uniform(gl(SECOND ? d.chu_base : d.chl_base)[ch.local])};
+        v = SECOND ? sub_plain<V>(zi, f, o0 + (i - ch.start), wd, wsrc)
+                   : sub_plain<V>(zi, f, o0 + (i - ch.start), wd, src);
+    } else {
+        v = SECOND ? sub_plain<V>(zi, fac_u(d), o0 + (i - ch.start), wd, wsrc)
+                   : sub_plain<V>(zi, fac_l(d), o0 + (i - ch.start), wd, src);
+    }
+    w[i] = v / gl(udiag<V>(d))[i];
+}
+
 template <typename V, bool SECOND, int WHICH>
 __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
@@ -1429,24 +1497,22 @@ __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys
                                      : (WHICH == 1 ? status == S_RUN : (status == S_RUN || status == S_START));
         if (want && lane < ch.count) {
             const int i = ch.start + lane;       // row of P A P^T
-            const KCFD_GLOBAL V *__restrict__ z = vec<V>(WHICH == 0 ? d.p : (WHICH == 1 ? d.s : d.r));
             KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 1 ? d.sh : d.ph);
             const int *off = SECOND ? d.chu_off : d.chl_off;
             const int o0 = uniform(gl(off)[ch.local]);
             const int wd = uniform((gl(off)[ch.local + 1] - o0) >> 6);
-            // the factor layout is chosen once per chunk (wave-uniform), so
-            // each form is straight-line code
-            V v;
-            const unsigned short *o16 = nullptr;
-            if constexpr (sizeof(V) == sizeof(float)) o16 = SECOND ? d.chu_o16 : d.chl_o16;
-            if (o16) {
-                const FacRef16 f{SECOND ? d.chu_v32 : d.chl_v32, o16,
-                                 uniform(gl(SECOND ? d.chu_base : d.chl_base)[ch.local])};
-                v = sub_plain<V>(z[i], f, o0 + lane, wd, SECOND ? w : z);
+            const SysState &st = sts[ch.sys];
+            if constexpr (WHICH == 0) {
+                const PSrc<V> src{vec<V>(d.p), vec<V>(d.v), vec<V>(d.r), (V)st.omega, (V)st.beta,
+                                  status == S_START};
+                ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, vec<V>(d.p), w);
+            } else if constexpr (WHICH == 1) {
+                const SSrc<V> src{vec<V>(d.r), vec<V>(d.v), (V)st.alpha};
+                ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, vec<V>(d.s), w);
             } else {
-                v = sub_plain<V>(z[i], SECOND ? fac_u(d) : fac_l(d), o0 + lane, wd, SECOND ? w : z);
+                const PlainSrc<V> src{vec<V>(d.r)};
+                ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w);
             }
-            w[i] = v / gl(udiag<V>(d))[i];
         }
     }
 }
@@ -1586,8 +1652,8 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
              b->d_partials);
     launch_k(b, 0, false, false, f_spmv_p, ns, kFinBlock, s, b->d_desc, b->d_state, b->d_partials,
              b->d_params);
-    launch_k(b, 0, false, false, k_s<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
-             b->d_partials, b->d_counters);
+    if (PC != KCFD_PC_ILU0)
+        launch_k(b, 0, false, false, k_s<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts);
     if (PC == KCFD_PC_ILU0) {
         if (b->run_two) {
             launch_k(b, 0, false, false, k_ilu2<V, false, 1>, g, kBlock, s, b->d_desc, b->d_state,
@@ -2684,13 +2750,16 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
         else if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
         else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
         else if (phase == 3) {
+            // BiCGSTAB's p apply also forms p: r, the previous p and v are
+            // read instead of p (+2 n), and p is stored (+n)
+            const double form = cg ? 0.0 : 3 * n * sv;
             if (m->two_colour)   // both kernels of the two-colour apply: L and U entries, z,
                                  // w written, the other colour's z / w gathered, u_ii
-                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv;
+                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv + form;
             else                 // the sync-free L solve (one triangle + its level list)
-                tot += MFh + 2 * n * sv;
+                tot += MFh + 2 * n * sv + form;
         }
-        else if (phase == 4) tot += 4 * n * sv;
+        else if (phase == 4) tot += (!cg && b->bp.precond == KCFD_PC_ILU0) ? 0.0 : 4 * n * sv;
     }
     *bytes = tot;
     return KCFD_OK;

@@ -60,7 +60,7 @@ def test_trace_slice_served_and_verified(served):
     from kcfd_amd import serve as S
     tlist, trd, sols, rep = served
     cells = [tlist[ti].n for _, ti, _ in trd]
-    assert rep.requests == len(trd) and rep.not_converged == 0
+    assert rep.requests == len(trd) and rep.not_converged == 0 and rep.max_rel_resid < 1e-8
     assert min(cells) < 2000 and max(cells) > 900_000          # the slice spans 1e3..1e6 cells
     mats = {}
     worst = 0.0
