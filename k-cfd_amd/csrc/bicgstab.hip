@@ -18,8 +18,7 @@
 //
 // Per iteration (Jacobi folded into the gathers; ILU(0) adds two sync-free
 // triangular solves after K1 and after K3):
-//   K1 p = r + beta (p - omega v)        (p = r at iteration 0; with ILU(0)
-//                                         the p apply forms p as it reads it)
+//   K1 p = r + beta (p - omega v)        (p = r at iteration 0)
 //   K2 v = A M p; rt.v           F2: alpha
 //   K3 s = r - alpha v                   (with ILU(0): formed by the s apply)
 //   K4 t = A M s; t.s, t.t, ||s||^2  F4: half-step exit on ||s|| (scipy's test,
@@ -521,8 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
             for (int r = 0; r < R; ++r)
                 if (tr.ok[r]) glw(d.xb)[tr.ix[r]] = xv[r];
         }
-        // ILU(0): the p apply forms p itself (PSrc)
-        if (PC != KCFD_PC_ILU0 && (status == S_RUN || status == S_RESUME)) {
+        if (status == S_RUN || status == S_RESUME) {
             KCFD_GLOBAL V *p = vec<V>(d.p);
             const KCFD_GLOBAL V *vv = vec<V>(d.v), *rr = vec<V>(d.r);
             const V om = (V)st.omega, be = (V)st.beta;
@@ -540,7 +538,7 @@ __global__ __launch_bounds__(kBlock) void k_p(const SysDesc *__restrict__ sys,
                     pi = pi * be;
                     p[tr.ix[r]] = pi + rv[r];
                 }
-        } else if (PC != KCFD_PC_ILU0 && status == S_START) {
+        } else if (status == S_START) {
             V rv[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) rv[r] = vec<V>(d.r)[tr.ix[r]];
@@ -1239,28 +1237,14 @@ __global__ __launch_bounds__(kBlock) void k_perm_out(const SysDesc *__restrict__
     }
 }
 
-// Operands of the ILU(0) applies.  PlainSrc reads a vector.  The p and s
-// applies form their operand where they read it, with scipy's operations
-// (the p-update and the s-update of the BiCGSTAB loop), so p and s are not
-// a pass of their own; the own row's value is stored by the apply.
+// Operands of the ILU(0) applies.  PlainSrc reads a vector.  The s apply
+// forms s = r - alpha v where it reads it, with scipy's operations (the
+// s-update of the BiCGSTAB loop), so s is not a pass of its own; the own
+// row's value is stored by the apply.
 template <typename V>
 struct PlainSrc {
     const KCFD_GLOBAL V *__restrict__ a;
     __device__ __forceinline__ V operator()(int c) const { return a[c]; }
-};
-// p = r (iteration 0) or p = (p - omega v) beta + r, from the previous p
-template <typename V>
-struct PSrc {
-    const KCFD_GLOBAL V *__restrict__ p, *__restrict__ v, *__restrict__ r;
-    V om, be;
-    bool start;
-    __device__ __forceinline__ V operator()(int c) const {
-#pragma clang fp contract(off)
-        if (start) return r[c];
-        V pi = p[c] - om * v[c];
-        pi = pi * be;
-        return pi + r[c];
-    }
 };
 // s = r - alpha v
 template <typename V>
@@ -1378,14 +1362,12 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
             const int o0 = uniform(gl(off)[ch.local]);
             const int w = uniform((gl(off)[ch.local + 1] - o0) >> 6);
             if (!UPPER) {
-                // the own operand: p and s are formed here (and stored), as in
-                // the two-colour apply
+                // the own operand: s is formed here (and stored), as in the
+                // two-colour apply
                 V zi;
                 const SysState &st = sts[ch.sys];
                 if constexpr (WHICH == 0) {
-                    zi = PSrc<V>{vec<V>(d.p), vec<V>(d.v), vec<V>(d.r), (V)st.omega, (V)st.beta,
-                                 status == S_START}(i);
-                    vec<V>(d.p)[i] = zi;
+                    zi = vec<V>(d.p)[i];
                 } else if constexpr (WHICH == 1) {
                     zi = SSrc<V>{vec<V>(d.r), vec<V>(d.v), (V)st.alpha}(i);
                     vec<V>(d.s)[i] = zi;
@@ -1503,9 +1485,8 @@ __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys
             const int wd = uniform((gl(off)[ch.local + 1] - o0) >> 6);
             const SysState &st = sts[ch.sys];
             if constexpr (WHICH == 0) {
-                const PSrc<V> src{vec<V>(d.p), vec<V>(d.v), vec<V>(d.r), (V)st.omega, (V)st.beta,
-                                  status == S_START};
-                ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, vec<V>(d.p), w);
+                const PlainSrc<V> src{vec<V>(d.p)};
+                ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w);
             } else if constexpr (WHICH == 1) {
                 const SSrc<V> src{vec<V>(d.r), vec<V>(d.v), (V)st.alpha};
                 ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, vec<V>(d.s), w);
@@ -2750,16 +2731,13 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
         else if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
         else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
         else if (phase == 3) {
-            // BiCGSTAB's p apply also forms p: r, the previous p and v are
-            // read instead of p (+2 n), and p is stored (+n)
-            const double form = cg ? 0.0 : 3 * n * sv;
             if (m->two_colour)   // both kernels of the two-colour apply: L and U entries, z,
                                  // w written, the other colour's z / w gathered, u_ii
-                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv + form;
+                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv;
             else                 // the sync-free L solve (one triangle + its level list)
-                tot += MFh + 2 * n * sv + form;
+                tot += MFh + 2 * n * sv;
         }
-        else if (phase == 4) tot += (!cg && b->bp.precond == KCFD_PC_ILU0) ? 0.0 : 4 * n * sv;
+        else if (phase == 4) tot += 4 * n * sv;
     }
     *bytes = tot;
     return KCFD_OK;
