@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE / WRITE_SIZE counter passes: the narrow-index FP32 SELL copy
 # (6 bytes an entry, the library's default) and the packed one (8 bytes an
 # entry, KCFD_SELL16=0)
-TRAFFIC_SPMV_P_MIXED_HEX128 = 115.2e6          # 106.7 MB read + 8.5 MB written
+TRAFFIC_SPMV_P_MIXED_HEX128 = 115.1e6          # 106.6 MB read + 8.5 MB written
 TRAFFIC_SPMV_P_MIXED_HEX128_PACKED = 153.9e6   # 145.4 MB read + 8.5 MB written
 
 
@@ -50,10 +50,16 @@ def log(*a):
 # The reference's own CPU path for the 128^3 pressure system -- scipy
 # bicgstab with spilu(A.tocsc(), fill_factor=1, drop_tol=0) as
 # core/preconditioner.py:49-54 and core/linear_solver.py:59 call them -- run
-# once to completion on the GPU box's host (1 thread): about seven minutes,
-# too long for every bench run.  Written by `bench.py --workload
+# once to completion on the MI355X host (1 thread, AMD EPYC): three minutes,
+# too long for every bench run.  Measured by `bench.py --workload
 # cpu-reference`; profiles/cpu_reference_hex128.md has the run.
-CPU_REFERENCE_HEX128 = None
+CPU_REFERENCE_HEX128 = dict(
+    value=0.005611, unit="solves/s", cores=1, kind="reference",
+    sample=("the reference's call (spilu(A.tocsc(), fill_factor=1, drop_tol=0) as a LinearOperator + "
+            "scipy bicgstab, rtol 1e-8, x0 = 0) on the 2097152-row hex 128^3 pressure system, once, to "
+            "completion: 52.5 s spilu + 125.7 s for 788 iterations = 178.2 s; 1 thread, AMD EPYC host of "
+            "the MI355X; measured by bench.py --workload cpu-reference"),
+    iters=788, s_per_solve=178.23, spilu_s=52.5, rel_resid=5.63e-9)
 
 
 def cpu_info():
@@ -77,19 +83,31 @@ def _one_thread():
     return threadpool_limits(limits=1)
 
 
-def cpu_solve(A, b, precond, reps=1):
+def cpu_solve(A, b, precond, reps=1, progress=False):
     """scipy bicgstab with the reference's call, rtol 1e-8, x0 = 0, timed to
     completion: set-up + solve, one warm-up when reps > 1, the median of
     reps runs.  precond: 'jacobi' (the reference's LinearOperator), 'spilu'
     (the reference's spilu call) or 'ilu0_struct' (the oracle's structural
-    ILU(0), the preconditioner the device runs)."""
+    ILU(0), the preconditioner the device runs).  ``progress`` logs the
+    set-up time and then the iteration count about every 30 s."""
     from oracle import solve as osolve
     times, its = [], []
+    last = [0.0]
+
+    def tick(k):
+        now = time.perf_counter()
+        if now - last[0] >= 30.0:
+            log(f"  iteration {k}, {now - t0:.0f} s")
+            last[0] = now
+
     with _one_thread():
         for r in range(reps + (1 if reps > 1 else 0)):
             t0 = time.perf_counter()
             M = osolve.make_preconditioner(A, "ilu0" if precond == "spilu" else precond)
-            x, info, it = osolve.solve_bicgstab(A, b, rtol=1e-8, M=M)
+            if progress:
+                last[0] = time.perf_counter()
+                log(f"  preconditioner set up in {last[0] - t0:.1f} s")
+            x, info, it = osolve.solve_bicgstab(A, b, rtol=1e-8, M=M, on_iter=tick if progress else None)
             dt = time.perf_counter() - t0
             if reps == 1 or r > 0:
                 times.append(dt)
@@ -206,7 +224,7 @@ def bench_cpu_reference(args):
     case = workloads.hex_pressure(args.n, seed=1002)
     A, b, _ = ofvm.system_of_case(case)
     log(f"reference CPU path on {A.shape[0]} rows: spilu + bicgstab, 1 thread ({model})")
-    r = cpu_solve(A, b, "spilu")
+    r = cpu_solve(A, b, "spilu", progress=True)
     r.update(cells=A.shape[0], nnz=int(A.nnz), cores_available=cores, cpu_model=model,
              solves_per_s=1.0 / r["s_per_solve"])
     print(json.dumps(r))

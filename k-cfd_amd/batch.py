@@ -15,15 +15,17 @@ Workflow::
     sb = SystemBatch.assemble(ctx, cases)           # device meshes, FVM assembly
     res = sb.solve(precision="mixed", precond="ilu0", order="color")
 
-``build_cases`` uses worker processes and must run before the process first
-touches the GPU (the workers are forked).
+``build_cases`` uses worker processes: forked before the process first
+touches the GPU, spawned after.
 """
 
 from __future__ import annotations
 
+import functools
 import math
 import multiprocessing as mp
 import os
+import runpy
 import time
 from concurrent.futures import ProcessPoolExecutor
 from dataclasses import dataclass, field
@@ -42,15 +44,21 @@ def _build_one(spec):
 
 def build_cases(specs, workers: int | None = None):
     """Host-side meshes and assembly recipes of ``specs`` (kind, cells, seed),
-    in order.  ``workers`` > 1 forks that many builder processes; the caller
-    must not have initialised the GPU yet (forking a HIP process is unsafe)."""
+    in order.  ``workers`` > 1 runs that many builder processes: forked while
+    this process has not touched the GPU, started fresh (``spawn``, which
+    costs each worker an interpreter start) once it has, since forking a
+    process that holds a HIP context is unsafe."""
     if workers is None:
         workers = min(16, os.cpu_count() or 1)
     if workers <= 1 or len(specs) < 4:
         return [_build_one(s) for s in specs]
-    if L._default_ctx is not None:
-        raise RuntimeError("build_cases with workers must run before the GPU context is created")
-    with ProcessPoolExecutor(workers, mp_context=mp.get_context("fork")) as ex:
+    if L._default_ctx is None:
+        pool = ProcessPoolExecutor(workers, mp_context=mp.get_context("fork"))
+    else:
+        boot = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_boot.py")
+        pool = ProcessPoolExecutor(workers, mp_context=mp.get_context("spawn"),
+                                   initializer=functools.partial(runpy.run_path, boot))
+    with pool as ex:
         return list(ex.map(_build_one, specs, chunksize=2))
 
 

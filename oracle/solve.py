@@ -177,9 +177,11 @@ def make_preconditioner(A, method: str):
     raise ValueError(f"Unknown preconditioner: {method}")
 
 
-def solve_bicgstab(A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=None, precond="jacobi", M=None):
+def solve_bicgstab(A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=None, precond="jacobi", M=None,
+                   on_iter=None):
     """scipy BiCGSTAB with the reference's call; returns (x, info, iterations)
-    where iterations = number of callback calls = completed full iterations."""
+    where iterations = number of callback calls = completed full iterations.
+    ``on_iter(k)``, when given, is called after iteration k."""
     A = sparse.csr_matrix(A)
     n = A.shape[0]
     if x0 is None:
@@ -190,6 +192,8 @@ def solve_bicgstab(A, b, x0=None, rtol=1e-8, atol=0.0, maxiter=None, precond="ja
 
     def cb(_xk):
         count[0] += 1
+        if on_iter is not None:
+            on_iter(count[0])
 
     x, info = bicgstab(A, b, x0=np.array(x0, np.float64), rtol=rtol, atol=atol,
                        maxiter=maxiter, M=M, callback=cb)
