@@ -743,7 +743,8 @@ def bench_serve(args):
         "vs_baseline": None, "dtype": "f32 inner / f64 refinement", "data": "synthetic",
         "config": {"workload": f"serving: Poisson {args.rate:.0f} req/s for {args.duration:.0f} s, "
                                f"each request pressure correction + 3 momentum solves on a 1e3..1e6-cell "
-                               f"mesh (quad/hex/tet 50/30/20), BiCGSTAB+{args.precond}, rtol 1e-8",
+                               f"mesh (quad/hex/tet 50/30/20); pressure: FP64 CG+{args.precond}, momentum: "
+                               f"mixed BiCGSTAB+{args.precond}; rtol 1e-8, every answer FP64-verified",
                    "requests": len(tr), "templates_per_kind": len(sizes),
                    "step": "the whole trace (continuous batching)"},
         "latency_ms": {"p50": mixed["p50_ms"], "p99": mixed["p99_ms"]},

@@ -263,6 +263,44 @@ int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out);
 int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms);
 int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes);
 
+/* ------------------------------------------------------ mesh builder */
+
+/* The face arrays of a cell-centred FV mesh, built on the device from its
+ * cells: build_fvmesh_from_arrays (mesh/mesh_reader.py:141-242, 2D: the
+ * edges k, k+1 of each cell) and _build_fvmesh_3d (:245-323, 3D: each
+ * cell's face node lists).  Faces are numbered in the order the reference's
+ * cell walk first meets them; the first cell to meet a face owns it and the
+ * last other cell is its neighbour (-1 on the boundary); normals point out
+ * of the owner.  Geometry: face centre = mean of its nodes, Newell normal
+ * and area (3D) or the edge normal and length (2D); cell centre = mean of
+ * its nodes; volume by the divergence theorem over the cell's face list (3D,
+ * floored at 1e-30) or the shoelace area (2D).
+ * Inputs (host): nodes n_nodes x 3 (2D coordinates padded with z = 0);
+ * ndim 2 selects the edge path, 3 the face lists; cell_nodes n_cells x cell_width
+ * (3D: -1 pads a cell with fewer nodes); 3D only: entry_nodes n_entries x
+ * face_width (3 or 4; -1 pads a triangle) listing each cell's faces in
+ * order, entry_ptr (n_cells + 1) the offsets of each cell's faces (1..8
+ * faces a cell).  Boundary lists: n_bfaces rows of face_width (2 in 2D)
+ * node ids, bface_patch[b] in 0..n_patches-1 the patch it names; a face
+ * named by several rows takes the last one's patch, an unnamed boundary
+ * face gets n_patches (the reference's "default"). */
+typedef struct kcfd_mesh_faces kcfd_mesh_faces;
+int kcfd_mesh_build(kcfd_ctx *ctx, int ndim, int n_nodes, const double *nodes, int n_cells, int cell_width,
+                    const int *cell_nodes, int n_entries, int face_width, const int *entry_nodes,
+                    const int *entry_ptr, int n_bfaces, const int *bface_nodes, const int *bface_patch,
+                    int n_patches, kcfd_mesh_faces **out);
+/* counts of a built mesh (face_width: 2 in 2D) and the device time of the
+ * build (uploads included) */
+int kcfd_mesh_faces_info(const kcfd_mesh_faces *m, int *n_faces, int *n_internal, int *face_width,
+                         double *build_ms);
+/* copies into caller arrays (any may be null): owner, neighbour (nf),
+ * face_nodes (nf x face_width, first-meeting cell's node order), face_patch
+ * (nf: -1 internal), area (nf), normal, centre (nf x 3), cell_centre
+ * (n x 3), cell_volume (n) */
+int kcfd_mesh_faces_get(const kcfd_mesh_faces *m, int *owner, int *neighbour, int *face_nodes, int *face_patch,
+                        double *area, double *normal, double *centre, double *cell_centre, double *cell_volume);
+int kcfd_mesh_faces_free(kcfd_mesh_faces *m);
+
 /* ------------------------------------------------------ mesh + assembly */
 
 /* SoA mesh on the device (replaces the AoS FVMesh consumed by the operators,

@@ -117,6 +117,11 @@ SIGNATURES = {
     "kcfd_mat_solver": ([_P, _PP], _I),
     "kcfd_batch_profile": ([_P, _I, _I, ctypes.POINTER(_D)], _I),
     "kcfd_batch_kernel_bytes": ([_P, _I, ctypes.POINTER(_D)], _I),
+    "kcfd_mesh_build": ([_P, _I, _I, _P, _I, _I, _P, _I, _I, _P, _P, _I, _P, _P, _I, _PP], _I),
+    "kcfd_mesh_faces_info": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), ctypes.POINTER(_I),
+                              ctypes.POINTER(_D)], _I),
+    "kcfd_mesh_faces_get": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
+    "kcfd_mesh_faces_free": ([_P], _I),
     "kcfd_mesh_create": ([_P, ctypes.POINTER(MeshDesc), _PP], _I),
     "kcfd_mesh_destroy": ([_P], _I),
     "kcfd_mesh_new_system": ([_P, _PP], _I),

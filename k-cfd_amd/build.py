@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libkcfd.so")
-SOURCES = ["runtime.hip", "bicgstab.hip", "ilu.hip", "assemble.hip", "gmres.hip"]
+SOURCES = ["runtime.hip", "bicgstab.hip", "ilu.hip", "assemble.hip", "gmres.hip", "meshgen.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

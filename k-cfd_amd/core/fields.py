@@ -73,13 +73,41 @@ class VectorField:
         self.mesh = mesh
         self.name = name
         nd = getattr(mesh, "ndim", 2)
-        if default is None:
-            default = np.zeros(nd)
-        default = np.asarray(default, dtype=np.float64)
+        default = np.zeros(nd) if default is None else np.asarray(default, dtype=np.float64)
+        if default.shape[0] < nd:            # a short default is zero-padded to ndim
+            default = np.concatenate([default, np.zeros(nd - default.shape[0])])
         self.values = np.tile(default, (mesh.n_cells, 1))
         self.boundary_values: Dict[str, np.ndarray] = {
             nm: np.tile(default, (len(f), 1)) for nm, f in mesh.boundary_patches.items()}
         self.old_values: Optional[np.ndarray] = None
+
+    # component views: x, y always; z reads zeros and ignores writes below 3D
+    @property
+    def x(self) -> np.ndarray:
+        return self.values[:, 0]
+
+    @x.setter
+    def x(self, val):
+        self.values[:, 0] = val
+
+    @property
+    def y(self) -> np.ndarray:
+        return self.values[:, 1]
+
+    @y.setter
+    def y(self, val):
+        self.values[:, 1] = val
+
+    @property
+    def z(self) -> np.ndarray:
+        if self.values.shape[1] < 3:
+            return np.zeros(self.values.shape[0])
+        return self.values[:, 2]
+
+    @z.setter
+    def z(self, val):
+        if self.values.shape[1] >= 3:
+            self.values[:, 2] = val
 
     def copy(self) -> "VectorField":
         vf = VectorField(self.mesh, self.name)
@@ -92,6 +120,12 @@ class VectorField:
 
     def store_old(self):
         self.old_values = self.values.copy()
+
+    def set_uniform(self, value):
+        v = np.asarray(value, dtype=np.float64)
+        self.values[:] = v
+        for nm in self.boundary_values:
+            self.boundary_values[nm][:] = v
 
     def set_boundary(self, patch_name: str, value):
         if patch_name not in self.boundary_values:

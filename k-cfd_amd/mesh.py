@@ -364,6 +364,106 @@ def build_fvmesh_from_arrays(node_coords, cell_node_list, boundary_faces_dict,
     return mesh
 
 
+def build_fvmesh_on_device(node_coords, cell_node_list, boundary_faces_dict, cell_zone_map=None,
+                           ndim: int = 2, cell_face_node_list=None, ctx=None) -> FVMesh:
+    """``build_fvmesh_from_arrays`` (mesh/mesh_reader.py:141-323) with the
+    face matching and all geometry computed on the GPU (kcfd_mesh_build):
+    the same arguments, and the same FVMesh as the host builder above, array
+    for array and bit for bit.  ``ctx``: the library context (default: the
+    process's).  The build's device time is kept as ``mesh.build_ms``."""
+    from . import _lib as L
+    ctx = ctx or L.default_context()
+    nodes = np.asarray(node_coords, dtype=np.float64)
+    nodes3 = _pad3(nodes.reshape(len(nodes), -1))
+    mode = 2
+    entries = ptr = None
+    if ndim == 3 and cell_face_node_list is not None:
+        mode = 3
+        rag = _as_ragged(cell_node_list, cell_face_node_list)
+        if rag is not None:
+            cells = np.asarray(rag.cell_nodes, dtype=np.int64)
+            entries = np.asarray(rag.faces, dtype=np.int64)
+            ptr = np.asarray(rag.face_ptr, dtype=np.int64)
+            width = 4
+        else:
+            cells = np.asarray(cell_node_list, dtype=np.int64)
+            cfn = np.asarray(cell_face_node_list, dtype=np.int64)
+            nc, fpc, width = cfn.shape
+            entries = cfn.reshape(nc * fpc, width)
+            ptr = np.arange(0, nc * fpc + 1, fpc, dtype=np.int64)
+    else:
+        cells = np.asarray(cell_node_list, dtype=np.int64)
+        width = 2
+    n = cells.shape[0]
+    names = list((boundary_faces_dict or {}).keys())
+    brows, bname = [], []
+    for ni, (name, flist) in enumerate((boundary_faces_dict or {}).items()):
+        if len(flist) == 0:
+            continue
+        rows = _face_rows(flist, width)
+        brows.append(rows)
+        bname.append(np.full(len(rows), ni, np.int64))
+    brows = np.concatenate(brows) if brows else np.empty((0, width), np.int64)
+    bname = np.concatenate(bname) if bname else np.empty(0, np.int64)
+
+    def i32(a):
+        return np.ascontiguousarray(a, dtype=np.int32)
+
+    cells32 = i32(cells)
+    ent32 = i32(entries) if entries is not None else None
+    ptr32 = i32(ptr) if ptr is not None else None
+    b32, bn32 = i32(brows), i32(bname)
+    h = L.ctypes.c_void_p()
+    L.check(L.lib().kcfd_mesh_build(
+        ctx.h, mode, len(nodes3), L.ptr(nodes3), n, cells.shape[1] if n else 1, L.ptr(cells32),
+        len(ent32) if ent32 is not None else 0, width, L.ptr(ent32) if ent32 is not None else None,
+        L.ptr(ptr32) if ptr32 is not None else None, len(b32), L.ptr(b32) if len(b32) else None,
+        L.ptr(bn32) if len(bn32) else None, len(names), L.ctypes.byref(h)))
+    try:
+        nf, nint, fw = L.ctypes.c_int(), L.ctypes.c_int(), L.ctypes.c_int()
+        ms = L.ctypes.c_double()
+        L.check(L.lib().kcfd_mesh_faces_info(h, L.ctypes.byref(nf), L.ctypes.byref(nint), L.ctypes.byref(fw),
+                                             L.ctypes.byref(ms)))
+        nf = nf.value
+        owner = np.empty(nf, np.int32)
+        neighbour = np.empty(nf, np.int32)
+        fnodes = np.empty((nf, fw.value), np.int32)
+        fpatch = np.empty(nf, np.int32)
+        area = np.empty(nf)
+        nrm = np.empty((nf, 3))
+        fc = np.empty((nf, 3))
+        cc = np.empty((n, 3))
+        vol = np.empty(n)
+        L.check(L.lib().kcfd_mesh_faces_get(h, L.ptr(owner), L.ptr(neighbour), L.ptr(fnodes), L.ptr(fpatch),
+                                            L.ptr(area), L.ptr(nrm), L.ptr(fc), L.ptr(cc), L.ptr(vol)))
+    finally:
+        L.lib().kcfd_mesh_faces_free(h)
+    mesh = FVMesh(ndim)
+    mesh.nodes = nodes
+    mesh.n_cells = n
+    mesh.owner, mesh.neighbour = owner, neighbour
+    mesh.face_area, mesh.face_normal, mesh.face_center = area, nrm, fc
+    mesh.cell_center, mesh.cell_volume = cc, vol
+    mesh.face_nodes = fnodes.astype(np.int64)
+    mesh.n_faces = nf
+    mesh.n_internal_faces = int(nint.value)
+    mesh.n_boundary_faces = nf - mesh.n_internal_faces
+    # patches in the order of their first boundary face ("default" last index)
+    bfids = np.nonzero(neighbour < 0)[0]
+    fp = fpatch[bfids]
+    all_names = names + ["default"]
+    patches = {}
+    if len(fp):
+        uniq, first = np.unique(fp, return_index=True)
+        for idx in uniq[np.argsort(first)]:
+            patches[all_names[idx]] = bfids[fp == idx].astype(np.int64)
+    mesh.boundary_patches = patches
+    if cell_zone_map:
+        mesh.cell_zones = cell_zone_map
+    mesh.build_ms = ms.value
+    return mesh
+
+
 def _as_ragged(cell_node_list, cell_face_node_list):
     """RaggedCells for mixed 3D cells, or None when the per-cell face lists
     form one uniform (n, faces, nodes) array."""
