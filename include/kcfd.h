@@ -300,6 +300,20 @@ int kcfd_mesh_faces_info(const kcfd_mesh_faces *m, int *n_faces, int *n_internal
 int kcfd_mesh_faces_get(const kcfd_mesh_faces *m, int *owner, int *neighbour, int *face_nodes, int *face_patch,
                         double *area, double *normal, double *centre, double *cell_centre, double *cell_volume);
 int kcfd_mesh_faces_free(kcfd_mesh_faces *m);
+/* The structured configuration meshes generated on the device and built as
+ * kcfd_mesh_build does, with no host arrays: ndim 2, the quads of
+ * _make_structured_quad_mesh (mesh/mesh_generator.py:13-93; nodes
+ * (ox + i dx, oy + j dy) row-major, cells counter-clockwise, sides bottom,
+ * top, left, right); ndim 3, the hexes of generate_3d_channel_mesh
+ * (mesh/mesh_generator_3d.py:12-159; nodes (i dx, j dy, k dz), cells i
+ * fastest, faces z-, z+, y-, y+, x-, x+; sides x_min, x_max, y_min, y_max,
+ * z_min, z_max; ox, oy ignored).  side_patch[4 or 6]: the patch index each
+ * side's boundary faces are listed under, -1 to list them under none. */
+int kcfd_mesh_generate(kcfd_ctx *ctx, int ndim, int nx, int ny, int nz, double ox, double oy, double dx, double dy,
+                       double dz, const int *side_patch, int n_patches, kcfd_mesh_faces **out);
+/* node and cell counts and the node coordinates (n_nodes x 3) of a built
+ * mesh; any pointer may be null */
+int kcfd_mesh_faces_nodes(const kcfd_mesh_faces *m, int *n_nodes, int *n_cells, double *nodes);
 
 /* ------------------------------------------------------ mesh + assembly */
 

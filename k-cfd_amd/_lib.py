@@ -122,6 +122,8 @@ SIGNATURES = {
                               ctypes.POINTER(_D)], _I),
     "kcfd_mesh_faces_get": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _P], _I),
     "kcfd_mesh_faces_free": ([_P], _I),
+    "kcfd_mesh_generate": ([_P, _I, _I, _I, _I, _D, _D, _D, _D, _D, _P, _I, _PP], _I),
+    "kcfd_mesh_faces_nodes": ([_P, ctypes.POINTER(_I), ctypes.POINTER(_I), _P], _I),
     "kcfd_mesh_create": ([_P, ctypes.POINTER(MeshDesc), _PP], _I),
     "kcfd_mesh_destroy": ([_P], _I),
     "kcfd_mesh_new_system": ([_P, _PP], _I),

@@ -327,111 +327,30 @@ int sort_keys(Bufs &bf, int n, unsigned long long *hi, unsigned long long *lo, i
 }  // namespace
 
 struct kcfd_mesh_faces {
-    int ndim = 0, n_cells = 0, n_faces = 0, n_internal = 0, face_width = 0;
+    int ndim = 0, n_cells = 0, n_faces = 0, n_internal = 0, face_width = 0, n_nodes = 0;
     std::vector<int> owner, neighbour, face_nodes, face_patch;
-    std::vector<double> area, normal, centre, cell_centre, cell_volume;
+    std::vector<double> area, normal, centre, cell_centre, cell_volume, nodes;
     double ms = 0.0;
 };
 
-extern "C" int kcfd_mesh_build(kcfd_ctx *ctx, int ndim, int n_nodes, const double *nodes, int n_cells,
-                               int cell_width, const int *cell_nodes, int n_entries, int face_width,
-                               const int *entry_nodes_h, const int *entry_ptr, int n_bfaces,
-                               const int *bface_nodes, const int *bface_patch, int n_patches,
-                               kcfd_mesh_faces **out) {
-    KCFD_ARG(ctx && out, "null argument");
-    KCFD_ARG(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
-    KCFD_ARG(n_nodes >= 0 && n_cells >= 0 && n_bfaces >= 0 && n_patches >= 0, "negative size");
-    KCFD_ARG(cell_width >= 1 && cell_width <= 8, "cell_width must be 1..8");
-    KCFD_ARG(n_cells == 0 || (nodes && cell_nodes), "null mesh array");
-    KCFD_ARG(n_bfaces == 0 || (bface_nodes && bface_patch), "null boundary array");
-    long long E;
-    int fw;
-    if (ndim == 2) {
-        KCFD_ARG(cell_width >= 3, "a 2D cell needs at least 3 nodes");
-        E = (long long)n_cells * cell_width;
-        fw = 2;
-    } else {
-        KCFD_ARG(face_width >= 3 && face_width <= 4, "3D faces have 3 or 4 nodes");
-        KCFD_ARG(n_cells == 0 || (entry_nodes_h && entry_ptr), "null face lists");
-        E = n_entries;
-        fw = face_width;
-        KCFD_ARG(n_cells == 0 || entry_ptr[0] == 0, "entry_ptr[0] must be 0");
-        for (int c = 0; c < n_cells; ++c) {
-            const int k = entry_ptr[c + 1] - entry_ptr[c];
-            KCFD_ARG(k >= 1 && k <= 8, "a 3D cell has 1..8 faces");
-        }
-        KCFD_ARG(n_cells == 0 || entry_ptr[n_cells] == n_entries, "entry_ptr[n] must equal n_entries");
-    }
-    KCFD_ARG(E < (1LL << 31), "too many cell faces");
-    // node ids in range (a key packs them into 31 bits)
-    {
-        const int *ids = ndim == 2 ? cell_nodes : entry_nodes_h;
-        const long long m = ndim == 2 ? (long long)n_cells * cell_width : E * fw;
-        for (long long q = 0; q < m; ++q)
-            KCFD_ARG(ids[q] < n_nodes && (ids[q] >= 0 || ndim == 3), "node id out of range");
-        for (long long q = 0; q < (long long)n_cells * cell_width; ++q)
-            KCFD_ARG(cell_nodes[q] < n_nodes, "cell node id out of range");
-        const int bw = fw;
-        for (long long q = 0; q < (long long)n_bfaces * bw; ++q)
-            KCFD_ARG(bface_nodes[q] < n_nodes, "boundary node id out of range");
-        // every cell has a node; a 3D face has at least 3 nodes, padding last
-        for (int c = 0; c < n_cells; ++c) {
-            int cnt = 0;
-            for (int k = 0; k < cell_width; ++k) cnt += cell_nodes[(size_t)c * cell_width + k] >= 0;
-            KCFD_ARG(cnt >= 1, "a cell without nodes");
-        }
-        if (ndim == 3)
-            for (long long e = 0; e < E; ++e) {
-                int cnt = 0;
-                bool tail = false, ok = true;
-                for (int i = 0; i < fw; ++i) {
-                    const bool v = entry_nodes_h[e * fw + i] >= 0;
-                    if (v && tail) ok = false;
-                    if (!v) tail = true;
-                    cnt += v;
-                }
-                KCFD_ARG(ok && cnt >= 3, "a 3D face needs at least 3 nodes, padding (-1) last");
-            }
-        for (int b = 0; b < n_bfaces; ++b)
-            KCFD_ARG(bface_patch[b] >= 0 && bface_patch[b] < n_patches, "boundary patch index out of range");
-    }
-    std::lock_guard<std::recursive_mutex> g(ctx->mu);
-    KCFD_HIP(hipSetDevice(ctx->device));
+namespace {
+
+// The build from device-resident inputs: xyz (n_nodes x 3), cells
+// (n x cw), 3D face lists (E x fw, per-cell offsets ptr), boundary rows
+// (nb x fw) and their patch indices.  Results go to r (host).  The stream's
+// event ev0 was recorded before the inputs were written.
+int build_dev(kcfd_ctx *ctx, Bufs &bf, int ndim, int n_nodes, double *xyz, int n, int cell_width, int *cells,
+              long long E, int fw, int *faces, int *ptr, int nb, int *bn, int *bname, int n_patches,
+              kcfd_mesh_faces *r) {
     hipStream_t s = ctx->stream;
-    auto *r = new kcfd_mesh_faces();
-    std::unique_ptr<kcfd_mesh_faces> own(r);
     r->ndim = ndim;
-    r->n_cells = n_cells;
+    r->n_cells = n;
     r->face_width = fw;
-    KCFD_HIP(hipEventRecord(ctx->ev0, s));
-    Bufs bf;
-    const int n = n_cells, En = (int)E;
-    double *xyz;
-    int *cells, *faces = nullptr, *ptr = nullptr;
-    KCFD_CHECK(bf.get(&xyz, (size_t)n_nodes * 3));
-    KCFD_CHECK(bf.get(&cells, (size_t)n * cell_width));
-    {
-        KCFD_HIP(hipMemcpyAsync(xyz, nodes, sizeof(double) * (size_t)n_nodes * 3, hipMemcpyHostToDevice, s));
-        KCFD_HIP(hipMemcpyAsync(cells, cell_nodes, sizeof(int) * (size_t)n * cell_width, hipMemcpyHostToDevice,
-                                s));
-        KCFD_HIP(hipStreamSynchronize(s));
-    }
+    r->n_nodes = n_nodes;
+    const int En = (int)E;
     int *ent_cell;
     KCFD_CHECK(bf.get(&ent_cell, E));
-    KCFD_CHECK(bf.get(&ptr, (size_t)n + 1));
-    std::vector<int> hp;
-    if (ndim == 3) {
-        KCFD_CHECK(bf.get(&faces, E * fw));
-        KCFD_HIP(hipMemcpyAsync(faces, entry_nodes_h, sizeof(int) * E * fw, hipMemcpyHostToDevice, s));
-        KCFD_HIP(hipMemcpyAsync(ptr, entry_ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, s));
-    } else {
-        // 2D: cell c owns entries c * cell_width .. (c + 1) * cell_width - 1
-        hp.resize((size_t)n + 1);
-        for (int c = 0; c <= n; ++c) hp[c] = c * cell_width;
-        KCFD_HIP(hipMemcpyAsync(ptr, hp.data(), sizeof(int) * hp.size(), hipMemcpyHostToDevice, s));
-    }
     if (n) hipLaunchKernelGGL(k_mb_entry_cell, dim3(mb_grid(n)), dim3(kMbBlock), 0, s, n, ptr, ent_cell);
-    KCFD_HIP(hipStreamSynchronize(s));
     unsigned long long *hi, *lo, *hs, *ls;
     int *idx, *is;
     KCFD_CHECK(bf.get(&hi, E));
@@ -499,20 +418,14 @@ extern "C" int kcfd_mesh_build(kcfd_ctx *ctx, int ndim, int n_nodes, const doubl
     // boundary patches
     if (nf) {
         unsigned long long *bhi, *blo, *bhs, *bls;
-        int *bidx, *bis, *bname;
-        const int nb = n_bfaces;
+        int *bidx, *bis;
         KCFD_CHECK(bf.get(&bhi, nb));
         KCFD_CHECK(bf.get(&blo, nb));
         KCFD_CHECK(bf.get(&bidx, nb));
-        KCFD_CHECK(bf.get(&bname, nb));
         bhs = bhi;
         bls = blo;
         bis = bidx;
         if (nb) {
-            int *bn;
-            KCFD_CHECK(bf.get(&bn, (size_t)nb * fw));
-            KCFD_HIP(hipMemcpyAsync(bn, bface_nodes, sizeof(int) * (size_t)nb * fw, hipMemcpyHostToDevice, s));
-            KCFD_HIP(hipMemcpyAsync(bname, bface_patch, sizeof(int) * nb, hipMemcpyHostToDevice, s));
             hipLaunchKernelGGL(k_mb_bkeys, dim3(mb_grid(nb)), dim3(kMbBlock), 0, s, nb, bn, fw, bhi, blo, bidx);
             KCFD_CHECK(sort_keys(bf, nb, bhi, blo, bidx, s, &bhs, &bls, &bis));
         }
@@ -530,6 +443,7 @@ extern "C" int kcfd_mesh_build(kcfd_ctx *ctx, int ndim, int n_nodes, const doubl
     r->centre.resize((size_t)nf * 3);
     r->cell_centre.resize((size_t)n * 3);
     r->cell_volume.resize(n);
+    r->nodes.resize((size_t)n_nodes * 3);
     KCFD_HIP(hipMemcpyAsync(r->owner.data(), owner, sizeof(int) * nf, hipMemcpyDeviceToHost, s));
     KCFD_HIP(hipMemcpyAsync(r->neighbour.data(), neigh, sizeof(int) * nf, hipMemcpyDeviceToHost, s));
     KCFD_HIP(hipMemcpyAsync(r->face_nodes.data(), fnodes, sizeof(int) * (size_t)nf * fw, hipMemcpyDeviceToHost, s));
@@ -539,6 +453,7 @@ extern "C" int kcfd_mesh_build(kcfd_ctx *ctx, int ndim, int n_nodes, const doubl
     KCFD_HIP(hipMemcpyAsync(r->centre.data(), fc, sizeof(double) * (size_t)nf * 3, hipMemcpyDeviceToHost, s));
     KCFD_HIP(hipMemcpyAsync(r->cell_centre.data(), cc, sizeof(double) * (size_t)n * 3, hipMemcpyDeviceToHost, s));
     KCFD_HIP(hipMemcpyAsync(r->cell_volume.data(), vol, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    KCFD_HIP(hipMemcpyAsync(r->nodes.data(), xyz, sizeof(double) * (size_t)n_nodes * 3, hipMemcpyDeviceToHost, s));
     KCFD_HIP(hipStreamSynchronize(s));
     float ms = 0.f;
     KCFD_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -546,7 +461,275 @@ extern "C" int kcfd_mesh_build(kcfd_ctx *ctx, int ndim, int n_nodes, const doubl
     int ni = 0;
     for (int f = 0; f < nf; ++f) ni += r->neighbour[f] >= 0;
     r->n_internal = ni;
-    *out = own.release();
+    return KCFD_OK;
+}
+
+// ---- structured generators: mesh/mesh_generator.py:13-93 (quads, nodes
+// row-major, cells counter-clockwise) and mesh/mesh_generator_3d.py:12-159
+// (hexes, node k (ny+1)(nx+1) + j (nx+1) + i, cells i fastest, faces z-, z+,
+// y-, y+, x-, x+); node coordinates o + i d, as the generators compute them
+
+__global__ void k_gen_nodes(int nx, int ny, int nz, double ox, double oy, double dx, double dy, double dz,
+                            long long nn, double *xyz) {
+    const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nn) return;
+    const long long px = nx + 1, py = ny + 1;
+    const int i = (int)(v % px), j = (int)((v / px) % py), k = (int)(v / (px * py));
+    xyz[v * 3 + 0] = ox + (double)i * dx;
+    xyz[v * 3 + 1] = oy + (double)j * dy;
+    xyz[v * 3 + 2] = nz > 0 ? (double)k * dz : 0.0;
+}
+
+__global__ void k_gen_quads(int nx, int ny, int *cells) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nx * ny) return;
+    const int i = c % nx, j = c / nx, n0 = j * (nx + 1) + i;
+    int *o = cells + (size_t)c * 4;
+    o[0] = n0;
+    o[1] = n0 + 1;
+    o[2] = n0 + 1 + (nx + 1);
+    o[3] = n0 + (nx + 1);
+}
+
+__device__ __forceinline__ int hid(int nx, int ny, int i, int j, int k) {
+    return k * (ny + 1) * (nx + 1) + j * (nx + 1) + i;
+}
+
+__global__ void k_gen_hexes(int nx, int ny, int nz, int *cells, int *faces, int *ptr) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nx * ny * nz) return;
+    const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    const int n[8] = {hid(nx, ny, i, j, k),         hid(nx, ny, i + 1, j, k),
+                      hid(nx, ny, i + 1, j + 1, k), hid(nx, ny, i, j + 1, k),
+                      hid(nx, ny, i, j, k + 1),     hid(nx, ny, i + 1, j, k + 1),
+                      hid(nx, ny, i + 1, j + 1, k + 1), hid(nx, ny, i, j + 1, k + 1)};
+    for (int q = 0; q < 8; ++q) cells[(size_t)c * 8 + q] = n[q];
+    const int fq[6][4] = {{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4}, {2, 3, 7, 6}, {0, 4, 7, 3}, {1, 2, 6, 5}};
+    for (int f = 0; f < 6; ++f)
+        for (int q = 0; q < 4; ++q) faces[((size_t)c * 6 + f) * 4 + q] = n[fq[f][q]];
+    ptr[c] = c * 6;
+    if (c == nx * ny * nz - 1) ptr[c + 1] = (c + 1) * 6;
+}
+
+// boundary rows of side `side` (quad: bottom, top, left, right; hex: x_min,
+// x_max, y_min, y_max, z_min, z_max), row q of it at out[q]
+__global__ void k_gen_bside(int ndim, int side, int nx, int ny, int nz, int rows, int patch, int *out, int *pout) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= rows) return;
+    int *o = out + (size_t)q * (ndim == 2 ? 2 : 4);
+    pout[q] = patch;
+    if (ndim == 2) {
+        const int w = nx + 1;
+        switch (side) {
+            case 0: o[0] = q; o[1] = q + 1; break;
+            case 1: o[0] = ny * w + q; o[1] = ny * w + q + 1; break;
+            case 2: o[0] = q * w; o[1] = (q + 1) * w; break;
+            default: o[0] = q * w + nx; o[1] = (q + 1) * w + nx; break;
+        }
+        return;
+    }
+    if (side < 2) {                 // (k, j), j fastest
+        const int jb = q % ny, kb = q / ny, x = side == 0 ? 0 : nx;
+        if (side == 0) {
+            o[0] = hid(nx, ny, x, jb, kb); o[1] = hid(nx, ny, x, jb, kb + 1);
+            o[2] = hid(nx, ny, x, jb + 1, kb + 1); o[3] = hid(nx, ny, x, jb + 1, kb);
+        } else {
+            o[0] = hid(nx, ny, x, jb, kb); o[1] = hid(nx, ny, x, jb + 1, kb);
+            o[2] = hid(nx, ny, x, jb + 1, kb + 1); o[3] = hid(nx, ny, x, jb, kb + 1);
+        }
+    } else if (side < 4) {          // (k, i), i fastest
+        const int ic = q % nx, kc = q / nx;
+        if (side == 2) {
+            o[0] = hid(nx, ny, ic, 0, kc); o[1] = hid(nx, ny, ic + 1, 0, kc);
+            o[2] = hid(nx, ny, ic + 1, 0, kc + 1); o[3] = hid(nx, ny, ic, 0, kc + 1);
+        } else {
+            o[0] = hid(nx, ny, ic + 1, ny, kc); o[1] = hid(nx, ny, ic, ny, kc);
+            o[2] = hid(nx, ny, ic, ny, kc + 1); o[3] = hid(nx, ny, ic + 1, ny, kc + 1);
+        }
+    } else {                        // (j, i), i fastest
+        const int id = q % nx, jd = q / nx;
+        if (side == 4) {
+            o[0] = hid(nx, ny, id, jd, 0); o[1] = hid(nx, ny, id, jd + 1, 0);
+            o[2] = hid(nx, ny, id + 1, jd + 1, 0); o[3] = hid(nx, ny, id + 1, jd, 0);
+        } else {
+            o[0] = hid(nx, ny, id, jd, nz); o[1] = hid(nx, ny, id + 1, jd, nz);
+            o[2] = hid(nx, ny, id + 1, jd + 1, nz); o[3] = hid(nx, ny, id, jd + 1, nz);
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" int kcfd_mesh_build(kcfd_ctx *ctx, int ndim, int n_nodes, const double *nodes, int n_cells,
+                               int cell_width, const int *cell_nodes, int n_entries, int face_width,
+                               const int *entry_nodes_h, const int *entry_ptr, int n_bfaces,
+                               const int *bface_nodes, const int *bface_patch, int n_patches,
+                               kcfd_mesh_faces **out) {
+    KCFD_ARG(ctx && out, "null argument");
+    KCFD_ARG(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
+    KCFD_ARG(n_nodes >= 0 && n_cells >= 0 && n_bfaces >= 0 && n_patches >= 0, "negative size");
+    KCFD_ARG(cell_width >= 1 && cell_width <= 8, "cell_width must be 1..8");
+    KCFD_ARG(n_cells == 0 || (nodes && cell_nodes), "null mesh array");
+    KCFD_ARG(n_bfaces == 0 || (bface_nodes && bface_patch), "null boundary array");
+    long long E;
+    int fw;
+    if (ndim == 2) {
+        KCFD_ARG(cell_width >= 3, "a 2D cell needs at least 3 nodes");
+        E = (long long)n_cells * cell_width;
+        fw = 2;
+    } else {
+        KCFD_ARG(face_width >= 3 && face_width <= 4, "3D faces have 3 or 4 nodes");
+        KCFD_ARG(n_cells == 0 || (entry_nodes_h && entry_ptr), "null face lists");
+        E = n_entries;
+        fw = face_width;
+        KCFD_ARG(n_cells == 0 || entry_ptr[0] == 0, "entry_ptr[0] must be 0");
+        for (int c = 0; c < n_cells; ++c) {
+            const int k = entry_ptr[c + 1] - entry_ptr[c];
+            KCFD_ARG(k >= 1 && k <= 8, "a 3D cell has 1..8 faces");
+        }
+        KCFD_ARG(n_cells == 0 || entry_ptr[n_cells] == n_entries, "entry_ptr[n] must equal n_entries");
+    }
+    KCFD_ARG(E < (1LL << 31), "too many cell faces");
+    // node ids in range (a key packs them into 31 bits)
+    {
+        const int *ids = ndim == 2 ? cell_nodes : entry_nodes_h;
+        const long long m = ndim == 2 ? (long long)n_cells * cell_width : E * fw;
+        for (long long q = 0; q < m; ++q)
+            KCFD_ARG(ids[q] < n_nodes && (ids[q] >= 0 || ndim == 3), "node id out of range");
+        for (long long q = 0; q < (long long)n_cells * cell_width; ++q)
+            KCFD_ARG(cell_nodes[q] < n_nodes, "cell node id out of range");
+        const int bw = fw;
+        for (long long q = 0; q < (long long)n_bfaces * bw; ++q)
+            KCFD_ARG(bface_nodes[q] < n_nodes, "boundary node id out of range");
+        // every cell has a node; a 3D face has at least 3 nodes, padding last
+        for (int c = 0; c < n_cells; ++c) {
+            int cnt = 0;
+            for (int k = 0; k < cell_width; ++k) cnt += cell_nodes[(size_t)c * cell_width + k] >= 0;
+            KCFD_ARG(cnt >= 1, "a cell without nodes");
+        }
+        if (ndim == 3)
+            for (long long e = 0; e < E; ++e) {
+                int cnt = 0;
+                bool tail = false, ok = true;
+                for (int i = 0; i < fw; ++i) {
+                    const bool v = entry_nodes_h[e * fw + i] >= 0;
+                    if (v && tail) ok = false;
+                    if (!v) tail = true;
+                    cnt += v;
+                }
+                KCFD_ARG(ok && cnt >= 3, "a 3D face needs at least 3 nodes, padding (-1) last");
+            }
+        for (int b = 0; b < n_bfaces; ++b)
+            KCFD_ARG(bface_patch[b] >= 0 && bface_patch[b] < n_patches, "boundary patch index out of range");
+    }
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    KCFD_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    std::unique_ptr<kcfd_mesh_faces> r(new kcfd_mesh_faces());
+    KCFD_HIP(hipEventRecord(ctx->ev0, s));
+    Bufs bf;
+    const int n = n_cells;
+    double *xyz;
+    int *cells, *faces = nullptr, *ptr, *bn, *bname;
+    KCFD_CHECK(bf.get(&xyz, (size_t)n_nodes * 3));
+    KCFD_CHECK(bf.get(&cells, (size_t)n * cell_width));
+    KCFD_CHECK(bf.get(&ptr, (size_t)n + 1));
+    KCFD_CHECK(bf.get(&bn, (size_t)n_bfaces * fw));
+    KCFD_CHECK(bf.get(&bname, n_bfaces));
+    KCFD_HIP(hipMemcpyAsync(xyz, nodes, sizeof(double) * (size_t)n_nodes * 3, hipMemcpyHostToDevice, s));
+    KCFD_HIP(hipMemcpyAsync(cells, cell_nodes, sizeof(int) * (size_t)n * cell_width, hipMemcpyHostToDevice, s));
+    std::vector<int> hp;
+    if (ndim == 3) {
+        KCFD_CHECK(bf.get(&faces, E * fw));
+        KCFD_HIP(hipMemcpyAsync(faces, entry_nodes_h, sizeof(int) * E * fw, hipMemcpyHostToDevice, s));
+        KCFD_HIP(hipMemcpyAsync(ptr, entry_ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice, s));
+    } else {
+        // 2D: cell c owns entries c * cell_width .. (c + 1) * cell_width - 1
+        hp.resize((size_t)n + 1);
+        for (int c = 0; c <= n; ++c) hp[c] = c * cell_width;
+        KCFD_HIP(hipMemcpyAsync(ptr, hp.data(), sizeof(int) * hp.size(), hipMemcpyHostToDevice, s));
+    }
+    if (n_bfaces) {
+        KCFD_HIP(hipMemcpyAsync(bn, bface_nodes, sizeof(int) * (size_t)n_bfaces * fw, hipMemcpyHostToDevice, s));
+        KCFD_HIP(hipMemcpyAsync(bname, bface_patch, sizeof(int) * n_bfaces, hipMemcpyHostToDevice, s));
+    }
+    KCFD_CHECK(build_dev(ctx, bf, ndim, n_nodes, xyz, n, cell_width, cells, E, fw, faces, ptr, n_bfaces, bn, bname,
+                         n_patches, r.get()));
+    *out = r.release();
+    return KCFD_OK;
+}
+
+extern "C" int kcfd_mesh_generate(kcfd_ctx *ctx, int ndim, int nx, int ny, int nz, double ox, double oy, double dx,
+                                  double dy, double dz, const int *side_patch, int n_patches,
+                                  kcfd_mesh_faces **out) {
+    KCFD_ARG(ctx && out && side_patch, "null argument");
+    KCFD_ARG(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
+    KCFD_ARG(nx >= 1 && ny >= 1 && (ndim == 2 || nz >= 1), "cells per direction must be at least 1");
+    const int nsides = ndim == 2 ? 4 : 6;
+    for (int q = 0; q < nsides; ++q)
+        KCFD_ARG(side_patch[q] >= -1 && side_patch[q] < n_patches, "side patch index out of range");
+    const long long nzz = ndim == 3 ? nz : 0;
+    const long long nn = (long long)(nx + 1) * (ny + 1) * (nzz + 1);
+    const long long n = (long long)nx * ny * (ndim == 3 ? nz : 1);
+    const int cw = ndim == 2 ? 4 : 8, fw = ndim == 2 ? 2 : 4;
+    const long long E = n * (ndim == 2 ? 4 : 6);
+    KCFD_ARG(nn < (1LL << 31) - 1 && E < (1LL << 31), "mesh too large");
+    long long rows[6] = {0, 0, 0, 0, 0, 0};
+    if (ndim == 2) {
+        rows[0] = rows[1] = nx;
+        rows[2] = rows[3] = ny;
+    } else {
+        rows[0] = rows[1] = (long long)ny * nz;
+        rows[2] = rows[3] = (long long)nx * nz;
+        rows[4] = rows[5] = (long long)nx * ny;
+    }
+    long long nb = 0;
+    for (int q = 0; q < nsides; ++q)
+        if (side_patch[q] >= 0) nb += rows[q];
+    std::lock_guard<std::recursive_mutex> g(ctx->mu);
+    KCFD_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    std::unique_ptr<kcfd_mesh_faces> r(new kcfd_mesh_faces());
+    KCFD_HIP(hipEventRecord(ctx->ev0, s));
+    Bufs bf;
+    double *xyz;
+    int *cells, *faces = nullptr, *ptr, *bn, *bname;
+    KCFD_CHECK(bf.get(&xyz, (size_t)nn * 3));
+    KCFD_CHECK(bf.get(&cells, (size_t)n * cw));
+    KCFD_CHECK(bf.get(&ptr, (size_t)n + 1));
+    KCFD_CHECK(bf.get(&bn, (size_t)nb * fw));
+    KCFD_CHECK(bf.get(&bname, nb));
+    hipLaunchKernelGGL(k_gen_nodes, dim3(mb_grid(nn)), dim3(kMbBlock), 0, s, nx, ny, (int)nzz, ox, oy, dx, dy, dz, nn,
+                       xyz);
+    if (ndim == 2) {
+        hipLaunchKernelGGL(k_gen_quads, dim3(mb_grid(n)), dim3(kMbBlock), 0, s, nx, ny, cells);
+        std::vector<int> hp((size_t)n + 1);
+        for (long long c = 0; c <= n; ++c) hp[c] = (int)(c * cw);
+        KCFD_HIP(hipMemcpyAsync(ptr, hp.data(), sizeof(int) * hp.size(), hipMemcpyHostToDevice, s));
+        KCFD_HIP(hipStreamSynchronize(s));   // hp is a local
+    } else {
+        KCFD_CHECK(bf.get(&faces, E * fw));
+        hipLaunchKernelGGL(k_gen_hexes, dim3(mb_grid(n)), dim3(kMbBlock), 0, s, nx, ny, nz, cells, faces, ptr);
+    }
+    long long at = 0;
+    for (int q = 0; q < nsides; ++q) {
+        if (side_patch[q] < 0 || rows[q] == 0) continue;
+        hipLaunchKernelGGL(k_gen_bside, dim3(mb_grid(rows[q])), dim3(kMbBlock), 0, s, ndim, q, nx, ny, (int)nzz,
+                           (int)rows[q], side_patch[q], bn + at * fw, bname + at);
+        at += rows[q];
+    }
+    KCFD_HIP(hipGetLastError());
+    KCFD_CHECK(build_dev(ctx, bf, ndim, (int)nn, xyz, (int)n, cw, cells, E, fw, faces, ptr, (int)nb, bn, bname,
+                         n_patches, r.get()));
+    *out = r.release();
+    return KCFD_OK;
+}
+
+extern "C" int kcfd_mesh_faces_nodes(const kcfd_mesh_faces *m, int *n_nodes, int *n_cells, double *nodes) {
+    KCFD_ARG(m, "null mesh faces");
+    if (n_nodes) *n_nodes = m->n_nodes;
+    if (n_cells) *n_cells = m->n_cells;
+    if (nodes && !m->nodes.empty()) std::memcpy(nodes, m->nodes.data(), sizeof(double) * m->nodes.size());
     return KCFD_OK;
 }
 

@@ -31,7 +31,8 @@ from typing import NamedTuple
 
 __all__ = ["FVMesh", "build_fvmesh_from_arrays", "structured_quad_arrays",
            "structured_hex_arrays", "tet_arrays", "RaggedCells", "hybrid_hex_tet_arrays",
-           "generate_hybrid_hex_tet_mesh"]
+           "generate_hybrid_hex_tet_mesh", "build_fvmesh_on_device", "structured_quad_mesh_on_device",
+           "structured_hex_mesh_on_device"]
 
 
 class FVMesh:
@@ -419,12 +420,25 @@ def build_fvmesh_on_device(node_coords, cell_node_list, boundary_faces_dict, cel
         len(ent32) if ent32 is not None else 0, width, L.ptr(ent32) if ent32 is not None else None,
         L.ptr(ptr32) if ptr32 is not None else None, len(b32), L.ptr(b32) if len(b32) else None,
         L.ptr(bn32) if len(bn32) else None, len(names), L.ctypes.byref(h)))
+    return _mesh_of_faces(h, ndim, nodes, names, cell_zone_map)
+
+
+def _mesh_of_faces(h, ndim, nodes, names, cell_zone_map=None) -> FVMesh:
+    """The FVMesh of a device-built face set (frees the handle).  ``nodes``:
+    the node array to keep (None: the device's, with ndim columns)."""
+    from . import _lib as L
     try:
         nf, nint, fw = L.ctypes.c_int(), L.ctypes.c_int(), L.ctypes.c_int()
+        nn, nc = L.ctypes.c_int(), L.ctypes.c_int()
         ms = L.ctypes.c_double()
         L.check(L.lib().kcfd_mesh_faces_info(h, L.ctypes.byref(nf), L.ctypes.byref(nint), L.ctypes.byref(fw),
                                              L.ctypes.byref(ms)))
-        nf = nf.value
+        L.check(L.lib().kcfd_mesh_faces_nodes(h, L.ctypes.byref(nn), L.ctypes.byref(nc), None))
+        if nodes is None:
+            xyz = np.empty((nn.value, 3))
+            L.check(L.lib().kcfd_mesh_faces_nodes(h, None, None, L.ptr(xyz)))
+            nodes = np.ascontiguousarray(xyz[:, :ndim])
+        nf, n = nf.value, nc.value
         owner = np.empty(nf, np.int32)
         neighbour = np.empty(nf, np.int32)
         fnodes = np.empty((nf, fw.value), np.int32)
@@ -451,7 +465,7 @@ def build_fvmesh_on_device(node_coords, cell_node_list, boundary_faces_dict, cel
     # patches in the order of their first boundary face ("default" last index)
     bfids = np.nonzero(neighbour < 0)[0]
     fp = fpatch[bfids]
-    all_names = names + ["default"]
+    all_names = list(names) + ["default"]
     patches = {}
     if len(fp):
         uniq, first = np.unique(fp, return_index=True)
@@ -462,6 +476,56 @@ def build_fvmesh_on_device(node_coords, cell_node_list, boundary_faces_dict, cel
         mesh.cell_zones = cell_zone_map
     mesh.build_ms = ms.value
     return mesh
+
+
+def _side_patches(keys, defaults, boundary_names):
+    """Patch names in the generators' dict order and, per side, the index of
+    the patch it is listed under (-1 where a later side of the same name
+    replaces its list, as the generators' dict assignment does)."""
+    names, side_name = [], []
+    for key, dflt in zip(keys, defaults):
+        nm = boundary_names.get(key, dflt)
+        side_name.append(nm)
+        if nm not in names:
+            names.append(nm)
+    last = {nm: q for q, nm in enumerate(side_name)}
+    side = np.array([names.index(nm) if last[nm] == q else -1 for q, nm in enumerate(side_name)], np.int32)
+    return names, side
+
+
+def structured_quad_mesh_on_device(x0, y0, x1, y1, nx, ny, boundary_names=None, ctx=None) -> FVMesh:
+    """The mesh of ``structured_quad_arrays`` + ``build_fvmesh_from_arrays``
+    (mesh/mesh_generator.py:13-93, mesh/mesh_reader.py:141-242), generated
+    and built on the GPU (kcfd_mesh_generate): bit for bit the host's."""
+    from . import _lib as L
+    ctx = ctx or L.default_context()
+    keys = ["bottom", "top", "left", "right"]
+    names, side = _side_patches(keys, keys, boundary_names or {k: k for k in keys})
+    dx = (x1 - x0) / nx
+    dy = (y1 - y0) / ny
+    h = L.ctypes.c_void_p()
+    L.check(L.lib().kcfd_mesh_generate(ctx.h, 2, int(nx), int(ny), 0, float(x0), float(y0), float(dx), float(dy),
+                                       0.0, L.ptr(side), len(names), L.ctypes.byref(h)))
+    return _mesh_of_faces(h, 2, None, names)
+
+
+def structured_hex_mesh_on_device(Lx, Ly, Lz, nx, ny, nz, boundary_names=None, ctx=None) -> FVMesh:
+    """The mesh of ``structured_hex_arrays`` + ``build_fvmesh_from_arrays``
+    (mesh/mesh_generator_3d.py:12-159, mesh/mesh_reader.py:245-323),
+    generated and built on the GPU (kcfd_mesh_generate): bit for bit the
+    host's."""
+    from . import _lib as L
+    ctx = ctx or L.default_context()
+    keys = ["x_min", "x_max", "y_min", "y_max", "z_min", "z_max"]
+    dflt = ["inlet", "outlet", "wall_bottom", "wall_top", "wall_front", "wall_back"]
+    if boundary_names is None:
+        boundary_names = dict(zip(keys, dflt))
+    names, side = _side_patches(keys, dflt, boundary_names)
+    h = L.ctypes.c_void_p()
+    L.check(L.lib().kcfd_mesh_generate(ctx.h, 3, int(nx), int(ny), int(nz), 0.0, 0.0, float(Lx / nx),
+                                       float(Ly / ny), float(Lz / nz), L.ptr(side), len(names),
+                                       L.ctypes.byref(h)))
+    return _mesh_of_faces(h, 3, None, names)
 
 
 def _as_ragged(cell_node_list, cell_face_node_list):

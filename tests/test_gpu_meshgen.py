@@ -124,3 +124,52 @@ def test_argument_checks(ctx):
     faces[0, 0, 2] = -1                    # a quad face with padding inside
     with pytest.raises(L.KcfdError, match="padding"):
         M.build_fvmesh_on_device(hn, hc, hb, ndim=3, cell_face_node_list=faces)
+
+
+def _gen_both(label, host_args, dev_fn, dev_args, kind):
+    from kcfd_amd import mesh as M
+    t = time.perf_counter()
+    if kind == "quad":
+        nodes, cells, bf = M.structured_quad_arrays(*host_args)
+        mh = M.build_fvmesh_from_arrays(nodes, cells, bf, ndim=2)
+    else:
+        nodes, cells, faces, bf = M.structured_hex_arrays(*host_args)
+        mh = M.build_fvmesh_from_arrays(nodes, cells, bf, ndim=3, cell_face_node_list=faces)
+    th = time.perf_counter() - t
+    t = time.perf_counter()
+    md = dev_fn(*dev_args)
+    td = time.perf_counter() - t
+    print(f"{label}: {mh.n_cells} cells; host generate + build {th:.3f} s, device {td:.3f} s "
+          f"({md.build_ms:.1f} ms on the device)")
+    assert md.nodes.shape == mh.nodes.shape and np.array_equal(md.nodes, mh.nodes), label
+    _same(md, mh, label)
+    return md, mh
+
+
+@pytest.mark.parametrize("n", [1, 9, 128])
+def test_generated_quad(ctx, n):
+    from kcfd_amd import mesh as M
+    args = (0.25, -1.0, 1.75, 2.0, n, n + 5)
+    _gen_both(f"generated quad {n}x{n + 5}", args, M.structured_quad_mesh_on_device, args, "quad")
+
+
+@pytest.mark.parametrize("n", [1, 6, 128])
+def test_generated_hex(ctx, n):
+    from kcfd_amd import mesh as M
+    args = (1.0, 0.5, 2.0, n, max(1, n // 2), n + 1) if n < 128 else (1.0, 1.0, 1.0, n, n, n)
+    _gen_both(f"generated hex {args[3:]}", args, M.structured_hex_mesh_on_device, args, "hex")
+
+
+def test_generated_shared_patch_names(ctx):
+    """Sides given the same name: the generators' dict keeps the name at its
+    first side's position with the last side's faces; the earlier sides'
+    faces then fall to 'default'."""
+    from kcfd_amd import mesh as M
+    names = {"x_min": "in", "x_max": "out", "y_min": "walls", "y_max": "walls", "z_min": "walls",
+             "z_max": "walls"}
+    args = (1.0, 1.0, 1.0, 5, 4, 3)
+    md, mh = _gen_both("hex shared names", args + (names,), M.structured_hex_mesh_on_device, args + (names,), "hex")
+    assert "default" in mh.boundary_patches
+    qn = {"bottom": "wall", "top": "wall", "left": "in"}
+    qargs = (0.0, 0.0, 1.0, 1.0, 7, 4, qn)
+    _gen_both("quad shared names", qargs, M.structured_quad_mesh_on_device, qargs, "quad")
