@@ -189,8 +189,10 @@ def main():
                     help="serve: skip the per-template precision calibration")
     ap.add_argument("--no-single", action="store_true",
                     help="batched: skip the FP64 one-system-at-a-time GPU baseline")
-    ap.add_argument("--check-every", type=int, default=8,
-                    help="iterations per captured graph between host status checks")
+    ap.add_argument("--check-every", type=int, default=None,
+                    help="iterations per captured graph between host status checks (default: 32 for the "
+                         "single pressure system, where it measured fastest; 8 for the configs 1 / 3 "
+                         "solves and the server)")
     ap.add_argument("--precisions", default="fp64,mixed",
                     help="comma list; a profiler run can take one precision per process")
     ap.add_argument("--cpu-workers", type=int, default=16, help="processes of the batch's CPU baseline")
@@ -376,7 +378,7 @@ def run_pressure(args, ctx, case):
         precs = ["fp64"]                 # the CG and GMRES paths run in FP64
     for prec in precs:
         opts = device.default_opts(rtol=1e-8, precond=args.precond, precision=prec,
-                                   inner_rtol=args.delta, check_every=args.check_every,
+                                   inner_rtol=args.delta, check_every=args.check_every or 32,
                                    grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu,
                                    tile_walk=args.tile_walk, method=args.method,
                                    rows_per_thread=args.rows_per_thread)
@@ -489,7 +491,7 @@ def run_single(args, ctx, case, precond, steps):
     sols = {}
     for prec in ("fp64", "mixed"):
         A.setup_precond(precond, prec)
-        opts = device.default_opts(rtol=1e-8, precond=precond, precision=prec, check_every=args.check_every)
+        opts = device.default_opts(rtol=1e-8, precond=precond, precision=prec, check_every=args.check_every or 8)
         for _ in range(2):
             x.fill(0.0)
             A.solve_dev(b.p, x.p, opts, setup=False)
@@ -544,7 +546,7 @@ def run_batched(args, ctx, cases, steps):
         if args.no_tune:
             choice = dict(grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu,
                           tile_walk=args.tile_walk, rows_per_thread=args.rows_per_thread,
-                          check_every=args.check_every)
+                          check_every=args.check_every or 8)
         else:
             choice = tune.autotune(sb, prec, args.precond, order or "natural", delta=args.delta, log=log)
         log(f"{prec}: launch shape {tune.describe(choice)}")
@@ -720,7 +722,7 @@ def bench_serve(args):
 
     # warm-up: a few requests (graph capture, first launches)
     srv = S.Server(ctx, temps, precision="mixed", precond=args.precond, order=args.order,
-                   batched=True, check_every=args.check_every, log=progress)
+                   batched=True, check_every=args.check_every or 8, log=progress)
     srv.run([(0.0, ti, sd) for _, ti, sd in tr[:8]])
     t = time.perf_counter()
     rep = srv.run(tr)
