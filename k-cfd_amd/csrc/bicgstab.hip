@@ -41,6 +41,9 @@
 // rho and beta are recomputed from the replaced residual).  Convergence is
 // judged on the FP64 true residual; a stop without convergence returns the
 // best iterate seen.
+#include <chrono>
+#include <cstdlib>
+
 #include "common.hpp"
 
 #include <hip/hip_ext.h>
@@ -2282,6 +2285,12 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
     // every graph iteration advances each running system by one iteration or
     // one reliable update
     const long long limit = 2LL * maxit + 2LL * b->bp.max_refine + 64;
+    // KCFD_BATCH_TRACE=1: one stderr line per host check (iterations so far,
+    // systems still running, tiles in the tables, host time since the start)
+    const char *tr_env = std::getenv("KCFD_BATCH_TRACE");
+    const bool trace = tr_env && tr_env[0] && tr_env[0] != '0';
+    const auto t_start = std::chrono::steady_clock::now();
+    int tr_tiles = b->ntiles;
     for (;;) {
         KCFD_CHECK(replay_iterations(b, s));
         guard += b->graph_iters;
@@ -2292,6 +2301,14 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
                                     hipMemcpyDeviceToHost, s));
         KCFD_HIP(hipStreamSynchronize(s));
         KCFD_CHECK(check_trsv(b, fail));
+        if (trace) {
+            int running = 0;
+            for (int k = 0; k < b->nsys; ++k) running += b->mats[k] && b->h_state[k].status != S_DONE;
+            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start)
+                                  .count();
+            std::fprintf(stderr, "[kcfd batch] iterations %lld running %d tiles %d at %.3f ms\n", guard, running,
+                         tr_tiles, ms);
+        }
         bool all = true;
         for (int k = 0; k < b->nsys; ++k)
             if (b->h_state[k].status != S_DONE) { all = false; break; }
@@ -2310,7 +2327,10 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
                 act[k] = 1;
                 act_tiles += (int)b->sys_tables[k].tiles.size();
             }
-        if (act_tiles < b->ntiles - b->ntiles / 5) KCFD_CHECK(rebuild_tables(b, &act));
+        if (act_tiles < b->ntiles - b->ntiles / 5) {
+            KCFD_CHECK(rebuild_tables(b, &act));
+            tr_tiles = act_tiles;
+        }
     }
     // the finishing kernels (true residual, restore, scatter) serve every
     // system: the full tables again
