@@ -182,6 +182,9 @@ def main():
                     help="launch shape: rows per thread of the tile kernels (0: library default 2)")
     ap.add_argument("--tile-walk", type=int, default=0,
                     help="launch shape: 1 interleaved, 2 contiguous tiles per workgroup (0: default)")
+    ap.add_argument("--fold", type=int, default=0, choices=[-1, 0, 1],
+                    help="two-colour ILU(0) BiCGSTAB: fold the colour-0 rows of each product into the "
+                         "apply (0: library default, on in mixed precision; 1: on; -1: off)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true",
                     help="default line: only the pressure system (no batch, no configs 1 and 3)")
@@ -381,7 +384,7 @@ def run_pressure(args, ctx, case):
                                    inner_rtol=args.delta, check_every=args.check_every or 32,
                                    grid_per_cu=args.grid_per_cu, trsv_per_cu=args.trsv_per_cu,
                                    tile_walk=args.tile_walk, method=args.method,
-                                   rows_per_thread=args.rows_per_thread)
+                                   rows_per_thread=args.rows_per_thread, fold_colour0=args.fold)
         lat, rs = [], []
 
         def step():
@@ -551,12 +554,12 @@ def run_batched(args, ctx, cases, steps):
             choice = tune.autotune(sb, prec, args.precond, order or "natural", delta=args.delta, log=log)
         log(f"{prec}: launch shape {tune.describe(choice)}")
         opts = dict(precision=prec, precond=args.precond, order=order or "natural", delta=args.delta,
-                    **choice)
+                    fold_colour0=args.fold, **choice)
         bobj = sb.make_batch(device.default_opts(
             rtol=1e-8, precond=args.precond, precision=prec, inner_rtol=args.delta,
             check_every=choice["check_every"], grid_per_cu=choice["grid_per_cu"],
             trsv_per_cu=choice["trsv_per_cu"], tile_walk=choice.get("tile_walk", 0),
-            rows_per_thread=choice.get("rows_per_thread", 0)))
+            rows_per_thread=choice.get("rows_per_thread", 0), fold_colour0=args.fold))
         for w in range(args.warmup):
             r = sb.solve(batch=bobj, **opts)
             log(f"{prec} warmup {w}: {r.summary()}")

@@ -170,7 +170,8 @@ typedef struct kcfd_solve_opts {
     int max_refine;     /* MIXED: cap on reliable updates (default 1000) */
     int check_every;    /* iterations per captured graph between host checks */
     int grid_per_cu;    /* launch shape: persistent workgroups (256 lanes) per CU of
-                           the SpMV / vector kernels, 1..8 (0: 8) */
+                           the SpMV / vector kernels, 1..32 (0: 8; above 8 the
+                           workgroups beyond the resident ones start as others end) */
     int trsv_per_cu;    /* launch shape: workgroups per CU of the ILU(0) solves,
                            1..4 (0: 4; the solve needs its grid resident) */
     int tile_walk;      /* launch shape: how a persistent workgroup walks the row
@@ -182,6 +183,16 @@ typedef struct kcfd_solve_opts {
     int rows_per_thread; /* launch shape: rows each thread of the SpMV / vector kernels
                             takes per tile, 1, 2 or 4 (a wave covers 64x that many rows
                             of a tile of 256x that many); 0: 2 */
+    int fold_colour0;   /* BiCGSTAB with a two-colour ILU(0) (KCFD_ORDER_COLOR on a
+                           bipartite pattern): the colour-0 rows of the U factor hold A's
+                           own off-diagonal entries, so the colour-0 half of each
+                           preconditioner apply also computes those rows of A M^-1 z and
+                           the product kernel streams the colour-1 rows only.  In
+                           KCFD_MIXED the folded rows use the FP64 diagonal
+                           a_ii + sum_j (a_ij - fl32(a_ij)), which keeps A's row sums as the
+                           difference-form FP32 copy does.  0: on in KCFD_MIXED, off in
+                           KCFD_FP64 (whose products follow scipy's CSR row sums);
+                           1: on; -1: off */
 } kcfd_solve_opts;
 
 typedef struct kcfd_solve_result {

@@ -44,7 +44,8 @@ class SolveOpts(ctypes.Structure):
                 ("inner_rtol", ctypes.c_double), ("max_refine", ctypes.c_int),
                 ("check_every", ctypes.c_int), ("grid_per_cu", ctypes.c_int),
                 ("trsv_per_cu", ctypes.c_int), ("tile_walk", ctypes.c_int),
-                ("method", ctypes.c_int), ("restart", ctypes.c_int), ("rows_per_thread", ctypes.c_int)]
+                ("method", ctypes.c_int), ("restart", ctypes.c_int), ("rows_per_thread", ctypes.c_int),
+                ("fold_colour0", ctypes.c_int)]
 
 
 class SolveResult(ctypes.Structure):

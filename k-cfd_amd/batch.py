@@ -144,7 +144,7 @@ class SystemBatch:
 
     def solve(self, precision="mixed", precond="jacobi", order="color", rtol=1e-8,
               check_every=8, grid_per_cu=0, trsv_per_cu=0, delta=0.1, batch=None,
-              fallback=True, tile_walk=0, rows_per_thread=0) -> BatchResult:
+              fallback=True, tile_walk=0, rows_per_thread=0, fold_colour0=0) -> BatchResult:
         """One batched solve from x0 = 0 (preconditioners set up beforehand,
         outside the timed region).  In mixed precision, the systems that stop
         without converging continue in FP64 from their best iterate, as one
@@ -153,7 +153,7 @@ class SystemBatch:
         opts = device.default_opts(rtol=rtol, precond=precond, precision=precision, inner_rtol=delta,
                                    check_every=check_every, grid_per_cu=grid_per_cu,
                                    trsv_per_cu=trsv_per_cu, tile_walk=tile_walk,
-                                   rows_per_thread=rows_per_thread)
+                                   rows_per_thread=rows_per_thread, fold_colour0=fold_colour0)
         own = batch is None
         b = batch if batch is not None else self.make_batch(opts)
         nfb = 0

@@ -26,6 +26,10 @@
 //   K5 x += alpha M p; x += omega M s; r = s - omega t; ||r||, rt.r
 //      (half-step exit: x += alpha M p only)
 //                                F5: convergence / breakdown / beta
+// Folded products (two-colour ILU(0), kcfd_solve_opts.fold_colour0; on by
+// default in MIXED): the colour-0 half of each apply also writes the
+// colour-0 rows of v = A M p (t = A M s), and K2 / K4 stream the colour-1
+// rows only.
 // Reductions: wave64 xor-shuffle + LDS per tile, one partial record per tile,
 // and a finalize kernel (one workgroup per system) that sums the partials in
 // tile order -- deterministic, and free of in-kernel fences (on gfx950 an
@@ -70,6 +74,13 @@ enum : int {
 struct SysDesc {
     int n, tile0, ntiles;
     int polled;   // ILU(0) solved by the sync-free kernels (sentinel-flagged buffers)
+    // Folded product (kcfd_solve_opts.fold_colour0, two-colour ILU(0)):
+    // rows [0, nid) of the solver's order are colour 0; the colour-0 half
+    // of each apply also writes those rows of A M z (k_ilu2 SECOND), so the
+    // product kernels only read them.  c0piv: MIXED, their FP64 diagonal.
+    // nid 0: no folding.
+    int nid;
+    const double *c0piv;
     // ILU(0) factor structure: CSR of P A P^T (A's own CSR in the natural
     // ordering); uix = columns as rows of A.  perm: the solver's row order,
     // row of A at each position (null: identity).  With a permuted ILU(0)
@@ -145,13 +156,19 @@ struct Chunk {
 // tiles j, j + J, j + 2J, ...; walk 1: block j takes the contiguous run
 // [j*c, (j+1)*c), c = ceil(per / J).  The walk mode is counts[5].
 // gridDim.x is a multiple of 8; the trip count is uniform over a block.
-#define KCFD_TILE_LOOP(tile, ntiles)                                                            \
+// KCFD_TILE_LOOP_W takes the walk mode as an expression: the two product
+// kernels walk contiguous runs when the batch has folded members
+// (counts[6]), whose tile list alternates cheap colour-0 tiles with full
+// colour-1 ones -- an interleaved walk with an even J would give each
+// workgroup tiles of one colour only.
+#define KCFD_TILE_LOOP_W(tile, ntiles, walk)                                                     \
     for (int per_ = ((ntiles) + kXcds - 1) / kXcds, J_ = (int)gridDim.x / kXcds,                  \
-             j_ = (int)blockIdx.x / kXcds, x0_ = (int)(blockIdx.x % kXcds) * per_,                \
-             c_ = counts[5] ? (per_ + J_ - 1) / J_ : 1, st_ = counts[5] ? 1 : J_,                 \
+             j_ = (int)blockIdx.x / kXcds, x0_ = (int)(blockIdx.x % kXcds) * per_, w_ = (walk),   \
+             c_ = w_ ? (per_ + J_ - 1) / J_ : 1, st_ = w_ ? 1 : J_,                                 \
              tile = x0_ + j_ * c_,                                                                 \
-             e_ = min(min(x0_ + (counts[5] ? (j_ + 1) * c_ : per_), x0_ + per_), (ntiles));        \
+             e_ = min(min(x0_ + (w_ ? (j_ + 1) * c_ : per_), x0_ + per_), (ntiles));               \
          tile < e_; tile += st_)
+#define KCFD_TILE_LOOP(tile, ntiles) KCFD_TILE_LOOP_W(tile, ntiles, counts[5])
 
 // ----------------------------------------------------- element accessors
 
@@ -578,7 +595,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                                                    double *__restrict__ partials) {
 #pragma clang fp contract(off)
     const int ntiles = counts[0];   // tiles of the current membership
-    KCFD_TILE_LOOP(tile, ntiles) {
+    KCFD_TILE_LOOP_W(tile, ntiles, counts[5] | counts[6]) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
@@ -592,11 +609,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
             double v[1] = {0.0};
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                const double a = row_amz<V, PC>(d, tr.ix[r], (const V *)d.p, (const V *)d.ph);
-                const double rtv = (double)vec<V>(d.rt)[tr.ix[r]];
+                const int i = tr.ix[r];
+                if (PC == KCFD_PC_ILU0 && i < d.nid) {   // folded: the p apply wrote v_i
+                    const double rtv = (double)vec<V>(d.rt)[i], vv = (double)vec<V>(d.v)[i];
+                    if (tr.ok[r]) v[0] += rtv * vv;
+                    continue;
+                }
+                const double a = row_amz<V, PC>(d, i, (const V *)d.p, (const V *)d.ph);
+                const double rtv = (double)vec<V>(d.rt)[i];
                 if (tr.ok[r]) {
                     const V vv = (V)a;
-                    vec<V>(d.v)[tr.ix[r]] = vv;
+                    vec<V>(d.v)[i] = vv;
                     v[0] += rtv * (double)vv;
                 }
             }
@@ -741,7 +764,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
                                                    double *__restrict__ partials, const BatchParams *__restrict__ bpp) {
 #pragma clang fp contract(off)
     const int ntiles = counts[0];   // tiles of the current membership
-    KCFD_TILE_LOOP(tile, ntiles) {
+    KCFD_TILE_LOOP_W(tile, ntiles, counts[5] | counts[6]) {
         const int2 ti = tiles[tile];
         const SysDesc &d = sys[ti.x];
         const SysState &st = sts[ti.x];
@@ -750,13 +773,18 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
         double v[3] = {0.0, 0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < R; ++r) {   // one row after the other, as in K2
-            const double a = row_amz<V, PC>(d, tr.ix[r], (const V *)d.s, (const V *)d.sh);
-            const double sv = (double)vec<V>(d.s)[tr.ix[r]];
+            const int i = tr.ix[r];
+            double tv;
+            if (PC == KCFD_PC_ILU0 && i < d.nid) {   // folded: the s apply wrote t_i
+                tv = (double)vec<V>(d.t)[i];
+            } else {
+                tv = (double)(V)row_amz<V, PC>(d, i, (const V *)d.s, (const V *)d.sh);
+                if (tr.ok[r]) vec<V>(d.t)[i] = (V)tv;
+            }
+            const double sv = (double)vec<V>(d.s)[i];
             if (tr.ok[r]) {
-                const V tv = (V)a;
-                vec<V>(d.t)[tr.ix[r]] = tv;
-                v[0] += (double)tv * sv;
-                v[1] += (double)tv * (double)tv;
+                v[0] += tv * sv;
+                v[1] += tv * tv;
                 v[2] += sv * sv;
             }
         }
@@ -1462,6 +1490,72 @@ __device__ __forceinline__ void ilu2_row(const SysDesc &d, const Chunk &ch, int 
     w[i] = v / gl(udiag<V>(d))[i];
 }
 
+// A colour-0 row of a folded apply (SECOND), which also writes the row's
+// product (A M z)_i -- v for the p apply, t for the s apply.  Colour-0
+// rows of U hold A's own off-diagonal entries (no L entries, so the
+// factorisation leaves them alone), so the product needs nothing the apply
+// does not already read: (A M z)_i = P_i + D_i w_i with P_i = sum_j u_ij w_j,
+// summed here in FP64 beside the apply's own sum, w_i the apply's result
+// and D_i the diagonal: a_ii (FP64), or in MIXED the FP64 pivot
+// a_ii + sum_j (a_ij - fl32(a_ij)) (kcfd_mat::c0piv), for which the row
+// [D_i, fl32(a_ij)] keeps A's row sum, as the difference-form copy of A
+// does.  The apply's own arithmetic is unchanged.
+template <typename V, int G, typename F, typename S>
+__device__ __forceinline__ void sub_group_prod(V &acc, double &prod, const F &f, int base, int k0, int last,
+                                               int valid, const S &src) {
+#pragma clang fp contract(off)
+    int c[G];
+    V av[G], xv[G];
+#pragma unroll
+    for (int k = 0; k < G; ++k) fac_get<V>(f, base + 64 * min(k0 + k, last), c[k], av[k]);
+#pragma unroll
+    for (int k = 0; k < G; ++k) xv[k] = src(max(c[k], 0));
+#pragma unroll
+    for (int k = 0; k < G; ++k)
+        if (k < valid && c[k] >= 0) {
+            acc -= av[k] * xv[k];
+            prod += (double)av[k] * (double)xv[k];
+        }
+}
+
+template <typename V, typename F, typename S>
+__device__ __forceinline__ V sub_plain_prod(V acc, double &prod, const F &f, int base, int w, const S &src) {
+    if (w == 6) {
+        sub_group_prod<V, 6>(acc, prod, f, base, 0, 5, 6, src);
+    } else if (w == 4) {
+        sub_group_prod<V, 4>(acc, prod, f, base, 0, 3, 4, src);
+    } else {
+        constexpr int G = 8;
+        for (int kb = 0; kb < w; kb += G) sub_group_prod<V, G>(acc, prod, f, base, kb, w - 1, w - kb, src);
+    }
+    return acc;
+}
+
+template <typename V, typename S>
+__device__ __forceinline__ void ilu2_row_fold(const SysDesc &d, const Chunk &ch, int i, int o0, int wd,
+                                              const S &src, KCFD_GLOBAL V *__restrict__ zout,
+                                              KCFD_GLOBAL V *__restrict__ w, KCFD_GLOBAL V *__restrict__ prod) {
+#pragma clang fp contract(off)
+    const V zi = src(i);
+    if (zout) zout[i] = zi;
+    const PlainSrc<V> wsrc{w};
+    double pr = 0.0;
+    V v;
+    const unsigned short *o16 = nullptr;
+    if constexpr (sizeof(V) == sizeof(float)) o16 = d.chu_o16;
+    if (o16) {
+        const FacRef16 f{d.chu_v32, o16, uniform(gl(d.chu_base)[ch.local])};
+        v = sub_plain_prod<V>(zi, pr, f, o0 + (i - ch.start), wd, wsrc);
+    } else {
+        v = sub_plain_prod<V>(zi, pr, fac_u(d), o0 + (i - ch.start), wd, wsrc);
+    }
+    const V ud = gl(udiag<V>(d))[i];
+    const V wi = v / ud;
+    w[i] = wi;
+    const double dg = sizeof(V) == sizeof(float) ? gl(d.c0piv)[i] : (double)ud;
+    prod[i] = (V)(pr + dg * (double)wi);
+}
+
 template <typename V, bool SECOND, int WHICH>
 __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
@@ -1487,6 +1581,18 @@ __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys
             const int o0 = uniform(gl(off)[ch.local]);
             const int wd = uniform((gl(off)[ch.local + 1] - o0) >> 6);
             const SysState &st = sts[ch.sys];
+            if constexpr (SECOND && WHICH != 2) {
+                if (d.nid > 0) {   // folded: the colour-0 rows of A M z as well
+                    if constexpr (WHICH == 0) {
+                        const PlainSrc<V> src{vec<V>(d.p)};
+                        ilu2_row_fold<V>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w, vec<V>(d.v));
+                    } else {
+                        const SSrc<V> src{vec<V>(d.r), vec<V>(d.v), (V)st.alpha};
+                        ilu2_row_fold<V>(d, ch, i, o0, wd, src, vec<V>(d.s), w, vec<V>(d.t));
+                    }
+                    continue;
+                }
+            }
             if constexpr (WHICH == 0) {
                 const PlainSrc<V> src{vec<V>(d.p)};
                 ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w);
@@ -1579,6 +1685,7 @@ struct kcfd_batch {
     int graph_iters = 0;
     bool dirty = false;             // membership changed: tables to rebuild
     bool no_graph = false;          // direct launches only (kcfd_solve_noalloc)
+    bool fold = false;              // kcfd_solve_opts.fold_colour0 resolved for this batch
     // per-system table pieces (rebuild_tables); tables_dirty: the membership
     // or a member's structure changed since they were built; compacted: the
     // device tables hold only the systems that were still running
@@ -1771,8 +1878,12 @@ LaunchFn pick_finish(int rows) {
 // algorithmic bytes (roofline model): CSR stream M_A, ILU factor stream M_F,
 // vector streams of the fused schedule; MIXED adds one FP64 residual
 // replacement (M_A at 8 bytes + 6 n*8) per reliable update
+// folded product (nid > 0): the two products per iteration skip the
+// colour-0 rows' entries and slice pointers (those rows only read v or t,
+// and rt or s); the colour-0 half of each apply writes v or t (n0 s_v) and,
+// in MIXED, reads the FP64 pivots (8 n0)
 double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st,
-                   int method = KCFD_METHOD_BICGSTAB) {
+                   int method = KCFD_METHOD_BICGSTAB, int nid = 0) {
     const double n = m->n, nnz = m->nnz;
     auto MA = [&](double sv) { return nnz * (sv + 4) + (n + 1) * 4; };
     auto MF = [&](double sv) { return nnz * (sv + 4) + 2 * (n + 1) * 4; };
@@ -1788,10 +1899,15 @@ double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st,
         if (pc == KCFD_PC_ILU0) return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv;
         return 2 * MA(sv) + 19 * n * sv;
     };
-    if (precision == KCFD_FP64) return (st.total_iters + 0.5 * st.half) * per_iter(8) + MA(8) + 3 * n * 8;
+    const double n0 = nid, nnz0 = nid > 0 ? (double)m->nnz_colour0 : 0.0;
+    auto fold = [&](double sv) { return nid > 0 ? -2 * (nnz0 * (sv + 4) + n0 * 4) + 2 * n0 * sv : 0.0; };
+    if (precision == KCFD_FP64)
+        return (st.total_iters + 0.5 * st.half) * (per_iter(8) + fold(8)) + MA(8) + 3 * n * 8;
     // MIXED: the iterate x is FP64 (read + write, +8 n over an FP32 one);
     // a residual replacement reads A, x, b in FP64 and writes r (+ xb copy)
-    return st.total_iters * (per_iter(4) + 8 * n) + st.refinements * (MA(8) + 2 * n * 8 + n * 4);
+    const double fold64 = nid > 0 ? 2 * 8 * n0 : 0.0;   // the pivots, twice
+    return st.total_iters * (per_iter(4) + 8 * n + fold(4) + fold64) +
+           st.refinements * (MA(8) + 2 * n * 8 + n * 4);
 }
 
 void fill_desc(SysDesc &d, kcfd_mat *m) {
@@ -1836,7 +1952,7 @@ int check_opts(const kcfd_solve_opts *o) {
     KCFD_ARG(pc == KCFD_PC_NONE || pc == KCFD_PC_JACOBI || pc == KCFD_PC_ILU0,
              "unknown preconditioner");
     KCFD_ARG(prec == KCFD_FP64 || prec == KCFD_MIXED, "unknown precision");
-    KCFD_ARG(o->grid_per_cu >= 0 && o->grid_per_cu <= 8, "grid_per_cu must be 0..8");
+    KCFD_ARG(o->grid_per_cu >= 0 && o->grid_per_cu <= 32, "grid_per_cu must be 0..32");
     KCFD_ARG(o->trsv_per_cu >= 0 && o->trsv_per_cu <= 4, "trsv_per_cu must be 0..4");
     KCFD_ARG(o->tile_walk >= 0 && o->tile_walk <= 2, "tile_walk must be 0..2");
     KCFD_ARG(o->rows_per_thread == 0 || o->rows_per_thread == 1 || o->rows_per_thread == 2 ||
@@ -1846,6 +1962,7 @@ int check_opts(const kcfd_solve_opts *o) {
              o->method == KCFD_METHOD_GMRES ? "GMRES runs through kcfd_solve only (no batches)"
                                             : "unknown Krylov method");
     KCFD_ARG(o->method != KCFD_METHOD_CG || prec == KCFD_FP64, "CG runs in FP64 only");
+    KCFD_ARG(o->fold_colour0 >= -1 && o->fold_colour0 <= 1, "fold_colour0 must be -1, 0 or 1");
     return KCFD_OK;
 }
 
@@ -1890,6 +2007,7 @@ void init_batch_host(kcfd_batch *b, kcfd_ctx *ctx, int slots, const kcfd_solve_o
     b->bp.rhotol = eps * eps;
     b->bp.precision = o->precision;
     b->bp.precond = o->precond;
+    b->fold = o->fold_colour0 > 0 || (o->fold_colour0 == 0 && o->precision == KCFD_MIXED);
     b->bp.max_refine = o->max_refine > 0 ? o->max_refine : 1000;
     b->bp.nsys = slots;
     b->bp.inner_maxiter = 5000;
@@ -1942,10 +2060,10 @@ int vectors_per_system(const kcfd_batch *b) { return b->bp.precond == KCFD_PC_IL
 size_t value_size(const kcfd_batch *b) { return b->bp.precision == KCFD_FP64 ? sizeof(double) : sizeof(float); }
 size_t vec_stride(const kcfd_batch *b, int n) { return align_up((size_t)std::max(n, 1) * value_size(b), 256); }
 size_t f64_stride(int n) { return align_up((size_t)std::max(n, 1) * sizeof(double), 256); }
-size_t slot_bytes(const kcfd_batch *b, int n, bool permuted) {
-    size_t v = (size_t)vectors_per_system(b) * vec_stride(b, n);
-    if (b->bp.precision == KCFD_MIXED) v += f64_stride(n);
-    if (permuted) v += 2 * f64_stride(n);
+size_t slot_bytes(const kcfd_batch *b, const SysDesc &d) {
+    size_t v = (size_t)vectors_per_system(b) * vec_stride(b, d.n);
+    if (b->bp.precision == KCFD_MIXED) v += f64_stride(d.n);
+    if (d.perm) v += 2 * f64_stride(d.n);
     return v;
 }
 
@@ -1974,6 +2092,14 @@ void describe_slot(kcfd_batch *b, int k, kcfd_mat *m, const double *bvec, double
     std::memset(&d, 0, sizeof(d));
     d.n = m->n;
     fill_desc(d, m);
+    // folded product: BiCGSTAB over a two-colour ILU(0); MIXED needs the
+    // FP64 colour-0 pivots of the mixed set-up
+    const bool mixed = b->bp.precision == KCFD_MIXED;
+    if (b->fold && b->bp.method == KCFD_METHOD_BICGSTAB && b->bp.precond == KCFD_PC_ILU0 && m->two_colour &&
+        m->ncolour0 > 0 && (!mixed || m->c0piv)) {
+        d.nid = m->ncolour0;
+        d.c0piv = mixed ? m->c0piv : nullptr;
+    }
     d.b = d.b_user = bvec;
     d.x = d.x_user = x;
     b->mats[k] = m;
@@ -2124,7 +2250,9 @@ int rebuild_tables(kcfd_batch *b, const std::vector<char> *only = nullptr) {
         KCFD_HIP(hipMemcpyAsync(b->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMemcpyAsync(b->d_desc, b->h_desc.data(), sizeof(SysDesc) * b->nsys, hipMemcpyHostToDevice, s));
     const int walk = b->opts.tile_walk == 2 ? 1 : 0;
-    const int counts[8] = {b->ntiles, b->nchunks_l, b->nchunks_u, b->nchunks_a, b->nchunks_b, walk, 0, 0};
+    int folded = 0;
+    for (int k = 0; k < b->nsys; ++k) folded |= in(k) && b->h_desc[k].nid > 0;
+    const int counts[8] = {b->ntiles, b->nchunks_l, b->nchunks_u, b->nchunks_a, b->nchunks_b, walk, folded, 0};
     KCFD_HIP(hipMemcpyAsync(b->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice, s));
     // the host arrays above are locals: finish the copies before they go
     KCFD_HIP(hipStreamSynchronize(s));
@@ -2154,7 +2282,7 @@ int build_batch(kcfd_ctx *ctx, int nsys, kcfd_mat *const *mats, const double *co
         if ((rc = check_mat(b, mats[k], k)) != KCFD_OK) return fail(rc);
         describe_slot(b, k, mats[k], bs[k], xs[k]);
         off[k] = slab;
-        slab += align_up(slot_bytes(b, mats[k]->n, b->h_desc[k].perm != nullptr), 256);
+        slab += align_up(slot_bytes(b, b->h_desc[k]), 256);
     }
     if (hipMalloc(&b->slab, slab ? slab : 256) != hipSuccess) {
         set_error("out of device memory for the batch vectors (%zu bytes)", slab);
@@ -2353,7 +2481,8 @@ int run_batch(kcfd_batch *b, kcfd_solve_result *res) {
             r.half_step = st.half;
             r.refinements = st.refinements;
             r.rel_resid = st.true_bb > 0 ? sqrt(st.true_rr / st.true_bb) : sqrt(st.true_rr);
-            r.bytes = model_bytes(b->mats[k], b->bp.precond, b->bp.precision, st, b->bp.method);
+            r.bytes = model_bytes(b->mats[k], b->bp.precond, b->bp.precision, st, b->bp.method,
+                                  b->h_desc[k].nid);
             r.ms = ms;
         }
     }
@@ -2420,7 +2549,7 @@ int solve_noalloc(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_o
     b->d_chunks_b = c.take<Chunk>(nb);
     b->d_lvl_l = c.take<const int *>(1);
     b->d_lvl_u = c.take<const int *>(1);
-    char *vecs = c.take<char>(slot_bytes(b, m->n, b->h_desc[0].perm != nullptr));
+    char *vecs = c.take<char>(slot_bytes(b, b->h_desc[0]));
     if (c.used > c.cap) {
         set_error("kcfd_solve_noalloc: a system of %d rows needs %zu bytes of workspace, more than the "
                   "%zu-byte scratch array holds", m->n, c.used, c.cap);
@@ -2527,7 +2656,7 @@ int kcfd_batch_attach(kcfd_batch *b, kcfd_mat *m, const double *bvec, double *x,
     KCFD_CHECK(check_mat(b, m, k));
     describe_slot(b, k, m, bvec, x);
     void *mem = nullptr;
-    const size_t bytes = slot_bytes(b, m->n, b->h_desc[k].perm != nullptr);
+    const size_t bytes = slot_bytes(b, b->h_desc[k]);
     if (hipMalloc(&mem, bytes) != hipSuccess) {
         (void)hipGetLastError();
         std::memset(&b->h_desc[k], 0, sizeof(SysDesc));
@@ -2610,7 +2739,8 @@ int kcfd_batch_detach(kcfd_batch *b, int slot, kcfd_solve_result *res) {
         res->refinements = st.refinements;
         res->rel_resid = st.true_done ? (st.true_bb > 0 ? sqrt(st.true_rr / st.true_bb) : sqrt(st.true_rr))
                                       : NAN;
-        res->bytes = model_bytes(b->mats[slot], b->bp.precond, b->bp.precision, st, b->bp.method);
+        res->bytes = model_bytes(b->mats[slot], b->bp.precond, b->bp.precision, st, b->bp.method,
+                                 b->h_desc[slot].nid);
         res->ms = 0.0;
     }
     KCFD_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -2655,7 +2785,7 @@ int kcfd_batch_bytes(kcfd_batch *b, long long *bytes) {
     KCFD_ARG(b && bytes, "null argument");
     long long v = (long long)b->slab_bytes;
     for (int k = 0; k < b->nsys; ++k)
-        if (b->slot_mem[k]) v += (long long)slot_bytes(b, b->sys_n[k], b->h_desc[k].perm != nullptr);
+        if (b->slot_mem[k]) v += (long long)slot_bytes(b, b->h_desc[k]);
     *bytes = v + (long long)b->cap_tiles * (8 + 8 * kMaxPartials) +
              (long long)b->nsys * (long long)(sizeof(SysDesc) + sizeof(SysState) + 2 * sizeof(int *)) +
              (long long)(b->cap_chunks_l + b->cap_chunks_u) * (long long)sizeof(Chunk);
@@ -2739,21 +2869,28 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
     KCFD_ARG(b && bytes, "bad argument");
     const double sv = b->bp.precision == KCFD_FP64 ? 8.0 : 4.0;
     double tot = 0.0;
-    for (auto *m : b->mats) {
+    for (size_t k = 0; k < b->mats.size(); ++k) {
+        const kcfd_mat *m = b->mats[k];
         if (!m) continue;
         const double n = m->n, nnz = m->nnz;
         const double MA = nnz * (sv + 4) + (n + 1) * 4;
+        // folded product: the product kernel's colour-0 rows read v (t) and
+        // rt (s) only, the colour-0 half of the apply writes v (t) and, in
+        // MIXED, reads the FP64 pivots
+        const double n0 = b->h_desc[k].nid, nnz0 = n0 > 0 ? (double)m->nnz_colour0 : 0.0;
+        const double fold = -(nnz0 * (sv + 4) + n0 * 4);
+        const double x64 = b->bp.precision == KCFD_MIXED && n0 > 0 ? 1.0 : 0.0;
         const double MFh = 0.5 * nnz * (sv + 4) + (n + 1) * 4;  // one triangle + its level list
         const bool cg = b->bp.method == KCFD_METHOD_CG;
         if (cg && phase == 1) tot += MA + 2 * n * sv;          // q = A p: p gathered, q written
         else if (cg && phase == 2) tot += 6 * n * sv;          // x, p, r, q read; x, r written
         else if (cg && phase == 4) tot += (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
-        else if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
+        else if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv + fold;
         else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
         else if (phase == 3) {
             if (m->two_colour)   // both kernels of the two-colour apply: L and U entries, z,
                                  // w written, the other colour's z / w gathered, u_ii
-                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv;
+                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv + n0 * sv + x64 * 8 * n0;
             else                 // the sync-free L solve (one triangle + its level list)
                 tot += MFh + 2 * n * sv;
         }
@@ -2782,7 +2919,8 @@ int kcfd_solve(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_opts
                  m->solver->h_desc[0].x_user == x && m->solver->opts.grid_per_cu == opts->grid_per_cu &&
                  m->solver->opts.trsv_per_cu == opts->trsv_per_cu &&
                  m->solver->opts.tile_walk == opts->tile_walk && m->solver->opts.method == opts->method &&
-                 m->solver->opts.rows_per_thread == opts->rows_per_thread;
+                 m->solver->opts.rows_per_thread == opts->rows_per_thread &&
+                 m->solver->opts.fold_colour0 == opts->fold_colour0;
     if (!reuse) {
         if (m->solver) kcfd_batch_destroy(m->solver);
         m->solver = nullptr;

@@ -195,6 +195,11 @@ struct kcfd_mat {
     // ncolour0..n-1) and the U layout the colour-0 rows, as single runs of
     // 64-row chunks; a colour-0 row's y is z itself.
     bool two_colour = false;
+    long long nnz_colour0 = 0;   // entries of the colour-0 rows (diagonal included)
+    // MIXED, two colours: FP64 pivots of the colour-0 rows for the folded
+    // BiCGSTAB product (kcfd_solve_opts.fold_colour0): a_ii + sum_j (a_ij -
+    // fl32(a_ij)), so that the row [pivot, fl32(a_ij)] has A's row sum
+    double *c0piv = nullptr;
     std::vector<int> h_chl_ptr, h_chu_ptr;
     int *chl_off = nullptr, *chu_off = nullptr, *chl_col = nullptr, *chu_col = nullptr;
     int *chl_map = nullptr, *chu_map = nullptr;

@@ -322,6 +322,22 @@ __global__ void k_chunk_vals(size_t ne, const int *__restrict__ map, const int *
     if (v32) v32[e] = (float)v;
 }
 
+// Colour-0 pivots of the folded mixed BiCGSTAB product, rows i < n0 of
+// P A P^T (two colours: these rows have no L entries, so their factor row
+// is A's own row): piv_i = a_ii + sum_{j != i} (a_ij - fl32(a_ij)).  The FP32
+// U entries of the row are fl32(a_ij), so the row [piv_i, fl32(a_ij)] sums to
+// A's row sum up to FP64 rounding, as the difference-form SELL copy does.
+__global__ void k_colour0_pivots(int n0, const int *__restrict__ ip, const int *__restrict__ dpos,
+                                 const double *__restrict__ lu, double *__restrict__ piv) {
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n0) return;
+    double d = lu[dpos[i]];
+    for (int q = ip[i]; q < ip[i + 1]; ++q)
+        if (q != dpos[i]) d += lu[q] - (double)(float)lu[q];
+    piv[i] = d;
+}
+
 // Ordering, level schedule and chunked solve layout: they depend on the
 // structure only and are built once per ordering.
 int ilu0_structure(kcfd_mat *m) {
@@ -353,6 +369,7 @@ int ilu0_structure(kcfd_mat *m) {
         m->two_colour = m->perm && m->ncolors == 2;
         if (m->two_colour) {
             const int n0 = m->ncolour0;
+            m->nnz_colour0 = m->hf_indptr[n0] - m->hf_indptr[0];
             rl.resize(n - n0);
             for (int k = n0; k < n; ++k) rl[k - n0] = k;
             ru.resize(n0);
@@ -452,6 +469,11 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
         if (n)
             hipLaunchKernelGGL(k_chunk_vals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (size_t)n,
                                m->f_diagpos, nullptr, m->lu, m->ud64, nullptr, want32 ? m->ud32 : nullptr);
+        if (want32 && m->two_colour && m->ncolour0 > 0) {
+            if (!m->c0piv) KCFD_HIP(hipMalloc(&m->c0piv, sizeof(double) * m->ncolour0));
+            hipLaunchKernelGGL(k_colour0_pivots, dim3((m->ncolour0 + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                               m->ncolour0, m->f_indptr, m->f_diagpos, m->lu, m->c0piv);
+        }
         KCFD_HIP(hipGetLastError());
     }
     // pivot check: first row with a zero or NaN u_ii (or n if none)
@@ -646,6 +668,7 @@ extern "C" int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src) {
     KCFD_CHECK(dup_ints(src->chl_map, (size_t)src->nent_l, &dst->chl_map, dst->ctx->stream));
     dst->two_colour = src->two_colour;
     dst->ncolour0 = src->ncolour0;
+    dst->nnz_colour0 = src->nnz_colour0;
     if (src->ch16) {
         KCFD_HIP(hipMalloc(&dst->chl_o16, sizeof(unsigned short) * std::max(src->nent_l, 1)));
         KCFD_HIP(hipMalloc(&dst->chu_o16, sizeof(unsigned short) * std::max(src->nent_u, 1)));

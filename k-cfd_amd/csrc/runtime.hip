@@ -338,6 +338,7 @@ int mat_free_precond(kcfd_mat *m) {
     hipFree(m->chl_v32); m->chl_v32 = nullptr;
     hipFree(m->chu_v32); m->chu_v32 = nullptr;
     hipFree(m->ud32); m->ud32 = nullptr;
+    hipFree(m->c0piv); m->c0piv = nullptr;
     m->have32 = false;
     m->pc_kind = KCFD_PC_NONE;
     return KCFD_OK;
@@ -365,6 +366,8 @@ int mat_free_ilu_structure(kcfd_mat *m) {
     hipFree(m->chl_v32); m->chl_v32 = nullptr;
     hipFree(m->chu_v32); m->chu_v32 = nullptr;
     hipFree(m->ud32); m->ud32 = nullptr;
+    hipFree(m->c0piv); m->c0piv = nullptr;
+    m->nnz_colour0 = 0;
     hipFree(m->chl_o16); m->chl_o16 = nullptr;
     hipFree(m->chu_o16); m->chu_o16 = nullptr;
     hipFree(m->chl_base); m->chl_base = nullptr;

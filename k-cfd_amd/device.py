@@ -46,7 +46,8 @@ def fp64_continuation(mat, b_ptr, x_ptr, opts, mixed_res):
 
 def default_opts(rtol=1e-8, atol=0.0, maxiter=0, precond="jacobi", precision="fp64",
                  inner_rtol=0.1, max_refine=1000, check_every=8, grid_per_cu=0,
-                 trsv_per_cu=0, tile_walk=0, method="bicgstab", restart=0, rows_per_thread=0) -> L.SolveOpts:
+                 trsv_per_cu=0, tile_walk=0, method="bicgstab", restart=0, rows_per_thread=0,
+                 fold_colour0=0) -> L.SolveOpts:
     o = L.SolveOpts()
     o.rtol = float(rtol)
     o.atol = float(atol)
@@ -63,6 +64,7 @@ def default_opts(rtol=1e-8, atol=0.0, maxiter=0, precond="jacobi", precision="fp
     o.method = L.METHODS[method] if isinstance(method, str) else int(method)
     o.restart = int(restart or 0)
     o.rows_per_thread = int(rows_per_thread or 0)
+    o.fold_colour0 = int(fold_colour0 or 0)
     return o
 
 
