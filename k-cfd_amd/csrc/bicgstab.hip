@@ -1324,8 +1324,8 @@ __device__ __forceinline__ void publish(V *p, V v) {
 // it into KCFD_ESTATE) and uses a quiet NaN, so that its own row is still
 // published and every wave of the launch runs to its end.
 template <typename V>
-__device__ __forceinline__ V sub_ready(V acc, const FacRef &f, int base, int w, const V *vec, int limit,
-                                       unsigned int *fail) {
+__device__ __forceinline__ double sub_ready(double acc, const FacRef &f, int base, int w, const V *vec, int limit,
+                                            unsigned int *fail) {
 #pragma clang fp contract(off)
     using U = typename Sentinel<V>::U;
     constexpr int G = 8;
@@ -1353,7 +1353,7 @@ __device__ __forceinline__ V sub_ready(V acc, const FacRef &f, int base, int w, 
                     bits[k] = __hip_atomic_load(reinterpret_cast<const U *>(vec + c[k]), __ATOMIC_RELAXED,
                                                 __HIP_MEMORY_SCOPE_AGENT);
                 }
-                acc -= av[k] * __builtin_bit_cast(V, bits[k]);
+                acc -= (double)av[k] * (double)__builtin_bit_cast(V, bits[k]);
             }
     }
     return acc;
@@ -1406,13 +1406,13 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
                     zi = vec<V>(d.r)[i];
                 }
                 V *y = (V *)d.y;
-                const V acc = sub_ready<V>(zi, fac_l(d), o0 + lane, w, y, limit, fail);
+                const V acc = (V)sub_ready<V>((double)zi, fac_l(d), o0 + lane, w, y, limit, fail);
                 publish<V>(y + i, acc);
             } else {
                 V *y = (V *)d.y;
                 V *wv = (V *)(WHICH == 1 ? d.sh : d.ph);
-                const V acc = sub_ready<V>(glw(y)[i], fac_u(d), o0 + lane, w, wv, limit, fail);
-                publish<V>(wv + i, acc / gl(udiag<V>(d))[i]);
+                const double acc = sub_ready<V>((double)glw(y)[i], fac_u(d), o0 + lane, w, wv, limit, fail);
+                publish<V>(wv + i, (V)(acc / (double)gl(udiag<V>(d))[i]));
                 glw(y)[i] = sentinel_value<V>();
             }
         }
@@ -1425,7 +1425,7 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
 // past the width re-load the last entry; padding (column -1) gathers entry
 // 0.  Neither is accumulated.
 template <typename V, int G, typename F, typename S>
-__device__ __forceinline__ void sub_group(V &acc, const F &f, int base, int k0, int last, int valid,
+__device__ __forceinline__ void sub_group(double &acc, const F &f, int base, int k0, int last, int valid,
                                           const S &src) {
 #pragma clang fp contract(off)
     int c[G];
@@ -1436,11 +1436,11 @@ __device__ __forceinline__ void sub_group(V &acc, const F &f, int base, int k0, 
     for (int k = 0; k < G; ++k) xv[k] = src(max(c[k], 0));
 #pragma unroll
     for (int k = 0; k < G; ++k)
-        if (k < valid && c[k] >= 0) acc -= av[k] * xv[k];
+        if (k < valid && c[k] >= 0) acc -= (double)av[k] * (double)xv[k];
 }
 
 template <typename V, typename F, typename S>
-__device__ __forceinline__ V sub_plain(V acc, const F &f, int base, int w, const S &src) {
+__device__ __forceinline__ double sub_plain(double acc, const F &f, int base, int w, const S &src) {
     if (w == 6) {
         sub_group<V, 6>(acc, f, base, 0, 5, 6, src);
     } else if (w == 4) {
@@ -1475,19 +1475,19 @@ __device__ __forceinline__ void ilu2_row(const SysDesc &d, const Chunk &ch, int 
     const V zi = src(i);
     if (zout) zout[i] = zi;
     const PlainSrc<V> wsrc{w};
-    V v;
+    double v;
     const unsigned short *o16 = nullptr;
     if constexpr (sizeof(V) == sizeof(float)) o16 = SECOND ? d.chu_o16 : d.chl_o16;
     if (o16) {
         const FacRef16 f{SECOND ? d.chu_v32 : d.chl_v32, o16,
                          uniform(gl(SECOND ? d.chu_base : d.chl_base)[ch.local])};
-        v = SECOND ? sub_plain<V>(zi, f, o0 + (i - ch.start), wd, wsrc)
-                   : sub_plain<V>(zi, f, o0 + (i - ch.start), wd, src);
+        v = SECOND ? sub_plain<V>((double)zi, f, o0 + (i - ch.start), wd, wsrc)
+                   : sub_plain<V>((double)zi, f, o0 + (i - ch.start), wd, src);
     } else {
-        v = SECOND ? sub_plain<V>(zi, fac_u(d), o0 + (i - ch.start), wd, wsrc)
-                   : sub_plain<V>(zi, fac_l(d), o0 + (i - ch.start), wd, src);
+        v = SECOND ? sub_plain<V>((double)zi, fac_u(d), o0 + (i - ch.start), wd, wsrc)
+                   : sub_plain<V>((double)zi, fac_l(d), o0 + (i - ch.start), wd, src);
     }
-    w[i] = v / gl(udiag<V>(d))[i];
+    w[i] = (V)(v / (double)gl(udiag<V>(d))[i]);
 }
 
 // A colour-0 row of a folded apply (SECOND), which also writes the row's
@@ -1501,7 +1501,7 @@ __device__ __forceinline__ void ilu2_row(const SysDesc &d, const Chunk &ch, int 
 // [D_i, fl32(a_ij)] keeps A's row sum, as the difference-form copy of A
 // does.  The apply's own arithmetic is unchanged.
 template <typename V, int G, typename F, typename S>
-__device__ __forceinline__ void sub_group_prod(V &acc, double &prod, const F &f, int base, int k0, int last,
+__device__ __forceinline__ void sub_group_prod(double &acc, double &prod, const F &f, int base, int k0, int last,
                                                int valid, const S &src) {
 #pragma clang fp contract(off)
     int c[G];
@@ -1513,13 +1513,13 @@ __device__ __forceinline__ void sub_group_prod(V &acc, double &prod, const F &f,
 #pragma unroll
     for (int k = 0; k < G; ++k)
         if (k < valid && c[k] >= 0) {
-            acc -= av[k] * xv[k];
+            acc -= (double)av[k] * (double)xv[k];
             prod += (double)av[k] * (double)xv[k];
         }
 }
 
 template <typename V, typename F, typename S>
-__device__ __forceinline__ V sub_plain_prod(V acc, double &prod, const F &f, int base, int w, const S &src) {
+__device__ __forceinline__ double sub_plain_prod(double acc, double &prod, const F &f, int base, int w, const S &src) {
     if (w == 6) {
         sub_group_prod<V, 6>(acc, prod, f, base, 0, 5, 6, src);
     } else if (w == 4) {
@@ -1539,25 +1539,67 @@ __device__ __forceinline__ void ilu2_row_fold(const SysDesc &d, const Chunk &ch,
     const V zi = src(i);
     if (zout) zout[i] = zi;
     const PlainSrc<V> wsrc{w};
-    double pr = 0.0;
-    V v;
+    double pr = 0.0, v;
     const unsigned short *o16 = nullptr;
     if constexpr (sizeof(V) == sizeof(float)) o16 = d.chu_o16;
     if (o16) {
         const FacRef16 f{d.chu_v32, o16, uniform(gl(d.chu_base)[ch.local])};
-        v = sub_plain_prod<V>(zi, pr, f, o0 + (i - ch.start), wd, wsrc);
+        v = sub_plain_prod<V>((double)zi, pr, f, o0 + (i - ch.start), wd, wsrc);
     } else {
-        v = sub_plain_prod<V>(zi, pr, fac_u(d), o0 + (i - ch.start), wd, wsrc);
+        v = sub_plain_prod<V>((double)zi, pr, fac_u(d), o0 + (i - ch.start), wd, wsrc);
     }
     const V ud = gl(udiag<V>(d))[i];
-    const V wi = v / ud;
+    const V wi = (V)(v / (double)ud);
     w[i] = wi;
     const double dg = sizeof(V) == sizeof(float) ? gl(d.c0piv)[i] : (double)ud;
     prod[i] = (V)(pr + dg * (double)wi);
 }
 
+template <typename V, int WHICH>
+__device__ __forceinline__ bool ilu2_wants(int status) {
+    return WHICH == 0 ? (status == S_RUN || status == S_START || status == S_RESUME)
+                      : (WHICH == 1 ? status == S_RUN : (status == S_RUN || status == S_START));
+}
+
+// One chunk alone (the general path): one row per lane.
 template <typename V, bool SECOND, int WHICH>
-__global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys,
+__device__ __forceinline__ void ilu2_chunk(const SysDesc *__restrict__ sys, const SysState *__restrict__ sts,
+                                           const Chunk &ch, int lane) {
+#pragma clang fp contract(off)
+    const SysDesc &d = sys[ch.sys];
+    const SysState &st = sts[ch.sys];
+    if (!ilu2_wants<V, WHICH>(st.status) || lane >= ch.count) return;
+    const int i = ch.start + lane;       // row of P A P^T
+    KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 1 ? d.sh : d.ph);
+    const int *off = SECOND ? d.chu_off : d.chl_off;
+    const int o0 = uniform(gl(off)[ch.local]);
+    const int wd = uniform((gl(off)[ch.local + 1] - o0) >> 6);
+    if constexpr (SECOND && WHICH != 2) {
+        if (d.nid > 0) {   // folded: the colour-0 rows of A M z as well
+            if constexpr (WHICH == 0) {
+                const PlainSrc<V> src{vec<V>(d.p)};
+                ilu2_row_fold<V>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w, vec<V>(d.v));
+            } else {
+                const SSrc<V> src{vec<V>(d.r), vec<V>(d.v), (V)st.alpha};
+                ilu2_row_fold<V>(d, ch, i, o0, wd, src, vec<V>(d.s), w, vec<V>(d.t));
+            }
+            return;
+        }
+    }
+    if constexpr (WHICH == 0) {
+        const PlainSrc<V> src{vec<V>(d.p)};
+        ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w);
+    } else if constexpr (WHICH == 1) {
+        const SSrc<V> src{vec<V>(d.r), vec<V>(d.v), (V)st.alpha};
+        ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, vec<V>(d.s), w);
+    } else {
+        const PlainSrc<V> src{vec<V>(d.r)};
+        ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w);
+    }
+}
+
+template <typename V, bool SECOND, int WHICH>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_ilu2(const SysDesc *__restrict__ sys,
                                                  const SysState *__restrict__ sts,
                                                  const Chunk *__restrict__ chunks,
                                                  const int *__restrict__ counts) {
@@ -1568,43 +1610,7 @@ __global__ __launch_bounds__(kBlock) void k_ilu2(const SysDesc *__restrict__ sys
     const int c0 = (int)(blockIdx.x % kXcds) * per, c1 = min(c0 + per, nchunks);
     const int wx = (int)(gridDim.x / kXcds) * kWaves;   // waves per XCD
     const int lw = uniform((int)(blockIdx.x / kXcds) * kWaves + (int)(threadIdx.x >> 6));
-    for (int c = c0 + lw; c < c1; c += wx) {
-        const Chunk ch = chunks[c];
-        const SysDesc &d = sys[ch.sys];
-        const int status = sts[ch.sys].status;
-        const bool want = WHICH == 0 ? (status == S_RUN || status == S_START || status == S_RESUME)
-                                     : (WHICH == 1 ? status == S_RUN : (status == S_RUN || status == S_START));
-        if (want && lane < ch.count) {
-            const int i = ch.start + lane;       // row of P A P^T
-            KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 1 ? d.sh : d.ph);
-            const int *off = SECOND ? d.chu_off : d.chl_off;
-            const int o0 = uniform(gl(off)[ch.local]);
-            const int wd = uniform((gl(off)[ch.local + 1] - o0) >> 6);
-            const SysState &st = sts[ch.sys];
-            if constexpr (SECOND && WHICH != 2) {
-                if (d.nid > 0) {   // folded: the colour-0 rows of A M z as well
-                    if constexpr (WHICH == 0) {
-                        const PlainSrc<V> src{vec<V>(d.p)};
-                        ilu2_row_fold<V>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w, vec<V>(d.v));
-                    } else {
-                        const SSrc<V> src{vec<V>(d.r), vec<V>(d.v), (V)st.alpha};
-                        ilu2_row_fold<V>(d, ch, i, o0, wd, src, vec<V>(d.s), w, vec<V>(d.t));
-                    }
-                    continue;
-                }
-            }
-            if constexpr (WHICH == 0) {
-                const PlainSrc<V> src{vec<V>(d.p)};
-                ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w);
-            } else if constexpr (WHICH == 1) {
-                const SSrc<V> src{vec<V>(d.r), vec<V>(d.v), (V)st.alpha};
-                ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, vec<V>(d.s), w);
-            } else {
-                const PlainSrc<V> src{vec<V>(d.r)};
-                ilu2_row<V, SECOND>(d, ch, i, o0, wd, src, (KCFD_GLOBAL V *)nullptr, w);
-            }
-        }
-    }
+    for (int c = c0 + lw; c < c1; c += wx) ilu2_chunk<V, SECOND, WHICH>(sys, sts, chunks[c], lane);
 }
 
 template <typename V>
