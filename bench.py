@@ -35,12 +35,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_spmv_p launch on the 128^3 mixed default workload, from the
-# FETCH_SIZE / WRITE_SIZE counter passes: the narrow-index FP32 SELL copy
-# (6 bytes an entry, the library's default) and the packed one (8 bytes an
-# entry, KCFD_SELL16=0)
-TRAFFIC_SPMV_P_MIXED_HEX128 = 115.1e6          # 106.6 MB read + 8.5 MB written
-TRAFFIC_SPMV_P_MIXED_HEX128_PACKED = 153.9e6   # 145.4 MB read + 8.5 MB written
+# HBM bytes per launch on the 128^3 mixed default workload (folded products,
+# narrow-index FP32 copies), from the FETCH_SIZE / WRITE_SIZE counter passes
+# scaled by the copy-kernel calibration: one ILU(0) apply (the two k_ilu2
+# launches of the p apply) and one k_spmv_p.  None: not measured for the
+# current kernels.
+TRAFFIC_ILU_APPLY_MIXED_HEX128 = None
+TRAFFIC_SPMV_P_MIXED_HEX128 = None
 
 
 def log(*a):
@@ -435,11 +436,14 @@ def run_pressure(args, ctx, case):
 
     mixed = results.get("mixed") or results[precs[0]]
     fp64 = results.get("fp64") or results[precs[0]]
-    # roofline kernel: the SpMV k_spmv_p (v = A M p with the rt.v partials);
-    # with k_spmv_s, the same operation on s, it is the largest part of an
-    # iteration.  Timed per launch with HIP events on the library stream
-    # inside real iterations; its algorithmic bytes are M_A + 3 n s_v (ILU)
-    # or M_A + 4 n s_v (Jacobi).
+    # roofline kernel: the dominant kernel of an iteration.  With the
+    # two-colour ILU(0) that is k_ilu2: two launches per apply (colour-1
+    # rows, then colour-0 rows), two applies per iteration, 40% of the mixed
+    # iteration's kernel time.  The unit is one apply (both launches), timed
+    # with HIP events in the launches' own dispatch packets inside real
+    # iterations; its algorithmic bytes are the factor entries at 4-byte
+    # indices plus the vector streams (kcfd_batch_kernel_bytes, phase 3).
+    # Without ILU(0) (and beside it) the SpMV k_spmv_p.
     kern = mixed["kernels"]
     solve_gbs = mixed["bytes"] / (mixed["solve_ms"] * 1e-3) / 1e9
     if args.method == "gmres":
@@ -450,29 +454,40 @@ def run_pressure(args, ctx, case):
                     "unit": "GB/s", "frac": round(solve_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                     "kernel": "whole GMRES solve"}
     else:
-        dk = kern["spmv_p"]
+        two_colour = args.precond == "ilu0" and args.order == "color"
+        name = "ilu_apply" if two_colour else "spmv_p"
+        dk = kern[name]
         achieved = dk["gbs"]
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_spmv_p",
-                    "avg_us": round(1e3 * dk["avg_ms"], 2), "bytes_per_launch": dk["bytes"],
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": "k_ilu2" if two_colour else "k_spmv_p",
+                    "work_unit": ("one ILU(0) apply: the colour-1 and colour-0 launches of k_ilu2"
+                                  if two_colour else "one launch"),
+                    "avg_us": round(1e3 * dk["avg_ms"], 2), "bytes_per_unit": dk["bytes"],
                     "solve_avg_gbs": round(solve_gbs, 1)}
-        # HBM bytes per k_spmv_p launch measured by the FETCH_SIZE / WRITE_SIZE
-        # counter passes on this exact workload (default shape, mixed,
+        if two_colour:
+            sp = kern["spmv_p"]
+            roofline["k_spmv_p"] = {"avg_us": round(1e3 * sp["avg_ms"], 2), "bytes": sp["bytes"],
+                                    "gbs": round(sp["gbs"], 1), "frac": round(sp["gbs"] / HBM_PEAK_GBS, 4)}
+        # HBM bytes per unit measured by the FETCH_SIZE / WRITE_SIZE counter
+        # passes on this exact workload (default shape, mixed, folded,
         # two-colour ILU(0), 128^3), scaled by the copy-kernel calibration
-        if (args.method == "bicgstab" and args.n == 128 and args.precond == "ilu0" and args.order == "color"
-                and "mixed" in results and not (args.grid_per_cu or args.trsv_per_cu or args.tile_walk
-                                                 or args.rows_per_thread not in (0, 2))):
-            packed = os.environ.get("KCFD_SELL16", "1").startswith("0")
-            traffic = TRAFFIC_SPMV_P_MIXED_HEX128_PACKED if packed else TRAFFIC_SPMV_P_MIXED_HEX128
+        if (args.method == "bicgstab" and args.n == 128 and two_colour and "mixed" in results
+                and args.fold == 0 and not os.environ.get("KCFD_SELL16", "1").startswith("0")
+                and not (args.grid_per_cu or args.trsv_per_cu or args.tile_walk
+                         or args.rows_per_thread not in (0, 2))
+                and TRAFFIC_ILU_APPLY_MIXED_HEX128 is not None):
+            traffic = TRAFFIC_ILU_APPLY_MIXED_HEX128
             roofline["traffic"] = traffic
             # the rate of the bytes the kernel actually moves (the algorithmic
-            # model counts 4-byte column indices; the narrow-index copy
-            # streams 2-byte offsets)
+            # model counts 4-byte column indices; the narrow-index copies
+            # stream 2-byte offsets)
             roofline["traffic_gbs"] = round(traffic / (dk["avg_ms"] * 1e-3) / 1e9, 1)
             roofline["traffic_frac"] = round(roofline["traffic_gbs"] / HBM_PEAK_GBS, 4)
             roofline["traffic_source"] = ("profiles/hbm_traffic_bicgstab_ilu0_mixed_hex128.md: FETCH_SIZE x 2.000 "
-                                          "+ WRITE_SIZE x 1.000 KiB per dispatch, median working dispatch"
-                                          + (" (packed SELL copy, KCFD_SELL16=0)" if packed else ""))
+                                          "+ WRITE_SIZE x 1.000 KiB per dispatch, median working dispatch")
+            if TRAFFIC_SPMV_P_MIXED_HEX128 is not None:
+                roofline["k_spmv_p"]["traffic"] = TRAFFIC_SPMV_P_MIXED_HEX128
     err = (float(np.linalg.norm(sols["mixed"] - sols["fp64"]) / np.linalg.norm(sols["fp64"]))
            if "mixed" in sols and "fp64" in sols else None)
     return dict(mixed=mixed, fp64=fp64, precs=precs, n=n, nnz=nnz, roofline=roofline,

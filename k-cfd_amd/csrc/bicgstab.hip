@@ -144,9 +144,14 @@ struct BatchParams {
 constexpr int kTrsvFail = 6;
 constexpr int kDefaultPollLimit = 1 << 20;
 
-// ILU trisolve work chunk: up to 64 consecutive rows of one system's level list
+// ILU trisolve work chunk: up to 64 consecutive rows of one system's level
+// list.  local: the chunk's index in its matrix's layout; o0, wd: its first
+// entry and width there (kcfd_mat::chl_off / chu_off), cb: the base column
+// of a narrow-index two-colour chunk -- carried here so that a wave's
+// entry loads follow the chunk's own load, not another dependent load.
 struct Chunk {
-    int sys, start, count, local;   // local: the chunk's index in its matrix's layout
+    int sys, start, count, local;
+    int o0, wd, cb, pad;
 };
 
 // ----------------------------------------------------------- tile walk
@@ -1479,8 +1484,7 @@ __device__ __forceinline__ void ilu2_row(const SysDesc &d, const Chunk &ch, int 
     const unsigned short *o16 = nullptr;
     if constexpr (sizeof(V) == sizeof(float)) o16 = SECOND ? d.chu_o16 : d.chl_o16;
     if (o16) {
-        const FacRef16 f{SECOND ? d.chu_v32 : d.chl_v32, o16,
-                         uniform(gl(SECOND ? d.chu_base : d.chl_base)[ch.local])};
+        const FacRef16 f{SECOND ? d.chu_v32 : d.chl_v32, o16, ch.cb};
         v = SECOND ? sub_plain<V>((double)zi, f, o0 + (i - ch.start), wd, wsrc)
                    : sub_plain<V>((double)zi, f, o0 + (i - ch.start), wd, src);
     } else {
@@ -1543,7 +1547,7 @@ __device__ __forceinline__ void ilu2_row_fold(const SysDesc &d, const Chunk &ch,
     const unsigned short *o16 = nullptr;
     if constexpr (sizeof(V) == sizeof(float)) o16 = d.chu_o16;
     if (o16) {
-        const FacRef16 f{d.chu_v32, o16, uniform(gl(d.chu_base)[ch.local])};
+        const FacRef16 f{d.chu_v32, o16, ch.cb};
         v = sub_plain_prod<V>((double)zi, pr, f, o0 + (i - ch.start), wd, wsrc);
     } else {
         v = sub_plain_prod<V>((double)zi, pr, fac_u(d), o0 + (i - ch.start), wd, wsrc);
@@ -1571,9 +1575,7 @@ __device__ __forceinline__ void ilu2_chunk(const SysDesc *__restrict__ sys, cons
     if (!ilu2_wants<V, WHICH>(st.status) || lane >= ch.count) return;
     const int i = ch.start + lane;       // row of P A P^T
     KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 1 ? d.sh : d.ph);
-    const int *off = SECOND ? d.chu_off : d.chl_off;
-    const int o0 = uniform(gl(off)[ch.local]);
-    const int wd = uniform((gl(off)[ch.local + 1] - o0) >> 6);
+    const int o0 = ch.o0, wd = ch.wd;
     if constexpr (SECOND && WHICH != 2) {
         if (d.nid > 0) {   // folded: the colour-0 rows of A M z as well
             if constexpr (WHICH == 0) {
@@ -2153,8 +2155,14 @@ void build_sys_tables(kcfd_batch *b, int k) {
     if (b->bp.precond != KCFD_PC_ILU0) return;
     if (m->two_colour) {
         const int n0 = m->ncolour0, n1 = m->n - m->ncolour0;
-        for (int s0 = 0; s0 < n1; s0 += 64) t.ca.push_back(Chunk{k, n0 + s0, std::min(64, n1 - s0), s0 / 64});
-        for (int s0 = 0; s0 < n0; s0 += 64) t.cb.push_back(Chunk{k, s0, std::min(64, n0 - s0), s0 / 64});
+        auto chunk = [&](int start, int count, int local, bool upper) {
+            const std::vector<int> &off = upper ? m->h_chu_off : m->h_chl_off;
+            const std::vector<int> &base = upper ? m->h_chu_base : m->h_chl_base;
+            return Chunk{k, start, count, local, off[local], (off[local + 1] - off[local]) >> 6,
+                         m->ch16 ? base[local] : 0, 0};
+        };
+        for (int s0 = 0; s0 < n1; s0 += 64) t.ca.push_back(chunk(n0 + s0, std::min(64, n1 - s0), s0 / 64, false));
+        for (int s0 = 0; s0 < n0; s0 += 64) t.cb.push_back(chunk(s0, std::min(64, n0 - s0), s0 / 64, true));
         return;
     }
     for (int upper = 0; upper < 2; ++upper) {
@@ -2163,9 +2171,13 @@ void build_sys_tables(kcfd_batch *b, int k) {
         const int nl = upper ? m->nlev_u : m->nlev_l;
         std::vector<std::vector<Chunk>> &lv = upper ? t.lvl_u : t.lvl_l;
         lv.assign(nl, {});
+        const std::vector<int> &off = upper ? m->h_chu_off : m->h_chl_off;
         for (int l = 0; l < nl; ++l)
-            for (int s0 = lp[l]; s0 < lp[l + 1]; s0 += 64)
-                lv[l].push_back(Chunk{k, s0, std::min(64, lp[l + 1] - s0), cp[l] + (s0 - lp[l]) / 64});
+            for (int s0 = lp[l]; s0 < lp[l + 1]; s0 += 64) {
+                const int local = cp[l] + (s0 - lp[l]) / 64;
+                lv[l].push_back(Chunk{k, s0, std::min(64, lp[l + 1] - s0), local, off[local],
+                                      (off[local + 1] - off[local]) >> 6, 0, 0});
+            }
     }
 }
 

@@ -201,6 +201,9 @@ struct kcfd_mat {
     // fl32(a_ij)), so that the row [pivot, fl32(a_ij)] has A's row sum
     double *c0piv = nullptr;
     std::vector<int> h_chl_ptr, h_chu_ptr;
+    // host copies of the per-chunk entry offsets and (narrow-index form)
+    // base columns, which the batch's chunk tables carry
+    std::vector<int> h_chl_off, h_chu_off, h_chl_base, h_chu_base;
     int *chl_off = nullptr, *chu_off = nullptr, *chl_col = nullptr, *chu_col = nullptr;
     int *chl_map = nullptr, *chu_map = nullptr;
     double *chl_v64 = nullptr, *chu_v64 = nullptr, *ud64 = nullptr;

@@ -387,6 +387,7 @@ int ilu0_structure(kcfd_mat *m) {
             chunk_layout(m, rl, lvl_l, false, m->h_chl_ptr, off, col, map);
             m->nch_l = (int)off.size() - 1;
             m->nent_l = (int)col.size();
+            m->h_chl_off = off;
             if (narrow) narrow = chunk_offsets16(off, col, lo16, lbase);
             KCFD_CHECK(upload_ints(off, &m->chl_off, s));
             KCFD_CHECK(upload_ints(col, &m->chl_col, s));
@@ -395,11 +396,14 @@ int ilu0_structure(kcfd_mat *m) {
             chunk_layout(m, ru, lvl_u, true, m->h_chu_ptr, off, col, map);
             m->nch_u = (int)off.size() - 1;
             m->nent_u = (int)col.size();
+            m->h_chu_off = off;
             if (narrow) narrow = chunk_offsets16(off, col, uo16, ubase);
             KCFD_CHECK(upload_ints(off, &m->chu_off, s));
             KCFD_CHECK(upload_ints(col, &m->chu_col, s));
             KCFD_CHECK(upload_ints(map, &m->chu_map, s));
             if (narrow) {
+                m->h_chl_base = lbase;
+                m->h_chu_base = ubase;
                 KCFD_CHECK(upload_vec(lo16, &m->chl_o16, s));
                 KCFD_CHECK(upload_vec(uo16, &m->chu_o16, s));
                 KCFD_CHECK(upload_vec(lbase, &m->chl_base, s));
@@ -685,6 +689,10 @@ extern "C" int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src) {
     KCFD_CHECK(dup_ints(src->chu_map, (size_t)src->nent_u, &dst->chu_map, dst->ctx->stream));
     dst->h_chl_ptr = src->h_chl_ptr;
     dst->h_chu_ptr = src->h_chu_ptr;
+    dst->h_chl_off = src->h_chl_off;
+    dst->h_chu_off = src->h_chu_off;
+    dst->h_chl_base = src->h_chl_base;
+    dst->h_chu_base = src->h_chu_base;
     dst->nch_l = src->nch_l;
     dst->nch_u = src->nch_u;
     dst->nent_l = src->nent_l;

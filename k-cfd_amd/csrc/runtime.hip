@@ -355,6 +355,10 @@ int mat_free_ilu_structure(kcfd_mat *m) {
     }
     m->h_chl_ptr.clear();
     m->h_chu_ptr.clear();
+    m->h_chl_off.clear();
+    m->h_chu_off.clear();
+    m->h_chl_base.clear();
+    m->h_chu_base.clear();
     m->nch_l = m->nch_u = m->nent_l = m->nent_u = 0;
     double **v64[] = {&m->chl_v64, &m->chu_v64, &m->ud64};
     for (double **a : v64) {
