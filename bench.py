@@ -40,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # scaled by the copy-kernel calibration: one ILU(0) apply (the two k_ilu2
 # launches of the p apply) and one k_spmv_p.  None: not measured for the
 # current kernels.
-TRAFFIC_ILU_APPLY_MIXED_HEX128 = None
-TRAFFIC_SPMV_P_MIXED_HEX128 = None
+TRAFFIC_ILU_APPLY_MIXED_HEX128 = 124.4e6   # 55.9 MB colour-1 launch + 68.5 MB colour-0 launch
+TRAFFIC_SPMV_P_MIXED_HEX128 = 70.3e6       # 65.9 MB read + 4.3 MB written
 
 
 def log(*a):
@@ -423,14 +423,23 @@ def run_pressure(args, ctx, case):
         bh = L.ctypes.c_void_p()
         L.check(L.lib().kcfd_mat_solver(A.h, L.ctypes.byref(bh)))
         kern = {}
-        ilu_name = "ilu_apply" if args.order == "color" else "ilu_lower_solve"
-        for phase, name in [(1, "spmv_p"), (2, "xr_update")] + ([(3, ilu_name)] if args.precond == "ilu0" else []):
+        two_colour = args.precond == "ilu0" and args.order == "color"
+        phases = [(1, "spmv_p"), (2, "xr_update"), (4, "p_update")]
+        if args.precond == "ilu0":
+            phases += ([(3, "ilu_colour1"), (5, "ilu_colour0")] if two_colour and args.method == "bicgstab"
+                       else [(3, "ilu_apply" if two_colour else "ilu_lower_solve")])
+        for phase, name in phases:
             ms = L.ctypes.c_double()
             by = L.ctypes.c_double()
             L.check(L.lib().kcfd_batch_profile(bh, phase, 10, L.ctypes.byref(ms)))
             L.check(L.lib().kcfd_batch_kernel_bytes(bh, phase, L.ctypes.byref(by)))
             kern[name] = dict(avg_ms=ms.value, bytes=by.value,
                               gbs=by.value / (ms.value * 1e-3) / 1e9)
+        if "ilu_colour1" in kern:   # one apply = its two launches
+            a, c = kern["ilu_colour1"], kern["ilu_colour0"]
+            ms = a["avg_ms"] + c["avg_ms"]
+            kern["ilu_apply"] = dict(avg_ms=ms, bytes=a["bytes"] + c["bytes"],
+                                     gbs=(a["bytes"] + c["bytes"]) / (ms * 1e-3) / 1e9)
         results[prec]["kernels"] = kern
         log(prec, "kernels", kern)
 
@@ -599,10 +608,20 @@ def run_batched(args, ctx, cases, steps):
         L.check(L.lib().kcfd_batch_kernel_bytes(bobj.h, 1, L.ctypes.byref(by)))
         summ["spmv"] = dict(avg_ms=ms.value, bytes=by.value, gbs=by.value / (ms.value * 1e-3) / 1e9)
         if args.precond == "ilu0":
-            L.check(L.lib().kcfd_batch_profile(bobj.h, 3, 10, L.ctypes.byref(ms)))
-            L.check(L.lib().kcfd_batch_kernel_bytes(bobj.h, 3, L.ctypes.byref(by)))
+            # the apply to p: phase 3 (the two-colour members' colour-1 rows
+            # and the other members' L solve) plus, with two-colour members,
+            # phase 5 (their colour-0 rows)
+            tot_ms, tot_by = 0.0, 0.0
+            for phase in ((3, 5) if order == "color" else (3,)):
+                if phase == 5 and L.lib().kcfd_batch_kernel_bytes(bobj.h, 5, L.ctypes.byref(by)) == 0 \
+                        and by.value == 0.0:
+                    continue   # no two-colour member
+                L.check(L.lib().kcfd_batch_profile(bobj.h, phase, 10, L.ctypes.byref(ms)))
+                L.check(L.lib().kcfd_batch_kernel_bytes(bobj.h, phase, L.ctypes.byref(by)))
+                tot_ms += ms.value
+                tot_by += by.value
             summ["ilu_apply" if order == "color" else "ilu_lower_solve"] = dict(
-                avg_ms=ms.value, bytes=by.value, gbs=by.value / (ms.value * 1e-3) / 1e9)
+                avg_ms=tot_ms, bytes=tot_by, gbs=tot_by / (tot_ms * 1e-3) / 1e9)
         # the profile overwrote x: solve once more for the solutions
         sb.solve(batch=bobj, **opts)
         sols[prec] = sb.solutions()

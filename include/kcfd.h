@@ -266,11 +266,13 @@ int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out);
  * context stream, with HIP events in the dispatch packets of one phase's
  * kernels in each (hipExtLaunchKernelGGL: the kernels' own start and end),
  * and returns that phase's mean duration: phase 1 = v = A M p with the fused
- * rt.v reduction, 2 = fused x/r update, 3 = ILU(0) apply to p (for a
- * two-colour ordering both of its kernels, otherwise the sync-free lower
- * triangular solve), 4 = p update.  The solver state is restored afterwards; the vectors (x
- * included) are overwritten.  kcfd_batch_kernel_bytes gives that kernel's
- * algorithmic HBM bytes per launch (summed over the batch). */
+ * rt.v reduction, 2 = fused x/r update, 3 = ILU(0) apply to p (BiCGSTAB
+ * with a two-colour ordering: its colour-1 launch; CG: both launches;
+ * otherwise the sync-free lower triangular solve), 4 = p update, 5 = the
+ * colour-0 launch of the two-colour apply to p (BiCGSTAB).  The solver
+ * state is restored afterwards; the vectors (x included) are overwritten.
+ * kcfd_batch_kernel_bytes gives that phase's algorithmic HBM bytes per
+ * launch (summed over the batch). */
 int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms);
 int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes);
 

@@ -1733,19 +1733,21 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     launch_k(b, 4, true, true, k_p<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
              b->d_counters);
     if (PC == KCFD_PC_ILU0) {
-        // phase 3: the p apply (two-colour: both kernels; sync-free: the L solve)
-        if (b->run_two) {
-            launch_k(b, 3, true, false, k_ilu2<V, false, 0>, g, kBlock, s, b->d_desc, b->d_state,
+        // phase 3: the p apply's first launches (two-colour: the colour-1
+        // rows; sync-free: the L solve); phase 5: the two-colour colour-0
+        // rows, launched last (the two kinds of members are independent)
+        if (b->run_two)
+            launch_k(b, 3, true, !b->run_polled, k_ilu2<V, false, 0>, g, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_a, b->d_counts);
-            launch_k(b, 3, false, !b->run_polled, k_ilu2<V, true, 0>, g, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_b, b->d_counts);
-        }
         if (b->run_polled) {
             launch_k(b, 3, !b->run_two, true, k_trsv<V, false, 0>, tg, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
             launch_k(b, 0, false, false, k_trsv<V, true, 0>, tg, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
         }
+        if (b->run_two)
+            launch_k(b, 5, true, true, k_ilu2<V, true, 0>, g, kBlock, s, b->d_desc, b->d_state,
+                     b->d_chunks_b, b->d_counts);
     }
     launch_k(b, 1, true, true, k_spmv_p<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
              b->d_partials);
@@ -1754,18 +1756,18 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
     if (PC != KCFD_PC_ILU0)
         launch_k(b, 0, false, false, k_s<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts);
     if (PC == KCFD_PC_ILU0) {
-        if (b->run_two) {
+        if (b->run_two)
             launch_k(b, 0, false, false, k_ilu2<V, false, 1>, g, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_a, b->d_counts);
-            launch_k(b, 0, false, false, k_ilu2<V, true, 1>, g, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_b, b->d_counts);
-        }
         if (b->run_polled) {
             launch_k(b, 0, false, false, k_trsv<V, false, 1>, tg, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
             launch_k(b, 0, false, false, k_trsv<V, true, 1>, tg, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
         }
+        if (b->run_two)
+            launch_k(b, 0, false, false, k_ilu2<V, true, 1>, g, kBlock, s, b->d_desc, b->d_state,
+                     b->d_chunks_b, b->d_counts);
     }
     launch_k(b, 0, false, false, k_spmv_s<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles,
              b->d_counts, b->d_partials, b->d_params);
@@ -2817,8 +2819,10 @@ int kcfd_batch_bytes(kcfd_batch *b, long long *bytes) {
 // 4 = K1 (p update).
 int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     KCFD_ARG(b && avg_ms && reps > 0, "bad argument");
-    KCFD_ARG(phase >= 1 && phase <= 4, "unknown phase");
-    KCFD_ARG(phase != 3 || b->bp.precond == KCFD_PC_ILU0, "phase 3 needs ILU(0)");
+    KCFD_ARG(phase >= 1 && phase <= 5, "unknown phase");
+    KCFD_ARG((phase != 3 && phase != 5) || b->bp.precond == KCFD_PC_ILU0, "phases 3 and 5 need ILU(0)");
+    KCFD_ARG(phase != 5 || (b->run_two && b->bp.method == KCFD_METHOD_BICGSTAB),
+             "phase 5 needs a two-colour ILU(0) member and BiCGSTAB");
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
     KCFD_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
@@ -2905,11 +2909,20 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
         else if (cg && phase == 4) tot += (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
         else if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv + fold;
         else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
-        else if (phase == 3) {
-            if (m->two_colour)   // both kernels of the two-colour apply: L and U entries, z,
-                                 // w written, the other colour's z / w gathered, u_ii
-                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv + n0 * sv + x64 * 8 * n0;
-            else                 // the sync-free L solve (one triangle + its level list)
+        else if (phase == 3 || phase == 5) {
+            // two-colour apply, colour-1 launch (3): its L entries, own z
+            // read, w written, u_ii, the colour-0 operands gathered (once);
+            // colour-0 launch (5): its U entries, own z, w, u_ii, the colour-1
+            // results gathered, and folded: the product rows written and, in
+            // MIXED, the FP64 pivots read
+            const double n1 = n - m->ncolour0, nc0 = m->ncolour0;
+            if (m->two_colour && cg)   // CG times both launches of its apply as phase 3
+                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv;
+            else if (m->two_colour && phase == 3)
+                tot += (double)m->nent_l * (sv + 4) + (3 * n1 + nc0) * sv;
+            else if (m->two_colour)
+                tot += (double)m->nent_u * (sv + 4) + (3 * nc0 + n1) * sv + n0 * sv + x64 * 8 * n0;
+            else if (phase == 3)   // the sync-free L solve (one triangle + its level list)
                 tot += MFh + 2 * n * sv;
         }
         else if (phase == 4) tot += 4 * n * sv;
