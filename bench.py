@@ -759,7 +759,7 @@ def bench_serve(args):
 
     # warm-up: a few requests (graph capture, first launches)
     srv = S.Server(ctx, temps, precision="mixed", precond=args.precond, order=args.order,
-                   batched=True, check_every=args.check_every or 8, log=progress)
+                   batched=True, check_every=args.check_every or 8, log=progress, fold_colour0=args.fold)
     srv.run([(0.0, ti, sd) for _, ti, sd in tr[:8]])
     t = time.perf_counter()
     rep = srv.run(tr)

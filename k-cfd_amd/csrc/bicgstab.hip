@@ -145,14 +145,23 @@ constexpr int kTrsvFail = 6;
 constexpr int kDefaultPollLimit = 1 << 20;
 
 // ILU trisolve work chunk: up to 64 consecutive rows of one system's level
-// list.  local: the chunk's index in its matrix's layout; o0, wd: its first
-// entry and width there (kcfd_mat::chl_off / chu_off), cb: the base column
-// of a narrow-index two-colour chunk -- carried here so that a wave's
-// entry loads follow the chunk's own load, not another dependent load.
+// list, 16 bytes: the system (slot), row count and width packed in `meta`
+// (12, 7 and 13 bits), the first row, the first entry in the matrix's chunk
+// layout (kcfd_mat::chl_off / chu_off) and the base column of a
+// narrow-index two-colour chunk.  Carrying the layout position here lets a
+// wave's entry loads follow the chunk's own load, not another dependent
+// load; keeping the record small keeps the per-round table uploads of a
+// serving batch small.
+constexpr int kChunkMaxSys = 1 << 12, kChunkMaxWidth = (1 << 13) - 1;
 struct Chunk {
-    int sys, start, count, local;
-    int o0, wd, cb, pad;
+    int meta, start, o0, cb;
+    __host__ __device__ int sys() const { return meta & (kChunkMaxSys - 1); }
+    __host__ __device__ int count() const { return (meta >> 12) & 127; }
+    __host__ __device__ int wd() const { return (int)((unsigned)meta >> 19); }
 };
+__host__ __device__ inline Chunk make_chunk(int sys, int start, int count, int o0, int wd, int cb) {
+    return Chunk{(int)((unsigned)sys | ((unsigned)count << 12) | ((unsigned)wd << 19)), start, o0, cb};
+}
 
 // ----------------------------------------------------------- tile walk
 // Blocks with blockIdx % 8 == x (the XCD they are dispatched to) cover the
@@ -1383,25 +1392,24 @@ __global__ __launch_bounds__(kBlock) void k_trsv(const SysDesc *__restrict__ sys
     const int wave = (int)__builtin_amdgcn_readfirstlane(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6));
     for (int c = wave; c < nchunks; c += nwaves) {
         const Chunk ch = chunks[c];
-        const SysDesc &d = sys[ch.sys];
-        const int status = sts[ch.sys].status;
+        const SysDesc &d = sys[ch.sys()];
+        const int status = sts[ch.sys()].status;
         // the p solves (after K1) serve systems in iteration 0 or later; the s
         // solves (after K3) only systems still running -- a system that the
         // refinement just restarted (S_START set inside F2) must not run them,
         // or stale values would be left in sh
         const bool want = WHICH == 0 ? (status == S_RUN || status == S_START || status == S_RESUME)
                                      : (WHICH == 1 ? status == S_RUN : (status == S_RUN || status == S_START));
-        const bool live = want && lane < ch.count;
+        const bool live = want && lane < ch.count();
         if (live) {
-            const int i = gl(lvl_rows[ch.sys])[ch.start + lane];
+            const int i = gl(lvl_rows[ch.sys()])[ch.start + lane];
             const int *off = UPPER ? d.chu_off : d.chl_off;
-            const int o0 = uniform(gl(off)[ch.local]);
-            const int w = uniform((gl(off)[ch.local + 1] - o0) >> 6);
+            const int o0 = ch.o0, w = ch.wd();
             if (!UPPER) {
                 // the own operand: s is formed here (and stored), as in the
                 // two-colour apply
                 V zi;
-                const SysState &st = sts[ch.sys];
+                const SysState &st = sts[ch.sys()];
                 if constexpr (WHICH == 0) {
                     zi = vec<V>(d.p)[i];
                 } else if constexpr (WHICH == 1) {
@@ -1570,12 +1578,12 @@ template <typename V, bool SECOND, int WHICH>
 __device__ __forceinline__ void ilu2_chunk(const SysDesc *__restrict__ sys, const SysState *__restrict__ sts,
                                            const Chunk &ch, int lane) {
 #pragma clang fp contract(off)
-    const SysDesc &d = sys[ch.sys];
-    const SysState &st = sts[ch.sys];
-    if (!ilu2_wants<V, WHICH>(st.status) || lane >= ch.count) return;
+    const SysDesc &d = sys[ch.sys()];
+    const SysState &st = sts[ch.sys()];
+    if (!ilu2_wants<V, WHICH>(st.status) || lane >= ch.count()) return;
     const int i = ch.start + lane;       // row of P A P^T
     KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 1 ? d.sh : d.ph);
-    const int o0 = ch.o0, wd = ch.wd;
+    const int o0 = ch.o0, wd = ch.wd();
     if constexpr (SECOND && WHICH != 2) {
         if (d.nid > 0) {   // folded: the colour-0 rows of A M z as well
             if constexpr (WHICH == 0) {
@@ -1612,7 +1620,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     const int c0 = (int)(blockIdx.x % kXcds) * per, c1 = min(c0 + per, nchunks);
     const int wx = (int)(gridDim.x / kXcds) * kWaves;   // waves per XCD
     const int lw = uniform((int)(blockIdx.x / kXcds) * kWaves + (int)(threadIdx.x >> 6));
-    for (int c = c0 + lw; c < c1; c += wx) ilu2_chunk<V, SECOND, WHICH>(sys, sts, chunks[c], lane);
+    // the next chunk's record is loaded while the current one is worked on
+    Chunk next{};
+    if (c0 + lw < c1) next = chunks[c0 + lw];
+    for (int c = c0 + lw; c < c1; c += wx) {
+        const Chunk cur = next;
+        if (c + wx < c1) next = chunks[c + wx];
+        ilu2_chunk<V, SECOND, WHICH>(sys, sts, cur, lane);
+    }
 }
 
 template <typename V>
@@ -2040,6 +2055,8 @@ void init_batch_host(kcfd_batch *b, kcfd_ctx *ctx, int slots, const kcfd_solve_o
 
 int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **out) {
     KCFD_CHECK(check_opts(o));
+    KCFD_ARG(o->precond != KCFD_PC_ILU0 || slots <= kChunkMaxSys,
+             "an ILU(0) batch holds at most 4096 systems (slots)");
     kcfd_batch *b = new kcfd_batch();
     init_batch_host(b, ctx, slots, o);
     *out = b;
@@ -2160,8 +2177,8 @@ void build_sys_tables(kcfd_batch *b, int k) {
         auto chunk = [&](int start, int count, int local, bool upper) {
             const std::vector<int> &off = upper ? m->h_chu_off : m->h_chl_off;
             const std::vector<int> &base = upper ? m->h_chu_base : m->h_chl_base;
-            return Chunk{k, start, count, local, off[local], (off[local + 1] - off[local]) >> 6,
-                         m->ch16 ? base[local] : 0, 0};
+            return make_chunk(k, start, count, off[local], (off[local + 1] - off[local]) >> 6,
+                              m->ch16 ? base[local] : 0);
         };
         for (int s0 = 0; s0 < n1; s0 += 64) t.ca.push_back(chunk(n0 + s0, std::min(64, n1 - s0), s0 / 64, false));
         for (int s0 = 0; s0 < n0; s0 += 64) t.cb.push_back(chunk(s0, std::min(64, n0 - s0), s0 / 64, true));
@@ -2177,8 +2194,8 @@ void build_sys_tables(kcfd_batch *b, int k) {
         for (int l = 0; l < nl; ++l)
             for (int s0 = lp[l]; s0 < lp[l + 1]; s0 += 64) {
                 const int local = cp[l] + (s0 - lp[l]) / 64;
-                lv[l].push_back(Chunk{k, s0, std::min(64, lp[l + 1] - s0), local, off[local],
-                                      (off[local + 1] - off[local]) >> 6, 0, 0});
+                lv[l].push_back(make_chunk(k, s0, std::min(64, lp[l + 1] - s0), off[local],
+                                           (off[local + 1] - off[local]) >> 6, 0));
             }
     }
 }

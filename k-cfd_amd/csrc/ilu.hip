@@ -308,6 +308,16 @@ static bool chunk_offsets16(const std::vector<int> &off, const std::vector<int> 
     return true;
 }
 
+// The solvers' chunk records carry a chunk's width in 13 bits.
+static int check_chunk_widths(const std::vector<int> &off) {
+    for (size_t c = 0; c + 1 < off.size(); ++c)
+        if (((off[c + 1] - off[c]) >> 6) > 8191) {
+            set_error("ILU(0): a row has more than 8191 entries in its L or U part");
+            return KCFD_EARG;
+        }
+    return KCFD_OK;
+}
+
 // chunk-layout values: FP64, and (MIXED) the packed {column, FP32 value}
 // entries; with col null the FP32 copy goes to v32 (u_ii by row)
 __global__ void k_chunk_vals(size_t ne, const int *__restrict__ map, const int *__restrict__ col,
@@ -385,6 +395,7 @@ int ilu0_structure(kcfd_mat *m) {
             std::vector<int> off, col, map, lbase, ubase;
             std::vector<unsigned short> lo16, uo16;
             chunk_layout(m, rl, lvl_l, false, m->h_chl_ptr, off, col, map);
+            KCFD_CHECK(check_chunk_widths(off));
             m->nch_l = (int)off.size() - 1;
             m->nent_l = (int)col.size();
             m->h_chl_off = off;
@@ -394,6 +405,7 @@ int ilu0_structure(kcfd_mat *m) {
             KCFD_CHECK(upload_ints(map, &m->chl_map, s));
             KCFD_HIP(hipStreamSynchronize(s));
             chunk_layout(m, ru, lvl_u, true, m->h_chu_ptr, off, col, map);
+            KCFD_CHECK(check_chunk_widths(off));
             m->nch_u = (int)off.size() - 1;
             m->nent_u = (int)col.size();
             m->h_chu_off = off;

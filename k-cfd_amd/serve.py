@@ -284,7 +284,7 @@ class Server:
 
     def __init__(self, ctx, templates, precision="mixed", precond="ilu0", order="color",
                  batched=True, capacity=2048, hbm_budget=None, rtol=1e-8, check_every=8,
-                 grid_per_cu=0, trsv_per_cu=0, delta=0.1, log=None, pressure_method="cg"):
+                 grid_per_cu=0, trsv_per_cu=0, delta=0.1, log=None, pressure_method="cg", fold_colour0=0):
         self.ctx = ctx
         self.templates = templates
         self.precision, self.precond, self.order = precision, precond, order
@@ -294,7 +294,7 @@ class Server:
         self.hbm_budget = int(0.6 * info["free_bytes"]) if hbm_budget is None else int(hbm_budget)
         self.opts = device.default_opts(rtol=rtol, precond=precond, precision=precision, inner_rtol=delta,
                                         check_every=check_every, grid_per_cu=grid_per_cu,
-                                        trsv_per_cu=trsv_per_cu)
+                                        trsv_per_cu=trsv_per_cu, fold_colour0=fold_colour0)
         self.log = log or (lambda *a: None)
         self.batch = None
         if batched:
