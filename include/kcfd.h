@@ -269,7 +269,9 @@ int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out);
  * rt.v reduction, 2 = fused x/r update, 3 = ILU(0) apply to p (BiCGSTAB
  * with a two-colour ordering: its colour-1 launch; CG: both launches;
  * otherwise the sync-free lower triangular solve), 4 = p update, 5 = the
- * colour-0 launch of the two-colour apply to p (BiCGSTAB).  The solver
+ * colour-0 launch of the two-colour apply to p (BiCGSTAB), 6 = the whole
+ * apply to p, one event pair from its first launch to its last
+ * (BiCGSTAB).  The solver
  * state is restored afterwards; the vectors (x included) are overwritten.
  * kcfd_batch_kernel_bytes gives that phase's algorithmic HBM bytes per
  * launch (summed over the batch). */

@@ -1751,17 +1751,20 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
         // phase 3: the p apply's first launches (two-colour: the colour-1
         // rows; sync-free: the L solve); phase 5: the two-colour colour-0
         // rows, launched last (the two kinds of members are independent)
+        // (phase 6: the whole apply, one event pair from its first launch to
+        // its last)
+        const bool whole = b->timed_phase == 6;
         if (b->run_two)
-            launch_k(b, 3, true, !b->run_polled, k_ilu2<V, false, 0>, g, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_a, b->d_counts);
+            launch_k(b, whole ? 6 : 3, true, !whole && !b->run_polled, k_ilu2<V, false, 0>, g, kBlock, s,
+                     b->d_desc, b->d_state, b->d_chunks_a, b->d_counts);
         if (b->run_polled) {
-            launch_k(b, 3, !b->run_two, true, k_trsv<V, false, 0>, tg, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
-            launch_k(b, 0, false, false, k_trsv<V, true, 0>, tg, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
+            launch_k(b, whole ? 6 : 3, !b->run_two, !whole, k_trsv<V, false, 0>, tg, kBlock, s, b->d_desc,
+                     b->d_state, b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
+            launch_k(b, whole ? 6 : 0, false, whole && !b->run_two, k_trsv<V, true, 0>, tg, kBlock, s,
+                     b->d_desc, b->d_state, b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
         }
         if (b->run_two)
-            launch_k(b, 5, true, true, k_ilu2<V, true, 0>, g, kBlock, s, b->d_desc, b->d_state,
+            launch_k(b, whole ? 6 : 5, !whole, true, k_ilu2<V, true, 0>, g, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_b, b->d_counts);
     }
     launch_k(b, 1, true, true, k_spmv_p<V, PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
@@ -2836,8 +2839,9 @@ int kcfd_batch_bytes(kcfd_batch *b, long long *bytes) {
 // 4 = K1 (p update).
 int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     KCFD_ARG(b && avg_ms && reps > 0, "bad argument");
-    KCFD_ARG(phase >= 1 && phase <= 5, "unknown phase");
-    KCFD_ARG((phase != 3 && phase != 5) || b->bp.precond == KCFD_PC_ILU0, "phases 3 and 5 need ILU(0)");
+    KCFD_ARG(phase >= 1 && phase <= 6, "unknown phase");
+    KCFD_ARG(phase < 3 || phase == 4 || b->bp.precond == KCFD_PC_ILU0, "phases 3, 5 and 6 need ILU(0)");
+    KCFD_ARG(phase != 6 || b->bp.method == KCFD_METHOD_BICGSTAB, "phase 6 needs BiCGSTAB");
     KCFD_ARG(phase != 5 || (b->run_two && b->bp.method == KCFD_METHOD_BICGSTAB),
              "phase 5 needs a two-colour ILU(0) member and BiCGSTAB");
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
@@ -2926,6 +2930,10 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
         else if (cg && phase == 4) tot += (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
         else if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv + fold;
         else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
+        else if (phase == 6) {   // the whole apply: both halves, or both sync-free solves
+            tot += m->two_colour ? (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv + n0 * sv + x64 * 8 * n0
+                                 : 2 * (MFh + 2 * n * sv);
+        }
         else if (phase == 3 || phase == 5) {
             // two-colour apply, colour-1 launch (3): its L entries, own z
             // read, w written, u_ii, the colour-0 operands gathered (once);

@@ -20,8 +20,11 @@ one-sided-Dirichlet pressure systems, in FP32 and even in FP64, where CG
 converges.  The momentum solves run in mixed precision.  A mixed solve that
 stalls (FP32 operator error of about eps_32 * cond(A) per cycle) continues
 in FP64 from its best iterate, and from then on the same kind of system on
-the same mesh template goes to the FP64 batch directly instead of repeating
-the mixed attempt.  ``Template.calibrate`` makes that choice before serving:
+the same mesh template goes to FP64 directly instead of repeating the mixed
+attempt.  With ``pressure_method='bicgstab'`` the pressure corrections run
+in mixed BiCGSTAB too, and those that stall continue by FP64 CG (the
+reference's pressure solver); a stalled momentum solve continues by FP64
+BiCGSTAB.  ``Template.calibrate`` makes that choice before serving:
 one mixed solve of each prototype matrix (seeded right-hand side) marks the
 kinds that stall.  Every result is FP64-verified: an FP64 solve that
 meets scipy's stopping test (on the recursive residual) but not the
@@ -428,9 +431,14 @@ class Server:
                         owner64[("cg", slot.value)] = (req, k, None)
                     elif self.precision == "mixed" and ((k == 0) in req.template.fp64_kinds or
                                                       (id(req.template), k == 0) in self.force64):
-                        check(lib().kcfd_batch_attach(self._fp64_batch(), A.h, b.p, x.p,
-                                                      L.ctypes.byref(slot)))
-                        owner64[slot.value] = (req, k, None)
+                        if k == 0:   # a pressure correction that needs FP64: the CG batch
+                            check(lib().kcfd_batch_attach(self._cg_batch(), A.h, b.p, x.p,
+                                                          L.ctypes.byref(slot)))
+                            owner64[("cg", slot.value)] = (req, k, None)
+                        else:
+                            check(lib().kcfd_batch_attach(self._fp64_batch(), A.h, b.p, x.p,
+                                                          L.ctypes.byref(slot)))
+                            owner64[slot.value] = (req, k, None)
                         self.direct64 += 1
                     else:
                         check(lib().kcfd_batch_attach(self.batch, A.h, b.p, x.p, L.ctypes.byref(slot)))
@@ -460,11 +468,18 @@ class Server:
                     if res.info != 0 and self.precision == "mixed":
                         # FP32 recurrences stalled: continue in FP64 from the
                         # best iterate (the matrix also holds its FP64 set-up)
+                        # (a pressure correction continues by CG, the
+                        # reference's pressure solver)
                         A, b, x = req.systems()[k]
                         s64 = L.ctypes.c_int()
-                        check(lib().kcfd_batch_attach(self._fp64_batch(), A.h, b.p, x.p,
-                                                      L.ctypes.byref(s64)))
-                        owner64[s64.value] = (req, k, r)
+                        if k == 0:
+                            check(lib().kcfd_batch_attach(self._cg_batch(), A.h, b.p, x.p,
+                                                          L.ctypes.byref(s64)))
+                            owner64[("cg", s64.value)] = (req, k, r)
+                        else:
+                            check(lib().kcfd_batch_attach(self._fp64_batch(), A.h, b.p, x.p,
+                                                          L.ctypes.byref(s64)))
+                            owner64[s64.value] = (req, k, r)
                         self.fallbacks += 1
                         self.force64.add((id(req.template), k == 0))
                     else:

@@ -87,7 +87,7 @@ def test_profile_leaves_batch_usable(small_batch):
         r0 = sb.solve(precision="fp64", precond="ilu0", order="color", batch=b)
         x0 = sb.solutions()
         ms = L.ctypes.c_double()
-        for phase in (1, 2, 3, 4, 5):
+        for phase in (1, 2, 3, 4, 5, 6):
             L.check(L.lib().kcfd_batch_profile(b.h, phase, 3, L.ctypes.byref(ms)))
             assert ms.value > 0
         r1 = sb.solve(precision="fp64", precond="ilu0", order="color", batch=b)

@@ -90,10 +90,11 @@ def test_serve_one_at_a_time_fp64(ctx, templates):
 
 
 def test_serve_routes_stalled_kind_to_fp64(ctx, templates):
-    """With every system solved by BiCGSTAB (pressure_method='bicgstab'), a
-    (template, system kind) marked as stalling in mixed precision is solved
-    in the FP64 batch from the start; its results equal the FP64 solve of
-    the same system alone, and the other systems stay mixed."""
+    """With every system solved by mixed BiCGSTAB (pressure_method='bicgstab'),
+    a (template, system kind) marked as stalling in mixed precision is solved
+    in FP64 from the start -- a pressure correction by the FP64 CG batch --;
+    its results equal the FP64 CG solve of the same system alone, and the
+    other systems stay mixed."""
     from kcfd_amd import device, serve as S
     trace = [(0.0, 0, 7100 + i) for i in range(3)] + [(0.0, 1, 7200)]
     srv = S.Server(ctx, templates, precision="mixed", precond="ilu0", batched=True, check_every=4,
@@ -104,7 +105,7 @@ def test_serve_routes_stalled_kind_to_fp64(ctx, templates):
         rep = srv.run(trace)
         assert rep.requests == 4 and rep.not_converged == 0 and rep.max_rel_resid < 1e-8
         assert rep.fp64_direct == 3 and rep.fp64_fallbacks == 0
-        o64 = device.default_opts(rtol=1e-8, precond="ilu0", precision="fp64", check_every=4)
+        o64 = device.default_opts(rtol=1e-8, precond="ilu0", precision="fp64", method="cg", check_every=4)
         for rid, (t, ti, seed) in enumerate(trace):
             req = S.Request(rid, 0.0, templates[ti], seed)
             srv._admit(req, 0.0)
