@@ -35,7 +35,11 @@ Admission is by HBM footprint: a request is admitted only while the bytes
 of the admitted requests (matrices, preconditioners, vectors, right-hand
 sides, solutions) stay within ``hbm_budget`` and four slots are free; the
 rest wait in arrival order.  Matrices and Krylov state stay on the device
-from admission to completion.
+from admission to completion.  A finished request's matrices and vectors
+go to a per-template pool and the template's next request is assembled
+into them, so that the steady state allocates and frees no device memory
+(each device free synchronises the device); pooled arrays are freed,
+largest first, when a new allocation would not fit the budget beside them.
 
 The reference solves one system per call (core/linear_solver.py:14-62);
 ``Server(batched=False, precision="fp64")`` is that one-at-a-time mode on
@@ -259,6 +263,7 @@ class ServeReport:
     fp64_fallbacks: int = 0
     fp64_direct: int = 0
     fp64_continued: int = 0
+    arrays_reused: int = 0
 
     @property
     def solves_per_s(self):
@@ -279,7 +284,7 @@ class ServeReport:
                     peak_hbm_gb=round(self.peak_hbm_bytes / 1e9, 2), peak_requests=self.peak_requests,
                     not_converged=self.not_converged, max_rel_resid=self.max_rel_resid,
                     fp64_fallbacks=self.fp64_fallbacks, fp64_direct=self.fp64_direct,
-                    fp64_continued=self.fp64_continued)
+                    fp64_continued=self.fp64_continued, arrays_reused=self.arrays_reused)
 
 
 class Server:
@@ -320,13 +325,38 @@ class Server:
         self.force64 = set()
         self.keep_solutions = False
         self.solutions = {}
+        # finished requests' device arrays by template, reused by the next
+        # request of that template (pool_arrays=False: freed at once)
+        self.pool_arrays = True
+        self._pool = {}
+        self._pool_bytes = {id(T): T.bytes for T in templates}
+        self.pooled_bytes = 0
+        self.reused = 0
 
     # -- request lifecycle ---------------------------------------------------
-    def _admit(self, req: Request, now):
+    def _admit(self, req: Request, now, in_use=0):
+        """Assemble the request's matrices, set up their preconditioners and
+        upload its right-hand sides.  A finished request's device arrays of
+        the same template are reused when pooled (re-assembled in place:
+        no device allocation); otherwise new ones are allocated, after
+        freeing pooled arrays until the admitted requests (``in_use``
+        bytes), the pool and this request fit the HBM budget."""
         T = req.template
-        req.mats = T.new_matrices(self.precond, self.precision)
-        req.bufs = [DevBuf.from_array(self.ctx, b) for b in req.rhs()]
-        req.xs = [DevBuf(self.ctx, T.n) for _ in range(4)]
+        pool = self._pool.get(id(T))
+        if pool:
+            req.mats, req.bufs, req.xs = pool.pop()
+            self.pooled_bytes -= T.bytes
+            for r, A in zip(T.rec, req.mats):
+                r.assemble(A)
+                A.setup_precond(self.precond, self.precision)
+            for buf, b in zip(req.bufs, req.rhs()):
+                buf.upload(b)
+            self.reused += 1
+        else:
+            self._evict(in_use + T.bytes)
+            req.mats = T.new_matrices(self.precond, self.precision)
+            req.bufs = [DevBuf.from_array(self.ctx, b) for b in req.rhs()]
+            req.xs = [DevBuf(self.ctx, T.n) for _ in range(4)]
         for x in req.xs:
             x.fill(0.0)
         req.t_admit = now
@@ -335,7 +365,30 @@ class Server:
         req.t_done = now
         if self.keep_solutions:
             self.solutions[req.rid] = ([x.download() for x in req.xs], list(req.results))
-        req.release()
+        if self.pool_arrays and req.mats:
+            # keep the arrays for the template's next request (a device
+            # free synchronises the device; an allocation costs as much)
+            self._pool.setdefault(id(req.template), []).append((req.mats, req.bufs, req.xs))
+            self.pooled_bytes += req.template.bytes
+            req.mats, req.bufs, req.xs = [], [], []
+        else:
+            req.release()
+
+    def _evict(self, need):
+        """Free pooled arrays, largest template first, until ``need`` bytes
+        plus the pool fit the HBM budget."""
+        while self.pooled_bytes and need + self.pooled_bytes > self.hbm_budget:
+            tid = max((k for k, v in self._pool.items() if v), key=lambda k: self._pool_bytes[k])
+            mats, bufs, xs = self._pool[tid].pop()
+            self.pooled_bytes -= self._pool_bytes[tid]
+            for A in mats:
+                A.close()
+            for b in bufs + xs:
+                b.free()
+
+    def free_pool(self):
+        """Free every pooled array."""
+        self._evict(self.hbm_budget + 1)
 
     def _fp64_batch(self):
         """The FP64 batch that takes over mixed solves that stalled."""
@@ -359,6 +412,7 @@ class Server:
         return self.batch_cg
 
     def close(self):
+        self.free_pool()
         for name in ("batch", "batch64", "batch_cg"):
             h = getattr(self, name, None)
             if h is not None:
@@ -375,6 +429,7 @@ class Server:
         owner, owner64 = {}, {}
         self.fallbacks = 0
         self.continued = 0
+        reused0 = self.reused
         cont = {}
         in_use = 0
         peak_bytes = peak_reqs = 0
@@ -401,7 +456,7 @@ class Server:
                     time.sleep(max(0.0, min(pending[0].t_arrival - now, 0.05)) if pending else 0.0)
                     continue
                 req = queue.popleft()
-                self._admit(req, time.perf_counter() - t0)
+                self._admit(req, time.perf_counter() - t0, in_use)
                 peak_reqs = max(peak_reqs, 1)
                 peak_bytes = max(peak_bytes, req.template.bytes)
                 for k, (A, b, x) in enumerate(req.systems()):
@@ -423,7 +478,7 @@ class Server:
             while queue and in_use + queue[0].template.bytes <= self.hbm_budget and \
                     4 * (active + 1) <= self.capacity:
                 req = queue.popleft()
-                self._admit(req, time.perf_counter() - t0)
+                self._admit(req, time.perf_counter() - t0, in_use)
                 for k, (A, b, x) in enumerate(req.systems()):
                     slot = L.ctypes.c_int()
                     if k == 0 and self.pressure_cg:
@@ -528,4 +583,4 @@ class Server:
                            not_converged=sum(1 for x in allres if x[0] != 0),
                            max_rel_resid=float(max((x[3] for x in allres), default=0.0)),
                            fp64_fallbacks=self.fallbacks, fp64_direct=self.direct64,
-                           fp64_continued=self.continued)
+                           fp64_continued=self.continued, arrays_reused=self.reused - reused0)

@@ -60,6 +60,34 @@ def test_serve_mix_equals_alone(ctx, templates):
         srv.close()
 
 
+def test_serve_pooled_arrays_equal_fresh(ctx, templates):
+    """Requests assembled into a finished request's pooled matrices and
+    vectors give exactly the solutions and iteration counts of requests
+    given freshly allocated ones; the pool is reused, and freed on close."""
+    from kcfd_amd import serve as S
+    trace = _trace(16, 150.0, 2, seed=4)       # two templates: later requests reuse arrays
+    sols = {}
+    for pool in (True, False):
+        srv = S.Server(ctx, templates, precision="mixed", precond="ilu0", batched=True, check_every=4)
+        srv.pool_arrays = pool
+        srv.keep_solutions = True
+        try:
+            rep = srv.run(trace)
+            print(pool, rep.summary(), "reused", srv.reused)
+            assert rep.requests == 16 and rep.not_converged == 0
+            assert (srv.reused > 0) == pool and (srv.pooled_bytes > 0) == pool
+            sols[pool] = srv.solutions
+        finally:
+            srv.close()
+        assert srv.pooled_bytes == 0 and not any(srv._pool.values())
+    for rid in range(16):
+        xa, ra = sols[True][rid]
+        xb, rb = sols[False][rid]
+        assert ra == rb, rid
+        for k in range(4):
+            assert np.array_equal(xa[k], xb[k]), (rid, k)
+
+
 def test_serve_admission_by_hbm(ctx, templates):
     """A burst admitted under an HBM budget: the footprint in use never
     exceeds it, fewer requests overlap than without it, all complete."""
