@@ -251,12 +251,17 @@ int kcfd_batch_bytes(kcfd_batch *b, long long *bytes);
  * an attached system (set up for opts' preconditioner and precision) and its
  * b / x buffers must stay alive until it is detached; one matrix may back
  * several members (one per right-hand side).  kcfd_batch_attach returns
- * KCFD_EBUSY when every slot is taken. */
+ * KCFD_EBUSY when every slot is taken.  A detached system's vector block
+ * stays with the batch and is reused by the next attach that needs a block
+ * of the same size (no device allocation or free, each of which would
+ * synchronise the device); kcfd_batch_trim frees the blocks kept this way
+ * (kcfd_batch_bytes counts them). */
 int kcfd_batch_create_dynamic(kcfd_ctx *ctx, int capacity, const kcfd_solve_opts *opts,
                               kcfd_batch **out);
 int kcfd_batch_attach(kcfd_batch *b, kcfd_mat *m, const double *bvec, double *x, int *slot);
 int kcfd_batch_step(kcfd_batch *b, int *finished, int *nfinished);
 int kcfd_batch_detach(kcfd_batch *b, int slot, kcfd_solve_result *res);
+int kcfd_batch_trim(kcfd_batch *b);
 int kcfd_batch_members(kcfd_batch *b, int *members);
 /* the single-system workspace kcfd_solve cached on a matrix */
 int kcfd_mat_solver(kcfd_mat *m, kcfd_batch **out);

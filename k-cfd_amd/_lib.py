@@ -105,6 +105,7 @@ SIGNATURES = {
     "kcfd_batch_step": ([_P, _P, ctypes.POINTER(_I)], _I),
     "kcfd_batch_detach": ([_P, _I, ctypes.POINTER(SolveResult)], _I),
     "kcfd_batch_members": ([_P, ctypes.POINTER(_I)], _I),
+    "kcfd_batch_trim": ([_P], _I),
     "kcfd_ilu0_order": ([_P, _P, ctypes.POINTER(_I)], _I),
     "kcfd_precond_apply": ([_P, _P, _P], _I),
     "kcfd_solve": ([_P, _P, _P, ctypes.POINTER(SolveOpts), ctypes.POINTER(SolveResult)], _I),

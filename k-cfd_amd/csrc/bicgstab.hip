@@ -1683,6 +1683,10 @@ struct kcfd_batch {
     std::vector<SysDesc> h_desc;
     std::vector<int> sys_n;
     std::vector<void *> slot_mem;   // dynamic: vectors of each slot
+    // dynamic: vector blocks of detached systems, {bytes, pointer}, reused by
+    // attaches of the same size (a device free synchronises the device);
+    // kcfd_batch_trim frees them
+    std::vector<std::pair<size_t, void *>> spare;
     std::vector<char> fresh, reported;
     SysDesc *d_desc = nullptr;
     SysState *d_state = nullptr;
@@ -2644,6 +2648,7 @@ int kcfd_batch_destroy(kcfd_batch *b) {
     hipFree(b->d_params);
     hipFree(b->slab);
     for (void *p : b->slot_mem) hipFree(p);
+    for (auto &sp : b->spare) hipFree(sp.second);
     hipFree(b->d_chunks_l);
     hipFree(b->d_chunks_u);
     hipFree(b->d_chunks_a);
@@ -2697,7 +2702,14 @@ int kcfd_batch_attach(kcfd_batch *b, kcfd_mat *m, const double *bvec, double *x,
     describe_slot(b, k, m, bvec, x);
     void *mem = nullptr;
     const size_t bytes = slot_bytes(b, b->h_desc[k]);
-    if (hipMalloc(&mem, bytes) != hipSuccess) {
+    for (size_t q = 0; q < b->spare.size(); ++q)
+        if (b->spare[q].first == bytes) {   // a detached system's block of this size
+            mem = b->spare[q].second;
+            b->spare[q] = b->spare.back();
+            b->spare.pop_back();
+            break;
+        }
+    if (!mem && hipMalloc(&mem, bytes) != hipSuccess) {
         (void)hipGetLastError();
         std::memset(&b->h_desc[k], 0, sizeof(SysDesc));
         b->mats[k] = nullptr;
@@ -2783,8 +2795,9 @@ int kcfd_batch_detach(kcfd_batch *b, int slot, kcfd_solve_result *res) {
                                  b->h_desc[slot].nid);
         res->ms = 0.0;
     }
-    KCFD_HIP(hipStreamSynchronize(b->ctx->stream));
-    hipFree(b->slot_mem[slot]);
+    // the block is kept for a later attach: its kernels follow this batch's
+    // on the same stream
+    b->spare.emplace_back(slot_bytes(b, b->h_desc[slot]), b->slot_mem[slot]);
     b->slot_mem[slot] = nullptr;
     b->mats[slot] = nullptr;
     b->sys_n[slot] = 0;
@@ -2800,6 +2813,17 @@ int kcfd_batch_detach(kcfd_batch *b, int slot, kcfd_solve_result *res) {
     b->dirty = true;
     if ((int)b->sys_tables.size() == b->nsys) build_sys_tables(b, slot);
     else b->tables_dirty = true;
+    return KCFD_OK;
+}
+
+int kcfd_batch_trim(kcfd_batch *b) {
+    KCFD_ARG(b != nullptr, "null batch");
+    std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
+    if (b->spare.empty()) return KCFD_OK;
+    KCFD_HIP(hipSetDevice(b->ctx->device));
+    KCFD_HIP(hipStreamSynchronize(b->ctx->stream));
+    for (auto &sp : b->spare) hipFree(sp.second);
+    b->spare.clear();
     return KCFD_OK;
 }
 
@@ -2826,6 +2850,7 @@ int kcfd_batch_bytes(kcfd_batch *b, long long *bytes) {
     long long v = (long long)b->slab_bytes;
     for (int k = 0; k < b->nsys; ++k)
         if (b->slot_mem[k]) v += (long long)slot_bytes(b, b->h_desc[k]);
+    for (auto &sp : b->spare) v += (long long)sp.first;
     *bytes = v + (long long)b->cap_tiles * (8 + 8 * kMaxPartials) +
              (long long)b->nsys * (long long)(sizeof(SysDesc) + sizeof(SysState) + 2 * sizeof(int *)) +
              (long long)(b->cap_chunks_l + b->cap_chunks_u) * (long long)sizeof(Chunk);

@@ -39,7 +39,10 @@ from admission to completion.  A finished request's matrices and vectors
 go to a per-template pool and the template's next request is assembled
 into them, so that the steady state allocates and frees no device memory
 (each device free synchronises the device); pooled arrays are freed,
-largest first, when a new allocation would not fit the budget beside them.
+largest first, when a new allocation would not fit beside them within
+``pool_budget`` (by default 80% of the free HBM at start, against 60% for
+the admitted requests) or the device's free memory would fall below a 5%
+reserve.
 
 The reference solves one system per call (core/linear_solver.py:14-62);
 ``Server(batched=False, precision="fp64")`` is that one-at-a-time mode on
@@ -291,7 +294,7 @@ class Server:
     """Continuous-batching solver service on one GPU (see the module doc)."""
 
     def __init__(self, ctx, templates, precision="mixed", precond="ilu0", order="color",
-                 batched=True, capacity=2048, hbm_budget=None, rtol=1e-8, check_every=8,
+                 batched=True, capacity=2048, hbm_budget=None, pool_budget=None, rtol=1e-8, check_every=8,
                  grid_per_cu=0, trsv_per_cu=0, delta=0.1, log=None, pressure_method="cg", fold_colour0=0):
         self.ctx = ctx
         self.templates = templates
@@ -300,6 +303,10 @@ class Server:
         self.capacity = capacity
         info = ctx.info()
         self.hbm_budget = int(0.6 * info["free_bytes"]) if hbm_budget is None else int(hbm_budget)
+        # the admitted requests plus the pooled arrays of finished ones
+        self.pool_budget = (max(self.hbm_budget, int(0.8 * info["free_bytes"])) if hbm_budget is None
+                            else self.hbm_budget) if pool_budget is None else int(pool_budget)
+        self.pool_reserve = int(0.05 * info["total_bytes"])
         self.opts = device.default_opts(rtol=rtol, precond=precond, precision=precision, inner_rtol=delta,
                                         check_every=check_every, grid_per_cu=grid_per_cu,
                                         trsv_per_cu=trsv_per_cu, fold_colour0=fold_colour0)
@@ -353,7 +360,7 @@ class Server:
                 buf.upload(b)
             self.reused += 1
         else:
-            self._evict(in_use + T.bytes)
+            self._evict(in_use + T.bytes, T.bytes)
             req.mats = T.new_matrices(self.precond, self.precision)
             req.bufs = [DevBuf.from_array(self.ctx, b) for b in req.rhs()]
             req.xs = [DevBuf(self.ctx, T.n) for _ in range(4)]
@@ -374,10 +381,24 @@ class Server:
         else:
             req.release()
 
-    def _evict(self, need):
+    def _evict(self, need, alloc=0):
         """Free pooled arrays, largest template first, until ``need`` bytes
-        plus the pool fit the HBM budget."""
-        while self.pooled_bytes and need + self.pooled_bytes > self.hbm_budget:
+        plus the pool fit ``pool_budget`` and the device has ``alloc``
+        bytes free beside a reserve (the byte estimates do not see the
+        allocator's rounding).  The batches' kept vector blocks of detached
+        systems (counted in the pooled requests' bytes) are freed first."""
+        def over():
+            if need + self.pooled_bytes > self.pool_budget:
+                return True
+            return alloc > 0 and self.ctx.info()["free_bytes"] < alloc + self.pool_reserve
+        trimmed = False
+        while self.pooled_bytes and over():
+            if not trimmed:
+                trimmed = True
+                for h in (self.batch, self.batch64, self.batch_cg):
+                    if h is not None:
+                        check(lib().kcfd_batch_trim(h))
+                continue
             tid = max((k for k, v in self._pool.items() if v), key=lambda k: self._pool_bytes[k])
             mats, bufs, xs = self._pool[tid].pop()
             self.pooled_bytes -= self._pool_bytes[tid]
@@ -388,7 +409,7 @@ class Server:
 
     def free_pool(self):
         """Free every pooled array."""
-        self._evict(self.hbm_budget + 1)
+        self._evict(self.pool_budget + 1)
 
     def _fp64_batch(self):
         """The FP64 batch that takes over mixed solves that stalled."""

@@ -88,6 +88,55 @@ def test_serve_pooled_arrays_equal_fresh(ctx, templates):
             assert np.array_equal(xa[k], xb[k]), (rid, k)
 
 
+def test_dynamic_batch_reuses_detached_blocks(ctx, templates):
+    """A detached system's vector block is reused by the next attach of the
+    same size (the batch's device bytes stay the same), the solve from it
+    equals the first one bit for bit, and kcfd_batch_trim frees the kept
+    blocks."""
+    from kcfd_amd import _lib as L, serve as S
+    lib, check = L.lib(), L.check
+    srv = S.Server(ctx, templates, precision="mixed", precond="ilu0", batched=True, check_every=4)
+    req = S.Request(0, 0.0, templates[1], 7300)
+    srv._admit(req, 0.0)
+    A, b, x = req.systems()[1]                    # a momentum component, mixed BiCGSTAB
+    fin = (L.ctypes.c_int * srv.capacity)()
+    nfin = L.ctypes.c_int()
+    res = L.SolveResult()
+    by = L.ctypes.c_longlong()
+
+    def solve_once():
+        x.fill(0.0)
+        slot = L.ctypes.c_int()
+        check(lib.kcfd_batch_attach(srv.batch, A.h, b.p, x.p, L.ctypes.byref(slot)))
+        check(lib.kcfd_batch_bytes(srv.batch, L.ctypes.byref(by)))
+        attached = by.value
+        for _ in range(10000):
+            check(lib.kcfd_batch_step(srv.batch, fin, L.ctypes.byref(nfin)))
+            if nfin.value:
+                break
+        assert nfin.value == 1 and fin[0] == slot.value
+        check(lib.kcfd_batch_detach(srv.batch, slot.value, L.ctypes.byref(res)))
+        return attached, (res.info, res.iters, res.refinements, res.rel_resid), x.download()
+
+    try:
+        check(lib.kcfd_batch_bytes(srv.batch, L.ctypes.byref(by)))
+        empty = by.value
+        b1, r1, x1 = solve_once()
+        check(lib.kcfd_batch_bytes(srv.batch, L.ctypes.byref(by)))
+        assert by.value == b1 > empty                 # detached: the block is kept
+        b2, r2, x2 = solve_once()
+        assert b2 == b1                               # reused, not allocated
+        assert r1 == r2 and r1[0] == 0 and np.array_equal(x1, x2)
+        check(lib.kcfd_batch_trim(srv.batch))
+        check(lib.kcfd_batch_bytes(srv.batch, L.ctypes.byref(by)))
+        assert by.value == empty
+        b3, r3, x3 = solve_once()                     # a fresh block after the trim
+        assert b3 == b1 and r3 == r1 and np.array_equal(x3, x1)
+    finally:
+        req.release()
+        srv.close()
+
+
 def test_serve_admission_by_hbm(ctx, templates):
     """A burst admitted under an HBM budget: the footprint in use never
     exceeds it, fewer requests overlap than without it, all complete."""
