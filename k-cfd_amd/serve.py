@@ -267,6 +267,7 @@ class ServeReport:
     fp64_direct: int = 0
     fp64_continued: int = 0
     arrays_reused: int = 0
+    time_s: dict = field(default_factory=dict)
 
     @property
     def solves_per_s(self):
@@ -287,7 +288,8 @@ class ServeReport:
                     peak_hbm_gb=round(self.peak_hbm_bytes / 1e9, 2), peak_requests=self.peak_requests,
                     not_converged=self.not_converged, max_rel_resid=self.max_rel_resid,
                     fp64_fallbacks=self.fp64_fallbacks, fp64_direct=self.fp64_direct,
-                    fp64_continued=self.fp64_continued, arrays_reused=self.arrays_reused)
+                    fp64_continued=self.fp64_continued, arrays_reused=self.arrays_reused,
+                    time_s={k: round(v, 2) for k, v in sorted(self.time_s.items())})
 
 
 class Server:
@@ -339,6 +341,7 @@ class Server:
         self._pool_bytes = {id(T): T.bytes for T in templates}
         self.pooled_bytes = 0
         self.reused = 0
+        self._t = collections.defaultdict(float)   # host seconds by part of the loop
 
     # -- request lifecycle ---------------------------------------------------
     def _admit(self, req: Request, now, in_use=0):
@@ -348,8 +351,10 @@ class Server:
         no device allocation); otherwise new ones are allocated, after
         freeing pooled arrays until the admitted requests (``in_use``
         bytes), the pool and this request fit the HBM budget."""
+        t = time.perf_counter()
         T = req.template
         pool = self._pool.get(id(T))
+        reuse = bool(pool)
         if pool:
             req.mats, req.bufs, req.xs = pool.pop()
             self.pooled_bytes -= T.bytes
@@ -360,13 +365,16 @@ class Server:
                 buf.upload(b)
             self.reused += 1
         else:
+            te = time.perf_counter()
             self._evict(in_use + T.bytes, T.bytes)
+            self._t["evict"] += time.perf_counter() - te
             req.mats = T.new_matrices(self.precond, self.precision)
             req.bufs = [DevBuf.from_array(self.ctx, b) for b in req.rhs()]
             req.xs = [DevBuf(self.ctx, T.n) for _ in range(4)]
         for x in req.xs:
             x.fill(0.0)
         req.t_admit = now
+        self._t["admit_reused" if reuse else "admit_new"] += time.perf_counter() - t
 
     def _finish(self, req: Request, now):
         req.t_done = now
@@ -451,6 +459,7 @@ class Server:
         self.fallbacks = 0
         self.continued = 0
         reused0 = self.reused
+        self._t = collections.defaultdict(float)
         cont = {}
         in_use = 0
         peak_bytes = peak_reqs = 0
@@ -480,6 +489,7 @@ class Server:
                 self._admit(req, time.perf_counter() - t0, in_use)
                 peak_reqs = max(peak_reqs, 1)
                 peak_bytes = max(peak_bytes, req.template.bytes)
+                ts = time.perf_counter()
                 for k, (A, b, x) in enumerate(req.systems()):
                     o = self.opts_cg if (k == 0 and self.pressure_cg) else self.opts
                     r = A.solve_dev(b.p, x.p, o, setup=False)
@@ -490,6 +500,7 @@ class Server:
                         its += r.iters
                         self.continued += 1
                     req.results[k] = (r.info, its, r.refinements, r.rel_resid)
+                self._t["solve_one_at_a_time"] += time.perf_counter() - ts
                 self._finish(req, time.perf_counter() - t0)
                 done.append(req)
                 rounds += 1
@@ -533,7 +544,9 @@ class Server:
                 continue
             finished = []
             if owner:
+                ts = time.perf_counter()
                 check(lib().kcfd_batch_step(self.batch, fin, L.ctypes.byref(nfin)))
+                self._t[f"step_{self.precision}_bicgstab"] += time.perf_counter() - ts
                 rounds += 1
                 members_sum += len(owner)
                 for j in range(nfin.value):
@@ -565,7 +578,9 @@ class Server:
             for tag, bh in (("cg", self.batch_cg), (None, self.batch64)):
                 if bh is None or not any(isinstance(key, tuple) == (tag == "cg") for key in owner64):
                     continue
+                ts = time.perf_counter()
                 check(lib().kcfd_batch_step(bh, fin, L.ctypes.byref(nfin)))
+                self._t["step_fp64_cg" if tag == "cg" else "step_fp64_bicgstab"] += time.perf_counter() - ts
                 for j in range(nfin.value):
                     key = (tag, fin[j]) if tag else fin[j]
                     req, k, r1 = owner64.pop(key)
@@ -604,4 +619,5 @@ class Server:
                            not_converged=sum(1 for x in allres if x[0] != 0),
                            max_rel_resid=float(max((x[3] for x in allres), default=0.0)),
                            fp64_fallbacks=self.fallbacks, fp64_direct=self.direct64,
-                           fp64_continued=self.continued, arrays_reused=self.reused - reused0)
+                           fp64_continued=self.continued, arrays_reused=self.reused - reused0,
+                           time_s=dict(self._t))
