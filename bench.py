@@ -175,7 +175,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="batched: library default launch shape")
     ap.add_argument("--delta", type=float, default=0.1,
                     help="mixed: reliable-update trigger (FP32 residual reduction)")
-    ap.add_argument("--order", default="color", choices=["natural", "color"],
+    ap.add_argument("--order", default="color", choices=["natural", "color", "segment"],
                     help="ILU(0) row ordering (colour: first-fit colouring, 2 levels per solve)")
     ap.add_argument("--grid-per-cu", type=int, default=0, help="launch shape (0: library default)")
     ap.add_argument("--trsv-per-cu", type=int, default=0, help="launch shape (0: library default)")

@@ -138,10 +138,16 @@ int kcfd_ilu0_get(kcfd_mat *m, double *lu, int *levels_l, int *levels_u);
  * 52).  KCFD_ORDER_NATURAL factors A itself.  KCFD_ORDER_COLOR factors
  * P A P^T, P sorting rows by a first-fit colouring of the graph of A + A^T
  * (rows of one colour are never coupled), so each triangular solve has as
- * many dependency levels as colours (2 on the 5- and 7-point grids).  Takes
- * effect at the next kcfd_precond_setup. */
+ * many dependency levels as colours (2 on the 5- and 7-point grids).
+ * KCFD_ORDER_SEGMENT colours segments instead of rows: runs of up to 4
+ * consecutive rows each coupled to the one before (pieces of grid lines on a
+ * structured mesh in its natural numbering), kept contiguous; the factors
+ * keep the couplings inside a segment, which costs the solves up to 4 levels
+ * per colour and saves Krylov iterations on grids.  Takes effect at the next
+ * kcfd_precond_setup. */
 #define KCFD_ORDER_NATURAL 0
 #define KCFD_ORDER_COLOR 1
+#define KCFD_ORDER_SEGMENT 2
 int kcfd_precond_set_order(kcfd_mat *m, int order);
 /* The ordering of the current ILU(0) factors: perm (host n, may be null)
  * receives the row of A at each position of P A P^T, ncolors the number of

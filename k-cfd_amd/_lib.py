@@ -25,8 +25,8 @@ METHODS = {"bicgstab": METHOD_BICGSTAB, "cg": METHOD_CG, "gmres": METHOD_GMRES}
 BC_ZERO_GRADIENT, BC_DIRICHLET, BC_NEUMANN, BC_NONE = 0, 1, 2, 3
 LIMITERS = {"minmod": 0, "superbee": 1, "van_albada": 2, "van_leer": 3}
 PRECONDS = {"none": PC_NONE, "jacobi": PC_JACOBI, "ilu0": PC_ILU0}
-ORDER_NATURAL, ORDER_COLOR = 0, 1
-ORDERS = {"natural": ORDER_NATURAL, "color": ORDER_COLOR}
+ORDER_NATURAL, ORDER_COLOR, ORDER_SEGMENT = 0, 1, 2
+ORDERS = {"natural": ORDER_NATURAL, "color": ORDER_COLOR, "segment": ORDER_SEGMENT}
 
 _ERRNAMES = {KCFD_EARG: "bad argument", KCFD_EHIP: "HIP error", KCFD_EOOM: "out of device memory",
              KCFD_ESTATE: "bad state", KCFD_EPIVOT: "zero pivot", KCFD_EBUSY: "admission refused"}

@@ -23,6 +23,11 @@ constexpr int kFinBlock = 1024;      // threads of a finalize workgroup (one per
 
 // Number of XCDs on MI355X; workgroups are dispatched round-robin over them.
 constexpr int kXcds = 8;
+// rows per segment of the KCFD_ORDER_SEGMENT ILU(0) ordering (at most)
+#ifndef KCFD_SEG_LEN
+#define KCFD_SEG_LEN 4
+#endif
+constexpr int kSegLen = KCFD_SEG_LEN;
 
 // narrow-index SELL copy: the offset that marks column = row, and the widest
 // span of off-diagonal columns one 64-row slice may have

@@ -15,6 +15,16 @@
 // ascending order; within a colour rows keep their order).  No two rows of a
 // colour are coupled, so every row of colour c depends only on colours < c
 // (L) or > c (U): the solves have as many levels as colours.
+//
+// With KCFD_ORDER_SEGMENT the colouring is of segments instead of rows: runs
+// of up to kSegLen consecutive rows of A, each coupled to the one before it
+// (on a structured grid in its natural numbering, pieces of a grid line).
+// Segments are coloured first-fit on the graph of A + A^T contracted to
+// segments, and rows sorted by colour keep their order, so a segment stays
+// contiguous.  The coupling inside a segment is kept by the factorisation,
+// which costs the solves up to kSegLen levels per colour and buys fewer
+// Krylov iterations than the row colouring on grids (the strong couplings
+// along a line survive).
 #include "common.hpp"
 
 #include <algorithm>
@@ -138,6 +148,55 @@ static int color_rows(const std::vector<int> &ip, const std::vector<int> &ix, st
     return ncol;
 }
 
+// First-fit colouring of segments (see the ordering note above): col[i] =
+// colour of row i's segment; returns the number of colours.
+static int color_segments(const std::vector<int> &ip, const std::vector<int> &ix, std::vector<int> &col) {
+    const int n = (int)ip.size() - 1;
+    auto has = [&](int i, int j) {   // a_ij in the (sorted) pattern
+        return std::binary_search(ix.begin() + ip[i], ix.begin() + ip[i + 1], j);
+    };
+    std::vector<int> seg(n), start;
+    for (int i = 0, len = 0; i < n; ++i) {
+        if (i == 0 || len == kSegLen || !(has(i, i - 1) || has(i - 1, i))) {
+            start.push_back(i);
+            len = 0;
+        }
+        seg[i] = (int)start.size() - 1;
+        ++len;
+    }
+    const int ns = (int)start.size();
+    start.push_back(n);
+    // transpose pattern, for the A^T couplings
+    std::vector<int> tp(n + 1, 0), tx(ix.size());
+    for (int c : ix) tp[c + 1]++;
+    for (int i = 0; i < n; ++i) tp[i + 1] += tp[i];
+    {
+        std::vector<int> f(tp.begin(), tp.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int q = ip[i]; q < ip[i + 1]; ++q) tx[f[ix[q]]++] = i;
+    }
+    std::vector<int> scol(ns, -1), mark;
+    int ncol = 0;
+    for (int g = 0; g < ns; ++g) {
+        auto take = [&](int j) {
+            const int h = seg[j];
+            if (h != g && scol[h] >= 0) mark[scol[h]] = g;
+        };
+        for (int i = start[g]; i < start[g + 1]; ++i) {
+            for (int q = ip[i]; q < ip[i + 1]; ++q) take(ix[q]);
+            for (int q = tp[i]; q < tp[i + 1]; ++q) take(tx[q]);
+        }
+        int c = 0;
+        while (c < (int)mark.size() && mark[c] == g) ++c;
+        if (c == (int)mark.size()) mark.push_back(-1);
+        scol[g] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    col.resize(n);
+    for (int i = 0; i < n; ++i) col[i] = scol[seg[i]];
+    return ncol;
+}
+
 void ilu_free_order(kcfd_mat *m) {
     if (m->perm) {
         hipFree(m->perm);
@@ -177,7 +236,8 @@ static int build_order(kcfd_mat *m) {
         return KCFD_OK;
     }
     std::vector<int> col;
-    m->ncolors = color_rows(m->h_indptr, m->h_indices, col);
+    m->ncolors = m->ilu_order == KCFD_ORDER_SEGMENT ? color_segments(m->h_indptr, m->h_indices, col)
+                                                     : color_rows(m->h_indptr, m->h_indices, col);
     // counting sort by colour, stable in the row id
     std::vector<int> cp(m->ncolors + 1, 0);
     for (int c : col) cp[c + 1]++;
@@ -376,7 +436,7 @@ int ilu0_structure(kcfd_mat *m) {
         }
         // chunked L / U layout of the solves: per level of the level lists,
         // or, with two colours, the colour-1 rows (L) and colour-0 rows (U)
-        m->two_colour = m->perm && m->ncolors == 2;
+        m->two_colour = m->perm && m->ncolors == 2 && m->ilu_order == KCFD_ORDER_COLOR;
         if (m->two_colour) {
             const int n0 = m->ncolour0;
             m->nnz_colour0 = m->hf_indptr[n0] - m->hf_indptr[0];
@@ -629,7 +689,8 @@ extern "C" int kcfd_ilu0_get(kcfd_mat *m, double *lu, int *levels_l, int *levels
 
 extern "C" int kcfd_precond_set_order(kcfd_mat *m, int order) {
     KCFD_ARG(m != nullptr, "null matrix");
-    KCFD_ARG(order == KCFD_ORDER_NATURAL || order == KCFD_ORDER_COLOR, "unknown ILU(0) ordering");
+    KCFD_ARG(order == KCFD_ORDER_NATURAL || order == KCFD_ORDER_COLOR || order == KCFD_ORDER_SEGMENT,
+             "unknown ILU(0) ordering");
     std::lock_guard<std::recursive_mutex> g(m->ctx->mu);
     m->ilu_order = order;
     return KCFD_OK;

@@ -18,7 +18,8 @@ TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
   (core/preconditioner.py:40-54): Jacobi ``where(|diag|>1e-30, 1/diag, 0)``
   and scipy ``spilu(A.tocsc(), fill_factor=1, drop_tol=0)``; ``'ilu0_struct'``
   is the textbook ILU(0) of ilu0.c that the GPU implements, ``'ilu0_color'``
-  the same on P A P^T for the first-fit colour ordering.
+  the same on P A P^T for the first-fit colour ordering, ``'ilu0_segment'``
+  for the segment ordering (``segment_order``).
 """
 
 from __future__ import annotations
@@ -134,6 +135,54 @@ def color_order(A):
     return perm, col, nc
 
 
+def segment_order(A, seg_len=4):
+    """Segment ordering (the device's KCFD_ORDER_SEGMENT), restated in plain
+    Python: rows in ascending order cut into segments of at most ``seg_len``
+    rows, a new segment starting at a row not coupled (a_{i,i-1} or
+    a_{i-1,i}) to the row before; segments coloured first-fit over A + A^T
+    in ascending order; rows stably sorted by their segment's colour.
+    Returns (perm, colour per row, number of colours).  A design choice of
+    the GPU build with no reference counterpart: tests pin it by its
+    invariants as well (contiguous coupled segments, no coupling inside a
+    colour between segments)."""
+    A = sparse.csr_matrix(A)
+    A.sort_indices()
+    n = A.shape[0]
+    S = (A + A.T).tocsr()
+    S.sort_indices()
+    seg = np.empty(n, np.int64)
+    starts = []
+    length = 0
+    for i in range(n):
+        if i == 0 or length == seg_len or not _coupled(A, i - 1, i):
+            starts.append(i)
+            length = 0
+        seg[i] = len(starts) - 1
+        length += 1
+    starts.append(n)
+    ns = len(starts) - 1
+    scol = np.full(ns, -1, np.int64)
+    for g in range(ns):
+        taken = set()
+        for i in range(starts[g], starts[g + 1]):
+            for j in S.indices[S.indptr[i]:S.indptr[i + 1]]:
+                h = seg[j]
+                if h != g and scol[h] >= 0:
+                    taken.add(int(scol[h]))
+        c = 0
+        while c in taken:
+            c += 1
+        scol[g] = c
+    col = scol[seg]
+    perm = np.argsort(col, kind="stable").astype(np.int32)
+    return perm, col, int(scol.max()) + 1 if ns else 0
+
+
+def _coupled(A, i, j):
+    """a_ij or a_ji in A's pattern (an explicit zero counts)."""
+    return (j in A.indices[A.indptr[i]:A.indptr[i + 1]]) or (i in A.indices[A.indptr[j]:A.indptr[j + 1]])
+
+
 def permuted(A, perm):
     """P A P^T as canonical CSR (row k = row perm[k] of A)."""
     A = sparse.csr_matrix(A)
@@ -173,6 +222,9 @@ def make_preconditioner(A, method: str):
         return LinearOperator((n, n), matvec=f.solve)
     if method == "ilu0_color":
         f = ILU0Ordered(A, color_order(A)[0])
+        return LinearOperator((n, n), matvec=f.solve)
+    if method == "ilu0_segment":
+        f = ILU0Ordered(A, segment_order(A)[0])
         return LinearOperator((n, n), matvec=f.solve)
     raise ValueError(f"Unknown preconditioner: {method}")
 

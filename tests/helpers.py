@@ -200,7 +200,7 @@ def restated_device(A, b, solver="bicgstab", rows_per_thread=2, perm=None, x0=No
     perm = np.asarray(perm, np.int64)
     Ap = permuted_keep_order(A, perm)
     if M is None:
-        if precond in ("ilu0_color", "ilu0_perm"):
+        if precond in ("ilu0_color", "ilu0_segment", "ilu0_perm"):
             f = osolve.ILU0(osolve.permuted(A, perm))
             M = LinearOperator(A.shape, matvec=f.solve)
         elif precond != "none":
