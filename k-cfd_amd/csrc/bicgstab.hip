@@ -99,6 +99,11 @@ struct SysDesc {
     const unsigned short *chl_o16, *chu_o16;
     const int *chl_base, *chu_base;
     const float *ud32;
+    // segment ordering (kcfd_mat::seg2): {first row, length} per segment,
+    // {entry offset, width} per step, in-segment L / U values by row
+    const int2 *sg, *sg_steps;
+    const double *sgl64, *sgu64;
+    const float *sgl32, *sgu32;
     const int *sptr, *six;          // SELL-64 structure
     const double *sa64;
     const int2 *spk32;              // MIXED: packed {column, FP32 value} SELL entries
@@ -143,6 +148,8 @@ struct BatchParams {
 // an operand (the solve's result is invalid; the host reports KCFD_ESTATE)
 constexpr int kTrsvFail = 6;
 constexpr int kDefaultPollLimit = 1 << 20;
+// ints of the kernels' counts table (kcfd_batch::d_counts)
+constexpr int kCounts = 10;
 
 // ILU trisolve work chunk: up to 64 consecutive rows of one system's level
 // list, 16 bytes: the system (slot), row count and width packed in `meta`
@@ -1630,6 +1637,131 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     }
 }
 
+// ILU(0) apply for a two-colour segment ordering (kcfd_mat::seg2).  Rows
+// of P A P^T come in segments (colour 0 first); a row's entries outside its
+// segment are of the other colour, so one lane per segment can walk its
+// rows, as the sequential L then U solves do (same operations, same order:
+// the entries outside the segment come before the row before it in a
+// colour-1 row's L part, after the row after it in a colour-0 row's U part):
+//   PART 0, colour-0 segments: y_i = z_i - l_{i,i-1} y_{i-1}
+//   PART 1, colour-1 segments: y_i = z_i - sum_j l_ij y_j - l_{i,i-1} y_{i-1},
+//           then w_i = (y_i - u_{i,i+1} w_{i+1}) / u_ii, last row first
+//   PART 2, colour-0 segments: w_i = (y_i - u_{i,i+1} w_{i+1} - sum_j u_ij w_j)
+//           / u_ii, last row first
+// y lives in d.y.  A group record (Chunk) holds up to 64 segments: start =
+// its first segment, count, wd = its longest segment, o0 = its first step
+// {entry offset, width} -- the entries outside the segments of row k of
+// each.  Groups are walked like k_ilu2's chunks.  The own operand z and the
+// formed s are as in k_ilu2 (WHICH 0: p, 1: s = r - alpha v, stored, 2: r).
+template <typename V>
+__device__ __forceinline__ const V *seg_l(const SysDesc &d);
+template <> __device__ __forceinline__ const double *seg_l<double>(const SysDesc &d) { return d.sgl64; }
+template <> __device__ __forceinline__ const float *seg_l<float>(const SysDesc &d) { return d.sgl32; }
+template <typename V>
+__device__ __forceinline__ const V *seg_u(const SysDesc &d);
+template <> __device__ __forceinline__ const double *seg_u<double>(const SysDesc &d) { return d.sgu64; }
+template <> __device__ __forceinline__ const float *seg_u<float>(const SysDesc &d) { return d.sgu32; }
+
+template <typename V, int PART, int WHICH, typename S>
+__device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int lane, const S &src,
+                                         KCFD_GLOBAL V *__restrict__ zout) {
+#pragma clang fp contract(off)
+    const kint2 sv = gl(reinterpret_cast<const kint2 *>(d.sg))[g.start + lane];
+    const int r0 = sv.x, len = sv.y;
+    const kint2 *__restrict__ steps = reinterpret_cast<const kint2 *>(d.sg_steps) + g.o0;
+    KCFD_GLOBAL V *__restrict__ y = vec<V>(d.y);
+    KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 1 ? d.sh : d.ph);
+    const V *__restrict__ lv = seg_l<V>(d);
+    const V *__restrict__ uv = seg_u<V>(d);
+    if constexpr (PART == 0) {
+        double yp = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSegLen; ++k)
+            if (k < len) {
+                const int i = r0 + k;
+                const V zi = src(i);
+                if (zout) zout[i] = zi;
+                double acc = (double)zi;
+                if (k > 0) acc -= (double)gl(lv)[i] * yp;
+                y[i] = (V)acc;
+                yp = acc;
+            }
+    } else if constexpr (PART == 1) {
+        double yk[kSegLen];
+        double yp = 0.0;
+        const PlainSrc<V> ysrc{y};
+#pragma unroll
+        for (int k = 0; k < kSegLen; ++k) {
+            yk[k] = 0.0;
+            if (k < len) {
+                const int i = r0 + k;
+                const V zi = src(i);
+                if (zout) zout[i] = zi;
+                const kint2 st = gl(steps)[k];
+                double acc = sub_plain<V>((double)zi, fac_l(d), st.x + lane, st.y, ysrc);
+                if (k > 0) acc -= (double)gl(lv)[i] * yp;
+                yk[k] = acc;
+                yp = acc;
+            }
+        }
+        double wn = 0.0;
+#pragma unroll
+        for (int k = kSegLen - 1; k >= 0; --k)
+            if (k < len) {
+                const int i = r0 + k;
+                double acc = yk[k];
+                if (k + 1 < len) acc -= (double)gl(uv)[i] * wn;
+                const V wv = (V)(acc / (double)gl(udiag<V>(d))[i]);
+                w[i] = wv;
+                wn = (double)wv;
+            }
+    } else {
+        const PlainSrc<V> wsrc{w};
+        double wn = 0.0;
+#pragma unroll
+        for (int k = kSegLen - 1; k >= 0; --k)
+            if (k < len) {
+                const int i = r0 + k;
+                double acc = (double)y[i];
+                if (k + 1 < len) acc -= (double)gl(uv)[i] * wn;
+                const kint2 st = gl(steps)[k];
+                acc = sub_plain<V>(acc, fac_u(d), st.x + lane, st.y, wsrc);
+                const V wv = (V)(acc / (double)gl(udiag<V>(d))[i]);
+                w[i] = wv;
+                wn = (double)wv;
+            }
+    }
+}
+
+template <typename V, int PART, int WHICH>
+__global__ __launch_bounds__(kBlock) void k_seg(const SysDesc *__restrict__ sys, const SysState *__restrict__ sts,
+                                                const Chunk *__restrict__ groups,
+                                                const int *__restrict__ counts) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const int ng = counts[PART == 1 ? 9 : 8];
+    const int per = (ng + kXcds - 1) / kXcds;
+    const int c0 = (int)(blockIdx.x % kXcds) * per, c1 = min(c0 + per, ng);
+    const int wx = (int)(gridDim.x / kXcds) * kWaves;   // waves per XCD
+    const int lw = uniform((int)(blockIdx.x / kXcds) * kWaves + (int)(threadIdx.x >> 6));
+    for (int c = c0 + lw; c < c1; c += wx) {
+        const Chunk g = groups[c];
+        const SysDesc &d = sys[g.sys()];
+        const SysState &st = sts[g.sys()];
+        if (!ilu2_wants<V, WHICH>(st.status) || lane >= g.count()) continue;
+        if constexpr (PART == 2 || WHICH == 0) {
+            const PlainSrc<V> src{vec<V>(d.p)};
+            seg_walk<V, PART, WHICH>(d, g, lane, src, (KCFD_GLOBAL V *)nullptr);
+        } else if constexpr (WHICH == 1) {
+            const SSrc<V> src{vec<V>(d.r), vec<V>(d.v), (V)st.alpha};
+            seg_walk<V, PART, WHICH>(d, g, lane, src, vec<V>(d.s));
+        } else {
+            const PlainSrc<V> src{vec<V>(d.r)};
+            seg_walk<V, PART, WHICH>(d, g, lane, src, (KCFD_GLOBAL V *)nullptr);
+        }
+    }
+}
+
 template <typename V>
 __global__ void k_fill_sentinel(V *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1668,6 +1800,7 @@ struct SysTables {
     std::vector<int2> tiles;
     std::vector<std::vector<Chunk>> lvl_l, lvl_u;
     std::vector<Chunk> ca, cb;
+    std::vector<Chunk> sg0, sg1;   // segment groups of colour 0 and colour 1
 };
 }  // namespace kcfd
 
@@ -1695,8 +1828,8 @@ struct kcfd_batch {
     int cap_tiles = 0;
     double *d_partials = nullptr;
     unsigned int *d_counters = nullptr;
-    int *d_counts = nullptr;        // {ntiles, nchunks_l, nchunks_u, nchunks_a, nchunks_b, walk},
-                                    // read by the kernels
+    int *d_counts = nullptr;        // {ntiles, nchunks_l, nchunks_u, nchunks_a, nchunks_b, walk,
+                                    // folded, 0, nsg0, nsg1}, read by the kernels
     void *slab = nullptr;           // static: all per-system vectors
     size_t slab_bytes = 0;
     // ILU chunk tables (level-major over systems) for the L and U solves
@@ -1704,6 +1837,10 @@ struct kcfd_batch {
     int nchunks_l = 0, nchunks_u = 0, cap_chunks_l = 0, cap_chunks_u = 0, trsv_grid = 0;
     // two-colour members: colour-1 (a) and colour-0 (b) chunk runs
     Chunk *d_chunks_a = nullptr, *d_chunks_b = nullptr;
+    // segment-ordered members: groups of colour 0 and colour 1
+    Chunk *d_sg0 = nullptr, *d_sg1 = nullptr;
+    int nsg0 = 0, nsg1 = 0, cap_sg0 = 0, cap_sg1 = 0;
+    bool run_seg = false;
     int nchunks_a = 0, nchunks_b = 0, cap_chunks_a = 0, cap_chunks_b = 0;
     bool run_two = false, run_polled = false;   // which ILU kernels the graph launches
     const int **d_lvl_l = nullptr, **d_lvl_u = nullptr;
@@ -1764,8 +1901,16 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
         if (b->run_polled) {
             launch_k(b, whole ? 6 : 3, !b->run_two, !whole, k_trsv<V, false, 0>, tg, kBlock, s, b->d_desc,
                      b->d_state, b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
-            launch_k(b, whole ? 6 : 0, false, whole && !b->run_two, k_trsv<V, true, 0>, tg, kBlock, s,
-                     b->d_desc, b->d_state, b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
+            launch_k(b, whole ? 6 : 0, false, whole && !b->run_two && !b->run_seg, k_trsv<V, true, 0>, tg, kBlock,
+                     s, b->d_desc, b->d_state, b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
+        }
+        if (b->run_seg) {   // segment-ordered members (timed in phase 6 only)
+            launch_k(b, whole ? 6 : 0, whole && !b->run_two && !b->run_polled, false, k_seg<V, 0, 0>, g, kBlock,
+                     s, b->d_desc, b->d_state, b->d_sg0, b->d_counts);
+            launch_k(b, 0, false, false, k_seg<V, 1, 0>, g, kBlock, s, b->d_desc, b->d_state, b->d_sg1,
+                     b->d_counts);
+            launch_k(b, whole ? 6 : 0, false, whole && !b->run_two, k_seg<V, 2, 0>, g, kBlock, s, b->d_desc,
+                     b->d_state, b->d_sg0, b->d_counts);
         }
         if (b->run_two)
             launch_k(b, whole ? 6 : 5, !whole, true, k_ilu2<V, true, 0>, g, kBlock, s, b->d_desc, b->d_state,
@@ -1786,6 +1931,14 @@ void launch_iteration(kcfd_batch *b, hipStream_t s) {
                      b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
             launch_k(b, 0, false, false, k_trsv<V, true, 1>, tg, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
+        }
+        if (b->run_seg) {
+            launch_k(b, 0, false, false, k_seg<V, 0, 1>, g, kBlock, s, b->d_desc, b->d_state, b->d_sg0,
+                     b->d_counts);
+            launch_k(b, 0, false, false, k_seg<V, 1, 1>, g, kBlock, s, b->d_desc, b->d_state, b->d_sg1,
+                     b->d_counts);
+            launch_k(b, 0, false, false, k_seg<V, 2, 1>, g, kBlock, s, b->d_desc, b->d_state, b->d_sg0,
+                     b->d_counts);
         }
         if (b->run_two)
             launch_k(b, 0, false, false, k_ilu2<V, true, 1>, g, kBlock, s, b->d_desc, b->d_state,
@@ -1809,14 +1962,22 @@ void launch_iteration_cg(kcfd_batch *b, hipStream_t s) {
         if (b->run_two) {
             launch_k(b, 3, true, false, k_ilu2<V, false, 2>, g, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_a, b->d_counts);
-            launch_k(b, 3, false, !b->run_polled, k_ilu2<V, true, 2>, g, kBlock, s, b->d_desc, b->d_state,
-                     b->d_chunks_b, b->d_counts);
+            launch_k(b, 3, false, !b->run_polled && !b->run_seg, k_ilu2<V, true, 2>, g, kBlock, s, b->d_desc,
+                     b->d_state, b->d_chunks_b, b->d_counts);
         }
         if (b->run_polled) {
             launch_k(b, 3, !b->run_two, false, k_trsv<V, false, 2>, tg, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_l, b->d_counts, b->d_lvl_l, b->d_params, b->d_counters);
-            launch_k(b, 3, false, true, k_trsv<V, true, 2>, tg, kBlock, s, b->d_desc, b->d_state,
+            launch_k(b, 3, false, !b->run_seg, k_trsv<V, true, 2>, tg, kBlock, s, b->d_desc, b->d_state,
                      b->d_chunks_u, b->d_counts, b->d_lvl_u, b->d_params, b->d_counters);
+        }
+        if (b->run_seg) {
+            launch_k(b, 3, !b->run_two && !b->run_polled, false, k_seg<V, 0, 2>, g, kBlock, s, b->d_desc,
+                     b->d_state, b->d_sg0, b->d_counts);
+            launch_k(b, 3, false, false, k_seg<V, 1, 2>, g, kBlock, s, b->d_desc, b->d_state, b->d_sg1,
+                     b->d_counts);
+            launch_k(b, 3, false, true, k_seg<V, 2, 2>, g, kBlock, s, b->d_desc, b->d_state, b->d_sg0,
+                     b->d_counts);
         }
     }
     launch_k(b, 0, false, false, k_cg_rz<PC, R>, g, kBlock, s, b->d_desc, b->d_state, b->d_tiles, b->d_counts,
@@ -1914,6 +2075,17 @@ LaunchFn pick_finish(int rows) {
 // colour-0 rows' entries and slice pointers (those rows only read v or t,
 // and rt or s); the colour-0 half of each apply writes v or t (n0 s_v) and,
 // in MIXED, reads the FP64 pivots (8 n0)
+// Algorithmic bytes of one segment-ordered ILU(0) apply: the entries
+// outside the segments (value + 4-byte column), the in-segment L and U
+// values and u_ii by row, z read, w written twice over (colour 1 written,
+// gathered once; colour 0 written), the segment and step records, and the
+// colour-0 rows' y (written, read back).
+double seg2_apply_bytes(const kcfd_mat *m, double sv) {
+    const double n = m->n, n0 = m->ncolour0;
+    return (double)(m->nent_l + m->nent_u) * (sv + 4) + 3 * n * sv + 3 * n * sv + 8.0 * m->h_sg.size() +
+           8.0 * m->h_sg_steps.size() + 2 * n0 * sv;
+}
+
 double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st,
                    int method = KCFD_METHOD_BICGSTAB, int nid = 0) {
     const double n = m->n, nnz = m->nnz;
@@ -1922,13 +2094,15 @@ double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st,
     if (method == KCFD_METHOD_CG) {
         // C1..C4 per iteration: Jacobi M_A + 14 n s_v, ILU(0) M_A + M_F + 17 n s_v
         // (the apply's 4 streams included), none M_A + 12 n s_v
-        const double per = pc == KCFD_PC_JACOBI ? MA(8) + 14 * n * 8
-                         : pc == KCFD_PC_ILU0 ? MA(8) + MF(8) + 17 * n * 8 : MA(8) + 12 * n * 8;
+        const double per = (pc == KCFD_PC_JACOBI ? MA(8) + 14 * n * 8
+                            : pc == KCFD_PC_ILU0 ? MA(8) + MF(8) + 17 * n * 8 : MA(8) + 12 * n * 8) +
+                           (pc == KCFD_PC_ILU0 && m->seg2 ? 2 * m->ncolour0 * 8.0 : 0.0);
         return st.total_iters * per + MA(8) + 3 * n * 8;
     }
     auto per_iter = [&](double sv) {
         if (pc == KCFD_PC_JACOBI) return 2 * MA(sv) + 22 * n * sv;
-        if (pc == KCFD_PC_ILU0) return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv;
+        if (pc == KCFD_PC_ILU0)   // (segment ordering: the colour-0 rows' y, written and read, per apply)
+            return 2 * MA(sv) + 2 * MF(sv) + 29 * n * sv + (m->seg2 ? 4 * m->ncolour0 * sv : 0.0);
         return 2 * MA(sv) + 19 * n * sv;
     };
     const double n0 = nid, nnz0 = nid > 0 ? (double)m->nnz_colour0 : 0.0;
@@ -1959,7 +2133,13 @@ void fill_desc(SysDesc &d, kcfd_mat *m) {
     d.dinv32 = m->dinv32;
     d.lu64 = m->lu;
     d.lu32 = m->lu32;
-    d.polled = m->pc_kind == KCFD_PC_ILU0 && !m->two_colour;
+    d.polled = m->pc_kind == KCFD_PC_ILU0 && !m->two_colour && !m->seg2;
+    d.sg = m->seg2 ? m->sg : nullptr;
+    d.sg_steps = m->sg_steps;
+    d.sgl64 = m->sgl64;
+    d.sgu64 = m->sgu64;
+    d.sgl32 = m->sgl32;
+    d.sgu32 = m->sgu32;
     d.chl_off = m->chl_off;
     d.chl_col = m->chl_col;
     d.chu_off = m->chu_off;
@@ -2021,6 +2201,12 @@ int check_mat(const kcfd_batch *b, const kcfd_mat *m, int k) {
         set_error("matrix %d: no FP32 copy (set up the preconditioner with KCFD_MIXED)", k);
         return KCFD_ESTATE;
     }
+    if (b->dynamic && b->bp.precond == KCFD_PC_ILU0 && m->seg2) {
+        // a dynamic batch's graph is captured once for any membership, with
+        // the colour and sync-free launches only
+        set_error("matrix %d: a dynamic batch does not take the segment ILU(0) ordering", k);
+        return KCFD_EARG;
+    }
     return KCFD_OK;
 }
 
@@ -2075,7 +2261,7 @@ int new_batch(kcfd_ctx *ctx, int slots, const kcfd_solve_opts *o, kcfd_batch **o
     KCFD_HIP(hipMemcpyAsync(b->d_state, b->h_state.data(), sizeof(SysState) * slots, hipMemcpyHostToDevice, s));
     KCFD_HIP(hipMalloc(&b->d_counters, sizeof(unsigned int) * 8));
     KCFD_HIP(hipMemsetAsync(b->d_counters, 0, sizeof(unsigned int) * 8, s));
-    KCFD_HIP(hipMalloc(&b->d_counts, sizeof(int) * 8));
+    KCFD_HIP(hipMalloc(&b->d_counts, sizeof(int) * kCounts));
     KCFD_HIP(hipMemsetAsync(b->d_counts, 0, sizeof(int) * 8, s));
     KCFD_HIP(hipMalloc(&b->d_params, sizeof(BatchParams)));
     KCFD_HIP(hipMemcpyAsync(b->d_params, &b->bp, sizeof(BatchParams), hipMemcpyHostToDevice, s));
@@ -2179,6 +2365,15 @@ void build_sys_tables(kcfd_batch *b, int k) {
         for (int q = 0; q < nt; ++q) t.tiles.push_back(make_int2(k, q * kTile));
     }
     if (b->bp.precond != KCFD_PC_ILU0) return;
+    if (m->seg2) {
+        for (int c = 0; c < 2; ++c) {
+            const std::vector<int> &grp = m->h_sgg[c];
+            std::vector<Chunk> &out = c ? t.sg1 : t.sg0;
+            for (size_t q = 0; q + 3 < grp.size(); q += 4)
+                out.push_back(make_chunk(k, grp[q], grp[q + 1], grp[q + 3], grp[q + 2], 0));
+        }
+        return;
+    }
     if (m->two_colour) {
         const int n0 = m->ncolour0, n1 = m->n - m->ncolour0;
         auto chunk = [&](int start, int count, int local, bool upper) {
@@ -2240,7 +2435,7 @@ int rebuild_tables(kcfd_batch *b, const std::vector<char> *only = nullptr) {
         b->d_partials = nullptr;
         KCFD_HIP(hipMalloc(&b->d_partials, sizeof(double) * kMaxPartials * (size_t)b->cap_tiles));
     }
-    std::vector<Chunk> cl, cu, ca, cb;
+    std::vector<Chunk> cl, cu, ca, cb, sg0, sg1;
     if (b->bp.precond == KCFD_PC_ILU0) {
         // sync-free chunks in level-major order over the systems (the solve's
         // progress argument needs every chunk of level l listed before any of
@@ -2264,7 +2459,17 @@ int rebuild_tables(kcfd_batch *b, const std::vector<char> *only = nullptr) {
             if (in(k)) {
                 ca.insert(ca.end(), b->sys_tables[k].ca.begin(), b->sys_tables[k].ca.end());
                 cb.insert(cb.end(), b->sys_tables[k].cb.begin(), b->sys_tables[k].cb.end());
+                sg0.insert(sg0.end(), b->sys_tables[k].sg0.begin(), b->sys_tables[k].sg0.end());
+                sg1.insert(sg1.end(), b->sys_tables[k].sg1.begin(), b->sys_tables[k].sg1.end());
             }
+        b->nsg0 = (int)sg0.size();
+        b->nsg1 = (int)sg1.size();
+        KCFD_CHECK(ensure_table(&b->d_sg0, &b->cap_sg0, b->nsg0, &grew));
+        KCFD_CHECK(ensure_table(&b->d_sg1, &b->cap_sg1, b->nsg1, &grew));
+        if (!sg0.empty())
+            KCFD_HIP(hipMemcpyAsync(b->d_sg0, sg0.data(), sizeof(Chunk) * sg0.size(), hipMemcpyHostToDevice, s));
+        if (!sg1.empty())
+            KCFD_HIP(hipMemcpyAsync(b->d_sg1, sg1.data(), sizeof(Chunk) * sg1.size(), hipMemcpyHostToDevice, s));
         b->nchunks_l = (int)cl.size();
         b->nchunks_u = (int)cu.size();
         b->nchunks_a = (int)ca.size();
@@ -2296,7 +2501,8 @@ int rebuild_tables(kcfd_batch *b, const std::vector<char> *only = nullptr) {
     const int walk = b->opts.tile_walk == 2 ? 1 : 0;
     int folded = 0;
     for (int k = 0; k < b->nsys; ++k) folded |= in(k) && b->h_desc[k].nid > 0;
-    const int counts[8] = {b->ntiles, b->nchunks_l, b->nchunks_u, b->nchunks_a, b->nchunks_b, walk, folded, 0};
+    const int counts[kCounts] = {b->ntiles, b->nchunks_l, b->nchunks_u, b->nchunks_a, b->nchunks_b,
+                                 walk, folded, 0, b->nsg0, b->nsg1};
     KCFD_HIP(hipMemcpyAsync(b->d_counts, counts, sizeof(counts), hipMemcpyHostToDevice, s));
     // the host arrays above are locals: finish the copies before they go
     KCFD_HIP(hipStreamSynchronize(s));
@@ -2345,6 +2551,7 @@ void set_launch_shape(kcfd_batch *b, const kcfd_solve_opts *o) {
     kcfd_ctx *ctx = b->ctx;
     b->run_two = b->nchunks_a + b->nchunks_b > 0;
     b->run_polled = b->nchunks_l + b->nchunks_u > 0;
+    b->run_seg = b->nsg0 + b->nsg1 > 0;
     // persistent grid: at most grid_per_cu workgroups per CU, a multiple of
     // the XCD count
     const int gpc = o->grid_per_cu > 0 ? o->grid_per_cu : 8;
@@ -2569,9 +2776,12 @@ int solve_noalloc(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_o
     describe_slot(b, 0, m, bvec, x);
     // table sizes of this system
     const int ntiles = std::max(1, (m->n + b->rows * kBlock - 1) / (b->rows * kBlock));
-    int nl = 0, nu = 0, na = 0, nb = 0;
+    int nl = 0, nu = 0, na = 0, nb = 0, ns0 = 0, ns1 = 0;
     if (b->bp.precond == KCFD_PC_ILU0) {
-        if (m->two_colour) {
+        if (m->seg2) {
+            ns0 = (int)m->h_sgg[0].size() / 4;
+            ns1 = (int)m->h_sgg[1].size() / 4;
+        } else if (m->two_colour) {
             na = (m->n - m->ncolour0 + 63) / 64;
             nb = (m->ncolour0 + 63) / 64;
         } else {
@@ -2583,7 +2793,7 @@ int solve_noalloc(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_o
     b->d_desc = c.take<SysDesc>(1);
     b->d_state = c.take<SysState>(1);
     b->d_params = c.take<BatchParams>(1);
-    b->d_counts = c.take<int>(8);
+    b->d_counts = c.take<int>(kCounts);
     b->d_counters = c.take<unsigned int>(8);
     b->d_tiles = c.take<int2>(ntiles);
     b->d_partials = c.take<double>((size_t)kMaxPartials * ntiles);
@@ -2591,6 +2801,8 @@ int solve_noalloc(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_o
     b->d_chunks_u = c.take<Chunk>(nu);
     b->d_chunks_a = c.take<Chunk>(na);
     b->d_chunks_b = c.take<Chunk>(nb);
+    b->d_sg0 = c.take<Chunk>(ns0);
+    b->d_sg1 = c.take<Chunk>(ns1);
     b->d_lvl_l = c.take<const int *>(1);
     b->d_lvl_u = c.take<const int *>(1);
     char *vecs = c.take<char>(slot_bytes(b, b->h_desc[0]));
@@ -2604,6 +2816,8 @@ int solve_noalloc(kcfd_mat *m, const double *bvec, double *x, const kcfd_solve_o
     b->cap_chunks_u = nu;
     b->cap_chunks_a = na;
     b->cap_chunks_b = nb;
+    b->cap_sg0 = ns0;
+    b->cap_sg1 = ns1;
     place_vectors(b, 0, vecs);
     // every table fits its slice, so rebuild_tables allocates nothing
     KCFD_CHECK(rebuild_tables(b));
@@ -2653,6 +2867,8 @@ int kcfd_batch_destroy(kcfd_batch *b) {
     hipFree(b->d_chunks_u);
     hipFree(b->d_chunks_a);
     hipFree(b->d_chunks_b);
+    hipFree(b->d_sg0);
+    hipFree(b->d_sg1);
     hipFree((void *)b->d_lvl_l);
     hipFree((void *)b->d_lvl_u);
     delete b;
@@ -2869,6 +3085,8 @@ int kcfd_batch_profile(kcfd_batch *b, int phase, int reps, double *avg_ms) {
     KCFD_ARG(phase != 6 || b->bp.method == KCFD_METHOD_BICGSTAB, "phase 6 needs BiCGSTAB");
     KCFD_ARG(phase != 5 || (b->run_two && b->bp.method == KCFD_METHOD_BICGSTAB),
              "phase 5 needs a two-colour ILU(0) member and BiCGSTAB");
+    KCFD_ARG(phase != 3 || b->bp.method == KCFD_METHOD_CG || b->run_two || b->run_polled,
+             "phase 3 of BiCGSTAB needs a two-colour or sync-free ILU(0) member (phase 6 times the whole apply)");
     std::lock_guard<std::recursive_mutex> g(b->ctx->mu);
     KCFD_HIP(hipSetDevice(b->ctx->device));
     hipStream_t s = b->ctx->stream;
@@ -2955,9 +3173,14 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
         else if (cg && phase == 4) tot += (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv;
         else if (phase == 1) tot += MA + (b->bp.precond == KCFD_PC_JACOBI ? 4 : 3) * n * sv + fold;
         else if (phase == 2) tot += 8 * n * sv + (b->bp.precision == KCFD_MIXED ? 8 * n : 0);  // FP64 x
-        else if (phase == 6) {   // the whole apply: both halves, or both sync-free solves
-            tot += m->two_colour ? (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv + n0 * sv + x64 * 8 * n0
-                                 : 2 * (MFh + 2 * n * sv);
+        else if (phase == 6) {   // the whole apply: both halves, both sync-free solves, or the
+                                 // three segment launches
+            if (m->two_colour)
+                tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv + n0 * sv + x64 * 8 * n0;
+            else if (m->seg2)    // entries, the in-segment values, z, w, u_ii, y of colour 0 (written, read)
+                tot += seg2_apply_bytes(m, sv);
+            else
+                tot += 2 * (MFh + 2 * n * sv);
         }
         else if (phase == 3 || phase == 5) {
             // two-colour apply, colour-1 launch (3): its L entries, own z
@@ -2968,6 +3191,8 @@ int kcfd_batch_kernel_bytes(kcfd_batch *b, int phase, double *bytes) {
             const double n1 = n - m->ncolour0, nc0 = m->ncolour0;
             if (m->two_colour && cg)   // CG times both launches of its apply as phase 3
                 tot += (double)(m->nent_l + m->nent_u) * (sv + 4) + 4 * n * sv;
+            else if (m->seg2 && cg)
+                tot += seg2_apply_bytes(m, sv);
             else if (m->two_colour && phase == 3)
                 tot += (double)m->nent_l * (sv + 4) + (3 * n1 + nc0) * sv;
             else if (m->two_colour)

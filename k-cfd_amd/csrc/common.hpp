@@ -205,6 +205,26 @@ struct kcfd_mat {
     // BiCGSTAB product (kcfd_solve_opts.fold_colour0): a_ii + sum_j (a_ij -
     // fl32(a_ij)), so that the row [pivot, fl32(a_ij)] has A's row sum
     double *c0piv = nullptr;
+    // Segment ordering with two colours (KCFD_ORDER_SEGMENT): the solves
+    // walk segments, one lane per segment.  Segments are contiguous runs of
+    // rows of P A P^T, colour 0 first; h_sg[g] = {first row, length}.  A
+    // segment group is up to 64 consecutive segments of one colour; for each
+    // step k (row k of every segment of the group) the chunk layout above
+    // holds the rows' entries outside their segment -- colour-1 rows: their
+    // L entries (colour-0 columns) in the L layout; colour-0 rows: their U
+    // entries (colour-1 columns) in the U layout -- with entry e of lane l at
+    // off + 64e + l.  h_sg_steps[t] = {entry offset, width} of step t; a
+    // group's steps are consecutive.  sgl / sgu map each row's in-segment L
+    // entry (row before) / U entry (row after) to its factor slot (-1: none).
+    bool seg2 = false;
+    std::vector<int2> h_sg, h_sg_steps;
+    // groups of colour 0 and colour 1, 4 ints each: first segment, segments,
+    // longest segment, first step
+    std::vector<int> h_sgg[2];
+    int2 *sg = nullptr, *sg_steps = nullptr;
+    int *sgl_map = nullptr, *sgu_map = nullptr;
+    double *sgl64 = nullptr, *sgu64 = nullptr;
+    float *sgl32 = nullptr, *sgu32 = nullptr;
     std::vector<int> h_chl_ptr, h_chu_ptr;
     // host copies of the per-chunk entry offsets and (narrow-index form)
     // base columns, which the batch's chunk tables carry

@@ -149,13 +149,16 @@ static int color_rows(const std::vector<int> &ip, const std::vector<int> &ix, st
 }
 
 // First-fit colouring of segments (see the ordering note above): col[i] =
-// colour of row i's segment; returns the number of colours.
-static int color_segments(const std::vector<int> &ip, const std::vector<int> &ix, std::vector<int> &col) {
+// colour of row i's segment, seg[i] its segment; returns the number of
+// colours.
+static int color_segments(const std::vector<int> &ip, const std::vector<int> &ix, std::vector<int> &col,
+                          std::vector<int> &seg) {
     const int n = (int)ip.size() - 1;
     auto has = [&](int i, int j) {   // a_ij in the (sorted) pattern
         return std::binary_search(ix.begin() + ip[i], ix.begin() + ip[i + 1], j);
     };
-    std::vector<int> seg(n), start;
+    std::vector<int> start;
+    seg.assign(n, 0);
     for (int i = 0, len = 0; i < n; ++i) {
         if (i == 0 || len == kSegLen || !(has(i, i - 1) || has(i - 1, i))) {
             start.push_back(i);
@@ -235,8 +238,8 @@ static int build_order(kcfd_mat *m) {
         m->ilu_built_order = m->ilu_order;
         return KCFD_OK;
     }
-    std::vector<int> col;
-    m->ncolors = m->ilu_order == KCFD_ORDER_SEGMENT ? color_segments(m->h_indptr, m->h_indices, col)
+    std::vector<int> col, seg;
+    m->ncolors = m->ilu_order == KCFD_ORDER_SEGMENT ? color_segments(m->h_indptr, m->h_indices, col, seg)
                                                      : color_rows(m->h_indptr, m->h_indices, col);
     // counting sort by colour, stable in the row id
     std::vector<int> cp(m->ncolors + 1, 0);
@@ -247,6 +250,13 @@ static int build_order(kcfd_mat *m) {
     perm.assign(n, 0);
     for (int i = 0; i < n; ++i) perm[cp[col[i]]++] = i;
     for (int k = 0; k < n; ++k) iperm[perm[k]] = k;
+    // segments in P-order (each stays contiguous): {first row, length}
+    m->h_sg.clear();
+    if (m->ilu_order == KCFD_ORDER_SEGMENT)
+        for (int k = 0; k < n; ++k) {
+            if (k == 0 || seg[perm[k]] != seg[perm[k - 1]]) m->h_sg.push_back(make_int2(k, 0));
+            m->h_sg.back().y++;
+        }
     // P A P^T: row k = row perm[k] of A, columns renumbered and re-sorted
     std::vector<int> &fp = m->hf_indptr, &fx = m->hf_indices, &fd = m->hf_diagpos;
     std::vector<int> vmap(nnz), uix(nnz);
@@ -329,6 +339,88 @@ static void chunk_layout(const kcfd_mat *m, const std::vector<int> &rows, const 
         }
         ch_ptr.push_back((int)off.size() - 1);
     }
+}
+
+// Segment layout of a two-colour segment ordering (kcfd_mat::seg2): the
+// in-segment entries of each row (sgl: the row before, sgu: the row after)
+// and, per group of up to 64 segments of one colour and per step k, the
+// entries of row k of each segment outside its segment -- colour 1: its L
+// entries (loff/lcol/lmap), colour 0: its U entries (uoff/ucol/umap) -- in
+// CSR order, column-major over the group's lanes.  False when some row has
+// an entry outside its segment to a row of its own colour, which the
+// segment solves cannot take (the level layout serves such a matrix).
+static bool seg_layout(kcfd_mat *m, std::vector<int> &loff, std::vector<int> &lcol, std::vector<int> &lmap,
+                       std::vector<int> &uoff, std::vector<int> &ucol, std::vector<int> &umap,
+                       std::vector<int> &sgl, std::vector<int> &sgu) {
+    const std::vector<int> &ip = m->hf_indptr, &ix = m->hf_indices, &dp = m->hf_diagpos;
+    const int n = m->n, n0 = m->ncolour0;
+    const std::vector<int2> &sg = m->h_sg;
+    sgl.assign(n, -1);
+    sgu.assign(n, -1);
+    int nseg0 = 0;
+    for (const int2 &g : sg) {
+        const bool c1 = g.x >= n0;
+        if (c1 != (g.x + g.y - 1 >= n0) || g.y > kSegLen || g.y < 1) return false;
+        if (!c1) ++nseg0;
+        for (int k = 0; k < g.y; ++k) {
+            const int i = g.x + k;
+            for (int q = ip[i]; q < ip[i + 1]; ++q) {
+                const int j = ix[q];
+                if (j == i) continue;
+                if (k > 0 && j == i - 1) { sgl[i] = q; continue; }
+                if (k + 1 < g.y && j == i + 1) { sgu[i] = q; continue; }
+                if ((j >= n0) == c1) return false;   // outside the segment: the other colour only
+            }
+        }
+    }
+    m->h_sg_steps.clear();
+    loff.assign(1, 0);
+    uoff.assign(1, 0);
+    lcol.clear(); lmap.clear(); ucol.clear(); umap.clear();
+    for (int c = 0; c < 2; ++c) {
+        std::vector<int> &grp = m->h_sgg[c], &off = c ? loff : uoff, &col = c ? lcol : ucol,
+                         &map = c ? lmap : umap;
+        grp.clear();
+        const int s0 = c ? nseg0 : 0, s1 = c ? (int)sg.size() : nseg0;
+        for (int f = s0; f < s1; f += 64) {
+            const int cnt = std::min(64, s1 - f);
+            int maxlen = 0;
+            for (int t = 0; t < cnt; ++t) maxlen = std::max(maxlen, sg[f + t].y);
+            grp.insert(grp.end(), {f, cnt, maxlen, (int)m->h_sg_steps.size()});
+            for (int k = 0; k < maxlen; ++k) {
+                // colour 1: the L part without the row before; colour 0: the
+                // U part without the row after
+                auto outside = [&](int i, int q) { return q != sgl[i] && q != sgu[i]; };
+                int w = 0;
+                for (int t = 0; t < cnt; ++t) {
+                    if (k >= sg[f + t].y) continue;
+                    const int i = sg[f + t].x + k;
+                    const int q0 = c ? ip[i] : dp[i] + 1, q1 = c ? dp[i] : ip[i + 1];
+                    int e = 0;
+                    for (int q = q0; q < q1; ++q) e += outside(i, q);
+                    w = std::max(w, e);
+                }
+                const size_t base = col.size();
+                col.resize(base + (size_t)64 * w, -1);
+                map.resize(base + (size_t)64 * w, -1);
+                for (int t = 0; t < cnt; ++t) {
+                    if (k >= sg[f + t].y) continue;
+                    const int i = sg[f + t].x + k;
+                    const int q0 = c ? ip[i] : dp[i] + 1, q1 = c ? dp[i] : ip[i + 1];
+                    int e = 0;
+                    for (int q = q0; q < q1; ++q)
+                        if (outside(i, q)) {
+                            col[base + (size_t)64 * e + t] = ix[q];
+                            map[base + (size_t)64 * e + t] = q;
+                            ++e;
+                        }
+                }
+                m->h_sg_steps.push_back(make_int2((int)base, w));
+                off.push_back((int)col.size());
+            }
+        }
+    }
+    return true;
 }
 
 static int upload_ints(const std::vector<int> &h, int **d, hipStream_t s) {
@@ -437,6 +529,11 @@ int ilu0_structure(kcfd_mat *m) {
         // chunked L / U layout of the solves: per level of the level lists,
         // or, with two colours, the colour-1 rows (L) and colour-0 rows (U)
         m->two_colour = m->perm && m->ncolors == 2 && m->ilu_order == KCFD_ORDER_COLOR;
+        // segment ordering with two colours: the segment layout, when the
+        // factor structure allows it
+        std::vector<int> sl_off, sl_col, sl_map, su_off, su_col, su_map, sgl, sgu;
+        m->seg2 = m->ilu_order == KCFD_ORDER_SEGMENT && m->perm && m->ncolors == 2 &&
+                  seg_layout(m, sl_off, sl_col, sl_map, su_off, su_col, su_map, sgl, sgu);
         if (m->two_colour) {
             const int n0 = m->ncolour0;
             m->nnz_colour0 = m->hf_indptr[n0] - m->hf_indptr[0];
@@ -454,7 +551,12 @@ int ilu0_structure(kcfd_mat *m) {
             bool narrow = m->two_colour && !(env && env[0] == '0');
             std::vector<int> off, col, map, lbase, ubase;
             std::vector<unsigned short> lo16, uo16;
-            chunk_layout(m, rl, lvl_l, false, m->h_chl_ptr, off, col, map);
+            if (m->seg2) {
+                off = sl_off; col = sl_col; map = sl_map;
+                m->h_chl_ptr = {0, (int)off.size() - 1};
+            } else {
+                chunk_layout(m, rl, lvl_l, false, m->h_chl_ptr, off, col, map);
+            }
             KCFD_CHECK(check_chunk_widths(off));
             m->nch_l = (int)off.size() - 1;
             m->nent_l = (int)col.size();
@@ -464,7 +566,12 @@ int ilu0_structure(kcfd_mat *m) {
             KCFD_CHECK(upload_ints(col, &m->chl_col, s));
             KCFD_CHECK(upload_ints(map, &m->chl_map, s));
             KCFD_HIP(hipStreamSynchronize(s));
-            chunk_layout(m, ru, lvl_u, true, m->h_chu_ptr, off, col, map);
+            if (m->seg2) {
+                off = su_off; col = su_col; map = su_map;
+                m->h_chu_ptr = {0, (int)off.size() - 1};
+            } else {
+                chunk_layout(m, ru, lvl_u, true, m->h_chu_ptr, off, col, map);
+            }
             KCFD_CHECK(check_chunk_widths(off));
             m->nch_u = (int)off.size() - 1;
             m->nent_u = (int)col.size();
@@ -482,6 +589,18 @@ int ilu0_structure(kcfd_mat *m) {
                 KCFD_CHECK(upload_vec(ubase, &m->chu_base, s));
             }
             m->ch16 = narrow;
+            if (m->seg2) {
+                std::vector<int> sgi(2 * m->h_sg.size()), sti(2 * m->h_sg_steps.size());
+                for (size_t g = 0; g < m->h_sg.size(); ++g) { sgi[2 * g] = m->h_sg[g].x; sgi[2 * g + 1] = m->h_sg[g].y; }
+                for (size_t t = 0; t < m->h_sg_steps.size(); ++t) {
+                    sti[2 * t] = m->h_sg_steps[t].x;
+                    sti[2 * t + 1] = m->h_sg_steps[t].y;
+                }
+                KCFD_CHECK(upload_ints(sgi, (int **)&m->sg, s));
+                KCFD_CHECK(upload_ints(sti, (int **)&m->sg_steps, s));
+                KCFD_CHECK(upload_ints(sgl, &m->sgl_map, s));
+                KCFD_CHECK(upload_ints(sgu, &m->sgu_map, s));
+            }
             KCFD_HIP(hipStreamSynchronize(s));   // the host vectors are locals
         }
     }
@@ -545,6 +664,20 @@ int ilu0_setup(kcfd_mat *m, bool want32) {
         if (n)
             hipLaunchKernelGGL(k_chunk_vals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (size_t)n,
                                m->f_diagpos, nullptr, m->lu, m->ud64, nullptr, want32 ? m->ud32 : nullptr);
+        if (m->seg2 && n) {   // the in-segment entries by row
+            if (!m->sgl64) {
+                KCFD_HIP(hipMalloc(&m->sgl64, sizeof(double) * n));
+                KCFD_HIP(hipMalloc(&m->sgu64, sizeof(double) * n));
+            }
+            if (want32 && !m->sgl32) {
+                KCFD_HIP(hipMalloc(&m->sgl32, sizeof(float) * n));
+                KCFD_HIP(hipMalloc(&m->sgu32, sizeof(float) * n));
+            }
+            hipLaunchKernelGGL(k_chunk_vals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (size_t)n,
+                               m->sgl_map, nullptr, m->lu, m->sgl64, nullptr, want32 ? m->sgl32 : nullptr);
+            hipLaunchKernelGGL(k_chunk_vals, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, (size_t)n,
+                               m->sgu_map, nullptr, m->lu, m->sgu64, nullptr, want32 ? m->sgu32 : nullptr);
+        }
         if (want32 && m->two_colour && m->ncolour0 > 0) {
             if (!m->c0piv) KCFD_HIP(hipMalloc(&m->c0piv, sizeof(double) * m->ncolour0));
             hipLaunchKernelGGL(k_colour0_pivots, dim3((m->ncolour0 + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
@@ -744,6 +877,18 @@ extern "C" int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src) {
     KCFD_CHECK(dup_ints(src->chu_col, (size_t)src->nent_u, &dst->chu_col, dst->ctx->stream));
     KCFD_CHECK(dup_ints(src->chl_map, (size_t)src->nent_l, &dst->chl_map, dst->ctx->stream));
     dst->two_colour = src->two_colour;
+    dst->seg2 = src->seg2;
+    if (src->seg2) {
+        KCFD_CHECK(dup_ints((const int *)src->sg, 2 * src->h_sg.size(), (int **)&dst->sg, dst->ctx->stream));
+        KCFD_CHECK(dup_ints((const int *)src->sg_steps, 2 * src->h_sg_steps.size(), (int **)&dst->sg_steps,
+                            dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->sgl_map, n, &dst->sgl_map, dst->ctx->stream));
+        KCFD_CHECK(dup_ints(src->sgu_map, n, &dst->sgu_map, dst->ctx->stream));
+        dst->h_sg = src->h_sg;
+        dst->h_sg_steps = src->h_sg_steps;
+        dst->h_sgg[0] = src->h_sgg[0];
+        dst->h_sgg[1] = src->h_sgg[1];
+    }
     dst->ncolour0 = src->ncolour0;
     dst->nnz_colour0 = src->nnz_colour0;
     if (src->ch16) {

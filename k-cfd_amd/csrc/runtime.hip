@@ -339,6 +339,10 @@ int mat_free_precond(kcfd_mat *m) {
     hipFree(m->chu_v32); m->chu_v32 = nullptr;
     hipFree(m->ud32); m->ud32 = nullptr;
     hipFree(m->c0piv); m->c0piv = nullptr;
+    hipFree(m->sgl64); m->sgl64 = nullptr;
+    hipFree(m->sgu64); m->sgu64 = nullptr;
+    hipFree(m->sgl32); m->sgl32 = nullptr;
+    hipFree(m->sgu32); m->sgu32 = nullptr;
     m->have32 = false;
     m->pc_kind = KCFD_PC_NONE;
     return KCFD_OK;
@@ -377,6 +381,19 @@ int mat_free_ilu_structure(kcfd_mat *m) {
     hipFree(m->chl_base); m->chl_base = nullptr;
     hipFree(m->chu_base); m->chu_base = nullptr;
     m->ch16 = false;
+    hipFree(m->sg); m->sg = nullptr;
+    hipFree(m->sg_steps); m->sg_steps = nullptr;
+    hipFree(m->sgl_map); m->sgl_map = nullptr;
+    hipFree(m->sgu_map); m->sgu_map = nullptr;
+    hipFree(m->sgl64); m->sgl64 = nullptr;
+    hipFree(m->sgu64); m->sgu64 = nullptr;
+    hipFree(m->sgl32); m->sgl32 = nullptr;
+    hipFree(m->sgu32); m->sgu32 = nullptr;
+    m->seg2 = false;
+    m->h_sg.clear();
+    m->h_sg_steps.clear();
+    m->h_sgg[0].clear();
+    m->h_sgg[1].clear();
     ilu_free_order(m);
     m->h_lvl_ptr_l.clear();
     m->h_lvl_ptr_u.clear();
