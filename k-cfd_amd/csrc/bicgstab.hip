@@ -1662,6 +1662,34 @@ __device__ __forceinline__ const V *seg_u(const SysDesc &d);
 template <> __device__ __forceinline__ const double *seg_u<double>(const SysDesc &d) { return d.sgu64; }
 template <> __device__ __forceinline__ const float *seg_u<float>(const SysDesc &d) { return d.sgu32; }
 
+// The products a_ij x_j of the entries outside the segment in step k's row,
+// in CSR order (padding, column -1, comes last and is left out: nv).  A
+// product is rounded as the sequential solve rounds it before subtracting.
+template <typename V, typename F, typename S>
+__device__ __forceinline__ void seg_products(const F &f, int base, int w, const S &src, double (&pr)[kSegW],
+                                             int &nv) {
+#pragma clang fp contract(off)
+    int c[kSegW];
+    V av[kSegW], xv[kSegW];
+#pragma unroll
+    for (int e = 0; e < kSegW; ++e) {
+        c[e] = -1;
+        av[e] = (V)0;
+        if (e < w) fac_get<V>(f, base + 64 * e, c[e], av[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < kSegW; ++e) xv[e] = e < w ? src(max(c[e], 0)) : (V)0;
+    nv = 0;
+#pragma unroll
+    for (int e = 0; e < kSegW; ++e) {
+        pr[e] = (double)av[e] * (double)xv[e];
+        nv += c[e] >= 0;
+    }
+}
+
+// One segment (one lane).  Every load of the segment's rows is issued
+// before its arithmetic, which then runs in the sequential solves' order,
+// and the stores come last: the rows' loads do not wait on each other.
 template <typename V, int PART, int WHICH, typename S>
 __device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int lane, const S &src,
                                          KCFD_GLOBAL V *__restrict__ zout) {
@@ -1673,62 +1701,94 @@ __device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int l
     KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 1 ? d.sh : d.ph);
     const V *__restrict__ lv = seg_l<V>(d);
     const V *__restrict__ uv = seg_u<V>(d);
+    const V *__restrict__ dg = udiag<V>(d);
+    V out[kSegLen];
     if constexpr (PART == 0) {
+        V z[kSegLen], l[kSegLen];
+#pragma unroll
+        for (int k = 0; k < kSegLen; ++k) {
+            z[k] = l[k] = (V)0;
+            if (k < len) {
+                z[k] = src(r0 + k);
+                l[k] = gl(lv)[r0 + k];
+            }
+        }
         double yp = 0.0;
+#pragma unroll
+        for (int k = 0; k < kSegLen; ++k) {
+            double acc = (double)z[k];
+            if (k > 0) acc -= (double)l[k] * yp;
+            out[k] = (V)acc;
+            yp = acc;
+        }
 #pragma unroll
         for (int k = 0; k < kSegLen; ++k)
             if (k < len) {
-                const int i = r0 + k;
-                const V zi = src(i);
-                if (zout) zout[i] = zi;
-                double acc = (double)zi;
-                if (k > 0) acc -= (double)gl(lv)[i] * yp;
-                y[i] = (V)acc;
-                yp = acc;
+                if (zout) zout[r0 + k] = z[k];
+                y[r0 + k] = out[k];
             }
-    } else if constexpr (PART == 1) {
-        double yk[kSegLen];
-        double yp = 0.0;
-        const PlainSrc<V> ysrc{y};
+    } else {
+        double pr[kSegLen][kSegW];
+        int nv[kSegLen];
+        V z[kSegLen], l[kSegLen], u[kSegLen], dd[kSegLen];
+        const PlainSrc<V> osrc{PART == 1 ? y : w};   // colour-0 y, or colour-1 w
 #pragma unroll
         for (int k = 0; k < kSegLen; ++k) {
-            yk[k] = 0.0;
+            z[k] = l[k] = u[k] = (V)0;
+            dd[k] = (V)1;
+            nv[k] = 0;
+#pragma unroll
+            for (int e = 0; e < kSegW; ++e) pr[k][e] = 0.0;
             if (k < len) {
                 const int i = r0 + k;
-                const V zi = src(i);
-                if (zout) zout[i] = zi;
+                z[k] = PART == 1 ? src(i) : y[i];
+                if (PART == 1) l[k] = gl(lv)[i];
+                u[k] = gl(uv)[i];
+                dd[k] = gl(dg)[i];
                 const kint2 st = gl(steps)[k];
-                double acc = sub_plain<V>((double)zi, fac_l(d), st.x + lane, st.y, ysrc);
-                if (k > 0) acc -= (double)gl(lv)[i] * yp;
+                if (PART == 1) seg_products<V>(fac_l(d), st.x + lane, st.y, osrc, pr[k], nv[k]);
+                else seg_products<V>(fac_u(d), st.x + lane, st.y, osrc, pr[k], nv[k]);
+            }
+        }
+        if constexpr (PART == 1) {
+            double yk[kSegLen];
+            double yp = 0.0;
+#pragma unroll
+            for (int k = 0; k < kSegLen; ++k) {
+                double acc = (double)z[k];
+#pragma unroll
+                for (int e = 0; e < kSegW; ++e)
+                    if (e < nv[k]) acc -= pr[k][e];
+                if (k > 0) acc -= (double)l[k] * yp;
                 yk[k] = acc;
                 yp = acc;
             }
-        }
-        double wn = 0.0;
+            double wn = 0.0;
 #pragma unroll
-        for (int k = kSegLen - 1; k >= 0; --k)
-            if (k < len) {
-                const int i = r0 + k;
+            for (int k = kSegLen - 1; k >= 0; --k) {
                 double acc = yk[k];
-                if (k + 1 < len) acc -= (double)gl(uv)[i] * wn;
-                const V wv = (V)(acc / (double)gl(udiag<V>(d))[i]);
-                w[i] = wv;
-                wn = (double)wv;
+                if (k + 1 < len) acc -= (double)u[k] * wn;
+                out[k] = (V)(acc / (double)dd[k]);
+                wn = (double)out[k];
             }
-    } else {
-        const PlainSrc<V> wsrc{w};
-        double wn = 0.0;
+        } else {
+            double wn = 0.0;
 #pragma unroll
-        for (int k = kSegLen - 1; k >= 0; --k)
+            for (int k = kSegLen - 1; k >= 0; --k) {
+                double acc = (double)z[k];
+                if (k + 1 < len) acc -= (double)u[k] * wn;
+#pragma unroll
+                for (int e = 0; e < kSegW; ++e)
+                    if (e < nv[k]) acc -= pr[k][e];
+                out[k] = (V)(acc / (double)dd[k]);
+                wn = (double)out[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kSegLen; ++k)
             if (k < len) {
-                const int i = r0 + k;
-                double acc = (double)y[i];
-                if (k + 1 < len) acc -= (double)gl(uv)[i] * wn;
-                const kint2 st = gl(steps)[k];
-                acc = sub_plain<V>(acc, fac_u(d), st.x + lane, st.y, wsrc);
-                const V wv = (V)(acc / (double)gl(udiag<V>(d))[i]);
-                w[i] = wv;
-                wn = (double)wv;
+                if (PART == 1 && zout) zout[r0 + k] = z[k];
+                w[r0 + k] = out[k];
             }
     }
 }

@@ -400,6 +400,7 @@ static bool seg_layout(kcfd_mat *m, std::vector<int> &loff, std::vector<int> &lc
                     for (int q = q0; q < q1; ++q) e += outside(i, q);
                     w = std::max(w, e);
                 }
+                if (w > kSegW) return false;   // the segment kernel holds at most kSegW per row
                 const size_t base = col.size();
                 col.resize(base + (size_t)64 * w, -1);
                 map.resize(base + (size_t)64 * w, -1);
