@@ -427,11 +427,15 @@ def run_pressure(args, ctx, case):
         L.check(L.lib().kcfd_mat_solver(A.h, L.ctypes.byref(bh)))
         kern = {}
         two_colour = args.precond == "ilu0" and args.order == "color"
+        seg = args.precond == "ilu0" and args.order == "segment"
         phases = [(1, "spmv_p"), (2, "xr_update"), (4, "p_update")]
         if args.precond == "ilu0":
-            phases += ([(3, "ilu_colour1"), (5, "ilu_colour0"), (6, "ilu_apply")]
-                       if two_colour and args.method == "bicgstab"
-                       else [(3, "ilu_apply" if two_colour else "ilu_lower_solve")])
+            if two_colour and args.method == "bicgstab":
+                phases += [(3, "ilu_colour1"), (5, "ilu_colour0"), (6, "ilu_apply")]
+            elif seg and args.method == "bicgstab":     # the three segment launches
+                phases += [(6, "ilu_apply")]
+            else:
+                phases += [(3, "ilu_apply" if two_colour or seg else "ilu_lower_solve")]
         for phase, name in phases:
             ms = L.ctypes.c_double()
             by = L.ctypes.c_double()
@@ -466,17 +470,20 @@ def run_pressure(args, ctx, case):
                     "kernel": "whole GMRES solve"}
     else:
         two_colour = args.precond == "ilu0" and args.order == "color"
-        name = "ilu_apply" if two_colour else "spmv_p"
+        seg = args.precond == "ilu0" and args.order == "segment" and args.method == "bicgstab"
+        name = "ilu_apply" if two_colour or seg else "spmv_p"
         dk = kern[name]
         achieved = dk["gbs"]
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": "k_ilu2" if two_colour else "k_spmv_p",
+                    "kernel": "k_ilu2" if two_colour else ("k_seg" if seg else "k_spmv_p"),
                     "work_unit": ("one ILU(0) apply: the colour-1 and colour-0 launches of k_ilu2, timed from "
-                                  "the first's start to the second's end" if two_colour else "one launch"),
+                                  "the first's start to the second's end" if two_colour else
+                                  ("one ILU(0) apply: the three k_seg launches, timed from the first's start "
+                                   "to the last's end" if seg else "one launch")),
                     "avg_us": round(1e3 * dk["avg_ms"], 2), "bytes_per_unit": dk["bytes"],
                     "solve_avg_gbs": round(solve_gbs, 1)}
-        if two_colour:
+        if two_colour or seg:
             sp = kern["spmv_p"]
             roofline["k_spmv_p"] = {"avg_us": round(1e3 * sp["avg_ms"], 2), "bytes": sp["bytes"],
                                     "gbs": round(sp["gbs"], 1), "frac": round(sp["gbs"] / HBM_PEAK_GBS, 4)}
@@ -613,8 +620,9 @@ def run_batched(args, ctx, cases, steps):
             # the apply to p: phase 3 (the two-colour members' colour-1 rows
             # and the other members' L solve) plus, with two-colour members,
             # phase 5 (their colour-0 rows)
+            # (segment order: phase 6, the whole apply of every member)
             tot_ms, tot_by = 0.0, 0.0
-            for phase in ((3, 5) if order == "color" else (3,)):
+            for phase in ((3, 5) if order == "color" else ((6,) if order == "segment" else (3,))):
                 if phase == 5 and L.lib().kcfd_batch_kernel_bytes(bobj.h, 5, L.ctypes.byref(by)) == 0 \
                         and by.value == 0.0:
                     continue   # no two-colour member
@@ -622,7 +630,7 @@ def run_batched(args, ctx, cases, steps):
                 L.check(L.lib().kcfd_batch_kernel_bytes(bobj.h, phase, L.ctypes.byref(by)))
                 tot_ms += ms.value
                 tot_by += by.value
-            summ["ilu_apply" if order == "color" else "ilu_lower_solve"] = dict(
+            summ["ilu_apply" if order in ("color", "segment") else "ilu_lower_solve"] = dict(
                 avg_ms=tot_ms, bytes=tot_by, gbs=tot_by / (tot_ms * 1e-3) / 1e9)
         # the profile overwrote x: solve once more for the solutions
         sb.solve(batch=bobj, **opts)
