@@ -23,17 +23,20 @@ def small_batch(ctx):
     sb.close()
 
 
-@pytest.mark.parametrize("precond", ["jacobi", "ilu0"])
-def test_batch_fp64_matches_oracle(small_batch, precond):
+@pytest.mark.parametrize("precond,order", [("jacobi", "color"), ("ilu0", "color"), ("ilu0", "segment")])
+def test_batch_fp64_matches_oracle(small_batch, precond, order):
+    """(the segment ordering: its quad and hex members take the segment
+    apply kernel, the tet members the sync-free solves, in one batch)"""
     sb, refs, specs = small_batch
-    r = sb.solve(precision="fp64", precond=precond, order="color")
+    r = sb.solve(precision="fp64", precond=precond, order=order)
     xs = sb.solutions()
     for k, (res, (A, b), x) in enumerate(zip(r.results, refs, xs)):
         # the device-assembled matrix is the oracle's
         Ad = sb.mats[k].to_scipy()
         assert abs(Ad - A).max() <= 1e-14 * abs(A).max()
         tr = osolve.true_rel_residual(A, x, b)
-        print(f"{specs[k]} {precond}: iters={res.iters} info={res.info} resid={res.rel_resid:.2e} host={tr:.2e}")
+        print(f"{specs[k]} {precond} {order}: iters={res.iters} info={res.info} resid={res.rel_resid:.2e} "
+              f"host={tr:.2e}")
         assert res.info == 0 and tr < 1e-8 and abs(tr - res.rel_resid) <= 1e-3 * tr + 1e-15
     # every system equals the restated solver in the device's dot order (each
     # system's tiles are summed as if it were alone in the batch)

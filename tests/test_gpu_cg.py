@@ -55,6 +55,18 @@ def test_cg_colour_ilu0(ctx):
     _check(x, res, A, b, "cg colour", perm=M.ilu0_order()[0], rtol=1e-8, precond="ilu0_color")
 
 
+def test_cg_segment_ilu0(ctx):
+    """ILU(0) in the segment ordering (the segment apply kernel, CG's
+    variant): equal to the restated CG in that ordering."""
+    from kcfd_amd import device
+    A, b, _ = oracle_system(_case("hex12"))
+    M = device.DeviceMatrix.from_csr(ctx, A)
+    M.setup_precond("ilu0", "fp64", order="segment")
+    x, res = M.solve(b, rtol=1e-8, precond="ilu0", method="cg")
+    assert res.info == 0 and res.rel_resid < 1e-8
+    _check(x, res, A, b, "cg segment", perm=M.ilu0_order()[0], rtol=1e-8, precond="ilu0_segment")
+
+
 def test_cg_known_answers(ctx):
     """Three distinct eigenvalues: at most 3 iterations; Jacobi on a
     diagonal matrix: exact after 1; maxiter reached: info = maxiter."""
