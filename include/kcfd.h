@@ -141,10 +141,11 @@ int kcfd_ilu0_get(kcfd_mat *m, double *lu, int *levels_l, int *levels_u);
  * many dependency levels as colours (2 on the 5- and 7-point grids).
  * KCFD_ORDER_SEGMENT colours segments instead of rows: runs of up to 4
  * consecutive rows each coupled to the one before (pieces of grid lines on a
- * structured mesh in its natural numbering), kept contiguous; the factors
- * keep the couplings inside a segment, which costs the solves up to 4 levels
- * per colour and saves Krylov iterations on grids.  Takes effect at the next
- * kcfd_precond_setup. */
+ * structured mesh in its natural numbering); groups of 64 segments of one
+ * colour are numbered step-major (first rows, then second rows, ...).  The
+ * factors keep the couplings inside a segment, which costs the solves up to
+ * 4 levels per colour and saves Krylov iterations on grids.  Takes effect at
+ * the next kcfd_precond_setup. */
 #define KCFD_ORDER_NATURAL 0
 #define KCFD_ORDER_COLOR 1
 #define KCFD_ORDER_SEGMENT 2

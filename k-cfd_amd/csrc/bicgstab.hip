@@ -101,7 +101,8 @@ struct SysDesc {
     const float *ud32;
     // segment ordering (kcfd_mat::seg2): {first row, length} per segment,
     // {entry offset, width} per step, in-segment L / U values by row
-    const int2 *sg, *sg_steps;
+    const int2 *sg;
+    const int4 *sg_steps;
     const double *sgl64, *sgu64;
     const float *sgl32, *sgu32;
     const int *sptr, *six;          // SELL-64 structure
@@ -1637,22 +1638,24 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) 
     }
 }
 
-// ILU(0) apply for a two-colour segment ordering (kcfd_mat::seg2).  Rows
-// of P A P^T come in segments (colour 0 first); a row's entries outside its
-// segment are of the other colour, so one lane per segment can walk its
-// rows, as the sequential L then U solves do (same operations, same order:
-// the entries outside the segment come before the row before it in a
-// colour-1 row's L part, after the row after it in a colour-0 row's U part):
-//   PART 0, colour-0 segments: y_i = z_i - l_{i,i-1} y_{i-1}
-//   PART 1, colour-1 segments: y_i = z_i - sum_j l_ij y_j - l_{i,i-1} y_{i-1},
-//           then w_i = (y_i - u_{i,i+1} w_{i+1}) / u_ii, last row first
-//   PART 2, colour-0 segments: w_i = (y_i - u_{i,i+1} w_{i+1} - sum_j u_ij w_j)
+// ILU(0) apply for a two-colour segment ordering (kcfd_mat::seg2).  A
+// row's entries outside its segment are of the other colour (colour 0
+// first), so one lane per segment can walk its rows, as the sequential L
+// then U solves do (same operations, same order: the entries outside the
+// segment come before the row before it in a colour-1 row's L part, after
+// the row after it in a colour-0 row's U part):
+//   PART 0, colour-0 segments: y_i = z_i - l_{i,i-} y_{i-}
+//   PART 1, colour-1 segments: y_i = z_i - sum_j l_ij y_j - l_{i,i-} y_{i-},
+//           then w_i = (y_i - u_{i,i+} w_{i+}) / u_ii, last row first
+//   PART 2, colour-0 segments: w_i = (y_i - u_{i,i+} w_{i+} - sum_j u_ij w_j)
 //           / u_ii, last row first
-// y lives in d.y.  A group record (Chunk) holds up to 64 segments: start =
-// its first segment, count, wd = its longest segment, o0 = its first step
-// {entry offset, width} -- the entries outside the segments of row k of
-// each.  Groups are walked like k_ilu2's chunks.  The own operand z and the
-// formed s are as in k_ilu2 (WHICH 0: p, 1: s = r - alpha v, stored, 2: r).
+// (i- / i+: the row before / after i in its segment).  y lives in d.y.  A
+// group record (Chunk) holds up to 64 segments numbered step-major, so
+// that a step's rows are consecutive over the lanes: start = its first
+// segment, count, wd = its longest segment, o0 = its first step {entry
+// offset, width, row offset}.  Groups are walked like k_ilu2's chunks.  The
+// own operand z and the formed s are as in k_ilu2 (WHICH 0: p, 1: s = r -
+// alpha v, stored, 2: r).
 template <typename V>
 __device__ __forceinline__ const V *seg_l(const SysDesc &d);
 template <> __device__ __forceinline__ const double *seg_l<double>(const SysDesc &d) { return d.sgl64; }
@@ -1696,12 +1699,22 @@ __device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int l
 #pragma clang fp contract(off)
     const kint2 sv = gl(reinterpret_cast<const kint2 *>(d.sg))[g.start + lane];
     const int r0 = sv.x, len = sv.y;
-    const kint2 *__restrict__ steps = reinterpret_cast<const kint2 *>(d.sg_steps) + g.o0;
+    const kint4 *__restrict__ steps = reinterpret_cast<const kint4 *>(d.sg_steps) + g.o0;
     KCFD_GLOBAL V *__restrict__ y = vec<V>(d.y);
     KCFD_GLOBAL V *__restrict__ w = vec<V>(WHICH == 1 ? d.sh : d.ph);
     const V *__restrict__ lv = seg_l<V>(d);
     const V *__restrict__ uv = seg_u<V>(d);
     const V *__restrict__ dg = udiag<V>(d);
+    // the segment's rows (step-major numbering: row k at r0 + the step's row
+    // offset) and its steps' entry layouts
+    int ri[kSegLen];
+    kint4 st[kSegLen];
+#pragma unroll
+    for (int k = 0; k < kSegLen; ++k) {
+        st[k] = kint4{0, 0, 0, 0};
+        if (k < len) st[k] = gl(steps)[k];
+        ri[k] = r0 + st[k].z;
+    }
     V out[kSegLen];
     if constexpr (PART == 0) {
         V z[kSegLen], l[kSegLen];
@@ -1709,8 +1722,8 @@ __device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int l
         for (int k = 0; k < kSegLen; ++k) {
             z[k] = l[k] = (V)0;
             if (k < len) {
-                z[k] = src(r0 + k);
-                l[k] = gl(lv)[r0 + k];
+                z[k] = src(ri[k]);
+                l[k] = gl(lv)[ri[k]];
             }
         }
         double yp = 0.0;
@@ -1724,8 +1737,8 @@ __device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int l
 #pragma unroll
         for (int k = 0; k < kSegLen; ++k)
             if (k < len) {
-                if (zout) zout[r0 + k] = z[k];
-                y[r0 + k] = out[k];
+                if (zout) zout[ri[k]] = z[k];
+                y[ri[k]] = out[k];
             }
     } else {
         double pr[kSegLen][kSegW];
@@ -1740,14 +1753,13 @@ __device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int l
 #pragma unroll
             for (int e = 0; e < kSegW; ++e) pr[k][e] = 0.0;
             if (k < len) {
-                const int i = r0 + k;
+                const int i = ri[k];
                 z[k] = PART == 1 ? src(i) : y[i];
                 if (PART == 1) l[k] = gl(lv)[i];
                 u[k] = gl(uv)[i];
                 dd[k] = gl(dg)[i];
-                const kint2 st = gl(steps)[k];
-                if (PART == 1) seg_products<V>(fac_l(d), st.x + lane, st.y, osrc, pr[k], nv[k]);
-                else seg_products<V>(fac_u(d), st.x + lane, st.y, osrc, pr[k], nv[k]);
+                if (PART == 1) seg_products<V>(fac_l(d), st[k].x + lane, st[k].y, osrc, pr[k], nv[k]);
+                else seg_products<V>(fac_u(d), st[k].x + lane, st[k].y, osrc, pr[k], nv[k]);
             }
         }
         if constexpr (PART == 1) {
@@ -1787,8 +1799,8 @@ __device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int l
 #pragma unroll
         for (int k = 0; k < kSegLen; ++k)
             if (k < len) {
-                if (PART == 1 && zout) zout[r0 + k] = z[k];
-                w[r0 + k] = out[k];
+                if (PART == 1 && zout) zout[ri[k]] = z[k];
+                w[ri[k]] = out[k];
             }
     }
 }
@@ -2143,7 +2155,7 @@ LaunchFn pick_finish(int rows) {
 double seg2_apply_bytes(const kcfd_mat *m, double sv) {
     const double n = m->n, n0 = m->ncolour0;
     return (double)(m->nent_l + m->nent_u) * (sv + 4) + 3 * n * sv + 3 * n * sv + 8.0 * m->h_sg.size() +
-           8.0 * m->h_sg_steps.size() + 2 * n0 * sv;
+           16.0 * m->h_sg_steps.size() + 2 * n0 * sv;
 }
 
 double model_bytes(const kcfd_mat *m, int pc, int precision, const SysState &st,

@@ -53,6 +53,7 @@ __device__ __forceinline__ KCFD_GLOBAL T *glw(T *p) {
 // clang's own 2 x int vector: loadable through a global-space pointer (HIP's
 // int2 wrapper type cannot be copied out of one)
 typedef int kint2 __attribute__((ext_vector_type(2)));
+typedef int kint4 __attribute__((ext_vector_type(4)));
 
 // a value the caller knows is the same in every lane of the wave, as such
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -209,22 +210,25 @@ struct kcfd_mat {
     // fl32(a_ij)), so that the row [pivot, fl32(a_ij)] has A's row sum
     double *c0piv = nullptr;
     // Segment ordering with two colours (KCFD_ORDER_SEGMENT): the solves
-    // walk segments, one lane per segment.  Segments are contiguous runs of
-    // rows of P A P^T, colour 0 first; h_sg[g] = {first row, length}.  A
-    // segment group is up to 64 consecutive segments of one colour; for each
-    // step k (row k of every segment of the group) the chunk layout above
-    // holds the rows' entries outside their segment -- colour-1 rows: their
-    // L entries (colour-0 columns) in the L layout; colour-0 rows: their U
-    // entries (colour-1 columns) in the U layout -- with entry e of lane l at
-    // off + 64e + l.  h_sg_steps[t] = {entry offset, width} of step t; a
-    // group's steps are consecutive.  sgl / sgu map each row's in-segment L
-    // entry (row before) / U entry (row after) to its factor slot (-1: none).
+    // walk segments, one lane per segment.  A segment group is up to 64
+    // consecutive segments of one colour (colour 0 first), longest first,
+    // its rows numbered step-major: row k of lane t is h_sg[g].x + the
+    // step's row offset, h_sg[g] = {row at step 0, length}.  For each step
+    // the chunk layout above holds the rows' entries outside their segment
+    // -- colour-1 rows: their L entries (colour-0 columns) in the L layout;
+    // colour-0 rows: their U entries (colour-1 columns) in the U layout --
+    // with entry e of lane l at off + 64e + l.  h_sg_steps[t] = {entry
+    // offset, width, row offset, 0} of step t; a group's steps are
+    // consecutive.  sgl / sgu map each row's in-segment L entry (the row
+    // before) / U entry (the row after) to its factor slot (-1: none).
     bool seg2 = false;
-    std::vector<int2> h_sg, h_sg_steps;
+    std::vector<int2> h_sg;
+    std::vector<int4> h_sg_steps;
     // groups of colour 0 and colour 1, 4 ints each: first segment, segments,
     // longest segment, first step
     std::vector<int> h_sgg[2];
-    int2 *sg = nullptr, *sg_steps = nullptr;
+    int2 *sg = nullptr;
+    int4 *sg_steps = nullptr;
     int *sgl_map = nullptr, *sgu_map = nullptr;
     double *sgl64 = nullptr, *sgu64 = nullptr;
     float *sgl32 = nullptr, *sgu32 = nullptr;

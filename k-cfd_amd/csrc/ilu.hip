@@ -20,11 +20,14 @@
 // of up to kSegLen consecutive rows of A, each coupled to the one before it
 // (on a structured grid in its natural numbering, pieces of a grid line).
 // Segments are coloured first-fit on the graph of A + A^T contracted to
-// segments, and rows sorted by colour keep their order, so a segment stays
-// contiguous.  The coupling inside a segment is kept by the factorisation,
-// which costs the solves up to kSegLen levels per colour and buys fewer
-// Krylov iterations than the row colouring on grids (the strong couplings
-// along a line survive).
+// segments.  Within a colour, segments in the order of their first rows are
+// taken 64 at a time; a group's segments are sorted longest first (stable)
+// and its rows numbered step-major: the first row of every segment of the
+// group, then the second rows, and so on.  A segment's rows keep their
+// order.  The coupling inside a segment is kept by the factorisation, which
+// costs the solves up to kSegLen levels per colour and buys fewer Krylov
+// iterations than the row colouring on grids (the strong couplings along a
+// line survive).
 #include "common.hpp"
 
 #include <algorithm>
@@ -250,13 +253,36 @@ static int build_order(kcfd_mat *m) {
     perm.assign(n, 0);
     for (int i = 0; i < n; ++i) perm[cp[col[i]]++] = i;
     for (int k = 0; k < n; ++k) iperm[perm[k]] = k;
-    // segments in P-order (each stays contiguous): {first row, length}
+    // Segment ordering: groups of 64 consecutive segments of one colour,
+    // longest first within a group, numbered step-major (row k of every
+    // segment of the group together), so that the lanes of the segment
+    // solves read consecutive rows.  h_sg: per segment, in group order,
+    // {its row at step 0, length}.
     m->h_sg.clear();
-    if (m->ilu_order == KCFD_ORDER_SEGMENT)
+    if (m->ilu_order == KCFD_ORDER_SEGMENT) {
+        std::vector<int2> segs;   // {first row of A, length}, colour-major, natural order
         for (int k = 0; k < n; ++k) {
-            if (k == 0 || seg[perm[k]] != seg[perm[k - 1]]) m->h_sg.push_back(make_int2(k, 0));
-            m->h_sg.back().y++;
+            if (k == 0 || seg[perm[k]] != seg[perm[k - 1]]) segs.push_back(make_int2(perm[k], 0));
+            segs.back().y++;
         }
+        std::vector<int> np;
+        np.reserve(n);
+        const int nseg = (int)segs.size();
+        for (int s0 = 0; s0 < nseg;) {
+            const int c = col[segs[s0].x];
+            int s1 = s0;
+            while (s1 < nseg && s1 - s0 < 64 && col[segs[s1].x] == c) ++s1;
+            std::stable_sort(segs.begin() + s0, segs.begin() + s1,
+                             [](const int2 &a, const int2 &b) { return a.y > b.y; });
+            const int base = (int)np.size();
+            for (int k = 0; k < segs[s0].y; ++k)
+                for (int t = s0; t < s1 && segs[t].y > k; ++t) np.push_back(segs[t].x + k);
+            for (int t = s0; t < s1; ++t) m->h_sg.push_back(make_int2(base + (t - s0), segs[t].y));
+            s0 = s1;
+        }
+        perm = np;
+        for (int k = 0; k < n; ++k) iperm[perm[k]] = k;
+    }
     // P A P^T: row k = row perm[k] of A, columns renumbered and re-sorted
     std::vector<int> &fp = m->hf_indptr, &fx = m->hf_indices, &fd = m->hf_diagpos;
     std::vector<int> vmap(nnz), uix(nnz);
@@ -358,21 +384,7 @@ static bool seg_layout(kcfd_mat *m, std::vector<int> &loff, std::vector<int> &lc
     sgl.assign(n, -1);
     sgu.assign(n, -1);
     int nseg0 = 0;
-    for (const int2 &g : sg) {
-        const bool c1 = g.x >= n0;
-        if (c1 != (g.x + g.y - 1 >= n0) || g.y > kSegLen || g.y < 1) return false;
-        if (!c1) ++nseg0;
-        for (int k = 0; k < g.y; ++k) {
-            const int i = g.x + k;
-            for (int q = ip[i]; q < ip[i + 1]; ++q) {
-                const int j = ix[q];
-                if (j == i) continue;
-                if (k > 0 && j == i - 1) { sgl[i] = q; continue; }
-                if (k + 1 < g.y && j == i + 1) { sgu[i] = q; continue; }
-                if ((j >= n0) == c1) return false;   // outside the segment: the other colour only
-            }
-        }
-    }
+    for (const int2 &g : sg) nseg0 += g.x < n0;
     m->h_sg_steps.clear();
     loff.assign(1, 0);
     uoff.assign(1, 0);
@@ -384,8 +396,32 @@ static bool seg_layout(kcfd_mat *m, std::vector<int> &loff, std::vector<int> &lc
         const int s0 = c ? nseg0 : 0, s1 = c ? (int)sg.size() : nseg0;
         for (int f = s0; f < s1; f += 64) {
             const int cnt = std::min(64, s1 - f);
-            int maxlen = 0;
-            for (int t = 0; t < cnt; ++t) maxlen = std::max(maxlen, sg[f + t].y);
+            const int maxlen = sg[f].y;   // the longest segment leads its group
+            // row offsets of the steps: row k of lane t is sg[f + t].x + roff[k]
+            std::vector<int> roff(maxlen + 1, 0);
+            for (int k = 0; k < maxlen; ++k) {
+                int ck = 0;
+                for (int t = 0; t < cnt; ++t) ck += sg[f + t].y > k;
+                roff[k + 1] = roff[k] + ck;
+            }
+            // structure: inside a segment only the rows before and after,
+            // outside it only rows of the other colour
+            for (int t = 0; t < cnt; ++t) {
+                const int2 g = sg[f + t];
+                if (g.y < 1 || g.y > kSegLen || g.y > maxlen || (g.x >= n0) != (c == 1)) return false;
+                for (int k = 0; k < g.y; ++k) {
+                    const int i = g.x + roff[k];
+                    if ((i >= n0) != (c == 1)) return false;
+                    const int prev = k > 0 ? g.x + roff[k - 1] : -1, next = k + 1 < g.y ? g.x + roff[k + 1] : -1;
+                    for (int q = ip[i]; q < ip[i + 1]; ++q) {
+                        const int j = ix[q];
+                        if (j == i) continue;
+                        if (j == prev) { sgl[i] = q; continue; }
+                        if (j == next) { sgu[i] = q; continue; }
+                        if ((j >= n0) == (c == 1)) return false;   // outside: the other colour only
+                    }
+                }
+            }
             grp.insert(grp.end(), {f, cnt, maxlen, (int)m->h_sg_steps.size()});
             for (int k = 0; k < maxlen; ++k) {
                 // colour 1: the L part without the row before; colour 0: the
@@ -394,7 +430,7 @@ static bool seg_layout(kcfd_mat *m, std::vector<int> &loff, std::vector<int> &lc
                 int w = 0;
                 for (int t = 0; t < cnt; ++t) {
                     if (k >= sg[f + t].y) continue;
-                    const int i = sg[f + t].x + k;
+                    const int i = sg[f + t].x + roff[k];
                     const int q0 = c ? ip[i] : dp[i] + 1, q1 = c ? dp[i] : ip[i + 1];
                     int e = 0;
                     for (int q = q0; q < q1; ++q) e += outside(i, q);
@@ -406,7 +442,7 @@ static bool seg_layout(kcfd_mat *m, std::vector<int> &loff, std::vector<int> &lc
                 map.resize(base + (size_t)64 * w, -1);
                 for (int t = 0; t < cnt; ++t) {
                     if (k >= sg[f + t].y) continue;
-                    const int i = sg[f + t].x + k;
+                    const int i = sg[f + t].x + roff[k];
                     const int q0 = c ? ip[i] : dp[i] + 1, q1 = c ? dp[i] : ip[i + 1];
                     int e = 0;
                     for (int q = q0; q < q1; ++q)
@@ -416,7 +452,7 @@ static bool seg_layout(kcfd_mat *m, std::vector<int> &loff, std::vector<int> &lc
                             ++e;
                         }
                 }
-                m->h_sg_steps.push_back(make_int2((int)base, w));
+                m->h_sg_steps.push_back(make_int4((int)base, w, roff[k], 0));
                 off.push_back((int)col.size());
             }
         }
@@ -591,11 +627,13 @@ int ilu0_structure(kcfd_mat *m) {
             }
             m->ch16 = narrow;
             if (m->seg2) {
-                std::vector<int> sgi(2 * m->h_sg.size()), sti(2 * m->h_sg_steps.size());
+                std::vector<int> sgi(2 * m->h_sg.size()), sti(4 * m->h_sg_steps.size());
                 for (size_t g = 0; g < m->h_sg.size(); ++g) { sgi[2 * g] = m->h_sg[g].x; sgi[2 * g + 1] = m->h_sg[g].y; }
                 for (size_t t = 0; t < m->h_sg_steps.size(); ++t) {
-                    sti[2 * t] = m->h_sg_steps[t].x;
-                    sti[2 * t + 1] = m->h_sg_steps[t].y;
+                    sti[4 * t] = m->h_sg_steps[t].x;
+                    sti[4 * t + 1] = m->h_sg_steps[t].y;
+                    sti[4 * t + 2] = m->h_sg_steps[t].z;
+                    sti[4 * t + 3] = 0;
                 }
                 KCFD_CHECK(upload_ints(sgi, (int **)&m->sg, s));
                 KCFD_CHECK(upload_ints(sti, (int **)&m->sg_steps, s));
@@ -881,7 +919,7 @@ extern "C" int kcfd_mat_copy_order(kcfd_mat *dst, const kcfd_mat *src) {
     dst->seg2 = src->seg2;
     if (src->seg2) {
         KCFD_CHECK(dup_ints((const int *)src->sg, 2 * src->h_sg.size(), (int **)&dst->sg, dst->ctx->stream));
-        KCFD_CHECK(dup_ints((const int *)src->sg_steps, 2 * src->h_sg_steps.size(), (int **)&dst->sg_steps,
+        KCFD_CHECK(dup_ints((const int *)src->sg_steps, 4 * src->h_sg_steps.size(), (int **)&dst->sg_steps,
                             dst->ctx->stream));
         KCFD_CHECK(dup_ints(src->sgl_map, n, &dst->sgl_map, dst->ctx->stream));
         KCFD_CHECK(dup_ints(src->sgu_map, n, &dst->sgu_map, dst->ctx->stream));

@@ -140,11 +140,14 @@ def segment_order(A, seg_len=4):
     Python: rows in ascending order cut into segments of at most ``seg_len``
     rows, a new segment starting at a row not coupled (a_{i,i-1} or
     a_{i-1,i}) to the row before; segments coloured first-fit over A + A^T
-    in ascending order; rows stably sorted by their segment's colour.
+    in ascending order.  Within each colour (colour 0 first) the segments,
+    in the order of their first rows, are taken 64 at a time; a group's
+    segments are sorted longest first (stable) and its rows numbered
+    step-major (the first row of each segment, then the second rows, ...).
     Returns (perm, colour per row, number of colours).  A design choice of
     the GPU build with no reference counterpart: tests pin it by its
-    invariants as well (contiguous coupled segments, no coupling inside a
-    colour between segments)."""
+    invariants as well (coupled segments, no coupling inside a colour
+    between segments, each segment's rows in order)."""
     A = sparse.csr_matrix(A)
     A.sort_indices()
     n = A.shape[0]
@@ -174,8 +177,16 @@ def segment_order(A, seg_len=4):
             c += 1
         scol[g] = c
     col = scol[seg]
-    perm = np.argsort(col, kind="stable").astype(np.int32)
-    return perm, col, int(scol.max()) + 1 if ns else 0
+    nc = int(scol.max()) + 1 if ns else 0
+    perm = []
+    for c in range(nc):
+        segs = [g for g in range(ns) if scol[g] == c]
+        for f in range(0, len(segs), 64):
+            group = sorted(segs[f:f + 64], key=lambda g: -(starts[g + 1] - starts[g]))   # stable
+            lens = [starts[g + 1] - starts[g] for g in group]
+            for k in range(max(lens)):
+                perm.extend(starts[g] + k for g, ln in zip(group, lens) if ln > k)
+    return np.asarray(perm, np.int32), col, nc
 
 
 def _coupled(A, i, j):

@@ -140,48 +140,60 @@ def test_color_ordered_preconditioner_converges():
     assert it_nat <= it <= it_jac
 
 
-@pytest.mark.parametrize("which", ["quad12", "hex6", "quad7x5"])
+@pytest.mark.parametrize("which", ["quad12", "hex6", "quad7x5", "quad100"])
 def test_segment_order_invariants(which):
-    """The segment ordering restated for the device's KCFD_ORDER_SEGMENT:
-    segments are runs of at most 4 rows each coupled to the one before, no
-    two segments of one colour are coupled, rows keep their order within a
-    colour, and on grids in their natural numbering two colours suffice
-    (segments of grid lines, checkerboard over the segment grid)."""
+    """The segment ordering restated for the device's KCFD_ORDER_SEGMENT,
+    against its definition: segments (recomputed here: runs of at most 4
+    rows each coupled to the one before) are never coupled to another
+    segment of their colour, colour 0 comes first, every segment's rows keep
+    their order, and within a colour the rows come in groups of 64 segments
+    numbered step-major, longest segments first; on grids in their natural
+    numbering two colours suffice."""
     from kcfd_amd import workloads as W
     from helpers import oracle_system
     case = {"quad12": lambda: W.quad_convdiff(12, seed=1001),
             "hex6": lambda: W.hex_pressure(6, seed=1002),
-            "quad7x5": lambda: W.pressure_case(W.quad_mesh(7, 5), seed=3)}[which]()
+            "quad7x5": lambda: W.pressure_case(W.quad_mesh(7, 5), seed=3),
+            "quad100": lambda: W.quad_convdiff(100, seed=1001)}[which]()
     A, _, _ = oracle_system(case)
     A = sparse.csr_matrix(A)
     n = A.shape[0]
     perm, col, nc = osolve.segment_order(A, seg_len=4)
     assert nc == 2
     assert np.array_equal(np.sort(perm), np.arange(n))
-    assert np.all(np.diff(col[perm]) >= 0)                      # sorted by colour
-    for c in range(nc):                                          # stable within a colour
-        rows = perm[col[perm] == c]
-        assert np.all(np.diff(rows) > 0)
-    # recover the segments: maximal runs of consecutive rows, same colour,
-    # consecutive in perm, coupled to the previous row
+    pos = np.empty(n, np.int64)
+    pos[perm] = np.arange(n)
+    assert np.all(np.diff(col[perm]) >= 0)                      # colour 0 first
     S = (abs(A) + abs(A.T)).tocsr()
     seg = np.zeros(n, np.int64)
     g, length = 0, 0
-    for k in range(1, n):
-        i = perm[k]
-        same = perm[k - 1] == i - 1 and col[i] == col[i - 1] and S[i, i - 1] != 0 and length < 3
-        if same:
+    for i in range(1, n):
+        if length < 3 and S[i, i - 1] != 0:
             length += 1
         else:
             g, length = g + 1, 0
         seg[i] = g
-    # no coupling between two different segments of one colour
+    assert np.all(col == col[np.r_[0, np.flatnonzero(np.diff(seg)) + 1]][seg])   # one colour per segment
     Sc = S.tocoo()
     off = (Sc.row != Sc.col) & (seg[Sc.row] != seg[Sc.col])
     assert not np.any(col[Sc.row[off]] == col[Sc.col[off]])
-    # segments have at most 4 rows and each row is coupled to the one before
-    _, counts = np.unique(seg, return_counts=True)
-    assert counts.max() <= 4
+    same = np.flatnonzero(np.diff(seg) == 0)                    # i, i + 1 in one segment
+    assert np.all(pos[same + 1] > pos[same])
+    # step-major groups: within a colour, segments 64 at a time; a group
+    # lists the first rows of its segments (longest first), then the second
+    # rows in the same lane order, ...
+    starts = np.r_[0, np.flatnonzero(np.diff(seg)) + 1]
+    lens = np.diff(np.r_[starts, n])
+    k0 = 0
+    for c in range(nc):
+        segs = [s_ for s_ in range(len(starts)) if col[starts[s_]] == c]
+        for f in range(0, len(segs), 64):
+            grp = segs[f:f + 64]
+            order = sorted(grp, key=lambda s_: -lens[s_])
+            rows = [starts[s_] + k for k in range(max(lens[grp])) for s_ in order if lens[s_] > k]
+            assert np.array_equal(perm[k0:k0 + len(rows)], rows)
+            k0 += len(rows)
+    assert k0 == n
 
 
 def test_segment_order_cuts_iterations():
