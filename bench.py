@@ -794,7 +794,8 @@ def bench_serve(args):
         "config": {"workload": f"serving: Poisson {args.rate:.0f} req/s for {args.duration:.0f} s, "
                                f"each request pressure correction + 3 momentum solves on a 1e3..1e6-cell "
                                f"mesh (quad/hex/tet 50/30/20); pressure: FP64 CG+{args.precond}, momentum: "
-                               f"mixed BiCGSTAB+{args.precond}; rtol 1e-8, every answer FP64-verified",
+                               f"mixed BiCGSTAB+{args.precond} (FP64 on the segment ordering for the templates whose "
+                               f"mixed calibration solve stalls); rtol 1e-8, every answer FP64-verified",
                    "requests": len(tr), "templates_per_kind": len(sizes),
                    "step": "the whole trace (continuous batching)"},
         "latency_ms": {"p50": mixed["p50_ms"], "p99": mixed["p99_ms"]},

@@ -262,7 +262,8 @@ int kcfd_batch_bytes(kcfd_batch *b, long long *bytes);
  * stays with the batch and is reused by the next attach that needs a block
  * of the same size (no device allocation or free, each of which would
  * synchronise the device); kcfd_batch_trim frees the blocks kept this way
- * (kcfd_batch_bytes counts them). */
+ * (kcfd_batch_bytes counts them).  The graph launches the segment-ordered
+ * ILU(0) apply (KCFD_ORDER_SEGMENT) once the first such member attaches. */
 int kcfd_batch_create_dynamic(kcfd_ctx *ctx, int capacity, const kcfd_solve_opts *opts,
                               kcfd_batch **out);
 int kcfd_batch_attach(kcfd_batch *b, kcfd_mat *m, const double *bvec, double *x, int *slot);

@@ -2273,12 +2273,6 @@ int check_mat(const kcfd_batch *b, const kcfd_mat *m, int k) {
         set_error("matrix %d: no FP32 copy (set up the preconditioner with KCFD_MIXED)", k);
         return KCFD_ESTATE;
     }
-    if (b->dynamic && b->bp.precond == KCFD_PC_ILU0 && m->seg2) {
-        // a dynamic batch's graph is captured once for any membership, with
-        // the colour and sync-free launches only
-        set_error("matrix %d: a dynamic batch does not take the segment ILU(0) ordering", k);
-        return KCFD_EARG;
-    }
     return KCFD_OK;
 }
 
@@ -2987,6 +2981,16 @@ int kcfd_batch_attach(kcfd_batch *b, kcfd_mat *m, const double *bvec, double *x,
         return KCFD_EBUSY;
     }
     KCFD_CHECK(check_mat(b, m, k));
+    if (b->bp.precond == KCFD_PC_ILU0 && m->seg2 && !b->run_seg) {
+        // the first segment-ordered member: the iterations launch the
+        // segment apply from now on (the graph is captured again)
+        b->run_seg = true;
+        if (b->graph) {
+            KCFD_HIP(hipStreamSynchronize(b->ctx->stream));
+            hipGraphExecDestroy(b->graph);
+            b->graph = nullptr;
+        }
+    }
     describe_slot(b, k, m, bvec, x);
     void *mem = nullptr;
     const size_t bytes = slot_bytes(b, b->h_desc[k]);

@@ -195,3 +195,39 @@ def test_serve_routes_stalled_kind_to_fp64(ctx, templates):
             req.release()
     finally:
         srv.close()
+
+
+def test_serve_fp64_momentum_on_segment_order(ctx, templates):
+    """A template whose momentum kind is routed to FP64 (route_fp64) solves
+    its momentum components in the FP64 batch on the segment ILU(0)
+    ordering; each result equals the FP64 solve of the same system alone,
+    bit for bit, and the other template stays mixed."""
+    from kcfd_amd import device, serve as S
+    T = templates[3]                                  # quad, 9,000 cells
+    T.route_fp64(False, "segment")
+    try:
+        assert T.proto[1].ilu0_order()[1] == 2        # two segment colours
+        trace = [(0.0, 3, 7400 + i) for i in range(3)] + [(0.0, 1, 7500)]
+        srv = S.Server(ctx, templates, precision="mixed", precond="ilu0", batched=True, check_every=4)
+        srv.keep_solutions = True
+        try:
+            rep = srv.run(trace)
+            print(rep.summary())
+            assert rep.requests == 4 and rep.not_converged == 0 and rep.max_rel_resid < 1e-8
+            assert rep.fp64_direct == 9 and rep.fp64_fallbacks == 0
+            o64 = device.default_opts(rtol=1e-8, precond="ilu0", precision="fp64", check_every=4)
+            for rid, (t, ti, seed) in enumerate(trace):
+                req = S.Request(rid, 0.0, templates[ti], seed)
+                srv._admit(req, 0.0)
+                xs, res = srv.solutions[rid]
+                for k, (A, b, x) in enumerate(req.systems()):
+                    o = srv.opts_cg if k == 0 else (o64 if ti == 3 else srv.opts)
+                    r = A.solve_dev(b.p, x.p, o, setup=False)
+                    assert r.iters == res[k][1] and r.info == res[k][0] == 0, (rid, k)
+                    assert np.array_equal(x.download(), xs[k]), (rid, k)
+                req.release()
+        finally:
+            srv.close()
+    finally:
+        T.fp64_kinds.discard(False)
+        T.proto[1].setup_precond("ilu0", "mixed", order="color")
