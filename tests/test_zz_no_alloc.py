@@ -4,8 +4,8 @@ The allocation-free entry point runs the same kernels on a workspace in a
 static device array.  Two calls in a row, the second on a smaller system,
 each compared with kcfd_solve on the same system: x bit for bit, and the
 same info, iteration count, half-step flag and reliable-update count --
-for FP64 BiCGSTAB with Jacobi and with the colour-ordered ILU(0), mixed
-precision and CG.  A system whose workspace does not fit the array is
+for FP64 BiCGSTAB with Jacobi and with the colour- and segment-ordered
+ILU(0), mixed precision and CG.  A system whose workspace does not fit the array is
 refused with a message.
 """
 
@@ -37,12 +37,12 @@ def _both(ctx, A, b, pc, precision="fp64", method="bicgstab", order=None):
     return out
 
 
-@pytest.mark.parametrize("kind", ["jacobi", "ilu0_color", "mixed", "cg"])
+@pytest.mark.parametrize("kind", ["jacobi", "ilu0_color", "ilu0_segment", "mixed", "cg"])
 def test_noalloc_equals_solve(ctx, kind):
     from kcfd_amd import workloads
     if kind == "cg":
         cases = [workloads.hex_pressure(24, seed=1002), workloads.hex_pressure(12, seed=1003)]
-    elif kind == "ilu0_color":
+    elif kind in ("ilu0_color", "ilu0_segment"):
         cases = [workloads.hex_pressure(32, seed=1002), workloads.quad_convdiff(40, seed=1004)]
     else:
         cases = [workloads.quad_convdiff(96, seed=1001), workloads.quad_convdiff(48, seed=1005)]
@@ -50,8 +50,8 @@ def test_noalloc_equals_solve(ctx, kind):
         A, b, _ = oracle_system(case)
         if kind == "jacobi":
             (xa, ra), (xn, rn) = _both(ctx, A, b, "jacobi")
-        elif kind == "ilu0_color":
-            (xa, ra), (xn, rn) = _both(ctx, A, b, "ilu0", order="color")
+        elif kind in ("ilu0_color", "ilu0_segment"):
+            (xa, ra), (xn, rn) = _both(ctx, A, b, "ilu0", order=kind[5:])
         elif kind == "mixed":
             (xa, ra), (xn, rn) = _both(ctx, A, b, "jacobi", precision="mixed")
         else:
