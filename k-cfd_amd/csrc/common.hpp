@@ -30,7 +30,7 @@ constexpr int kXcds = 8;
 constexpr int kSegLen = KCFD_SEG_LEN;
 // entries outside its segment in one row of a segment-ordered factor (at
 // most; a structure with more takes the level layout)
-constexpr int kSegW = 8;
+constexpr int kSegW = 6;
 
 // narrow-index SELL copy: the offset that marks column = row, and the widest
 // span of off-diagonal columns one 64-row slice may have
