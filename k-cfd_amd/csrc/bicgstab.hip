@@ -1754,8 +1754,8 @@ __device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int l
             for (int e = 0; e < kSegW; ++e) pr[k][e] = 0.0;
             if (k < len) {
                 const int i = ri[k];
-                z[k] = PART == 1 ? src(i) : y[i];
-                if (PART == 1) l[k] = gl(lv)[i];
+                z[k] = src(i);
+                l[k] = gl(lv)[i];
                 u[k] = gl(uv)[i];
                 dd[k] = gl(dg)[i];
                 if (PART == 1) seg_products<V>(fac_l(d), st[k].x + lane, st[k].y, osrc, pr[k], nv[k]);
@@ -1784,10 +1784,22 @@ __device__ __forceinline__ void seg_walk(const SysDesc &d, const Chunk &g, int l
                 wn = (double)out[k];
             }
         } else {
+            // the colour-0 forward solve again, in registers: y is not read
+            // back (in MIXED that keeps it unrounded; in FP64 it is the same
+            // arithmetic as part 0's)
+            double yk[kSegLen];
+            double yp = 0.0;
+#pragma unroll
+            for (int k = 0; k < kSegLen; ++k) {
+                double acc = (double)z[k];
+                if (k > 0) acc -= (double)l[k] * yp;
+                yk[k] = acc;
+                yp = acc;
+            }
             double wn = 0.0;
 #pragma unroll
             for (int k = kSegLen - 1; k >= 0; --k) {
-                double acc = (double)z[k];
+                double acc = yk[k];
                 if (k + 1 < len) acc -= (double)u[k] * wn;
 #pragma unroll
                 for (int e = 0; e < kSegW; ++e)
@@ -1821,8 +1833,11 @@ __global__ __launch_bounds__(kBlock) void k_seg(const SysDesc *__restrict__ sys,
         const SysDesc &d = sys[g.sys()];
         const SysState &st = sts[g.sys()];
         if (!ilu2_wants<V, WHICH>(st.status) || lane >= g.count()) continue;
-        if constexpr (PART == 2 || WHICH == 0) {
+        if constexpr (WHICH == 0) {
             const PlainSrc<V> src{vec<V>(d.p)};
+            seg_walk<V, PART, WHICH>(d, g, lane, src, (KCFD_GLOBAL V *)nullptr);
+        } else if constexpr (PART == 2 && WHICH == 1) {   // s as part 0 stored it
+            const PlainSrc<V> src{vec<V>(d.s)};
             seg_walk<V, PART, WHICH>(d, g, lane, src, (KCFD_GLOBAL V *)nullptr);
         } else if constexpr (WHICH == 1) {
             const SSrc<V> src{vec<V>(d.r), vec<V>(d.v), (V)st.alpha};
