@@ -7,7 +7,7 @@ default ``rtol=1e-5`` (linear_solver.py:59), so the drop-in stops at
 ``||r|| < max(tol, rtol*||b||)`` with ``rtol`` defaulting to 1e-5 unless an
 ``rtol=`` keyword is passed.  Extra keywords: ``rtol``, ``precision``
 ('fp64' | 'mixed'), ``inner_rtol`` (mixed: residual-replacement trigger,
-default 0.1), ``order`` ('natural' | 'color', the ILU(0) row ordering),
+default 0.1), ``order`` ('natural' | 'color' | 'segment', the ILU(0) row ordering),
 ``return_info``.  A non-converged solve
 prints the reference's message and returns x; a breakdown returns x
 silently (linear_solver.py:60-62).
